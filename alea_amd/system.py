@@ -1,0 +1,212 @@
+"""Host-side system description: cell, particle groups, trial-wavefunction
+factors and Hamiltonian terms, built the way the reference's builders build
+them for the 2-D electron-gas inputs.
+
+What is mirrored (reference file:line):
+  cell from rs            src/qmc/app.cpp:274-341   (area = N*pi*rs^2)
+  Kato cusp a1            src/builders/jastrow.cpp:48,159-175
+  image-summed Padé       src/builders/jastrow.cpp:366-368
+  closed-shell plane waves src/wavefunction/orbital_set.h:566-691
+  twisted plane waves     src/wavefunction/complex_orbital_set.h:278-341
+  Ewald                   src/builders/ewald.cpp:40-66 (tolerance 1e-6)
+
+The Natoli-Ceperley breakup tables (Ewald and the image-summed Padé) are
+setup-time inputs.  They are read from alea_amd/tables/cells.json, which the
+reference's own breakup produced (alea_amd/tables/make_tables.py); a drop-in
+caller passes the tables its own builders computed.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+TABLES = os.path.join(os.path.dirname(__file__), "tables", "cells.json")
+
+
+def cell_key(lattice) -> str:
+    return ",".join(f"{x:.12g}" for x in lattice)
+
+
+def pade_key(a, b) -> str:
+    return "a=" + ",".join(f"{x:.15g}" for x in a) + ";b=" + ",".join(f"{x:.15g}" for x in b)
+
+
+_tables_cache = None
+
+
+def load_tables():
+    global _tables_cache
+    if _tables_cache is None:
+        with open(TABLES) as f:
+            _tables_cache = json.load(f)
+    return _tables_cache
+
+
+def square_lattice(n_particles: int, rs: float):
+    L = math.sqrt(math.pi * n_particles) * rs
+    return [L, 0.0, 0.0, L]
+
+
+def hexagonal_lattice(n_particles: int, rs: float):
+    area = n_particles * math.pi * rs * rs
+    L = math.sqrt(2.0 * area / math.sqrt(3.0))
+    return [L, 0.0, 0.5 * L, L * math.sqrt(3.0) / 2.0]
+
+
+def reciprocal(lattice):
+    """2*pi*inverse-transpose of the (reduced) basis, as SimulationCell does
+    for axis-aligned cells (simulation_cell.cpp:146-201)."""
+    a = np.array(lattice, dtype=float).reshape(2, 2)
+    det = a[0, 0] * a[1, 1] - a[0, 1] * a[1, 0]
+    ainv = np.array([[a[1, 1], -a[1, 0]], [-a[0, 1], a[0, 0]]]) / det
+    return 2.0 * math.pi * ainv
+
+
+def plane_wave_shells(lattice, n_orb: int):
+    """Closed-shell real plane waves (orbital_set.h:566-691): returns k [n,2]
+    and is_sin [n]; refuses an open shell."""
+    b = reciprocal(lattice)
+    area = abs(b[0, 0] * b[1, 1] - b[0, 1] * b[1, 0])
+    h_min = area / max(math.hypot(*b[0]), math.hypot(*b[1]))
+    M = 4
+    while M <= 4096:
+        r_safe2 = (M * h_min) ** 2 * (1.0 + 1e-12)
+        kv = []
+        for i in range(-M, M + 1):
+            for j in range(-M, M + 1):
+                k = i * b[0] + j * b[1]
+                k2 = k[0] * k[0] + k[1] * k[1]
+                if k2 <= r_safe2:
+                    kv.append((k2, k[0], k[1]))
+        kv.sort()
+        ks, sins, closed = [], [], []
+        s = 0
+        while s < len(kv):
+            e = s + 1
+            while e < len(kv) and abs(kv[e][0] - kv[s][0]) <= 1e-9 * (1.0 + kv[s][0]):
+                e += 1
+            if kv[s][0] <= 1e-12 * (1.0 + kv[s][0]):
+                ks.append((0.0, 0.0))
+                sins.append(0)
+            else:
+                for q in range(s, e):
+                    kx, ky = kv[q][1], kv[q][2]
+                    positive = kx > 1e-12 or (not kx < -1e-12 and ky > 1e-12)
+                    if not positive:
+                        continue
+                    for which in (0, 1):
+                        ks.append((kx, ky))
+                        sins.append(which)
+            closed.append(len(sins))
+            s = e
+        if len(sins) >= n_orb:
+            if n_orb not in closed:
+                raise ValueError(f"plane-wave orbitals: {n_orb} leaves a shell open; closed: {closed[:12]}")
+            return np.array(ks[:n_orb]), np.array(sins[:n_orb], dtype=np.uint8)
+        M *= 2
+    raise RuntimeError("shell enumeration did not converge")
+
+
+def twisted_wavevectors(lattice, n: int, twist):
+    """q = k + theta for the n lowest |k + theta| (complex_orbital_set.h:278-341);
+    twist is in units of the reciprocal vectors as given."""
+    b = reciprocal(lattice)
+    theta = twist[0] * b[0] + twist[1] * b[1]
+    tn = math.hypot(*theta)
+    area = abs(b[0, 0] * b[1, 1] - b[0, 1] * b[1, 0])
+    h_min = area / max(math.hypot(*b[0]), math.hypot(*b[1]))
+    M = 4
+    while M <= 4096:
+        qs = []
+        for i in range(-M, M + 1):
+            for j in range(-M, M + 1):
+                q = i * b[0] + theta + j * b[1]
+                qs.append((q[0] * q[0] + q[1] * q[1], q[0], q[1]))
+        if len(qs) > n:
+            qs.sort(key=lambda t: t[0])
+            r_safe = M * h_min - tn
+            if r_safe > 0 and qs[n][0] <= r_safe * r_safe:
+                if n > 0 and abs(qs[n][0] - qs[n - 1][0]) <= 1e-9 * (1.0 + qs[n][0]):
+                    raise ValueError("twist splits a degenerate |k+theta| shell")
+                return np.array([(t[1], t[2]) for t in qs[:n]])
+        M *= 2
+    raise RuntimeError("twisted enumeration did not converge")
+
+
+@dataclass
+class Spec:
+    """Everything the C-ABI needs to build one system."""
+    n_particles: int
+    lattice: list | None
+    groups: list = field(default_factory=list)          # (first, last, charge, inv_mass)
+    pade: list = field(default_factory=list)            # dicts
+    drummond: list = field(default_factory=list)
+    nosanow: list = field(default_factory=list)
+    slater_pw: list = field(default_factory=list)       # (first, last, k, is_sin)
+    slater_twisted: list = field(default_factory=list)  # (first, last, q)
+    slater_gauss: list = field(default_factory=list)    # (first, last, centers, alphas)
+    ewald: dict | None = None
+    coulomb: list = field(default_factory=list)         # (ga, gb, eps)
+    name: str = ""
+
+
+def heg_spec(n_up: int, n_dn: int, rs: float, twist=None, slater=True, jastrow=True,
+             ewald=True, tables=None, lattice=None, drummond=None) -> Spec:
+    """The 2-D paramagnetic (or polarized) electron gas of the HEG inputs."""
+    N = n_up + n_dn
+    lat = lattice or square_lattice(N, rs)
+    tables = tables or load_tables()
+    cell = tables["cells"].get(cell_key(lat))
+    if cell is None and (jastrow or ewald):
+        raise KeyError(f"no breakup tables for cell {cell_key(lat)}; run alea_amd/tables/make_tables.py")
+    s = Spec(n_particles=N, lattice=lat, name=f"heg N={N} rs={rs}")
+    s.groups.append((0, n_up, -1.0, 1.0))
+    if n_dn:
+        s.groups.append((n_up, N, -1.0, 1.0))
+    b = [0.5 / rs]
+    if jastrow:
+        pairs = [(0, 0, 1.0 / 3.0)]
+        if n_dn:
+            pairs += [(1, 1, 1.0 / 3.0), (0, 1, 1.0)]
+        for ga, gb, a1 in pairs:
+            tab = cell["pade"][pade_key([a1], b)]
+            s.pade.append(dict(ga=ga, gb=gb, num=1, den=1, coeffs=[a1] + b, image=tab))
+    if drummond is not None:
+        s.drummond.extend(drummond)
+    if slater:
+        for gi, (first, last, _, _) in enumerate(s.groups):
+            n = last - first
+            if twist is None:
+                k, sn = plane_wave_shells(lat, n)
+                s.slater_pw.append((first, last, k, sn))
+            else:
+                s.slater_twisted.append((first, last, twisted_wavevectors(lat, n, twist)))
+    if ewald:
+        s.ewald = cell["ewald"]
+    return s
+
+
+def exciton_spec() -> Spec:
+    """Open-cell 2-D exciton with the exact u = -r (tests/configs/exciton_2d_cusp_m1.yaml):
+    E_L = -1 for every configuration."""
+    s = Spec(n_particles=2, lattice=None, name="exciton")
+    s.groups = [(0, 1, -1.0, 1.0), (1, 2, 1.0, 1.0)]
+    s.pade.append(dict(ga=0, gb=1, num=1, den=0, coeffs=[-1.0], image=None))
+    s.coulomb.append((0, 1, 1.0))
+    return s
+
+
+def uniform_positions(spec: Spec, nw: int, rng: np.random.Generator, box=1.0):
+    """Uniform fractional coordinates in [-1/2, 1/2) mapped through the cell
+    (app.cpp:56-116), SoA [p*2*nw + d*nw + w]."""
+    N = spec.n_particles
+    f = rng.uniform(-0.5, 0.5, size=(N, 2, nw))
+    if spec.lattice is None:
+        return np.ascontiguousarray((f * box).reshape(-1))
+    a = np.array(spec.lattice).reshape(2, 2)
+    pos = np.einsum("pin,id->pdn", f, a)
+    return np.ascontiguousarray(pos.reshape(-1))

@@ -1,0 +1,166 @@
+"""Walker batch handle over a C-ABI library (include/alea_amd.h).
+
+Mirrors the reference's host-side interfaces at the granularity the C-ABI
+exposes them: WalkerSet (positions, log psi, gradients), TrialWavefunction
+(evaluate_batch, ratio_batch, apply_move_batch), Hamiltonian
+(evaluate_batch) and the drivers' sweeps.  Arrays use the reference's
+walker-fastest SoA layouts:
+
+  positions, gradients, laplacians   [p*dim*nw + d*nw + w]
+  proposed position, new_grad        [d*nw + w]
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._capi import CApi, f64, i32, ptr, u8, _P_I, _P_U8
+from .system import Spec
+
+FP64, MIXED = 0, 1
+
+
+class Walkers:
+    def __init__(self, api: CApi, spec: Spec, nw: int, precision: int = FP64):
+        self.api = api
+        self.spec = spec
+        self.nw = nw
+        self.np = spec.n_particles
+        self.dim = 2
+        h = C.c_void_p()
+        lat = None if spec.lattice is None else f64(spec.lattice)
+        api("create", C.byref(h), nw, self.np, 2, ptr(lat), precision)
+        self.h = h
+        self._keep = []
+        try:
+            self._register(spec)
+        except Exception:
+            self.close()
+            raise
+
+    def _k(self, a):
+        self._keep.append(a)
+        return a
+
+    def _register(self, s: Spec):
+        api, h = self.api, self.h
+        for first, last, q, im in s.groups:
+            api("add_group", h, first, last, q, im)
+        for j in s.pade:
+            c = self._k(f64(j["coeffs"]))
+            t = j.get("image")
+            if t:
+                n1, n2 = self._k(i32(t["n1"])), self._k(i32(t["n2"]))
+                kx, ky, w = self._k(f64(t["kx"])), self._k(f64(t["ky"])), self._k(f64(t["w"]))
+                sp = self._k(f64(t["spline"]))
+                api("add_pade_jastrow", h, j["ga"], j["gb"], j["num"], j["den"], ptr(c), len(w),
+                    ptr(n1, _P_I), ptr(n2, _P_I), ptr(kx), ptr(ky), ptr(w), t["n_knots"], ptr(sp),
+                    t["r_cut"], t["inv_delta"], t["u_inf"])
+            else:
+                api("add_pade_jastrow", h, j["ga"], j["gb"], j["num"], j["den"], ptr(c), 0, None,
+                    None, None, None, None, 0, None, 0.0, 0.0, 0.0)
+        for d in s.drummond:
+            c = self._k(f64(d["coeffs"]))
+            api("add_drummond_jastrow", h, d["ga"], d["gb"], d["C"], d["gamma"], d["L"], len(c), ptr(c))
+        for n in s.nosanow:
+            st = self._k(f64(n["sites"]))
+            api("add_nosanow_jastrow", h, n["group"], n["alpha"], ptr(st))
+        for first, last, k, sn in s.slater_pw:
+            kk, ss = self._k(f64(k)), self._k(u8(sn))
+            api("add_slater_plane_wave", h, first, last, ptr(kk), ptr(ss, _P_U8))
+        for first, last, q in s.slater_twisted:
+            qq = self._k(f64(q))
+            api("add_slater_twisted", h, first, last, ptr(qq))
+        for first, last, cen, al in s.slater_gauss:
+            cc, aa = self._k(f64(cen)), self._k(f64(al))
+            api("add_slater_gaussian", h, first, last, ptr(cc), ptr(aa))
+        if s.ewald:
+            e = s.ewald
+            n1, n2 = self._k(i32(e["n1"])), self._k(i32(e["n2"]))
+            kx, ky, w = self._k(f64(e["kx"])), self._k(f64(e["ky"])), self._k(f64(e["w"]))
+            sp = self._k(f64(e["spline"]))
+            api("add_ewald", h, len(w), ptr(n1, _P_I), ptr(n2, _P_I), ptr(kx), ptr(ky), ptr(w),
+                e["n_knots"], ptr(sp), e["r_cut"], e["inv_delta"], e["v_long_zero"], e["sr_k0"])
+        for ga, gb, eps in s.coulomb:
+            api("add_coulomb", h, ga, gb, eps)
+
+    # ── state ──────────────────────────────────────────────────────────────
+    def set_positions(self, pos):
+        p = f64(pos)
+        assert p.size == self.np * 2 * self.nw
+        self.api("set_positions", self.h, ptr(p))
+
+    def positions(self):
+        out = np.empty(self.np * 2 * self.nw)
+        self.api("get_positions", self.h, ptr(out))
+        return out
+
+    def evaluate(self):
+        self.api("evaluate", self.h)
+
+    def log_psi(self):
+        out = np.empty(self.nw)
+        self.api("get_log_psi", self.h, ptr(out))
+        return out
+
+    def gradients(self):
+        g = np.empty(self.np * 2 * self.nw)
+        l = np.empty_like(g)
+        self.api("get_gradients", self.h, ptr(g), ptr(l))
+        return g, l
+
+    def phase_gradients(self):
+        g = np.empty(self.np * 2 * self.nw)
+        l = np.empty_like(g)
+        self.api("get_phase_gradients", self.h, ptr(g), ptr(l))
+        return g, l
+
+    def local_energy(self):
+        re = np.empty(self.nw)
+        im = np.empty(self.nw)
+        self.api("local_energy", self.h, ptr(re), ptr(im))
+        return re, im
+
+    # ── per-call path ──────────────────────────────────────────────────────
+    def ratio(self, p, new_pos):
+        npos = f64(new_pos)
+        lr = np.empty(self.nw)
+        ph = np.empty(self.nw)
+        ng = np.empty(2 * self.nw)
+        nl = np.empty(2 * self.nw)
+        self.api("ratio", self.h, p, ptr(npos), ptr(lr), ptr(ph), ptr(ng), ptr(nl))
+        return lr, ph, ng, nl
+
+    def accept(self, p, mask):
+        m = u8(mask)
+        self.api("accept", self.h, p, ptr(m, _P_U8))
+
+    # ── drivers ────────────────────────────────────────────────────────────
+    def vmc_sweeps(self, n_sweeps, tau, tape=None):
+        acc = C.c_longlong()
+        t = None if tape is None else f64(tape)
+        self.api("vmc_sweeps", self.h, n_sweeps, tau, ptr(t), C.byref(acc))
+        return acc.value
+
+    def dmc_sweeps(self, n_sweeps, tau, tape, stride, rr_p, rr_a):
+        acc = C.c_longlong()
+        t = f64(tape)
+        self.api("dmc_sweeps", self.h, n_sweeps, tau, ptr(t), stride, ptr(rr_p), ptr(rr_a), C.byref(acc))
+        return acc.value
+
+    def permute(self, src):
+        s = i32(src)
+        self.api("permute", self.h, len(s), ptr(s, _P_I))
+        self.nw = len(s)
+
+    def close(self):
+        if self.h:
+            self.api("destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

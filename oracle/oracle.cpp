@@ -1,0 +1,2059 @@
+// Oracle: FP64 CPU restatement of Alea's batched VMC/DMC walker sweep.
+//
+// TEST INFRASTRUCTURE ONLY.  It is the checker for the HIP path, never the
+// thing measured or shipped: only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py load it.  The product library (alea_amd) never
+// links, loads or calls it.
+//
+// It restates the reference's algorithms in the reference's own layouts
+// (walker-fastest SoA, [.. * nw + w]) and in the reference's order of
+// operations where that matters for bit-level agreement (the minimum-image
+// fold, the RNG, the tape order).  Each function cites the reference file:line
+// it follows.  How each part is pinned (tests/test_oracle_*.py):
+//   cell, distances, Padé (+ image sum), Drummond, Nosanow, kinetic, Ewald,
+//   Coulomb, VMC sweep, RNG  -> against oracle/_ref (the reference's own
+//                               sources compiled here) and the reference's
+//                               golden vectors (xoshiro state {1,2,3,4},
+//                               Madelung 1.6155426267, exciton E = -1)
+//   Slater real/complex     -> closed forms (N <= 2 determinants, plane-wave
+//                               eigenstates: E_L = sum |k|^2/2 exactly),
+//                               brute-force permutation-sum determinants,
+//                               finite differences; the reference's own
+//                               Slater files need Eigen (absent), so they are
+//                               not built: "parity pinned by closed forms".
+//   Eigen stand-in          -> the initial inverse uses the reference's own
+//                               pivoted Gauss-Jordan (slater_determinant.cpp:
+//                               905-993) in place of Eigen's D.inverse().
+
+#include "oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <bit>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <limits>
+#include <memory>
+#include <numbers>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_error;
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double kDetFloor = 1e-300;
+using cplx = std::complex<double>;
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        g_error = e.what();
+        return 1;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return 2;
+    }
+}
+
+// ── RNG: src/numerics/random.h:39-56, src/numerics/xoshiro_random.h:23-193 ──
+constexpr uint64_t splitmix64(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+constexpr uint64_t stream_seed(uint64_t base, uint64_t stream, uint64_t index) {
+    return splitmix64(splitmix64(splitmix64(base) ^ stream) ^ index);
+}
+
+struct Xoshiro {
+    uint64_t s[4];
+    explicit Xoshiro(uint64_t seed) {
+        for (uint64_t i = 0; i < 4; ++i) s[i] = splitmix64(seed + i * 0x9e3779b97f4a7c15ULL);
+    }
+    Xoshiro(const uint64_t* st) { std::memcpy(s, st, sizeof s); }
+    uint64_t next() {
+        const uint64_t result = std::rotl(s[0] + s[3], 23) + s[0];
+        const uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = std::rotl(s[3], 45);
+        return result;
+    }
+};
+
+struct Ziggurat {
+    std::array<uint64_t, 256> k{};
+    std::array<double, 256> w{}, f{};
+    double r = 0.0;
+};
+
+// Marsaglia-Tsang tables with 256 equal-area regions, built as in
+// xoshiro_random.h:119-163 (bisection on the top-layer mismatch).
+Ziggurat make_ziggurat() {
+    constexpr int n = 256;
+    auto f = [](double x) { return std::exp(-0.5 * x * x); };
+    auto v_r = [&](double r) {
+        return r * f(r) + std::sqrt(0.5 * std::numbers::pi) * std::erfc(r / std::numbers::sqrt2);
+    };
+    std::array<double, n> x{};
+    auto top_mismatch = [&](double r) {
+        const double v = v_r(r);
+        x[n - 1] = r;
+        for (int i = n - 1; i >= 2; --i) {
+            const double y = f(x[i]) + v / x[i];
+            if (y >= 1.0) return 1.0;
+            x[i - 1] = std::sqrt(-2.0 * std::log(y));
+        }
+        return f(x[1]) + v / x[1] - 1.0;
+    };
+    double lo = 3.0, hi = 4.5;
+    for (;;) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid == lo || mid == hi) break;
+        (top_mismatch(mid) > 0.0 ? lo : hi) = mid;
+    }
+    Ziggurat z;
+    z.r = hi;
+    (void)top_mismatch(z.r);
+    const double two52 = 0x1.0p52;
+    const double base = v_r(z.r) / f(z.r);
+    z.k[0] = static_cast<uint64_t>(two52 * (z.r / base));
+    z.w[0] = base / two52;
+    z.f[0] = 1.0;
+    for (int i = 1; i < n; ++i) {
+        z.k[i] = i == 1 ? 0 : static_cast<uint64_t>(two52 * (x[i - 1] / x[i]));
+        z.w[i] = x[i] / two52;
+        z.f[i] = f(x[i]);
+    }
+    return z;
+}
+const Ziggurat& ziggurat() {
+    static const Ziggurat z = make_ziggurat();
+    return z;
+}
+
+struct Rng {
+    Xoshiro g;
+    explicit Rng(uint64_t seed) : g(seed) {}
+    double uniform() { return static_cast<double>(static_cast<int64_t>(g.next() >> 11)) * 0x1.0p-53; }
+    static double with_sign(double x, uint64_t sign) {
+        return std::bit_cast<double>(std::bit_cast<uint64_t>(x) ^ sign);
+    }
+    double tail(double r) {
+        for (;;) {
+            const double xi = -std::log1p(-uniform()) / r;
+            const double y = -std::log1p(-uniform());
+            if (y + y > xi * xi) return r + xi;
+        }
+    }
+    double gaussian() {
+        const Ziggurat& z = ziggurat();
+        constexpr uint64_t mask = (uint64_t{1} << 52) - 1;
+        for (;;) {
+            const uint64_t bits = g.next();
+            const std::size_t i = bits & 0xff;
+            const uint64_t sign = (bits & 0x100) << 55;
+            const uint64_t position = (bits >> 9) & mask;
+            const double x = static_cast<double>(static_cast<int64_t>(position)) * z.w[i];
+            if (position < z.k[i]) return with_sign(x, sign);
+            if (i == 0) return with_sign(tail(z.r), sign);
+            if (z.f[i] + uniform() * (z.f[i - 1] - z.f[i]) < std::exp(-0.5 * x * x))
+                return with_sign(x, sign);
+        }
+    }
+};
+
+// draw_sweep_randoms (src/qmc/sweep_blocking.h:39-48).
+void draw_sweep(Rng& rng, std::size_t np, std::size_t dim, std::size_t stride,
+                std::size_t w0, std::size_t w1, double* out) {
+    for (std::size_t p = 0; p < np; ++p) {
+        double* row = out + p * (dim + 1) * stride;
+        for (std::size_t d = 0; d < dim; ++d)
+            for (std::size_t w = w0; w < w1; ++w) row[d * stride + w] = rng.gaussian();
+        for (std::size_t w = w0; w < w1; ++w) row[dim * stride + w] = rng.uniform();
+    }
+}
+
+// ── Cell: src/population/simulation_cell.{h:109-261, cpp:16-317} ────────────
+struct Cell {
+    bool periodic = false, ortho = false;
+    std::size_t dim = 2, ncand = 1;
+    double a[4]{}, ainv[4]{}, b[4]{}, cand[9 * 2]{}, box[2]{}, inv_box[2]{};
+    double volume = 0.0, ws = 0.0;
+
+    void init(std::size_t d, const double* v) {
+        dim = d;
+        if (!v) return;
+        if (d != 2) throw std::invalid_argument("oracle: periodic cells are 2-D");
+        periodic = true;
+        for (int k = 0; k < 4; ++k) a[k] = v[k];
+        // axis_align (simulation_cell.cpp:126-144)
+        {
+            std::size_t axis[2];
+            bool used[2] = {false, false}, ok = true;
+            for (std::size_t i = 0; i < 2 && ok; ++i) {
+                std::size_t nonzero = 0, which = 0;
+                for (std::size_t e = 0; e < 2; ++e)
+                    if (a[i * 2 + e] != 0.0) { ++nonzero; which = e; }
+                if (nonzero != 1 || used[which]) ok = false;
+                else { axis[i] = which; used[which] = true; }
+            }
+            if (ok) {
+                double re[4] = {};
+                for (std::size_t i = 0; i < 2; ++i)
+                    for (std::size_t e = 0; e < 2; ++e) re[axis[i] * 2 + e] = a[i * 2 + e];
+                std::memcpy(a, re, sizeof re);
+            }
+            ortho = ok;
+        }
+        if (!ortho) {  // gauss_reduce_2d (simulation_cell.cpp:22-36)
+            for (int guard = 0; guard < 64; ++guard) {
+                const double n0 = a[0] * a[0] + a[1] * a[1];
+                const double n1 = a[2] * a[2] + a[3] * a[3];
+                if (n1 < n0) { std::swap(a[0], a[2]); std::swap(a[1], a[3]); continue; }
+                if (n0 == 0.0) break;
+                const double mu = std::nearbyint((a[0] * a[2] + a[1] * a[3]) / n0);
+                if (mu == 0.0) break;
+                a[2] -= mu * a[0];
+                a[3] -= mu * a[1];
+            }
+        }
+        const double det = a[0] * a[3] - a[1] * a[2];
+        volume = std::abs(det);
+        if (volume == 0.0) throw std::invalid_argument("oracle: singular cell");
+        const double inv_det = 1.0 / det;
+        ainv[0] = a[3] * inv_det;
+        ainv[1] = -a[2] * inv_det;
+        ainv[2] = -a[1] * inv_det;
+        ainv[3] = a[0] * inv_det;
+        for (int k = 0; k < 4; ++k) b[k] = kTwoPi * ainv[k];
+        if (ortho)
+            for (std::size_t e = 0; e < 2; ++e) {
+                box[e] = std::abs(a[e * 2 + e]);
+                inv_box[e] = 1.0 / box[e];
+            }
+        ncand = 1;
+        cand[0] = cand[1] = 0.0;
+        if (!ortho)
+            for (int n0 = -1; n0 <= 1; ++n0)
+                for (int n1 = -1; n1 <= 1; ++n1) {
+                    if (n0 == 0 && n1 == 0) continue;
+                    for (std::size_t e = 0; e < 2; ++e) {
+                        double s = 0.0;
+                        s += n0 * a[0 * 2 + e];
+                        s += n1 * a[1 * 2 + e];
+                        cand[ncand * 2 + e] = s;
+                    }
+                    ++ncand;
+                }
+        double shortest2 = 0.0;
+        for (int n0 = -2; n0 <= 2; ++n0)
+            for (int n1 = -2; n1 <= 2; ++n1) {
+                if (n0 == 0 && n1 == 0) continue;
+                double r2 = 0.0;
+                for (std::size_t e = 0; e < 2; ++e) {
+                    double s = 0.0;
+                    s += n0 * a[0 * 2 + e];
+                    s += n1 * a[1 * 2 + e];
+                    r2 += s * s;
+                }
+                if (shortest2 == 0.0 || r2 < shortest2) shortest2 = r2;
+            }
+        ws = 0.5 * std::sqrt(shortest2);
+    }
+
+    // fold_bravais_2d (simulation_cell.h:201-234)
+    void fold2(double& x0, double& x1) const {
+        double s0 = 0.0;
+        s0 += ainv[0] * x0;
+        s0 += ainv[1] * x1;
+        double s1 = 0.0;
+        s1 += ainv[2] * x0;
+        s1 += ainv[3] * x1;
+        const double f0 = std::nearbyint(s0), f1 = std::nearbyint(s1);
+        double t0 = 0.0;
+        t0 += f0 * a[0];
+        t0 += f1 * a[2];
+        double t1 = 0.0;
+        t1 += f0 * a[1];
+        t1 += f1 * a[3];
+        x0 -= t0;
+        x1 -= t1;
+        double best0 = x0, best1 = x1, best2 = 0.0;
+        best2 += x0 * x0;
+        best2 += x1 * x1;
+        for (std::size_t c = 1; c < 9; ++c) {
+            const double y0 = x0 + cand[c * 2 + 0];
+            double r2 = 0.0;
+            r2 += y0 * y0;
+            const double y1 = x1 + cand[c * 2 + 1];
+            r2 += y1 * y1;
+            const double take = (r2 < best2) ? 1.0 : 0.0;
+            best2 += take * (r2 - best2);
+            best0 += take * (y0 - best0);
+            best1 += take * (y1 - best1);
+        }
+        x0 = best0;
+        x1 = best1;
+    }
+    // min_image_batch per lane (simulation_cell.cpp:252-299)
+    void min_image(double& d0, double& d1) const {
+        if (!periodic) return;
+        if (ortho) {
+            d0 = d0 - box[0] * std::nearbyint(d0 * inv_box[0]);
+            d1 = d1 - box[1] * std::nearbyint(d1 * inv_box[1]);
+            return;
+        }
+        fold2(d0, d1);
+    }
+    // wrap (simulation_cell.h:141-159)
+    void wrap(double& x0, double& x1) const {
+        if (!periodic) return;
+        if (ortho) {
+            x0 -= box[0] * std::nearbyint(x0 * inv_box[0]);
+            x1 -= box[1] * std::nearbyint(x1 * inv_box[1]);
+            return;
+        }
+        double f[2];
+        for (std::size_t i = 0; i < 2; ++i) {
+            double s = 0.0;
+            s += ainv[i * 2 + 0] * x0;
+            s += ainv[i * 2 + 1] * x1;
+            f[i] = std::nearbyint(s);
+        }
+        double s0 = 0.0, s1 = 0.0;
+        s0 += f[0] * a[0];
+        s0 += f[1] * a[2];
+        s1 += f[0] * a[1];
+        s1 += f[1] * a[3];
+        x0 -= s0;
+        x1 -= s1;
+    }
+};
+
+struct Group {
+    std::size_t first, last;
+    double charge, inv_mass;
+    bool contains(std::size_t i) const { return i >= first && i < last; }
+    std::size_t size() const { return last - first; }
+};
+
+// LongRangeSpline::eval (src/numerics/long_range_spline.h:41-60,66-110)
+struct Spline {
+    std::vector<double> scaled;
+    double r_cut = 0.0, inv_delta = 0.0;
+    int n_knots = 0;
+    void eval(double r, double& f, double& df, double& d2f) const {
+        if (r >= r_cut) { f = df = d2f = 0.0; return; }
+        const double t = r * inv_delta;
+        int i = static_cast<int>(t);
+        if (i > n_knots - 2) i = n_knots - 2;
+        const double x = t - i, y = 1.0 - x;
+        const double* a = scaled.data() + 3 * i;
+        const double* b = a + 3;
+        const double x2 = x * x, x3 = x2 * x, x4 = x3 * x, x5 = x4 * x;
+        const double y2 = y * y, y3 = y2 * y, y4 = y3 * y, y5 = y4 * y;
+        const double p0 = 1.0 - 10.0 * x3 + 15.0 * x4 - 6.0 * x5;
+        const double p1 = x - 6.0 * x3 + 8.0 * x4 - 3.0 * x5;
+        const double p2 = 0.5 * x2 - 1.5 * x3 + 1.5 * x4 - 0.5 * x5;
+        const double q0 = 1.0 - 10.0 * y3 + 15.0 * y4 - 6.0 * y5;
+        const double q1 = y - 6.0 * y3 + 8.0 * y4 - 3.0 * y5;
+        const double q2 = 0.5 * y2 - 1.5 * y3 + 1.5 * y4 - 0.5 * y5;
+        const double dp0 = -30.0 * x2 + 60.0 * x3 - 30.0 * x4;
+        const double dp1 = 1.0 - 18.0 * x2 + 32.0 * x3 - 15.0 * x4;
+        const double dp2 = x - 4.5 * x2 + 6.0 * x3 - 2.5 * x4;
+        const double dq0 = -30.0 * y2 + 60.0 * y3 - 30.0 * y4;
+        const double dq1 = 1.0 - 18.0 * y2 + 32.0 * y3 - 15.0 * y4;
+        const double dq2 = y - 4.5 * y2 + 6.0 * y3 - 2.5 * y4;
+        const double ddp0 = -60.0 * x + 180.0 * x2 - 120.0 * x3;
+        const double ddp1 = -36.0 * x + 96.0 * x2 - 60.0 * x3;
+        const double ddp2 = 1.0 - 9.0 * x + 18.0 * x2 - 10.0 * x3;
+        const double ddq0 = -60.0 * y + 180.0 * y2 - 120.0 * y3;
+        const double ddq1 = -36.0 * y + 96.0 * y2 - 60.0 * y3;
+        const double ddq2 = 1.0 - 9.0 * y + 18.0 * y2 - 10.0 * y3;
+        f = a[0] * p0 + a[1] * p1 + a[2] * p2 + b[0] * q0 - b[1] * q1 + b[2] * q2;
+        df = inv_delta * (a[0] * dp0 + a[1] * dp1 + a[2] * dp2 - (b[0] * dq0 - b[1] * dq1 + b[2] * dq2));
+        d2f = inv_delta * inv_delta *
+              (a[0] * ddp0 + a[1] * ddp1 + a[2] * ddp2 + b[0] * ddq0 - b[1] * ddq1 + b[2] * ddq2);
+    }
+    double value(double r) const {
+        double f, df, d2f;
+        eval(r, f, df, d2f);
+        return f;
+    }
+};
+
+// Pair radial functions: Padé (pade_jastrow.cpp:26-56) with the image-sum
+// short-range transform (periodic_pair_sum.cpp:66-83), or Drummond
+// (drummond_jastrow.cpp:27-80).
+struct Radial {
+    bool drummond = false;
+    int num = 0, den = 0;
+    std::vector<double> c;
+    bool image = false;
+    Spline sp;
+    double u_inf = 0.0;
+    int C = 3;
+    double L = 0.0, alpha1 = 0.0;
+
+    static void poly(const double* c, int n, double r, double& v, double& dv, double& d2v) {
+        double r_km1 = 1.0, r_km2 = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const int k = i + 1;
+            v += c[i] * r_km1 * r;
+            dv += c[i] * k * r_km1;
+            d2v += c[i] * k * (k - 1) * r_km2;
+            r_km2 = r_km1;
+            r_km1 *= r;
+        }
+    }
+    void eval(double r, double& u, double& du, double& d2u) const {
+        if (drummond) {
+            auto ipow = [](double b, int e) { double x = 1.0; for (int i = 0; i < e; ++i) x *= b; return x; };
+            const double t = r - L;
+            const double tCm2 = ipow(t, C - 2), tCm1 = tCm2 * t, tC = tCm1 * t;
+            const double g = tC, dg = static_cast<double>(C) * tCm1,
+                         d2g = static_cast<double>(C) * (C - 1) * tCm2;
+            const int n = static_cast<int>(c.size());
+            double P = c[0] + alpha1 * r, dP = alpha1, d2P = 0.0;
+            double r_lm2 = 1.0, r_lm1 = r, r_l = r * r;
+            for (int k = 1; k < n; ++k) {
+                const int l = k + 1;
+                P += c[k] * r_l;
+                dP += c[k] * l * r_lm1;
+                d2P += c[k] * l * (l - 1) * r_lm2;
+                r_lm2 = r_lm1;
+                r_lm1 = r_l;
+                r_l *= r;
+            }
+            const double m = (r < L) ? 1.0 : 0.0;
+            u = m * (g * P);
+            du = m * (dg * P + g * dP);
+            d2u = m * (d2g * P + 2.0 * dg * dP + g * d2P);
+            return;
+        }
+        double p = 0.0, dp = 0.0, d2p = 0.0;
+        poly(c.data(), num, r, p, dp, d2p);
+        double q = 1.0, dq = 0.0, d2q = 0.0;
+        poly(c.data() + num, den, r, q, dq, d2q);
+        const double inv_q = 1.0 / q;
+        u = p * inv_q;
+        du = (dp * q - p * dq) * inv_q * inv_q;
+        d2u = (d2p - (2.0 * dp * dq - 2.0 * p * dq * dq * inv_q + p * d2q) * inv_q) * inv_q;
+        if (image) {
+            if (r >= sp.r_cut) { u = du = d2u = 0.0; return; }
+            double f, df, d2f;
+            sp.eval(r, f, df, d2f);
+            u -= u_inf + f;
+            du -= df;
+            d2u -= d2f;
+        }
+    }
+};
+
+struct Sys;
+
+// Base factor caches (wavefunction_component.h:266-287).
+struct Component {
+    virtual ~Component() = default;
+    virtual void evaluate(Sys& s) = 0;
+    virtual void ratio(Sys& s, std::size_t p) = 0;
+    virtual void accept(Sys& s, std::size_t p, const std::vector<double>& mask) = 0;
+    virtual void refresh_particle(Sys&, std::size_t) {}
+    virtual void permute(const std::vector<std::size_t>& src, std::size_t old_nw) = 0;
+    virtual bool complex() const { return false; }
+    std::vector<double> log_psi_c, grad_c, lap_c, pgrad_c, plap_c;
+    void resize_base(std::size_t nw, std::size_t np, std::size_t dim) {
+        log_psi_c.assign(nw, 0.0);
+        grad_c.assign(np * dim * nw, 0.0);
+        lap_c.assign(np * dim * nw, 0.0);
+        if (complex()) {
+            pgrad_c.assign(np * dim * nw, 0.0);
+            plap_c.assign(np * dim * nw, 0.0);
+        }
+    }
+    static void gather(std::vector<double>& v, const std::vector<std::size_t>& src,
+                       std::size_t old_nw) {
+        if (v.empty()) return;
+        const std::size_t width = v.size() / old_nw;
+        std::vector<double> out(width * src.size());
+        for (std::size_t k = 0; k < width; ++k)
+            for (std::size_t j = 0; j < src.size(); ++j)
+                out[k * src.size() + j] = v[k * old_nw + src[j]];
+        v.swap(out);
+    }
+    void permute_base(const std::vector<std::size_t>& src, std::size_t old_nw) {
+        gather(log_psi_c, src, old_nw);
+        gather(grad_c, src, old_nw);
+        gather(lap_c, src, old_nw);
+        gather(pgrad_c, src, old_nw);
+        gather(plap_c, src, old_nw);
+    }
+};
+
+struct Ewald {
+    bool active = false;
+    std::vector<int> n1, n2;
+    std::vector<double> kx, ky, w;
+    Spline sp;
+    double r_cut = 0.0, v_long_zero = 0.0, sr_k0 = 0.0, inv_eps = 1.0;
+    int n1_max = 0, n2_max = 0;
+};
+
+struct CoulombPair {
+    int ga, gb;
+    double inv_eps;
+};
+
+struct Sys {
+    std::size_t nw = 0, np = 0, dim = 2;
+    Cell cell;
+    std::vector<Group> groups;
+    std::vector<double> pos, dist, pending, new_pos;
+    std::vector<double> log_ratios, phase_diffs, new_grad, new_lap, new_pgrad, new_plap;
+    std::vector<std::unique_ptr<Component>> comps;
+    Ewald ewald;
+    std::vector<CoulombPair> coulomb;
+    std::vector<double> inv_mass;  // [np]
+    std::vector<double> e_loc, e_loc_im;
+    bool has_phase = false;
+
+    double& P(std::size_t p, std::size_t d, std::size_t w) { return pos[(p * dim + d) * nw + w]; }
+    double& D(std::size_t i, std::size_t j, std::size_t w) { return dist[(i * np + j) * nw + w]; }
+
+    void alloc() {
+        pos.assign(np * dim * nw, 0.0);
+        dist.assign(np * np * nw, 0.0);
+        pending.assign(np * nw, 0.0);
+        new_pos.assign(dim * nw, 0.0);
+        log_ratios.assign(nw, 0.0);
+        phase_diffs.assign(nw, 0.0);
+        new_grad.assign(dim * nw, 0.0);
+        new_lap.assign(dim * nw, 0.0);
+        new_pgrad.assign(dim * nw, 0.0);
+        new_plap.assign(dim * nw, 0.0);
+        e_loc.assign(nw, 0.0);
+        e_loc_im.assign(nw, 0.0);
+    }
+    // BatchedDistanceTable::build_all (batched_distance_table.cpp:20-58)
+    void build_dist() {
+        for (std::size_t i = 0; i < np; ++i)
+            for (std::size_t j = 0; j < np; ++j)
+                for (std::size_t w = 0; w < nw; ++w) {
+                    double d0 = P(i, 0, w) - P(j, 0, w), d1 = P(i, 1, w) - P(j, 1, w);
+                    cell.min_image(d0, d1);
+                    double r2 = 0.0;
+                    r2 += d0 * d0;
+                    r2 += d1 * d1;
+                    D(i, j, w) = std::sqrt(r2);
+                }
+    }
+    // update_batch (batched_distance_table.cpp:63-101)
+    void propose(std::size_t p) {
+        for (std::size_t k = 0; k < np; ++k)
+            for (std::size_t w = 0; w < nw; ++w) {
+                if (k == p) { pending[k * nw + w] = 0.0; continue; }
+                double d0 = new_pos[w] - P(k, 0, w), d1 = new_pos[nw + w] - P(k, 1, w);
+                cell.min_image(d0, d1);
+                double r2 = 0.0;
+                r2 += d0 * d0;
+                r2 += d1 * d1;
+                pending[k * nw + w] = std::sqrt(r2);
+            }
+    }
+    // ParticleBatch::commit_move + WalkerSet::apply_move position part
+    // (particle_batch.cpp:65-85, batched_distance_table.cpp:115-143)
+    void commit(std::size_t p, const std::vector<double>& m) {
+        for (std::size_t k = 0; k < np; ++k) {
+            if (k == p) continue;
+            for (std::size_t w = 0; w < nw; ++w) {
+                double& pk = D(p, k, w);
+                double& kp = D(k, p, w);
+                const double nr = pending[k * nw + w];
+                pk += m[w] * (nr - pk);
+                kp += m[w] * (nr - kp);
+            }
+        }
+        for (std::size_t w = 0; w < nw; ++w) {
+            if (m[w] == 0.0) continue;
+            double x0 = new_pos[w], x1 = new_pos[nw + w];
+            cell.wrap(x0, x1);
+            P(p, 0, w) = x0;
+            P(p, 1, w) = x1;
+        }
+    }
+};
+
+// Displacement r_a - r_b folded by minimum image, for walker w.
+inline void fold(const Sys& s, double xa0, double xa1, double xb0, double xb1, double& d0,
+                 double& d1) {
+    d0 = xa0 - xb0;
+    d1 = xa1 - xb1;
+    s.cell.min_image(d0, d1);
+}
+
+// ── KSpacePairSum (src/wavefunction/kspace_pair_sum.cpp:91-301) with
+//    ReciprocalPhases (src/numerics/reciprocal_phases.h:26-75) ───────────────
+struct KSpace {
+    Group a, b;
+    bool same = true;
+    std::vector<int> n1, n2;
+    std::vector<double> kx, ky, kx2, ky2, w;
+    int m1 = 0, m2 = 0;
+    double b1x = 0, b1y = 0, b2x = 0, b2y = 0;
+    std::size_t nw = 0, n_cached = 0;
+    std::vector<double> ra_re, ra_im, rb_re, rb_im, pc, ps, pend_c, pend_s;
+
+    std::size_t nk() const { return kx.size(); }
+    std::size_t slot(std::size_t i) const { return a.contains(i) ? i - a.first : a.size() + (i - b.first); }
+    double* crow(std::size_t w, std::size_t i) { return pc.data() + (w * n_cached + slot(i)) * nk(); }
+    double* srow(std::size_t w, std::size_t i) { return ps.data() + (w * n_cached + slot(i)) * nk(); }
+
+    static void ladder(double theta, int m, std::vector<double>& re, std::vector<double>& im) {
+        re.assign(2 * m + 1, 0.0);
+        im.assign(2 * m + 1, 0.0);
+        double* r = re.data() + m;
+        double* i = im.data() + m;
+        const double c = std::cos(theta), s = std::sin(theta);
+        r[0] = 1.0;
+        i[0] = 0.0;
+        for (int n = 1; n <= m; ++n) {
+            r[n] = r[n - 1] * c - i[n - 1] * s;
+            i[n] = r[n - 1] * s + i[n - 1] * c;
+            r[-n] = r[n];
+            i[-n] = -i[n];
+        }
+    }
+    void phases(double x, double y, double* c, double* s) const {
+        std::vector<double> e1r, e1i, e2r, e2i;
+        ladder(b1x * x + b1y * y, m1, e1r, e1i);
+        ladder(b2x * x + b2y * y, m2, e2r, e2i);
+        const double* f1r = e1r.data() + m1;
+        const double* f1i = e1i.data() + m1;
+        const double* f2r = e2r.data() + m2;
+        const double* f2i = e2i.data() + m2;
+        for (std::size_t k = 0; k < nk(); ++k) {
+            const double ar = f1r[n1[k]], ai = f1i[n1[k]];
+            const double br = f2r[n2[k]], bi = f2i[n2[k]];
+            c[k] = ar * br - ai * bi;
+            s[k] = ar * bi + ai * br;
+        }
+    }
+    void prepare(std::size_t nw_) {
+        nw = nw_;
+        const std::size_t n = nw * nk();
+        ra_re.assign(n, 0.0);
+        ra_im.assign(n, 0.0);
+        if (!same) { rb_re.assign(n, 0.0); rb_im.assign(n, 0.0); }
+        pc.assign(n * n_cached, 0.0);
+        ps.assign(n * n_cached, 0.0);
+        pend_c.assign(n, 0.0);
+        pend_s.assign(n, 0.0);
+    }
+    struct KT { double gx, gy, lx, ly; };
+    KT kterms(const double* c, const double* s, const double* pr, const double* pi, double self) const {
+        double gx = 0, gy = 0, lx = 0, ly = 0;
+        for (std::size_t k = 0; k < nk(); ++k) {
+            const double g = 2.0 * w[k] * (c[k] * pi[k] - s[k] * pr[k]);
+            const double l = 2.0 * w[k] * (self - (c[k] * pr[k] + s[k] * pi[k]));
+            gx += kx[k] * g;
+            gy += ky[k] * g;
+            lx += kx2[k] * l;
+            ly += ky2[k] * l;
+        }
+        return {gx, gy, lx, ly};
+    }
+    double klog(const double* ar, const double* ai, const double* br, const double* bi) const {
+        double t = 0.0;
+        if (same)
+            for (std::size_t k = 0; k < nk(); ++k) t += w[k] * (ar[k] * ar[k] + ai[k] * ai[k]);
+        else
+            for (std::size_t k = 0; k < nk(); ++k) t += 2.0 * w[k] * (ar[k] * br[k] + ai[k] * bi[k]);
+        return t;
+    }
+    void evaluate(Sys& s, std::vector<double>& lp, std::vector<double>& grad, std::vector<double>& lap) {
+        prepare(s.nw);
+        const std::size_t K = nk();
+        for (std::size_t wk = 0; wk < s.nw; ++wk) {
+            auto build = [&](const Group& g, double* re, double* im) {
+                std::fill_n(re, K, 0.0);
+                std::fill_n(im, K, 0.0);
+                for (std::size_t i = g.first; i < g.last; ++i) {
+                    double* c = crow(wk, i);
+                    double* sn = srow(wk, i);
+                    phases(s.P(i, 0, wk), s.P(i, 1, wk), c, sn);
+                    for (std::size_t k = 0; k < K; ++k) { re[k] += c[k]; im[k] += sn[k]; }
+                }
+            };
+            build(a, ra_re.data() + wk * K, ra_im.data() + wk * K);
+            if (!same) build(b, rb_re.data() + wk * K, rb_im.data() + wk * K);
+            const double* ar = ra_re.data() + wk * K;
+            const double* ai = ra_im.data() + wk * K;
+            const double* br = same ? ar : rb_re.data() + wk * K;
+            const double* bi = same ? ai : rb_im.data() + wk * K;
+            lp[wk] += klog(ar, ai, br, bi);
+            auto add = [&](std::size_t i, const double* pr, const double* pi, double self) {
+                const KT t = kterms(crow(wk, i), srow(wk, i), pr, pi, self);
+                grad[(i * 2 + 0) * s.nw + wk] += t.gx;
+                grad[(i * 2 + 1) * s.nw + wk] += t.gy;
+                lap[(i * 2 + 0) * s.nw + wk] += t.lx;
+                lap[(i * 2 + 1) * s.nw + wk] += t.ly;
+            };
+            for (std::size_t i = a.first; i < a.last; ++i) add(i, br, bi, same ? 1.0 : 0.0);
+            if (!same)
+                for (std::size_t j = b.first; j < b.last; ++j) add(j, ar, ai, 0.0);
+        }
+    }
+    void ratio(Sys& s, std::size_t p, std::vector<double>& lr, std::vector<double>& ng,
+               std::vector<double>& nl) {
+        const bool in_a = a.contains(p), in_b = !same && b.contains(p);
+        if (!in_a && !in_b) return;
+        const std::size_t K = nk(), nw_ = s.nw;
+        std::vector<double> qr(K), qi(K);
+        for (std::size_t wk = 0; wk < nw_; ++wk) {
+            double* cn = pend_c.data() + wk * K;
+            double* sn = pend_s.data() + wk * K;
+            phases(s.new_pos[wk], s.new_pos[nw_ + wk], cn, sn);
+            const double* co = crow(wk, p);
+            const double* so = srow(wk, p);
+            const double* mr = (in_a ? ra_re : rb_re).data() + wk * K;
+            const double* mi = (in_a ? ra_im : rb_im).data() + wk * K;
+            double dlog = 0.0;
+            KT t{};
+            if (same) {
+                for (std::size_t k = 0; k < K; ++k) {
+                    qr[k] = mr[k] + cn[k] - co[k];
+                    qi[k] = mi[k] + sn[k] - so[k];
+                    dlog += w[k] * (qr[k] * qr[k] + qi[k] * qi[k] - mr[k] * mr[k] - mi[k] * mi[k]);
+                }
+                t = kterms(cn, sn, qr.data(), qi.data(), 1.0);
+            } else {
+                const double* orr = (in_a ? rb_re : ra_re).data() + wk * K;
+                const double* oi = (in_a ? rb_im : ra_im).data() + wk * K;
+                for (std::size_t k = 0; k < K; ++k)
+                    dlog += 2.0 * w[k] * ((cn[k] - co[k]) * orr[k] + (sn[k] - so[k]) * oi[k]);
+                t = kterms(cn, sn, orr, oi, 0.0);
+            }
+            lr[wk] += dlog;
+            ng[0 * nw_ + wk] += t.gx;
+            ng[1 * nw_ + wk] += t.gy;
+            nl[0 * nw_ + wk] += t.lx;
+            nl[1 * nw_ + wk] += t.ly;
+        }
+    }
+    void accept(Sys& s, std::size_t p, const std::vector<double>& mask, std::vector<double>& lp,
+                std::vector<double>& grad, std::vector<double>& lap) {
+        const bool in_a = a.contains(p), in_b = !same && b.contains(p);
+        if (!in_a && !in_b) return;
+        const std::size_t K = nk(), nw_ = s.nw;
+        std::vector<double> dr(K), di(K);
+        const Group& seers = same ? a : (in_a ? b : a);
+        for (std::size_t wk = 0; wk < nw_; ++wk) {
+            if (mask[wk] == 0.0) continue;
+            const double* cn = pend_c.data() + wk * K;
+            const double* sn = pend_s.data() + wk * K;
+            double* co = crow(wk, p);
+            double* so = srow(wk, p);
+            double* mr = (in_a ? ra_re : rb_re).data() + wk * K;
+            double* mi = (in_a ? ra_im : rb_im).data() + wk * K;
+            double dlog = 0.0;
+            if (same) {
+                for (std::size_t k = 0; k < K; ++k) {
+                    dr[k] = cn[k] - co[k];
+                    di[k] = sn[k] - so[k];
+                    dlog += w[k] * (2.0 * (mr[k] * dr[k] + mi[k] * di[k]) + dr[k] * dr[k] + di[k] * di[k]);
+                }
+            } else {
+                const double* orr = (in_a ? rb_re : ra_re).data() + wk * K;
+                const double* oi = (in_a ? rb_im : ra_im).data() + wk * K;
+                for (std::size_t k = 0; k < K; ++k) {
+                    dr[k] = cn[k] - co[k];
+                    di[k] = sn[k] - so[k];
+                    dlog += 2.0 * w[k] * (dr[k] * orr[k] + di[k] * oi[k]);
+                }
+            }
+            lp[wk] += dlog;
+            for (std::size_t i = seers.first; i < seers.last; ++i) {
+                if (i == p) continue;
+                const KT t = kterms(crow(wk, i), srow(wk, i), dr.data(), di.data(), 0.0);
+                grad[(i * 2 + 0) * nw_ + wk] += t.gx;
+                grad[(i * 2 + 1) * nw_ + wk] += t.gy;
+                lap[(i * 2 + 0) * nw_ + wk] += t.lx;
+                lap[(i * 2 + 1) * nw_ + wk] += t.ly;
+            }
+            for (std::size_t k = 0; k < K; ++k) {
+                mr[k] += dr[k];
+                mi[k] += di[k];
+                co[k] = cn[k];
+                so[k] = sn[k];
+            }
+        }
+    }
+    void permute(const std::vector<std::size_t>& src) {
+        const std::size_t K = nk();
+        auto carry = [&](std::vector<double>& v, std::size_t row) {
+            if (v.empty()) return;
+            std::vector<double> out(src.size() * row);
+            for (std::size_t j = 0; j < src.size(); ++j)
+                std::copy_n(v.data() + src[j] * row, row, out.data() + j * row);
+            v.swap(out);
+        };
+        carry(ra_re, K); carry(ra_im, K); carry(rb_re, K); carry(rb_im, K);
+        carry(pc, n_cached * K); carry(ps, n_cached * K);
+        pend_c.assign(src.size() * K, 0.0);
+        pend_s.assign(src.size() * K, 0.0);
+        nw = src.size();
+    }
+};
+
+// ── Pair Jastrow: PadeJastrow (pade_jastrow.cpp:250-634) and
+//    RadialPairJastrow<Drummond> (radial_pair_jastrow_impl.h:47-350) ────────
+struct PairJastrow : Component {
+    Group ga, gb;
+    bool same;
+    Radial rad;
+    bool has_k = false;
+    KSpace ks;
+    std::vector<double> pg, pl, prop;  // pending p grad/lap [dim*nw]; [k*6*nw + f*nw + w]
+
+    template <class F>
+    void partners(std::size_t p, F&& f) const {
+        if (same) {
+            if (ga.contains(p))
+                for (std::size_t k = ga.first; k < ga.last; ++k)
+                    if (k != p) f(k);
+        } else if (ga.contains(p)) {
+            for (std::size_t k = gb.first; k < gb.last; ++k) f(k);
+        } else if (gb.contains(p)) {
+            for (std::size_t k = ga.first; k < ga.last; ++k) f(k);
+        }
+    }
+    void evaluate(Sys& s) override {
+        const std::size_t nw = s.nw;
+        resize_base(nw, s.np, s.dim);
+        pg.assign(2 * nw, 0.0);
+        pl.assign(2 * nw, 0.0);
+        prop.assign(6 * s.np * nw, 0.0);
+        auto pair = [&](std::size_t i, std::size_t j) {
+            for (std::size_t w = 0; w < nw; ++w) {
+                const double r = s.D(i, j, w);
+                double u, du, d2u;
+                rad.eval(r, u, du, d2u);
+                const double invr = 1.0 / r;
+                log_psi_c[w] += u;
+                double del[2];
+                fold(s, s.P(i, 0, w), s.P(i, 1, w), s.P(j, 0, w), s.P(j, 1, w), del[0], del[1]);
+                for (std::size_t d = 0; d < 2; ++d) {
+                    const double dor = del[d] * invr;
+                    const double g = du * dor;
+                    grad_c[(i * 2 + d) * nw + w] += g;
+                    grad_c[(j * 2 + d) * nw + w] -= g;
+                    const double ld = d2u * dor * dor + du * invr * (1.0 - dor * dor);
+                    lap_c[(i * 2 + d) * nw + w] += ld;
+                    lap_c[(j * 2 + d) * nw + w] += ld;
+                }
+            }
+        };
+        if (same) {
+            for (std::size_t i = ga.first; i < ga.last; ++i)
+                for (std::size_t j = i + 1; j < ga.last; ++j) pair(i, j);
+        } else {
+            for (std::size_t i = ga.first; i < ga.last; ++i)
+                for (std::size_t j = gb.first; j < gb.last; ++j) pair(i, j);
+        }
+        if (has_k) ks.evaluate(s, log_psi_c, grad_c, lap_c);
+    }
+    void ratio(Sys& s, std::size_t p) override {
+        const std::size_t nw = s.nw;
+        std::fill(pg.begin(), pg.end(), 0.0);
+        std::fill(pl.begin(), pl.end(), 0.0);
+        partners(p, [&](std::size_t k) {
+            for (std::size_t w = 0; w < nw; ++w) {
+                double uo, duo, d2uo, un, dun, d2un;
+                rad.eval(s.D(p, k, w), uo, duo, d2uo);
+                const double rnew = s.pending[k * nw + w];
+                rad.eval(rnew, un, dun, d2un);
+                const double invr = 1.0 / rnew;
+                s.log_ratios[w] += un - uo;
+                double* pr = prop.data() + 6 * k * nw;
+                pr[0 * nw + w] = uo; pr[1 * nw + w] = duo; pr[2 * nw + w] = d2uo;
+                pr[3 * nw + w] = un; pr[4 * nw + w] = dun; pr[5 * nw + w] = d2un;
+                double del[2];
+                fold(s, s.new_pos[w], s.new_pos[nw + w], s.P(k, 0, w), s.P(k, 1, w), del[0], del[1]);
+                for (std::size_t d = 0; d < 2; ++d) {
+                    const double dor = del[d] * invr;
+                    pg[d * nw + w] += dun * dor;
+                    pl[d * nw + w] += d2un * dor * dor + dun * invr * (1.0 - dor * dor);
+                }
+            }
+        });
+        if (has_k) ks.ratio(s, p, s.log_ratios, pg, pl);
+        for (std::size_t i = 0; i < 2 * nw; ++i) {
+            s.new_grad[i] += pg[i];
+            s.new_lap[i] += pl[i];
+        }
+    }
+    void accept(Sys& s, std::size_t p, const std::vector<double>& m) override {
+        const std::size_t nw = s.nw;
+        bool acts = false;
+        partners(p, [&](std::size_t k) {
+            acts = true;
+            for (std::size_t w = 0; w < nw; ++w) {
+                const double* pr = prop.data() + 6 * k * nw;
+                const double uo = pr[0 * nw + w], duo = pr[1 * nw + w], d2uo = pr[2 * nw + w];
+                const double un = pr[3 * nw + w], dun = pr[4 * nw + w], d2un = pr[5 * nw + w];
+                const double iro = 1.0 / s.D(p, k, w), irn = 1.0 / s.pending[k * nw + w];
+                log_psi_c[w] += m[w] * (un - uo);
+                double dlo[2], dln[2];
+                fold(s, s.P(k, 0, w), s.P(k, 1, w), s.P(p, 0, w), s.P(p, 1, w), dlo[0], dlo[1]);
+                fold(s, s.P(k, 0, w), s.P(k, 1, w), s.new_pos[w], s.new_pos[nw + w], dln[0], dln[1]);
+                for (std::size_t d = 0; d < 2; ++d) {
+                    const double dor_old = dlo[d] * iro, dor_new = dln[d] * irn;
+                    grad_c[(k * 2 + d) * nw + w] += m[w] * (dun * dor_new - duo * dor_old);
+                    const double lo = d2uo * dor_old * dor_old + duo * iro * (1.0 - dor_old * dor_old);
+                    const double ln = d2un * dor_new * dor_new + dun * irn * (1.0 - dor_new * dor_new);
+                    lap_c[(k * 2 + d) * nw + w] += m[w] * (ln - lo);
+                }
+            }
+        });
+        if (!acts) return;
+        if (has_k) ks.accept(s, p, m, log_psi_c, grad_c, lap_c);
+        for (std::size_t d = 0; d < 2; ++d)
+            for (std::size_t w = 0; w < nw; ++w) {
+                double& g = grad_c[(p * 2 + d) * nw + w];
+                double& l = lap_c[(p * 2 + d) * nw + w];
+                g += m[w] * (pg[d * nw + w] - g);
+                l += m[w] * (pl[d * nw + w] - l);
+            }
+    }
+    void permute(const std::vector<std::size_t>& src, std::size_t old_nw) override {
+        permute_base(src, old_nw);
+        pg.assign(2 * src.size(), 0.0);
+        pl.assign(2 * src.size(), 0.0);
+        prop.assign(6 * (grad_c.size() / (2 * src.size())) * src.size(), 0.0);
+        if (has_k) ks.permute(src);
+    }
+};
+
+// ── Nosanow one-body (src/wavefunction/nosanow_jastrow.cpp:58-174) ─────────
+struct Nosanow : Component {
+    Group g;
+    double alpha = 0.0;
+    std::vector<double> sites, value_c, pv, pg, pl;
+    void terms(const Sys& s, double x0, double x1, std::size_t site, double& u, double* gr,
+               double* lp) const {
+        double d0 = x0 - sites[site * 2], d1 = x1 - sites[site * 2 + 1];
+        if (s.cell.periodic) s.cell.min_image(d0, d1);
+        u = 0.0;
+        u -= alpha * d0 * d0;
+        u -= alpha * d1 * d1;
+        gr[0] = -2.0 * alpha * d0;
+        gr[1] = -2.0 * alpha * d1;
+        lp[0] = lp[1] = -2.0 * alpha;
+    }
+    void evaluate(Sys& s) override {
+        const std::size_t nw = s.nw;
+        resize_base(nw, s.np, 2);
+        value_c.assign(s.np * nw, 0.0);
+        pv.assign(nw, 0.0);
+        pg.assign(2 * nw, 0.0);
+        pl.assign(2 * nw, 0.0);
+        for (std::size_t p = g.first; p < g.last; ++p)
+            for (std::size_t w = 0; w < nw; ++w) {
+                double u, gr[2], lp[2];
+                terms(s, s.P(p, 0, w), s.P(p, 1, w), p - g.first, u, gr, lp);
+                value_c[p * nw + w] = u;
+                log_psi_c[w] += u;
+                for (int d = 0; d < 2; ++d) {
+                    grad_c[(p * 2 + d) * nw + w] = gr[d];
+                    lap_c[(p * 2 + d) * nw + w] = lp[d];
+                }
+            }
+    }
+    void ratio(Sys& s, std::size_t p) override {
+        if (!g.contains(p)) return;
+        const std::size_t nw = s.nw;
+        for (std::size_t w = 0; w < nw; ++w) {
+            double u, gr[2], lp[2];
+            terms(s, s.new_pos[w], s.new_pos[nw + w], p - g.first, u, gr, lp);
+            pv[w] = u;
+            s.log_ratios[w] += u - value_c[p * nw + w];
+            for (int d = 0; d < 2; ++d) {
+                pg[d * nw + w] = gr[d];
+                pl[d * nw + w] = lp[d];
+                s.new_grad[d * nw + w] += gr[d];
+                s.new_lap[d * nw + w] += lp[d];
+            }
+        }
+    }
+    void accept(Sys& s, std::size_t p, const std::vector<double>& m) override {
+        if (!g.contains(p)) return;
+        const std::size_t nw = s.nw;
+        for (std::size_t w = 0; w < nw; ++w) {
+            const double dlog = m[w] * (pv[w] - value_c[p * nw + w]);
+            log_psi_c[w] += dlog;
+            if (m[w] != 0.0) {
+                value_c[p * nw + w] = pv[w];
+                for (int d = 0; d < 2; ++d) {
+                    grad_c[(p * 2 + d) * nw + w] = pg[d * nw + w];
+                    lap_c[(p * 2 + d) * nw + w] = pl[d * nw + w];
+                }
+            }
+        }
+    }
+    void permute(const std::vector<std::size_t>& src, std::size_t old_nw) override {
+        permute_base(src, old_nw);
+        gather(value_c, src, old_nw);
+        pv.assign(src.size(), 0.0);
+        pg.assign(2 * src.size(), 0.0);
+        pl.assign(2 * src.size(), 0.0);
+    }
+};
+
+// ── Orbital sets ────────────────────────────────────────────────────────────
+// Real: PlaneWaveOrbitalSet (orbital_set.h:427-474), GaussianOrbitalSet
+// (orbital_set.h:164-224).  Values at one position, val[m], grad[m*2+d],
+// lap[m*2+d].
+struct RealOrbitals {
+    bool gaussian = false;
+    std::vector<double> k;          // [n*2]
+    std::vector<unsigned char> is_sin;
+    std::vector<double> centers, alphas;
+    std::size_t n = 0;
+    void eval(const Cell& cell, double x0, double x1, double* val, double* grad, double* lap) const {
+        for (std::size_t i = 0; i < n; ++i) {
+            if (gaussian) {
+                double d0 = x0 - centers[i * 2], d1 = x1 - centers[i * 2 + 1];
+                if (cell.periodic) cell.min_image(d0, d1);
+                double r2 = 0.0;
+                r2 += d0 * d0;
+                r2 += d1 * d1;
+                const double a = alphas[i];
+                const double phi = std::exp(-a * r2);
+                val[i] = phi;
+                const double del[2] = {d0, d1};
+                for (int d = 0; d < 2; ++d) {
+                    grad[i * 2 + d] = phi * (-2.0 * a * del[d]);
+                    lap[i * 2 + d] = phi * (4.0 * a * a * del[d] * del[d] - 2.0 * a);
+                }
+            } else {
+                double phase = 0.0;
+                phase += k[i * 2] * x0;
+                phase += k[i * 2 + 1] * x1;
+                const double s = std::sin(phase), c = std::cos(phase);
+                val[i] = is_sin[i] ? s : c;
+                for (int d = 0; d < 2; ++d) {
+                    const double kd = k[i * 2 + d];
+                    grad[i * 2 + d] = is_sin[i] ? kd * c : -kd * s;
+                    lap[i * 2 + d] = is_sin[i] ? -kd * kd * s : -kd * kd * c;
+                }
+            }
+        }
+    }
+};
+
+// Pivoted Gauss-Jordan inverse of a row-major n x n matrix (the reference's
+// rebuild_inverse, slater_determinant.cpp:905-993; also the stand-in for
+// Eigen's D.inverse() at evaluate time).  Returns log|det|; throws if singular.
+template <class T>
+double gauss_jordan(std::size_t N, const T* d, T* a) {
+    std::vector<T> m(d, d + N * N);
+    for (std::size_t r = 0; r < N; ++r)
+        for (std::size_t c = 0; c < N; ++c) a[r * N + c] = r == c ? T(1.0) : T(0.0);
+    double log_det = 0.0;
+    for (std::size_t k = 0; k < N; ++k) {
+        std::size_t piv = k;
+        for (std::size_t r = k + 1; r < N; ++r)
+            if (std::abs(m[r * N + k]) > std::abs(m[piv * N + k])) piv = r;
+        const T diag = m[piv * N + k];
+        if (std::abs(diag) == 0.0 || !std::isfinite(std::abs(diag)))
+            throw std::runtime_error("Slater determinant: accepted matrix is singular or nonfinite");
+        log_det += std::log(std::abs(diag));
+        if (piv != k) {
+            std::swap_ranges(m.begin() + k * N + k, m.begin() + k * N + N, m.begin() + piv * N + k);
+            std::swap_ranges(a + k * N, a + k * N + N, a + piv * N);
+        }
+        for (std::size_t c = k + 1; c < N; ++c) m[k * N + c] /= diag;
+        for (std::size_t c = 0; c < N; ++c) a[k * N + c] /= diag;
+        for (std::size_t r = 0; r < N; ++r) {
+            if (r == k) continue;
+            const T f = m[r * N + k];
+            for (std::size_t c = k + 1; c < N; ++c) m[r * N + c] -= f * m[k * N + c];
+            for (std::size_t c = 0; c < N; ++c) a[r * N + c] -= f * a[k * N + c];
+        }
+    }
+    return log_det;
+}
+
+// ── SlaterDeterminant, real (slater_determinant.cpp:305-437, 744-900) ──────
+// Matrices are contiguous per walker (the reference's layout for N >= 32 or
+// nw >= 64, slater_determinant.cpp:26-44).
+struct RealSlater : Component {
+    struct Block {
+        std::size_t first, last, N;
+        RealOrbitals orb;
+        std::vector<double> A, D, dD, d2D;  // [w*N*N + ...], dD [w*N*N*2 + (kr*N+m)*2+d]
+        std::vector<double> logd, cond, row_abs;
+        std::vector<int> updates;
+        std::vector<double> nval, ngrad, nlap;  // proposal [w*N + m], [w*N*2 + m*2 + d]
+    };
+    std::vector<Block> blocks;
+    std::size_t block_of(std::size_t p) const {
+        for (std::size_t b = 0; b < blocks.size(); ++b)
+            if (p >= blocks[b].first && p < blocks[b].last) return b;
+        return SIZE_MAX;
+    }
+    void particle_derivs(const Sys& s, Block& B, std::size_t kr, std::size_t w) {
+        const std::size_t N = B.N, nw = s.nw, particle = B.first + kr;
+        const double* A = B.A.data() + w * N * N;
+        for (std::size_t d = 0; d < 2; ++d) {
+            double G = 0.0, Lr = 0.0;
+            for (std::size_t m = 0; m < N; ++m) {
+                const double a = A[m * N + kr];
+                G += a * B.dD[w * N * N * 2 + (kr * N + m) * 2 + d];
+                Lr += a * B.d2D[w * N * N * 2 + (kr * N + m) * 2 + d];
+            }
+            grad_c[(particle * 2 + d) * nw + w] = G;
+            lap_c[(particle * 2 + d) * nw + w] = Lr - G * G;
+        }
+    }
+    void evaluate(Sys& s) override {
+        const std::size_t nw = s.nw;
+        resize_base(nw, s.np, 2);
+        for (auto& B : blocks) {
+            const std::size_t N = B.N;
+            B.A.assign(nw * N * N, 0.0);
+            B.D.assign(nw * N * N, 0.0);
+            B.dD.assign(nw * N * N * 2, 0.0);
+            B.d2D.assign(nw * N * N * 2, 0.0);
+            B.logd.assign(nw, 0.0);
+            B.cond.assign(nw, 1.0);
+            B.row_abs.assign(N * nw, 0.0);
+            B.updates.assign(nw, 0);
+            B.nval.assign(nw * N, 0.0);
+            B.ngrad.assign(nw * N * 2, 0.0);
+            B.nlap.assign(nw * N * 2, 0.0);
+            for (std::size_t w = 0; w < nw; ++w) {
+                for (std::size_t kr = 0; kr < N; ++kr) {
+                    const std::size_t i = B.first + kr;
+                    B.orb.eval(s.cell, s.P(i, 0, w), s.P(i, 1, w), B.D.data() + w * N * N + kr * N,
+                               B.dD.data() + w * N * N * 2 + kr * N * 2,
+                               B.d2D.data() + w * N * N * 2 + kr * N * 2);
+                }
+                const double* Dw = B.D.data() + w * N * N;
+                double* Aw = B.A.data() + w * N * N;
+                for (std::size_t kr = 0; kr < N; ++kr) {
+                    double rs = 0.0;
+                    for (std::size_t i = 0; i < N; ++i) rs += std::abs(Dw[kr * N + i]);
+                    B.row_abs[kr * nw + w] = rs;
+                }
+                B.logd[w] = std::max(gauss_jordan<double>(N, Dw, Aw), std::log(kDetFloor));
+                log_psi_c[w] += B.logd[w];
+                double nd = 0.0, na = 0.0;
+                for (std::size_t r = 0; r < N; ++r) {
+                    double sd = 0.0, sa = 0.0;
+                    for (std::size_t c = 0; c < N; ++c) { sd += std::abs(Dw[r * N + c]); sa += std::abs(Aw[r * N + c]); }
+                    nd = std::max(nd, sd);
+                    na = std::max(na, sa);
+                }
+                B.cond[w] = nd * na;
+                for (std::size_t kr = 0; kr < N; ++kr) particle_derivs(s, B, kr, w);
+            }
+        }
+    }
+    void ratio(Sys& s, std::size_t p) override {
+        const std::size_t b = block_of(p);
+        if (b == SIZE_MAX) return;
+        Block& B = blocks[b];
+        const std::size_t N = B.N, j = p - B.first, nw = s.nw;
+        for (std::size_t w = 0; w < nw; ++w) {
+            double* v = B.nval.data() + w * N;
+            double* g = B.ngrad.data() + w * N * 2;
+            double* l = B.nlap.data() + w * N * 2;
+            B.orb.eval(s.cell, s.new_pos[w], s.new_pos[nw + w], v, g, l);
+            const double* A = B.A.data() + w * N * N;
+            double R = 0.0;
+            for (std::size_t m = 0; m < N; ++m) R += A[m * N + j] * v[m];
+            s.log_ratios[w] += std::log(std::max(std::abs(R), kDetFloor));
+            if (R < 0.0) s.phase_diffs[w] += kPi;
+            const double invR = (R != 0.0) ? 1.0 / R : 0.0;
+            for (std::size_t d = 0; d < 2; ++d) {
+                double G = 0.0, Lr = 0.0;
+                for (std::size_t m = 0; m < N; ++m) {
+                    G += A[m * N + j] * g[m * 2 + d];
+                    Lr += A[m * N + j] * l[m * 2 + d];
+                }
+                const double Gp = invR * G;
+                s.new_grad[d * nw + w] += Gp;
+                s.new_lap[d * nw + w] += invR * Lr - Gp * Gp;
+            }
+        }
+    }
+    double rebuild(Block& B, std::size_t w) {
+        const std::size_t N = B.N;
+        const double ld = gauss_jordan<double>(N, B.D.data() + w * N * N, B.A.data() + w * N * N);
+        B.updates[w] = 0;
+        return ld;
+    }
+    // commit_walker (slater_determinant.cpp:744-858), deferred partners.
+    void accept(Sys& s, std::size_t p, const std::vector<double>& m) override {
+        const std::size_t b = block_of(p);
+        if (b == SIZE_MAX) return;
+        Block& B = blocks[b];
+        const std::size_t N = B.N, j = p - B.first, nw = s.nw;
+        std::vector<double> column(N), rowp(N);
+        for (std::size_t w = 0; w < nw; ++w) {
+            if (m[w] == 0.0) continue;
+            double* A = B.A.data() + w * N * N;
+            double* Dv = B.D.data() + w * N * N;
+            double* new_row = Dv + j * N;
+            for (std::size_t mm = 0; mm < N; ++mm) new_row[mm] = B.nval[w * N + mm];
+            for (std::size_t mm = 0; mm < N; ++mm)
+                for (std::size_t d = 0; d < 2; ++d) {
+                    B.dD[w * N * N * 2 + (j * N + mm) * 2 + d] = B.ngrad[w * N * 2 + mm * 2 + d];
+                    B.d2D[w * N * N * 2 + (j * N + mm) * 2 + d] = B.nlap[w * N * 2 + mm * 2 + d];
+                }
+            double ratio = 0.0, scale = 0.0;
+            std::fill(rowp.begin(), rowp.end(), 0.0);
+            for (std::size_t mm = 0; mm < N; ++mm) {
+                const double* ir = A + mm * N;
+                const double v = new_row[mm];
+                const double term = ir[j] * v;
+                ratio += term;
+                scale += std::abs(term);
+                column[mm] = ir[j];
+                for (std::size_t c = 0; c < N; ++c) rowp[c] += v * ir[c];
+            }
+            rowp[j] -= 1.0;
+            const double old_log = B.logd[w];
+            const bool rb = ++B.updates[w] >= 64 || !std::isfinite(ratio) || std::abs(ratio) < 1e-5 ||
+                            B.cond[w] > 1e8 ||
+                            std::abs(ratio) <= 1e4 * std::numeric_limits<double>::epsilon() * scale;
+            double norm_a = 0.0;
+            if (rb) {
+                B.logd[w] = rebuild(B, w);
+                for (std::size_t a = 0; a < N; ++a) {
+                    double rs = 0.0;
+                    for (std::size_t c = 0; c < N; ++c) rs += std::abs(A[a * N + c]);
+                    norm_a = std::max(norm_a, rs);
+                }
+            } else {
+                B.logd[w] += std::log(std::abs(ratio));
+                for (std::size_t a = 0; a < N; ++a) {
+                    const double f = column[a] / ratio;
+                    double rs = 0.0;
+                    for (std::size_t c = 0; c < N; ++c) {
+                        A[a * N + c] = A[a * N + c] - f * rowp[c];
+                        rs += std::abs(A[a * N + c]);
+                    }
+                    norm_a = std::max(norm_a, rs);
+                }
+            }
+            double nrs = 0.0;
+            for (std::size_t c = 0; c < N; ++c) nrs += std::abs(new_row[c]);
+            B.row_abs[j * nw + w] = nrs;
+            double norm_d = 0.0;
+            for (std::size_t a = 0; a < N; ++a) norm_d = std::max(norm_d, B.row_abs[a * nw + w]);
+            B.cond[w] = norm_d * norm_a;
+            if (!rb && (!std::isfinite(B.cond[w]) || B.cond[w] > 1e8)) B.logd[w] = rebuild(B, w);
+            log_psi_c[w] += B.logd[w] - old_log;
+            particle_derivs(s, B, j, w);
+        }
+    }
+    void refresh_particle(Sys& s, std::size_t p) override {
+        const std::size_t b = block_of(p);
+        if (b == SIZE_MAX) return;
+        for (std::size_t w = 0; w < s.nw; ++w) particle_derivs(s, blocks[b], p - blocks[b].first, w);
+    }
+    void permute(const std::vector<std::size_t>& src, std::size_t old_nw) override {
+        permute_base(src, old_nw);
+        for (auto& B : blocks) {
+            auto carry = [&](std::vector<double>& v) {
+                const std::size_t row = v.size() / old_nw;
+                std::vector<double> out(src.size() * row);
+                for (std::size_t j = 0; j < src.size(); ++j)
+                    std::copy_n(v.data() + src[j] * row, row, out.data() + j * row);
+                v.swap(out);
+            };
+            carry(B.A); carry(B.D); carry(B.dD); carry(B.d2D);
+            gather(B.logd, src, old_nw);
+            gather(B.cond, src, old_nw);
+            gather(B.row_abs, src, old_nw);
+            std::vector<int> up(src.size());
+            for (std::size_t j = 0; j < src.size(); ++j) up[j] = B.updates[src[j]];
+            B.updates.swap(up);
+            B.nval.assign(src.size() * B.N, 0.0);
+            B.ngrad.assign(src.size() * B.N * 2, 0.0);
+            B.nlap.assign(src.size() * B.N * 2, 0.0);
+        }
+    }
+};
+
+// ── ComplexSlaterDeterminant with twisted plane waves
+//    (complex_slater_determinant.cpp:85-283, complex_orbital_set.h:91-124) ──
+struct ComplexSlater : Component {
+    struct Block {
+        std::size_t first, last, N;
+        std::vector<double> q;  // [N*2]
+        std::vector<cplx> A;    // [w*N*N + a*N + c]
+        std::vector<cplx> nval, ngrad, nlap;  // [w*N + m], [w*N*2 + m*2 + d]
+    };
+    std::vector<Block> blocks;
+    bool complex() const override { return true; }
+    std::size_t block_of(std::size_t p) const {
+        for (std::size_t b = 0; b < blocks.size(); ++b)
+            if (p >= blocks[b].first && p < blocks[b].last) return b;
+        return SIZE_MAX;
+    }
+    static void orb(const Block& B, double x0, double x1, cplx* val, cplx* grad, cplx* lap) {
+        for (std::size_t i = 0; i < B.N; ++i) {
+            const double* qi = B.q.data() + i * 2;
+            double arg = 0.0;
+            arg += qi[0] * x0;
+            arg += qi[1] * x1;
+            const double c = std::cos(arg), s = std::sin(arg);
+            val[i] = cplx(c, s);
+            for (int d = 0; d < 2; ++d) {
+                grad[i * 2 + d] = cplx(-qi[d] * s, qi[d] * c);
+                lap[i * 2 + d] = cplx(-qi[d] * qi[d] * c, -qi[d] * qi[d] * s);
+            }
+        }
+    }
+    // Rows at a configuration, p at (x0,x1) if moved < N; inverse; derivatives.
+    void full(Sys& s, Block& B, std::size_t w, std::size_t moved, bool set_log) {
+        const std::size_t N = B.N, nw = s.nw;
+        std::vector<cplx> D(N * N), G(N * N * 2), L(N * N * 2);
+        for (std::size_t kr = 0; kr < N; ++kr) {
+            const std::size_t i = B.first + kr;
+            const double x0 = (i == moved) ? s.new_pos[w] : s.P(i, 0, w);
+            const double x1 = (i == moved) ? s.new_pos[nw + w] : s.P(i, 1, w);
+            orb(B, x0, x1, D.data() + kr * N, G.data() + kr * N * 2, L.data() + kr * N * 2);
+        }
+        cplx* A = B.A.data() + w * N * N;
+        const double ld = gauss_jordan<cplx>(N, D.data(), A);
+        if (set_log) log_psi_c[w] += std::max(ld, std::log(kDetFloor));
+        for (std::size_t kr = 0; kr < N; ++kr) {
+            const std::size_t i = B.first + kr;
+            for (std::size_t d = 0; d < 2; ++d) {
+                cplx g(0.0, 0.0), lr(0.0, 0.0);
+                for (std::size_t m = 0; m < N; ++m) {
+                    g += A[m * N + kr] * G[(kr * N + m) * 2 + d];
+                    lr += A[m * N + kr] * L[(kr * N + m) * 2 + d];
+                }
+                const cplx l = lr - g * g;
+                const std::size_t idx = (i * 2 + d) * nw + w;
+                grad_c[idx] = g.real();
+                lap_c[idx] = l.real();
+                pgrad_c[idx] = g.imag();
+                plap_c[idx] = l.imag();
+            }
+        }
+    }
+    void evaluate(Sys& s) override {
+        const std::size_t nw = s.nw;
+        resize_base(nw, s.np, 2);
+        for (auto& B : blocks) {
+            B.A.assign(nw * B.N * B.N, cplx(0.0, 0.0));
+            B.nval.assign(nw * B.N, cplx(0.0, 0.0));
+            B.ngrad.assign(nw * B.N * 2, cplx(0.0, 0.0));
+            B.nlap.assign(nw * B.N * 2, cplx(0.0, 0.0));
+            for (std::size_t w = 0; w < nw; ++w) full(s, B, w, SIZE_MAX, true);
+        }
+    }
+    void ratio(Sys& s, std::size_t p) override {
+        const std::size_t b = block_of(p);
+        if (b == SIZE_MAX) return;
+        Block& B = blocks[b];
+        const std::size_t N = B.N, j = p - B.first, nw = s.nw;
+        for (std::size_t w = 0; w < nw; ++w) {
+            cplx* v = B.nval.data() + w * N;
+            cplx* g = B.ngrad.data() + w * N * 2;
+            cplx* l = B.nlap.data() + w * N * 2;
+            orb(B, s.new_pos[w], s.new_pos[nw + w], v, g, l);
+            const cplx* A = B.A.data() + w * N * N;
+            cplx R(0.0, 0.0);
+            for (std::size_t m = 0; m < N; ++m) R += A[m * N + j] * v[m];
+            const double absR = std::abs(R);
+            s.log_ratios[w] += std::log(std::max(absR, kDetFloor));
+            s.phase_diffs[w] += std::arg(R);
+            const cplx invR = absR > 0.0 ? 1.0 / R : cplx(0.0, 0.0);
+            for (std::size_t d = 0; d < 2; ++d) {
+                cplx G(0.0, 0.0), Lr(0.0, 0.0);
+                for (std::size_t m = 0; m < N; ++m) {
+                    G += A[m * N + j] * g[m * 2 + d];
+                    Lr += A[m * N + j] * l[m * 2 + d];
+                }
+                const cplx Gp = invR * G;
+                const cplx Lp = invR * Lr - Gp * Gp;
+                s.new_grad[d * nw + w] += Gp.real();
+                s.new_lap[d * nw + w] += Lp.real();
+                s.new_pgrad[d * nw + w] += Gp.imag();
+                s.new_plap[d * nw + w] += Lp.imag();
+            }
+        }
+    }
+    void accept(Sys& s, std::size_t p, const std::vector<double>& m) override {
+        const std::size_t b = block_of(p);
+        if (b == SIZE_MAX) return;
+        Block& B = blocks[b];
+        const std::size_t N = B.N, j = p - B.first, nw = s.nw;
+        for (std::size_t w = 0; w < nw; ++w) {
+            if (m[w] == 0.0) continue;
+            const cplx* A = B.A.data() + w * N * N;
+            cplx R(0.0, 0.0);
+            for (std::size_t mm = 0; mm < N; ++mm) R += A[mm * N + j] * B.nval[w * N + mm];
+            log_psi_c[w] += std::log(std::max(std::abs(R), kDetFloor));
+            full(s, B, w, p, false);
+        }
+    }
+    void permute(const std::vector<std::size_t>& src, std::size_t old_nw) override {
+        permute_base(src, old_nw);
+        for (auto& B : blocks) {
+            const std::size_t row = B.N * B.N;
+            std::vector<cplx> out(src.size() * row);
+            for (std::size_t j = 0; j < src.size(); ++j)
+                std::copy_n(B.A.data() + src[j] * row, row, out.data() + j * row);
+            B.A.swap(out);
+            B.nval.assign(src.size() * B.N, cplx(0.0, 0.0));
+            B.ngrad.assign(src.size() * B.N * 2, cplx(0.0, 0.0));
+            B.nlap.assign(src.size() * B.N * 2, cplx(0.0, 0.0));
+        }
+    }
+};
+
+// ── Hamiltonian ────────────────────────────────────────────────────────────
+// EwaldTerm::evaluate_potential (ewald_term.cpp:368-612), single Coulomb
+// kernel: real-space V - V_long strictly inside r_cut, reciprocal sum from
+// exponential ladders, self + background constants (ewald_term.cpp:186-262).
+void ewald_energy(Sys& s, std::vector<double>& le) {
+    const Ewald& e = s.ewald;
+    const std::size_t nw = s.nw, np = s.np;
+    std::vector<double> q(np);
+    double q2 = 0.0, Q = 0.0;
+    for (const auto& g : s.groups)
+        for (std::size_t i = g.first; i < g.last; ++i) {
+            q[i] = g.charge;
+            q2 += g.charge * g.charge;
+            Q += g.charge;
+        }
+    for (std::size_t i = 0; i < np; ++i)
+        for (std::size_t j = i + 1; j < np; ++j) {
+            const double coeff = q[i] * q[j];
+            if (coeff == 0.0) continue;
+            for (std::size_t w = 0; w < nw; ++w) {
+                const double r = s.D(i, j, w);
+                const double v = e.inv_eps / r;
+                le[w] += r < e.r_cut ? coeff * (v - e.sp.value(r)) : 0.0;
+            }
+        }
+    const double b1x = s.cell.b[0], b1y = s.cell.b[1], b2x = s.cell.b[2], b2y = s.cell.b[3];
+    for (std::size_t w = 0; w < nw; ++w) {
+        const int M1 = e.n1_max, M2 = e.n2_max;
+        std::vector<double> e1r((M1 + 1) * np), e1i((M1 + 1) * np), e2r((M2 + 1) * np), e2i((M2 + 1) * np);
+        auto build = [&](double bx, double by, int nmax, std::vector<double>& er, std::vector<double>& ei) {
+            for (std::size_t i = 0; i < np; ++i) {
+                er[i] = 1.0;
+                ei[i] = 0.0;
+                if (nmax < 1) continue;
+                const double ph = bx * s.P(i, 0, w) + by * s.P(i, 1, w);
+                er[np + i] = std::cos(ph);
+                ei[np + i] = std::sin(ph);
+                for (int n = 2; n <= nmax; ++n) {
+                    const double pr = er[(n - 1) * np + i], pi = ei[(n - 1) * np + i];
+                    er[n * np + i] = pr * er[np + i] - pi * ei[np + i];
+                    ei[n * np + i] = pr * ei[np + i] + pi * er[np + i];
+                }
+            }
+        };
+        build(b1x, b1y, M1, e1r, e1i);
+        build(b2x, b2y, M2, e2r, e2i);
+        for (std::size_t kk = 0; kk < e.w.size(); ++kk) {
+            const int n1 = e.n1[kk], n2 = e.n2[kk];
+            const std::size_t a2 = static_cast<std::size_t>(std::abs(n2));
+            const double sgn = n2 < 0 ? -1.0 : 1.0;
+            double rr = 0.0, ri = 0.0;
+            for (std::size_t i = 0; i < np; ++i) {
+                if (q[i] == 0.0) continue;
+                const double ar = e1r[n1 * np + i], ai = e1i[n1 * np + i];
+                const double br = e2r[a2 * np + i], bim = sgn * e2i[a2 * np + i];
+                rr += q[i] * (ar * br - ai * bim);
+                ri += q[i] * (ar * bim + ai * br);
+            }
+            le[w] += e.w[kk] * (rr * rr + ri * ri);
+        }
+    }
+    const double constants = -0.5 * e.v_long_zero * q2 - 0.5 * Q * Q * e.sr_k0;
+    for (std::size_t w = 0; w < nw; ++w) le[w] += constants;
+}
+
+// CoulombTerm::evaluate_potential (coulomb.cpp:31-61), in-plane (d = 0).
+void coulomb_energy(Sys& s, const CoulombPair& c, std::vector<double>& le) {
+    const Group& a = s.groups[c.ga];
+    const Group& b = s.groups[c.gb];
+    const bool intra = c.ga == c.gb;
+    const double coeff = a.charge * b.charge * c.inv_eps;
+    for (std::size_t i = a.first; i < a.last; ++i)
+        for (std::size_t j = intra ? i + 1 : b.first; j < (intra ? a.last : b.last); ++j)
+            for (std::size_t w = 0; w < s.nw; ++w) {
+                const double r = s.D(i, j, w);
+                le[w] += coeff / std::sqrt(r * r + 0.0);
+            }
+}
+
+void totals(Sys& s, std::vector<double>& g, std::vector<double>& l, std::vector<double>* pg,
+            std::vector<double>* pl) {
+    const std::size_t n = s.np * 2 * s.nw;
+    g.assign(n, 0.0);
+    l.assign(n, 0.0);
+    if (pg) pg->assign(n, 0.0);
+    if (pl) pl->assign(n, 0.0);
+    for (auto& c : s.comps) {
+        for (std::size_t i = 0; i < n; ++i) { g[i] += c->grad_c[i]; l[i] += c->lap_c[i]; }
+        if (c->complex() && pg && pl)
+            for (std::size_t i = 0; i < n; ++i) { (*pg)[i] += c->pgrad_c[i]; (*pl)[i] += c->plap_c[i]; }
+    }
+}
+
+void refresh_all(Sys& s) {
+    for (auto& c : s.comps)
+        for (std::size_t p = 0; p < s.np; ++p) c->refresh_particle(s, p);
+}
+
+// Kinetic::evaluate_batch + add_phase_terms (kinetic.cpp:45-110), then the
+// potential terms: Hamiltonian::evaluate_batch (hamiltonian.cpp:18-29).
+void local_energy(Sys& s) {
+    refresh_all(s);
+    std::vector<double> g, l, pg, pl;
+    totals(s, g, l, &pg, &pl);
+    const std::size_t nw = s.nw;
+    std::fill(s.e_loc.begin(), s.e_loc.end(), 0.0);
+    std::fill(s.e_loc_im.begin(), s.e_loc_im.end(), 0.0);
+    for (std::size_t i = 0; i < s.np; ++i)
+        for (std::size_t d = 0; d < 2; ++d) {
+            const double im = s.inv_mass[i];
+            const std::size_t off = (i * 2 + d) * nw;
+            for (std::size_t w = 0; w < nw; ++w) s.e_loc[w] += -0.5 * im * (l[off + w] + g[off + w] * g[off + w]);
+        }
+    if (s.has_phase)
+        for (std::size_t i = 0; i < s.np; ++i)
+            for (std::size_t d = 0; d < 2; ++d) {
+                const double im = s.inv_mass[i];
+                const std::size_t off = (i * 2 + d) * nw;
+                for (std::size_t w = 0; w < nw; ++w) {
+                    s.e_loc[w] += 0.5 * im * pg[off + w] * pg[off + w];
+                    s.e_loc_im[w] += -0.5 * im * (pl[off + w] + 2.0 * g[off + w] * pg[off + w]);
+                }
+            }
+    if (s.ewald.active) ewald_energy(s, s.e_loc);
+    for (const auto& c : s.coulomb) coulomb_energy(s, c, s.e_loc);
+}
+
+// TrialWavefunction::ratio_batch (trial_wavefunction.cpp:46-78).
+void ratio_all(Sys& s, std::size_t p) {
+    s.propose(p);
+    std::fill(s.log_ratios.begin(), s.log_ratios.end(), 0.0);
+    std::fill(s.phase_diffs.begin(), s.phase_diffs.end(), 0.0);
+    std::fill(s.new_grad.begin(), s.new_grad.end(), 0.0);
+    std::fill(s.new_lap.begin(), s.new_lap.end(), 0.0);
+    std::fill(s.new_pgrad.begin(), s.new_pgrad.end(), 0.0);
+    std::fill(s.new_plap.begin(), s.new_plap.end(), 0.0);
+    for (auto& c : s.comps) c->ratio(s, p);
+}
+
+// Deferred commit, then WalkerSet::apply_move (vmc.cpp:147-152).
+void accept_all(Sys& s, std::size_t p, const std::vector<double>& m) {
+    for (auto& c : s.comps) c->accept(s, p, m);
+    s.commit(p, m);
+}
+
+double log_psi(const Sys& s, std::size_t w) {
+    double t = 0.0;
+    for (const auto& c : s.comps) t += c->log_psi_c[w];
+    return t;
+}
+
+Sys* S(oracle_system* h) { return reinterpret_cast<Sys*>(h); }
+
+void check_index(const Sys& s, int g) {
+    if (g < 0 || static_cast<std::size_t>(g) >= s.groups.size())
+        throw std::invalid_argument("oracle: group index out of range");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error(void) { return g_error.c_str(); }
+
+uint64_t oracle_stream_seed(uint64_t base, uint64_t stream, uint64_t index) {
+    return stream_seed(base, stream, index);
+}
+
+int oracle_xoshiro_state_u64(const uint64_t* state, int n, uint64_t* out) {
+    Xoshiro g(state);
+    for (int i = 0; i < n; ++i) out[i] = g.next();
+    return 0;
+}
+
+int oracle_xoshiro_draws(uint64_t seed, int kind, int n, double* out) {
+    Rng r(seed);
+    for (int i = 0; i < n; ++i) out[i] = kind == 0 ? r.uniform() : r.gaussian();
+    return 0;
+}
+
+int oracle_sweep_tape(uint64_t seed, int np, int dim, int nw, int n_sweeps, double* out) {
+    Rng r(seed);
+    const std::size_t len = static_cast<std::size_t>(np) * (dim + 1) * nw;
+    for (int s = 0; s < n_sweeps; ++s) draw_sweep(r, np, dim, nw, 0, nw, out + s * len);
+    return 0;
+}
+
+int oracle_ziggurat_tables(uint64_t* k, double* w, double* f, double* r) {
+    const Ziggurat& z = ziggurat();
+    for (int i = 0; i < 256; ++i) { k[i] = z.k[i]; w[i] = z.w[i]; f[i] = z.f[i]; }
+    *r = z.r;
+    return 0;
+}
+
+int oracle_create(oracle_system** out, int nw, int np, int dim, const double* lattice,
+                  int precision) {
+    return guarded([&] {
+        (void)precision;
+        if (dim != 2) throw std::invalid_argument("oracle: dim must be 2");
+        if (nw <= 0 || np <= 0) throw std::invalid_argument("oracle: empty batch");
+        auto* s = new Sys;
+        s->nw = static_cast<std::size_t>(nw);
+        s->np = static_cast<std::size_t>(np);
+        s->dim = 2;
+        s->cell.init(2, lattice);
+        s->alloc();
+        s->inv_mass.assign(s->np, 1.0);
+        *out = reinterpret_cast<oracle_system*>(s);
+    });
+}
+
+int oracle_destroy(oracle_system* h) {
+    delete S(h);
+    return 0;
+}
+
+int oracle_add_group(oracle_system* h, int first, int last, double charge, double inv_mass) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        if (first < 0 || last < first || static_cast<std::size_t>(last) > s.np)
+            throw std::invalid_argument("oracle: group range out of bounds");
+        s.groups.push_back({static_cast<std::size_t>(first), static_cast<std::size_t>(last), charge, inv_mass});
+        for (int i = first; i < last; ++i) s.inv_mass[i] = inv_mass;
+    });
+}
+
+int oracle_add_pade_jastrow(oracle_system* h, int ga, int gb, int num_order, int den_order,
+                            const double* coeffs, int n_k, const int* n1, const int* n2,
+                            const double* kx, const double* ky, const double* weights,
+                            int n_knots, const double* spline, double r_cut, double inv_delta,
+                            double u_inf) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        check_index(s, ga);
+        check_index(s, gb);
+        auto c = std::make_unique<PairJastrow>();
+        c->ga = s.groups[ga];
+        c->gb = s.groups[gb];
+        c->same = ga == gb;
+        c->rad.num = num_order;
+        c->rad.den = den_order;
+        c->rad.c.assign(coeffs, coeffs + num_order + den_order);
+        if (n_k > 0) {
+            if (!s.cell.periodic) throw std::invalid_argument("oracle: image sum needs a periodic cell");
+            c->rad.image = true;
+            c->rad.sp.scaled.assign(spline, spline + 3 * n_knots);
+            c->rad.sp.n_knots = n_knots;
+            c->rad.sp.r_cut = r_cut;
+            c->rad.sp.inv_delta = inv_delta;
+            c->rad.u_inf = u_inf;
+            c->has_k = true;
+            KSpace& k = c->ks;
+            k.a = c->ga;
+            k.b = c->gb;
+            k.same = c->same;
+            k.n1.assign(n1, n1 + n_k);
+            k.n2.assign(n2, n2 + n_k);
+            k.kx.assign(kx, kx + n_k);
+            k.ky.assign(ky, ky + n_k);
+            k.w.assign(weights, weights + n_k);
+            for (int i = 0; i < n_k; ++i) {
+                k.kx2.push_back(kx[i] * kx[i]);
+                k.ky2.push_back(ky[i] * ky[i]);
+                k.m1 = std::max(k.m1, std::abs(n1[i]));
+                k.m2 = std::max(k.m2, std::abs(n2[i]));
+            }
+            k.b1x = s.cell.b[0]; k.b1y = s.cell.b[1]; k.b2x = s.cell.b[2]; k.b2y = s.cell.b[3];
+            k.n_cached = k.a.size() + (k.same ? 0 : k.b.size());
+        }
+        s.comps.push_back(std::move(c));
+    });
+}
+
+int oracle_add_drummond_jastrow(oracle_system* h, int ga, int gb, int C, double gamma, double L,
+                                int n_coeffs, const double* coeffs) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        check_index(s, ga);
+        check_index(s, gb);
+        if (n_coeffs < 1 || C < 2) throw std::invalid_argument("oracle: drummond needs n >= 1, C >= 2");
+        auto c = std::make_unique<PairJastrow>();
+        c->ga = s.groups[ga];
+        c->gb = s.groups[gb];
+        c->same = ga == gb;
+        c->rad.drummond = true;
+        c->rad.C = C;
+        c->rad.L = L;
+        c->rad.c.assign(coeffs, coeffs + n_coeffs);
+        double ipw = 1.0;
+        for (int i = 0; i < C; ++i) ipw *= -L;
+        c->rad.alpha1 = gamma / ipw + coeffs[0] * static_cast<double>(C) / L;
+        s.comps.push_back(std::move(c));
+    });
+}
+
+int oracle_add_nosanow_jastrow(oracle_system* h, int group, double alpha, const double* sites) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        check_index(s, group);
+        // NosanowJastrow refuses a Gaussian that is still above 1e-6 of its
+        // peak at the Wigner-Seitz boundary (nosanow_jastrow.h:51).
+        if (s.cell.periodic && std::exp(-alpha * s.cell.ws * s.cell.ws) > 1e-6)
+            throw std::invalid_argument("nosanow: alpha too small for the cell (folded tail above 1e-6)");
+        auto c = std::make_unique<Nosanow>();
+        c->g = s.groups[group];
+        c->alpha = alpha;
+        c->sites.assign(sites, sites + c->g.size() * 2);
+        s.comps.push_back(std::move(c));
+    });
+}
+
+int oracle_add_slater_plane_wave(oracle_system* h, int first, int last, const double* kvecs,
+                                 const unsigned char* is_sin) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        RealSlater::Block B;
+        B.first = first;
+        B.last = last;
+        B.N = static_cast<std::size_t>(last - first);
+        B.orb.n = B.N;
+        B.orb.k.assign(kvecs, kvecs + 2 * B.N);
+        B.orb.is_sin.assign(is_sin, is_sin + B.N);
+        RealSlater* c = nullptr;
+        for (auto& x : s.comps)
+            if (auto* r = dynamic_cast<RealSlater*>(x.get())) c = r;
+        if (!c) {
+            s.comps.push_back(std::make_unique<RealSlater>());
+            c = static_cast<RealSlater*>(s.comps.back().get());
+        }
+        c->blocks.push_back(std::move(B));
+    });
+}
+
+int oracle_add_slater_gaussian(oracle_system* h, int first, int last, const double* centers,
+                               const double* alphas) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        RealSlater::Block B;
+        B.first = first;
+        B.last = last;
+        B.N = static_cast<std::size_t>(last - first);
+        B.orb.n = B.N;
+        B.orb.gaussian = true;
+        B.orb.centers.assign(centers, centers + 2 * B.N);
+        B.orb.alphas.assign(alphas, alphas + B.N);
+        RealSlater* c = nullptr;
+        for (auto& x : s.comps)
+            if (auto* r = dynamic_cast<RealSlater*>(x.get())) c = r;
+        if (!c) {
+            s.comps.push_back(std::make_unique<RealSlater>());
+            c = static_cast<RealSlater*>(s.comps.back().get());
+        }
+        c->blocks.push_back(std::move(B));
+    });
+}
+
+int oracle_add_slater_twisted(oracle_system* h, int first, int last, const double* q) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        ComplexSlater::Block B;
+        B.first = first;
+        B.last = last;
+        B.N = static_cast<std::size_t>(last - first);
+        B.q.assign(q, q + 2 * B.N);
+        ComplexSlater* c = nullptr;
+        for (auto& x : s.comps)
+            if (auto* r = dynamic_cast<ComplexSlater*>(x.get())) c = r;
+        if (!c) {
+            s.comps.push_back(std::make_unique<ComplexSlater>());
+            c = static_cast<ComplexSlater*>(s.comps.back().get());
+        }
+        c->blocks.push_back(std::move(B));
+        s.has_phase = true;
+    });
+}
+
+int oracle_add_ewald(oracle_system* h, int n_k, const int* n1, const int* n2, const double* kx,
+                     const double* ky, const double* weights, int n_knots, const double* spline,
+                     double r_cut, double inv_delta, double v_long_zero, double sr_k0) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        if (!s.cell.periodic) throw std::invalid_argument("oracle: ewald needs a periodic cell");
+        Ewald& e = s.ewald;
+        e.active = true;
+        e.n1.assign(n1, n1 + n_k);
+        e.n2.assign(n2, n2 + n_k);
+        e.kx.assign(kx, kx + n_k);
+        e.ky.assign(ky, ky + n_k);
+        e.w.assign(weights, weights + n_k);
+        e.sp.scaled.assign(spline, spline + 3 * n_knots);
+        e.sp.n_knots = n_knots;
+        e.sp.r_cut = r_cut;
+        e.sp.inv_delta = inv_delta;
+        e.r_cut = r_cut;
+        e.v_long_zero = v_long_zero;
+        e.sr_k0 = sr_k0;
+        e.inv_eps = 1.0;
+        for (int i = 0; i < n_k; ++i) {
+            if (n1[i] < 0) throw std::invalid_argument("oracle: ewald k-list must have n1 >= 0");
+            e.n1_max = std::max(e.n1_max, n1[i]);
+            e.n2_max = std::max(e.n2_max, std::abs(n2[i]));
+        }
+    });
+}
+
+int oracle_add_coulomb(oracle_system* h, int ga, int gb, double epsilon) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        check_index(s, ga);
+        check_index(s, gb);
+        s.coulomb.push_back({ga, gb, 1.0 / epsilon});
+    });
+}
+
+int oracle_set_positions(oracle_system* h, const double* pos) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        std::copy(pos, pos + s.pos.size(), s.pos.begin());
+        for (std::size_t w = 0; w < s.nw; ++w)
+            for (std::size_t i = 0; i < s.np; ++i) s.cell.wrap(s.P(i, 0, w), s.P(i, 1, w));
+        s.build_dist();
+    });
+}
+
+int oracle_get_positions(oracle_system* h, double* pos) {
+    Sys& s = *S(h);
+    std::copy(s.pos.begin(), s.pos.end(), pos);
+    return 0;
+}
+
+int oracle_evaluate(oracle_system* h) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        for (auto& c : s.comps) c->evaluate(s);
+    });
+}
+
+int oracle_get_log_psi(oracle_system* h, double* out) {
+    Sys& s = *S(h);
+    for (std::size_t w = 0; w < s.nw; ++w) out[w] = log_psi(s, w);
+    return 0;
+}
+
+int oracle_get_gradients(oracle_system* h, double* grad, double* lap) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        refresh_all(s);
+        std::vector<double> g, l;
+        totals(s, g, l, nullptr, nullptr);
+        std::copy(g.begin(), g.end(), grad);
+        std::copy(l.begin(), l.end(), lap);
+    });
+}
+
+int oracle_get_phase_gradients(oracle_system* h, double* pgrad, double* plap) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        refresh_all(s);
+        std::vector<double> g, l, pg, pl;
+        totals(s, g, l, &pg, &pl);
+        std::copy(pg.begin(), pg.end(), pgrad);
+        std::copy(pl.begin(), pl.end(), plap);
+    });
+}
+
+int oracle_local_energy(oracle_system* h, double* e_re, double* e_im) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        local_energy(s);
+        std::copy(s.e_loc.begin(), s.e_loc.end(), e_re);
+        if (e_im) std::copy(s.e_loc_im.begin(), s.e_loc_im.end(), e_im);
+    });
+}
+
+int oracle_ratio(oracle_system* h, int p, const double* new_pos, double* log_ratio, double* phase,
+                 double* new_grad, double* new_lap) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        if (p < 0 || static_cast<std::size_t>(p) >= s.np) throw std::invalid_argument("oracle: particle out of range");
+        std::copy(new_pos, new_pos + 2 * s.nw, s.new_pos.begin());
+        ratio_all(s, static_cast<std::size_t>(p));
+        std::copy(s.log_ratios.begin(), s.log_ratios.end(), log_ratio);
+        if (phase) std::copy(s.phase_diffs.begin(), s.phase_diffs.end(), phase);
+        if (new_grad) std::copy(s.new_grad.begin(), s.new_grad.end(), new_grad);
+        if (new_lap) std::copy(s.new_lap.begin(), s.new_lap.end(), new_lap);
+    });
+}
+
+int oracle_accept(oracle_system* h, int p, const unsigned char* mask) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        std::vector<double> m(s.nw);
+        for (std::size_t w = 0; w < s.nw; ++w) m[w] = mask[w] ? 1.0 : 0.0;
+        accept_all(s, static_cast<std::size_t>(p), m);
+    });
+}
+
+// VMC::sweep_walkers with symmetric proposals (vmc.cpp:82-155).
+int oracle_vmc_sweeps(oracle_system* h, int n_sweeps, double tau, const double* tape,
+                      long long* n_accepted) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        const std::size_t nw = s.nw, np = s.np;
+        long long acc = 0;
+        std::vector<double> m(nw);
+        for (int sw = 0; sw < n_sweeps; ++sw) {
+            const double* randoms = tape + static_cast<std::size_t>(sw) * np * 3 * nw;
+            for (std::size_t p = 0; p < np; ++p) {
+                const double* gauss = randoms + p * 3 * nw;
+                const double* uni = gauss + 2 * nw;
+                const double sigma = std::sqrt(tau * s.inv_mass[p]);
+                for (std::size_t d = 0; d < 2; ++d)
+                    for (std::size_t w = 0; w < nw; ++w)
+                        s.new_pos[d * nw + w] = s.P(p, d, w) + sigma * gauss[d * nw + w];
+                ratio_all(s, p);
+                for (std::size_t w = 0; w < nw; ++w) {
+                    m[w] = uni[w] < std::exp(2.0 * s.log_ratios[w]) ? 1.0 : 0.0;
+                    if (m[w] != 0.0) ++acc;
+                }
+                accept_all(s, p, m);
+            }
+        }
+        *n_accepted = acc;
+    });
+}
+
+// DMC::sweep_walkers (dmc.cpp:203-337): Umrigar-scaled drift, fixed-node
+// gate (real trials) or fixed phase (complex), Metropolis-Hastings with the
+// drift Green's function, rr accounting.  tape rows have stride tape_stride.
+int oracle_dmc_sweeps(oracle_system* h, int n_sweeps, double tau, const double* tape,
+                      int tape_stride, double* rr_proposed, double* rr_accepted,
+                      long long* n_accepted) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        const std::size_t nw = s.nw, np = s.np, stride = static_cast<std::size_t>(tape_stride);
+        long long acc = 0;
+        std::vector<double> m(nw), scale(nw), g, l;
+        for (int sw = 0; sw < n_sweeps; ++sw) {
+            const double* randoms = tape + static_cast<std::size_t>(sw) * np * 3 * stride;
+            for (std::size_t p = 0; p < np; ++p) {
+                for (auto& c : s.comps) c->refresh_particle(s, p);
+                totals(s, g, l, nullptr, nullptr);
+                const double tm = tau * s.inv_mass[p];
+                const double sigma = std::sqrt(tm);
+                const double* gauss = randoms + p * 3 * stride;
+                const double* uni = gauss + 2 * stride;
+                auto umrigar = [](double t, double v2) {
+                    if (v2 < 1e-10) return t;
+                    return (-1.0 + std::sqrt(1.0 + 2.0 * t * v2)) / v2;
+                };
+                for (std::size_t w = 0; w < nw; ++w) {
+                    double v2 = 0.0;
+                    for (std::size_t d = 0; d < 2; ++d) {
+                        const double gg = g[(p * 2 + d) * nw + w];
+                        v2 += gg * gg;
+                    }
+                    scale[w] = umrigar(tm, v2);
+                }
+                for (std::size_t d = 0; d < 2; ++d)
+                    for (std::size_t w = 0; w < nw; ++w)
+                        s.new_pos[d * nw + w] = s.P(p, d, w) + scale[w] * g[(p * 2 + d) * nw + w] +
+                                                sigma * gauss[d * stride + w];
+                ratio_all(s, p);
+                for (std::size_t w = 0; w < nw; ++w) {
+                    if (!s.has_phase && std::cos(s.phase_diffs[w]) < std::numeric_limits<double>::epsilon()) {
+                        m[w] = 0.0;
+                        continue;
+                    }
+                    double vn = 0.0;
+                    for (std::size_t d = 0; d < 2; ++d) vn += s.new_grad[d * nw + w] * s.new_grad[d * nw + w];
+                    const double sb = umrigar(tm, vn);
+                    double lc = 0.0;
+                    for (std::size_t d = 0; d < 2; ++d) {
+                        const double delta = s.new_pos[d * nw + w] - s.P(p, d, w);
+                        const double df = scale[w] * g[(p * 2 + d) * nw + w];
+                        const double db = sb * s.new_grad[d * nw + w];
+                        lc += tm <= 0.0 ? 0.0 : (-delta * (df + db) + 0.5 * (df * df - db * db)) / tm;
+                    }
+                    const double la = 2.0 * s.log_ratios[w] + lc;
+                    const double pa = la >= 0.0 ? 1.0 : std::exp(la);
+                    m[w] = (la >= 0.0 || uni[w] < pa) ? 1.0 : 0.0;
+                    if (m[w] != 0.0) ++acc;
+                }
+                for (std::size_t w = 0; w < nw; ++w) {
+                    double rr = 0.0;
+                    for (std::size_t d = 0; d < 2; ++d) {
+                        const double dr = s.new_pos[d * nw + w] - s.P(p, d, w);
+                        rr += tm * dr * dr;
+                    }
+                    rr_proposed[w] += rr;
+                    if (m[w] != 0.0) rr_accepted[w] += rr;
+                }
+                accept_all(s, p, m);
+            }
+        }
+        *n_accepted = acc;
+    });
+}
+
+// WalkerSet::apply_permutation + permute_caches (walker_set.cpp:109-178,
+// wavefunction_component.cpp:18-48).
+int oracle_permute(oracle_system* h, int new_nw, const int* src) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        const std::size_t old_nw = s.nw;
+        std::vector<std::size_t> sv(static_cast<std::size_t>(new_nw));
+        for (int j = 0; j < new_nw; ++j) {
+            if (src[j] < 0 || static_cast<std::size_t>(src[j]) >= old_nw)
+                throw std::invalid_argument("oracle: permutation source out of range");
+            sv[j] = static_cast<std::size_t>(src[j]);
+        }
+        Component::gather(s.pos, sv, old_nw);
+        Component::gather(s.dist, sv, old_nw);
+        for (auto& c : s.comps) c->permute(sv, old_nw);
+        s.nw = sv.size();
+        std::vector<double> pos = s.pos, dist = s.dist;
+        s.alloc();
+        s.pos = pos;
+        s.dist = dist;
+    });
+}
+
+}  // extern "C"

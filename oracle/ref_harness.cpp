@@ -1,0 +1,437 @@
+// Harness over the reference implementation's own hot-path sources.
+//
+// oracle/Makefile compiles this file together with the reference sources where
+// they lie (under /root/reference/src) into oracle/_ref/libalea_ref.so.  The
+// harness only SEQUENCES the reference's public calls; every number it returns
+// is computed by the reference's code.  It exists to pin the oracle
+// (oracle/oracle.cpp) and the HIP path against the real thing: test
+// infrastructure only, never linked into the product.
+//
+// What it cannot build: the Slater determinants (src/wavefunction/
+// slater_determinant.cpp, complex_slater_determinant.cpp) include Eigen, which
+// is absent from this image, so they are left out; determinant parity is pinned
+// by closed forms and brute force instead.
+//
+// The registration entry points mirror include/alea_amd.h with a `ref_` prefix
+// so one Python driver can run the same call sequence on all three libraries.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <span>
+#include <string>
+#include <vector>
+
+#include "hamiltonian/coulomb.h"
+#include "hamiltonian/ewald_term.h"
+#include "hamiltonian/hamiltonian.h"
+#include "hamiltonian/kinetic.h"
+#include "numerics/ewald_breakup.h"
+#include "numerics/long_range_breakup.h"
+#include "numerics/pair_radial.h"
+#include "numerics/random.h"
+#include "numerics/xoshiro_random.h"
+#include "population/simulation_cell.h"
+#include "population/walker_set.h"
+#include "qmc/sweep_blocking.h"
+#include "wavefunction/drummond_jastrow.h"
+#include "wavefunction/nosanow_jastrow.h"
+#include "wavefunction/pade_jastrow.h"
+#include "wavefunction/trial_wavefunction.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return 2;
+    }
+}
+
+struct RefSystem {
+    std::size_t nw = 0, np = 0, dim = 2;
+    bool periodic = false;
+    std::vector<double> lattice;
+    alea::SimulationCell cell;
+    std::vector<alea::ParticleGroup> groups;
+    alea::TrialWavefunction psi;
+    alea::Hamiltonian ham;
+    std::unique_ptr<alea::WalkerSet> ws;
+    bool ham_built = false;
+    double ewald_tol = 0.0, ewald_eps = 1.0;
+    bool use_ewald = false;
+    struct CoulombPair { int a, b; double eps; };
+    std::vector<CoulombPair> coulomb;
+};
+
+alea::ParticleGroup make_group(std::size_t first, std::size_t last, double charge,
+                               double inv_mass, std::size_t dim) {
+    alea::ParticleGroup g;
+    g.name = "g" + std::to_string(first);
+    g.mass.assign(dim, 1.0 / inv_mass);
+    g.inv_mass.assign(dim, inv_mass);
+    g.charge = charge;
+    g.first = first;
+    g.last = last;
+    g.count = last - first;
+    return g;
+}
+
+void build_hamiltonian(RefSystem& s) {
+    if (s.ham_built) return;
+    s.ham.add_component(std::make_unique<alea::Kinetic>(s.groups));
+    if (s.use_ewald)
+        s.ham.add_component(std::make_unique<alea::EwaldTerm>(s.groups, s.ewald_eps, s.cell,
+                                                              s.ewald_tol));
+    for (const auto& c : s.coulomb)
+        s.ham.add_component(std::make_unique<alea::CoulombTerm>(
+            s.groups[static_cast<std::size_t>(c.a)], s.groups[static_cast<std::size_t>(c.b)],
+            c.eps));
+    s.ham_built = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ref_last_error() { return g_error.c_str(); }
+
+// ── RNG (src/numerics/xoshiro_random.h, src/numerics/random.h) ──────────────
+uint64_t ref_stream_seed(uint64_t base, uint64_t stream, uint64_t index) {
+    return alea::numerics::stream_seed(base, stream, index);
+}
+
+int ref_xoshiro_state_u64(const uint64_t* state, int n, uint64_t* out) {
+    std::array<std::uint64_t, 4> st{state[0], state[1], state[2], state[3]};
+    alea::numerics::Xoshiro256pp g(st);
+    for (int i = 0; i < n; ++i) out[i] = g();
+    return 0;
+}
+
+int ref_xoshiro_seed_u64(uint64_t seed, int n, uint64_t* out) {
+    alea::numerics::Xoshiro256pp g(seed);
+    for (int i = 0; i < n; ++i) out[i] = g();
+    return 0;
+}
+
+// kind 0: uniform(), 1: gaussian()
+int ref_xoshiro_draws(uint64_t seed, int kind, int n, double* out) {
+    alea::numerics::XoshiroRandom r(seed);
+    for (int i = 0; i < n; ++i) out[i] = kind == 0 ? r.uniform() : r.gaussian();
+    return 0;
+}
+
+// n_sweeps consecutive draw_sweep_randoms tapes of one stream (sweep_blocking.h:39).
+int ref_sweep_tape(uint64_t seed, int np, int dim, int nw, int n_sweeps, double* out) {
+    alea::numerics::XoshiroRandom r(seed);
+    const std::size_t len = static_cast<std::size_t>(np) * (dim + 1) * nw;
+    for (int s = 0; s < n_sweeps; ++s)
+        alea::draw_sweep_randoms(r, np, dim, nw, 0, nw,
+                                 std::span<double>(out + s * len, len));
+    return 0;
+}
+
+// ── Breakup tables (setup; src/numerics/ewald_breakup.cpp:205) ─────────────
+int ref_ewald_setup(const double* lattice, double tol, double eps, int cap_k, int* nk,
+                    int* n1, int* n2, double* kx, double* ky, double* weight, int cap_knots,
+                    int* n_knots, double* scaled, double* scalars /* r_cut, inv_delta,
+                    v_long_zero, sr_k0, k_cut, area */) {
+    return guarded([&] {
+        alea::SimulationCell cell(2, std::span<const double>(lattice, 4));
+        const auto s = alea::numerics::build_ewald(
+            cell, tol, alea::numerics::PotentialSpec::coulomb(eps));
+        if (static_cast<int>(s.n_k()) > cap_k) throw std::runtime_error("cap_k too small");
+        *nk = static_cast<int>(s.n_k());
+        for (std::size_t k = 0; k < s.n_k(); ++k) {
+            n1[k] = s.n1[k];
+            n2[k] = s.n2[k];
+            kx[k] = s.kx[k];
+            ky[k] = s.ky[k];
+            weight[k] = s.weight[k];
+        }
+        if (static_cast<int>(s.long_range.scaled.size()) > 3 * cap_knots)
+            throw std::runtime_error("cap_knots too small");
+        *n_knots = s.long_range.n_knots;
+        std::memcpy(scaled, s.long_range.scaled.data(),
+                    s.long_range.scaled.size() * sizeof(double));
+        scalars[0] = s.long_range.r_cut;
+        scalars[1] = s.long_range.inv_delta;
+        scalars[2] = s.v_long_zero;
+        scalars[3] = s.sr_k0;
+        scalars[4] = s.k_cut;
+        scalars[5] = s.area;
+    });
+}
+
+// The image-sum split of a Padé u = P/Q exactly as PeriodicPairSum builds it
+// (src/wavefunction/periodic_pair_sum.cpp:16-64, src/wavefunction/pade_jastrow.cpp:204-238).
+int ref_pade_breakup(const double* lattice, int na, const double* a, int nb, const double* b,
+                     int cap_k, int* nk, int* n1, int* n2, double* kx, double* ky,
+                     double* weight, int cap_knots, int* n_knots, double* scaled,
+                     double* scalars /* r_cut, inv_delta, u_inf */) {
+    return guarded([&] {
+        alea::SimulationCell cell(2, std::span<const double>(lattice, 4));
+        const double rc = cell.ws_radius();
+        std::vector<double> av(a, a + na), bv(b, b + nb);
+        alea::numerics::BreakupOptions opt{};
+        opt.k_continuum = 20.0 * 3.14159265358979323846 / rc;
+        opt.k_max = 300.0 / rc;
+        opt.n_continuum = 150;
+        opt.k_cut = 5.0 * std::min(std::hypot(cell.reciprocal()[0], cell.reciprocal()[1]),
+                                   std::hypot(cell.reciprocal()[2], cell.reciprocal()[3]));
+        const auto br = alea::numerics::build_long_range_breakup(
+            cell, alea::numerics::pade_radial(av, bv, rc), opt);
+        if (static_cast<int>(br.weight().size()) > cap_k)
+            throw std::runtime_error("cap_k too small");
+        *nk = static_cast<int>(br.weight().size());
+        for (std::size_t k = 0; k < br.weight().size(); ++k) {
+            n1[k] = br.n1()[k];
+            n2[k] = br.n2()[k];
+            kx[k] = br.kx()[k];
+            ky[k] = br.ky()[k];
+            weight[k] = br.weight()[k];
+        }
+        const auto& sp = br.spline();
+        if (static_cast<int>(sp.scaled.size()) > 3 * cap_knots)
+            throw std::runtime_error("cap_knots too small");
+        *n_knots = sp.n_knots;
+        std::memcpy(scaled, sp.scaled.data(), sp.scaled.size() * sizeof(double));
+        scalars[0] = sp.r_cut;
+        scalars[1] = sp.inv_delta;
+        scalars[2] = alea::numerics::pade_limit(av, bv);
+    });
+}
+
+// ── System registration (mirrors include/alea_amd.h) ───────────────────────
+int ref_create(void** out, int nw, int np, int dim, const double* lattice) {
+    return guarded([&] {
+        auto* s = new RefSystem;
+        s->nw = static_cast<std::size_t>(nw);
+        s->np = static_cast<std::size_t>(np);
+        s->dim = static_cast<std::size_t>(dim);
+        s->periodic = lattice != nullptr;
+        if (s->periodic) {
+            s->lattice.assign(lattice, lattice + dim * dim);
+            s->cell = alea::SimulationCell(s->dim, s->lattice);
+        } else {
+            s->cell = alea::SimulationCell(s->dim);
+        }
+        s->ws = std::make_unique<alea::WalkerSet>(s->nw, s->np, s->dim, s->cell);
+        *out = s;
+    });
+}
+
+int ref_destroy(void* h) {
+    delete static_cast<RefSystem*>(h);
+    return 0;
+}
+
+int ref_add_group(void* h, int first, int last, double charge, double inv_mass) {
+    auto& s = *static_cast<RefSystem*>(h);
+    s.groups.push_back(make_group(static_cast<std::size_t>(first), static_cast<std::size_t>(last),
+                                  charge, inv_mass, s.dim));
+    return 0;
+}
+
+int ref_add_pade_jastrow(void* h, int ga, int gb, int num_order, int den_order,
+                         const double* coeffs, int image_sum) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        std::vector<alea::Parameter> params;
+        for (int k = 0; k < num_order + den_order; ++k)
+            params.push_back(alea::Parameter{"c" + std::to_string(k), coeffs[k], true});
+        auto c = std::make_unique<alea::PadeJastrow>(num_order, den_order,
+                                                     s.groups[static_cast<std::size_t>(ga)],
+                                                     s.groups[static_cast<std::size_t>(gb)],
+                                                     std::move(params));
+        if (image_sum) c->sum_over_images(s.cell);
+        s.psi.add_component(std::move(c));
+    });
+}
+
+int ref_add_drummond_jastrow(void* h, int ga, int gb, int C, double gamma, double L,
+                             int n_coeffs, const double* coeffs) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        std::vector<alea::Parameter> params;
+        for (int k = 0; k < n_coeffs; ++k)
+            params.push_back(alea::Parameter{"a" + std::to_string(k), coeffs[k], true});
+        params.push_back(alea::Parameter{"L", L, true});
+        s.psi.add_component(std::make_unique<alea::DrummondJastrow>(
+            C, gamma, s.groups[static_cast<std::size_t>(ga)],
+            s.groups[static_cast<std::size_t>(gb)], std::move(params)));
+    });
+}
+
+int ref_add_nosanow_jastrow(void* h, int g, double alpha, const double* sites) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        const auto& grp = s.groups[static_cast<std::size_t>(g)];
+        std::vector<double> st(sites, sites + grp.size() * s.dim);
+        std::vector<alea::Parameter> params{alea::Parameter{"alpha", alpha, true}};
+        s.psi.add_component(std::make_unique<alea::NosanowJastrow>(
+            s.dim, grp.first, std::move(st), s.cell, std::move(params)));
+    });
+}
+
+int ref_add_ewald(void* h, double tol, double eps) {
+    auto& s = *static_cast<RefSystem*>(h);
+    s.use_ewald = true;
+    s.ewald_tol = tol;
+    s.ewald_eps = eps;
+    return 0;
+}
+
+int ref_add_coulomb(void* h, int ga, int gb, double eps) {
+    auto& s = *static_cast<RefSystem*>(h);
+    s.coulomb.push_back({ga, gb, eps});
+    return 0;
+}
+
+// Positions in the reference SoA layout [p*dim*nw + d*nw + w]; they are wrapped
+// into the cell and the distance tables rebuilt (WalkerSet::load_all,
+// src/population/walker_set.cpp:51-70), then ψ is evaluated.
+int ref_set_positions(void* h, const double* pos) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        auto p = s.ws->positions();
+        std::copy(pos, pos + p.size(), p.begin());
+        if (s.periodic) {
+            double r[3];
+            for (std::size_t w = 0; w < s.nw; ++w)
+                for (std::size_t i = 0; i < s.np; ++i) {
+                    for (std::size_t d = 0; d < s.dim; ++d) r[d] = p[(i * s.dim + d) * s.nw + w];
+                    s.cell.wrap(r);
+                    for (std::size_t d = 0; d < s.dim; ++d) p[(i * s.dim + d) * s.nw + w] = r[d];
+                }
+        }
+        s.ws->dist_tables().build_all(p);
+    });
+}
+
+int ref_evaluate(void* h) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        build_hamiltonian(s);
+        s.psi.evaluate_batch(*s.ws);
+    });
+}
+
+int ref_get_positions(void* h, double* out) {
+    auto& s = *static_cast<RefSystem*>(h);
+    auto p = s.ws->positions();
+    std::copy(p.begin(), p.end(), out);
+    return 0;
+}
+
+int ref_get_log_psi(void* h, double* out) {
+    auto& s = *static_cast<RefSystem*>(h);
+    auto p = s.ws->log_psi();
+    std::copy(p.begin(), p.end(), out);
+    return 0;
+}
+
+int ref_get_gradients(void* h, double* grad, double* lap) {
+    auto& s = *static_cast<RefSystem*>(h);
+    s.psi.refresh_derivatives(*s.ws);
+    auto g = s.ws->gradients();
+    auto l = s.ws->laplacians();
+    std::copy(g.begin(), g.end(), grad);
+    std::copy(l.begin(), l.end(), lap);
+    return 0;
+}
+
+// refresh_derivatives + Hamiltonian::evaluate_batch (src/qmc/vmc.cpp:171-173).
+int ref_local_energy(void* h, double* e_re) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        build_hamiltonian(s);
+        s.psi.refresh_derivatives(*s.ws);
+        s.ham.evaluate_batch(*s.ws);
+        auto le = s.ws->local_energies();
+        std::copy(le.begin(), le.end(), e_re);
+    });
+}
+
+// Per-call path: propose particle p at new_pos ([dim*nw]) and return
+// TrialWavefunction::ratio_batch's outputs without committing.
+int ref_ratio(void* h, int p, const double* new_pos, double* log_ratio, double* new_grad,
+              double* new_lap) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        auto np = s.ws->new_pos();
+        std::copy(new_pos, new_pos + np.size(), np.begin());
+        s.ws->propose_move_batch(static_cast<std::size_t>(p), np);
+        s.psi.ratio_batch(*s.ws, static_cast<std::size_t>(p));
+        auto lr = s.ws->log_ratios();
+        std::copy(lr.begin(), lr.end(), log_ratio);
+        auto g = s.ws->new_grad();
+        auto l = s.ws->new_lap();
+        std::copy(g.begin(), g.end(), new_grad);
+        std::copy(l.begin(), l.end(), new_lap);
+    });
+}
+
+// Accept the move proposed by the last ref_ratio where mask[w] != 0
+// (apply_move_batch, then WalkerSet::apply_move; src/qmc/vmc.cpp:147-152).
+int ref_accept(void* h, int p, const unsigned char* mask) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        auto m = s.ws->accept_mask();
+        for (std::size_t w = 0; w < s.nw; ++w) m[w] = mask[w] != 0;
+        s.psi.apply_move_batch(*s.ws, static_cast<std::size_t>(p), m);
+        s.ws->apply_move(static_cast<std::size_t>(p), s.ws->new_pos(), s.ws->new_grad(),
+                         s.ws->new_lap(), m);
+    });
+}
+
+// VMC Metropolis sweeps from a given tape (one draw_sweep_randoms block per
+// sweep, layout [np][dim+1][nw]), in the order of VMC::sweep_walkers
+// (src/qmc/vmc.cpp:82-155) with symmetric proposals, sigma = sqrt(tau * inv_mass).
+int ref_vmc_sweeps(void* h, int n_sweeps, double tau, const double* tape,
+                   long long* n_accepted) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        const std::size_t nw = s.nw, np = s.np, dim = s.dim;
+        std::vector<double> sigma(np * dim);
+        for (const auto& g : s.groups)
+            for (std::size_t i = g.first; i < g.last; ++i)
+                for (std::size_t d = 0; d < dim; ++d)
+                    sigma[i * dim + d] = std::sqrt(tau * g.inv_mass[d]);
+        long long acc = 0;
+        for (int sw = 0; sw < n_sweeps; ++sw) {
+            const double* randoms = tape + static_cast<std::size_t>(sw) * np * (dim + 1) * nw;
+            auto new_pos = s.ws->new_pos();
+            auto mask = s.ws->accept_mask();
+            for (std::size_t p = 0; p < np; ++p) {
+                const double* gaussian = randoms + p * (dim + 1) * nw;
+                const double* uniform = gaussian + dim * nw;
+                const auto pos = s.ws->positions();
+                for (std::size_t d = 0; d < dim; ++d)
+                    for (std::size_t w = 0; w < nw; ++w)
+                        new_pos[d * nw + w] = pos[p * dim * nw + d * nw + w] +
+                                              sigma[p * dim + d] * gaussian[d * nw + w];
+                s.ws->propose_move_batch(p, new_pos);
+                s.psi.ratio_batch(*s.ws, p);
+                const auto lr = s.ws->log_ratios();
+                for (std::size_t w = 0; w < nw; ++w) {
+                    mask[w] = uniform[w] < std::exp(2.0 * lr[w]);
+                    if (mask[w]) ++acc;
+                }
+                s.psi.apply_move_batch_deferred(*s.ws, p, mask);
+                s.ws->apply_move(p, new_pos, s.ws->new_grad(), s.ws->new_lap(), mask);
+            }
+        }
+        *n_accepted = acc;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,241 @@
+"""Pins the oracle (oracle/oracle.cpp) before anything is checked against it.
+
+Each check is one the oracle could fail by a plausible mistake:
+  * reference golden vectors: xoshiro256++ from state {1,2,3,4}
+    (tests/test_xoshiro_random.cpp:38-50), the square Madelung constant
+    1.6155426267 (tests/test_ewald.cpp:212-224), the exact 2-D exciton
+    E_L = -1 (tests/test_vmc_physics.cpp:86);
+  * closed forms: a plane-wave determinant is an eigenfunction of the kinetic
+    energy, E_L = sum |k|^2/2 for every configuration (real and twisted; the
+    reference's own check, tests/test_vmc_physics.cpp:1031-1041 and
+    tests/test_twisted_boundary.cpp:291); a determinant equals its
+    permutation sum; derivatives equal finite differences;
+  * the reference itself (oracle/_ref, skipped where it is not built): RNG
+    streams, sweep tapes, Jastrow/Ewald values, gradients, local energies,
+    ratios and whole VMC trajectories.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from alea_amd.system import Spec, exciton_spec, heg_spec, hexagonal_lattice, load_tables, uniform_positions
+from backends import RefWalkers, oracle_lib, oracle_tape, oracle_walkers, ref_lib, rel_err
+import ctypes as C
+
+XOSHIRO_1234 = [41943041, 58720359, 3588806011781223, 3591011842654386,
+                9228616714210784205, 9973669472204895162, 14011001112246962877,
+                12406186145184390807, 15849039046786891736, 10450023813501588000]
+
+
+def test_xoshiro_reference_vector():
+    lib = oracle_lib()[0]
+    st = (C.c_uint64 * 4)(1, 2, 3, 4)
+    out = (C.c_uint64 * 10)()
+    lib.oracle_xoshiro_state_u64(st, 10, out)
+    assert list(out) == XOSHIRO_1234
+
+
+def test_ziggurat_moments():
+    lib = oracle_lib()[0]
+    n = 400000
+    g = np.empty(n)
+    lib.oracle_xoshiro_draws(7, 1, n, g.ctypes.data_as(C.POINTER(C.c_double)))
+    assert abs(g.mean()) < 5 / math.sqrt(n)
+    assert abs(g.var() - 1) < 6 * math.sqrt(2 / n)
+    assert abs(np.mean(g ** 4) - 3) < 0.05
+    u = np.empty(n)
+    lib.oracle_xoshiro_draws(8, 0, n, u.ctypes.data_as(C.POINTER(C.c_double)))
+    assert u.min() >= 0 and u.max() < 1 and abs(u.mean() - 0.5) < 5 / math.sqrt(12 * n)
+
+
+def test_rng_matches_reference_build():
+    ref = ref_lib()
+    lib = oracle_lib()[0]
+    for seed in (1, 42, 2 ** 63 + 5):
+        assert lib.oracle_stream_seed(seed, 0x766d632d74687264, 3) == ref.ref_stream_seed(seed, 0x766d632d74687264, 3)
+        for kind in (0, 1):
+            a, b = np.empty(5000), np.empty(5000)
+            lib.oracle_xoshiro_draws(seed, kind, 5000, a.ctypes.data_as(C.POINTER(C.c_double)))
+            ref.ref_xoshiro_draws(seed, kind, 5000, b.ctypes.data_as(C.POINTER(C.c_double)))
+            assert np.array_equal(a, b)
+    ta = oracle_tape(99, 5, 7, 3)
+    tb = np.empty_like(ta)
+    ref.ref_sweep_tape(99, 5, 2, 7, 3, tb.ctypes.data_as(C.POINTER(C.c_double)))
+    assert np.array_equal(ta, tb)
+
+
+def test_madelung_square():
+    t = load_tables()["cells"]["2,0,0,2"]["ewald"]
+    s = Spec(n_particles=4, lattice=[2.0, 0, 0, 2.0], name="madelung")
+    s.groups = [(0, 2, 1.0, 1.0), (2, 4, -1.0, 1.0)]
+    s.ewald = t
+    w = oracle_walkers(s, 1)
+    pos = np.array([[0, 0], [1, 1], [1, 0], [0, 1]], float)  # (x, y) per ion
+    soa = np.zeros(8)
+    for i in range(4):
+        soa[i * 2 + 0] = pos[i, 0]
+        soa[i * 2 + 1] = pos[i, 1]
+    w.set_positions(soa)
+    w.evaluate()
+    e, _ = w.local_energy()   # no kinetic contribution: no wavefunction factors
+    M = -2.0 * e[0] / 4.0
+    assert abs(M - 1.6155426267) < 1e-6
+
+
+def test_exciton_exact():
+    s = exciton_spec()
+    nw = 64
+    w = oracle_walkers(s, nw)
+    rng = np.random.default_rng(1)
+    w.set_positions(uniform_positions(s, nw, rng, box=3.0))
+    w.evaluate()
+    e, _ = w.local_energy()
+    assert np.allclose(e, -1.0, atol=1e-12)
+    tape = oracle_tape(5, 2, nw, 4)
+    w.vmc_sweeps(4, 0.3, tape)
+    e, _ = w.local_energy()
+    assert np.allclose(e, -1.0, atol=1e-12) and np.std(e) < 1e-12
+
+
+@pytest.mark.parametrize("twist", [None, (0.21, -0.37)])
+def test_plane_wave_determinant_is_exact(twist):
+    # N = 10 (5 + 5), a determinant alone: E_L = sum |k|^2 / 2 for every walker.
+    s = heg_spec(5, 5, 2.0, twist=twist, jastrow=False, ewald=False)
+    nw = 16
+    w = oracle_walkers(s, nw)
+    w.set_positions(uniform_positions(s, nw, np.random.default_rng(3)))
+    w.evaluate()
+    e, im = w.local_energy()
+    if twist is None:
+        ks = np.concatenate([b[2] for b in s.slater_pw])
+    else:
+        ks = np.concatenate([b[2] for b in s.slater_twisted])
+    exact = 0.5 * np.sum(ks ** 2)
+    assert np.allclose(e, exact, rtol=1e-10)
+    assert np.allclose(im, 0.0, atol=1e-9 * exact)
+    # still exact after a few hundred Sherman-Morrison / re-inversion updates
+    w.vmc_sweeps(20, 0.5, oracle_tape(11, 10, nw, 20))
+    e, im = w.local_energy()
+    assert np.allclose(e, exact, rtol=1e-9)
+
+
+def _brute_logdet(M):
+    n = M.shape[0]
+    tot = 0j
+    for perm in itertools.permutations(range(n)):
+        sign = np.linalg.det(np.eye(n)[list(perm)])
+        prod = 1.0 + 0j
+        for i, j in enumerate(perm):
+            prod *= M[i, j]
+        tot += sign * prod
+    return math.log(abs(tot))
+
+
+@pytest.mark.parametrize("twist", [None, (0.3, 0.1)])
+def test_determinant_equals_permutation_sum(twist):
+    # 5 + 5 plane waves; brute force over 5! permutations per block.
+    s = heg_spec(5, 5, 2.0, twist=twist, jastrow=False, ewald=False)
+    nw = 3
+    w = oracle_walkers(s, nw)
+    pos = uniform_positions(s, nw, np.random.default_rng(4))
+    w.set_positions(pos)
+    w.evaluate()
+    lp = w.log_psi()
+    P = w.positions().reshape(10, 2, nw)
+    for iw in range(nw):
+        tot = 0.0
+        blocks = s.slater_pw if twist is None else s.slater_twisted
+        for b in blocks:
+            first, last = b[0], b[1]
+            M = np.zeros((last - first, last - first), complex)
+            for kr in range(last - first):
+                r = P[first + kr, :, iw]
+                for m in range(last - first):
+                    if twist is None:
+                        ph = b[2][m] @ r
+                        M[kr, m] = math.sin(ph) if b[3][m] else math.cos(ph)
+                    else:
+                        M[kr, m] = np.exp(1j * (b[2][m] @ r))
+            tot += _brute_logdet(M)
+        assert abs(lp[iw] - tot) < 1e-10 * max(1.0, abs(tot))
+
+
+def test_gradients_match_finite_differences():
+    s = heg_spec(5, 5, 2.0, twist=(0.13, 0.27))
+    nw = 2
+    w = oracle_walkers(s, nw)
+    pos = uniform_positions(s, nw, np.random.default_rng(5))
+    w.set_positions(pos)
+    w.evaluate()
+    g, lap = w.gradients()
+    pos = w.positions()
+    h = 1e-5
+    for p in (0, 7):
+        for d in (0, 1):
+            lps = []
+            for sgn in (-1, 0, 1):
+                q = pos.copy()
+                q[(p * 2 + d) * nw:(p * 2 + d + 1) * nw] += sgn * h
+                w2 = oracle_walkers(s, nw)
+                w2.set_positions(q)
+                w2.evaluate()
+                lps.append(w2.log_psi())
+            fd = (lps[2] - lps[0]) / (2 * h)
+            fd2 = (lps[2] - 2 * lps[1] + lps[0]) / h ** 2
+            idx = slice((p * 2 + d) * nw, (p * 2 + d + 1) * nw)
+            assert np.allclose(g[idx], fd, atol=1e-6 * max(1, np.abs(fd).max()))
+            assert np.allclose(lap[idx], fd2, atol=1e-3 * max(1, np.abs(fd2).max()))
+
+
+# ── against the reference's own code ──────────────────────────────────────
+
+def _jastrow_ewald_specs():
+    specs = [heg_spec(5, 5, 2.0, slater=False), heg_spec(13, 13, 30.0, slater=False)]
+    hexa = heg_spec(3, 3, 2.0, slater=False, lattice=hexagonal_lattice(6, 2.0))
+    hexa.name = "hexagonal N=6"
+    specs.append(hexa)
+    dr = heg_spec(5, 5, 2.0, slater=False, jastrow=False)
+    dr.name = "drummond+nosanow N=10"
+    L = dr.lattice[0] / 2
+    dr.drummond.append(dict(ga=0, gb=1, C=3, gamma=1.0, L=L, coeffs=[1e-3, -2e-4, 3e-5]))
+    dr.drummond.append(dict(ga=0, gb=0, C=3, gamma=1 / 3, L=L, coeffs=[2e-3, 1e-4]))
+    dr.nosanow.append(dict(group=1, alpha=0.6, sites=np.random.default_rng(9).uniform(-5, 5, 10)))
+    specs.append(dr)
+    return specs
+
+
+@pytest.mark.parametrize("spec", _jastrow_ewald_specs(), ids=lambda s: s.name)
+def test_oracle_matches_reference(spec):
+    nw = 8
+    pos = uniform_positions(spec, nw, np.random.default_rng(12))
+    o, r = oracle_walkers(spec, nw), RefWalkers(spec, nw)
+    for x in (o, r):
+        x.set_positions(pos)
+        x.evaluate()
+    assert rel_err(o.log_psi(), r.log_psi()) < 1e-12
+    go, lo = o.gradients()
+    gr, lr = r.gradients()
+    assert rel_err(go, gr) < 1e-11 and rel_err(lo, lr) < 1e-11
+    assert rel_err(o.local_energy()[0], r.local_energy()[0]) < 1e-11
+    # per-call ratio / accept
+    rng = np.random.default_rng(13)
+    P = o.positions().reshape(spec.n_particles, 2, nw)
+    for p in (0, spec.n_particles - 1):
+        newp = (P[p] + rng.normal(scale=0.4, size=(2, nw))).reshape(-1)
+        ao, bo = o.ratio(p, newp), r.ratio(p, newp)
+        assert rel_err(ao[0], bo[0]) < 1e-11
+        assert rel_err(ao[2], bo[2]) < 1e-11 and rel_err(ao[3], bo[3]) < 1e-10
+        mask = rng.random(nw) < 0.5
+        o.accept(p, mask)
+        r.accept(p, mask)
+    assert rel_err(o.log_psi(), r.log_psi()) < 1e-12
+    # whole VMC trajectories from one tape
+    n_sw = 3
+    tape = oracle_tape(21, spec.n_particles, nw, n_sw)
+    tau = (spec.lattice[0] / 8) ** 2
+    assert o.vmc_sweeps(n_sw, tau, tape) == r.vmc_sweeps(n_sw, tau, tape)
+    assert np.array_equal(o.positions(), r.positions())
+    assert rel_err(o.log_psi(), r.log_psi()) < 1e-11
+    assert rel_err(o.local_energy()[0], r.local_energy()[0]) < 1e-10
