@@ -1,0 +1,35 @@
+"""alea_amd: MI355X-native batched VMC/DMC walker sweep.
+
+The compute path is libalea_amd.so (HIP kernels for gfx950 behind the C-ABI
+in include/alea_amd.h).  Importing this package never builds anything;
+load() raises if the library is missing, so nothing can silently fall back to
+a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from ._capi import CApi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libalea_amd.so")
+
+_api = None
+
+
+def load() -> CApi:
+    """The bound C-ABI of libalea_amd.so (raises if it is not built)."""
+    global _api
+    if _api is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with python -m alea_amd.build "
+                               "(or __graft_entry__.build()) before using the GPU path")
+        lib = C.CDLL(LIB_PATH)
+        api = CApi(lib, "alea_")
+        fn = lib.alea_seed
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
+        api.fns["seed"] = fn
+        _api = api
+    return _api

@@ -1,0 +1,904 @@
+// Host side of the C-ABI (include/alea_amd.h): builds the device model from
+// the registration calls, owns the walker-major device state, converts
+// between the reference's walker-fastest SoA and the device layout, and
+// launches the kernels on one stream per system.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <numbers>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/alea_amd.h"
+#include "model.h"
+
+namespace alea_amd {
+int set_ziggurat(const uint64_t* k, const double* w, const double* f, double r);
+void launch_evaluate(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, hipStream_t st);
+void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
+                  bool cplx, hipStream_t st);
+void launch_call(const DevModel* m, const DevState& s, const DevCall& c, int p, int mode, int nw,
+                 bool mixed, bool cplx, hipStream_t st);
+void launch_measure(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, hipStream_t st);
+void launch_gather(const double* src_d, double* dst_d, int stride, const int* src, int n_new,
+                   hipStream_t st);
+}  // namespace alea_amd
+
+using namespace alea_amd;
+
+namespace {
+
+thread_local std::string g_error;
+
+struct InvalidArg : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return ALEA_OK;
+    } catch (const std::invalid_argument& e) {
+        g_error = e.what();
+        return ALEA_ERR_INVALID;
+    } catch (const DeviceError& e) {
+        g_error = e.what();
+        return ALEA_ERR_DEVICE;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return ALEA_ERR_RUNTIME;
+    }
+}
+
+// Ziggurat tables of XoshiroRandom (xoshiro_random.h:119-163): 256 equal-area
+// regions, r from bisection on the top-layer mismatch.
+struct Zig {
+    std::array<uint64_t, 256> k{};
+    std::array<double, 256> w{}, f{};
+    double r = 0;
+};
+Zig make_zig() {
+    constexpr int n = 256;
+    auto f = [](double x) { return std::exp(-0.5 * x * x); };
+    auto v_r = [&](double r) {
+        return r * f(r) + std::sqrt(0.5 * std::numbers::pi) * std::erfc(r / std::numbers::sqrt2);
+    };
+    std::array<double, n> x{};
+    auto top = [&](double r) {
+        const double v = v_r(r);
+        x[n - 1] = r;
+        for (int i = n - 1; i >= 2; --i) {
+            const double y = f(x[i]) + v / x[i];
+            if (y >= 1.0) return 1.0;
+            x[i - 1] = std::sqrt(-2.0 * std::log(y));
+        }
+        return f(x[1]) + v / x[1] - 1.0;
+    };
+    double lo = 3.0, hi = 4.5;
+    for (;;) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid == lo || mid == hi) break;
+        (top(mid) > 0.0 ? lo : hi) = mid;
+    }
+    Zig z;
+    z.r = hi;
+    (void)top(z.r);
+    const double two52 = 0x1.0p52, base = v_r(z.r) / f(z.r);
+    z.k[0] = static_cast<uint64_t>(two52 * (z.r / base));
+    z.w[0] = base / two52;
+    z.f[0] = 1.0;
+    for (int i = 1; i < n; ++i) {
+        z.k[i] = i == 1 ? 0 : static_cast<uint64_t>(two52 * (x[i - 1] / x[i]));
+        z.w[i] = x[i] / two52;
+        z.f[i] = f(x[i]);
+    }
+    return z;
+}
+
+constexpr uint64_t splitmix64(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        free();
+        n = count;
+        if (count) hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+        if (count) hip_check(hipMemset(p, 0, count * sizeof(T)), "hipMemset");
+    }
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { free(); }
+};
+
+}  // namespace
+
+struct alea_system {
+    int capacity = 0, nw = 0, np = 0, precision = 0;
+    DevModel host;        // host copy of the model
+    DevModel* dmodel = nullptr;
+    bool built = false, evaluated = false;
+    bool cplx = false;
+    hipStream_t stream = nullptr;
+    DevState st;
+    DevBuf<double> pos, rho, logs, row_abs, cond, work, e_loc, e_loc_im, rr;
+    DevBuf<double> pos2, rho2, logs2, row_abs2, cond2, inv2, rr2;
+    DevBuf<char> inv_bytes;
+    DevBuf<int> updates, updates2, status, src;
+    DevBuf<uint64_t> rng, rng2;
+    DevBuf<long long> nacc;
+    DevBuf<double> call_buf;       // new_pos, outputs
+    DevBuf<unsigned char> call_mask;
+    DevBuf<double> grad_buf;       // 4 * np * 2 * nw
+    DevBuf<double> tape;
+    std::vector<int> pade_klist_n1, pade_klist_n2;
+
+    bool mixed() const { return precision == ALEA_MIXED; }
+    size_t tsize() const { return mixed() ? 4 : 8; }
+
+    void build() {
+        if (built) return;
+        DevModel& m = host;
+        m.nw = capacity;
+        m.np = np;
+        // block membership
+        for (int i = 0; i < np; ++i) m.block_of[i] = -1;
+        int nb_max = 0, inv_elems = 0;
+        for (int b = 0; b < m.n_blocks; ++b) {
+            for (int i = m.blk_first[b]; i < m.blk_first[b] + m.blk_n[b]; ++i) m.block_of[i] = b;
+            nb_max = std::max(nb_max, m.blk_n[b]);
+            inv_elems += (cplx ? 2 : 1) * m.blk_n[b] * m.blk_n[b];
+        }
+        // pair lists
+        for (int g = 0; g < kMaxGroups; ++g)
+            for (int h = 0; h < kMaxGroups; ++h) m.pair_list_n[g][h] = 0;
+        for (int f = 0; f < m.n_pair; ++f) {
+            const int a = m.pair[f].ga, b = m.pair[f].gb;
+            m.pair_list[a][b][m.pair_list_n[a][b]++] = f;
+            if (a != b) m.pair_list[b][a][m.pair_list_n[b][a]++] = f;
+        }
+        for (int g = 0; g < m.ng; ++g)
+            for (int i = m.g_first[g]; i < m.g_last[g]; ++i) m.group_of[i] = g;
+        hip_check(hipMalloc(&dmodel, sizeof(DevModel)), "hipMalloc model");
+        hip_check(hipMemcpy(dmodel, &m, sizeof(DevModel), hipMemcpyHostToDevice), "copy model");
+        const size_t nw_ = static_cast<size_t>(capacity);
+        pos.alloc(nw_ * np * 2);
+        rho.alloc(std::max<size_t>(1, nw_ * m.ng * m.nkj * 2));
+        logs.alloc(nw_ * 4);
+        inv_elems += inv_elems & 1;   // whole doubles per record (branching gathers doubles)
+        st.inv_stride = inv_elems;
+        st.nb_max = std::max(nb_max, 1);
+        inv_bytes.alloc(std::max<size_t>(1, nw_ * inv_elems * tsize()));
+        row_abs.alloc(nw_ * kMaxBlocks * st.nb_max);
+        cond.alloc(nw_ * kMaxBlocks);
+        updates.alloc(nw_ * kMaxBlocks);
+        st.work_stride = 2 * (cplx ? 2 : 1) * nb_max * nb_max;
+        work.alloc(std::max<size_t>(1, nw_ * st.work_stride));
+        rng.alloc(nw_ * 4);
+        e_loc.alloc(nw_);
+        e_loc_im.alloc(nw_);
+        rr.alloc(nw_ * 2);
+        status.alloc(nw_);
+        nacc.alloc(nw_);
+        src.alloc(nw_);
+        st.pos = pos.p;
+        st.rho = rho.p;
+        st.logs = logs.p;
+        st.inv = inv_bytes.p;
+        st.row_abs = row_abs.p;
+        st.cond = cond.p;
+        st.updates = updates.p;
+        st.work = work.p;
+        st.rng = rng.p;
+        st.e_loc = e_loc.p;
+        st.e_loc_im = e_loc_im.p;
+        st.rr = rr.p;
+        st.status = status.p;
+        st.n_accepted = nacc.p;
+        static bool zig_done = false;
+        if (!zig_done) {
+            const Zig z = make_zig();
+            if (set_ziggurat(z.k.data(), z.w.data(), z.f.data(), z.r)) throw DeviceError("ziggurat upload");
+            zig_done = true;
+        }
+        built = true;
+    }
+
+    void sync() { hip_check(hipStreamSynchronize(stream), "kernel"); }
+
+    void check_status() {
+        std::vector<int> s(nw);
+        hip_check(hipMemcpy(s.data(), status.p, nw * sizeof(int), hipMemcpyDeviceToHost), "status");
+        for (int w = 0; w < nw; ++w)
+            if (s[w]) throw std::runtime_error("Slater determinant: accepted matrix is singular or nonfinite");
+    }
+
+    // The model's nw is the walker stride of the per-call SoA buffers.
+    void set_model_nw(int n) {
+        host.nw = n;
+        hip_check(hipMemcpy(&dmodel->nw, &n, sizeof(int), hipMemcpyHostToDevice), "model nw");
+    }
+};
+
+namespace {
+
+alea_system* Sys(alea_system* s) {
+    if (!s) throw InvalidArg("null system handle");
+    return s;
+}
+
+void need_unbuilt(alea_system* s) {
+    if (s->built) throw InvalidArg("components must be registered before positions are set");
+}
+
+void load_spline(DevRadial& R, int n_knots, const double* spline, double r_cut, double inv_delta) {
+    if (n_knots < 2 || n_knots > kMaxKnots) throw InvalidArg("spline knot count out of range");
+    std::copy(spline, spline + 3 * n_knots, R.spline);
+    R.n_knots = n_knots;
+    R.r_cut = r_cut;
+    R.inv_delta = inv_delta;
+}
+
+void set_cell(DevCell& c, const double* v) {
+    // SimulationCell::finalize_periodic (simulation_cell.cpp:146-244), 2-D.
+    double a[4] = {v[0], v[1], v[2], v[3]};
+    std::size_t axis[2];
+    bool used[2] = {false, false}, ok = true;
+    for (int i = 0; i < 2 && ok; ++i) {
+        int nonzero = 0, which = 0;
+        for (int e = 0; e < 2; ++e)
+            if (a[i * 2 + e] != 0.0) { ++nonzero; which = e; }
+        if (nonzero != 1 || used[which]) ok = false;
+        else { axis[i] = which; used[which] = true; }
+    }
+    if (ok) {
+        double re[4] = {};
+        for (int i = 0; i < 2; ++i)
+            for (int e = 0; e < 2; ++e) re[axis[i] * 2 + e] = a[i * 2 + e];
+        std::memcpy(a, re, sizeof re);
+    } else {
+        for (int guard = 0; guard < 64; ++guard) {
+            const double n0 = a[0] * a[0] + a[1] * a[1], n1 = a[2] * a[2] + a[3] * a[3];
+            if (n1 < n0) { std::swap(a[0], a[2]); std::swap(a[1], a[3]); continue; }
+            if (n0 == 0.0) break;
+            const double mu = std::nearbyint((a[0] * a[2] + a[1] * a[3]) / n0);
+            if (mu == 0.0) break;
+            a[2] -= mu * a[0];
+            a[3] -= mu * a[1];
+        }
+    }
+    c.periodic = 1;
+    c.ortho = ok ? 1 : 0;
+    const double det = a[0] * a[3] - a[1] * a[2];
+    if (det == 0.0) throw InvalidArg("lattice vectors are singular");
+    const double id = 1.0 / det;
+    c.ainv[0] = a[3] * id;
+    c.ainv[1] = -a[2] * id;
+    c.ainv[2] = -a[1] * id;
+    c.ainv[3] = a[0] * id;
+    for (int k = 0; k < 4; ++k) {
+        c.a[k] = a[k];
+        c.b[k] = 6.283185307179586476925286766559 * c.ainv[k];
+    }
+    if (c.ortho)
+        for (int e = 0; e < 2; ++e) {
+            c.box[e] = std::abs(a[e * 2 + e]);
+            c.inv_box[e] = 1.0 / c.box[e];
+        }
+    int nc = 1;
+    c.cand[0] = c.cand[1] = 0.0;
+    if (!c.ortho)
+        for (int n0 = -1; n0 <= 1; ++n0)
+            for (int n1 = -1; n1 <= 1; ++n1) {
+                if (n0 == 0 && n1 == 0) continue;
+                for (int e = 0; e < 2; ++e) {
+                    double s = 0.0;
+                    s += n0 * a[0 * 2 + e];
+                    s += n1 * a[1 * 2 + e];
+                    c.cand[nc * 2 + e] = s;
+                }
+                ++nc;
+            }
+    double shortest2 = 0.0;
+    for (int n0 = -2; n0 <= 2; ++n0)
+        for (int n1 = -2; n1 <= 2; ++n1) {
+            if (n0 == 0 && n1 == 0) continue;
+            double r2 = 0.0;
+            for (int e = 0; e < 2; ++e) {
+                const double s = n0 * a[e] + n1 * a[2 + e];
+                r2 += s * s;
+            }
+            if (shortest2 == 0.0 || r2 < shortest2) shortest2 = r2;
+        }
+    c.ws = 0.5 * std::sqrt(shortest2);
+}
+
+void add_kspace(alea_system* s, int ga, int gb, int n_k, const double* kx, const double* ky,
+                const double* weights) {
+    DevModel& m = s->host;
+    if (n_k > kMaxKJ) throw InvalidArg("too many k-space Jastrow vectors");
+    if (m.nkj == 0) {
+        m.nkj = n_k;
+        std::copy(kx, kx + n_k, m.kjx);
+        std::copy(ky, ky + n_k, m.kjy);
+    } else {
+        if (m.nkj != n_k) throw InvalidArg("image-summed Jastrow factors must share one k set");
+        for (int k = 0; k < n_k; ++k)
+            if (m.kjx[k] != kx[k] || m.kjy[k] != ky[k])
+                throw InvalidArg("image-summed Jastrow factors must share one k set");
+    }
+    for (int k = 0; k < n_k; ++k) {
+        m.kjw[ga][gb][k] += weights[k];
+        if (ga != gb) m.kjw[gb][ga][k] += weights[k];
+    }
+    m.kj_active[ga] = m.kj_active[gb] = 1;
+}
+
+void check_group(alea_system* s, int g) {
+    if (g < 0 || g >= s->host.ng) throw InvalidArg("group index out of range");
+}
+
+void soa_to_walker(const double* soa, std::vector<double>& dev, int nw, int np, int stride_nw) {
+    dev.assign(static_cast<size_t>(nw) * np * 2, 0.0);
+    for (int p = 0; p < np; ++p)
+        for (int d = 0; d < 2; ++d)
+            for (int w = 0; w < nw; ++w)
+                dev[(static_cast<size_t>(w) * np + p) * 2 + d] = soa[(p * 2 + d) * stride_nw + w];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* alea_last_error(void) { return g_error.c_str(); }
+
+int alea_create(alea_system** out, int n_walkers, int n_particles, int dim, const double* lattice,
+                int precision) {
+    return guarded([&] {
+        if (!out) throw InvalidArg("null output handle");
+        if (dim != 2) throw InvalidArg("only 2-D systems are supported");
+        if (n_walkers <= 0 || n_particles <= 0) throw InvalidArg("empty walker batch");
+        if (n_particles > kMaxParticles) throw InvalidArg("too many particles (max 256)");
+        if (precision != ALEA_FP64 && precision != ALEA_MIXED) throw InvalidArg("unknown precision");
+        auto s = std::make_unique<alea_system>();
+        s->capacity = s->nw = n_walkers;
+        s->np = n_particles;
+        s->precision = precision;
+        if (lattice) set_cell(s->host.cell, lattice);
+        hip_check(hipStreamCreate(&s->stream), "hipStreamCreate");
+        *out = s.release();
+    });
+}
+
+int alea_destroy(alea_system* s) {
+    if (!s) return ALEA_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->dmodel) (void)hipFree(s->dmodel);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return ALEA_OK;
+}
+
+int alea_add_group(alea_system* s, int first, int last, double charge, double inv_mass) {
+    return guarded([&] {
+        Sys(s);
+        need_unbuilt(s);
+        DevModel& m = s->host;
+        if (m.ng >= kMaxGroups) throw InvalidArg("too many groups (max 4)");
+        if (first < 0 || last < first || last > s->np) throw InvalidArg("group range out of bounds");
+        m.g_first[m.ng] = first;
+        m.g_last[m.ng] = last;
+        for (int i = first; i < last; ++i) {
+            m.inv_mass[i] = inv_mass;
+            m.charge[i] = charge;
+            m.group_of[i] = m.ng;
+        }
+        ++m.ng;
+    });
+}
+
+int alea_add_pade_jastrow(alea_system* s, int ga, int gb, int num_order, int den_order,
+                          const double* coeffs, int n_k, const int* n1, const int* n2,
+                          const double* kx, const double* ky, const double* weights, int n_knots,
+                          const double* spline, double r_cut, double inv_delta, double u_inf) {
+    return guarded([&] {
+        Sys(s);
+        need_unbuilt(s);
+        check_group(s, ga);
+        check_group(s, gb);
+        DevModel& m = s->host;
+        if (m.n_pair >= kMaxPair) throw InvalidArg("too many pair Jastrow factors");
+        if (num_order < 0 || den_order < 0 || num_order + den_order > kMaxCoef)
+            throw InvalidArg("Padé order out of range");
+        DevPair& P = m.pair[m.n_pair];
+        P = DevPair{};
+        P.ga = ga;
+        P.gb = gb;
+        P.rad.kind = 0;
+        P.rad.num = num_order;
+        P.rad.den = den_order;
+        std::copy(coeffs, coeffs + num_order + den_order, P.rad.c);
+        if (n_k > 0) {
+            if (!m.cell.periodic) throw InvalidArg("an image sum needs a periodic cell");
+            (void)n1;
+            (void)n2;
+            P.rad.image = 1;
+            P.rad.u_inf = u_inf;
+            load_spline(P.rad, n_knots, spline, r_cut, inv_delta);
+            add_kspace(s, ga, gb, n_k, kx, ky, weights);
+        }
+        ++m.n_pair;
+    });
+}
+
+int alea_add_drummond_jastrow(alea_system* s, int ga, int gb, int C, double gamma, double L,
+                              int n_coeffs, const double* coeffs) {
+    return guarded([&] {
+        Sys(s);
+        need_unbuilt(s);
+        check_group(s, ga);
+        check_group(s, gb);
+        DevModel& m = s->host;
+        if (m.n_pair >= kMaxPair) throw InvalidArg("too many pair Jastrow factors");
+        if (n_coeffs < 1 || n_coeffs > kMaxCoef || C < 2) throw InvalidArg("Drummond needs 1..12 coefficients and C >= 2");
+        if (m.cell.periodic && L > m.cell.ws) throw InvalidArg("Drummond cutoff exceeds the Wigner-Seitz inradius");
+        DevPair& P = m.pair[m.n_pair];
+        P = DevPair{};
+        P.ga = ga;
+        P.gb = gb;
+        P.rad.kind = 1;
+        P.rad.num = n_coeffs;
+        P.rad.C = C;
+        P.rad.L = L;
+        std::copy(coeffs, coeffs + n_coeffs, P.rad.c);
+        double ipw = 1.0;
+        for (int i = 0; i < C; ++i) ipw *= -L;
+        P.rad.alpha1 = gamma / ipw + coeffs[0] * static_cast<double>(C) / L;
+        ++m.n_pair;
+    });
+}
+
+int alea_add_nosanow_jastrow(alea_system* s, int group, double alpha, const double* sites) {
+    return guarded([&] {
+        Sys(s);
+        need_unbuilt(s);
+        check_group(s, group);
+        DevModel& m = s->host;
+        if (m.cell.periodic && std::exp(-alpha * m.cell.ws * m.cell.ws) > 1e-6)
+            throw InvalidArg("nosanow: alpha too small for the cell (folded tail above 1e-6)");
+        for (int i = m.g_first[group]; i < m.g_last[group]; ++i) {
+            m.nos_alpha[i] = alpha;
+            m.nos_site[i][0] = sites[2 * (i - m.g_first[group])];
+            m.nos_site[i][1] = sites[2 * (i - m.g_first[group]) + 1];
+        }
+        m.has_nosanow = 1;
+    });
+}
+
+static void add_block(alea_system* s, int first, int last, int kind) {
+    DevModel& m = s->host;
+    if (m.n_blocks >= kMaxBlocks) throw InvalidArg("too many determinant blocks (max 2)");
+    const int n = last - first;
+    if (first < 0 || n <= 0 || last > s->np) throw InvalidArg("block range out of bounds");
+    if (n > kMaxOrb) throw InvalidArg("block larger than 128 orbitals");
+    const bool c = kind == kTwisted;
+    if (m.n_blocks > 0 && (m.det_complex != 0) != c) throw InvalidArg("mixing real and complex determinants");
+    m.det_complex = c ? 1 : 0;
+    s->cplx = c;
+    m.blk_first[m.n_blocks] = first;
+    m.blk_n[m.n_blocks] = n;
+    m.blk_kind[m.n_blocks] = kind;
+}
+
+int alea_add_slater_plane_wave(alea_system* s, int first, int last, const double* kvecs,
+                               const unsigned char* is_sin) {
+    return guarded([&] {
+        Sys(s);
+        need_unbuilt(s);
+        if (!s->host.cell.periodic) throw InvalidArg("plane-wave orbitals need a periodic cell");
+        add_block(s, first, last, kPlaneWave);
+        DevModel& m = s->host;
+        const int b = m.n_blocks;
+        for (int i = 0; i < last - first; ++i) {
+            m.orb_k[b][i][0] = kvecs[2 * i];
+            m.orb_k[b][i][1] = kvecs[2 * i + 1];
+            m.orb_sin[b][i] = is_sin[i] ? 1 : 0;
+        }
+        ++m.n_blocks;
+    });
+}
+
+int alea_add_slater_gaussian(alea_system* s, int first, int last, const double* centers,
+                             const double* alphas) {
+    return guarded([&] {
+        Sys(s);
+        need_unbuilt(s);
+        add_block(s, first, last, kGaussian);
+        DevModel& m = s->host;
+        const int b = m.n_blocks;
+        for (int i = 0; i < last - first; ++i) {
+            m.orb_k[b][i][0] = centers[2 * i];
+            m.orb_k[b][i][1] = centers[2 * i + 1];
+            m.orb_alpha[b][i] = alphas[i];
+        }
+        ++m.n_blocks;
+    });
+}
+
+int alea_add_slater_twisted(alea_system* s, int first, int last, const double* q) {
+    return guarded([&] {
+        Sys(s);
+        need_unbuilt(s);
+        if (!s->host.cell.periodic) throw InvalidArg("twisted plane waves need a periodic cell");
+        add_block(s, first, last, kTwisted);
+        DevModel& m = s->host;
+        const int b = m.n_blocks;
+        for (int i = 0; i < last - first; ++i) {
+            m.orb_k[b][i][0] = q[2 * i];
+            m.orb_k[b][i][1] = q[2 * i + 1];
+        }
+        ++m.n_blocks;
+    });
+}
+
+int alea_add_ewald(alea_system* s, int n_k, const int* n1, const int* n2, const double* kx,
+                   const double* ky, const double* weights, int n_knots, const double* spline,
+                   double r_cut, double inv_delta, double v_long_zero, double sr_k0) {
+    return guarded([&] {
+        Sys(s);
+        need_unbuilt(s);
+        DevModel& m = s->host;
+        if (!m.cell.periodic) throw InvalidArg("Ewald needs a periodic cell");
+        if (n_k > kMaxKE) throw InvalidArg("too many Ewald vectors");
+        if (n_knots < 2 || n_knots > kMaxKnots) throw InvalidArg("spline knot count out of range");
+        (void)n1;
+        (void)n2;
+        m.ewald = 1;
+        m.nke = n_k;
+        std::copy(kx, kx + n_k, m.kex);
+        std::copy(ky, ky + n_k, m.key);
+        std::copy(weights, weights + n_k, m.kew);
+        std::copy(spline, spline + 3 * n_knots, m.ew_spline);
+        m.ew_knots = n_knots;
+        m.ew_rcut = r_cut;
+        m.ew_inv_delta = inv_delta;
+        // self and background constants (ewald_term.cpp:186-262, one kernel)
+        double q2 = 0.0, Q = 0.0;
+        for (int i = 0; i < s->np; ++i) {
+            q2 += m.charge[i] * m.charge[i];
+            Q += m.charge[i];
+        }
+        m.ew_const = -0.5 * v_long_zero * q2 - 0.5 * Q * Q * sr_k0;
+        m.ew_inv_eps = 1.0;
+    });
+}
+
+int alea_add_coulomb(alea_system* s, int ga, int gb, double epsilon) {
+    return guarded([&] {
+        Sys(s);
+        need_unbuilt(s);
+        check_group(s, ga);
+        check_group(s, gb);
+        DevModel& m = s->host;
+        if (m.n_coul >= kMaxCoulomb) throw InvalidArg("too many Coulomb pairs");
+        if (!(epsilon > 0.0)) throw InvalidArg("epsilon must be positive");
+        const double qa = m.charge[m.g_first[ga]], qb = m.charge[m.g_first[gb]];
+        m.coul_ga[m.n_coul] = ga;
+        m.coul_gb[m.n_coul] = gb;
+        m.coul_coeff[m.n_coul] = qa * qb / epsilon;
+        ++m.n_coul;
+    });
+}
+
+int alea_set_positions(alea_system* s, const double* pos) {
+    return guarded([&] {
+        Sys(s);
+        s->build();
+        std::vector<double> dev;
+        soa_to_walker(pos, dev, s->nw, s->np, s->nw);
+        // wrap into the cell (WalkerSet::load_all, walker_set.cpp:51-70)
+        if (s->host.cell.periodic) {
+            const DevCell& c = s->host.cell;
+            for (size_t i = 0; i < dev.size(); i += 2) {
+                double& x0 = dev[i];
+                double& x1 = dev[i + 1];
+                if (c.ortho) {
+                    x0 -= c.box[0] * std::nearbyint(x0 * c.inv_box[0]);
+                    x1 -= c.box[1] * std::nearbyint(x1 * c.inv_box[1]);
+                } else {
+                    double f0 = 0.0, f1 = 0.0;
+                    f0 += c.ainv[0] * x0;
+                    f0 += c.ainv[1] * x1;
+                    f1 += c.ainv[2] * x0;
+                    f1 += c.ainv[3] * x1;
+                    f0 = std::nearbyint(f0);
+                    f1 = std::nearbyint(f1);
+                    double s0 = 0.0, s1 = 0.0;
+                    s0 += f0 * c.a[0];
+                    s0 += f1 * c.a[2];
+                    s1 += f0 * c.a[1];
+                    s1 += f1 * c.a[3];
+                    x0 -= s0;
+                    x1 -= s1;
+                }
+            }
+        }
+        hip_check(hipMemcpy(s->pos.p, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice),
+                  "upload positions");
+        s->evaluated = false;
+    });
+}
+
+int alea_get_positions(alea_system* s, double* pos) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->built) throw InvalidArg("positions were never set");
+        s->sync();
+        std::vector<double> dev(static_cast<size_t>(s->nw) * s->np * 2);
+        hip_check(hipMemcpy(dev.data(), s->pos.p, dev.size() * sizeof(double), hipMemcpyDeviceToHost),
+                  "download positions");
+        for (int p = 0; p < s->np; ++p)
+            for (int d = 0; d < 2; ++d)
+                for (int w = 0; w < s->nw; ++w)
+                    pos[(p * 2 + d) * s->nw + w] = dev[(static_cast<size_t>(w) * s->np + p) * 2 + d];
+    });
+}
+
+int alea_evaluate(alea_system* s) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->built) throw InvalidArg("positions were never set");
+        launch_evaluate(s->dmodel, s->st, s->nw, s->mixed(), s->cplx, s->stream);
+        hip_check(hipGetLastError(), "launch evaluate");
+        s->sync();
+        s->check_status();
+        s->evaluated = true;
+    });
+}
+
+int alea_get_log_psi(alea_system* s, double* out) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->evaluated) throw InvalidArg("evaluate first");
+        s->sync();
+        std::vector<double> l(static_cast<size_t>(s->nw) * 4);
+        hip_check(hipMemcpy(l.data(), s->logs.p, l.size() * sizeof(double), hipMemcpyDeviceToHost), "logs");
+        for (int w = 0; w < s->nw; ++w) {
+            double t = l[w * 4];
+            for (int b = 0; b < s->host.n_blocks; ++b) t += l[w * 4 + 1 + b];
+            out[w] = t;
+        }
+    });
+}
+
+static void measure(alea_system* s, bool want_grads) {
+    if (!s->evaluated) throw InvalidArg("evaluate first");
+    DevState st = s->st;
+    if (want_grads) {
+        const size_t n = static_cast<size_t>(s->np) * 2 * s->nw;
+        if (s->grad_buf.n < 4 * n) s->grad_buf.alloc(4 * n);
+        st.grad_out = s->grad_buf.p;
+        st.lap_out = s->grad_buf.p + n;
+        st.pgrad_out = s->grad_buf.p + 2 * n;
+        st.plap_out = s->grad_buf.p + 3 * n;
+    }
+    s->set_model_nw(s->nw);
+    launch_measure(s->dmodel, st, s->nw, s->mixed(), s->cplx, s->stream);
+    hip_check(hipGetLastError(), "launch measure");
+    s->sync();
+}
+
+int alea_get_gradients(alea_system* s, double* grad, double* lap) {
+    return guarded([&] {
+        Sys(s);
+        measure(s, true);
+        const size_t n = static_cast<size_t>(s->np) * 2 * s->nw;
+        hip_check(hipMemcpy(grad, s->grad_buf.p, n * sizeof(double), hipMemcpyDeviceToHost), "grad");
+        hip_check(hipMemcpy(lap, s->grad_buf.p + n, n * sizeof(double), hipMemcpyDeviceToHost), "lap");
+    });
+}
+
+int alea_get_phase_gradients(alea_system* s, double* pgrad, double* plap) {
+    return guarded([&] {
+        Sys(s);
+        measure(s, true);
+        const size_t n = static_cast<size_t>(s->np) * 2 * s->nw;
+        hip_check(hipMemcpy(pgrad, s->grad_buf.p + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost), "pgrad");
+        hip_check(hipMemcpy(plap, s->grad_buf.p + 3 * n, n * sizeof(double), hipMemcpyDeviceToHost), "plap");
+    });
+}
+
+int alea_local_energy(alea_system* s, double* e_re, double* e_im) {
+    return guarded([&] {
+        Sys(s);
+        measure(s, false);
+        hip_check(hipMemcpy(e_re, s->e_loc.p, s->nw * sizeof(double), hipMemcpyDeviceToHost), "e_loc");
+        if (e_im)
+            hip_check(hipMemcpy(e_im, s->e_loc_im.p, s->nw * sizeof(double), hipMemcpyDeviceToHost), "e_loc_im");
+    });
+}
+
+int alea_ratio(alea_system* s, int p, const double* new_pos, double* log_ratio, double* phase,
+               double* new_grad, double* new_lap) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->evaluated) throw InvalidArg("evaluate first");
+        if (p < 0 || p >= s->np) throw InvalidArg("particle index out of range");
+        const size_t nw = s->nw;
+        if (s->call_buf.n < 12 * nw) s->call_buf.alloc(12 * nw);
+        double* b = s->call_buf.p;
+        hip_check(hipMemcpy(b, new_pos, 2 * nw * sizeof(double), hipMemcpyHostToDevice), "new_pos");
+        DevCall c;
+        c.new_pos = b;
+        c.log_ratio = b + 2 * nw;
+        c.phase = b + 3 * nw;
+        c.new_grad = b + 4 * nw;
+        c.new_lap = b + 6 * nw;
+        c.new_pgrad = b + 8 * nw;
+        c.new_plap = b + 10 * nw;
+        s->set_model_nw(s->nw);
+        launch_call(s->dmodel, s->st, c, p, 1, s->nw, s->mixed(), s->cplx, s->stream);
+        hip_check(hipGetLastError(), "launch ratio");
+        s->sync();
+        hip_check(hipMemcpy(log_ratio, c.log_ratio, nw * sizeof(double), hipMemcpyDeviceToHost), "lr");
+        if (phase) hip_check(hipMemcpy(phase, c.phase, nw * sizeof(double), hipMemcpyDeviceToHost), "ph");
+        if (new_grad) hip_check(hipMemcpy(new_grad, c.new_grad, 2 * nw * sizeof(double), hipMemcpyDeviceToHost), "ng");
+        if (new_lap) hip_check(hipMemcpy(new_lap, c.new_lap, 2 * nw * sizeof(double), hipMemcpyDeviceToHost), "nl");
+    });
+}
+
+int alea_accept(alea_system* s, int p, const unsigned char* mask) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->evaluated) throw InvalidArg("evaluate first");
+        if (p < 0 || p >= s->np) throw InvalidArg("particle index out of range");
+        const size_t nw = s->nw;
+        if (s->call_buf.n < 12 * nw) throw InvalidArg("accept needs a preceding ratio call");
+        if (s->call_mask.n < nw) s->call_mask.alloc(nw);
+        hip_check(hipMemcpy(s->call_mask.p, mask, nw, hipMemcpyHostToDevice), "mask");
+        DevCall c;
+        c.new_pos = s->call_buf.p;
+        c.mask = s->call_mask.p;
+        s->set_model_nw(s->nw);
+        launch_call(s->dmodel, s->st, c, p, 2, s->nw, s->mixed(), s->cplx, s->stream);
+        hip_check(hipGetLastError(), "launch accept");
+        s->sync();
+        s->check_status();
+    });
+}
+
+static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape, int stride,
+                       bool dmc, long long* n_accepted) {
+    if (!s->evaluated) throw InvalidArg("evaluate first");
+    if (n_sweeps < 0) throw InvalidArg("negative sweep count");
+    SweepParams P;
+    P.n_sweeps = n_sweeps;
+    P.tau = tau;
+    P.dmc = dmc ? 1 : 0;
+    if (tape) {
+        const size_t len = static_cast<size_t>(n_sweeps) * s->np * 3 * stride;
+        if (s->tape.n < len) s->tape.alloc(len);
+        hip_check(hipMemcpy(s->tape.p, tape, len * sizeof(double), hipMemcpyHostToDevice), "tape");
+        P.tape = s->tape.p;
+        P.tape_stride = stride;
+    }
+    hip_check(hipMemsetAsync(s->nacc.p, 0, s->nw * sizeof(long long), s->stream), "memset");
+    s->set_model_nw(s->nw);
+    launch_sweep(s->dmodel, s->st, P, s->nw, s->mixed(), s->cplx, s->stream);
+    hip_check(hipGetLastError(), "launch sweep");
+    if (n_accepted) {
+        s->sync();
+        std::vector<long long> a(s->nw);
+        hip_check(hipMemcpy(a.data(), s->nacc.p, s->nw * sizeof(long long), hipMemcpyDeviceToHost), "nacc");
+        long long t = 0;
+        for (long long x : a) t += x;
+        *n_accepted = t;
+        s->check_status();
+    }
+}
+
+int alea_vmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape,
+                    long long* n_accepted) {
+    return guarded([&] {
+        Sys(s);
+        run_sweeps(s, n_sweeps, tau, tape, s->nw, false, n_accepted);
+    });
+}
+
+int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape, int tape_stride,
+                    double* rr_proposed, double* rr_accepted, long long* n_accepted) {
+    return guarded([&] {
+        Sys(s);
+        if (tape && tape_stride < s->nw) throw InvalidArg("tape stride smaller than the walker count");
+        const size_t nw = s->nw;
+        std::vector<double> rr(2 * nw, 0.0);
+        for (size_t w = 0; w < nw; ++w) {
+            rr[2 * w] = rr_proposed ? rr_proposed[w] : 0.0;
+            rr[2 * w + 1] = rr_accepted ? rr_accepted[w] : 0.0;
+        }
+        hip_check(hipMemcpy(s->rr.p, rr.data(), rr.size() * sizeof(double), hipMemcpyHostToDevice), "rr");
+        run_sweeps(s, n_sweeps, tau, tape, tape ? tape_stride : s->nw, true, n_accepted);
+        s->sync();
+        hip_check(hipMemcpy(rr.data(), s->rr.p, rr.size() * sizeof(double), hipMemcpyDeviceToHost), "rr");
+        for (size_t w = 0; w < nw; ++w) {
+            if (rr_proposed) rr_proposed[w] = rr[2 * w];
+            if (rr_accepted) rr_accepted[w] = rr[2 * w + 1];
+        }
+    });
+}
+
+int alea_seed(alea_system* s, uint64_t seed, uint64_t stream) {
+    return guarded([&] {
+        Sys(s);
+        s->build();
+        std::vector<uint64_t> st(static_cast<size_t>(s->capacity) * 4);
+        for (int w = 0; w < s->capacity; ++w) {
+            // stream_seed(seed, stream, w) (random.h:52-56), then splitmix64
+            // seeding of xoshiro256++ (xoshiro_random.h:29-32)
+            const uint64_t sd = splitmix64(splitmix64(splitmix64(seed) ^ stream) ^ static_cast<uint64_t>(w));
+            for (uint64_t i = 0; i < 4; ++i) st[w * 4 + i] = splitmix64(sd + i * 0x9e3779b97f4a7c15ULL);
+        }
+        hip_check(hipMemcpy(s->rng.p, st.data(), st.size() * sizeof(uint64_t), hipMemcpyHostToDevice), "rng");
+    });
+}
+
+int alea_permute(alea_system* s, int new_nw, const int* src) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->evaluated) throw InvalidArg("evaluate first");
+        if (new_nw <= 0 || new_nw > s->capacity) throw InvalidArg("new walker count exceeds the capacity");
+        for (int j = 0; j < new_nw; ++j)
+            if (src[j] < 0 || src[j] >= s->nw) throw InvalidArg("permutation source out of range");
+        hip_check(hipMemcpy(s->src.p, src, new_nw * sizeof(int), hipMemcpyHostToDevice), "src");
+        auto gather = [&](double* base, size_t stride_doubles) {
+            if (stride_doubles == 0) return;
+            DevBuf<double> tmp;
+            tmp.alloc(static_cast<size_t>(new_nw) * stride_doubles);
+            launch_gather(base, tmp.p, static_cast<int>(stride_doubles), s->src.p, new_nw, s->stream);
+            hip_check(hipMemcpyAsync(base, tmp.p, tmp.n * sizeof(double), hipMemcpyDeviceToDevice, s->stream), "copy");
+            s->sync();
+        };
+        const DevModel& m = s->host;
+        gather(s->pos.p, static_cast<size_t>(m.np) * 2);
+        gather(s->rho.p, static_cast<size_t>(m.ng) * m.nkj * 2);
+        gather(s->logs.p, 4);
+        gather(s->row_abs.p, static_cast<size_t>(kMaxBlocks) * s->st.nb_max);
+        gather(s->cond.p, kMaxBlocks);
+        gather(s->e_loc.p, 1);
+        gather(s->e_loc_im.p, 1);
+        gather(reinterpret_cast<double*>(s->rng.p), 4);
+        if (s->st.inv_stride) {
+            const size_t bytes = static_cast<size_t>(s->st.inv_stride) * s->tsize();
+            gather(reinterpret_cast<double*>(s->inv_bytes.p), bytes / sizeof(double));
+        }
+        // updates: int pairs (kMaxBlocks = 2 ints = 1 double)
+        gather(reinterpret_cast<double*>(s->updates.p), kMaxBlocks * sizeof(int) / sizeof(double));
+        s->nw = new_nw;
+    });
+}
+
+}  // extern "C"

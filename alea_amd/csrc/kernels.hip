@@ -1,0 +1,1286 @@
+// HIP kernels for the batched VMC/DMC walker sweep on CDNA4 (gfx950).
+//
+// One 64-lane wavefront owns one walker (workgroup = one wave, so a
+// __syncthreads() is a wave-local fence and costs nothing).  Inside a wave the
+// lanes split whatever the current step iterates over: partner particles for
+// the pair Jastrow, reciprocal vectors for the k-space Jastrow, orbitals for a
+// determinant ratio, determinant columns for the Sherman-Morrison update.
+// Per-walker state lives in HBM in walker-major records (model.h).
+//
+// Reference algorithms followed (file:line):
+//   minimum image            src/population/simulation_cell.h:109-261
+//   Padé / image-sum split   src/wavefunction/pade_jastrow.cpp:26-56,
+//                            src/wavefunction/periodic_pair_sum.cpp:66-83,
+//                            src/numerics/long_range_spline.h:41-110
+//   Drummond                 src/wavefunction/drummond_jastrow.cpp:27-80
+//   k-space pair sum         src/wavefunction/kspace_pair_sum.cpp:91-301
+//   Nosanow                  src/wavefunction/nosanow_jastrow.cpp:58-174
+//   orbitals                 src/wavefunction/orbital_set.h:164-224,427-474,
+//                            src/wavefunction/complex_orbital_set.h:91-124
+//   Slater ratio / SM / GJ   src/wavefunction/slater_determinant.cpp:395-437,
+//                            744-858, 905-993
+//   kinetic                  src/hamiltonian/kinetic.cpp:45-110
+//   Ewald                    src/hamiltonian/ewald_term.cpp:368-612
+//   VMC / DMC sweeps         src/qmc/vmc.cpp:82-155, src/qmc/dmc.cpp:203-337
+//   RNG                      src/numerics/xoshiro_random.h:23-193
+//
+// Differences from the reference, by design:
+//   * per-particle gradients are not carried incrementally: a move computes
+//     the moved particle's old gradient (DMC drift) from the current state, and
+//     measurement recomputes every particle's gradient and Laplacian.  The
+//     O(N n_k) k-space "seer" update per accepted move disappears.
+//   * the complex determinant is updated by Sherman-Morrison with the real
+//     determinant's rebuild triggers, instead of a full re-inversion per
+//     accepted move (complex_slater_determinant.cpp:245-253).
+//   * the condition estimate uses the 1-norm of A (free in the column-per-lane
+//     update pass) with the infinity-norm row sums of D.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "model.h"
+
+namespace alea_amd {
+
+__constant__ uint64_t c_zig_k[256];
+__constant__ double c_zig_w[256];
+__constant__ double c_zig_f[256];
+__constant__ double c_zig_r;
+
+namespace {
+
+constexpr int W = 64;   // wavefront
+
+__device__ inline double wsum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+    return v;
+}
+__device__ inline double wmax(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, W));
+    return v;
+}
+
+// ── cell ────────────────────────────────────────────────────────────────────
+__device__ inline void min_image(const DevCell& c, double& d0, double& d1) {
+#pragma clang fp contract(off)
+    if (!c.periodic) return;
+    if (c.ortho) {
+        d0 = d0 - c.box[0] * rint(d0 * c.inv_box[0]);
+        d1 = d1 - c.box[1] * rint(d1 * c.inv_box[1]);
+        return;
+    }
+    double s0 = 0.0;
+    s0 += c.ainv[0] * d0;
+    s0 += c.ainv[1] * d1;
+    double s1 = 0.0;
+    s1 += c.ainv[2] * d0;
+    s1 += c.ainv[3] * d1;
+    const double f0 = rint(s0), f1 = rint(s1);
+    double t0 = 0.0;
+    t0 += f0 * c.a[0];
+    t0 += f1 * c.a[2];
+    double t1 = 0.0;
+    t1 += f0 * c.a[1];
+    t1 += f1 * c.a[3];
+    d0 -= t0;
+    d1 -= t1;
+    double b0 = d0, b1 = d1, b2 = 0.0;
+    b2 += d0 * d0;
+    b2 += d1 * d1;
+    for (int k = 1; k < 9; ++k) {
+        const double y0 = d0 + c.cand[2 * k];
+        double r2 = 0.0;
+        r2 += y0 * y0;
+        const double y1 = d1 + c.cand[2 * k + 1];
+        r2 += y1 * y1;
+        const double take = (r2 < b2) ? 1.0 : 0.0;
+        b2 += take * (r2 - b2);
+        b0 += take * (y0 - b0);
+        b1 += take * (y1 - b1);
+    }
+    d0 = b0;
+    d1 = b1;
+}
+
+__device__ inline void wrap(const DevCell& c, double& x0, double& x1) {
+#pragma clang fp contract(off)
+    if (!c.periodic) return;
+    if (c.ortho) {
+        x0 -= c.box[0] * rint(x0 * c.inv_box[0]);
+        x1 -= c.box[1] * rint(x1 * c.inv_box[1]);
+        return;
+    }
+    double f[2];
+    for (int i = 0; i < 2; ++i) {
+        double s = 0.0;
+        s += c.ainv[i * 2 + 0] * x0;
+        s += c.ainv[i * 2 + 1] * x1;
+        f[i] = rint(s);
+    }
+    double s0 = 0.0, s1 = 0.0;
+    s0 += f[0] * c.a[0];
+    s0 += f[1] * c.a[2];
+    s1 += f[0] * c.a[1];
+    s1 += f[1] * c.a[3];
+    x0 -= s0;
+    x1 -= s1;
+}
+
+__device__ inline double dist(const DevCell& c, double& d0, double& d1) {
+    min_image(c, d0, d1);
+    return sqrt(d0 * d0 + d1 * d1);
+}
+
+// ── radial functions ───────────────────────────────────────────────────────
+__device__ inline void spline_eval(const double* sc, int n_knots, double r_cut, double inv_delta,
+                                   double r, double& f, double& df, double& d2f) {
+    if (r >= r_cut) { f = df = d2f = 0.0; return; }
+    const double t = r * inv_delta;
+    int i = static_cast<int>(t);
+    if (i > n_knots - 2) i = n_knots - 2;
+    const double x = t - i, y = 1.0 - x;
+    const double* a = sc + 3 * i;
+    const double* b = a + 3;
+    const double x2 = x * x, x3 = x2 * x, x4 = x3 * x, x5 = x4 * x;
+    const double y2 = y * y, y3 = y2 * y, y4 = y3 * y, y5 = y4 * y;
+    const double p0 = 1.0 - 10.0 * x3 + 15.0 * x4 - 6.0 * x5;
+    const double p1 = x - 6.0 * x3 + 8.0 * x4 - 3.0 * x5;
+    const double p2 = 0.5 * x2 - 1.5 * x3 + 1.5 * x4 - 0.5 * x5;
+    const double q0 = 1.0 - 10.0 * y3 + 15.0 * y4 - 6.0 * y5;
+    const double q1 = y - 6.0 * y3 + 8.0 * y4 - 3.0 * y5;
+    const double q2 = 0.5 * y2 - 1.5 * y3 + 1.5 * y4 - 0.5 * y5;
+    const double dp0 = -30.0 * x2 + 60.0 * x3 - 30.0 * x4;
+    const double dp1 = 1.0 - 18.0 * x2 + 32.0 * x3 - 15.0 * x4;
+    const double dp2 = x - 4.5 * x2 + 6.0 * x3 - 2.5 * x4;
+    const double dq0 = -30.0 * y2 + 60.0 * y3 - 30.0 * y4;
+    const double dq1 = 1.0 - 18.0 * y2 + 32.0 * y3 - 15.0 * y4;
+    const double dq2 = y - 4.5 * y2 + 6.0 * y3 - 2.5 * y4;
+    const double ddp0 = -60.0 * x + 180.0 * x2 - 120.0 * x3;
+    const double ddp1 = -36.0 * x + 96.0 * x2 - 60.0 * x3;
+    const double ddp2 = 1.0 - 9.0 * x + 18.0 * x2 - 10.0 * x3;
+    const double ddq0 = -60.0 * y + 180.0 * y2 - 120.0 * y3;
+    const double ddq1 = -36.0 * y + 96.0 * y2 - 60.0 * y3;
+    const double ddq2 = 1.0 - 9.0 * y + 18.0 * y2 - 10.0 * y3;
+    f = a[0] * p0 + a[1] * p1 + a[2] * p2 + b[0] * q0 - b[1] * q1 + b[2] * q2;
+    df = inv_delta * (a[0] * dp0 + a[1] * dp1 + a[2] * dp2 - (b[0] * dq0 - b[1] * dq1 + b[2] * dq2));
+    d2f = inv_delta * inv_delta *
+          (a[0] * ddp0 + a[1] * ddp1 + a[2] * ddp2 + b[0] * ddq0 - b[1] * ddq1 + b[2] * ddq2);
+}
+
+__device__ inline void radial_eval(const DevRadial& R, double r, double& u, double& du, double& d2u) {
+    if (R.kind == 1) {
+        const double t = r - R.L;
+        double tCm2 = 1.0;
+        for (int i = 0; i < R.C - 2; ++i) tCm2 *= t;
+        const double tCm1 = tCm2 * t, tC = tCm1 * t;
+        const double g = tC, dg = static_cast<double>(R.C) * tCm1,
+                     d2g = static_cast<double>(R.C) * (R.C - 1) * tCm2;
+        double P = R.c[0] + R.alpha1 * r, dP = R.alpha1, d2P = 0.0;
+        double r_lm2 = 1.0, r_lm1 = r, r_l = r * r;
+        for (int k = 1; k < R.num; ++k) {
+            const int l = k + 1;
+            P += R.c[k] * r_l;
+            dP += R.c[k] * l * r_lm1;
+            d2P += R.c[k] * l * (l - 1) * r_lm2;
+            r_lm2 = r_lm1;
+            r_lm1 = r_l;
+            r_l *= r;
+        }
+        const double m = (r < R.L) ? 1.0 : 0.0;
+        u = m * (g * P);
+        du = m * (dg * P + g * dP);
+        d2u = m * (d2g * P + 2.0 * dg * dP + g * d2P);
+        return;
+    }
+    double p = 0.0, dp = 0.0, d2p = 0.0;
+    {
+        double r1 = 1.0, r2 = 0.0;
+        for (int i = 0; i < R.num; ++i) {
+            const int k = i + 1;
+            p += R.c[i] * r1 * r;
+            dp += R.c[i] * k * r1;
+            d2p += R.c[i] * k * (k - 1) * r2;
+            r2 = r1;
+            r1 *= r;
+        }
+    }
+    double q = 1.0, dq = 0.0, d2q = 0.0;
+    {
+        double r1 = 1.0, r2 = 0.0;
+        for (int i = 0; i < R.den; ++i) {
+            const int k = i + 1;
+            q += R.c[R.num + i] * r1 * r;
+            dq += R.c[R.num + i] * k * r1;
+            d2q += R.c[R.num + i] * k * (k - 1) * r2;
+            r2 = r1;
+            r1 *= r;
+        }
+    }
+    const double iq = 1.0 / q;
+    u = p * iq;
+    du = (dp * q - p * dq) * iq * iq;
+    d2u = (d2p - (2.0 * dp * dq - 2.0 * p * dq * dq * iq + p * d2q) * iq) * iq;
+    if (R.image) {
+        if (r >= R.r_cut) { u = du = d2u = 0.0; return; }
+        double f, df, d2f;
+        spline_eval(R.spline, R.n_knots, R.r_cut, R.inv_delta, r, f, df, d2f);
+        u -= R.u_inf + f;
+        du -= df;
+        d2u -= d2f;
+    }
+}
+
+// Pair contribution of a radial u to the gradient and per-dimension
+// Laplacian of the particle at +delta (pade_jastrow.cpp:268-295).
+__device__ inline void pair_terms(double du, double d2u, double r, double d0, double d1,
+                                  double& g0, double& g1, double& l0, double& l1) {
+    const double invr = 1.0 / r;
+    const double o0 = d0 * invr, o1 = d1 * invr;
+    g0 = du * o0;
+    g1 = du * o1;
+    l0 = d2u * o0 * o0 + du * invr * (1.0 - o0 * o0);
+    l1 = d2u * o1 * o1 + du * invr * (1.0 - o1 * o1);
+}
+
+// ── orbitals (computed in FP64, stored in the determinant precision) ───────
+struct Orb {
+    double vr, vi;          // value
+    double gr[2], gi[2];    // gradient
+    double lr[2], li[2];    // per-dimension second derivative
+};
+
+__device__ inline void orbital(const DevModel& M, int b, int m, double x, double y, Orb& o) {
+    const int kind = M.blk_kind[b];
+    const double kx = M.orb_k[b][m][0], ky = M.orb_k[b][m][1];
+    if (kind == kGaussian) {
+        double d0 = x - kx, d1 = y - ky;
+        if (M.cell.periodic) min_image(M.cell, d0, d1);
+        const double a = M.orb_alpha[b][m];
+        const double phi = exp(-a * (d0 * d0 + d1 * d1));
+        o.vr = phi; o.vi = 0.0;
+        o.gr[0] = phi * (-2.0 * a * d0); o.gr[1] = phi * (-2.0 * a * d1);
+        o.lr[0] = phi * (4.0 * a * a * d0 * d0 - 2.0 * a);
+        o.lr[1] = phi * (4.0 * a * a * d1 * d1 - 2.0 * a);
+        o.gi[0] = o.gi[1] = o.li[0] = o.li[1] = 0.0;
+        return;
+    }
+    double s, c;
+    sincos(kx * x + ky * y, &s, &c);
+    if (kind == kTwisted) {
+        o.vr = c; o.vi = s;
+        o.gr[0] = -kx * s; o.gi[0] = kx * c;
+        o.gr[1] = -ky * s; o.gi[1] = ky * c;
+        o.lr[0] = -kx * kx * c; o.li[0] = -kx * kx * s;
+        o.lr[1] = -ky * ky * c; o.li[1] = -ky * ky * s;
+        return;
+    }
+    const bool sn = M.orb_sin[b][m] != 0;
+    o.vr = sn ? s : c; o.vi = 0.0;
+    o.gr[0] = sn ? kx * c : -kx * s;
+    o.gr[1] = sn ? ky * c : -ky * s;
+    o.lr[0] = sn ? -kx * kx * s : -kx * kx * c;
+    o.lr[1] = sn ? -ky * ky * s : -ky * ky * c;
+    o.gi[0] = o.gi[1] = o.li[0] = o.li[1] = 0.0;
+}
+
+// ── xoshiro256++ and the ziggurat, every lane stepping the same state ──────
+struct Rng {
+    uint64_t s[4];
+    __device__ uint64_t next() {
+        const uint64_t result = ((s[0] + s[3]) << 23 | (s[0] + s[3]) >> 41) + s[0];
+        const uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = (s[3] << 45) | (s[3] >> 19);
+        return result;
+    }
+    __device__ double uniform() {
+        return static_cast<double>(static_cast<int64_t>(next() >> 11)) * 0x1.0p-53;
+    }
+    __device__ double gaussian() {
+        constexpr uint64_t mask = (uint64_t{1} << 52) - 1;
+        for (;;) {
+            const uint64_t bits = next();
+            const int i = static_cast<int>(bits & 0xff);
+            const uint64_t sign = (bits & 0x100) << 55;
+            const uint64_t position = (bits >> 9) & mask;
+            const double x = static_cast<double>(static_cast<int64_t>(position)) * c_zig_w[i];
+            if (position < c_zig_k[i]) return __longlong_as_double(__double_as_longlong(x) ^ sign);
+            if (i == 0) {
+                for (;;) {
+                    const double xi = -log1p(-uniform()) / c_zig_r;
+                    const double y = -log1p(-uniform());
+                    if (y + y > xi * xi)
+                        return __longlong_as_double(__double_as_longlong(c_zig_r + xi) ^ sign);
+                }
+            }
+            if (c_zig_f[i] + uniform() * (c_zig_f[i - 1] - c_zig_f[i]) < exp(-0.5 * x * x))
+                return __longlong_as_double(__double_as_longlong(x) ^ sign);
+        }
+    }
+};
+
+// ── complex helpers in the determinant precision ───────────────────────────
+template <typename T>
+struct C2 {
+    T re, im;
+};
+template <typename T>
+__device__ inline C2<T> cmul(C2<T> a, C2<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// Determinant inverse element access; complex entries are (re, im) pairs.
+template <typename T, bool CPLX>
+struct InvRef {
+    T* base;
+    int n;
+    __device__ C2<T> get(int m, int c) const {
+        if constexpr (CPLX) {
+            const T* p = base + 2 * (m * n + c);
+            return {p[0], p[1]};
+        } else {
+            return {base[m * n + c], T(0)};
+        }
+    }
+    __device__ void set(int m, int c, C2<T> v) const {
+        if constexpr (CPLX) {
+            T* p = base + 2 * (m * n + c);
+            p[0] = v.re;
+            p[1] = v.im;
+        } else {
+            base[m * n + c] = v.re;
+        }
+    }
+};
+
+// LDS scratch of one walker.
+struct Shared {
+    double pos[kMaxParticles * 2];
+    double rho[kMaxGroups * kMaxKJ * 2];
+    double v[2 * kMaxOrb];       // proposed orbital values
+    double u[2 * kMaxOrb];       // column j of A (old)
+    double f[2 * kMaxOrb];       // u / R
+    double red[64];
+};
+
+// Sum reduction of n lane values at once (n <= 16) through LDS-free shuffles.
+template <int N>
+__device__ inline void wsum_n(double (&v)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = wsum(v[i]);
+}
+
+// ── the per-walker engine ──────────────────────────────────────────────────
+template <typename T, bool CPLX>
+struct Walker {
+    const DevModel& M;
+    const DevState& S;
+    Shared& sh;
+    int w, lane;
+    double log_j;            // Jastrow + one-body part of log|psi|
+    double log_b[kMaxBlocks];
+
+    __device__ Walker(const DevModel& m, const DevState& s, Shared& shm, int walker)
+        : M(m), S(s), sh(shm), w(walker), lane(threadIdx.x) {}
+
+    __device__ InvRef<T, CPLX> inv(int b) const {
+        T* base = static_cast<T*>(S.inv) + static_cast<size_t>(w) * S.inv_stride;
+        int off = 0;
+        for (int i = 0; i < b; ++i) off += (CPLX ? 2 : 1) * M.blk_n[i] * M.blk_n[i];
+        return {base + off, M.blk_n[b]};
+    }
+
+    __device__ void load() {
+        const double* p = S.pos + static_cast<size_t>(w) * M.np * 2;
+        for (int i = lane; i < 2 * M.np; i += W) sh.pos[i] = p[i];
+        const int nr = M.ng * M.nkj * 2;
+        const double* r = S.rho + static_cast<size_t>(w) * nr;
+        for (int i = lane; i < nr; i += W) sh.rho[i] = r[i];
+        log_j = S.logs[w * 4 + 0];
+        for (int b = 0; b < M.n_blocks; ++b) log_b[b] = S.logs[w * 4 + 1 + b];
+        __syncthreads();
+    }
+    __device__ void store() {
+        __syncthreads();
+        double* p = S.pos + static_cast<size_t>(w) * M.np * 2;
+        for (int i = lane; i < 2 * M.np; i += W) p[i] = sh.pos[i];
+        const int nr = M.ng * M.nkj * 2;
+        double* r = S.rho + static_cast<size_t>(w) * nr;
+        for (int i = lane; i < nr; i += W) r[i] = sh.rho[i];
+        if (lane == 0) {
+            S.logs[w * 4 + 0] = log_j;
+            for (int b = 0; b < M.n_blocks; ++b) S.logs[w * 4 + 1 + b] = log_b[b];
+        }
+    }
+
+    // ── Jastrow + one-body: log-ratio, and the moved particle's gradient and
+    //    Laplacian at the old (o*) and new (n*) positions ────────────────────
+    struct JOut {
+        double dlog;
+        double og[2], ng[2], nl[2];
+    };
+    __device__ void jastrow_move(int p, double x0, double x1, double nx0, double nx1, bool derivs,
+                                 JOut& out) {
+        const int gp = M.group_of[p];
+        double acc[7] = {0, 0, 0, 0, 0, 0, 0};   // dlog, og0, og1, ng0, ng1, nl0, nl1
+        for (int k = lane; k < M.np; k += W) {
+            if (k == p) continue;
+            const int gk = M.group_of[k];
+            const int nf = M.pair_list_n[gp][gk];
+            if (nf == 0) continue;
+            const double xk0 = sh.pos[2 * k], xk1 = sh.pos[2 * k + 1];
+            double o0 = x0 - xk0, o1 = x1 - xk1;
+            const double ro = dist(M.cell, o0, o1);
+            double n0 = nx0 - xk0, n1 = nx1 - xk1;
+            const double rn = dist(M.cell, n0, n1);
+            for (int fi = 0; fi < nf; ++fi) {
+                const DevRadial& R = M.pair[M.pair_list[gp][gk][fi]].rad;
+                double uo, duo, d2uo, un, dun, d2un;
+                radial_eval(R, ro, uo, duo, d2uo);
+                radial_eval(R, rn, un, dun, d2un);
+                acc[0] += un - uo;
+                if (derivs) {
+                    double g0, g1, l0, l1;
+                    pair_terms(duo, d2uo, ro, o0, o1, g0, g1, l0, l1);
+                    acc[1] += g0;
+                    acc[2] += g1;
+                    pair_terms(dun, d2un, rn, n0, n1, g0, g1, l0, l1);
+                    acc[3] += g0;
+                    acc[4] += g1;
+                    acc[5] += l0;
+                    acc[6] += l1;
+                }
+            }
+        }
+        // k-space: lanes over k
+        if (M.nkj > 0 && M.kj_active[gp]) {
+            for (int k = lane; k < M.nkj; k += W) {
+                const double kx = M.kjx[k], ky = M.kjy[k];
+                double so, co, sn, cn;
+                sincos(kx * x0 + ky * x1, &so, &co);
+                sincos(kx * nx0 + ky * nx1, &sn, &cn);
+                const double dr = cn - co, di = sn - so;
+                double dl = 0.0;
+                // moved particle's gradient at the old / new position
+                double gold = 0.0, gnew = 0.0, lnew = 0.0;
+                for (int h = 0; h < M.ng; ++h) {
+                    if (!M.kj_active[h]) continue;
+                    const double wgh = M.kjw[gp][h][k];
+                    if (wgh == 0.0) continue;
+                    const double rr = sh.rho[(h * M.nkj + k) * 2], ri = sh.rho[(h * M.nkj + k) * 2 + 1];
+                    if (h == gp) {
+                        dl += wgh * (2.0 * (rr * dr + ri * di) + dr * dr + di * di);
+                        if (derivs) {
+                            gold += 2.0 * wgh * (co * ri - so * rr);
+                            const double qr = rr + dr, qi = ri + di;
+                            gnew += 2.0 * wgh * (cn * qi - sn * qr);
+                            lnew += 2.0 * wgh * (1.0 - (cn * qr + sn * qi));
+                        }
+                    } else {
+                        dl += 2.0 * wgh * (dr * rr + di * ri);
+                        if (derivs) {
+                            gold += 2.0 * wgh * (co * ri - so * rr);
+                            gnew += 2.0 * wgh * (cn * ri - sn * rr);
+                            lnew += -2.0 * wgh * (cn * rr + sn * ri);
+                        }
+                    }
+                }
+                acc[0] += dl;
+                if (derivs) {
+                    acc[1] += kx * gold;
+                    acc[2] += ky * gold;
+                    acc[3] += kx * gnew;
+                    acc[4] += ky * gnew;
+                    acc[5] += kx * kx * lnew;
+                    acc[6] += ky * ky * lnew;
+                }
+            }
+        }
+        wsum_n(acc);
+        // Nosanow one-body (uniform across lanes)
+        if (M.has_nosanow && M.nos_alpha[p] != 0.0) {
+            const double a = M.nos_alpha[p];
+            double o0 = x0 - M.nos_site[p][0], o1 = x1 - M.nos_site[p][1];
+            double n0 = nx0 - M.nos_site[p][0], n1 = nx1 - M.nos_site[p][1];
+            if (M.cell.periodic) { min_image(M.cell, o0, o1); min_image(M.cell, n0, n1); }
+            acc[0] += (-a * n0 * n0 - a * n1 * n1) - (-a * o0 * o0 - a * o1 * o1);
+            acc[1] += -2.0 * a * o0;
+            acc[2] += -2.0 * a * o1;
+            acc[3] += -2.0 * a * n0;
+            acc[4] += -2.0 * a * n1;
+            acc[5] += -2.0 * a;
+            acc[6] += -2.0 * a;
+        }
+        out.dlog = acc[0];
+        out.og[0] = acc[1]; out.og[1] = acc[2];
+        out.ng[0] = acc[3]; out.ng[1] = acc[4];
+        out.nl[0] = acc[5]; out.nl[1] = acc[6];
+    }
+
+    // Commit the k-space densities of an accepted move of p.
+    __device__ void jastrow_commit(int p, double x0, double x1, double nx0, double nx1) {
+        const int gp = M.group_of[p];
+        if (M.nkj > 0 && M.kj_active[gp]) {
+            for (int k = lane; k < M.nkj; k += W) {
+                double so, co, sn, cn;
+                sincos(M.kjx[k] * x0 + M.kjy[k] * x1, &so, &co);
+                sincos(M.kjx[k] * nx0 + M.kjy[k] * nx1, &sn, &cn);
+                sh.rho[(gp * M.nkj + k) * 2] += cn - co;
+                sh.rho[(gp * M.nkj + k) * 2 + 1] += sn - so;
+            }
+        }
+    }
+
+    // ── determinant: ratio of moving p (column j of block b) ───────────────
+    struct DOut {
+        double R[2];       // ratio (complex)
+        double scale;      // sum |A_mj v_m|
+        double og[2][2];   // old gradient of p, complex per dim: [d][re/im]
+        double ol[2][2];
+        double ng[2][2];   // Sum_m A_mj grad phi_m(r') (before the 1/R)
+        double nl[2][2];
+        double rowabs;     // sum_m |v_m|
+    };
+    __device__ void det_ratio(int b, int j, double x0, double x1, double nx0, double nx1, bool derivs,
+                              bool old_derivs, DOut& o) {
+        const InvRef<T, CPLX> A = inv(b);
+        const int n = M.blk_n[b];
+        double acc[20];
+        for (int i = 0; i < 20; ++i) acc[i] = 0.0;
+        for (int m = lane; m < n; m += W) {
+            const C2<T> a = A.get(m, j);
+            const double ar = static_cast<double>(a.re), ai = static_cast<double>(a.im);
+            Orb on;
+            orbital(M, b, m, nx0, nx1, on);
+            // values stored in the determinant precision
+            const T vr = static_cast<T>(on.vr), vi = static_cast<T>(on.vi);
+            sh.v[2 * m] = static_cast<double>(vr);
+            sh.v[2 * m + 1] = static_cast<double>(vi);
+            sh.u[2 * m] = ar;
+            sh.u[2 * m + 1] = ai;
+            const double tr = ar * static_cast<double>(vr) - ai * static_cast<double>(vi);
+            const double ti = ar * static_cast<double>(vi) + ai * static_cast<double>(vr);
+            acc[0] += tr;
+            acc[1] += ti;
+            acc[2] += CPLX ? sqrt(tr * tr + ti * ti) : fabs(tr);
+            acc[19] += CPLX ? sqrt(on.vr * on.vr + on.vi * on.vi) : fabs(static_cast<double>(vr));
+            if (derivs) {
+                for (int d = 0; d < 2; ++d) {
+                    acc[3 + 2 * d] += ar * on.gr[d] - ai * on.gi[d];
+                    acc[4 + 2 * d] += ar * on.gi[d] + ai * on.gr[d];
+                    acc[7 + 2 * d] += ar * on.lr[d] - ai * on.li[d];
+                    acc[8 + 2 * d] += ar * on.li[d] + ai * on.lr[d];
+                }
+            }
+            if (old_derivs) {
+                Orb oo;
+                orbital(M, b, m, x0, x1, oo);
+                for (int d = 0; d < 2; ++d) {
+                    acc[11 + 2 * d] += ar * oo.gr[d] - ai * oo.gi[d];
+                    acc[12 + 2 * d] += ar * oo.gi[d] + ai * oo.gr[d];
+                    acc[15 + 2 * d] += ar * oo.lr[d] - ai * oo.li[d];
+                    acc[16 + 2 * d] += ar * oo.li[d] + ai * oo.lr[d];
+                }
+            }
+        }
+        wsum_n(acc);
+        o.R[0] = acc[0];
+        o.R[1] = acc[1];
+        o.scale = acc[2];
+        o.rowabs = acc[19];
+        for (int d = 0; d < 2; ++d) {
+            o.ng[d][0] = acc[3 + 2 * d]; o.ng[d][1] = acc[4 + 2 * d];
+            o.nl[d][0] = acc[7 + 2 * d]; o.nl[d][1] = acc[8 + 2 * d];
+            o.og[d][0] = acc[11 + 2 * d]; o.og[d][1] = acc[12 + 2 * d];
+            o.ol[d][0] = acc[15 + 2 * d]; o.ol[d][1] = acc[16 + 2 * d];
+        }
+        __syncthreads();
+    }
+
+    // Pivoted Gauss-Jordan rebuild of block b from the current positions
+    // (slater_determinant.cpp:905-993), in FP64, in the walker's workspace.
+    // Returns log|det D|; sets status on a singular matrix.
+    __device__ double rebuild(int b) {
+        const int n = M.blk_n[b];
+        const int e = CPLX ? 2 : 1;
+        double* Mm = S.work + static_cast<size_t>(w) * S.work_stride;
+        double* Aa = Mm + e * n * n;
+        const int first = M.blk_first[b];
+        for (int m = lane; m < n; m += W)
+            for (int kr = 0; kr < n; ++kr) {
+                Orb o;
+                orbital(M, b, m, sh.pos[2 * (first + kr)], sh.pos[2 * (first + kr) + 1], o);
+                Mm[e * (kr * n + m)] = o.vr;
+                if (CPLX) Mm[e * (kr * n + m) + 1] = o.vi;
+            }
+        for (int c = lane; c < n; c += W)
+            for (int r = 0; r < n; ++r) {
+                Aa[e * (r * n + c)] = r == c ? 1.0 : 0.0;
+                if (CPLX) Aa[e * (r * n + c) + 1] = 0.0;
+            }
+        __syncthreads();
+        double log_det = 0.0;
+        for (int k = 0; k < n; ++k) {
+            // pivot: first row of largest |M[r][k]|, r >= k
+            double best = -1.0;
+            int piv = n;
+            for (int r = k + lane; r < n; r += W) {
+                const double re = Mm[e * (r * n + k)];
+                const double im = CPLX ? Mm[e * (r * n + k) + 1] : 0.0;
+                const double a = CPLX ? sqrt(re * re + im * im) : fabs(re);
+                if (a > best) { best = a; piv = r; }
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ob = __shfl_xor(best, o, W);
+                const int op = __shfl_xor(piv, o, W);
+                if (ob > best || (ob == best && op < piv)) { best = ob; piv = op; }
+            }
+            const double dr = Mm[e * (piv * n + k)];
+            const double di = CPLX ? Mm[e * (piv * n + k) + 1] : 0.0;
+            const double dabs = CPLX ? sqrt(dr * dr + di * di) : fabs(dr);
+            if (!(dabs > 0.0) || !isfinite(dabs)) {
+                if (lane == 0) S.status[w] = 1;
+                return log_det;
+            }
+            log_det += log(dabs);
+            __syncthreads();
+            if (piv != k) {
+                for (int c = lane; c < n; c += W)
+                    for (int q = 0; q < e; ++q) {
+                        double t = Mm[e * (k * n + c) + q];
+                        Mm[e * (k * n + c) + q] = Mm[e * (piv * n + c) + q];
+                        Mm[e * (piv * n + c) + q] = t;
+                        t = Aa[e * (k * n + c) + q];
+                        Aa[e * (k * n + c) + q] = Aa[e * (piv * n + c) + q];
+                        Aa[e * (piv * n + c) + q] = t;
+                    }
+                __syncthreads();
+            }
+            // row k /= diag
+            double ir, ii;
+            if (CPLX) {
+                const double d2 = dr * dr + di * di;
+                ir = dr / d2;
+                ii = -di / d2;
+            } else {
+                ir = 1.0 / dr;
+                ii = 0.0;
+            }
+            for (int c = lane; c < n; c += W) {
+                if (CPLX) {
+                    const double mr = Mm[2 * (k * n + c)], mi = Mm[2 * (k * n + c) + 1];
+                    Mm[2 * (k * n + c)] = mr * ir - mi * ii;
+                    Mm[2 * (k * n + c) + 1] = mr * ii + mi * ir;
+                    const double ar = Aa[2 * (k * n + c)], ai = Aa[2 * (k * n + c) + 1];
+                    Aa[2 * (k * n + c)] = ar * ir - ai * ii;
+                    Aa[2 * (k * n + c) + 1] = ar * ii + ai * ir;
+                } else {
+                    Mm[k * n + c] /= dr;
+                    Aa[k * n + c] /= dr;
+                }
+            }
+            __syncthreads();
+            // eliminate column k from every other row
+            for (int c = lane; c < n; c += W) {
+                const double mkr = Mm[e * (k * n + c)];
+                const double mki = CPLX ? Mm[e * (k * n + c) + 1] : 0.0;
+                const double akr = Aa[e * (k * n + c)];
+                const double aki = CPLX ? Aa[e * (k * n + c) + 1] : 0.0;
+                for (int r = 0; r < n; ++r) {
+                    if (r == k) continue;
+                    const double fr = Mm[e * (r * n + k)];
+                    const double fi = CPLX ? Mm[e * (r * n + k) + 1] : 0.0;
+                    if (c != k) {
+                        Mm[e * (r * n + c)] -= fr * mkr - fi * mki;
+                        if (CPLX) Mm[e * (r * n + c) + 1] -= fr * mki + fi * mkr;
+                    }
+                    Aa[e * (r * n + c)] -= fr * akr - fi * aki;
+                    if (CPLX) Aa[e * (r * n + c) + 1] -= fr * aki + fi * akr;
+                }
+            }
+            __syncthreads();
+            // column k of the reduced matrix is only read through M[r][k]
+            // above; set it to the identity column now that the step is done
+            for (int r = lane; r < n; r += W) {
+                Mm[e * (r * n + k)] = r == k ? 1.0 : 0.0;
+                if (CPLX) Mm[e * (r * n + k) + 1] = 0.0;
+            }
+            __syncthreads();
+        }
+        // copy to the inverse in the determinant precision, row sums of D
+        const InvRef<T, CPLX> A = inv(b);
+        double colmax = 0.0;
+        for (int c = lane; c < n; c += W) {
+            double cs = 0.0;
+            for (int m = 0; m < n; ++m) {
+                const double ar = Aa[e * (m * n + c)];
+                const double ai = CPLX ? Aa[e * (m * n + c) + 1] : 0.0;
+                A.set(m, c, C2<T>{static_cast<T>(ar), static_cast<T>(ai)});
+                cs += CPLX ? sqrt(ar * ar + ai * ai) : fabs(ar);
+            }
+            colmax = fmax(colmax, cs);
+        }
+        colmax = wmax(colmax);
+        double rowmax = 0.0;
+        for (int kr = lane; kr < n; kr += W) {
+            double rs = 0.0;
+            for (int m = 0; m < n; ++m) {
+                Orb o;
+                orbital(M, b, m, sh.pos[2 * (first + kr)], sh.pos[2 * (first + kr) + 1], o);
+                rs += CPLX ? sqrt(o.vr * o.vr + o.vi * o.vi) : fabs(o.vr);
+            }
+            S.row_abs[(static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max + kr] = rs;
+            rowmax = fmax(rowmax, rs);
+        }
+        rowmax = wmax(rowmax);
+        if (lane == 0) {
+            S.cond[w * kMaxBlocks + b] = rowmax * colmax;
+            S.updates[w * kMaxBlocks + b] = 0;
+        }
+        __syncthreads();
+        return log_det;
+    }
+
+    // Sherman-Morrison update of block b for the accepted move of column j
+    // (slater_determinant.cpp:744-858), with the reference's rebuild
+    // triggers.  sh.v holds v' (orbitals at r'), sh.u column j of A, and
+    // sh.pos already holds the committed position of the moved particle.
+    __device__ bool det_commit(int b, int j, const DOut& d) {
+        const InvRef<T, CPLX> A = inv(b);
+        const int n = M.blk_n[b];
+        const double Rr = d.R[0], Ri = d.R[1];
+        const double absR = CPLX ? sqrt(Rr * Rr + Ri * Ri) : fabs(Rr);
+        const double eps = sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
+        int* upd = S.updates + w * kMaxBlocks + b;
+        const int updates = *upd + 1;
+        double* rabs = S.row_abs + (static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max;
+        const double cond_old = S.cond[w * kMaxBlocks + b];
+        __syncthreads();
+        bool rebuild_now = updates >= 64 || !isfinite(absR) || absR < 1e-5 || cond_old > 1e8 ||
+                           absR <= 1e4 * eps * d.scale;
+        if (!rebuild_now) {
+            // pass 1: row product r_c = sum_m v'_m A[m][c] - delta_cj
+            const T invRr = static_cast<T>(CPLX ? Rr / (absR * absR) : 1.0 / Rr);
+            const T invRi = static_cast<T>(CPLX ? -Ri / (absR * absR) : 0.0);
+            for (int m = lane; m < n; m += W) {
+                const C2<T> um{static_cast<T>(sh.u[2 * m]), static_cast<T>(sh.u[2 * m + 1])};
+                const C2<T> fm = cmul(um, C2<T>{invRr, invRi});
+                sh.f[2 * m] = static_cast<double>(fm.re);
+                sh.f[2 * m + 1] = static_cast<double>(fm.im);
+            }
+            __syncthreads();
+            double colmax = 0.0;
+            for (int c = lane; c < n; c += W) {
+                C2<T> r{T(0), T(0)};
+                for (int m = 0; m < n; ++m) {
+                    const C2<T> vm{static_cast<T>(sh.v[2 * m]), static_cast<T>(sh.v[2 * m + 1])};
+                    const C2<T> a = A.get(m, c);
+                    const C2<T> pr = cmul(vm, a);
+                    r.re += pr.re;
+                    r.im += pr.im;
+                }
+                if (c == j) r.re -= T(1);
+                // pass 2: A[m][c] -= f_m r_c
+                double cs = 0.0;
+                for (int m = 0; m < n; ++m) {
+                    const C2<T> fm{static_cast<T>(sh.f[2 * m]), static_cast<T>(sh.f[2 * m + 1])};
+                    C2<T> a = A.get(m, c);
+                    const C2<T> pr = cmul(fm, r);
+                    a.re -= pr.re;
+                    a.im -= pr.im;
+                    A.set(m, c, a);
+                    cs += CPLX ? sqrt(static_cast<double>(a.re) * a.re + static_cast<double>(a.im) * a.im)
+                               : fabs(static_cast<double>(a.re));
+                }
+                colmax = fmax(colmax, cs);
+            }
+            colmax = wmax(colmax);
+            if (lane == 0) rabs[j] = d.rowabs;
+            __syncthreads();
+            double rowmax = 0.0;
+            for (int kr = lane; kr < n; kr += W) rowmax = fmax(rowmax, rabs[kr]);
+            rowmax = wmax(rowmax);
+            const double cond = rowmax * colmax;
+            if (!isfinite(cond) || cond > 1e8) {
+                rebuild_now = true;
+            } else {
+                log_b[b] += log(absR);
+                if (lane == 0) {
+                    *upd = updates;
+                    S.cond[w * kMaxBlocks + b] = cond;
+                }
+            }
+        }
+        if (rebuild_now) log_b[b] = rebuild(b);
+        __syncthreads();
+        return rebuild_now;
+    }
+
+    // A twisted plane wave is not periodic: phi(r - L) = phi(r) e^{-i q.L}.
+    // Wrapping the accepted position re-gauges row j of D by that phase, so
+    // column j of A = D^-1 takes the inverse phase (the reference re-inverts
+    // from the wrapped rows on the next accept, complex_slater_determinant.cpp:
+    // 222-253; every gauge-invariant quantity is unchanged).
+    __device__ void regauge_column(int b, int j, double shift0, double shift1) {
+        double s, c;
+        sincos(M.orb_k[b][0][0] * shift0 + M.orb_k[b][0][1] * shift1, &s, &c);
+        const InvRef<T, CPLX> A = inv(b);
+        const C2<T> ph{static_cast<T>(c), static_cast<T>(s)};
+        for (int m = lane; m < M.blk_n[b]; m += W) A.set(m, j, cmul(A.get(m, j), ph));
+        __syncthreads();
+    }
+
+    // ── one Metropolis / drift-diffusion move of particle p ────────────────
+    // mode 0: sweep (decide from the random numbers); 1: ratio only (per-call
+    // path, writes DevCall outputs); 2: commit where call.mask is set.
+    __device__ bool move(int p, double nx0, double nx1, double uni, int mode, const DevCall* call,
+                         double tau, bool dmc, double drift_scale, const double* gold,
+                         double& rr_acc) {
+        const double x0 = sh.pos[2 * p], x1 = sh.pos[2 * p + 1];
+        const bool derivs = dmc || mode == 1;
+        JOut J;
+        jastrow_move(p, x0, x1, nx0, nx1, derivs, J);
+        double lr = J.dlog, phase = 0.0;
+        double ng[2] = {J.ng[0], J.ng[1]}, nl[2] = {J.nl[0], J.nl[1]};
+        double npg[2] = {0, 0}, npl[2] = {0, 0};
+        const int b = M.block_of[p];
+        DOut D{};
+        if (b >= 0) {
+            const int j = p - M.blk_first[b];
+            det_ratio(b, j, x0, x1, nx0, nx1, derivs, false, D);
+            const double absR = sqrt(D.R[0] * D.R[0] + D.R[1] * D.R[1]);
+            lr += log(fmax(absR, 1e-300));
+            if (CPLX) {
+                phase += atan2(D.R[1], D.R[0]);
+            } else if (D.R[0] < 0.0) {
+                phase += 3.14159265358979323846;
+            }
+            if (derivs) {
+                // G' = (sum A grad phi)/R, L' = (sum A lap phi)/R - G'^2
+                double ir = 0.0, ii = 0.0;
+                if (absR > 0.0) { ir = D.R[0] / (absR * absR); ii = -D.R[1] / (absR * absR); }
+                for (int d = 0; d < 2; ++d) {
+                    const double gr = D.ng[d][0] * ir - D.ng[d][1] * ii;
+                    const double gi = D.ng[d][0] * ii + D.ng[d][1] * ir;
+                    const double lr_ = D.nl[d][0] * ir - D.nl[d][1] * ii;
+                    const double li = D.nl[d][0] * ii + D.nl[d][1] * ir;
+                    ng[d] += gr;
+                    nl[d] += lr_ - (gr * gr - gi * gi);
+                    npg[d] += gi;
+                    npl[d] += li - 2.0 * gr * gi;
+                }
+            }
+        }
+        if (mode == 1) {
+            if (lane == 0) {
+                const int nw = M.nw;
+                call->log_ratio[w] = lr;
+                call->phase[w] = phase;
+                for (int d = 0; d < 2; ++d) {
+                    call->new_grad[d * nw + w] = ng[d];
+                    call->new_lap[d * nw + w] = nl[d];
+                    if (call->new_pgrad) {
+                        call->new_pgrad[d * nw + w] = npg[d];
+                        call->new_plap[d * nw + w] = npl[d];
+                    }
+                }
+            }
+            return false;
+        }
+        bool acc;
+        if (mode == 2) {
+            acc = call->mask[w] != 0;
+        } else if (!dmc) {
+            acc = uni < exp(2.0 * lr);
+        } else {
+            const double tm = tau * M.inv_mass[p];
+            if (!CPLX && M.n_blocks > 0 && cos(phase) < 2.220446049250313e-16) {
+                acc = false;
+            } else {
+                const double vn = ng[0] * ng[0] + ng[1] * ng[1];
+                const double sb = vn < 1e-10 ? tm : (-1.0 + sqrt(1.0 + 2.0 * tm * vn)) / vn;
+                double lc = 0.0;
+                for (int d = 0; d < 2; ++d) {
+                    const double delta = (d == 0 ? nx0 - x0 : nx1 - x1);
+                    const double df = drift_scale * gold[d];
+                    const double db = sb * ng[d];
+                    lc += tm <= 0.0 ? 0.0 : (-delta * (df + db) + 0.5 * (df * df - db * db)) / tm;
+                }
+                const double la = 2.0 * lr + lc;
+                acc = la >= 0.0 || uni < exp(la);
+            }
+            const double d0 = nx0 - x0, d1 = nx1 - x1;
+            const double rr = tm * d0 * d0 + tm * d1 * d1;
+            rr_acc += acc ? rr : 0.0;
+        }
+        if (!acc) return false;
+        log_j += J.dlog;
+        jastrow_commit(p, x0, x1, nx0, nx1);
+        __syncthreads();
+        if (lane == 0) {
+            double w0 = nx0, w1 = nx1;
+            wrap(M.cell, w0, w1);
+            sh.pos[2 * p] = w0;
+            sh.pos[2 * p + 1] = w1;
+        }
+        __syncthreads();
+        if (b >= 0) {
+            const bool rebuilt = det_commit(b, p - M.blk_first[b], D);
+            const double s0 = nx0 - sh.pos[2 * p], s1 = nx1 - sh.pos[2 * p + 1];
+            if (CPLX && !rebuilt && (s0 != 0.0 || s1 != 0.0)) regauge_column(b, p - M.blk_first[b], s0, s1);
+        }
+        return true;
+    }
+
+    // Old gradient of p (DMC drift), all factors.
+    __device__ void grad_of(int p, double g[2]) {
+        const double x0 = sh.pos[2 * p], x1 = sh.pos[2 * p + 1];
+        JOut J;
+        jastrow_move(p, x0, x1, x0, x1, true, J);
+        g[0] = J.og[0];
+        g[1] = J.og[1];
+        const int b = M.block_of[p];
+        if (b >= 0) {
+            DOut D{};
+            det_ratio(b, p - M.blk_first[b], x0, x1, x0, x1, true, true, D);
+            // at r' = r the ratio is 1 up to rounding; G = sum A grad phi (old)
+            for (int d = 0; d < 2; ++d) g[d] += D.og[d][0];
+        }
+    }
+};
+
+// ── kernels ────────────────────────────────────────────────────────────────
+
+// Full evaluation: Jastrow log from all pairs, k-space densities, determinant
+// inverses by Gauss-Jordan (TrialWavefunction::evaluate_batch,
+// trial_wavefunction.cpp:19-35).
+template <typename T, bool CPLX>
+__global__ __launch_bounds__(64) void k_evaluate(const DevModel* __restrict__ Mp, DevState S) {
+    const DevModel& M = *Mp;
+    __shared__ Shared sh;
+    const int w = blockIdx.x, lane = threadIdx.x;
+    Walker<T, CPLX> wk(M, S, sh, w);
+    const double* p = S.pos + static_cast<size_t>(w) * M.np * 2;
+    for (int i = lane; i < 2 * M.np; i += W) sh.pos[i] = p[i];
+    if (lane == 0) S.status[w] = 0;
+    __syncthreads();
+    double lj = 0.0;
+    for (int i = lane; i < M.np; i += W) {
+        const int gi = M.group_of[i];
+        for (int k = i + 1; k < M.np; ++k) {
+            const int gk = M.group_of[k];
+            const int nf = M.pair_list_n[gi][gk];
+            if (nf == 0) continue;
+            double d0 = sh.pos[2 * i] - sh.pos[2 * k], d1 = sh.pos[2 * i + 1] - sh.pos[2 * k + 1];
+            const double r = dist(M.cell, d0, d1);
+            for (int fi = 0; fi < nf; ++fi) {
+                double u, du, d2u;
+                radial_eval(M.pair[M.pair_list[gi][gk][fi]].rad, r, u, du, d2u);
+                lj += u;
+            }
+        }
+        if (M.has_nosanow && M.nos_alpha[i] != 0.0) {
+            double d0 = sh.pos[2 * i] - M.nos_site[i][0], d1 = sh.pos[2 * i + 1] - M.nos_site[i][1];
+            if (M.cell.periodic) min_image(M.cell, d0, d1);
+            lj += -M.nos_alpha[i] * d0 * d0 - M.nos_alpha[i] * d1 * d1;
+        }
+    }
+    // k-space densities and their contribution
+    double lk = 0.0;
+    for (int k = lane; k < M.nkj; k += W) {
+        for (int g = 0; g < M.ng; ++g) {
+            double rr = 0.0, ri = 0.0;
+            if (M.kj_active[g])
+                for (int i = M.g_first[g]; i < M.g_last[g]; ++i) {
+                    double s, c;
+                    sincos(M.kjx[k] * sh.pos[2 * i] + M.kjy[k] * sh.pos[2 * i + 1], &s, &c);
+                    rr += c;
+                    ri += s;
+                }
+            sh.rho[(g * M.nkj + k) * 2] = rr;
+            sh.rho[(g * M.nkj + k) * 2 + 1] = ri;
+        }
+        for (int g = 0; g < M.ng; ++g)
+            for (int h = g; h < M.ng; ++h) {
+                const double wgh = M.kjw[g][h][k];
+                if (wgh == 0.0) continue;
+                const double ar = sh.rho[(g * M.nkj + k) * 2], ai = sh.rho[(g * M.nkj + k) * 2 + 1];
+                const double br = sh.rho[(h * M.nkj + k) * 2], bi = sh.rho[(h * M.nkj + k) * 2 + 1];
+                lk += (g == h ? 1.0 : 2.0) * wgh * (ar * br + ai * bi);
+            }
+    }
+    wk.log_j = wsum(lj) + wsum(lk);
+    __syncthreads();
+    for (int b = 0; b < M.n_blocks; ++b) {
+        const double ld = wk.rebuild(b);
+        wk.log_b[b] = fmax(ld, log(1e-300));
+    }
+    wk.store();
+}
+
+// Fused sweeps: every particle of the walker, n_sweeps times.
+template <typename T, bool CPLX>
+__global__ __launch_bounds__(64) void k_sweep(const DevModel* __restrict__ Mp, DevState S,
+                                              SweepParams P) {
+    const DevModel& M = *Mp;
+    __shared__ Shared sh;
+    const int w = blockIdx.x, lane = threadIdx.x;
+    Walker<T, CPLX> wk(M, S, sh, w);
+    wk.load();
+    Rng rng;
+    if (!P.tape)
+        for (int i = 0; i < 4; ++i) rng.s[i] = S.rng[w * 4 + i];
+    long long nacc = 0;
+    double rr_p = 0.0, rr_a = 0.0;
+    const int np = M.np;
+    for (int s = 0; s < P.n_sweeps; ++s) {
+        for (int p = 0; p < np; ++p) {
+            double xi0, xi1, uni;
+            if (P.tape) {
+                const double* row = P.tape + (static_cast<size_t>(s) * np + p) * 3 * P.tape_stride;
+                xi0 = row[w];
+                xi1 = row[P.tape_stride + w];
+                uni = row[2 * P.tape_stride + w];
+            } else {
+                xi0 = rng.gaussian();
+                xi1 = rng.gaussian();
+                uni = rng.uniform();
+            }
+            const double tm = P.tau * M.inv_mass[p];
+            const double sigma = sqrt(tm);
+            const double x0 = wk.sh.pos[2 * p], x1 = wk.sh.pos[2 * p + 1];
+            double gold[2] = {0.0, 0.0}, scale = 0.0;
+            if (P.dmc) {
+                wk.grad_of(p, gold);
+                const double v2 = gold[0] * gold[0] + gold[1] * gold[1];
+                scale = v2 < 1e-10 ? tm : (-1.0 + sqrt(1.0 + 2.0 * tm * v2)) / v2;
+            }
+            const double nx0 = x0 + scale * gold[0] + sigma * xi0;
+            const double nx1 = x1 + scale * gold[1] + sigma * xi1;
+            if (P.dmc) {
+                const double d0 = nx0 - x0, d1 = nx1 - x1;
+                rr_p += tm * d0 * d0 + tm * d1 * d1;
+            }
+            if (wk.move(p, nx0, nx1, uni, 0, nullptr, P.tau, P.dmc != 0, scale, gold, rr_a)) ++nacc;
+        }
+    }
+    wk.store();
+    if (lane == 0) {
+        S.n_accepted[w] += nacc;
+        if (P.dmc) {
+            S.rr[w * 2] += rr_p;
+            S.rr[w * 2 + 1] += rr_a;
+        }
+        if (!P.tape)
+            for (int i = 0; i < 4; ++i) S.rng[w * 4 + i] = rng.s[i];
+    }
+}
+
+// Per-call path: ratio of particle p at call.new_pos (mode 1) or commit of
+// the masked walkers (mode 2).
+template <typename T, bool CPLX>
+__global__ __launch_bounds__(64) void k_call(const DevModel* __restrict__ Mp, DevState S, DevCall call,
+                                             int p, int mode) {
+    const DevModel& M = *Mp;
+    __shared__ Shared sh;
+    const int w = blockIdx.x;
+    Walker<T, CPLX> wk(M, S, sh, w);
+    wk.load();
+    double rr = 0.0;
+    wk.move(p, call.new_pos[w], call.new_pos[M.nw + w], 0.0, mode, &call, 0.0, false, 0.0, nullptr, rr);
+    if (mode == 2) wk.store();
+}
+
+// Local energy: every particle's gradient and Laplacian from the current
+// state, kinetic (+ fixed-phase terms), Ewald, open Coulomb.
+template <typename T, bool CPLX>
+__global__ __launch_bounds__(64) void k_measure(const DevModel* __restrict__ Mp, DevState S) {
+    const DevModel& M = *Mp;
+    __shared__ Shared sh;
+    const int w = blockIdx.x, lane = threadIdx.x;
+    Walker<T, CPLX> wk(M, S, sh, w);
+    wk.load();
+    const int np = M.np, nw = M.nw;
+    double kin = 0.0, kin_im = 0.0;
+    for (int i = lane; i < np; i += W) {
+        const int gi = M.group_of[i];
+        const double x0 = sh.pos[2 * i], x1 = sh.pos[2 * i + 1];
+        double g[2] = {0, 0}, l[2] = {0, 0}, pg[2] = {0, 0}, pl[2] = {0, 0};
+        for (int k = 0; k < np; ++k) {
+            if (k == i) continue;
+            const int nf = M.pair_list_n[gi][M.group_of[k]];
+            if (nf == 0) continue;
+            double d0 = x0 - sh.pos[2 * k], d1 = x1 - sh.pos[2 * k + 1];
+            const double r = dist(M.cell, d0, d1);
+            for (int fi = 0; fi < nf; ++fi) {
+                double u, du, d2u, g0, g1, l0, l1;
+                radial_eval(M.pair[M.pair_list[gi][M.group_of[k]][fi]].rad, r, u, du, d2u);
+                pair_terms(du, d2u, r, d0, d1, g0, g1, l0, l1);
+                g[0] += g0; g[1] += g1; l[0] += l0; l[1] += l1;
+            }
+        }
+        if (M.nkj > 0 && M.kj_active[gi])
+            for (int k = 0; k < M.nkj; ++k) {
+                double s, c;
+                sincos(M.kjx[k] * x0 + M.kjy[k] * x1, &s, &c);
+                double gk = 0.0, lk = 0.0;
+                for (int h = 0; h < M.ng; ++h) {
+                    const double wgh = M.kjw[gi][h][k];
+                    if (wgh == 0.0 || !M.kj_active[h]) continue;
+                    const double rr = sh.rho[(h * M.nkj + k) * 2], ri = sh.rho[(h * M.nkj + k) * 2 + 1];
+                    gk += 2.0 * wgh * (c * ri - s * rr);
+                    lk += 2.0 * wgh * ((h == gi ? 1.0 : 0.0) - (c * rr + s * ri));
+                }
+                g[0] += M.kjx[k] * gk; g[1] += M.kjy[k] * gk;
+                l[0] += M.kjx[k] * M.kjx[k] * lk; l[1] += M.kjy[k] * M.kjy[k] * lk;
+            }
+        if (M.has_nosanow && M.nos_alpha[i] != 0.0) {
+            const double a = M.nos_alpha[i];
+            double d0 = x0 - M.nos_site[i][0], d1 = x1 - M.nos_site[i][1];
+            if (M.cell.periodic) min_image(M.cell, d0, d1);
+            g[0] += -2.0 * a * d0; g[1] += -2.0 * a * d1;
+            l[0] += -2.0 * a; l[1] += -2.0 * a;
+        }
+        const int b = M.block_of[i];
+        if (b >= 0) {
+            const InvRef<T, CPLX> A = wk.inv(b);
+            const int n = M.blk_n[b], kr = i - M.blk_first[b];
+            double G[2][2] = {{0, 0}, {0, 0}}, L[2][2] = {{0, 0}, {0, 0}};
+            for (int m = 0; m < n; ++m) {
+                const C2<T> a = A.get(m, kr);
+                const double ar = a.re, ai = a.im;
+                Orb o;
+                orbital(M, b, m, x0, x1, o);
+                for (int d = 0; d < 2; ++d) {
+                    G[d][0] += ar * o.gr[d] - ai * o.gi[d];
+                    G[d][1] += ar * o.gi[d] + ai * o.gr[d];
+                    L[d][0] += ar * o.lr[d] - ai * o.li[d];
+                    L[d][1] += ar * o.li[d] + ai * o.lr[d];
+                }
+            }
+            for (int d = 0; d < 2; ++d) {
+                g[d] += G[d][0];
+                l[d] += L[d][0] - (G[d][0] * G[d][0] - G[d][1] * G[d][1]);
+                pg[d] += G[d][1];
+                pl[d] += L[d][1] - 2.0 * G[d][0] * G[d][1];
+            }
+        }
+        const double im = M.inv_mass[i];
+        for (int d = 0; d < 2; ++d) {
+            kin += -0.5 * im * (l[d] + g[d] * g[d]);
+            if (CPLX) {
+                kin += 0.5 * im * pg[d] * pg[d];
+                kin_im += -0.5 * im * (pl[d] + 2.0 * g[d] * pg[d]);
+            }
+            if (S.grad_out) {
+                S.grad_out[(i * 2 + d) * nw + w] = g[d];
+                S.lap_out[(i * 2 + d) * nw + w] = l[d];
+                S.pgrad_out[(i * 2 + d) * nw + w] = pg[d];
+                S.plap_out[(i * 2 + d) * nw + w] = pl[d];
+            }
+        }
+    }
+    double pot = 0.0;
+    if (M.ewald) {
+        for (int i = lane; i < np; i += W)
+            for (int k = i + 1; k < np; ++k) {
+                const double coeff = M.charge[i] * M.charge[k];
+                if (coeff == 0.0) continue;
+                double d0 = sh.pos[2 * i] - sh.pos[2 * k], d1 = sh.pos[2 * i + 1] - sh.pos[2 * k + 1];
+                const double r = dist(M.cell, d0, d1);
+                if (r < M.ew_rcut) {
+                    double f, df, d2f;
+                    spline_eval(M.ew_spline, M.ew_knots, M.ew_rcut, M.ew_inv_delta, r, f, df, d2f);
+                    pot += coeff * (M.ew_inv_eps / r - f);
+                }
+            }
+        for (int k = lane; k < M.nke; k += W) {
+            double rr = 0.0, ri = 0.0;
+            for (int i = 0; i < np; ++i) {
+                const double q = M.charge[i];
+                if (q == 0.0) continue;
+                double s, c;
+                sincos(M.kex[k] * sh.pos[2 * i] + M.key[k] * sh.pos[2 * i + 1], &s, &c);
+                rr += q * c;
+                ri += q * s;
+            }
+            pot += M.kew[k] * (rr * rr + ri * ri);
+        }
+    }
+    for (int cp = 0; cp < M.n_coul; ++cp) {
+        const int ga = M.coul_ga[cp], gb = M.coul_gb[cp];
+        const bool intra = ga == gb;
+        for (int i = M.g_first[ga] + lane; i < M.g_last[ga]; i += W)
+            for (int k = intra ? i + 1 : M.g_first[gb]; k < M.g_last[gb]; ++k) {
+                double d0 = sh.pos[2 * i] - sh.pos[2 * k], d1 = sh.pos[2 * i + 1] - sh.pos[2 * k + 1];
+                const double r = sqrt(d0 * d0 + d1 * d1);
+                pot += M.coul_coeff[cp] / r;
+            }
+    }
+    const double e = wsum(kin) + wsum(pot) + (M.ewald ? M.ew_const : 0.0);
+    const double ei = wsum(kin_im);
+    if (lane == 0) {
+        S.e_loc[w] = e;
+        S.e_loc_im[w] = ei;
+    }
+}
+
+// Branching gather: new walker j takes old walker src[j]'s record.
+__global__ void k_gather(const double* __restrict__ src_d, double* __restrict__ dst_d, int stride,
+                         const int* __restrict__ src, int n_new) {
+    const int j = blockIdx.x;
+    if (j >= n_new) return;
+    const double* a = src_d + static_cast<size_t>(src[j]) * stride;
+    double* b = dst_d + static_cast<size_t>(j) * stride;
+    for (int i = threadIdx.x; i < stride; i += blockDim.x) b[i] = a[i];
+}
+
+}  // namespace
+
+// ── host launch wrappers ───────────────────────────────────────────────────
+int set_ziggurat(const uint64_t* k, const double* w, const double* f, double r) {
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_zig_k), k, 256 * sizeof(uint64_t)) != hipSuccess) return 1;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_zig_w), w, 256 * sizeof(double)) != hipSuccess) return 1;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_zig_f), f, 256 * sizeof(double)) != hipSuccess) return 1;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_zig_r), &r, sizeof(double)) != hipSuccess) return 1;
+    return 0;
+}
+
+#define DISPATCH(FN, ...)                                                              \
+    do {                                                                               \
+        if (mixed) {                                                                   \
+            if (cplx) hipLaunchKernelGGL((FN<float, true>), __VA_ARGS__);              \
+            else hipLaunchKernelGGL((FN<float, false>), __VA_ARGS__);                  \
+        } else {                                                                       \
+            if (cplx) hipLaunchKernelGGL((FN<double, true>), __VA_ARGS__);             \
+            else hipLaunchKernelGGL((FN<double, false>), __VA_ARGS__);                 \
+        }                                                                              \
+    } while (0)
+
+void launch_evaluate(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, hipStream_t st) {
+    DISPATCH(k_evaluate, dim3(nw), dim3(64), 0, st, m, s);
+}
+void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
+                  bool cplx, hipStream_t st) {
+    DISPATCH(k_sweep, dim3(nw), dim3(64), 0, st, m, s, p);
+}
+void launch_call(const DevModel* m, const DevState& s, const DevCall& c, int p, int mode, int nw,
+                 bool mixed, bool cplx, hipStream_t st) {
+    DISPATCH(k_call, dim3(nw), dim3(64), 0, st, m, s, c, p, mode);
+}
+void launch_measure(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, hipStream_t st) {
+    DISPATCH(k_measure, dim3(nw), dim3(64), 0, st, m, s);
+}
+void launch_gather(const double* src_d, double* dst_d, int stride, const int* src, int n_new,
+                   hipStream_t st) {
+    hipLaunchKernelGGL(k_gather, dim3(n_new), dim3(256), 0, st, src_d, dst_d, stride, src, n_new);
+}
+
+}  // namespace alea_amd

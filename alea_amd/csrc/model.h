@@ -1,0 +1,145 @@
+// Device-side description of one system (cell, groups, trial wavefunction,
+// Hamiltonian) and the per-walker state layout.  Shared by the host C-ABI
+// (capi.hip) and the kernels (kernels.hip).
+//
+// Walker-major layout in HBM: every walker owns a contiguous record in each
+// state array, so one wavefront streams its own walker's state with coalesced
+// loads.  (The reference's CPU layout is walker-fastest SoA; the C-ABI
+// converts at upload/download.)
+#pragma once
+#include <cstdint>
+
+namespace alea_amd {
+
+constexpr int kMaxParticles = 256;
+constexpr int kMaxGroups = 4;
+constexpr int kMaxPair = 8;        // pair-Jastrow factors
+constexpr int kMaxCoef = 12;       // Padé / Drummond coefficients
+constexpr int kMaxKnots = 64;      // breakup spline knots
+constexpr int kMaxKJ = 64;         // k-space Jastrow vectors
+constexpr int kMaxKE = 192;        // Ewald vectors
+constexpr int kMaxBlocks = 2;      // determinant blocks
+constexpr int kMaxOrb = 128;       // orbitals per block
+constexpr int kMaxCoulomb = 8;
+
+struct DevCell {
+    int periodic = 0, ortho = 0;
+    double box[2] = {0, 0}, inv_box[2] = {0, 0};
+    double a[4] = {0, 0, 0, 0}, ainv[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+    double cand[18] = {};
+    double ws = 0.0;
+};
+
+// Radial pair function u(r): Padé P/Q (pade_jastrow.cpp:26-56) optionally
+// reduced to its image-sum short-range part (periodic_pair_sum.cpp:66-83), or
+// Drummond (drummond_jastrow.cpp:27-80).
+struct DevRadial {
+    int kind = 0;   // 0 Padé, 1 Drummond
+    int num = 0, den = 0, image = 0;
+    double c[kMaxCoef] = {};
+    double r_cut = 0, inv_delta = 0, u_inf = 0;
+    int n_knots = 0;
+    int C = 3;
+    double L = 0, alpha1 = 0;
+    double spline[3 * kMaxKnots] = {};
+};
+
+struct DevPair {
+    int ga = 0, gb = 0;
+    DevRadial rad;
+};
+
+enum OrbitalKind { kPlaneWave = 0, kGaussian = 1, kTwisted = 2 };
+
+struct DevModel {
+    int nw = 0, np = 0, ng = 0;
+    DevCell cell;
+    int group_of[kMaxParticles] = {};
+    double inv_mass[kMaxParticles] = {};
+    double charge[kMaxParticles] = {};
+    int g_first[kMaxGroups] = {}, g_last[kMaxGroups] = {};
+
+    // pair Jastrow: factors and, per (group of p, group of partner), the list
+    int n_pair = 0;
+    DevPair pair[kMaxPair];
+    int pair_list_n[kMaxGroups][kMaxGroups] = {};
+    int pair_list[kMaxGroups][kMaxGroups][kMaxPair] = {};
+
+    // merged k-space Jastrow: U = sum_k sum_{g<=h} w_gh(k) T_gh
+    int nkj = 0;
+    double kjx[kMaxKJ] = {}, kjy[kMaxKJ] = {};
+    double kjw[kMaxGroups][kMaxGroups][kMaxKJ] = {};   // symmetric in (g,h)
+    int kj_active[kMaxGroups] = {};                   // group has a density
+
+    // Nosanow one-body: per particle alpha (0: none) and site
+    double nos_alpha[kMaxParticles] = {};
+    double nos_site[kMaxParticles][2] = {};
+    int has_nosanow = 0;
+
+    // Slater determinant blocks
+    int n_blocks = 0, det_complex = 0;
+    int blk_kind[kMaxBlocks] = {}, blk_first[kMaxBlocks] = {}, blk_n[kMaxBlocks] = {};
+    double orb_k[kMaxBlocks][kMaxOrb][2] = {};   // plane wave k, twisted q, Gaussian centre
+    int orb_sin[kMaxBlocks][kMaxOrb] = {};
+    double orb_alpha[kMaxBlocks][kMaxOrb] = {};
+    int block_of[kMaxParticles] = {};            // -1: in no block
+
+    // Ewald (single Coulomb kernel)
+    int ewald = 0, nke = 0;
+    double kex[kMaxKE] = {}, key[kMaxKE] = {}, kew[kMaxKE] = {};
+    double ew_spline[3 * kMaxKnots] = {};
+    int ew_knots = 0;
+    double ew_rcut = 0, ew_inv_delta = 0, ew_const = 0, ew_inv_eps = 1.0;
+
+    // open-cell Coulomb pairs
+    int n_coul = 0;
+    int coul_ga[kMaxCoulomb] = {}, coul_gb[kMaxCoulomb] = {};
+    double coul_coeff[kMaxCoulomb] = {};
+};
+
+// Per-walker state arrays (device pointers).  Strides are per walker.
+struct DevState {
+    double* pos = nullptr;        // [w][np][2]
+    double* rho = nullptr;        // [w][ng][nkj][2]  k-space Jastrow densities
+    double* logs = nullptr;       // [w][4]: 0 Jastrow+one-body, 1.. determinant blocks
+    void* inv = nullptr;          // [w][blocks][Nb*Nb] (T, complex: 2 T per entry)
+    double* row_abs = nullptr;    // [w][blocks][Nb]  |D| row sums
+    double* cond = nullptr;       // [w][blocks]
+    int* updates = nullptr;       // [w][blocks]
+    double* work = nullptr;       // [w][2*Nb*Nb*cplx] Gauss-Jordan workspace (FP64)
+    uint64_t* rng = nullptr;      // [w][4]
+    double* e_loc = nullptr;      // [w]
+    double* e_loc_im = nullptr;   // [w]
+    double* rr = nullptr;         // [w][2] DMC rr_proposed, rr_accepted
+    double* grad_out = nullptr;   // optional [np][2][w] SoA outputs of the measure kernel
+    double* lap_out = nullptr;
+    double* pgrad_out = nullptr;
+    double* plap_out = nullptr;
+    int* status = nullptr;        // [w] nonzero: singular determinant
+    long long* n_accepted = nullptr;   // [w]
+    int inv_stride = 0;           // T elements per walker in inv
+    int work_stride = 0;          // doubles per walker in work
+    int nb_max = 0;
+};
+
+// Per-call (parity path) buffers, walker-fastest like the reference.
+struct DevCall {
+    const double* new_pos = nullptr;   // [2][nw]
+    const unsigned char* mask = nullptr;
+    double* log_ratio = nullptr;       // [nw]
+    double* phase = nullptr;           // [nw]
+    double* new_grad = nullptr;        // [2][nw]
+    double* new_lap = nullptr;
+    double* new_pgrad = nullptr;
+    double* new_plap = nullptr;
+};
+
+struct SweepParams {
+    int n_sweeps = 1;
+    double tau = 0.0;
+    const double* tape = nullptr;   // [sweep][np][3][stride] or null (device RNG)
+    int tape_stride = 0;
+    int dmc = 0;
+};
+
+}  // namespace alea_amd

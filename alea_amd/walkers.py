@@ -164,3 +164,13 @@ class Walkers:
             self.close()
         except Exception:
             pass
+
+
+def gpu_walkers(spec: Spec, nw: int, precision: int = FP64) -> Walkers:
+    """A walker batch on the GPU (libalea_amd.so)."""
+    from . import load
+    return Walkers(load(), spec, nw, precision)
+
+
+def seed(w: Walkers, seed_value: int, stream: int = 0x766d632d74687264):
+    w.api("seed", w.h, seed_value, stream)

@@ -1,0 +1,150 @@
+/* alea_amd: MI355X-native batched VMC/DMC walker sweep — the C-ABI.
+ *
+ * One handle is one walker batch (the reference's WalkerSet with its
+ * SimulationCell) plus its trial wavefunction and Hamiltonian, resident in
+ * HBM.  Plain pointers and sizes only; every call returns ALEA_OK or an
+ * error code, with the message in alea_last_error() (thread-local).  The
+ * error codes map to the reference's exception types:
+ *   ALEA_ERR_INVALID  std::invalid_argument (refused input)
+ *   ALEA_ERR_RUNTIME  std::runtime_error   (e.g. a singular determinant,
+ *                     slater_determinant.cpp:927-928)
+ *   ALEA_ERR_DEVICE   a HIP runtime failure
+ *
+ * Host arrays use the reference's walker-fastest SoA layouts
+ * (docs/architecture.md:141-156):
+ *   positions, gradients, laplacians   [p*dim*nw + d*nw + w]
+ *   proposed position, new_grad        [d*nw + w]
+ *   per-walker scalars                 [w]
+ * The device keeps walker-major records; conversion happens at these calls.
+ *
+ * Registration order: create, add_group (all groups), components, then
+ * set_positions, evaluate.  Components cannot be added after positions are set.
+ */
+#ifndef ALEA_AMD_H
+#define ALEA_AMD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALEA_OK 0
+#define ALEA_ERR_INVALID 1
+#define ALEA_ERR_RUNTIME 2
+#define ALEA_ERR_DEVICE 3
+
+/* Precision of the sweep. */
+#define ALEA_FP64 0   /* plain: everything in FP64 */
+#define ALEA_MIXED 1  /* determinant inverse, orbitals, ratios and rank-1 updates in FP32;
+                         log-determinant, Jastrow, rebuilds, weights, energies in FP64 */
+
+typedef struct alea_system alea_system;
+
+const char* alea_last_error(void);
+
+/* WalkerSet(nw, np, dim, cell) — src/population/walker_set.h:37;
+ * SimulationCell(dim, vectors) — src/population/simulation_cell.h:66.
+ * lattice: dim*dim primitive vectors, row-major, or NULL for an open cell.
+ * n_walkers is the capacity (DMC populations may shrink below it). */
+int alea_create(alea_system** out, int n_walkers, int n_particles, int dim,
+                const double* lattice, int precision);
+int alea_destroy(alea_system* s);
+
+/* ParticleGroup — src/population/particle_group.h:17 (isotropic mass). */
+int alea_add_group(alea_system* s, int first, int last, double charge, double inv_mass);
+
+/* PadeJastrow(num_order, den_order, group_a, group_b, params) —
+ * src/wavefunction/pade_jastrow.h:60; coeffs [a_1..a_N, b_1..b_M].
+ * n_k > 0 turns on the image sum (PadeJastrow::sum_over_images,
+ * pade_jastrow.h:69; PeriodicPairSum, periodic_pair_sum.h:45): the breakup's
+ * k set (n1, n2, kx, ky), weights, quintic spline (LongRangeSpline::scaled,
+ * 3 per knot), r_cut, 1/delta and u_inf. */
+int alea_add_pade_jastrow(alea_system* s, int group_a, int group_b, int num_order, int den_order,
+                          const double* coeffs, int n_k, const int* n1, const int* n2,
+                          const double* kx, const double* ky, const double* weights,
+                          int n_knots, const double* spline, double r_cut, double inv_delta,
+                          double u_inf);
+
+/* DrummondJastrow(C, gamma, group_a, group_b, params) —
+ * src/wavefunction/drummond_jastrow.h:77; coeffs [alpha_0, alpha_2..alpha_N]. */
+int alea_add_drummond_jastrow(alea_system* s, int group_a, int group_b, int C, double gamma,
+                              double L, int n_coeffs, const double* coeffs);
+
+/* NosanowJastrow(dim, first, sites, cell, {alpha}) —
+ * src/wavefunction/nosanow_jastrow.h:55; sites [n*2], one per group particle. */
+int alea_add_nosanow_jastrow(alea_system* s, int group, double alpha, const double* sites);
+
+/* One determinant block [first, last):
+ * SlaterDeterminant + PlaneWaveOrbitalSet — src/wavefunction/orbital_set.h:427
+ *   (kvecs [n*2], is_sin [n]);
+ * SlaterDeterminant + GaussianOrbitalSet — src/wavefunction/orbital_set.h:164
+ *   (centers [n*2], alphas [n]);
+ * ComplexSlaterDeterminant + TwistedPlaneWaveOrbitalSet —
+ *   src/wavefunction/complex_orbital_set.h:91 (q = k + theta, [n*2]). */
+int alea_add_slater_plane_wave(alea_system* s, int first, int last, const double* kvecs,
+                               const unsigned char* is_sin);
+int alea_add_slater_gaussian(alea_system* s, int first, int last, const double* centers,
+                             const double* alphas);
+int alea_add_slater_twisted(alea_system* s, int first, int last, const double* q);
+
+/* EwaldTerm(groups, epsilon, cell, tol) — src/hamiltonian/ewald_term.h:70,
+ * with the EwaldSetup of numerics::build_ewald (ewald_breakup.h:186):
+ * k set, weights, long-range spline, r_cut, v_long_zero, sr_k0.  Charges come
+ * from the groups. */
+int alea_add_ewald(alea_system* s, int n_k, const int* n1, const int* n2, const double* kx,
+                   const double* ky, const double* weights, int n_knots, const double* spline,
+                   double r_cut, double inv_delta, double v_long_zero, double sr_k0);
+
+/* CoulombTerm(group_a, group_b, epsilon) — src/hamiltonian/coulomb.h:28 (open cell). */
+int alea_add_coulomb(alea_system* s, int group_a, int group_b, double epsilon);
+
+/* WalkerSet::load_all (walker_set.cpp:51-70): positions are wrapped into the cell. */
+int alea_set_positions(alea_system* s, const double* pos);
+int alea_get_positions(alea_system* s, double* pos);
+
+/* TrialWavefunction::evaluate_batch — src/wavefunction/trial_wavefunction.cpp:19. */
+int alea_evaluate(alea_system* s);
+/* WalkerSet::log_psi (walker_set.h:88): log|psi| per walker. */
+int alea_get_log_psi(alea_system* s, double* out);
+/* refresh_derivatives + WalkerSet::gradients/laplacians (and the phase planes
+ * of a complex trial, walker_set.h:98-110). */
+int alea_get_gradients(alea_system* s, double* grad, double* lap);
+int alea_get_phase_gradients(alea_system* s, double* pgrad, double* plap);
+/* refresh_derivatives + Hamiltonian::evaluate_batch — src/hamiltonian/hamiltonian.cpp:18;
+ * e_im (may be NULL) is Im E_L of a complex trial. */
+int alea_local_energy(alea_system* s, double* e_re, double* e_im);
+
+/* Per-call path: ParticleBatch::propose_move_batch + TrialWavefunction::ratio_batch
+ * (particle_batch.cpp:43-57, trial_wavefunction.cpp:46-78) for particle p at
+ * new_pos [dim*nw]; outputs log|ratio|, its phase, and the new gradient and
+ * Laplacian of p.  Then alea_accept commits the same proposal where mask[w] != 0
+ * (TrialWavefunction::apply_move_batch + WalkerSet::apply_move, vmc.cpp:147-152). */
+int alea_ratio(alea_system* s, int p, const double* new_pos, double* log_ratio, double* phase,
+               double* new_grad, double* new_lap);
+int alea_accept(alea_system* s, int p, const unsigned char* mask);
+
+/* Fused sweeps: VMC::sweep / sweep_walkers (vmc.cpp:69-155) with symmetric
+ * Metropolis proposals of width sqrt(tau/m).  tape: [sweep][np][dim+1][nw]
+ * random numbers in draw_sweep_randoms order (sweep_blocking.h:39-48), or NULL
+ * for the per-walker device streams seeded by alea_seed. */
+int alea_vmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape,
+                    long long* n_accepted);
+/* DMC::sweep_walkers (dmc.cpp:203-337): Umrigar-scaled drift, node gate for a
+ * real trial, Metropolis-Hastings with the drift Green's function; rr_proposed
+ * and rr_accepted [nw] are accumulated (may be NULL).  tape rows have stride
+ * tape_stride >= nw (the reference strides by the population capacity). */
+int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape, int tape_stride,
+                    double* rr_proposed, double* rr_accepted, long long* n_accepted);
+
+/* Per-walker xoshiro256++ streams: walker w gets stream_seed(seed, stream, w)
+ * (random.h:52-56) expanded by splitmix64 (xoshiro_random.h:29-32). */
+int alea_seed(alea_system* s, uint64_t seed, uint64_t stream);
+
+/* WalkerSet::apply_permutation + TrialWavefunction::permute_caches
+ * (walker_set.cpp:109-178): new walker j takes old walker src[j]'s state;
+ * new_nw <= capacity. */
+int alea_permute(alea_system* s, int new_nw, const int* src);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
