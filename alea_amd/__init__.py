@@ -31,5 +31,11 @@ def load() -> CApi:
         fn.restype = C.c_int
         fn.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         api.fns["seed"] = fn
+        for name, args in (("synchronize", [C.c_void_p]),
+                           ("last_sweep_ms", [C.c_void_p, C.POINTER(C.c_double)])):
+            f = getattr(lib, "alea_" + name)
+            f.restype = C.c_int
+            f.argtypes = args
+            api.fns[name] = f
         _api = api
     return _api

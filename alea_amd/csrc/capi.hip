@@ -155,7 +155,8 @@ struct alea_system {
     DevBuf<unsigned char> call_mask;
     DevBuf<double> grad_buf;       // 4 * np * 2 * nw
     DevBuf<double> tape;
-    std::vector<int> pade_klist_n1, pade_klist_n2;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;   // brackets the last sweep launch
+    bool timed = false;
 
     bool mixed() const { return precision == ALEA_MIXED; }
     size_t tsize() const { return mixed() ? 4 : 8; }
@@ -397,6 +398,8 @@ int alea_create(alea_system** out, int n_walkers, int n_particles, int dim, cons
 int alea_destroy(alea_system* s) {
     if (!s) return ALEA_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->dmodel) (void)hipFree(s->dmodel);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -808,8 +811,15 @@ static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* t
     }
     hip_check(hipMemsetAsync(s->nacc.p, 0, s->nw * sizeof(long long), s->stream), "memset");
     s->set_model_nw(s->nw);
+    if (!s->ev0) {
+        hip_check(hipEventCreate(&s->ev0), "event");
+        hip_check(hipEventCreate(&s->ev1), "event");
+    }
+    hip_check(hipEventRecord(s->ev0, s->stream), "event record");
     launch_sweep(s->dmodel, s->st, P, s->nw, s->mixed(), s->cplx, s->stream);
     hip_check(hipGetLastError(), "launch sweep");
+    hip_check(hipEventRecord(s->ev1, s->stream), "event record");
+    s->timed = true;
     if (n_accepted) {
         s->sync();
         std::vector<long long> a(s->nw);
@@ -848,6 +858,24 @@ int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape
             if (rr_proposed) rr_proposed[w] = rr[2 * w];
             if (rr_accepted) rr_accepted[w] = rr[2 * w + 1];
         }
+    });
+}
+
+int alea_synchronize(alea_system* s) {
+    return guarded([&] {
+        Sys(s);
+        s->sync();
+    });
+}
+
+int alea_last_sweep_ms(alea_system* s, double* ms) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->timed) throw InvalidArg("no sweep has been launched");
+        hip_check(hipEventSynchronize(s->ev1), "event sync");
+        float t = 0.0f;
+        hip_check(hipEventElapsedTime(&t, s->ev0, s->ev1), "event elapsed");
+        *ms = t;
     });
 }
 

@@ -1,0 +1,185 @@
+"""Headline benchmark: walker-moves/s of the batched VMC sweep on the N=122
+twisted 2-D electron gas, mixed precision vs plain FP64, on one MI355X.
+
+Workload (the north-star configuration that fits one GPU):
+  2-D paramagnetic HEG, N = 122 (61 up, 61 down), rs = 30, square cell
+  L = sqrt(pi N) rs; complex plane-wave orbitals at a twist drawn uniformly in
+  the first Brillouin zone (seed 3); image-summed Padé Jastrow
+  u = a r/(1 + r/(2 rs)) with the 2-D Kato cusps (a = 1/3 parallel, 1
+  antiparallel); 2-D Ewald; 65,536 walkers; single-particle Metropolis with
+  step 0.5 rs and per-walker xoshiro256++ streams.
+A step is one sweep: every particle of every walker proposed, ratioed,
+accepted or rejected, committed.  One walker-move = one proposed move.
+
+Prints ONE JSON line (the bench contract).  value = walker-moves/s of the mixed
+form; the FP64 form is reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+N_UP = N_DN = 61
+RS = 30.0
+TWIST_SEED = 3
+POS_SEED = 1
+
+
+def workload_spec():
+    from alea_amd.system import heg_spec
+    twist = tuple(np.random.default_rng(TWIST_SEED).uniform(-0.5, 0.5, 2))
+    return heg_spec(N_UP, N_DN, RS, twist=twist), twist
+
+
+def bytes_per_move(word, acceptance, spec):
+    """Algorithmic HBM bytes of one walker-move (the reference's state traffic):
+    s*[N*dim + N + a*2N + c*Nb + a*2c*Nb^2 + F_J*2*n_kJ*(1+a) + 2*N*dim*(1+a)]
+    with c = 2 (complex), F_J = 2 k-space factors touching the moved particle."""
+    N, dim, c = spec.n_particles, 2, 2
+    nb = N_UP
+    n_kj = len(spec.pade[0]["image"]["w"])
+    a = acceptance
+    words = (N * dim + N + a * 2 * N + c * nb + a * 2 * c * nb * nb
+             + 2 * 2 * n_kj * (1 + a) + 2 * N * dim * (1 + a))
+    return word * words
+
+
+def run_gpu(precision, nw, steps, warmup, tau):
+    from alea_amd.system import uniform_positions
+    from alea_amd.walkers import gpu_walkers, seed
+    import ctypes as C
+    spec, _ = workload_spec()
+    w = gpu_walkers(spec, nw, precision)
+    w.set_positions(uniform_positions(spec, nw, np.random.default_rng(POS_SEED)))
+    w.evaluate()
+    seed(w, 42 + precision)
+    api = w.api
+    ms = C.c_double()
+    for _ in range(warmup):
+        w.vmc_sweeps(1, tau)
+    api("synchronize", w.h)
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        w.vmc_sweeps(1, tau)
+        api("last_sweep_ms", w.h, C.byref(ms))   # waits for this sweep
+        kernel_ms.append(ms.value)
+    api("synchronize", w.h)
+    wall = time.perf_counter() - t0
+    acc = w.vmc_sweeps(1, tau)                   # untimed: acceptance
+    acceptance = acc / (nw * spec.n_particles)
+    e, e_im = w.local_energy()
+    out = dict(moves_per_s=nw * spec.n_particles * steps / wall, ms_per_step=1e3 * wall / steps,
+               kernel_ms=float(np.mean(kernel_ms)), acceptance=acceptance,
+               e_per_particle=float(np.mean(e)) / spec.n_particles,
+               e_per_particle_err=float(np.std(e) / math.sqrt(nw)) / spec.n_particles,
+               im_e_per_particle=float(np.mean(e_im)) / spec.n_particles)
+    w.close()
+    return out, spec
+
+
+def _cpu_worker(args):
+    nw, tau, seed_ = args
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from alea_amd.system import uniform_positions
+    from backends import oracle_tape, oracle_walkers
+    spec, _ = workload_spec()
+    o = oracle_walkers(spec, nw)
+    o.set_positions(uniform_positions(spec, nw, np.random.default_rng(seed_)))
+    o.evaluate()
+    tape = oracle_tape(seed_, spec.n_particles, nw, 1)
+    t0 = time.perf_counter()
+    o.vmc_sweeps(1, tau, tape)
+    return nw * spec.n_particles, time.perf_counter() - t0
+
+
+def cpu_baseline(tau, walkers_per_core=24):
+    """The oracle (the FP64 CPU restatement, which follows the reference's
+    full complex re-inversion per accepted move) on the host cores: one
+    single-threaded process per core, each sweeping `walkers_per_core` walkers
+    once.  Per-walker CPU cost does not depend on the walker count."""
+    cores = max(1, min(16, os.cpu_count() or 1))
+    jobs = [(walkers_per_core, tau, 100 + i) for i in range(cores)]
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(cores) as pool:
+        res = pool.map(_cpu_worker, jobs)
+    wall = time.perf_counter() - t0
+    moves = sum(r[0] for r in res)
+    sweep_time = max(r[1] for r in res)
+    return dict(value=moves / sweep_time, unit="walker-moves/s", cores=cores, kind="port",
+                sample=f"{cores} processes x {walkers_per_core} walkers x 1 sweep of the N=122 "
+                       f"workload on the oracle (FP64, complex re-inversion per accept); "
+                       f"{wall:.1f} s wall incl. setup")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--walkers", type=int, default=65536)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
+              "(walkers shard by GPU with no exchange inside a sweep; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    tau = (0.5 * RS) ** 2   # sigma = sqrt(tau/m) = 0.5 rs
+    fp64, spec = run_gpu(0, args.walkers, args.steps, args.warmup, tau)
+    mixed, _ = run_gpu(1, args.walkers, args.steps, args.warmup, tau)
+    b_move = bytes_per_move(4, mixed["acceptance"], spec)
+    moves_per_launch = args.walkers * spec.n_particles
+    achieved = b_move * moves_per_launch / (mixed["kernel_ms"] * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "sweep_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            t = json.load(f)
+        if t.get("walkers") == args.walkers and t.get("precision") == "mixed":
+            traffic = t.get("hbm_bytes_per_launch")
+    line = {
+        "metric": "walker-moves/s, 2D HEG N=122, mixed vs FP64 GPU",
+        "value": mixed["moves_per_s"],
+        "unit": "walker-moves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64 (mixed)",
+        "data": "synthetic: uniform initial walkers (seed 1), twist from seed 3",
+        "config": {"workload": "twisted 2D HEG VMC sweep, N=122 (61+61), rs=30, Slater-Jastrow-Ewald",
+                   "n_walkers": args.walkers, "n_particles": spec.n_particles,
+                   "step": "one sweep (every particle of every walker)", "tau": tau},
+        "fp64": {"value": fp64["moves_per_s"], "ms_per_step": fp64["ms_per_step"],
+                 "kernel_ms": fp64["kernel_ms"], "acceptance": fp64["acceptance"],
+                 "e_per_particle": fp64["e_per_particle"]},
+        "mixed_over_fp64": mixed["moves_per_s"] / fp64["moves_per_s"],
+        "acceptance": mixed["acceptance"],
+        "e_per_particle": mixed["e_per_particle"],
+        "im_e_per_particle": mixed["im_e_per_particle"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_sweep<float,true>", "kernel_ms": mixed["kernel_ms"],
+                     "bytes_per_move": b_move},
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(tau),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -135,6 +135,13 @@ int alea_vmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape
 int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape, int tape_stride,
                     double* rr_proposed, double* rr_accepted, long long* n_accepted);
 
+/* Waits for the system's stream (hipStreamSynchronize). */
+int alea_synchronize(alea_system* s);
+/* Device time of the last alea_vmc_sweeps / alea_dmc_sweeps launch, measured
+ * with HIP events on the system's stream (no counterpart in the reference:
+ * the measurement hook of the benchmark). */
+int alea_last_sweep_ms(alea_system* s, double* ms);
+
 /* Per-walker xoshiro256++ streams: walker w gets stream_seed(seed, stream, w)
  * (random.h:52-56) expanded by splitmix64 (xoshiro_random.h:29-32). */
 int alea_seed(alea_system* s, uint64_t seed, uint64_t stream);
