@@ -13,7 +13,8 @@ import os
 from ._capi import CApi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libalea_amd.so")
+# ALEA_AMD_LIB overrides the library path (tuning builds); default in-tree.
+LIB_PATH = os.environ.get("ALEA_AMD_LIB", os.path.join(HERE, "libalea_amd.so"))
 
 _api = None
 

@@ -21,12 +21,16 @@
 
 namespace alea_amd {
 int set_ziggurat(const uint64_t* k, const double* w, const double* f, double r);
-void launch_evaluate(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, hipStream_t st);
+void launch_evaluate(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, size_t lds,
+                     hipStream_t st);
+void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, bool mixed, bool cplx,
+                    size_t lds, hipStream_t st);
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
-                  bool cplx, hipStream_t st);
+                  bool cplx, size_t lds, hipStream_t st);
 void launch_call(const DevModel* m, const DevState& s, const DevCall& c, int p, int mode, int nw,
-                 bool mixed, bool cplx, hipStream_t st);
-void launch_measure(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, hipStream_t st);
+                 bool mixed, bool cplx, size_t lds, hipStream_t st);
+void launch_measure(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, size_t lds,
+                    hipStream_t st);
 void launch_gather(const double* src_d, double* dst_d, int stride, const int* src, int n_new,
                    hipStream_t st);
 }  // namespace alea_amd
@@ -157,6 +161,9 @@ struct alea_system {
     DevBuf<double> tape;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;   // brackets the last sweep launch
     bool timed = false;
+    size_t walker_lds = 0, refresh_lds = 0;
+    int refresh_every = 2;        // sweeps between batched inverse refreshes
+    long long since_refresh = 0;  // sweeps (or np accept calls) since the last refresh
 
     bool mixed() const { return precision == ALEA_MIXED; }
     size_t tsize() const { return mixed() ? 4 : 8; }
@@ -169,10 +176,13 @@ struct alea_system {
         // block membership
         for (int i = 0; i < np; ++i) m.block_of[i] = -1;
         int nb_max = 0, inv_elems = 0;
+        int rows_max = 1;
         for (int b = 0; b < m.n_blocks; ++b) {
             for (int i = m.blk_first[b]; i < m.blk_first[b] + m.blk_n[b]; ++i) m.block_of[i] = b;
             nb_max = std::max(nb_max, m.blk_n[b]);
-            inv_elems += (cplx ? 2 : 1) * m.blk_n[b] * m.blk_n[b];
+            m.blk_rows[b] = (m.blk_n[b] + kRowPad - 1) / kRowPad * kRowPad;
+            rows_max = std::max(rows_max, m.blk_rows[b]);
+            inv_elems += (cplx ? 2 : 1) * m.blk_rows[b] * m.blk_n[b];
         }
         // pair lists
         for (int g = 0; g < kMaxGroups; ++g)
@@ -220,6 +230,18 @@ struct alea_system {
         st.rr = rr.p;
         st.status = status.p;
         st.n_accepted = nacc.p;
+        int nbk = 1;
+        for (int b = 0; b < m.n_blocks; ++b) nbk = std::max(nbk, m.blk_n[b]);
+        // LDS of one walker: positions, k-space densities, v', column j, and
+        // the Sherman-Morrison operands (carve() in kernels.hip)
+        walker_lds = sizeof(double) * (2 * static_cast<size_t>(np) + 2 * m.ng * m.nkj + 4 * nbk +
+                                       4 * static_cast<size_t>(rows_max));
+        {
+            const size_t e = cplx ? 2 : 1, n = static_cast<size_t>(nb_max);
+            refresh_lds = 8 * (e * n * n + e * n + 8) + 4 * (8 + n + (n & 1)) + 8 * n;
+            if (m.n_blocks > 0 && refresh_lds > 160 * 1024)
+                throw InvalidArg("determinant block too large for the LDS refresh");
+        }
         static bool zig_done = false;
         if (!zig_done) {
             const Zig z = make_zig();
@@ -230,6 +252,13 @@ struct alea_system {
     }
 
     void sync() { hip_check(hipStreamSynchronize(stream), "kernel"); }
+
+    void refresh() {
+        if (host.n_blocks == 0) return;
+        launch_refresh(dmodel, st, nw, host.n_blocks, mixed(), cplx, refresh_lds, stream);
+        hip_check(hipGetLastError(), "launch refresh");
+        since_refresh = 0;
+    }
 
     void check_status() {
         std::vector<int> s(nw);
@@ -675,8 +704,10 @@ int alea_evaluate(alea_system* s) {
     return guarded([&] {
         Sys(s);
         if (!s->built) throw InvalidArg("positions were never set");
-        launch_evaluate(s->dmodel, s->st, s->nw, s->mixed(), s->cplx, s->stream);
+        s->set_model_nw(s->nw);
+        launch_evaluate(s->dmodel, s->st, s->nw, s->mixed(), s->cplx, s->walker_lds, s->stream);
         hip_check(hipGetLastError(), "launch evaluate");
+        s->refresh();
         s->sync();
         s->check_status();
         s->evaluated = true;
@@ -710,7 +741,7 @@ static void measure(alea_system* s, bool want_grads) {
         st.plap_out = s->grad_buf.p + 3 * n;
     }
     s->set_model_nw(s->nw);
-    launch_measure(s->dmodel, st, s->nw, s->mixed(), s->cplx, s->stream);
+    launch_measure(s->dmodel, st, s->nw, s->mixed(), s->cplx, s->walker_lds, s->stream);
     hip_check(hipGetLastError(), "launch measure");
     s->sync();
 }
@@ -764,7 +795,7 @@ int alea_ratio(alea_system* s, int p, const double* new_pos, double* log_ratio, 
         c.new_pgrad = b + 8 * nw;
         c.new_plap = b + 10 * nw;
         s->set_model_nw(s->nw);
-        launch_call(s->dmodel, s->st, c, p, 1, s->nw, s->mixed(), s->cplx, s->stream);
+        launch_call(s->dmodel, s->st, c, p, 1, s->nw, s->mixed(), s->cplx, s->walker_lds, s->stream);
         hip_check(hipGetLastError(), "launch ratio");
         s->sync();
         hip_check(hipMemcpy(log_ratio, c.log_ratio, nw * sizeof(double), hipMemcpyDeviceToHost), "lr");
@@ -787,8 +818,10 @@ int alea_accept(alea_system* s, int p, const unsigned char* mask) {
         c.new_pos = s->call_buf.p;
         c.mask = s->call_mask.p;
         s->set_model_nw(s->nw);
-        launch_call(s->dmodel, s->st, c, p, 2, s->nw, s->mixed(), s->cplx, s->stream);
+        launch_call(s->dmodel, s->st, c, p, 2, s->nw, s->mixed(), s->cplx, s->walker_lds, s->stream);
         hip_check(hipGetLastError(), "launch accept");
+        // one sweep's worth of accept calls counts as a sweep for the refresh
+        if (++s->since_refresh >= static_cast<long long>(s->refresh_every) * s->np) s->refresh();
         s->sync();
         s->check_status();
     });
@@ -815,11 +848,21 @@ static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* t
         hip_check(hipEventCreate(&s->ev0), "event");
         hip_check(hipEventCreate(&s->ev1), "event");
     }
-    hip_check(hipEventRecord(s->ev0, s->stream), "event record");
-    launch_sweep(s->dmodel, s->st, P, s->nw, s->mixed(), s->cplx, s->stream);
-    hip_check(hipGetLastError(), "launch sweep");
-    hip_check(hipEventRecord(s->ev1, s->stream), "event record");
-    s->timed = true;
+    // One launch per sweep; the batched inverse refresh runs every
+    // refresh_every sweeps (at most refresh_every * Nb Sherman-Morrison
+    // updates between rebuilds).  The events bracket the sweep launches only.
+    const size_t tape_step = static_cast<size_t>(s->np) * 3 * stride;
+    for (int i = 0; i < n_sweeps; ++i) {
+        SweepParams Pi = P;
+        Pi.n_sweeps = 1;
+        if (P.tape) Pi.tape = P.tape + i * tape_step;
+        hip_check(hipEventRecord(s->ev0, s->stream), "event record");
+        launch_sweep(s->dmodel, s->st, Pi, s->nw, s->mixed(), s->cplx, s->walker_lds, s->stream);
+        hip_check(hipGetLastError(), "launch sweep");
+        hip_check(hipEventRecord(s->ev1, s->stream), "event record");
+        if (++s->since_refresh >= s->refresh_every) s->refresh();
+    }
+    s->timed = n_sweeps > 0;
     if (n_accepted) {
         s->sync();
         std::vector<long long> a(s->nw);
@@ -858,6 +901,14 @@ int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape
             if (rr_proposed) rr_proposed[w] = rr[2 * w];
             if (rr_accepted) rr_accepted[w] = rr[2 * w + 1];
         }
+    });
+}
+
+int alea_set_refresh_interval(alea_system* s, int sweeps) {
+    return guarded([&] {
+        Sys(s);
+        if (sweeps < 1) throw InvalidArg("refresh interval must be at least one sweep");
+        s->refresh_every = sweeps;
     });
 }
 

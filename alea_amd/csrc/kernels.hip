@@ -40,6 +40,7 @@
 #include <cstdint>
 
 #include "model.h"
+#include "trig.h"
 
 namespace alea_amd {
 
@@ -268,7 +269,7 @@ __device__ inline void orbital(const DevModel& M, int b, int m, double x, double
         return;
     }
     double s, c;
-    sincos(kx * x + ky * y, &s, &c);
+    sincos_bounded(kx * x + ky * y, &s, &c);
     if (kind == kTwisted) {
         o.vr = c; o.vi = s;
         o.gr[0] = -kx * s; o.gi[0] = kx * c;
@@ -336,11 +337,17 @@ __device__ inline C2<T> cmul(C2<T> a, C2<T> b) {
     return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
-// Determinant inverse element access; complex entries are (re, im) pairs.
+// Determinant inverse A = D^-1 (A[m][c]: orbital m, particle c), row-major
+// per walker as in the reference (slater_determinant.cpp:363-365).  Lane c
+// owns column c in the Sherman-Morrison update, so for a fixed row the wave
+// reads and writes one contiguous run (coalesced); the row pointer is uniform
+// across the wave, which lets the unrolled loads take a scalar base address.
+// Complex entries are (re, im) pairs.
 template <typename T, bool CPLX>
 struct InvRef {
     T* base;
     int n;
+    __device__ T* row(int m) const { return base + (CPLX ? 2 : 1) * m * n; }
     __device__ C2<T> get(int m, int c) const {
         if constexpr (CPLX) {
             const T* p = base + 2 * (m * n + c);
@@ -360,15 +367,39 @@ struct InvRef {
     }
 };
 
-// LDS scratch of one walker.
+// One element of a row of A: a single 2-word load for a complex entry.
+template <typename T, bool CPLX>
+__device__ inline C2<T> load_elem_impl(const T* row, int c) {
+    if constexpr (CPLX) return reinterpret_cast<const C2<T>*>(row)[c];
+    else return C2<T>{row[c], T(0)};
+}
+template <typename T, bool CPLX>
+__device__ inline void store_elem_impl(T* row, int c, C2<T> v) {
+    if constexpr (CPLX) reinterpret_cast<C2<T>*>(row)[c] = v;
+    else row[c] = v.re;
+}
+
+// LDS scratch of one walker, carved from the dynamic shared memory the host
+// sizes per model (sweep_lds_bytes).
 struct Shared {
-    double pos[kMaxParticles * 2];
-    double rho[kMaxGroups * kMaxKJ * 2];
-    double v[2 * kMaxOrb];       // proposed orbital values
-    double u[2 * kMaxOrb];       // column j of A (old)
-    double f[2 * kMaxOrb];       // u / R
-    double red[64];
+    double* pos;   // [np*2]
+    double* rho;   // [ng*nkj*2]
+    double* v;     // [2*nb] proposed orbital values (stored in the determinant precision T)
+    double* u;     // [2*nb] column j of A (old)
+    double* f;     // [4*nb]: u / R, then v again, both as C2<T> (determinant precision)
 };
+
+__device__ inline Shared carve(const DevModel& M, double* base) {
+    int nb = 1;
+    for (int b = 0; b < M.n_blocks; ++b) nb = M.blk_n[b] > nb ? M.blk_n[b] : nb;
+    Shared sh;
+    sh.pos = base;
+    sh.rho = sh.pos + 2 * M.np;
+    sh.v = sh.rho + 2 * M.ng * M.nkj;
+    sh.u = sh.v + 2 * nb;
+    sh.f = sh.u + 2 * nb;
+    return sh;
+}
 
 // Sum reduction of n lane values at once (n <= 16) through LDS-free shuffles.
 template <int N>
@@ -382,18 +413,22 @@ template <typename T, bool CPLX>
 struct Walker {
     const DevModel& M;
     const DevState& S;
-    Shared& sh;
+    Shared sh;
     int w, lane;
     double log_j;            // Jastrow + one-body part of log|psi|
-    double log_b[kMaxBlocks];
+    double log_b0 = 0.0, log_b1 = 0.0;   // determinant blocks (kMaxBlocks = 2)
+    __device__ double& log_b(int b) { return b == 0 ? log_b0 : log_b1; }
 
-    __device__ Walker(const DevModel& m, const DevState& s, Shared& shm, int walker)
+    __device__ Walker(const DevModel& m, const DevState& s, const Shared& shm, int walker)
         : M(m), S(s), sh(shm), w(walker), lane(threadIdx.x) {}
+
+    static __device__ C2<T> load_elem(const T* row, int c) { return load_elem_impl<T, CPLX>(row, c); }
+    static __device__ void store_elem(T* row, int c, C2<T> v) { store_elem_impl<T, CPLX>(row, c, v); }
 
     __device__ InvRef<T, CPLX> inv(int b) const {
         T* base = static_cast<T*>(S.inv) + static_cast<size_t>(w) * S.inv_stride;
         int off = 0;
-        for (int i = 0; i < b; ++i) off += (CPLX ? 2 : 1) * M.blk_n[i] * M.blk_n[i];
+        for (int i = 0; i < b; ++i) off += (CPLX ? 2 : 1) * M.blk_rows[i] * M.blk_n[i];
         return {base + off, M.blk_n[b]};
     }
 
@@ -404,7 +439,8 @@ struct Walker {
         const double* r = S.rho + static_cast<size_t>(w) * nr;
         for (int i = lane; i < nr; i += W) sh.rho[i] = r[i];
         log_j = S.logs[w * 4 + 0];
-        for (int b = 0; b < M.n_blocks; ++b) log_b[b] = S.logs[w * 4 + 1 + b];
+        if (M.n_blocks > 0) log_b0 = S.logs[w * 4 + 1];
+        if (M.n_blocks > 1) log_b1 = S.logs[w * 4 + 2];
         __syncthreads();
     }
     __device__ void store() {
@@ -416,7 +452,8 @@ struct Walker {
         for (int i = lane; i < nr; i += W) r[i] = sh.rho[i];
         if (lane == 0) {
             S.logs[w * 4 + 0] = log_j;
-            for (int b = 0; b < M.n_blocks; ++b) S.logs[w * 4 + 1 + b] = log_b[b];
+            if (M.n_blocks > 0) S.logs[w * 4 + 1] = log_b0;
+            if (M.n_blocks > 1) S.logs[w * 4 + 2] = log_b1;
         }
     }
 
@@ -426,8 +463,8 @@ struct Walker {
         double dlog;
         double og[2], ng[2], nl[2];
     };
-    __device__ void jastrow_move(int p, double x0, double x1, double nx0, double nx1, bool derivs,
-                                 JOut& out) {
+    template <bool derivs>
+    __device__ void jastrow_move(int p, double x0, double x1, double nx0, double nx1, JOut& out) {
         const int gp = M.group_of[p];
         double acc[7] = {0, 0, 0, 0, 0, 0, 0};   // dlog, og0, og1, ng0, ng1, nl0, nl1
         for (int k = lane; k < M.np; k += W) {
@@ -446,7 +483,7 @@ struct Walker {
                 radial_eval(R, ro, uo, duo, d2uo);
                 radial_eval(R, rn, un, dun, d2un);
                 acc[0] += un - uo;
-                if (derivs) {
+                if constexpr (derivs) {
                     double g0, g1, l0, l1;
                     pair_terms(duo, d2uo, ro, o0, o1, g0, g1, l0, l1);
                     acc[1] += g0;
@@ -464,8 +501,8 @@ struct Walker {
             for (int k = lane; k < M.nkj; k += W) {
                 const double kx = M.kjx[k], ky = M.kjy[k];
                 double so, co, sn, cn;
-                sincos(kx * x0 + ky * x1, &so, &co);
-                sincos(kx * nx0 + ky * nx1, &sn, &cn);
+                sincos_bounded(kx * x0 + ky * x1, &so, &co);
+                sincos_bounded(kx * nx0 + ky * nx1, &sn, &cn);
                 const double dr = cn - co, di = sn - so;
                 double dl = 0.0;
                 // moved particle's gradient at the old / new position
@@ -477,7 +514,7 @@ struct Walker {
                     const double rr = sh.rho[(h * M.nkj + k) * 2], ri = sh.rho[(h * M.nkj + k) * 2 + 1];
                     if (h == gp) {
                         dl += wgh * (2.0 * (rr * dr + ri * di) + dr * dr + di * di);
-                        if (derivs) {
+                        if constexpr (derivs) {
                             gold += 2.0 * wgh * (co * ri - so * rr);
                             const double qr = rr + dr, qi = ri + di;
                             gnew += 2.0 * wgh * (cn * qi - sn * qr);
@@ -485,7 +522,7 @@ struct Walker {
                         }
                     } else {
                         dl += 2.0 * wgh * (dr * rr + di * ri);
-                        if (derivs) {
+                        if constexpr (derivs) {
                             gold += 2.0 * wgh * (co * ri - so * rr);
                             gnew += 2.0 * wgh * (cn * ri - sn * rr);
                             lnew += -2.0 * wgh * (cn * rr + sn * ri);
@@ -493,7 +530,7 @@ struct Walker {
                     }
                 }
                 acc[0] += dl;
-                if (derivs) {
+                if constexpr (derivs) {
                     acc[1] += kx * gold;
                     acc[2] += ky * gold;
                     acc[3] += kx * gnew;
@@ -503,7 +540,8 @@ struct Walker {
                 }
             }
         }
-        wsum_n(acc);
+        if constexpr (derivs) wsum_n(acc);
+        else acc[0] = wsum(acc[0]);
         // Nosanow one-body (uniform across lanes)
         if (M.has_nosanow && M.nos_alpha[p] != 0.0) {
             const double a = M.nos_alpha[p];
@@ -530,8 +568,8 @@ struct Walker {
         if (M.nkj > 0 && M.kj_active[gp]) {
             for (int k = lane; k < M.nkj; k += W) {
                 double so, co, sn, cn;
-                sincos(M.kjx[k] * x0 + M.kjy[k] * x1, &so, &co);
-                sincos(M.kjx[k] * nx0 + M.kjy[k] * nx1, &sn, &cn);
+                sincos_bounded(M.kjx[k] * x0 + M.kjy[k] * x1, &so, &co);
+                sincos_bounded(M.kjx[k] * nx0 + M.kjy[k] * nx1, &sn, &cn);
                 sh.rho[(gp * M.nkj + k) * 2] += cn - co;
                 sh.rho[(gp * M.nkj + k) * 2 + 1] += sn - so;
             }
@@ -548,8 +586,8 @@ struct Walker {
         double nl[2][2];
         double rowabs;     // sum_m |v_m|
     };
-    __device__ void det_ratio(int b, int j, double x0, double x1, double nx0, double nx1, bool derivs,
-                              bool old_derivs, DOut& o) {
+    template <bool derivs, bool old_derivs>
+    __device__ void det_ratio(int b, int j, double x0, double x1, double nx0, double nx1, DOut& o) {
         const InvRef<T, CPLX> A = inv(b);
         const int n = M.blk_n[b];
         double acc[20];
@@ -571,7 +609,7 @@ struct Walker {
             acc[1] += ti;
             acc[2] += CPLX ? sqrt(tr * tr + ti * ti) : fabs(tr);
             acc[19] += CPLX ? sqrt(on.vr * on.vr + on.vi * on.vi) : fabs(static_cast<double>(vr));
-            if (derivs) {
+            if constexpr (derivs) {
                 for (int d = 0; d < 2; ++d) {
                     acc[3 + 2 * d] += ar * on.gr[d] - ai * on.gi[d];
                     acc[4 + 2 * d] += ar * on.gi[d] + ai * on.gr[d];
@@ -579,7 +617,7 @@ struct Walker {
                     acc[8 + 2 * d] += ar * on.li[d] + ai * on.lr[d];
                 }
             }
-            if (old_derivs) {
+            if constexpr (old_derivs) {
                 Orb oo;
                 orbital(M, b, m, x0, x1, oo);
                 for (int d = 0; d < 2; ++d) {
@@ -590,7 +628,14 @@ struct Walker {
                 }
             }
         }
-        wsum_n(acc);
+        if constexpr (derivs || old_derivs) {
+            wsum_n(acc);
+        } else {
+            acc[0] = wsum(acc[0]);
+            acc[1] = wsum(acc[1]);
+            acc[2] = wsum(acc[2]);
+            acc[19] = wsum(acc[19]);
+        }
         o.R[0] = acc[0];
         o.R[1] = acc[1];
         o.scale = acc[2];
@@ -723,9 +768,10 @@ struct Walker {
                 const double ar = Aa[e * (m * n + c)];
                 const double ai = CPLX ? Aa[e * (m * n + c) + 1] : 0.0;
                 A.set(m, c, C2<T>{static_cast<T>(ar), static_cast<T>(ai)});
-                cs += CPLX ? sqrt(ar * ar + ai * ai) : fabs(ar);
+                cs += fabs(ar) + fabs(ai);
             }
             colmax = fmax(colmax, cs);
+            for (int m = n; m < M.blk_rows[b]; ++m) A.set(m, c, C2<T>{T(0), T(0)});
         }
         colmax = wmax(colmax);
         double rowmax = 0.0;
@@ -763,43 +809,78 @@ struct Walker {
         double* rabs = S.row_abs + (static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max;
         const double cond_old = S.cond[w * kMaxBlocks + b];
         __syncthreads();
-        bool rebuild_now = updates >= 64 || !isfinite(absR) || absR < 1e-5 || cond_old > 1e8 ||
-                           absR <= 1e4 * eps * d.scale;
+        // The reference also rebuilds after 64 updates (slater_determinant.cpp:
+        // 788-790); here that periodic rebuild is the batched refresh kernel
+        // (k_refresh), which the host runs every few sweeps.  The numerical
+        // triggers stay in the sweep.  The cancellation guard rebuilds when
+        // the rounding bound eps * sum|A_mj v_m| on R exceeds 1/kCancel of
+        // |R|: 1e-4 in FP64 (the reference's constant, slater_determinant.cpp:
+        // 790); 1e-2 in FP32, whose rounding bound passes 1e-4 on about 1% of
+        // the accepted moves of a 61-orbital plane-wave block.
+        constexpr double kCancel = sizeof(T) == 8 ? 1e4 : 1e2;
+        bool rebuild_now = !isfinite(absR) || absR < 1e-5 || cond_old > 1e8 ||
+                           absR <= kCancel * eps * d.scale;
         if (!rebuild_now) {
             // pass 1: row product r_c = sum_m v'_m A[m][c] - delta_cj
             const T invRr = static_cast<T>(CPLX ? Rr / (absR * absR) : 1.0 / Rr);
             const T invRi = static_cast<T>(CPLX ? -Ri / (absR * absR) : 0.0);
-            for (int m = lane; m < n; m += W) {
-                const C2<T> um{static_cast<T>(sh.u[2 * m]), static_cast<T>(sh.u[2 * m + 1])};
-                const C2<T> fm = cmul(um, C2<T>{invRr, invRi});
-                sh.f[2 * m] = static_cast<double>(fm.re);
-                sh.f[2 * m + 1] = static_cast<double>(fm.im);
+            const int rows = M.blk_rows[b];
+            C2<T>* fT = reinterpret_cast<C2<T>*>(sh.f);
+            const C2<T>* vT = reinterpret_cast<const C2<T>*>(sh.f) + rows;   // packed after f
+            for (int m = lane; m < rows; m += W) {
+                if (m < n) {
+                    const C2<T> um{static_cast<T>(sh.u[2 * m]), static_cast<T>(sh.u[2 * m + 1])};
+                    fT[m] = cmul(um, C2<T>{invRr, invRi});
+                    reinterpret_cast<C2<T>*>(sh.f)[rows + m] =
+                        C2<T>{static_cast<T>(sh.v[2 * m]), static_cast<T>(sh.v[2 * m + 1])};
+                } else {
+                    fT[m] = C2<T>{T(0), T(0)};
+                    reinterpret_cast<C2<T>*>(sh.f)[rows + m] = C2<T>{T(0), T(0)};
+                }
             }
             __syncthreads();
+            // Lane c owns column c and holds it in registers, CH rows per
+            // chunk: pass 1 issues a whole chunk of loads at once, and pass 2
+            // updates the last chunk straight from registers (a column of
+            // <= CH rows is read once and written once per accepted move).
+            constexpr int CH = kRowPad;   // rows per chunk; rows padded with zeros to a multiple
+            const int nch = M.blk_rows[b] / CH;
             double colmax = 0.0;
             for (int c = lane; c < n; c += W) {
+                C2<T> a[CH];
                 C2<T> r{T(0), T(0)};
-                for (int m = 0; m < n; ++m) {
-                    const C2<T> vm{static_cast<T>(sh.v[2 * m]), static_cast<T>(sh.v[2 * m + 1])};
-                    const C2<T> a = A.get(m, c);
-                    const C2<T> pr = cmul(vm, a);
-                    r.re += pr.re;
-                    r.im += pr.im;
+                for (int ch = 0; ch < nch; ++ch) {
+                    const int m0 = ch * CH;
+#pragma unroll
+                    for (int i = 0; i < CH; ++i) a[i] = load_elem(A.row(m0 + i), c);
+#pragma unroll
+                    for (int i = 0; i < CH; ++i) {
+                        const C2<T> pr = cmul(vT[m0 + i], a[i]);
+                        r.re += pr.re;
+                        r.im += pr.im;
+                    }
                 }
                 if (c == j) r.re -= T(1);
-                // pass 2: A[m][c] -= f_m r_c
-                double cs = 0.0;
-                for (int m = 0; m < n; ++m) {
-                    const C2<T> fm{static_cast<T>(sh.f[2 * m]), static_cast<T>(sh.f[2 * m + 1])};
-                    C2<T> a = A.get(m, c);
-                    const C2<T> pr = cmul(fm, r);
-                    a.re -= pr.re;
-                    a.im -= pr.im;
-                    A.set(m, c, a);
-                    cs += CPLX ? sqrt(static_cast<double>(a.re) * a.re + static_cast<double>(a.im) * a.im)
-                               : fabs(static_cast<double>(a.re));
+                // pass 2: A[m][c] -= f_m r_c, last chunk first (in registers)
+                T cs = T(0);
+                for (int ch = nch - 1; ch >= 0; --ch) {
+                    const int m0 = ch * CH;
+                    if (ch != nch - 1) {
+#pragma unroll
+                        for (int i = 0; i < CH; ++i) a[i] = load_elem(A.row(m0 + i), c);
+                    }
+#pragma unroll
+                    for (int i = 0; i < CH; ++i) {
+                        const C2<T> pr = cmul(fT[m0 + i], r);
+                        a[i].re -= pr.re;
+                        a[i].im -= pr.im;
+                        store_elem(A.row(m0 + i), c, a[i]);
+                        // |Re| + |Im| bounds the modulus within sqrt(2); it
+                        // only feeds the condition estimate
+                        cs += fabs(a[i].re) + (CPLX ? fabs(a[i].im) : T(0));
+                    }
                 }
-                colmax = fmax(colmax, cs);
+                colmax = fmax(colmax, static_cast<double>(cs));
             }
             colmax = wmax(colmax);
             if (lane == 0) rabs[j] = d.rowabs;
@@ -811,14 +892,14 @@ struct Walker {
             if (!isfinite(cond) || cond > 1e8) {
                 rebuild_now = true;
             } else {
-                log_b[b] += log(absR);
+                log_b(b) += log(absR);
                 if (lane == 0) {
                     *upd = updates;
                     S.cond[w * kMaxBlocks + b] = cond;
                 }
             }
         }
-        if (rebuild_now) log_b[b] = rebuild(b);
+        if (rebuild_now) log_b(b) = rebuild(b);
         __syncthreads();
         return rebuild_now;
     }
@@ -830,7 +911,7 @@ struct Walker {
     // 222-253; every gauge-invariant quantity is unchanged).
     __device__ void regauge_column(int b, int j, double shift0, double shift1) {
         double s, c;
-        sincos(M.orb_k[b][0][0] * shift0 + M.orb_k[b][0][1] * shift1, &s, &c);
+        sincos_bounded(M.orb_k[b][0][0] * shift0 + M.orb_k[b][0][1] * shift1, &s, &c);
         const InvRef<T, CPLX> A = inv(b);
         const C2<T> ph{static_cast<T>(c), static_cast<T>(s)};
         for (int m = lane; m < M.blk_n[b]; m += W) A.set(m, j, cmul(A.get(m, j), ph));
@@ -840,13 +921,13 @@ struct Walker {
     // ── one Metropolis / drift-diffusion move of particle p ────────────────
     // mode 0: sweep (decide from the random numbers); 1: ratio only (per-call
     // path, writes DevCall outputs); 2: commit where call.mask is set.
+    template <bool derivs>
     __device__ bool move(int p, double nx0, double nx1, double uni, int mode, const DevCall* call,
                          double tau, bool dmc, double drift_scale, const double* gold,
                          double& rr_acc) {
         const double x0 = sh.pos[2 * p], x1 = sh.pos[2 * p + 1];
-        const bool derivs = dmc || mode == 1;
         JOut J;
-        jastrow_move(p, x0, x1, nx0, nx1, derivs, J);
+        jastrow_move<derivs>(p, x0, x1, nx0, nx1, J);
         double lr = J.dlog, phase = 0.0;
         double ng[2] = {J.ng[0], J.ng[1]}, nl[2] = {J.nl[0], J.nl[1]};
         double npg[2] = {0, 0}, npl[2] = {0, 0};
@@ -854,7 +935,7 @@ struct Walker {
         DOut D{};
         if (b >= 0) {
             const int j = p - M.blk_first[b];
-            det_ratio(b, j, x0, x1, nx0, nx1, derivs, false, D);
+            det_ratio<derivs, false>(b, j, x0, x1, nx0, nx1, D);
             const double absR = sqrt(D.R[0] * D.R[0] + D.R[1] * D.R[1]);
             lr += log(fmax(absR, 1e-300));
             if (CPLX) {
@@ -862,7 +943,7 @@ struct Walker {
             } else if (D.R[0] < 0.0) {
                 phase += 3.14159265358979323846;
             }
-            if (derivs) {
+            if constexpr (derivs) {
                 // G' = (sum A grad phi)/R, L' = (sum A lap phi)/R - G'^2
                 double ir = 0.0, ii = 0.0;
                 if (absR > 0.0) { ir = D.R[0] / (absR * absR); ii = -D.R[1] / (absR * absR); }
@@ -943,13 +1024,13 @@ struct Walker {
     __device__ void grad_of(int p, double g[2]) {
         const double x0 = sh.pos[2 * p], x1 = sh.pos[2 * p + 1];
         JOut J;
-        jastrow_move(p, x0, x1, x0, x1, true, J);
+        jastrow_move<true>(p, x0, x1, x0, x1, J);
         g[0] = J.og[0];
         g[1] = J.og[1];
         const int b = M.block_of[p];
         if (b >= 0) {
             DOut D{};
-            det_ratio(b, p - M.blk_first[b], x0, x1, x0, x1, true, true, D);
+            det_ratio<false, true>(b, p - M.blk_first[b], x0, x1, x0, x1, D);
             // at r' = r the ratio is 1 up to rounding; G = sum A grad phi (old)
             for (int d = 0; d < 2; ++d) g[d] += D.og[d][0];
         }
@@ -964,7 +1045,8 @@ struct Walker {
 template <typename T, bool CPLX>
 __global__ __launch_bounds__(64) void k_evaluate(const DevModel* __restrict__ Mp, DevState S) {
     const DevModel& M = *Mp;
-    __shared__ Shared sh;
+    extern __shared__ double lds_[];
+    Shared sh = carve(M, lds_);
     const int w = blockIdx.x, lane = threadIdx.x;
     Walker<T, CPLX> wk(M, S, sh, w);
     const double* p = S.pos + static_cast<size_t>(w) * M.np * 2;
@@ -1000,7 +1082,7 @@ __global__ __launch_bounds__(64) void k_evaluate(const DevModel* __restrict__ Mp
             if (M.kj_active[g])
                 for (int i = M.g_first[g]; i < M.g_last[g]; ++i) {
                     double s, c;
-                    sincos(M.kjx[k] * sh.pos[2 * i] + M.kjy[k] * sh.pos[2 * i + 1], &s, &c);
+                    sincos_bounded(M.kjx[k] * sh.pos[2 * i] + M.kjy[k] * sh.pos[2 * i + 1], &s, &c);
                     rr += c;
                     ri += s;
                 }
@@ -1017,20 +1099,193 @@ __global__ __launch_bounds__(64) void k_evaluate(const DevModel* __restrict__ Mp
             }
     }
     wk.log_j = wsum(lj) + wsum(lk);
-    __syncthreads();
-    for (int b = 0; b < M.n_blocks; ++b) {
-        const double ld = wk.rebuild(b);
-        wk.log_b[b] = fmax(ld, log(1e-300));
-    }
+    wk.log_b0 = wk.log_b1 = 0.0;   // set by k_refresh
     wk.store();
 }
 
-// Fused sweeps: every particle of the walker, n_sweeps times.
+// Batched inverse refresh: one 256-thread workgroup per (walker, block)
+// recomputes D from the current positions and A = D^-1 by pivoted
+// Gauss-Jordan in place in LDS (partial pivoting on the reduced column, first
+// largest |.|, as slater_determinant.cpp:905-993; columns unscrambled in
+// reverse pivot order at the end), in FP64, then stores A in the determinant
+// precision with log|det D| (floored at 1e-300 like evaluate_batch), the row
+// sums of |D|, the condition estimate, and a zero update count.
 template <typename T, bool CPLX>
-__global__ __launch_bounds__(64) void k_sweep(const DevModel* __restrict__ Mp, DevState S,
+__global__ __launch_bounds__(256) void k_refresh(const DevModel* __restrict__ Mp, DevState S) {
+    const DevModel& M = *Mp;
+    extern __shared__ double lds_[];
+    const int nbk = M.n_blocks;
+    const int w = blockIdx.x / nbk, b = blockIdx.x % nbk;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int n = M.blk_n[b], first = M.blk_first[b];
+    constexpr int e = CPLX ? 2 : 1;
+    double* a = lds_;                  // [n][n] (x e)
+    double* fcol = a + e * n * n;      // [n] (x e)
+    double* red_v = fcol + e * n;      // [8]
+    int* red_i = reinterpret_cast<int*>(red_v + 8);   // [8]
+    int* perm = red_i + 8;             // [n]
+    double* rsum = reinterpret_cast<double*>(perm + n + (n & 1));   // [n]
+    const double* pos = S.pos + static_cast<size_t>(w) * M.np * 2;
+    for (int idx = tid; idx < n * n; idx += nt) {
+        const int kr = idx / n, m = idx % n;
+        Orb o;
+        orbital(M, b, m, pos[2 * (first + kr)], pos[2 * (first + kr) + 1], o);
+        a[e * idx] = o.vr;
+        if (CPLX) a[e * idx + 1] = o.vi;
+    }
+    __syncthreads();
+    for (int kr = tid; kr < n; kr += nt) {
+        double rs = 0.0;
+        for (int m = 0; m < n; ++m) {
+            const double re = a[e * (kr * n + m)];
+            const double im = CPLX ? a[e * (kr * n + m) + 1] : 0.0;
+            rs += CPLX ? sqrt(re * re + im * im) : fabs(re);
+        }
+        rsum[kr] = rs;
+    }
+    double log_det = 0.0;
+    bool singular = false;
+    for (int k = 0; k < n; ++k) {
+        // pivot: first row r >= k of largest |a[r][k]|
+        double best = -1.0;
+        int piv = n;
+        for (int r = k + tid; r < n; r += nt) {
+            const double re = a[e * (r * n + k)];
+            const double im = CPLX ? a[e * (r * n + k) + 1] : 0.0;
+            const double v = CPLX ? sqrt(re * re + im * im) : fabs(re);
+            if (v > best) { best = v; piv = r; }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ob = __shfl_xor(best, o, W);
+            const int op = __shfl_xor(piv, o, W);
+            if (ob > best || (ob == best && op < piv)) { best = ob; piv = op; }
+        }
+        if ((tid & (W - 1)) == 0) {
+            red_v[tid / W] = best;
+            red_i[tid / W] = piv;
+        }
+        __syncthreads();
+        best = red_v[0];
+        piv = red_i[0];
+        for (int q = 1; q < nt / W; ++q)
+            if (red_v[q] > best || (red_v[q] == best && red_i[q] < piv)) { best = red_v[q]; piv = red_i[q]; }
+        if (!(best > 0.0) || !isfinite(best)) { singular = true; break; }
+        log_det += log(best);
+        if (tid == 0) perm[k] = piv;
+        if (piv != k)
+            for (int c = tid; c < n; c += nt)
+                for (int q = 0; q < e; ++q) {
+                    const double t = a[e * (k * n + c) + q];
+                    a[e * (k * n + c) + q] = a[e * (piv * n + c) + q];
+                    a[e * (piv * n + c) + q] = t;
+                }
+        __syncthreads();
+        const double dr = a[e * (k * n + k)];
+        const double di = CPLX ? a[e * (k * n + k) + 1] : 0.0;
+        double pr, pi;
+        if (CPLX) {
+            const double d2 = dr * dr + di * di;
+            pr = dr / d2;
+            pi = -di / d2;
+        } else {
+            pr = 1.0 / dr;
+            pi = 0.0;
+        }
+        __syncthreads();
+        for (int c = tid; c < n; c += nt) {
+            const double xr = c == k ? 1.0 : a[e * (k * n + c)];
+            const double xi = c == k ? 0.0 : (CPLX ? a[e * (k * n + c) + 1] : 0.0);
+            a[e * (k * n + c)] = CPLX ? xr * pr - xi * pi : xr * pr;
+            if (CPLX) a[e * (k * n + c) + 1] = xr * pi + xi * pr;
+        }
+        for (int r = tid; r < n; r += nt) {
+            fcol[e * r] = r == k ? 0.0 : a[e * (r * n + k)];
+            if (CPLX) fcol[e * r + 1] = r == k ? 0.0 : a[e * (r * n + k) + 1];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < n * n; idx += nt) {
+            const int r = idx / n, c = idx % n;
+            if (r == k) continue;
+            const double fr = fcol[e * r], fi = CPLX ? fcol[e * r + 1] : 0.0;
+            const double kr_ = a[e * (k * n + c)], ki = CPLX ? a[e * (k * n + c) + 1] : 0.0;
+            double xr = c == k ? 0.0 : a[e * idx];
+            double xi = c == k ? 0.0 : (CPLX ? a[e * idx + 1] : 0.0);
+            xr -= fr * kr_ - fi * ki;
+            a[e * idx] = xr;
+            if (CPLX) {
+                xi -= fr * ki + fi * kr_;
+                a[e * idx + 1] = xi;
+            }
+        }
+        __syncthreads();
+    }
+    if (singular) {
+        if (tid == 0) S.status[w] = 1;
+        return;
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        const int pk = perm[k];
+        if (pk != k)
+            for (int r = tid; r < n; r += nt)
+                for (int q = 0; q < e; ++q) {
+                    const double t = a[e * (r * n + k) + q];
+                    a[e * (r * n + k) + q] = a[e * (r * n + pk) + q];
+                    a[e * (r * n + pk) + q] = t;
+                }
+        __syncthreads();
+    }
+    // store A[m][c] (row m = orbital) and the norms
+    T* out = static_cast<T*>(S.inv) + static_cast<size_t>(w) * S.inv_stride;
+    for (int i = 0; i < b; ++i) out += e * M.blk_rows[i] * M.blk_n[i];
+    for (int idx = tid; idx < n * n; idx += nt) {
+        out[e * idx] = static_cast<T>(a[e * idx]);
+        if (CPLX) out[e * idx + 1] = static_cast<T>(a[e * idx + 1]);
+    }
+    for (int idx = n * n + tid; idx < M.blk_rows[b] * n; idx += nt) {   // zero padding rows
+        out[e * idx] = T(0);
+        if (CPLX) out[e * idx + 1] = T(0);
+    }
+    double colmax = 0.0;
+    for (int c = tid; c < n; c += nt) {
+        double cs = 0.0;
+        for (int m = 0; m < n; ++m) {
+            const double re = a[e * (m * n + c)];
+            const double im = CPLX ? a[e * (m * n + c) + 1] : 0.0;
+            cs += fabs(re) + fabs(im);
+        }
+        colmax = fmax(colmax, cs);
+    }
+    double rowmax = 0.0;
+    for (int kr = tid; kr < n; kr += nt) {
+        rowmax = fmax(rowmax, rsum[kr]);
+        S.row_abs[(static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max + kr] = rsum[kr];
+    }
+    colmax = wmax(colmax);
+    rowmax = wmax(rowmax);
+    __syncthreads();
+    if ((tid & (W - 1)) == 0) {
+        red_v[tid / W] = colmax;
+        fcol[tid / W] = rowmax;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int q = 1; q < nt / W; ++q) {
+            colmax = fmax(colmax, red_v[q]);
+            rowmax = fmax(rowmax, fcol[q]);
+        }
+        S.cond[w * kMaxBlocks + b] = rowmax * colmax;
+        S.updates[w * kMaxBlocks + b] = 0;
+        S.logs[w * 4 + 1 + b] = fmax(log_det, log(1e-300));
+    }
+}
+
+// Fused sweeps: every particle of the walker, n_sweeps times.
+template <typename T, bool CPLX, bool DMC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_sweep(const DevModel* __restrict__ Mp, DevState S,
                                               SweepParams P) {
     const DevModel& M = *Mp;
-    __shared__ Shared sh;
+    extern __shared__ double lds_[];
+    Shared sh = carve(M, lds_);
     const int w = blockIdx.x, lane = threadIdx.x;
     Walker<T, CPLX> wk(M, S, sh, w);
     wk.load();
@@ -1057,24 +1312,26 @@ __global__ __launch_bounds__(64) void k_sweep(const DevModel* __restrict__ Mp, D
             const double sigma = sqrt(tm);
             const double x0 = wk.sh.pos[2 * p], x1 = wk.sh.pos[2 * p + 1];
             double gold[2] = {0.0, 0.0}, scale = 0.0;
-            if (P.dmc) {
+            if constexpr (DMC) {
                 wk.grad_of(p, gold);
                 const double v2 = gold[0] * gold[0] + gold[1] * gold[1];
                 scale = v2 < 1e-10 ? tm : (-1.0 + sqrt(1.0 + 2.0 * tm * v2)) / v2;
             }
             const double nx0 = x0 + scale * gold[0] + sigma * xi0;
             const double nx1 = x1 + scale * gold[1] + sigma * xi1;
-            if (P.dmc) {
+            if constexpr (DMC) {
                 const double d0 = nx0 - x0, d1 = nx1 - x1;
                 rr_p += tm * d0 * d0 + tm * d1 * d1;
             }
-            if (wk.move(p, nx0, nx1, uni, 0, nullptr, P.tau, P.dmc != 0, scale, gold, rr_a)) ++nacc;
+            const bool accepted =
+                wk.template move<DMC>(p, nx0, nx1, uni, 0, nullptr, P.tau, DMC, scale, gold, rr_a);
+            if (accepted) ++nacc;
         }
     }
     wk.store();
     if (lane == 0) {
         S.n_accepted[w] += nacc;
-        if (P.dmc) {
+        if constexpr (DMC) {
             S.rr[w * 2] += rr_p;
             S.rr[w * 2 + 1] += rr_a;
         }
@@ -1089,12 +1346,14 @@ template <typename T, bool CPLX>
 __global__ __launch_bounds__(64) void k_call(const DevModel* __restrict__ Mp, DevState S, DevCall call,
                                              int p, int mode) {
     const DevModel& M = *Mp;
-    __shared__ Shared sh;
+    extern __shared__ double lds_[];
+    Shared sh = carve(M, lds_);
     const int w = blockIdx.x;
     Walker<T, CPLX> wk(M, S, sh, w);
     wk.load();
     double rr = 0.0;
-    wk.move(p, call.new_pos[w], call.new_pos[M.nw + w], 0.0, mode, &call, 0.0, false, 0.0, nullptr, rr);
+    wk.template move<true>(p, call.new_pos[w], call.new_pos[M.nw + w], 0.0, mode, &call, 0.0, false, 0.0,
+                           nullptr, rr);
     if (mode == 2) wk.store();
 }
 
@@ -1103,7 +1362,8 @@ __global__ __launch_bounds__(64) void k_call(const DevModel* __restrict__ Mp, De
 template <typename T, bool CPLX>
 __global__ __launch_bounds__(64) void k_measure(const DevModel* __restrict__ Mp, DevState S) {
     const DevModel& M = *Mp;
-    __shared__ Shared sh;
+    extern __shared__ double lds_[];
+    Shared sh = carve(M, lds_);
     const int w = blockIdx.x, lane = threadIdx.x;
     Walker<T, CPLX> wk(M, S, sh, w);
     wk.load();
@@ -1129,7 +1389,7 @@ __global__ __launch_bounds__(64) void k_measure(const DevModel* __restrict__ Mp,
         if (M.nkj > 0 && M.kj_active[gi])
             for (int k = 0; k < M.nkj; ++k) {
                 double s, c;
-                sincos(M.kjx[k] * x0 + M.kjy[k] * x1, &s, &c);
+                sincos_bounded(M.kjx[k] * x0 + M.kjy[k] * x1, &s, &c);
                 double gk = 0.0, lk = 0.0;
                 for (int h = 0; h < M.ng; ++h) {
                     const double wgh = M.kjw[gi][h][k];
@@ -1207,7 +1467,7 @@ __global__ __launch_bounds__(64) void k_measure(const DevModel* __restrict__ Mp,
                 const double q = M.charge[i];
                 if (q == 0.0) continue;
                 double s, c;
-                sincos(M.kex[k] * sh.pos[2 * i] + M.key[k] * sh.pos[2 * i + 1], &s, &c);
+                sincos_bounded(M.kex[k] * sh.pos[2 * i] + M.key[k] * sh.pos[2 * i + 1], &s, &c);
                 rr += q * c;
                 ri += q * s;
             }
@@ -1264,19 +1524,41 @@ int set_ziggurat(const uint64_t* k, const double* w, const double* f, double r) 
         }                                                                              \
     } while (0)
 
-void launch_evaluate(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, hipStream_t st) {
-    DISPATCH(k_evaluate, dim3(nw), dim3(64), 0, st, m, s);
+void launch_evaluate(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, size_t lds,
+                     hipStream_t st) {
+    DISPATCH(k_evaluate, dim3(nw), dim3(64), lds, st, m, s);
+}
+void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, bool mixed, bool cplx,
+                    size_t lds, hipStream_t st) {
+    DISPATCH(k_refresh, dim3(nw * n_blocks), dim3(256), lds, st, m, s);
 }
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
-                  bool cplx, hipStream_t st) {
-    DISPATCH(k_sweep, dim3(nw), dim3(64), 0, st, m, s, p);
+                  bool cplx, size_t lds, hipStream_t st) {
+    if (p.dmc) {
+        if (mixed) {
+            if (cplx) hipLaunchKernelGGL((k_sweep<float, true, true>), dim3(nw), dim3(64), lds, st, m, s, p);
+            else hipLaunchKernelGGL((k_sweep<float, false, true>), dim3(nw), dim3(64), lds, st, m, s, p);
+        } else {
+            if (cplx) hipLaunchKernelGGL((k_sweep<double, true, true>), dim3(nw), dim3(64), lds, st, m, s, p);
+            else hipLaunchKernelGGL((k_sweep<double, false, true>), dim3(nw), dim3(64), lds, st, m, s, p);
+        }
+        return;
+    }
+    if (mixed) {
+        if (cplx) hipLaunchKernelGGL((k_sweep<float, true, false>), dim3(nw), dim3(64), lds, st, m, s, p);
+        else hipLaunchKernelGGL((k_sweep<float, false, false>), dim3(nw), dim3(64), lds, st, m, s, p);
+    } else {
+        if (cplx) hipLaunchKernelGGL((k_sweep<double, true, false>), dim3(nw), dim3(64), lds, st, m, s, p);
+        else hipLaunchKernelGGL((k_sweep<double, false, false>), dim3(nw), dim3(64), lds, st, m, s, p);
+    }
 }
 void launch_call(const DevModel* m, const DevState& s, const DevCall& c, int p, int mode, int nw,
-                 bool mixed, bool cplx, hipStream_t st) {
-    DISPATCH(k_call, dim3(nw), dim3(64), 0, st, m, s, c, p, mode);
+                 bool mixed, bool cplx, size_t lds, hipStream_t st) {
+    DISPATCH(k_call, dim3(nw), dim3(64), lds, st, m, s, c, p, mode);
 }
-void launch_measure(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, hipStream_t st) {
-    DISPATCH(k_measure, dim3(nw), dim3(64), 0, st, m, s);
+void launch_measure(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, size_t lds,
+                    hipStream_t st) {
+    DISPATCH(k_measure, dim3(nw), dim3(64), lds, st, m, s);
 }
 void launch_gather(const double* src_d, double* dst_d, int stride, const int* src, int n_new,
                    hipStream_t st) {

@@ -21,6 +21,7 @@ constexpr int kMaxKE = 192;        // Ewald vectors
 constexpr int kMaxBlocks = 2;      // determinant blocks
 constexpr int kMaxOrb = 128;       // orbitals per block
 constexpr int kMaxCoulomb = 8;
+constexpr int kRowPad = 16;        // determinant rows padded to a multiple of this
 
 struct DevCell {
     int periodic = 0, ortho = 0;
@@ -79,6 +80,9 @@ struct DevModel {
     // Slater determinant blocks
     int n_blocks = 0, det_complex = 0;
     int blk_kind[kMaxBlocks] = {}, blk_first[kMaxBlocks] = {}, blk_n[kMaxBlocks] = {};
+    // rows of A stored per block: blk_n rounded up to kRowPad; the extra rows
+    // are zero and stay zero, so the update loops run without tail guards
+    int blk_rows[kMaxBlocks] = {};
     double orb_k[kMaxBlocks][kMaxOrb][2] = {};   // plane wave k, twisted q, Gaussian centre
     int orb_sin[kMaxBlocks][kMaxOrb] = {};
     double orb_alpha[kMaxBlocks][kMaxOrb] = {};

@@ -135,6 +135,14 @@ int alea_vmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape
 int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape, int tape_stride,
                     double* rr_proposed, double* rr_accepted, long long* n_accepted);
 
+/* Sweeps between two batched recomputations of every determinant inverse
+ * (default 2).  The reference rebuilds one walker's inverse after 64 accepted
+ * updates (slater_determinant.cpp:788-790); here the periodic rebuild is one
+ * batched kernel over all walkers, at most interval * Nb updates apart.  The
+ * numerical triggers (tiny or non-finite ratio, cancellation, condition
+ * estimate above 1e8) still rebuild inside the sweep. */
+int alea_set_refresh_interval(alea_system* s, int sweeps);
+
 /* Waits for the system's stream (hipStreamSynchronize). */
 int alea_synchronize(alea_system* s);
 /* Device time of the last alea_vmc_sweeps / alea_dmc_sweeps launch, measured
