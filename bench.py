@@ -123,6 +123,19 @@ def cpu_baseline(tau, walkers_per_core=24):
                        f"{wall:.1f} s wall incl. setup")
 
 
+def copy_calibration(n_bytes, reps=3):
+    """A plain device-to-device copy of n_bytes, reps times: the known-size
+    reference that scales the FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
+    import torch
+    src = torch.empty(n_bytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": src.numel() * 4, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,13 +143,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--walkers", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
+                    help="time one form only (profiling runs); prints its dict, not the bench line")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="run a plain copy of BYTES bytes (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
-              "(walkers shard by GPU with no exchange inside a sweep; see DESIGN.md)",
-              file=sys.stderr)
+              "(walkers shard by GPU with no exchange inside a sweep)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     tau = (0.5 * RS) ** 2   # sigma = sqrt(tau/m) = 0.5 rs
+    if args.only:
+        out, _ = run_gpu(0 if args.only == "fp64" else 1, args.walkers, args.steps, args.warmup, tau)
+        print(json.dumps(dict(form=args.only, **out)), flush=True)
+        return
     fp64, spec = run_gpu(0, args.walkers, args.steps, args.warmup, tau)
     mixed, _ = run_gpu(1, args.walkers, args.steps, args.warmup, tau)
     b_move = bytes_per_move(4, mixed["acceptance"], spec)

@@ -9,7 +9,7 @@
 // (walker-fastest SoA, [.. * nw + w]) and in the reference's order of
 // operations where that matters for bit-level agreement (the minimum-image
 // fold, the RNG, the tape order).  Each function cites the reference file:line
-// it follows.  How each part is pinned (tests/test_oracle_*.py):
+// it follows.  How each part is pinned (tests/test_oracle.py):
 //   cell, distances, Padé (+ image sum), Drummond, Nosanow, kinetic, Ewald,
 //   Coulomb, VMC sweep, RNG  -> against oracle/_ref (the reference's own
 //                               sources compiled here) and the reference's
