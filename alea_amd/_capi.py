@@ -45,6 +45,13 @@ SIGNATURES = {
     "vmc_sweeps": (C.c_int, [_V, C.c_int, C.c_double, _P_D, _P_LL]),
     "dmc_sweeps": (C.c_int, [_V, C.c_int, C.c_double, _P_D, C.c_int, _P_D, _P_D, _P_LL]),
     "permute": (C.c_int, [_V, C.c_int, _P_I]),
+    "get_num_walkers": (C.c_int, [_V, _P_I]),
+    "dmc_begin": (C.c_int, [_V]),
+    "dmc_weights": (C.c_int, [_V, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "dmc_sums": (C.c_int, [_V, _P_D, _P_D, _P_LL]),
+    "get_weights": (C.c_int, [_V, _P_D]),
+    "set_weights": (C.c_int, [_V, _P_D]),
+    "branch": (C.c_int, [_V, _P_U64, C.c_int, _P_I, _P_I]),
 }
 
 

@@ -14,6 +14,7 @@
 #include <numbers>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/alea_amd.h"
@@ -33,6 +34,12 @@ void launch_measure(const DevModel* m, const DevState& s, int nw, bool mixed, bo
                     hipStream_t st);
 void launch_gather(const double* src_d, double* dst_d, int stride, const int* src, int n_new,
                    hipStream_t st);
+void launch_dmc_weights(const double* e_new, double* e_old, double* weight, long long* n_cut, int nw,
+                        double etrial, double eref, double e_cut, double e_max, double half_tau,
+                        hipStream_t st);
+void launch_branch(const double* weight, const double* xi, int* src, int src_cap, int nw,
+                   long long* total, hipStream_t st);
+void launch_fill(double* a, int n, double v, hipStream_t st);
 }  // namespace alea_amd
 
 using namespace alea_amd;
@@ -150,11 +157,15 @@ struct alea_system {
     hipStream_t stream = nullptr;
     DevState st;
     DevBuf<double> pos, rho, logs, row_abs, cond, work, e_loc, e_loc_im, rr;
-    DevBuf<double> pos2, rho2, logs2, row_abs2, cond2, inv2, rr2;
-    DevBuf<char> inv_bytes;
-    DevBuf<int> updates, updates2, status, src;
-    DevBuf<uint64_t> rng, rng2;
-    DevBuf<long long> nacc;
+    DevBuf<double> weights, e_prev, xi;   // DMC weights, E_L before the sweep, branch uniforms
+    DevBuf<char> inv_bytes, inv_spare;    // inverse records; the gather target of a permutation
+    DevBuf<double> perm_tmp;              // gather target of the other per-walker arrays
+    DevBuf<int> updates, status, src;
+    DevBuf<uint64_t> rng;
+    DevBuf<long long> nacc, ncut, branch_total;
+    bool energies_valid = false;          // e_loc holds E_L of the current configurations
+    bool seeded = false;                  // device streams seeded (alea_seed)
+    uint64_t seed_base = 0, seed_stream = 0, next_stream_index = 0;
     DevBuf<double> call_buf;       // new_pos, outputs
     DevBuf<unsigned char> call_mask;
     DevBuf<double> grad_buf;       // 4 * np * 2 * nw
@@ -213,6 +224,12 @@ struct alea_system {
         e_loc.alloc(nw_);
         e_loc_im.alloc(nw_);
         rr.alloc(nw_ * 2);
+        weights.alloc(nw_);
+        launch_fill(weights.p, static_cast<int>(nw_), 1.0, nullptr);
+        e_prev.alloc(nw_);
+        xi.alloc(nw_);
+        ncut.alloc(nw_);
+        branch_total.alloc(1);
         status.alloc(nw_);
         nacc.alloc(nw_);
         src.alloc(nw_);
@@ -682,6 +699,7 @@ int alea_set_positions(alea_system* s, const double* pos) {
         hip_check(hipMemcpy(s->pos.p, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice),
                   "upload positions");
         s->evaluated = false;
+        s->energies_valid = false;
     });
 }
 
@@ -744,6 +762,7 @@ static void measure(alea_system* s, bool want_grads) {
     launch_measure(s->dmodel, st, s->nw, s->mixed(), s->cplx, s->walker_lds, s->stream);
     hip_check(hipGetLastError(), "launch measure");
     s->sync();
+    s->energies_valid = true;
 }
 
 int alea_get_gradients(alea_system* s, double* grad, double* lap) {
@@ -820,6 +839,7 @@ int alea_accept(alea_system* s, int p, const unsigned char* mask) {
         s->set_model_nw(s->nw);
         launch_call(s->dmodel, s->st, c, p, 2, s->nw, s->mixed(), s->cplx, s->walker_lds, s->stream);
         hip_check(hipGetLastError(), "launch accept");
+        s->energies_valid = false;
         // one sweep's worth of accept calls counts as a sweep for the refresh
         if (++s->since_refresh >= static_cast<long long>(s->refresh_every) * s->np) s->refresh();
         s->sync();
@@ -844,6 +864,7 @@ static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* t
     }
     hip_check(hipMemsetAsync(s->nacc.p, 0, s->nw * sizeof(long long), s->stream), "memset");
     s->set_model_nw(s->nw);
+    if (n_sweeps > 0) s->energies_valid = false;
     if (!s->ev0) {
         hip_check(hipEventCreate(&s->ev0), "event");
         hip_check(hipEventCreate(&s->ev1), "event");
@@ -887,19 +908,24 @@ int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape
     return guarded([&] {
         Sys(s);
         if (tape && tape_stride < s->nw) throw InvalidArg("tape stride smaller than the walker count");
+        if (!rr_proposed != !rr_accepted) throw InvalidArg("give both rr arrays or neither");
         const size_t nw = s->nw;
         std::vector<double> rr(2 * nw, 0.0);
-        for (size_t w = 0; w < nw; ++w) {
-            rr[2 * w] = rr_proposed ? rr_proposed[w] : 0.0;
-            rr[2 * w + 1] = rr_accepted ? rr_accepted[w] : 0.0;
+        if (rr_proposed) {
+            for (size_t w = 0; w < nw; ++w) {
+                rr[2 * w] = rr_proposed[w];
+                rr[2 * w + 1] = rr_accepted[w];
+            }
+            hip_check(hipMemcpy(s->rr.p, rr.data(), rr.size() * sizeof(double), hipMemcpyHostToDevice), "rr");
         }
-        hip_check(hipMemcpy(s->rr.p, rr.data(), rr.size() * sizeof(double), hipMemcpyHostToDevice), "rr");
         run_sweeps(s, n_sweeps, tau, tape, tape ? tape_stride : s->nw, true, n_accepted);
-        s->sync();
-        hip_check(hipMemcpy(rr.data(), s->rr.p, rr.size() * sizeof(double), hipMemcpyDeviceToHost), "rr");
-        for (size_t w = 0; w < nw; ++w) {
-            if (rr_proposed) rr_proposed[w] = rr[2 * w];
-            if (rr_accepted) rr_accepted[w] = rr[2 * w + 1];
+        if (rr_proposed) {
+            s->sync();
+            hip_check(hipMemcpy(rr.data(), s->rr.p, rr.size() * sizeof(double), hipMemcpyDeviceToHost), "rr");
+            for (size_t w = 0; w < nw; ++w) {
+                rr_proposed[w] = rr[2 * w];
+                rr_accepted[w] = rr[2 * w + 1];
+            }
         }
     });
 }
@@ -942,7 +968,60 @@ int alea_seed(alea_system* s, uint64_t seed, uint64_t stream) {
             for (uint64_t i = 0; i < 4; ++i) st[w * 4 + i] = splitmix64(sd + i * 0x9e3779b97f4a7c15ULL);
         }
         hip_check(hipMemcpy(s->rng.p, st.data(), st.size() * sizeof(uint64_t), hipMemcpyHostToDevice), "rng");
+        s->seeded = true;
+        s->seed_base = seed;
+        s->seed_stream = stream;
+        s->next_stream_index = static_cast<uint64_t>(s->capacity);
     });
+}
+
+// WalkerSet::apply_permutation + TrialWavefunction::permute_caches
+// (walker_set.cpp:109-178): every per-walker record is gathered from src[j].
+// The inverse records (the largest) are gathered into a spare buffer that then
+// swaps with the live one; the others go through one scratch buffer.  Both
+// are allocated on first use and kept.
+static void permute_device(alea_system* s, int new_nw, const int* src_d) {
+    const size_t cap = s->capacity;
+    const DevModel& m = s->host;
+    // (record, bytes per walker); every size is a whole number of doubles
+    const std::pair<void*, size_t> recs[] = {
+        {s->pos.p, sizeof(double) * m.np * 2},
+        {s->rho.p, sizeof(double) * m.ng * m.nkj * 2},
+        {s->logs.p, sizeof(double) * 4},
+        {s->row_abs.p, sizeof(double) * kMaxBlocks * s->st.nb_max},
+        {s->cond.p, sizeof(double) * kMaxBlocks},
+        {s->updates.p, sizeof(int) * kMaxBlocks},
+        {s->e_loc.p, sizeof(double)},
+        {s->e_loc_im.p, sizeof(double)},
+        {s->rr.p, sizeof(double) * 2},
+        {s->weights.p, sizeof(double)},
+        {s->e_prev.p, sizeof(double)},
+        {s->ncut.p, sizeof(long long)},
+        {s->rng.p, sizeof(uint64_t) * 4},
+    };
+    static_assert(sizeof(int) * kMaxBlocks % sizeof(double) == 0, "update counters pack into doubles");
+    size_t widest = 0;
+    for (const auto& r : recs) widest = std::max(widest, r.second / sizeof(double));
+    if (s->perm_tmp.n < cap * widest) s->perm_tmp.alloc(cap * widest);   // before any gather is queued
+    for (const auto& r : recs) {
+        if (r.second == 0) continue;
+        launch_gather(static_cast<double*>(r.first), s->perm_tmp.p, static_cast<int>(r.second / sizeof(double)),
+                      src_d, new_nw, s->stream);
+        hip_check(hipMemcpyAsync(r.first, s->perm_tmp.p, static_cast<size_t>(new_nw) * r.second,
+                                 hipMemcpyDeviceToDevice, s->stream), "permute copy");
+    }
+    if (s->st.inv_stride) {
+        const size_t bytes = static_cast<size_t>(s->st.inv_stride) * s->tsize();
+        if (s->inv_spare.n < s->inv_bytes.n) s->inv_spare.alloc(s->inv_bytes.n);
+        launch_gather(reinterpret_cast<const double*>(s->inv_bytes.p), reinterpret_cast<double*>(s->inv_spare.p),
+                      static_cast<int>(bytes / sizeof(double)), src_d, new_nw, s->stream);
+        std::swap(s->inv_bytes.p, s->inv_spare.p);
+        std::swap(s->inv_bytes.n, s->inv_spare.n);
+        s->st.inv = s->inv_bytes.p;
+    }
+    hip_check(hipGetLastError(), "launch gather");
+    s->sync();
+    s->nw = new_nw;
 }
 
 int alea_permute(alea_system* s, int new_nw, const int* src) {
@@ -953,30 +1032,195 @@ int alea_permute(alea_system* s, int new_nw, const int* src) {
         for (int j = 0; j < new_nw; ++j)
             if (src[j] < 0 || src[j] >= s->nw) throw InvalidArg("permutation source out of range");
         hip_check(hipMemcpy(s->src.p, src, new_nw * sizeof(int), hipMemcpyHostToDevice), "src");
-        auto gather = [&](double* base, size_t stride_doubles) {
-            if (stride_doubles == 0) return;
-            DevBuf<double> tmp;
-            tmp.alloc(static_cast<size_t>(new_nw) * stride_doubles);
-            launch_gather(base, tmp.p, static_cast<int>(stride_doubles), s->src.p, new_nw, s->stream);
-            hip_check(hipMemcpyAsync(base, tmp.p, tmp.n * sizeof(double), hipMemcpyDeviceToDevice, s->stream), "copy");
-            s->sync();
-        };
-        const DevModel& m = s->host;
-        gather(s->pos.p, static_cast<size_t>(m.np) * 2);
-        gather(s->rho.p, static_cast<size_t>(m.ng) * m.nkj * 2);
-        gather(s->logs.p, 4);
-        gather(s->row_abs.p, static_cast<size_t>(kMaxBlocks) * s->st.nb_max);
-        gather(s->cond.p, kMaxBlocks);
-        gather(s->e_loc.p, 1);
-        gather(s->e_loc_im.p, 1);
-        gather(reinterpret_cast<double*>(s->rng.p), 4);
-        if (s->st.inv_stride) {
-            const size_t bytes = static_cast<size_t>(s->st.inv_stride) * s->tsize();
-            gather(reinterpret_cast<double*>(s->inv_bytes.p), bytes / sizeof(double));
+        permute_device(s, new_nw, s->src.p);
+    });
+}
+
+int alea_get_num_walkers(alea_system* s, int* nw) {
+    return guarded([&] {
+        Sys(s);
+        *nw = s->nw;
+    });
+}
+
+// DMC::advance_measure's set-up of a step (dmc.cpp:349-354): rr sums to zero,
+// weights to one, E_old = the current local energies.
+int alea_dmc_begin(alea_system* s) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->energies_valid) measure(s, false);
+        hip_check(hipMemsetAsync(s->rr.p, 0, 2 * s->nw * sizeof(double), s->stream), "rr");
+        hip_check(hipMemsetAsync(s->ncut.p, 0, s->nw * sizeof(long long), s->stream), "ncut");
+        launch_fill(s->weights.p, s->nw, 1.0, s->stream);
+        hip_check(hipMemcpyAsync(s->e_prev.p, s->e_loc.p, s->nw * sizeof(double), hipMemcpyDeviceToDevice,
+                                 s->stream), "e_prev");
+        hip_check(hipGetLastError(), "launch fill");
+        s->sync();
+    });
+}
+
+// After one drift-diffusion sweep: H (refresh_derivatives + evaluate_batch,
+// dmc.cpp:383-384) and the bare branch factor of every walker (dmc.cpp:
+// 404-408).  e_trial, e_ref, e_cut, e_max and tau are PopulationControl's
+// state (population_control.h: etrial_, eref_, e_cut_, e_max_, tau_).
+int alea_dmc_weights(alea_system* s, double e_trial, double e_ref, double e_cut, double e_max, double tau) {
+    return guarded([&] {
+        Sys(s);
+        measure(s, false);
+        launch_dmc_weights(s->e_loc.p, s->e_prev.p, s->weights.p, s->ncut.p, s->nw, e_trial, e_ref, e_cut,
+                           e_max, 0.5 * tau, s->stream);
+        hip_check(hipGetLastError(), "launch weights");
+        s->sync();
+    });
+}
+
+// The step's (sum w E_L, sum w) in walker order (dmc.cpp:412-418) and the
+// number of tapered walker-sweeps.
+int alea_dmc_sums(alea_system* s, double* energy_sum, double* weight_sum, long long* n_cut) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->energies_valid) throw InvalidArg("no local energies for the current configurations");
+        s->sync();
+        const size_t nw = s->nw;
+        std::vector<double> w(nw), e(nw);
+        std::vector<long long> c(nw);
+        hip_check(hipMemcpy(w.data(), s->weights.p, nw * sizeof(double), hipMemcpyDeviceToHost), "weights");
+        hip_check(hipMemcpy(e.data(), s->e_loc.p, nw * sizeof(double), hipMemcpyDeviceToHost), "e_loc");
+        hip_check(hipMemcpy(c.data(), s->ncut.p, nw * sizeof(long long), hipMemcpyDeviceToHost), "ncut");
+        double es = 0.0, ws = 0.0;
+        long long nc = 0;
+        for (size_t i = 0; i < nw; ++i) {
+            es += w[i] * e[i];
+            ws += w[i];
+            nc += c[i];
         }
-        // updates: int pairs (kMaxBlocks = 2 ints = 1 double)
-        gather(reinterpret_cast<double*>(s->updates.p), kMaxBlocks * sizeof(int) / sizeof(double));
-        s->nw = new_nw;
+        if (energy_sum) *energy_sum = es;
+        if (weight_sum) *weight_sum = ws;
+        if (n_cut) *n_cut = nc;
+    });
+}
+
+int alea_get_weights(alea_system* s, double* w) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->built) throw InvalidArg("positions were never set");
+        s->sync();
+        hip_check(hipMemcpy(w, s->weights.p, s->nw * sizeof(double), hipMemcpyDeviceToHost), "weights");
+    });
+}
+
+int alea_set_weights(alea_system* s, const double* w) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->built) throw InvalidArg("positions were never set");
+        for (int i = 0; i < s->nw; ++i)
+            if (!(w[i] >= 0.0) || !std::isfinite(w[i])) throw InvalidArg("weights must be finite and non-negative");
+        hip_check(hipMemcpy(s->weights.p, w, s->nw * sizeof(double), hipMemcpyHostToDevice), "weights");
+    });
+}
+
+namespace {
+// XoshiroRandom::uniform (xoshiro_random.h:90-92) on a caller-held state.
+struct HostXoshiro {
+    uint64_t* st;
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next() {
+        const uint64_t result = rotl(st[0] + st[3], 23) + st[0];
+        const uint64_t t = st[1] << 17;
+        st[2] ^= st[0];
+        st[3] ^= st[1];
+        st[1] ^= st[2];
+        st[0] ^= st[3];
+        st[2] ^= t;
+        st[3] = rotl(st[3], 45);
+        return result;
+    }
+    double uniform() { return static_cast<double>(static_cast<int64_t>(next() >> 11)) * 0x1.0p-53; }
+};
+}  // namespace
+
+// WalkerSet::branch (walker_set.cpp:187-239): m_w = int(w_w + xi) copies in
+// walker order (xi drawn from rng_state in walker order), the bounds
+// [target/5 + 1, 2 target + 1] by random removal / duplication, the single
+// best walker on extinction; then the permutation and weights reset to one.
+// Multiplicities and the integer scan run on the device; the rare bound
+// fix-ups run here with the same stream.  Copies (second and later
+// occurrences of a source) get fresh device streams when the device RNG is
+// seeded.
+int alea_branch(alea_system* s, uint64_t* rng_state, int target, int* new_nw, int* src_out) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->evaluated) throw InvalidArg("evaluate first");
+        if (!rng_state) throw InvalidArg("null branch stream state");
+        if (target < 1) throw InvalidArg("target population must be positive");
+        const size_t n_max = 2 * static_cast<size_t>(target) + 1, n_min = static_cast<size_t>(target) / 5 + 1;
+        if (n_max > static_cast<size_t>(s->capacity))
+            throw InvalidArg("2 * target + 1 walkers exceed the capacity");
+        const int n = s->nw;
+        HostXoshiro rng{rng_state};
+        std::vector<double> xi(n);
+        for (int w = 0; w < n; ++w) xi[w] = rng.uniform();
+        hip_check(hipMemcpyAsync(s->xi.p, xi.data(), n * sizeof(double), hipMemcpyHostToDevice, s->stream), "xi");
+        launch_branch(s->weights.p, s->xi.p, s->src.p, s->capacity, n, s->branch_total.p, s->stream);
+        hip_check(hipGetLastError(), "launch branch");
+        long long total = 0;
+        hip_check(hipMemcpyAsync(&total, s->branch_total.p, sizeof(long long), hipMemcpyDeviceToHost, s->stream),
+                  "total");
+        s->sync();
+        std::vector<size_t> src;
+        if (total > 0 && total <= s->capacity) {
+            std::vector<int> d(total);
+            hip_check(hipMemcpy(d.data(), s->src.p, total * sizeof(int), hipMemcpyDeviceToHost), "src");
+            src.assign(d.begin(), d.end());
+        } else {
+            // slow path: recompute the list here (the same multiplicities)
+            std::vector<double> w(n);
+            hip_check(hipMemcpy(w.data(), s->weights.p, n * sizeof(double), hipMemcpyDeviceToHost), "weights");
+            if (total == 0) {
+                size_t best = 0;
+                for (int i = 1; i < n; ++i)
+                    if (w[i] > w[best]) best = i;
+                src.push_back(best);
+            } else {
+                for (int i = 0; i < n; ++i) {
+                    const int m = static_cast<int>(w[i] + xi[i]);
+                    for (int c = 0; c < m; ++c) src.push_back(i);
+                }
+            }
+        }
+        while (src.size() > n_max) {
+            const size_t i = static_cast<size_t>(rng.uniform() * static_cast<double>(src.size()));
+            src[i] = src.back();
+            src.pop_back();
+        }
+        while (src.size() < n_min) {
+            const size_t i = static_cast<size_t>(rng.uniform() * static_cast<double>(src.size()));
+            src.push_back(src[i]);
+        }
+        const int nn = static_cast<int>(src.size());
+        std::vector<int> srci(src.begin(), src.end());
+        hip_check(hipMemcpy(s->src.p, srci.data(), nn * sizeof(int), hipMemcpyHostToDevice), "src");
+        permute_device(s, nn, s->src.p);
+        launch_fill(s->weights.p, nn, 1.0, s->stream);
+        hip_check(hipGetLastError(), "launch fill");
+        if (s->seeded) {
+            std::vector<char> seen(n, 0);
+            std::vector<uint64_t> st(static_cast<size_t>(nn) * 4);
+            hip_check(hipMemcpy(st.data(), s->rng.p, st.size() * sizeof(uint64_t), hipMemcpyDeviceToHost), "rng");
+            for (int j = 0; j < nn; ++j) {
+                if (!seen[srci[j]]) {
+                    seen[srci[j]] = 1;
+                    continue;
+                }
+                const uint64_t sd = splitmix64(splitmix64(splitmix64(s->seed_base) ^ s->seed_stream) ^
+                                               s->next_stream_index++);
+                for (uint64_t i = 0; i < 4; ++i) st[j * 4 + i] = splitmix64(sd + i * 0x9e3779b97f4a7c15ULL);
+            }
+            hip_check(hipMemcpy(s->rng.p, st.data(), st.size() * sizeof(uint64_t), hipMemcpyHostToDevice), "rng");
+        }
+        s->sync();
+        *new_nw = nn;
+        if (src_out) std::copy(srci.begin(), srci.end(), src_out);
     });
 }
 

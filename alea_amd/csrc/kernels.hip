@@ -1502,6 +1502,72 @@ __global__ void k_gather(const double* __restrict__ src_d, double* __restrict__ 
     for (int i = threadIdx.x; i < stride; i += blockDim.x) b[i] = a[i];
 }
 
+// One drift-diffusion sweep's branch factor (dmc.cpp:404-408): weight *=
+// PopulationControl::branch_weight(E_new, E_old) (population_control.cpp:
+// 58-67), the cut count of is_cut (population_control.h:189-191), then
+// E_old = E_new for the next sweep.
+__global__ void k_dmc_weights(const double* __restrict__ e_new, double* __restrict__ e_old,
+                              double* __restrict__ weight, long long* __restrict__ n_cut, int nw,
+                              double etrial, double eref, double e_cut, double e_max, double half_tau) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    const double en = e_new[w], eo = e_old[w];
+    double ht = half_tau;
+    const double x = fmax(eref - en, eref - eo);
+    if (x > e_max)
+        ht = 0.0;
+    else if (x > e_cut)
+        ht *= 1.0 - (x - e_cut) / (e_max - e_cut);
+    weight[w] *= exp(ht * (2.0 * etrial - en - eo));
+    if (eref - en > e_cut || eref - eo > e_cut) n_cut[w] += 1;
+    e_old[w] = en;
+}
+
+// Integer multiplicities m_w = int(weight_w + xi_w) (walker_set.cpp:193-197),
+// then one workgroup turns them into the copy list: an exclusive scan over
+// walkers in order and the expansion src[off_w + c] = w, c < m_w, so copies
+// of a walker are adjacent and walkers keep their order.  The total is
+// written even when it exceeds the list's capacity (the host then takes the
+// slow path).
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_branch(const double* __restrict__ weight,
+                                                        const double* __restrict__ xi,
+                                                        int* __restrict__ src, int src_cap, int nw,
+                                                        long long* __restrict__ total) {
+    __shared__ long long part[kScanThreads];
+    const int t = threadIdx.x;
+    const int per = (nw + kScanThreads - 1) / kScanThreads;
+    const int w0 = min(nw, t * per), w1 = min(nw, w0 + per);
+    auto mult = [&](int w) -> long long {
+        const double v = weight[w] + xi[w];
+        return v < 2147483647.0 ? static_cast<long long>(static_cast<int>(v)) : 2147483647LL;
+    };
+    long long sum = 0;
+    for (int w = w0; w < w1; ++w) sum += mult(w);
+    part[t] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over the per-thread sums
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const long long v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long long pos = part[t] - sum;
+    if (t == kScanThreads - 1) total[0] = part[t];
+    for (int w = w0; w < w1; ++w) {
+        const long long m = mult(w);
+        for (long long c = 0; c < m && pos + c < src_cap; ++c) src[pos + c] = w;
+        pos += m;
+    }
+}
+
+// Fill a per-walker double array with a value (weights reset after branching).
+__global__ void k_fill(double* __restrict__ a, int n, double v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 }  // namespace
 
 // ── host launch wrappers ───────────────────────────────────────────────────
@@ -1563,6 +1629,22 @@ void launch_measure(const DevModel* m, const DevState& s, int nw, bool mixed, bo
 void launch_gather(const double* src_d, double* dst_d, int stride, const int* src, int n_new,
                    hipStream_t st) {
     hipLaunchKernelGGL(k_gather, dim3(n_new), dim3(256), 0, st, src_d, dst_d, stride, src, n_new);
+}
+
+void launch_dmc_weights(const double* e_new, double* e_old, double* weight, long long* n_cut, int nw,
+                        double etrial, double eref, double e_cut, double e_max, double half_tau,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_dmc_weights, dim3((nw + 255) / 256), dim3(256), 0, st, e_new, e_old, weight,
+                       n_cut, nw, etrial, eref, e_cut, e_max, half_tau);
+}
+
+void launch_branch(const double* weight, const double* xi, int* src, int src_cap, int nw,
+                   long long* total, hipStream_t st) {
+    hipLaunchKernelGGL(k_branch, dim3(1), dim3(kScanThreads), 0, st, weight, xi, src, src_cap, nw, total);
+}
+
+void launch_fill(double* a, int n, double v, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, st, a, n, v);
 }
 
 }  // namespace alea_amd

@@ -1,0 +1,185 @@
+"""Host side of the DMC population step over a walker batch.
+
+The per-walker work (drift-diffusion sweeps, local energies, branch weights,
+integer multiplicities, the permutation) runs where the walkers live, behind
+the C-ABI.  Here are the scalar pieces the reference keeps on the host:
+
+  PopulationControl   qmc/population_control.cpp:14-108 (E_T feedback, the
+                      branch-weight cutoff, start / update)
+  robust_spread       numerics/statistics.cpp:297-315 (1.4826 MAD)
+  DMC.step            DMC::run_one_step with one crowd (dmc.cpp:693-739) over
+                      DMC::advance_measure with bare reweighting
+                      (dmc.cpp:339-419)
+
+Streams: branching draws from stream_seed(seed, "dmc-brch", 0)
+(dmc.cpp:79, dmc.h:36), uniforms as XoshiroRandom::uniform.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+K_DMC_THREAD_STREAM = 0x646d632d74687264    # "dmc-thrd"
+K_DMC_BRANCH_STREAM = 0x646d632d62726368    # "dmc-brch"
+K_DEFAULT_BRANCH_CUTOFF_ALPHA = 2.0
+_M64 = (1 << 64) - 1
+_GAMMA = 0x9e3779b97f4a7c15
+
+
+def splitmix64(x: int) -> int:
+    """numerics/random.h:39-44."""
+    x = (x + _GAMMA) & _M64
+    x = ((x ^ (x >> 30)) * 0xbf58476d1ce4e5b9) & _M64
+    x = ((x ^ (x >> 27)) * 0x94d049bb133111eb) & _M64
+    return x ^ (x >> 31)
+
+
+def stream_seed(base: int, stream: int, index: int) -> int:
+    """numerics/random.h:52-56."""
+    return splitmix64(splitmix64(splitmix64(base) ^ stream) ^ index)
+
+
+def xoshiro_state(seed: int) -> np.ndarray:
+    """Xoshiro256pp(seed): four consecutive splitmix64 outputs (xoshiro_random.h:26-29)."""
+    return np.array([splitmix64((seed + i * _GAMMA) & _M64) for i in range(4)], dtype=np.uint64)
+
+
+def _median_in_place(v: np.ndarray) -> float:
+    """statistics.cpp:297-306: the middle value, or the mean of the two middle ones."""
+    n = v.size
+    mid = n // 2
+    part = np.partition(v, mid)
+    upper = part[mid]
+    if n % 2 == 1:
+        return float(upper)
+    lower = part[:mid].max()
+    return 0.5 * (float(lower) + float(upper))
+
+
+def robust_spread(x) -> float:
+    """1.4826 x the median absolute deviation (statistics.cpp:309-315)."""
+    x = np.asarray(x, dtype=np.float64)
+    if x.size == 0:
+        return 0.0
+    centre = _median_in_place(x.copy())
+    return 1.4826 * _median_in_place(np.abs(x - centre))
+
+
+class PopulationControl:
+    """qmc/population_control.cpp:14-108; cutoff is "universal", "drv" or "none"."""
+
+    def __init__(self, tau, n_substeps, generations, target_walkers, cutoff="universal",
+                 alpha=K_DEFAULT_BRANCH_CUTOFF_ALPHA, n_particles=1):
+        if cutoff == "universal" and not (math.isfinite(alpha) and alpha > 0.0):
+            raise ValueError("dmc: the universal branch cutoff needs a finite alpha > 0")
+        if n_particles == 0:
+            raise ValueError("dmc: the branch cutoff needs at least one particle")
+        self.tau = tau
+        self.cutoff = cutoff
+        self.alpha = alpha
+        self.n_particles = float(n_particles)
+        self.target_walkers = target_walkers
+        t_step = tau * max(1, n_substeps)
+        self.feedback_rate = 1.0 / (generations * t_step) if (t_step > 0.0 and generations > 0.0) else 0.0
+        self.e_cut = math.inf
+        self.e_max = math.inf
+        self.sigma = 0.0
+        self.etrial = self.eref = 0.0
+        self.eref_sum = self.eref_wsum = 0.0
+        if cutoff == "drv" and tau > 0.0:
+            self._set_cutoff(2.0 / math.sqrt(tau))
+
+    def _set_cutoff(self, e_cut):
+        self.e_cut = e_cut
+        self.e_max = 1.5 * e_cut
+
+    def set_energy_spread(self, sigma):
+        self.sigma = sigma
+        if self.cutoff != "universal":
+            return
+        if not (math.isfinite(sigma) and sigma > 0.0 and self.tau > 0.0):
+            self._set_cutoff(math.inf)
+            return
+        self._set_cutoff(self.alpha * math.sqrt(sigma * math.sqrt(self.n_particles) / self.tau))
+
+    def start(self, energy):
+        self.etrial = self.eref = energy
+        self.eref_sum = self.eref_wsum = 0.0
+
+    def branch_weight(self, e_new, e_old):
+        half_tau = 0.5 * self.tau
+        x = max(self.eref - e_new, self.eref - e_old)
+        if x > self.e_max:
+            half_tau = 0.0
+        elif x > self.e_cut:
+            half_tau *= 1.0 - (x - self.e_cut) / (self.e_max - self.e_cut)
+        return math.exp(half_tau * (2.0 * self.etrial - e_new - e_old))
+
+    def is_cut(self, e_new, e_old):
+        return self.eref - e_new > self.e_cut or self.eref - e_old > self.e_cut
+
+    def update(self, ensemble_energy, total_weight, accumulate):
+        if accumulate:
+            self.eref_sum += ensemble_energy * total_weight
+            self.eref_wsum += total_weight
+            if self.eref_wsum > 0.0:
+                self.eref = self.eref_sum / self.eref_wsum
+        else:
+            self.eref = ensemble_energy
+        if total_weight > 0.0:
+            self.etrial = self.eref - self.feedback_rate * math.log(total_weight / self.target_walkers)
+
+    def begin_accumulation(self):
+        self.eref_sum = self.eref_wsum = 0.0
+
+
+class DMC:
+    """One crowd of DMC::run_one_step over a Walkers batch (product or oracle).
+
+    tapes(step, substep, nw) -> the sweep's random numbers [np][dim+1][stride]
+    with stride = the batch capacity, or None for the batch's own streams.
+    """
+
+    def __init__(self, walkers, tau, target, n_substeps=1, generations=10.0, cutoff="universal",
+                 alpha=K_DEFAULT_BRANCH_CUTOFF_ALPHA, seed=42, tapes=None, stride=None):
+        self.w = walkers
+        self.tau = tau
+        self.target = target
+        self.n_substeps = n_substeps
+        self.tapes = tapes
+        self.stride = stride or walkers.capacity
+        self.pc = PopulationControl(tau, n_substeps, generations, target, cutoff, alpha,
+                                    walkers.np)
+        self.branch_state = xoshiro_state(stream_seed(seed, K_DMC_BRANCH_STREAM, 0))
+        self.step_index = 0
+        e, _ = walkers.local_energy()
+        self.pc.start(float(np.sum(e)) / e.size)    # dmc.cpp:628-635, summed in walker order
+        self.refresh_energy_spread(e)
+        self.history = []
+
+    def refresh_energy_spread(self, e=None):
+        """dmc.cpp:640-644: sigma(E_L) of the current population."""
+        if e is None:
+            e, _ = self.w.local_energy()
+        self.pc.set_energy_spread(robust_spread(e))
+
+    def step(self, measure=False):
+        """advance_measure -> branch -> E_T update; returns (sum w E_L, sum w)."""
+        w, pc = self.w, self.pc
+        w.dmc_begin()
+        n_acc = 0
+        for sub in range(self.n_substeps):
+            tape = None if self.tapes is None else self.tapes(self.step_index, sub, w.nw)
+            n_acc += w.dmc_sweeps(1, self.tau, tape, self.stride if tape is not None else 0)
+            w.dmc_weights(pc.etrial, pc.eref, pc.e_cut, pc.e_max, pc.tau)
+        e_sum, w_sum, n_cut = w.dmc_sums()
+        ens_e = e_sum / w_sum if w_sum > 0.0 else pc.etrial
+        nw_before = w.nw
+        src = w.branch(self.branch_state, self.target)
+        pc.update(ens_e, w_sum, measure)
+        self.history.append(dict(step=self.step_index, nw_before=nw_before, nw=w.nw, e_sum=e_sum,
+                                 w_sum=w_sum, n_cut=n_cut, n_accepted=n_acc, etrial=pc.etrial,
+                                 src=src))
+        self.step_index += 1
+        return e_sum, w_sum

@@ -15,7 +15,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._capi import CApi, f64, i32, ptr, u8, _P_I, _P_U8
+from ._capi import CApi, f64, i32, ptr, u8, _P_I, _P_U8, _P_U64
 from .system import Spec
 
 FP64, MIXED = 0, 1
@@ -26,6 +26,7 @@ class Walkers:
         self.api = api
         self.spec = spec
         self.nw = nw
+        self.capacity = nw          # alea_create's walker count is the capacity
         self.np = spec.n_particles
         self.dim = 2
         h = C.c_void_p()
@@ -143,16 +144,54 @@ class Walkers:
         self.api("vmc_sweeps", self.h, n_sweeps, tau, ptr(t), C.byref(acc))
         return acc.value
 
-    def dmc_sweeps(self, n_sweeps, tau, tape, stride, rr_p, rr_a):
+    def dmc_sweeps(self, n_sweeps, tau, tape=None, stride=0, rr_p=None, rr_a=None):
+        """rr_p / rr_a None: the rr sums accumulate where the walkers live."""
         acc = C.c_longlong()
-        t = f64(tape)
-        self.api("dmc_sweeps", self.h, n_sweeps, tau, ptr(t), stride, ptr(rr_p), ptr(rr_a), C.byref(acc))
+        t = None if tape is None else f64(tape)
+        self.api("dmc_sweeps", self.h, n_sweeps, tau, ptr(t), stride or self.nw, ptr(rr_p), ptr(rr_a),
+                 C.byref(acc))
         return acc.value
 
     def permute(self, src):
         s = i32(src)
         self.api("permute", self.h, len(s), ptr(s, _P_I))
         self.nw = len(s)
+
+    def num_walkers(self):
+        n = C.c_int()
+        self.api("get_num_walkers", self.h, C.byref(n))
+        return n.value
+
+    # ── DMC population step ────────────────────────────────────────────────
+    def dmc_begin(self):
+        self.api("dmc_begin", self.h)
+
+    def dmc_weights(self, e_trial, e_ref, e_cut, e_max, tau):
+        self.api("dmc_weights", self.h, e_trial, e_ref, e_cut, e_max, tau)
+
+    def dmc_sums(self):
+        e, w, c = C.c_double(), C.c_double(), C.c_longlong()
+        self.api("dmc_sums", self.h, C.byref(e), C.byref(w), C.byref(c))
+        return e.value, w.value, c.value
+
+    def weights(self):
+        out = np.empty(self.nw)
+        self.api("get_weights", self.h, ptr(out))
+        return out
+
+    def set_weights(self, w):
+        a = f64(w)
+        assert a.size == self.nw
+        self.api("set_weights", self.h, ptr(a))
+
+    def branch(self, rng_state, target):
+        """WalkerSet::branch: rng_state (uint64[4]) advances in place; returns src."""
+        assert rng_state.dtype == np.uint64 and rng_state.size == 4 and rng_state.flags.c_contiguous
+        src = np.empty(2 * target + 1, dtype=np.int32)
+        n = C.c_int()
+        self.api("branch", self.h, rng_state.ctypes.data_as(_P_U64), target, C.byref(n), ptr(src, _P_I))
+        self.nw = n.value
+        return src[:n.value].copy()
 
     def close(self):
         if self.h:

@@ -129,9 +129,11 @@ int alea_accept(alea_system* s, int p, const unsigned char* mask);
 int alea_vmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape,
                     long long* n_accepted);
 /* DMC::sweep_walkers (dmc.cpp:203-337): Umrigar-scaled drift, node gate for a
- * real trial, Metropolis-Hastings with the drift Green's function; rr_proposed
- * and rr_accepted [nw] are accumulated (may be NULL).  tape rows have stride
- * tape_stride >= nw (the reference strides by the population capacity). */
+ * real trial, Metropolis-Hastings with the drift Green's function.
+ * rr_proposed and rr_accepted [nw] are accumulated into the given host arrays;
+ * with both NULL they accumulate on the device (reset by alea_dmc_begin).
+ * tape rows have stride tape_stride >= nw (the reference strides by the
+ * population capacity). */
 int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape, int tape_stride,
                     double* rr_proposed, double* rr_accepted, long long* n_accepted);
 
@@ -158,6 +160,37 @@ int alea_seed(alea_system* s, uint64_t seed, uint64_t stream);
  * (walker_set.cpp:109-178): new walker j takes old walker src[j]'s state;
  * new_nw <= capacity. */
 int alea_permute(alea_system* s, int new_nw, const int* src);
+/* WalkerSet::num_walkers (walker_set.h): the current population. */
+int alea_get_num_walkers(alea_system* s, int* nw);
+
+/* ── DMC population step (DMC::advance_measure, dmc.cpp:339-419, bare
+ * reweighting; DMC::run_one_step, dmc.cpp:693-739).  A step is:
+ *   alea_dmc_begin; n_substeps x { alea_dmc_sweeps(rr NULL); alea_dmc_weights };
+ *   alea_dmc_sums; alea_branch; PopulationControl::update on the host. */
+
+/* Step set-up (dmc.cpp:349-354): rr sums to zero, weights to one, E_old to
+ * the current local energies (evaluated here if not current). */
+int alea_dmc_begin(alea_system* s);
+/* After a drift-diffusion sweep: evaluate E_L (hamiltonian.cpp:18) and apply
+ * weight *= PopulationControl::branch_weight(E_new, E_old)
+ * (population_control.cpp:58-67) with the controller's state e_trial, e_ref,
+ * e_cut, e_max = 1.5 e_cut and tau; then E_old = E_new (dmc.cpp:404-408). */
+int alea_dmc_weights(alea_system* s, double e_trial, double e_ref, double e_cut, double e_max,
+                     double tau);
+/* (sum_w weight*E_L, sum_w weight) in walker order (dmc.cpp:412-418) and the
+ * walker-sweeps tapered by the cutoff since alea_dmc_begin (any may be NULL). */
+int alea_dmc_sums(alea_system* s, double* energy_sum, double* weight_sum, long long* n_cut);
+/* WalkerSet::weights (walker_set.h:135) download / upload, [nw]. */
+int alea_get_weights(alea_system* s, double* w);
+int alea_set_weights(alea_system* s, const double* w);
+/* WalkerSet::branch(rng, target, src) (walker_set.cpp:187-239): integer
+ * multiplicities int(weight + xi) in walker order, population bounds
+ * [target/5 + 1, 2 target + 1], the best walker on extinction; then
+ * apply_permutation and weights reset to one.  rng_state is the branch
+ * stream's xoshiro256++ state (4 words), advanced in place exactly as
+ * XoshiroRandom::uniform draws it.  src_out (may be NULL) receives the
+ * new_nw source indices; 2 target + 1 must not exceed the capacity. */
+int alea_branch(alea_system* s, uint64_t* rng_state, int target, int* new_nw, int* src_out);
 
 #ifdef __cplusplus
 }

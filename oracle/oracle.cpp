@@ -529,6 +529,10 @@ struct Sys {
     std::vector<CoulombPair> coulomb;
     std::vector<double> inv_mass;  // [np]
     std::vector<double> e_loc, e_loc_im;
+    // DMC population state (walker_set.h:135-173): weights, E_L before the
+    // sweep, rr sums, tapered walker-sweeps
+    std::vector<double> weights, e_prev, rr_p, rr_a;
+    std::vector<long long> n_cut;
     bool has_phase = false;
 
     double& P(std::size_t p, std::size_t d, std::size_t w) { return pos[(p * dim + d) * nw + w]; }
@@ -547,6 +551,11 @@ struct Sys {
         new_plap.assign(dim * nw, 0.0);
         e_loc.assign(nw, 0.0);
         e_loc_im.assign(nw, 0.0);
+        weights.assign(nw, 1.0);
+        e_prev.assign(nw, 0.0);
+        rr_p.assign(nw, 0.0);
+        rr_a.assign(nw, 0.0);
+        n_cut.assign(nw, 0);
     }
     // BatchedDistanceTable::build_all (batched_distance_table.cpp:20-58)
     void build_dist() {
@@ -2023,8 +2032,8 @@ int oracle_dmc_sweeps(oracle_system* h, int n_sweeps, double tau, const double* 
                         const double dr = s.new_pos[d * nw + w] - s.P(p, d, w);
                         rr += tm * dr * dr;
                     }
-                    rr_proposed[w] += rr;
-                    if (m[w] != 0.0) rr_accepted[w] += rr;
+                    (rr_proposed ? rr_proposed : s.rr_p.data())[w] += rr;
+                    if (m[w] != 0.0) (rr_accepted ? rr_accepted : s.rr_a.data())[w] += rr;
                 }
                 accept_all(s, p, m);
             }
@@ -2047,12 +2056,140 @@ int oracle_permute(oracle_system* h, int new_nw, const int* src) {
         }
         Component::gather(s.pos, sv, old_nw);
         Component::gather(s.dist, sv, old_nw);
+        for (auto* v : {&s.e_loc, &s.e_loc_im, &s.weights, &s.e_prev, &s.rr_p, &s.rr_a})
+            Component::gather(*v, sv, old_nw);
+        {
+            std::vector<long long> c(sv.size());
+            for (std::size_t j = 0; j < sv.size(); ++j) c[j] = s.n_cut[sv[j]];
+            s.n_cut = c;
+        }
         for (auto& c : s.comps) c->permute(sv, old_nw);
         s.nw = sv.size();
-        std::vector<double> pos = s.pos, dist = s.dist;
+        const std::vector<double> pos = s.pos, dist = s.dist, el = s.e_loc, eli = s.e_loc_im,
+                                  wt = s.weights, ep = s.e_prev, rp = s.rr_p, ra = s.rr_a;
+        const std::vector<long long> nc = s.n_cut;
         s.alloc();
         s.pos = pos;
         s.dist = dist;
+        s.e_loc = el;
+        s.e_loc_im = eli;
+        s.weights = wt;
+        s.e_prev = ep;
+        s.rr_p = rp;
+        s.rr_a = ra;
+        s.n_cut = nc;
+    });
+}
+
+int oracle_get_num_walkers(oracle_system* h, int* nw) {
+    return guarded([&] { *nw = static_cast<int>(S(h)->nw); });
+}
+
+// DMC::advance_measure set-up (dmc.cpp:349-354).
+int oracle_dmc_begin(oracle_system* h) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        local_energy(s);
+        for (std::size_t w = 0; w < s.nw; ++w) {
+            s.rr_p[w] = 0.0;
+            s.rr_a[w] = 0.0;
+            s.weights[w] = 1.0;
+            s.e_prev[w] = s.e_loc[w];
+            s.n_cut[w] = 0;
+        }
+    });
+}
+
+// H after a sweep and the bare branch factor (dmc.cpp:383-384, 404-408;
+// PopulationControl::branch_weight and is_cut, population_control.cpp:58-67,
+// population_control.h:189-191).
+int oracle_dmc_weights(oracle_system* h, double e_trial, double e_ref, double e_cut, double e_max,
+                       double tau) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        local_energy(s);
+        for (std::size_t w = 0; w < s.nw; ++w) {
+            const double e_new = s.e_loc[w], e_old = s.e_prev[w];
+            double half_tau = 0.5 * tau;
+            const double x = std::max(e_ref - e_new, e_ref - e_old);
+            if (x > e_max)
+                half_tau = 0.0;
+            else if (x > e_cut)
+                half_tau *= 1.0 - (x - e_cut) / (e_max - e_cut);
+            s.weights[w] *= std::exp(half_tau * (2.0 * e_trial - e_new - e_old));
+            if (e_ref - e_new > e_cut || e_ref - e_old > e_cut) ++s.n_cut[w];
+            s.e_prev[w] = e_new;
+        }
+    });
+}
+
+// (sum w E_L, sum w) in walker order (dmc.cpp:412-418).
+int oracle_dmc_sums(oracle_system* h, double* energy_sum, double* weight_sum, long long* n_cut) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        double es = 0.0, ws = 0.0;
+        long long nc = 0;
+        for (std::size_t w = 0; w < s.nw; ++w) {
+            es += s.weights[w] * s.e_loc[w];
+            ws += s.weights[w];
+            nc += s.n_cut[w];
+        }
+        if (energy_sum) *energy_sum = es;
+        if (weight_sum) *weight_sum = ws;
+        if (n_cut) *n_cut = nc;
+    });
+}
+
+int oracle_get_weights(oracle_system* h, double* w) {
+    return guarded([&] { std::copy(S(h)->weights.begin(), S(h)->weights.end(), w); });
+}
+
+int oracle_set_weights(oracle_system* h, const double* w) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        for (std::size_t i = 0; i < s.nw; ++i)
+            if (!(w[i] >= 0.0) || !std::isfinite(w[i]))
+                throw std::invalid_argument("oracle: weights must be finite and non-negative");
+        std::copy(w, w + s.nw, s.weights.begin());
+    });
+}
+
+// WalkerSet::branch (walker_set.cpp:187-239) with the branch stream's state
+// held by the caller (XoshiroRandom::uniform, xoshiro_random.h:90-92).
+int oracle_branch(oracle_system* h, uint64_t* rng_state, int target, int* new_nw, int* src_out) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        if (target < 1) throw std::invalid_argument("oracle: target population must be positive");
+        Xoshiro eng(static_cast<const uint64_t*>(rng_state));
+        auto uniform = [&] { return static_cast<double>(static_cast<int64_t>(eng.next() >> 11)) * 0x1.0p-53; };
+        const std::size_t n = s.nw, tgt = static_cast<std::size_t>(target);
+        std::vector<std::size_t> src;
+        for (std::size_t w = 0; w < n; ++w) {
+            const int m = static_cast<int>(s.weights[w] + uniform());
+            for (int c = 0; c < m; ++c) src.push_back(w);
+        }
+        const std::size_t n_max = 2 * tgt + 1, n_min = tgt / 5 + 1;
+        if (src.empty()) {
+            std::size_t best = 0;
+            for (std::size_t w = 1; w < n; ++w)
+                if (s.weights[w] > s.weights[best]) best = w;
+            src.push_back(best);
+        }
+        while (src.size() > n_max) {
+            const std::size_t i = static_cast<std::size_t>(uniform() * static_cast<double>(src.size()));
+            src[i] = src.back();
+            src.pop_back();
+        }
+        while (src.size() < n_min) {
+            const std::size_t i = static_cast<std::size_t>(uniform() * static_cast<double>(src.size()));
+            src.push_back(src[i]);
+        }
+        std::copy(eng.s, eng.s + 4, rng_state);
+        std::vector<int> si(src.begin(), src.end());
+        if (oracle_permute(h, static_cast<int>(si.size()), si.data()) != 0) throw std::runtime_error(g_error);
+        std::fill(s.weights.begin(), s.weights.end(), 1.0);
+        *new_nw = static_cast<int>(si.size());
+        if (src_out) std::copy(si.begin(), si.end(), src_out);
     });
 }
 

@@ -69,6 +69,14 @@ int oracle_dmc_sweeps(oracle_system* s, int n_sweeps, double tau, const double* 
                       int tape_stride, double* rr_proposed, double* rr_accepted,
                       long long* n_accepted);
 int oracle_permute(oracle_system* s, int new_nw, const int* src);
+int oracle_get_num_walkers(oracle_system* s, int* nw);
+int oracle_dmc_begin(oracle_system* s);
+int oracle_dmc_weights(oracle_system* s, double e_trial, double e_ref, double e_cut, double e_max,
+                       double tau);
+int oracle_dmc_sums(oracle_system* s, double* energy_sum, double* weight_sum, long long* n_cut);
+int oracle_get_weights(oracle_system* s, double* w);
+int oracle_set_weights(oracle_system* s, const double* w);
+int oracle_branch(oracle_system* s, uint64_t* rng_state, int target, int* new_nw, int* src_out);
 
 #ifdef __cplusplus
 }

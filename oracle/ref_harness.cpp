@@ -33,8 +33,11 @@
 #include "numerics/pair_radial.h"
 #include "numerics/random.h"
 #include "numerics/xoshiro_random.h"
+#include "numerics/statistics.h"
 #include "population/simulation_cell.h"
+#include "population/walker.h"
 #include "population/walker_set.h"
+#include "qmc/population_control.h"
 #include "qmc/sweep_blocking.h"
 #include "wavefunction/drummond_jastrow.h"
 #include "wavefunction/nosanow_jastrow.h"
@@ -432,6 +435,67 @@ int ref_vmc_sweeps(void* h, int n_sweeps, double tau, const double* tape,
         }
         *n_accepted = acc;
     });
+}
+
+// WalkerSet::branch (walker_set.cpp:187-239) on a one-particle walker set
+// whose walkers carry the given weights; rng = XoshiroRandom(seed).  Writes
+// the source map (the walker tagged w has x = w) and the next uniform the
+// stream would give after the branch.
+int ref_branch(const double* weights, int nw, uint64_t seed, int target, int* src_out,
+               int* new_nw, double* next_uniform) {
+    return guarded([&] {
+        alea::WalkerSet ws(static_cast<std::size_t>(nw), 1, 2);
+        std::vector<alea::Walker> walkers;
+        walkers.reserve(static_cast<std::size_t>(nw));
+        for (int w = 0; w < nw; ++w) walkers.emplace_back(1, 2);
+        for (auto& w : walkers) ws.add_walker(w);
+        ws.load_all();
+        auto pos = ws.positions();
+        for (int w = 0; w < nw; ++w) pos[w] = static_cast<double>(w);   // x of particle 0
+        for (int w = 0; w < nw; ++w) ws.weights()[static_cast<std::size_t>(w)] = weights[w];
+        alea::numerics::XoshiroRandom rng(seed);
+        (void)ws.branch(rng, static_cast<std::size_t>(target));
+        const std::size_t n = ws.num_walkers();
+        auto p = ws.positions();
+        for (std::size_t j = 0; j < n; ++j) src_out[j] = static_cast<int>(p[j]);
+        *new_nw = static_cast<int>(n);
+        *next_uniform = rng.uniform();
+    });
+}
+
+// PopulationControl (population_control.cpp) driven through a scripted run:
+// start(e0), set_energy_spread(sigma); then per step i the branch weights of
+// n_pairs (e_new, e_old) pairs (and is_cut) and update(ens_e[i], wsum[i],
+// accumulate[i]).  Writes the weights [n_steps][n_pairs], the cut flags, E_T
+// after every update (which fixes E_ref) and e_cut.
+int ref_popctl_run(double tau, int n_substeps, double generations, int target, int cutoff,
+                   double alpha, int n_particles, double e0, double sigma, int n_steps,
+                   const double* ens_e, const double* wsum, const int* accumulate, int n_pairs,
+                   const double* e_new, const double* e_old, double* weights_out, int* cut_out,
+                   double* etrial_out, double* ecut_out) {
+    return guarded([&] {
+        const alea::BranchCutoff kinds[3] = {alea::BranchCutoff::none, alea::BranchCutoff::drv,
+                                             alea::BranchCutoff::universal};
+        alea::PopulationControl pc(tau, n_substeps, generations, static_cast<std::size_t>(target),
+                                   kinds[cutoff], alpha, static_cast<std::size_t>(n_particles));
+        pc.start(e0);
+        pc.set_energy_spread(sigma);
+        *ecut_out = pc.energy_cutoff();
+        for (int i = 0; i < n_steps; ++i) {
+            for (int k = 0; k < n_pairs; ++k) {
+                weights_out[i * n_pairs + k] = pc.branch_weight(e_new[k], e_old[k]);
+                cut_out[i * n_pairs + k] = pc.is_cut(e_new[k], e_old[k]) ? 1 : 0;
+            }
+            pc.update(ens_e[i], wsum[i], accumulate[i] != 0);
+            etrial_out[i] = pc.etrial();
+        }
+    });
+}
+
+// numerics::robust_spread (statistics.cpp:309-315).
+double ref_robust_spread(const double* x, int n) {
+    std::vector<double> scratch;
+    return alea::numerics::robust_spread(std::span<const double>(x, static_cast<std::size_t>(n)), scratch);
 }
 
 }  // extern "C"

@@ -90,6 +90,12 @@ def ref_lib():
             "ref_ratio": (C.c_int, [C.c_void_p, C.c_int, D, D, D, D]),
             "ref_accept": (C.c_int, [C.c_void_p, C.c_int, _P_U8]),
             "ref_vmc_sweeps": (C.c_int, [C.c_void_p, C.c_int, C.c_double, D, C.POINTER(C.c_longlong)]),
+            "ref_branch": (C.c_int, [D, C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int), D]),
+            "ref_popctl_run": (C.c_int, [C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
+                                         C.c_int, C.c_double, C.c_double, C.c_int, D, D,
+                                         C.POINTER(C.c_int), C.c_int, D, D, D, C.POINTER(C.c_int), D, D]),
+            "ref_robust_spread": (C.c_double, [D, C.c_int]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(lib, name)
