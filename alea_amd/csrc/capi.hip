@@ -24,8 +24,8 @@ namespace alea_amd {
 int set_ziggurat(const uint64_t* k, const double* w, const double* f, double r);
 void launch_evaluate(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, size_t lds,
                      hipStream_t st);
-void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, bool mixed, bool cplx,
-                    size_t lds, hipStream_t st);
+void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, int nb_max, bool mixed,
+                    bool cplx, size_t lds, hipStream_t st);
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
                   bool cplx, size_t lds, hipStream_t st);
 void launch_call(const DevModel* m, const DevState& s, const DevCall& c, int p, int mode, int nw,
@@ -272,7 +272,7 @@ struct alea_system {
 
     void refresh() {
         if (host.n_blocks == 0) return;
-        launch_refresh(dmodel, st, nw, host.n_blocks, mixed(), cplx, refresh_lds, stream);
+        launch_refresh(dmodel, st, nw, host.n_blocks, st.nb_max, mixed(), cplx, refresh_lds, stream);
         hip_check(hipGetLastError(), "launch refresh");
         since_refresh = 0;
     }

@@ -1279,6 +1279,232 @@ __global__ __launch_bounds__(256) void k_refresh(const DevModel* __restrict__ Mp
     }
 }
 
+// Batched inverse refresh, register-tiled: one 256-thread workgroup per
+// (walker, block).  The block is padded to NP x NP with the identity, and
+// thread (tx, ty) = (tid / 16, tid % 16) holds rows ty*TT.. and columns
+// tx*TT.. of the working matrix in registers (TT = NP / 16), so the 16
+// owners of a column are adjacent lanes of one wave.  Gauss-Jordan in place
+// with implicit partial pivoting: step k takes the unused row of largest
+// |M[r][k]| (first largest on ties, as slater_determinant.cpp:905-993 picks
+// among the rows not yet pivoted), records it, and only that row and column k
+// cross LDS (double-buffered: two barriers a step).  Column k of the pivot
+// row becomes 1/pivot and of the other rows 0 before the rank-1 update, so
+// at the end A[pinv[r]][perm[k]] = M[r][k] for the real rows and columns.
+// FP64 throughout; A is stored in the determinant precision with log|det D|
+// (floored at 1e-300), the row sums of |D|, the condition estimate and a zero
+// update count, like k_refresh.
+template <typename T, bool CPLX, int NP>
+__global__ __launch_bounds__(256) void k_refresh_reg(const DevModel* __restrict__ Mp, DevState S) {
+    constexpr int TT = NP / 16;
+    constexpr int E = CPLX ? 2 : 1;
+    constexpr int NWORDS = (NP + 63) / 64;
+    const DevModel& M = *Mp;
+    __shared__ double prow[2][NP * E], fcol[2][NP * E];
+    __shared__ double part[16][NP];
+    __shared__ double best_sh[2], red[2][4];
+    __shared__ int piv_sh[2], perm[NP], pinv[NP];
+    const int nbk = M.n_blocks;
+    const int w = blockIdx.x / nbk, b = blockIdx.x % nbk;
+    const int tid = threadIdx.x, tx = tid >> 4, ty = tid & 15;
+    const int n = M.blk_n[b], first = M.blk_first[b];
+    const double* pos = S.pos + static_cast<size_t>(w) * M.np * 2;
+
+    double a[TT][TT][E];
+#pragma unroll
+    for (int i = 0; i < TT; ++i) {
+        const int r = ty * TT + i;
+        double rs = 0.0;
+#pragma unroll
+        for (int j = 0; j < TT; ++j) {
+            const int c = tx * TT + j;
+            if (r < n && c < n) {
+                Orb o;
+                orbital(M, b, c, pos[2 * (first + r)], pos[2 * (first + r) + 1], o);
+                a[i][j][0] = o.vr;
+                if (CPLX) a[i][j][E - 1] = o.vi;
+                rs += CPLX ? sqrt(o.vr * o.vr + o.vi * o.vi) : fabs(o.vr);
+            } else {
+                a[i][j][0] = r == c ? 1.0 : 0.0;
+                if (CPLX) a[i][j][E - 1] = 0.0;
+            }
+        }
+        part[tx][r] = rs;
+    }
+
+    unsigned long long used[NWORDS];
+#pragma unroll
+    for (int q = 0; q < NWORDS; ++q) used[q] = 0ull;
+    double log_det = 0.0;
+    bool singular = false;
+    for (int k = 0; k < NP; ++k) {
+        const int buf = k & 1, kb = k / TT, kj = k % TT;
+        if (tx == kb) {
+            double best = -1.0;
+            int bi = NP;
+#pragma unroll
+            for (int i = 0; i < TT; ++i) {
+                const int r = ty * TT + i;
+                if ((used[r >> 6] >> (r & 63)) & 1ull) continue;
+#pragma unroll
+                for (int j = 0; j < TT; ++j)
+                    if (j == kj) {
+                        const double re = a[i][j][0], im = CPLX ? a[i][j][E - 1] : 0.0;
+                        const double v = CPLX ? sqrt(re * re + im * im) : fabs(re);
+                        if (v > best) {
+                            best = v;
+                            bi = r;
+                        }
+                    }
+            }
+            for (int o = 8; o > 0; o >>= 1) {
+                const double ob = __shfl_xor(best, o, 16);
+                const int oi = __shfl_xor(bi, o, 16);
+                if (ob > best || (ob == best && oi < bi)) {
+                    best = ob;
+                    bi = oi;
+                }
+            }
+            if (ty == 0) {
+                piv_sh[buf] = bi;
+                best_sh[buf] = best;
+            }
+        }
+        __syncthreads();
+        const int p = piv_sh[buf];
+        const double best = best_sh[buf];
+        if (!(best > 0.0) || !isfinite(best) || p >= NP) {   // uniform: every thread reads the same
+            singular = true;
+            break;
+        }
+        log_det += log(best);
+        if (ty == p / TT) {
+#pragma unroll
+            for (int i = 0; i < TT; ++i)
+                if (i == p % TT)
+#pragma unroll
+                    for (int j = 0; j < TT; ++j)
+#pragma unroll
+                        for (int q = 0; q < E; ++q) prow[buf][(tx * TT + j) * E + q] = a[i][j][q];
+        }
+        if (tx == kb) {
+#pragma unroll
+            for (int j = 0; j < TT; ++j)
+                if (j == kj)
+#pragma unroll
+                    for (int i = 0; i < TT; ++i)
+#pragma unroll
+                        for (int q = 0; q < E; ++q) fcol[buf][(ty * TT + i) * E + q] = a[i][j][q];
+        }
+        __syncthreads();
+        const double dr = prow[buf][k * E], di = CPLX ? prow[buf][k * E + E - 1] : 0.0;
+        double ir, ii;
+        if (CPLX) {
+            const double d2 = dr * dr + di * di;
+            ir = dr / d2;
+            ii = -di / d2;
+        } else {
+            ir = 1.0 / dr;
+            ii = 0.0;
+        }
+        double nr[TT][E];
+#pragma unroll
+        for (int j = 0; j < TT; ++j) {
+            const int c = tx * TT + j;
+            const double xr = c == k ? 1.0 : prow[buf][c * E];
+            const double xi = c == k ? 0.0 : (CPLX ? prow[buf][c * E + E - 1] : 0.0);
+            nr[j][0] = CPLX ? xr * ir - xi * ii : xr * ir;
+            if (CPLX) nr[j][E - 1] = xr * ii + xi * ir;
+        }
+#pragma unroll
+        for (int i = 0; i < TT; ++i) {
+            const int r = ty * TT + i;
+            if (r == p) {
+#pragma unroll
+                for (int j = 0; j < TT; ++j)
+#pragma unroll
+                    for (int q = 0; q < E; ++q) a[i][j][q] = nr[j][q];
+            } else {
+                const double fr = fcol[buf][r * E], fi = CPLX ? fcol[buf][r * E + E - 1] : 0.0;
+#pragma unroll
+                for (int j = 0; j < TT; ++j) {
+                    const int c = tx * TT + j;
+                    const double xr = c == k ? 0.0 : a[i][j][0];
+                    a[i][j][0] = xr - (CPLX ? fr * nr[j][0] - fi * nr[j][E - 1] : fr * nr[j][0]);
+                    if (CPLX) {
+                        const double xi = c == k ? 0.0 : a[i][j][E - 1];
+                        a[i][j][E - 1] = xi - (fr * nr[j][E - 1] + fi * nr[j][0]);
+                    }
+                }
+            }
+        }
+        used[p >> 6] |= 1ull << (p & 63);
+        if (tid == 0) {
+            perm[k] = p;
+            pinv[p] = k;
+        }
+    }
+    if (singular) {
+        if (tid == 0) S.status[w] = 1;
+        return;
+    }
+    __syncthreads();   // perm, pinv
+    // column sums of |A| (a permutation of M's columns' sums), max over the
+    // real columns; |re| + |im| as in the sweep's update
+    double colmax = 0.0;
+#pragma unroll
+    for (int j = 0; j < TT; ++j) {
+        double cs = 0.0;
+#pragma unroll
+        for (int i = 0; i < TT; ++i) cs += fabs(a[i][j][0]) + (CPLX ? fabs(a[i][j][E - 1]) : 0.0);
+        for (int o = 8; o > 0; o >>= 1) cs += __shfl_xor(cs, o, 16);
+        if (tx * TT + j < n) colmax = fmax(colmax, cs);
+    }
+    T* out = static_cast<T*>(S.inv) + static_cast<size_t>(w) * S.inv_stride;
+    for (int i = 0; i < b; ++i) out += E * M.blk_rows[i] * M.blk_n[i];
+#pragma unroll
+    for (int i = 0; i < TT; ++i) {
+        const int r = ty * TT + i;
+        if (r >= n) continue;
+        const int m = pinv[r];
+        if (m >= n) continue;   // cannot happen for a non-singular block
+#pragma unroll
+        for (int j = 0; j < TT; ++j) {
+            const int k = tx * TT + j;
+            if (k >= n || perm[k] >= n) continue;
+            const size_t at = (static_cast<size_t>(m) * n + perm[k]) * E;
+            out[at] = static_cast<T>(a[i][j][0]);
+            if (CPLX) out[at + 1] = static_cast<T>(a[i][j][E - 1]);
+        }
+    }
+    for (int idx = n * n + tid; idx < M.blk_rows[b] * n; idx += 256) {   // zero padding rows
+        out[E * idx] = T(0);
+        if (CPLX) out[E * idx + 1] = T(0);
+    }
+    double rowmax = 0.0;
+    if (tid < n) {
+        double rs = 0.0;
+        for (int q = 0; q < 16; ++q) rs += part[q][tid];
+        S.row_abs[(static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max + tid] = rs;
+        rowmax = rs;
+    }
+    colmax = wmax(colmax);
+    rowmax = wmax(rowmax);
+    if ((tid & (W - 1)) == 0) {
+        red[0][tid / W] = colmax;
+        red[1][tid / W] = rowmax;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int q = 1; q < 4; ++q) {
+            colmax = fmax(colmax, red[0][q]);
+            rowmax = fmax(rowmax, red[1][q]);
+        }
+        S.cond[w * kMaxBlocks + b] = rowmax * colmax;
+        S.updates[w * kMaxBlocks + b] = 0;
+        S.logs[w * 4 + 1 + b] = fmax(log_det, log(1e-300));
+    }
+}
+
 // Fused sweeps: every particle of the walker, n_sweeps times.
 template <typename T, bool CPLX, bool DMC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_sweep(const DevModel* __restrict__ Mp, DevState S,
@@ -1594,9 +1820,36 @@ void launch_evaluate(const DevModel* m, const DevState& s, int nw, bool mixed, b
                      hipStream_t st) {
     DISPATCH(k_evaluate, dim3(nw), dim3(64), lds, st, m, s);
 }
-void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, bool mixed, bool cplx,
-                    size_t lds, hipStream_t st) {
-    DISPATCH(k_refresh, dim3(nw * n_blocks), dim3(256), lds, st, m, s);
+// The register-tiled refresh for blocks of at most 64 orbitals (128 real);
+// larger complex blocks use the LDS kernel.
+template <int NP>
+static void launch_refresh_reg(const DevModel* m, const DevState& s, int nw, int n_blocks, bool mixed,
+                               bool cplx, hipStream_t st) {
+    const dim3 g(nw * n_blocks), t(256);
+    if (mixed) {
+        if (cplx) hipLaunchKernelGGL((k_refresh_reg<float, true, NP>), g, t, 0, st, m, s);
+        else hipLaunchKernelGGL((k_refresh_reg<float, false, NP>), g, t, 0, st, m, s);
+    } else {
+        if (cplx) hipLaunchKernelGGL((k_refresh_reg<double, true, NP>), g, t, 0, st, m, s);
+        else hipLaunchKernelGGL((k_refresh_reg<double, false, NP>), g, t, 0, st, m, s);
+    }
+}
+
+void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, int nb_max, bool mixed,
+                    bool cplx, size_t lds, hipStream_t st) {
+    if (nb_max <= 16)
+        launch_refresh_reg<16>(m, s, nw, n_blocks, mixed, cplx, st);
+    else if (nb_max <= 32)
+        launch_refresh_reg<32>(m, s, nw, n_blocks, mixed, cplx, st);
+    else if (nb_max <= 64)
+        launch_refresh_reg<64>(m, s, nw, n_blocks, mixed, cplx, st);
+    else if (!cplx) {
+        const dim3 g(nw * n_blocks), t(256);
+        if (mixed) hipLaunchKernelGGL((k_refresh_reg<float, false, 128>), g, t, 0, st, m, s);
+        else hipLaunchKernelGGL((k_refresh_reg<double, false, 128>), g, t, 0, st, m, s);
+    } else {
+        DISPATCH(k_refresh, dim3(nw * n_blocks), dim3(256), lds, st, m, s);
+    }
 }
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
                   bool cplx, size_t lds, hipStream_t st) {
