@@ -33,7 +33,8 @@ def load() -> CApi:
         fn.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         api.fns["seed"] = fn
         for name, args in (("synchronize", [C.c_void_p]),
-                           ("last_sweep_ms", [C.c_void_p, C.POINTER(C.c_double)])):
+                           ("last_sweep_ms", [C.c_void_p, C.POINTER(C.c_double)]),
+                           ("set_refresh_interval", [C.c_void_p, C.c_int])):
             f = getattr(lib, "alea_" + name)
             f.restype = C.c_int
             f.argtypes = args

@@ -1376,32 +1376,42 @@ __global__ __launch_bounds__(256) void k_refresh_reg(const DevModel* __restrict_
             singular = true;
             break;
         }
-        log_det += log(best);
+        if (tid == 0) log_det += log(best);   // only thread 0 stores it
+        // the pivot row goes to LDS with 1 in column k (so every thread
+        // normalizes it without a select), column k goes as it is, and its
+        // owners then zero it: in place, column k of the other rows becomes
+        // -f_r / pivot and of the pivot row 1 / pivot
         if (ty == p / TT) {
 #pragma unroll
             for (int i = 0; i < TT; ++i)
                 if (i == p % TT)
 #pragma unroll
-                    for (int j = 0; j < TT; ++j)
-#pragma unroll
-                        for (int q = 0; q < E; ++q) prow[buf][(tx * TT + j) * E + q] = a[i][j][q];
+                    for (int j = 0; j < TT; ++j) {
+                        const bool kk = tx * TT + j == k;
+                        prow[buf][(tx * TT + j) * E] = kk ? 1.0 : a[i][j][0];
+                        if (CPLX) prow[buf][(tx * TT + j) * E + E - 1] = kk ? 0.0 : a[i][j][E - 1];
+                    }
         }
         if (tx == kb) {
 #pragma unroll
             for (int j = 0; j < TT; ++j)
                 if (j == kj)
 #pragma unroll
-                    for (int i = 0; i < TT; ++i)
+                    for (int i = 0; i < TT; ++i) {
 #pragma unroll
-                        for (int q = 0; q < E; ++q) fcol[buf][(ty * TT + i) * E + q] = a[i][j][q];
+                        for (int q = 0; q < E; ++q) {
+                            fcol[buf][(ty * TT + i) * E + q] = a[i][j][q];
+                            a[i][j][q] = 0.0;
+                        }
+                    }
         }
         __syncthreads();
-        const double dr = prow[buf][k * E], di = CPLX ? prow[buf][k * E + E - 1] : 0.0;
+        const double dr = fcol[buf][p * E], di = CPLX ? fcol[buf][p * E + E - 1] : 0.0;
         double ir, ii;
         if (CPLX) {
-            const double d2 = dr * dr + di * di;
-            ir = dr / d2;
-            ii = -di / d2;
+            const double inv_d2 = 1.0 / (dr * dr + di * di);
+            ir = dr * inv_d2;
+            ii = -di * inv_d2;
         } else {
             ir = 1.0 / dr;
             ii = 0.0;
@@ -1409,9 +1419,8 @@ __global__ __launch_bounds__(256) void k_refresh_reg(const DevModel* __restrict_
         double nr[TT][E];
 #pragma unroll
         for (int j = 0; j < TT; ++j) {
-            const int c = tx * TT + j;
-            const double xr = c == k ? 1.0 : prow[buf][c * E];
-            const double xi = c == k ? 0.0 : (CPLX ? prow[buf][c * E + E - 1] : 0.0);
+            const double xr = prow[buf][(tx * TT + j) * E];
+            const double xi = CPLX ? prow[buf][(tx * TT + j) * E + E - 1] : 0.0;
             nr[j][0] = CPLX ? xr * ir - xi * ii : xr * ir;
             if (CPLX) nr[j][E - 1] = xr * ii + xi * ir;
         }
@@ -1427,13 +1436,8 @@ __global__ __launch_bounds__(256) void k_refresh_reg(const DevModel* __restrict_
                 const double fr = fcol[buf][r * E], fi = CPLX ? fcol[buf][r * E + E - 1] : 0.0;
 #pragma unroll
                 for (int j = 0; j < TT; ++j) {
-                    const int c = tx * TT + j;
-                    const double xr = c == k ? 0.0 : a[i][j][0];
-                    a[i][j][0] = xr - (CPLX ? fr * nr[j][0] - fi * nr[j][E - 1] : fr * nr[j][0]);
-                    if (CPLX) {
-                        const double xi = c == k ? 0.0 : a[i][j][E - 1];
-                        a[i][j][E - 1] = xi - (fr * nr[j][E - 1] + fi * nr[j][0]);
-                    }
+                    a[i][j][0] -= CPLX ? fr * nr[j][0] - fi * nr[j][E - 1] : fr * nr[j][0];
+                    if (CPLX) a[i][j][E - 1] -= fr * nr[j][E - 1] + fi * nr[j][0];
                 }
             }
         }
