@@ -128,18 +128,34 @@ constexpr uint64_t splitmix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
+// A host <-> device copy ordered on a system's stream, complete on return.
+// (A plain hipMemcpy runs on the legacy default stream, which does not wait
+// for the systems' non-blocking streams.)
+hipError_t copy_on(hipStream_t st, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
+}
+
+// Zero-filled device buffer from the stream-ordered allocator on the
+// system's stream: allocating and freeing never synchronizes the device, so
+// one system's set-up or teardown does not stall another's sweeps.
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
-    void alloc(size_t count) {
+    hipStream_t stream = nullptr;
+    void alloc(size_t count, hipStream_t st) {
         free();
         n = count;
-        if (count) hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
-        if (count) hip_check(hipMemset(p, 0, count * sizeof(T)), "hipMemset");
+        stream = st;
+        if (count) {
+            hip_check(hipMallocAsync(reinterpret_cast<void**>(&p), count * sizeof(T), st), "hipMallocAsync");
+            hip_check(hipMemsetAsync(p, 0, count * sizeof(T), st), "hipMemsetAsync");
+        }
     }
     void free() {
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFreeAsync(p, stream);
         p = nullptr;
         n = 0;
     }
@@ -205,34 +221,35 @@ struct alea_system {
         }
         for (int g = 0; g < m.ng; ++g)
             for (int i = m.g_first[g]; i < m.g_last[g]; ++i) m.group_of[i] = g;
-        hip_check(hipMalloc(&dmodel, sizeof(DevModel)), "hipMalloc model");
-        hip_check(hipMemcpy(dmodel, &m, sizeof(DevModel), hipMemcpyHostToDevice), "copy model");
+        hip_check(hipMallocAsync(reinterpret_cast<void**>(&dmodel), sizeof(DevModel), stream), "hipMallocAsync model");
+        hip_check(hipMemcpyAsync(dmodel, &m, sizeof(DevModel), hipMemcpyHostToDevice, stream), "copy model");
+        model_nw_on_device = m.nw;
         const size_t nw_ = static_cast<size_t>(capacity);
-        pos.alloc(nw_ * np * 2);
-        rho.alloc(std::max<size_t>(1, nw_ * m.ng * m.nkj * 2));
-        logs.alloc(nw_ * 4);
+        pos.alloc(nw_ * np * 2, stream);
+        rho.alloc(std::max<size_t>(1, nw_ * m.ng * m.nkj * 2), stream);
+        logs.alloc(nw_ * 4, stream);
         inv_elems += inv_elems & 1;   // whole doubles per record (branching gathers doubles)
         st.inv_stride = inv_elems;
         st.nb_max = std::max(nb_max, 1);
-        inv_bytes.alloc(std::max<size_t>(1, nw_ * inv_elems * tsize()));
-        row_abs.alloc(nw_ * kMaxBlocks * st.nb_max);
-        cond.alloc(nw_ * kMaxBlocks);
-        updates.alloc(nw_ * kMaxBlocks);
+        inv_bytes.alloc(std::max<size_t>(1, nw_ * inv_elems * tsize()), stream);
+        row_abs.alloc(nw_ * kMaxBlocks * st.nb_max, stream);
+        cond.alloc(nw_ * kMaxBlocks, stream);
+        updates.alloc(nw_ * kMaxBlocks, stream);
         st.work_stride = 2 * (cplx ? 2 : 1) * nb_max * nb_max;
-        work.alloc(std::max<size_t>(1, nw_ * st.work_stride));
-        rng.alloc(nw_ * 4);
-        e_loc.alloc(nw_);
-        e_loc_im.alloc(nw_);
-        rr.alloc(nw_ * 2);
-        weights.alloc(nw_);
-        launch_fill(weights.p, static_cast<int>(nw_), 1.0, nullptr);
-        e_prev.alloc(nw_);
-        xi.alloc(nw_);
-        ncut.alloc(nw_);
-        branch_total.alloc(1);
-        status.alloc(nw_);
-        nacc.alloc(nw_);
-        src.alloc(nw_);
+        work.alloc(std::max<size_t>(1, nw_ * st.work_stride), stream);
+        rng.alloc(nw_ * 4, stream);
+        e_loc.alloc(nw_, stream);
+        e_loc_im.alloc(nw_, stream);
+        rr.alloc(nw_ * 2, stream);
+        weights.alloc(nw_, stream);
+        launch_fill(weights.p, static_cast<int>(nw_), 1.0, stream);
+        e_prev.alloc(nw_, stream);
+        xi.alloc(nw_, stream);
+        ncut.alloc(nw_, stream);
+        branch_total.alloc(1, stream);
+        status.alloc(nw_, stream);
+        nacc.alloc(nw_, stream);
+        src.alloc(nw_, stream);
         st.pos = pos.p;
         st.rho = rho.p;
         st.logs = logs.p;
@@ -256,8 +273,9 @@ struct alea_system {
         {
             const size_t e = cplx ? 2 : 1, n = static_cast<size_t>(nb_max);
             refresh_lds = 8 * (e * n * n + e * n + 8) + 4 * (8 + n + (n & 1)) + 8 * n;
-            if (m.n_blocks > 0 && refresh_lds > 160 * 1024)
-                throw InvalidArg("determinant block too large for the LDS refresh");
+            // blocks of up to 128 orbitals refresh in registers (k_refresh_reg);
+            // the LDS kernel is only sized, never launched, for them
+            if (m.n_blocks > 0 && nb_max > 128) throw InvalidArg("determinant block larger than 128 orbitals");
         }
         static bool zig_done = false;
         if (!zig_done) {
@@ -265,6 +283,7 @@ struct alea_system {
             if (set_ziggurat(z.k.data(), z.w.data(), z.f.data(), z.r)) throw DeviceError("ziggurat upload");
             zig_done = true;
         }
+        hip_check(hipStreamSynchronize(stream), "build");
         built = true;
     }
 
@@ -279,15 +298,19 @@ struct alea_system {
 
     void check_status() {
         std::vector<int> s(nw);
-        hip_check(hipMemcpy(s.data(), status.p, nw * sizeof(int), hipMemcpyDeviceToHost), "status");
+        hip_check(copy_on(stream, s.data(), status.p, nw * sizeof(int), hipMemcpyDeviceToHost), "status");
         for (int w = 0; w < nw; ++w)
             if (s[w]) throw std::runtime_error("Slater determinant: accepted matrix is singular or nonfinite");
     }
 
     // The model's nw is the walker stride of the per-call SoA buffers.
+    // ordered on the system's stream, and only when the count changed
+    int model_nw_on_device = -1;
     void set_model_nw(int n) {
         host.nw = n;
-        hip_check(hipMemcpy(&dmodel->nw, &n, sizeof(int), hipMemcpyHostToDevice), "model nw");
+        if (n == model_nw_on_device) return;
+        model_nw_on_device = n;
+        hip_check(hipMemcpyAsync(&dmodel->nw, &host.nw, sizeof(int), hipMemcpyHostToDevice, stream), "model nw");
     }
 };
 
@@ -436,20 +459,40 @@ int alea_create(alea_system** out, int n_walkers, int n_particles, int dim, cons
         s->np = n_particles;
         s->precision = precision;
         if (lattice) set_cell(s->host.cell, lattice);
-        hip_check(hipStreamCreate(&s->stream), "hipStreamCreate");
+        // non-blocking: copies on the legacy default stream never wait for
+        // this system's kernels, nor its kernels for other systems' copies
+        hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
         *out = s.release();
     });
 }
 
 int alea_destroy(alea_system* s) {
     if (!s) return ALEA_OK;
-    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    hipStream_t st = s->stream;
+    if (st) (void)hipStreamSynchronize(st);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
-    if (s->dmodel) (void)hipFree(s->dmodel);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
-    delete s;
+    if (s->dmodel) (void)hipFreeAsync(s->dmodel, st);
+    delete s;   // the buffers free on the stream, which must outlive them
+    if (st) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
     return ALEA_OK;
+}
+
+int alea_query(alea_system* s, int* done) {
+    return guarded([&] {
+        Sys(s);
+        if (!done) throw InvalidArg("null output");
+        const hipError_t e = hipStreamQuery(s->stream);
+        if (e == hipErrorNotReady) {
+            *done = 0;
+            return;
+        }
+        hip_check(e, "stream query");
+        *done = 1;
+    });
 }
 
 int alea_add_group(alea_system* s, int first, int last, double charge, double inv_mass) {
@@ -696,7 +739,7 @@ int alea_set_positions(alea_system* s, const double* pos) {
                 }
             }
         }
-        hip_check(hipMemcpy(s->pos.p, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice),
+        hip_check(copy_on(s->stream, s->pos.p, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice),
                   "upload positions");
         s->evaluated = false;
         s->energies_valid = false;
@@ -709,7 +752,7 @@ int alea_get_positions(alea_system* s, double* pos) {
         if (!s->built) throw InvalidArg("positions were never set");
         s->sync();
         std::vector<double> dev(static_cast<size_t>(s->nw) * s->np * 2);
-        hip_check(hipMemcpy(dev.data(), s->pos.p, dev.size() * sizeof(double), hipMemcpyDeviceToHost),
+        hip_check(copy_on(s->stream, dev.data(), s->pos.p, dev.size() * sizeof(double), hipMemcpyDeviceToHost),
                   "download positions");
         for (int p = 0; p < s->np; ++p)
             for (int d = 0; d < 2; ++d)
@@ -738,7 +781,7 @@ int alea_get_log_psi(alea_system* s, double* out) {
         if (!s->evaluated) throw InvalidArg("evaluate first");
         s->sync();
         std::vector<double> l(static_cast<size_t>(s->nw) * 4);
-        hip_check(hipMemcpy(l.data(), s->logs.p, l.size() * sizeof(double), hipMemcpyDeviceToHost), "logs");
+        hip_check(copy_on(s->stream, l.data(), s->logs.p, l.size() * sizeof(double), hipMemcpyDeviceToHost), "logs");
         for (int w = 0; w < s->nw; ++w) {
             double t = l[w * 4];
             for (int b = 0; b < s->host.n_blocks; ++b) t += l[w * 4 + 1 + b];
@@ -752,7 +795,7 @@ static void measure(alea_system* s, bool want_grads) {
     DevState st = s->st;
     if (want_grads) {
         const size_t n = static_cast<size_t>(s->np) * 2 * s->nw;
-        if (s->grad_buf.n < 4 * n) s->grad_buf.alloc(4 * n);
+        if (s->grad_buf.n < 4 * n) s->grad_buf.alloc(4 * n, s->stream);
         st.grad_out = s->grad_buf.p;
         st.lap_out = s->grad_buf.p + n;
         st.pgrad_out = s->grad_buf.p + 2 * n;
@@ -770,8 +813,8 @@ int alea_get_gradients(alea_system* s, double* grad, double* lap) {
         Sys(s);
         measure(s, true);
         const size_t n = static_cast<size_t>(s->np) * 2 * s->nw;
-        hip_check(hipMemcpy(grad, s->grad_buf.p, n * sizeof(double), hipMemcpyDeviceToHost), "grad");
-        hip_check(hipMemcpy(lap, s->grad_buf.p + n, n * sizeof(double), hipMemcpyDeviceToHost), "lap");
+        hip_check(copy_on(s->stream, grad, s->grad_buf.p, n * sizeof(double), hipMemcpyDeviceToHost), "grad");
+        hip_check(copy_on(s->stream, lap, s->grad_buf.p + n, n * sizeof(double), hipMemcpyDeviceToHost), "lap");
     });
 }
 
@@ -780,8 +823,8 @@ int alea_get_phase_gradients(alea_system* s, double* pgrad, double* plap) {
         Sys(s);
         measure(s, true);
         const size_t n = static_cast<size_t>(s->np) * 2 * s->nw;
-        hip_check(hipMemcpy(pgrad, s->grad_buf.p + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost), "pgrad");
-        hip_check(hipMemcpy(plap, s->grad_buf.p + 3 * n, n * sizeof(double), hipMemcpyDeviceToHost), "plap");
+        hip_check(copy_on(s->stream, pgrad, s->grad_buf.p + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost), "pgrad");
+        hip_check(copy_on(s->stream, plap, s->grad_buf.p + 3 * n, n * sizeof(double), hipMemcpyDeviceToHost), "plap");
     });
 }
 
@@ -789,9 +832,9 @@ int alea_local_energy(alea_system* s, double* e_re, double* e_im) {
     return guarded([&] {
         Sys(s);
         measure(s, false);
-        hip_check(hipMemcpy(e_re, s->e_loc.p, s->nw * sizeof(double), hipMemcpyDeviceToHost), "e_loc");
+        hip_check(copy_on(s->stream, e_re, s->e_loc.p, s->nw * sizeof(double), hipMemcpyDeviceToHost), "e_loc");
         if (e_im)
-            hip_check(hipMemcpy(e_im, s->e_loc_im.p, s->nw * sizeof(double), hipMemcpyDeviceToHost), "e_loc_im");
+            hip_check(copy_on(s->stream, e_im, s->e_loc_im.p, s->nw * sizeof(double), hipMemcpyDeviceToHost), "e_loc_im");
     });
 }
 
@@ -802,9 +845,9 @@ int alea_ratio(alea_system* s, int p, const double* new_pos, double* log_ratio, 
         if (!s->evaluated) throw InvalidArg("evaluate first");
         if (p < 0 || p >= s->np) throw InvalidArg("particle index out of range");
         const size_t nw = s->nw;
-        if (s->call_buf.n < 12 * nw) s->call_buf.alloc(12 * nw);
+        if (s->call_buf.n < 12 * nw) s->call_buf.alloc(12 * nw, s->stream);
         double* b = s->call_buf.p;
-        hip_check(hipMemcpy(b, new_pos, 2 * nw * sizeof(double), hipMemcpyHostToDevice), "new_pos");
+        hip_check(copy_on(s->stream, b, new_pos, 2 * nw * sizeof(double), hipMemcpyHostToDevice), "new_pos");
         DevCall c;
         c.new_pos = b;
         c.log_ratio = b + 2 * nw;
@@ -817,10 +860,10 @@ int alea_ratio(alea_system* s, int p, const double* new_pos, double* log_ratio, 
         launch_call(s->dmodel, s->st, c, p, 1, s->nw, s->mixed(), s->cplx, s->walker_lds, s->stream);
         hip_check(hipGetLastError(), "launch ratio");
         s->sync();
-        hip_check(hipMemcpy(log_ratio, c.log_ratio, nw * sizeof(double), hipMemcpyDeviceToHost), "lr");
-        if (phase) hip_check(hipMemcpy(phase, c.phase, nw * sizeof(double), hipMemcpyDeviceToHost), "ph");
-        if (new_grad) hip_check(hipMemcpy(new_grad, c.new_grad, 2 * nw * sizeof(double), hipMemcpyDeviceToHost), "ng");
-        if (new_lap) hip_check(hipMemcpy(new_lap, c.new_lap, 2 * nw * sizeof(double), hipMemcpyDeviceToHost), "nl");
+        hip_check(copy_on(s->stream, log_ratio, c.log_ratio, nw * sizeof(double), hipMemcpyDeviceToHost), "lr");
+        if (phase) hip_check(copy_on(s->stream, phase, c.phase, nw * sizeof(double), hipMemcpyDeviceToHost), "ph");
+        if (new_grad) hip_check(copy_on(s->stream, new_grad, c.new_grad, 2 * nw * sizeof(double), hipMemcpyDeviceToHost), "ng");
+        if (new_lap) hip_check(copy_on(s->stream, new_lap, c.new_lap, 2 * nw * sizeof(double), hipMemcpyDeviceToHost), "nl");
     });
 }
 
@@ -831,8 +874,8 @@ int alea_accept(alea_system* s, int p, const unsigned char* mask) {
         if (p < 0 || p >= s->np) throw InvalidArg("particle index out of range");
         const size_t nw = s->nw;
         if (s->call_buf.n < 12 * nw) throw InvalidArg("accept needs a preceding ratio call");
-        if (s->call_mask.n < nw) s->call_mask.alloc(nw);
-        hip_check(hipMemcpy(s->call_mask.p, mask, nw, hipMemcpyHostToDevice), "mask");
+        if (s->call_mask.n < nw) s->call_mask.alloc(nw, s->stream);
+        hip_check(copy_on(s->stream, s->call_mask.p, mask, nw, hipMemcpyHostToDevice), "mask");
         DevCall c;
         c.new_pos = s->call_buf.p;
         c.mask = s->call_mask.p;
@@ -857,8 +900,8 @@ static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* t
     P.dmc = dmc ? 1 : 0;
     if (tape) {
         const size_t len = static_cast<size_t>(n_sweeps) * s->np * 3 * stride;
-        if (s->tape.n < len) s->tape.alloc(len);
-        hip_check(hipMemcpy(s->tape.p, tape, len * sizeof(double), hipMemcpyHostToDevice), "tape");
+        if (s->tape.n < len) s->tape.alloc(len, s->stream);
+        hip_check(copy_on(s->stream, s->tape.p, tape, len * sizeof(double), hipMemcpyHostToDevice), "tape");
         P.tape = s->tape.p;
         P.tape_stride = stride;
     }
@@ -887,7 +930,7 @@ static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* t
     if (n_accepted) {
         s->sync();
         std::vector<long long> a(s->nw);
-        hip_check(hipMemcpy(a.data(), s->nacc.p, s->nw * sizeof(long long), hipMemcpyDeviceToHost), "nacc");
+        hip_check(copy_on(s->stream, a.data(), s->nacc.p, s->nw * sizeof(long long), hipMemcpyDeviceToHost), "nacc");
         long long t = 0;
         for (long long x : a) t += x;
         *n_accepted = t;
@@ -916,12 +959,12 @@ int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape
                 rr[2 * w] = rr_proposed[w];
                 rr[2 * w + 1] = rr_accepted[w];
             }
-            hip_check(hipMemcpy(s->rr.p, rr.data(), rr.size() * sizeof(double), hipMemcpyHostToDevice), "rr");
+            hip_check(copy_on(s->stream, s->rr.p, rr.data(), rr.size() * sizeof(double), hipMemcpyHostToDevice), "rr");
         }
         run_sweeps(s, n_sweeps, tau, tape, tape ? tape_stride : s->nw, true, n_accepted);
         if (rr_proposed) {
             s->sync();
-            hip_check(hipMemcpy(rr.data(), s->rr.p, rr.size() * sizeof(double), hipMemcpyDeviceToHost), "rr");
+            hip_check(copy_on(s->stream, rr.data(), s->rr.p, rr.size() * sizeof(double), hipMemcpyDeviceToHost), "rr");
             for (size_t w = 0; w < nw; ++w) {
                 rr_proposed[w] = rr[2 * w];
                 rr_accepted[w] = rr[2 * w + 1];
@@ -967,7 +1010,7 @@ int alea_seed(alea_system* s, uint64_t seed, uint64_t stream) {
             const uint64_t sd = splitmix64(splitmix64(splitmix64(seed) ^ stream) ^ static_cast<uint64_t>(w));
             for (uint64_t i = 0; i < 4; ++i) st[w * 4 + i] = splitmix64(sd + i * 0x9e3779b97f4a7c15ULL);
         }
-        hip_check(hipMemcpy(s->rng.p, st.data(), st.size() * sizeof(uint64_t), hipMemcpyHostToDevice), "rng");
+        hip_check(copy_on(s->stream, s->rng.p, st.data(), st.size() * sizeof(uint64_t), hipMemcpyHostToDevice), "rng");
         s->seeded = true;
         s->seed_base = seed;
         s->seed_stream = stream;
@@ -1002,7 +1045,7 @@ static void permute_device(alea_system* s, int new_nw, const int* src_d) {
     static_assert(sizeof(int) * kMaxBlocks % sizeof(double) == 0, "update counters pack into doubles");
     size_t widest = 0;
     for (const auto& r : recs) widest = std::max(widest, r.second / sizeof(double));
-    if (s->perm_tmp.n < cap * widest) s->perm_tmp.alloc(cap * widest);   // before any gather is queued
+    if (s->perm_tmp.n < cap * widest) s->perm_tmp.alloc(cap * widest, s->stream);   // before any gather is queued
     for (const auto& r : recs) {
         if (r.second == 0) continue;
         launch_gather(static_cast<double*>(r.first), s->perm_tmp.p, static_cast<int>(r.second / sizeof(double)),
@@ -1012,7 +1055,7 @@ static void permute_device(alea_system* s, int new_nw, const int* src_d) {
     }
     if (s->st.inv_stride) {
         const size_t bytes = static_cast<size_t>(s->st.inv_stride) * s->tsize();
-        if (s->inv_spare.n < s->inv_bytes.n) s->inv_spare.alloc(s->inv_bytes.n);
+        if (s->inv_spare.n < s->inv_bytes.n) s->inv_spare.alloc(s->inv_bytes.n, s->stream);
         launch_gather(reinterpret_cast<const double*>(s->inv_bytes.p), reinterpret_cast<double*>(s->inv_spare.p),
                       static_cast<int>(bytes / sizeof(double)), src_d, new_nw, s->stream);
         std::swap(s->inv_bytes.p, s->inv_spare.p);
@@ -1031,7 +1074,7 @@ int alea_permute(alea_system* s, int new_nw, const int* src) {
         if (new_nw <= 0 || new_nw > s->capacity) throw InvalidArg("new walker count exceeds the capacity");
         for (int j = 0; j < new_nw; ++j)
             if (src[j] < 0 || src[j] >= s->nw) throw InvalidArg("permutation source out of range");
-        hip_check(hipMemcpy(s->src.p, src, new_nw * sizeof(int), hipMemcpyHostToDevice), "src");
+        hip_check(copy_on(s->stream, s->src.p, src, new_nw * sizeof(int), hipMemcpyHostToDevice), "src");
         permute_device(s, new_nw, s->src.p);
     });
 }
@@ -1084,9 +1127,9 @@ int alea_dmc_sums(alea_system* s, double* energy_sum, double* weight_sum, long l
         const size_t nw = s->nw;
         std::vector<double> w(nw), e(nw);
         std::vector<long long> c(nw);
-        hip_check(hipMemcpy(w.data(), s->weights.p, nw * sizeof(double), hipMemcpyDeviceToHost), "weights");
-        hip_check(hipMemcpy(e.data(), s->e_loc.p, nw * sizeof(double), hipMemcpyDeviceToHost), "e_loc");
-        hip_check(hipMemcpy(c.data(), s->ncut.p, nw * sizeof(long long), hipMemcpyDeviceToHost), "ncut");
+        hip_check(copy_on(s->stream, w.data(), s->weights.p, nw * sizeof(double), hipMemcpyDeviceToHost), "weights");
+        hip_check(copy_on(s->stream, e.data(), s->e_loc.p, nw * sizeof(double), hipMemcpyDeviceToHost), "e_loc");
+        hip_check(copy_on(s->stream, c.data(), s->ncut.p, nw * sizeof(long long), hipMemcpyDeviceToHost), "ncut");
         double es = 0.0, ws = 0.0;
         long long nc = 0;
         for (size_t i = 0; i < nw; ++i) {
@@ -1105,7 +1148,7 @@ int alea_get_weights(alea_system* s, double* w) {
         Sys(s);
         if (!s->built) throw InvalidArg("positions were never set");
         s->sync();
-        hip_check(hipMemcpy(w, s->weights.p, s->nw * sizeof(double), hipMemcpyDeviceToHost), "weights");
+        hip_check(copy_on(s->stream, w, s->weights.p, s->nw * sizeof(double), hipMemcpyDeviceToHost), "weights");
     });
 }
 
@@ -1115,7 +1158,7 @@ int alea_set_weights(alea_system* s, const double* w) {
         if (!s->built) throw InvalidArg("positions were never set");
         for (int i = 0; i < s->nw; ++i)
             if (!(w[i] >= 0.0) || !std::isfinite(w[i])) throw InvalidArg("weights must be finite and non-negative");
-        hip_check(hipMemcpy(s->weights.p, w, s->nw * sizeof(double), hipMemcpyHostToDevice), "weights");
+        hip_check(copy_on(s->stream, s->weights.p, w, s->nw * sizeof(double), hipMemcpyHostToDevice), "weights");
     });
 }
 
@@ -1170,12 +1213,12 @@ int alea_branch(alea_system* s, uint64_t* rng_state, int target, int* new_nw, in
         std::vector<size_t> src;
         if (total > 0 && total <= s->capacity) {
             std::vector<int> d(total);
-            hip_check(hipMemcpy(d.data(), s->src.p, total * sizeof(int), hipMemcpyDeviceToHost), "src");
+            hip_check(copy_on(s->stream, d.data(), s->src.p, total * sizeof(int), hipMemcpyDeviceToHost), "src");
             src.assign(d.begin(), d.end());
         } else {
             // slow path: recompute the list here (the same multiplicities)
             std::vector<double> w(n);
-            hip_check(hipMemcpy(w.data(), s->weights.p, n * sizeof(double), hipMemcpyDeviceToHost), "weights");
+            hip_check(copy_on(s->stream, w.data(), s->weights.p, n * sizeof(double), hipMemcpyDeviceToHost), "weights");
             if (total == 0) {
                 size_t best = 0;
                 for (int i = 1; i < n; ++i)
@@ -1199,14 +1242,14 @@ int alea_branch(alea_system* s, uint64_t* rng_state, int target, int* new_nw, in
         }
         const int nn = static_cast<int>(src.size());
         std::vector<int> srci(src.begin(), src.end());
-        hip_check(hipMemcpy(s->src.p, srci.data(), nn * sizeof(int), hipMemcpyHostToDevice), "src");
+        hip_check(copy_on(s->stream, s->src.p, srci.data(), nn * sizeof(int), hipMemcpyHostToDevice), "src");
         permute_device(s, nn, s->src.p);
         launch_fill(s->weights.p, nn, 1.0, s->stream);
         hip_check(hipGetLastError(), "launch fill");
         if (s->seeded) {
             std::vector<char> seen(n, 0);
             std::vector<uint64_t> st(static_cast<size_t>(nn) * 4);
-            hip_check(hipMemcpy(st.data(), s->rng.p, st.size() * sizeof(uint64_t), hipMemcpyDeviceToHost), "rng");
+            hip_check(copy_on(s->stream, st.data(), s->rng.p, st.size() * sizeof(uint64_t), hipMemcpyDeviceToHost), "rng");
             for (int j = 0; j < nn; ++j) {
                 if (!seen[srci[j]]) {
                     seen[srci[j]] = 1;
@@ -1216,7 +1259,7 @@ int alea_branch(alea_system* s, uint64_t* rng_state, int target, int* new_nw, in
                                                s->next_stream_index++);
                 for (uint64_t i = 0; i < 4; ++i) st[j * 4 + i] = splitmix64(sd + i * 0x9e3779b97f4a7c15ULL);
             }
-            hip_check(hipMemcpy(s->rng.p, st.data(), st.size() * sizeof(uint64_t), hipMemcpyHostToDevice), "rng");
+            hip_check(copy_on(s->stream, s->rng.p, st.data(), st.size() * sizeof(uint64_t), hipMemcpyHostToDevice), "rng");
         }
         s->sync();
         *new_nw = nn;

@@ -408,6 +408,13 @@ __device__ inline void wsum_n(double (&v)[N]) {
     for (int i = 0; i < N; ++i) v[i] = wsum(v[i]);
 }
 
+// Condition estimate of a freshly computed inverse, as stored: above 1e8 it
+// is the block's own conditioning, which another rebuild cannot lower, and it
+// is stored negated ("latched") so that the kappa > 1e8 rebuild trigger
+// (slater_determinant.cpp:789, 825-826) stays quiet until the next batched
+// refresh instead of rebuilding on every accepted move.
+__device__ inline double fresh_cond(double kappa) { return kappa > 1e8 ? -kappa : kappa; }
+
 // ── the per-walker engine ──────────────────────────────────────────────────
 template <typename T, bool CPLX>
 struct Walker {
@@ -787,7 +794,7 @@ struct Walker {
         }
         rowmax = wmax(rowmax);
         if (lane == 0) {
-            S.cond[w * kMaxBlocks + b] = rowmax * colmax;
+            S.cond[w * kMaxBlocks + b] = fresh_cond(rowmax * colmax);
             S.updates[w * kMaxBlocks + b] = 0;
         }
         __syncthreads();
@@ -807,7 +814,9 @@ struct Walker {
         int* upd = S.updates + w * kMaxBlocks + b;
         const int updates = *upd + 1;
         double* rabs = S.row_abs + (static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max;
-        const double cond_old = S.cond[w * kMaxBlocks + b];
+        const double cond_stored = S.cond[w * kMaxBlocks + b];
+        const bool latched = cond_stored < 0.0;    // see fresh_cond
+        const double cond_old = fabs(cond_stored);
         __syncthreads();
         // The reference also rebuilds after 64 updates (slater_determinant.cpp:
         // 788-790); here that periodic rebuild is the batched refresh kernel
@@ -818,7 +827,7 @@ struct Walker {
         // 790); 1e-2 in FP32, whose rounding bound passes 1e-4 on about 1% of
         // the accepted moves of a 61-orbital plane-wave block.
         constexpr double kCancel = sizeof(T) == 8 ? 1e4 : 1e2;
-        bool rebuild_now = !isfinite(absR) || absR < 1e-5 || cond_old > 1e8 ||
+        bool rebuild_now = !isfinite(absR) || absR < 1e-5 || (!latched && cond_old > 1e8) ||
                            absR <= kCancel * eps * d.scale;
         if (!rebuild_now) {
             // pass 1: row product r_c = sum_m v'_m A[m][c] - delta_cj
@@ -889,13 +898,13 @@ struct Walker {
             for (int kr = lane; kr < n; kr += W) rowmax = fmax(rowmax, rabs[kr]);
             rowmax = wmax(rowmax);
             const double cond = rowmax * colmax;
-            if (!isfinite(cond) || cond > 1e8) {
+            if (!isfinite(cond) || (!latched && cond > 1e8)) {
                 rebuild_now = true;
             } else {
                 log_b(b) += log(absR);
                 if (lane == 0) {
                     *upd = updates;
-                    S.cond[w * kMaxBlocks + b] = cond;
+                    S.cond[w * kMaxBlocks + b] = latched ? -cond : cond;
                 }
             }
         }
@@ -1273,17 +1282,17 @@ __global__ __launch_bounds__(256) void k_refresh(const DevModel* __restrict__ Mp
             colmax = fmax(colmax, red_v[q]);
             rowmax = fmax(rowmax, fcol[q]);
         }
-        S.cond[w * kMaxBlocks + b] = rowmax * colmax;
+        S.cond[w * kMaxBlocks + b] = fresh_cond(rowmax * colmax);
         S.updates[w * kMaxBlocks + b] = 0;
         S.logs[w * 4 + 1 + b] = fmax(log_det, log(1e-300));
     }
 }
 
-// Batched inverse refresh, register-tiled: one 256-thread workgroup per
+// Batched inverse refresh, register-tiled: one GX*16-thread workgroup per
 // (walker, block).  The block is padded to NP x NP with the identity, and
-// thread (tx, ty) = (tid / 16, tid % 16) holds rows ty*TT.. and columns
-// tx*TT.. of the working matrix in registers (TT = NP / 16), so the 16
-// owners of a column are adjacent lanes of one wave.  Gauss-Jordan in place
+// thread (tx, ty) = (tid / 16, tid % 16) of GX x 16 holds rows ty*TR.. and
+// columns tx*TC.. of the working matrix in registers (TR = NP / 16,
+// TC = NP / GX), so the 16 owners of a column are adjacent lanes of one wave.  Gauss-Jordan in place
 // with implicit partial pivoting: step k takes the unused row of largest
 // |M[r][k]| (first largest on ties, as slater_determinant.cpp:905-993 picks
 // among the rows not yet pivoted), records it, and only that row and column k
@@ -1293,30 +1302,32 @@ __global__ __launch_bounds__(256) void k_refresh(const DevModel* __restrict__ Mp
 // FP64 throughout; A is stored in the determinant precision with log|det D|
 // (floored at 1e-300), the row sums of |D|, the condition estimate and a zero
 // update count, like k_refresh.
-template <typename T, bool CPLX, int NP>
-__global__ __launch_bounds__(256) void k_refresh_reg(const DevModel* __restrict__ Mp, DevState S) {
-    constexpr int TT = NP / 16;
+template <typename T, bool CPLX, int NP, int GX>
+__global__ __launch_bounds__(GX * 16) void k_refresh_reg(const DevModel* __restrict__ Mp, DevState S) {
+    constexpr int GY = 16;                 // owners of a column: 16 adjacent lanes
+    constexpr int TR = NP / GY, TC = NP / GX;   // tile: TR rows x TC columns per thread
     constexpr int E = CPLX ? 2 : 1;
     constexpr int NWORDS = (NP + 63) / 64;
+    constexpr int NT = GX * GY, NWAVES = NT / W;
     const DevModel& M = *Mp;
     __shared__ double prow[2][NP * E], fcol[2][NP * E];
-    __shared__ double part[16][NP];
-    __shared__ double best_sh[2], red[2][4];
+    __shared__ double part[GX][NP];
+    __shared__ double best_sh[2], red[2][NWAVES];
     __shared__ int piv_sh[2], perm[NP], pinv[NP];
     const int nbk = M.n_blocks;
     const int w = blockIdx.x / nbk, b = blockIdx.x % nbk;
-    const int tid = threadIdx.x, tx = tid >> 4, ty = tid & 15;
+    const int tid = threadIdx.x, tx = tid / GY, ty = tid % GY;
     const int n = M.blk_n[b], first = M.blk_first[b];
     const double* pos = S.pos + static_cast<size_t>(w) * M.np * 2;
 
-    double a[TT][TT][E];
+    double a[TR][TC][E];
 #pragma unroll
-    for (int i = 0; i < TT; ++i) {
-        const int r = ty * TT + i;
+    for (int i = 0; i < TR; ++i) {
+        const int r = ty * TR + i;
         double rs = 0.0;
 #pragma unroll
-        for (int j = 0; j < TT; ++j) {
-            const int c = tx * TT + j;
+        for (int j = 0; j < TC; ++j) {
+            const int c = tx * TC + j;
             if (r < n && c < n) {
                 Orb o;
                 orbital(M, b, c, pos[2 * (first + r)], pos[2 * (first + r) + 1], o);
@@ -1337,16 +1348,16 @@ __global__ __launch_bounds__(256) void k_refresh_reg(const DevModel* __restrict_
     double log_det = 0.0;
     bool singular = false;
     for (int k = 0; k < NP; ++k) {
-        const int buf = k & 1, kb = k / TT, kj = k % TT;
+        const int buf = k & 1, kb = k / TC, kj = k % TC;
         if (tx == kb) {
             double best = -1.0;
             int bi = NP;
 #pragma unroll
-            for (int i = 0; i < TT; ++i) {
-                const int r = ty * TT + i;
+            for (int i = 0; i < TR; ++i) {
+                const int r = ty * TR + i;
                 if ((used[r >> 6] >> (r & 63)) & 1ull) continue;
 #pragma unroll
-                for (int j = 0; j < TT; ++j)
+                for (int j = 0; j < TC; ++j)
                     if (j == kj) {
                         const double re = a[i][j][0], im = CPLX ? a[i][j][E - 1] : 0.0;
                         const double v = CPLX ? sqrt(re * re + im * im) : fabs(re);
@@ -1356,9 +1367,9 @@ __global__ __launch_bounds__(256) void k_refresh_reg(const DevModel* __restrict_
                         }
                     }
             }
-            for (int o = 8; o > 0; o >>= 1) {
-                const double ob = __shfl_xor(best, o, 16);
-                const int oi = __shfl_xor(bi, o, 16);
+            for (int o = GY / 2; o > 0; o >>= 1) {
+                const double ob = __shfl_xor(best, o, GY);
+                const int oi = __shfl_xor(bi, o, GY);
                 if (ob > best || (ob == best && oi < bi)) {
                     best = ob;
                     bi = oi;
@@ -1381,26 +1392,26 @@ __global__ __launch_bounds__(256) void k_refresh_reg(const DevModel* __restrict_
         // normalizes it without a select), column k goes as it is, and its
         // owners then zero it: in place, column k of the other rows becomes
         // -f_r / pivot and of the pivot row 1 / pivot
-        if (ty == p / TT) {
+        if (ty == p / TR) {
 #pragma unroll
-            for (int i = 0; i < TT; ++i)
-                if (i == p % TT)
+            for (int i = 0; i < TR; ++i)
+                if (i == p % TR)
 #pragma unroll
-                    for (int j = 0; j < TT; ++j) {
-                        const bool kk = tx * TT + j == k;
-                        prow[buf][(tx * TT + j) * E] = kk ? 1.0 : a[i][j][0];
-                        if (CPLX) prow[buf][(tx * TT + j) * E + E - 1] = kk ? 0.0 : a[i][j][E - 1];
+                    for (int j = 0; j < TC; ++j) {
+                        const bool kk = tx * TC + j == k;
+                        prow[buf][(tx * TC + j) * E] = kk ? 1.0 : a[i][j][0];
+                        if (CPLX) prow[buf][(tx * TC + j) * E + E - 1] = kk ? 0.0 : a[i][j][E - 1];
                     }
         }
         if (tx == kb) {
 #pragma unroll
-            for (int j = 0; j < TT; ++j)
+            for (int j = 0; j < TC; ++j)
                 if (j == kj)
 #pragma unroll
-                    for (int i = 0; i < TT; ++i) {
+                    for (int i = 0; i < TR; ++i) {
 #pragma unroll
                         for (int q = 0; q < E; ++q) {
-                            fcol[buf][(ty * TT + i) * E + q] = a[i][j][q];
+                            fcol[buf][(ty * TR + i) * E + q] = a[i][j][q];
                             a[i][j][q] = 0.0;
                         }
                     }
@@ -1416,26 +1427,26 @@ __global__ __launch_bounds__(256) void k_refresh_reg(const DevModel* __restrict_
             ir = 1.0 / dr;
             ii = 0.0;
         }
-        double nr[TT][E];
+        double nr[TC][E];
 #pragma unroll
-        for (int j = 0; j < TT; ++j) {
-            const double xr = prow[buf][(tx * TT + j) * E];
-            const double xi = CPLX ? prow[buf][(tx * TT + j) * E + E - 1] : 0.0;
+        for (int j = 0; j < TC; ++j) {
+            const double xr = prow[buf][(tx * TC + j) * E];
+            const double xi = CPLX ? prow[buf][(tx * TC + j) * E + E - 1] : 0.0;
             nr[j][0] = CPLX ? xr * ir - xi * ii : xr * ir;
             if (CPLX) nr[j][E - 1] = xr * ii + xi * ir;
         }
 #pragma unroll
-        for (int i = 0; i < TT; ++i) {
-            const int r = ty * TT + i;
+        for (int i = 0; i < TR; ++i) {
+            const int r = ty * TR + i;
             if (r == p) {
 #pragma unroll
-                for (int j = 0; j < TT; ++j)
+                for (int j = 0; j < TC; ++j)
 #pragma unroll
                     for (int q = 0; q < E; ++q) a[i][j][q] = nr[j][q];
             } else {
                 const double fr = fcol[buf][r * E], fi = CPLX ? fcol[buf][r * E + E - 1] : 0.0;
 #pragma unroll
-                for (int j = 0; j < TT; ++j) {
+                for (int j = 0; j < TC; ++j) {
                     a[i][j][0] -= CPLX ? fr * nr[j][0] - fi * nr[j][E - 1] : fr * nr[j][0];
                     if (CPLX) a[i][j][E - 1] -= fr * nr[j][E - 1] + fi * nr[j][0];
                 }
@@ -1456,38 +1467,38 @@ __global__ __launch_bounds__(256) void k_refresh_reg(const DevModel* __restrict_
     // real columns; |re| + |im| as in the sweep's update
     double colmax = 0.0;
 #pragma unroll
-    for (int j = 0; j < TT; ++j) {
+    for (int j = 0; j < TC; ++j) {
         double cs = 0.0;
 #pragma unroll
-        for (int i = 0; i < TT; ++i) cs += fabs(a[i][j][0]) + (CPLX ? fabs(a[i][j][E - 1]) : 0.0);
-        for (int o = 8; o > 0; o >>= 1) cs += __shfl_xor(cs, o, 16);
-        if (tx * TT + j < n) colmax = fmax(colmax, cs);
+        for (int i = 0; i < TR; ++i) cs += fabs(a[i][j][0]) + (CPLX ? fabs(a[i][j][E - 1]) : 0.0);
+        for (int o = GY / 2; o > 0; o >>= 1) cs += __shfl_xor(cs, o, GY);
+        if (tx * TC + j < n) colmax = fmax(colmax, cs);
     }
     T* out = static_cast<T*>(S.inv) + static_cast<size_t>(w) * S.inv_stride;
     for (int i = 0; i < b; ++i) out += E * M.blk_rows[i] * M.blk_n[i];
 #pragma unroll
-    for (int i = 0; i < TT; ++i) {
-        const int r = ty * TT + i;
+    for (int i = 0; i < TR; ++i) {
+        const int r = ty * TR + i;
         if (r >= n) continue;
         const int m = pinv[r];
         if (m >= n) continue;   // cannot happen for a non-singular block
 #pragma unroll
-        for (int j = 0; j < TT; ++j) {
-            const int k = tx * TT + j;
+        for (int j = 0; j < TC; ++j) {
+            const int k = tx * TC + j;
             if (k >= n || perm[k] >= n) continue;
             const size_t at = (static_cast<size_t>(m) * n + perm[k]) * E;
             out[at] = static_cast<T>(a[i][j][0]);
             if (CPLX) out[at + 1] = static_cast<T>(a[i][j][E - 1]);
         }
     }
-    for (int idx = n * n + tid; idx < M.blk_rows[b] * n; idx += 256) {   // zero padding rows
+    for (int idx = n * n + tid; idx < M.blk_rows[b] * n; idx += NT) {   // zero padding rows
         out[E * idx] = T(0);
         if (CPLX) out[E * idx + 1] = T(0);
     }
     double rowmax = 0.0;
     if (tid < n) {
         double rs = 0.0;
-        for (int q = 0; q < 16; ++q) rs += part[q][tid];
+        for (int q = 0; q < GX; ++q) rs += part[q][tid];
         S.row_abs[(static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max + tid] = rs;
         rowmax = rs;
     }
@@ -1499,11 +1510,11 @@ __global__ __launch_bounds__(256) void k_refresh_reg(const DevModel* __restrict_
     }
     __syncthreads();
     if (tid == 0) {
-        for (int q = 1; q < 4; ++q) {
+        for (int q = 1; q < NWAVES; ++q) {
             colmax = fmax(colmax, red[0][q]);
             rowmax = fmax(rowmax, red[1][q]);
         }
-        S.cond[w * kMaxBlocks + b] = rowmax * colmax;
+        S.cond[w * kMaxBlocks + b] = fresh_cond(rowmax * colmax);
         S.updates[w * kMaxBlocks + b] = 0;
         S.logs[w * 4 + 1 + b] = fmax(log_det, log(1e-300));
     }
@@ -1824,36 +1835,33 @@ void launch_evaluate(const DevModel* m, const DevState& s, int nw, bool mixed, b
                      hipStream_t st) {
     DISPATCH(k_evaluate, dim3(nw), dim3(64), lds, st, m, s);
 }
-// The register-tiled refresh for blocks of at most 64 orbitals (128 real);
-// larger complex blocks use the LDS kernel.
-template <int NP>
+// The register-tiled refresh: 16 x 16 threads up to 64 orbitals, 32 x 16 up
+// to 128 (the most a block can have).
+template <int NP, int GX>
 static void launch_refresh_reg(const DevModel* m, const DevState& s, int nw, int n_blocks, bool mixed,
                                bool cplx, hipStream_t st) {
-    const dim3 g(nw * n_blocks), t(256);
+    const dim3 g(nw * n_blocks), t(GX * 16);
     if (mixed) {
-        if (cplx) hipLaunchKernelGGL((k_refresh_reg<float, true, NP>), g, t, 0, st, m, s);
-        else hipLaunchKernelGGL((k_refresh_reg<float, false, NP>), g, t, 0, st, m, s);
+        if (cplx) hipLaunchKernelGGL((k_refresh_reg<float, true, NP, GX>), g, t, 0, st, m, s);
+        else hipLaunchKernelGGL((k_refresh_reg<float, false, NP, GX>), g, t, 0, st, m, s);
     } else {
-        if (cplx) hipLaunchKernelGGL((k_refresh_reg<double, true, NP>), g, t, 0, st, m, s);
-        else hipLaunchKernelGGL((k_refresh_reg<double, false, NP>), g, t, 0, st, m, s);
+        if (cplx) hipLaunchKernelGGL((k_refresh_reg<double, true, NP, GX>), g, t, 0, st, m, s);
+        else hipLaunchKernelGGL((k_refresh_reg<double, false, NP, GX>), g, t, 0, st, m, s);
     }
 }
 
 void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, int nb_max, bool mixed,
                     bool cplx, size_t lds, hipStream_t st) {
     if (nb_max <= 16)
-        launch_refresh_reg<16>(m, s, nw, n_blocks, mixed, cplx, st);
+        launch_refresh_reg<16, 16>(m, s, nw, n_blocks, mixed, cplx, st);
     else if (nb_max <= 32)
-        launch_refresh_reg<32>(m, s, nw, n_blocks, mixed, cplx, st);
+        launch_refresh_reg<32, 16>(m, s, nw, n_blocks, mixed, cplx, st);
     else if (nb_max <= 64)
-        launch_refresh_reg<64>(m, s, nw, n_blocks, mixed, cplx, st);
-    else if (!cplx) {
-        const dim3 g(nw * n_blocks), t(256);
-        if (mixed) hipLaunchKernelGGL((k_refresh_reg<float, false, 128>), g, t, 0, st, m, s);
-        else hipLaunchKernelGGL((k_refresh_reg<double, false, 128>), g, t, 0, st, m, s);
-    } else {
+        launch_refresh_reg<64, 16>(m, s, nw, n_blocks, mixed, cplx, st);
+    else if (nb_max <= 128)
+        launch_refresh_reg<128, 32>(m, s, nw, n_blocks, mixed, cplx, st);   // 512 threads, 8 x 4 tiles
+    else
         DISPATCH(k_refresh, dim3(nw * n_blocks), dim3(256), lds, st, m, s);
-    }
 }
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
                   bool cplx, size_t lds, hipStream_t st) {

@@ -1,0 +1,219 @@
+"""Continuous-batching serving loop for VMC jobs on one GPU.
+
+A request is a VMC job on the 2-D paramagnetic electron gas: N electrons at
+density rs with a twist, n_walkers walkers, n_blocks blocks of
+sweeps_per_block sweeps.  Requests arrive over time; the server admits them,
+in arrival order, as soon as their device footprint fits the HBM budget and an
+active slot is free; admitted jobs live on the GPU as their own walker batch
+(their own alea_system, whose work runs on its own non-blocking stream), so
+the blocks of all active jobs are in flight on the device together and a new
+job starts while others are mid-run.  Each block is queued asynchronously
+(alea_vmc_sweeps with no acceptance read-back) and polled (alea_query); when
+it completes, its local energy is measured and the job's next block is
+queued.  A finished job frees its memory and slot at once.
+
+Per job it records the energy per particle of every block, the block
+latencies (queued -> completed) and the request latency (arrival ->
+finished); over the stream, the p50/p99 of both and the aggregate
+walker-moves/s.  Jobs are independent (own walkers, own device streams of
+random numbers from the job's seed), so a job's results are the same alone
+and in any mix.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .system import heg_spec, uniform_positions
+from .walkers import FP64, MIXED, gpu_walkers, seed as seed_walkers
+
+
+@dataclass
+class Job:
+    job_id: int
+    arrival: float                 # seconds after the stream starts
+    n_particles: int               # paramagnetic: N/2 up, N/2 down
+    rs: float
+    twist: tuple
+    n_walkers: int
+    n_blocks: int = 20
+    sweeps_per_block: int = 50
+    seed: int = 0
+    precision: int = MIXED
+    # filled in by the server
+    admitted: float | None = None
+    finished: float | None = None
+    block_energy: list = field(default_factory=list)     # E_L per particle, mean over walkers
+    block_latency: list = field(default_factory=list)    # seconds, queued -> completed
+    accepted: int = 0
+
+    @property
+    def tau(self) -> float:
+        return (0.5 * self.rs) ** 2          # Metropolis step sigma = 0.5 rs
+
+    @property
+    def walker_moves(self) -> int:
+        return self.n_walkers * self.n_particles * self.n_blocks * self.sweeps_per_block
+
+
+def footprint_bytes(n_particles: int, n_walkers: int, precision: int, n_kj: int = 40,
+                    complex_orbitals: bool = True) -> int:
+    """Device bytes of one job's walker batch, as the C-ABI allocates them
+    (alea_system::build): positions, k-space densities, log parts, the two
+    determinant inverses (rows padded to 16) in the determinant precision,
+    row sums, condition estimates, rebuild workspace (FP64), RNG states and
+    the per-walker scalars."""
+    nb = n_particles // 2
+    rows = (nb + 15) // 16 * 16
+    e = 2 if complex_orbitals else 1
+    t = 4 if precision == MIXED else 8
+    inv = 2 * e * rows * nb
+    inv += inv & 1
+    per_walker = (16 * n_particles + 16 * 2 * n_kj + 32 + inv * t + 16 * nb + 16 + 8
+                  + 8 * 2 * e * nb * nb + 32 + 16 + 16 + 24 + 8 + 4 + 8 + 4)
+    return per_walker * n_walkers + (1 << 20)
+
+
+def poisson_jobs(n_jobs: int, rate: float, seed: int = 4, sizes=(26, 58, 74, 122, 242),
+                 rs_choices=(30.0, 40.0, 60.0), walkers=(512, 8192), n_blocks: int = 20,
+                 sweeps_per_block: int = 50, precision: int = MIXED) -> list:
+    """A stream of requests: Poisson arrivals at `rate` per second; N, rs and
+    the twist (uniform in the first Brillouin zone, redrawn while it splits a
+    degenerate |k + theta| shell at the Fermi level) and a log-uniform walker
+    count rounded to 64, all from `seed`."""
+    rng = np.random.default_rng(seed)
+    jobs, t = [], 0.0
+    for i in range(n_jobs):
+        t += rng.exponential(1.0 / rate)
+        n = int(rng.choice(sizes))
+        rs = float(rng.choice(rs_choices))
+        while True:
+            twist = tuple(float(x) for x in rng.uniform(-0.5, 0.5, 2))
+            try:
+                heg_spec(n // 2, n // 2, rs, twist=twist)
+                break
+            except ValueError:
+                continue
+        lo, hi = math.log2(walkers[0]), math.log2(walkers[1])
+        nw = int(round(2.0 ** rng.uniform(lo, hi) / 64.0)) * 64
+        # FP32 inverses only for blocks of up to 64 orbitals: the larger
+        # low-density blocks reach condition numbers (> 1e7) at which an FP32
+        # inverse has no correct digits, so those jobs run in FP64
+        prec = precision if n // 2 <= 64 else FP64
+        jobs.append(Job(i, t, n, rs, twist, max(64, nw), n_blocks, sweeps_per_block,
+                        seed=int(rng.integers(1 << 62)), precision=prec))
+    return jobs
+
+
+class GpuRunner:
+    """One admitted job on the GPU: its walker batch and the block in flight."""
+
+    def __init__(self, job: Job):
+        spec = heg_spec(job.n_particles // 2, job.n_particles // 2, job.rs, twist=job.twist)
+        self.job = job
+        self.w = gpu_walkers(spec, job.n_walkers, job.precision)
+        self.w.set_positions(uniform_positions(spec, job.n_walkers, np.random.default_rng(job.seed)))
+        self.w.evaluate()
+        seed_walkers(self.w, job.seed)
+
+    def start_block(self):
+        self.w.api("vmc_sweeps", self.w.h, self.job.sweeps_per_block, self.job.tau, None, None)
+
+    def done(self) -> bool:
+        import ctypes as C
+        d = C.c_int()
+        self.w.api("query", self.w.h, C.byref(d))
+        return d.value != 0
+
+    def block_energy(self) -> float:
+        e, _ = self.w.local_energy()
+        return float(np.mean(e)) / self.job.n_particles
+
+    def close(self):
+        self.w.close()
+
+
+def _pct(x, q):
+    return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else float("nan")
+
+
+class Server:
+    """Admission by HBM footprint and free slots, FIFO by arrival."""
+
+    def __init__(self, hbm_budget_bytes: int = 64 << 30, max_active: int = 8, runner=GpuRunner,
+                 clock=time.perf_counter, poll_interval: float = 1e-4, on_finish=None):
+        self.budget = hbm_budget_bytes
+        self.max_active = max_active
+        self.runner = runner
+        self.clock = clock
+        self.poll_interval = poll_interval
+        self.peak_bytes = 0
+        self.peak_active = 0
+        self.on_finish = on_finish      # called with each finished job (progress reporting)
+
+    def run(self, jobs: list) -> dict:
+        pending = sorted(jobs, key=lambda j: (j.arrival, j.job_id))
+        for j in pending:
+            if footprint_bytes(j.n_particles, j.n_walkers, j.precision) > self.budget:
+                raise ValueError(f"job {j.job_id} needs more than the whole HBM budget")
+        active = []          # (runner, bytes, block_start)
+        used = 0
+        t0 = self.clock()
+        done_jobs = []
+        while pending or active:
+            now = self.clock() - t0
+            # admit in arrival order while memory and slots allow
+            while pending and pending[0].arrival <= now and len(active) < self.max_active:
+                j = pending[0]
+                need = footprint_bytes(j.n_particles, j.n_walkers, j.precision)
+                if used + need > self.budget:
+                    break
+                pending.pop(0)
+                r = self.runner(j)
+                j.admitted = self.clock() - t0
+                r.start_block()
+                active.append([r, need, self.clock() - t0])
+                used += need
+                self.peak_bytes = max(self.peak_bytes, used)
+                self.peak_active = max(self.peak_active, len(active))
+            progressed = False
+            for entry in list(active):
+                r, need, start = entry
+                if not r.done():
+                    continue
+                progressed = True
+                j = r.job
+                j.block_latency.append(self.clock() - t0 - start)
+                j.block_energy.append(r.block_energy())
+                if len(j.block_energy) < j.n_blocks:
+                    entry[2] = self.clock() - t0
+                    r.start_block()
+                else:
+                    j.finished = self.clock() - t0
+                    r.close()
+                    active.remove(entry)
+                    used -= need
+                    done_jobs.append(j)
+                    if self.on_finish:
+                        self.on_finish(j)
+            if not progressed:
+                if not active and pending:
+                    wait = pending[0].arrival - (self.clock() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.05))
+                else:
+                    time.sleep(self.poll_interval)
+        wall = self.clock() - t0
+        blocks = [x for j in done_jobs for x in j.block_latency]
+        requests = [j.finished - j.arrival for j in done_jobs]
+        waits = [j.admitted - j.arrival for j in done_jobs]
+        moves = sum(j.walker_moves for j in done_jobs)
+        return dict(jobs=len(done_jobs), wall_s=wall, walker_moves=moves,
+                    walker_moves_per_s=moves / wall if wall > 0 else float("nan"),
+                    p50_block_ms=1e3 * _pct(blocks, 50), p99_block_ms=1e3 * _pct(blocks, 99),
+                    p50_request_s=_pct(requests, 50), p99_request_s=_pct(requests, 99),
+                    p50_admission_wait_s=_pct(waits, 50), peak_active=self.peak_active,
+                    peak_bytes=self.peak_bytes, budget_bytes=self.budget)

@@ -123,6 +123,28 @@ def cpu_baseline(tau, walkers_per_core=24):
                        f"{wall:.1f} s wall incl. setup")
 
 
+SERVING = dict(n_jobs=16, rate_per_s=4.0, seed=4, n_blocks=4, sweeps_per_block=10,
+               walkers=(512, 8192), sizes=(26, 58, 74, 122, 242), rs=(30.0, 40.0, 60.0))
+
+
+def serving_leg(precision=1):
+    """A bounded sample of the mixed-size request stream: Poisson arrivals of
+    VMC jobs (N, rs, twist, walker count drawn from the seed), continuously
+    batched on the GPU.  p50/p99 block and request latency, aggregate
+    walker-moves/s."""
+    from alea_amd import serving
+    c = SERVING
+    jobs = serving.poisson_jobs(c["n_jobs"], c["rate_per_s"], seed=c["seed"], sizes=c["sizes"],
+                                rs_choices=c["rs"], walkers=c["walkers"], n_blocks=c["n_blocks"],
+                                sweeps_per_block=c["sweeps_per_block"], precision=precision)
+    def progress(j):
+        print(f"bench: serving job {j.job_id} (N={j.n_particles}, {j.n_walkers} walkers) done at "
+              f"{j.finished:.1f} s", file=sys.stderr, flush=True)
+    st = serving.Server(hbm_budget_bytes=96 << 30, max_active=8, on_finish=progress).run(jobs)
+    st["config"] = dict(c, note="each job 4 blocks x 10 sweeps (the full stream: 20 x 50)")
+    return st
+
+
 def copy_calibration(n_bytes, reps=3):
     """A plain device-to-device copy of n_bytes, reps times: the known-size
     reference that scales the FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
@@ -147,6 +169,8 @@ def main():
                     help="time one form only (profiling runs); prints its dict, not the bench line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="run a plain copy of BYTES bytes (counter calibration) and exit")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serving-only", action="store_true", help="run the serving leg alone, print its dict")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
@@ -155,13 +179,19 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
+    if args.serving_only:
+        print(json.dumps(serving_leg()), flush=True)
+        return
     tau = (0.5 * RS) ** 2   # sigma = sqrt(tau/m) = 0.5 rs
     if args.only:
         out, _ = run_gpu(0 if args.only == "fp64" else 1, args.walkers, args.steps, args.warmup, tau)
         print(json.dumps(dict(form=args.only, **out)), flush=True)
         return
+    print("bench: FP64 sweeps", file=sys.stderr, flush=True)
     fp64, spec = run_gpu(0, args.walkers, args.steps, args.warmup, tau)
+    print(f"bench: FP64 {fp64['moves_per_s']:.4g} walker-moves/s; mixed sweeps", file=sys.stderr, flush=True)
     mixed, _ = run_gpu(1, args.walkers, args.steps, args.warmup, tau)
+    print(f"bench: mixed {mixed['moves_per_s']:.4g} walker-moves/s", file=sys.stderr, flush=True)
     b_move = bytes_per_move(4, mixed["acceptance"], spec)
     moves_per_launch = args.walkers * spec.n_particles
     achieved = b_move * moves_per_launch / (mixed["kernel_ms"] * 1e-3) / 1e9
@@ -199,6 +229,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_sweep<float,true>", "kernel_ms": mixed["kernel_ms"],
                      "bytes_per_move": b_move},
+        "serving": None if args.no_serving else serving_leg(),
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(tau),
     }
     print(json.dumps(line), flush=True)

@@ -125,7 +125,8 @@ int alea_accept(alea_system* s, int p, const unsigned char* mask);
 /* Fused sweeps: VMC::sweep / sweep_walkers (vmc.cpp:69-155) with symmetric
  * Metropolis proposals of width sqrt(tau/m).  tape: [sweep][np][dim+1][nw]
  * random numbers in draw_sweep_randoms order (sweep_blocking.h:39-48), or NULL
- * for the per-walker device streams seeded by alea_seed. */
+ * for the per-walker device streams seeded by alea_seed.  With n_accepted
+ * NULL the call returns once the sweeps are queued (see alea_query). */
 int alea_vmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape,
                     long long* n_accepted);
 /* DMC::sweep_walkers (dmc.cpp:203-337): Umrigar-scaled drift, node gate for a
@@ -147,6 +148,12 @@ int alea_set_refresh_interval(alea_system* s, int sweeps);
 
 /* Waits for the system's stream (hipStreamSynchronize). */
 int alea_synchronize(alea_system* s);
+/* Non-blocking: *done = 1 when all work queued on the system's stream has
+ * finished, else 0 (hipStreamQuery).  Every system has its own non-blocking
+ * stream, so work of different systems overlaps on the device; with
+ * n_accepted NULL, alea_vmc_sweeps only queues its launches.  No counterpart
+ * in the reference: the serving loop's completion poll. */
+int alea_query(alea_system* s, int* done);
 /* Device time of the last alea_vmc_sweeps / alea_dmc_sweeps launch, measured
  * with HIP events on the system's stream (no counterpart in the reference:
  * the measurement hook of the benchmark). */

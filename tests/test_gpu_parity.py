@@ -162,3 +162,18 @@ def test_mixed_trajectories_track_fp64_on_the_n122_workload():
     assert same.mean() >= 0.98
     de = np.abs(out[0][1] - out[1][1])[same] / s.n_particles
     assert de.max() < 1e-7
+
+
+def test_largest_complex_blocks_evaluate_like_the_oracle():
+    """N = 242 twisted (two 121-orbital complex blocks, padded to 128): the
+    1024-thread register-tiled inverse, then log|psi|, gradients and E_L."""
+    s = heg_spec(121, 121, 30.0, twist=(0.21, -0.37))
+    nw = 4
+    pos = _pos(s, nw, 14)
+    g, o = gpu_walkers(s, nw), oracle_walkers(s, nw)
+    for x in (g, o):
+        x.set_positions(pos)
+        x.evaluate()
+    assert rel_err(g.log_psi(), o.log_psi()) < TOL
+    assert rel_err(g.gradients()[0], o.gradients()[0]) < 1e-9
+    assert rel_err(g.local_energy()[0], o.local_energy()[0]) < TOL
