@@ -33,6 +33,8 @@ SIGNATURES = {
     "add_ewald": (C.c_int, [_V, C.c_int, _P_I, _P_I, _P_D, _P_D, _P_D, C.c_int, _P_D, C.c_double,
                             C.c_double, C.c_double, C.c_double]),
     "add_coulomb": (C.c_int, [_V, C.c_int, C.c_int, C.c_double]),
+    "add_inverse_power": (C.c_int, [_V, C.c_int, C.c_int, C.c_double, C.c_double]),
+    "add_mcmillan_jastrow": (C.c_int, [_V, C.c_int, C.c_int, C.c_double, C.c_int]),
     "set_positions": (C.c_int, [_V, _P_D]),
     "get_positions": (C.c_int, [_V, _P_D]),
     "evaluate": (C.c_int, [_V]),

@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstring>
 #include <exception>
+#include <limits>
 #include <memory>
 #include <numbers>
 #include <stdexcept>
@@ -166,6 +167,7 @@ struct DevBuf {
 
 struct alea_system {
     int capacity = 0, nw = 0, np = 0, precision = 0;
+    double lattice[4] = {0, 0, 0, 0};   // primitive vectors as given (rows), periodic cells
     DevModel host;        // host copy of the model
     DevModel* dmodel = nullptr;
     bool built = false, evaluated = false;
@@ -458,7 +460,10 @@ int alea_create(alea_system** out, int n_walkers, int n_particles, int dim, cons
         s->capacity = s->nw = n_walkers;
         s->np = n_particles;
         s->precision = precision;
-        if (lattice) set_cell(s->host.cell, lattice);
+        if (lattice) {
+            set_cell(s->host.cell, lattice);
+            std::copy(lattice, lattice + 4, s->lattice);
+        }
         // non-blocking: copies on the legacy default stream never wait for
         // this system's kernels, nor its kernels for other systems' copies
         hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -703,6 +708,100 @@ int alea_add_coulomb(alea_system* s, int ga, int gb, double epsilon) {
         m.coul_gb[m.n_coul] = gb;
         m.coul_coeff[m.n_coul] = qa * qb / epsilon;
         ++m.n_coul;
+    });
+}
+
+// McMillan pair Jastrow u(r) = -(b/r)^n at the minimum image (no
+// counterpart in the reference).
+int alea_add_mcmillan_jastrow(alea_system* s, int ga, int gb, double b, int n) {
+    return guarded([&] {
+        Sys(s);
+        need_unbuilt(s);
+        check_group(s, ga);
+        check_group(s, gb);
+        DevModel& m = s->host;
+        if (m.n_pair >= kMaxPair) throw InvalidArg("too many pair Jastrow factors");
+        if (!(b > 0.0) || !std::isfinite(b)) throw InvalidArg("mcmillan: b must be finite and positive");
+        if (n < 1 || n > 16) throw InvalidArg("mcmillan: the exponent must be 1..16");
+        DevPair& P = m.pair[m.n_pair++];
+        P.ga = ga;
+        P.gb = gb;
+        P.rad = DevRadial{};
+        P.rad.kind = 2;
+        P.rad.num = n;
+        P.rad.c[0] = b;
+    });
+}
+
+namespace {
+// The integral of r^-p over the plane outside the Wigner-Seitz cell of the
+// lattice a (rows a1, a2), 1/(p - 2) times the integral over angle of
+// rho(theta)^(2 - p), rho the distance from the centre to the cell boundary:
+// the midpoint rule on 2^18 angles, the boundary from the bisectors of the
+// 24 translations s a1 + t a2, |s|, |t| <= 2 (the method of
+// outside_wigner_seitz_integral, inverse_power.cpp:28-41, and
+// wigner_seitz_exterior.h:58-83).
+double ws_exterior_integral(const double* a, double p) {
+    double lx[24], ly[24];
+    int nl = 0;
+    for (int si = -2; si <= 2; ++si)
+        for (int ti = -2; ti <= 2; ++ti) {
+            if (si == 0 && ti == 0) continue;
+            lx[nl] = si * a[0] + ti * a[2];
+            ly[nl] = si * a[1] + ti * a[3];
+            ++nl;
+        }
+    constexpr size_t n_theta = size_t{1} << 18;
+    const double dtheta = 2.0 * 3.14159265358979323846 / static_cast<double>(n_theta);
+    double sum = 0.0;
+    for (size_t k = 0; k < n_theta; ++k) {
+        const double theta = (static_cast<double>(k) + 0.5) * dtheta;
+        const double ux = std::cos(theta), uy = std::sin(theta);
+        double rho = std::numeric_limits<double>::infinity();
+        for (int c = 0; c < nl; ++c) {
+            const double proj = ux * lx[c] + uy * ly[c];
+            if (proj > 0.0) rho = std::min(rho, 0.5 * (lx[c] * lx[c] + ly[c] * ly[c]) / proj);
+        }
+        sum += std::pow(rho, 2.0 - p);
+    }
+    return sum * dtheta / (p - 2.0);
+}
+}  // namespace
+
+// InversePowerTerm(group_a, group_b, C, p, cell) (inverse_power.h:56-60,
+// inverse_power.cpp:43-83): the same refusals, the same integer-power
+// product, and in a periodic cell the continuum tail outside the
+// Wigner-Seitz cell.
+int alea_add_inverse_power(alea_system* s, int ga, int gb, double strength, double exponent) {
+    return guarded([&] {
+        Sys(s);
+        need_unbuilt(s);
+        check_group(s, ga);
+        check_group(s, gb);
+        DevModel& m = s->host;
+        if (m.n_ipow >= kMaxInversePower) throw InvalidArg("too many inverse-power terms");
+        if (!std::isfinite(exponent) || exponent <= 0.0)
+            throw InvalidArg("inverse_power: exponent must be finite and positive");
+        if (!std::isfinite(strength)) throw InvalidArg("inverse_power: strength must be finite");
+        if (strength < 0.0 && exponent >= 2.0)
+            throw InvalidArg("inverse_power: an attractive core (strength < 0) with exponent >= 2 falls to the centre");
+        const int t = m.n_ipow;
+        m.ipow_ga[t] = ga;
+        m.ipow_gb[t] = gb;
+        m.ipow_c[t] = strength;
+        m.ipow_p[t] = exponent;
+        const double rounded = std::round(exponent);
+        m.ipow_int[t] = (rounded == exponent && rounded <= 16.0) ? static_cast<int>(rounded) : 0;
+        if (m.cell.periodic) {
+            if (exponent <= 2.0)
+                throw InvalidArg("inverse_power: in a periodic 2-D cell the exponent must exceed 2");
+            const double na = m.g_last[ga] - m.g_first[ga], nb = m.g_last[gb] - m.g_first[gb];
+            const double pairs = ga == gb ? 0.5 * na * na : na * nb;
+            const double area = std::fabs(s->lattice[0] * s->lattice[3] - s->lattice[1] * s->lattice[2]);
+            const double outside = ws_exterior_integral(s->lattice, exponent);
+            m.ipow_tail += pairs / area * strength * outside;
+        }
+        ++m.n_ipow;
     });
 }
 

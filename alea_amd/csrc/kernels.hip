@@ -172,6 +172,16 @@ __device__ inline void spline_eval(const double* sc, int n_knots, double r_cut, 
 }
 
 __device__ inline void radial_eval(const DevRadial& R, double r, double& u, double& du, double& d2u) {
+    if (R.kind == 2) {   // McMillan: u = -(b/r)^n
+        const double x = R.c[0] / r;
+        double xn = 1.0;
+        for (int i = 0; i < R.num; ++i) xn *= x;
+        const double n = static_cast<double>(R.num);
+        u = -xn;
+        du = n * xn / r;
+        d2u = -n * (n + 1.0) * xn / (r * r);
+        return;
+    }
     if (R.kind == 1) {
         const double t = r - R.L;
         double tCm2 = 1.0;
@@ -1725,7 +1735,27 @@ __global__ __launch_bounds__(64) void k_measure(const DevModel* __restrict__ Mp,
                 pot += M.coul_coeff[cp] / r;
             }
     }
-    const double e = wsum(kin) + wsum(pot) + (M.ewald ? M.ew_const : 0.0);
+    // InversePowerTerm::evaluate_potential (inverse_power.cpp:130-168)
+    for (int t = 0; t < M.n_ipow; ++t) {
+        const int ga = M.ipow_ga[t], gb = M.ipow_gb[t];
+        const bool intra = ga == gb;
+        const double c = M.ipow_c[t];
+        const int pi = M.ipow_int[t];
+        for (int i = M.g_first[ga] + lane; i < M.g_last[ga]; i += W)
+            for (int k = intra ? i + 1 : M.g_first[gb]; k < M.g_last[gb]; ++k) {
+                double d0 = sh.pos[2 * i] - sh.pos[2 * k], d1 = sh.pos[2 * i + 1] - sh.pos[2 * k + 1];
+                const double r = dist(M.cell, d0, d1);
+                if (pi > 0) {
+                    const double inv_r = 1.0 / r;
+                    double inv_rp = inv_r;
+                    for (int q = 1; q < pi; ++q) inv_rp *= inv_r;
+                    pot += c * inv_rp;
+                } else {
+                    pot += c * pow(r, -M.ipow_p[t]);
+                }
+            }
+    }
+    const double e = wsum(kin) + wsum(pot) + (M.ewald ? M.ew_const : 0.0) + M.ipow_tail;
     const double ei = wsum(kin_im);
     if (lane == 0) {
         S.e_loc[w] = e;

@@ -21,6 +21,7 @@ constexpr int kMaxKE = 192;        // Ewald vectors
 constexpr int kMaxBlocks = 2;      // determinant blocks
 constexpr int kMaxOrb = 128;       // orbitals per block
 constexpr int kMaxCoulomb = 8;
+constexpr int kMaxInversePower = 4;
 constexpr int kRowPad = 16;        // determinant rows padded to a multiple of this
 
 struct DevCell {
@@ -32,10 +33,11 @@ struct DevCell {
 };
 
 // Radial pair function u(r): Padé P/Q (pade_jastrow.cpp:26-56) optionally
-// reduced to its image-sum short-range part (periodic_pair_sum.cpp:66-83), or
-// Drummond (drummond_jastrow.cpp:27-80).
+// reduced to its image-sum short-range part (periodic_pair_sum.cpp:66-83),
+// Drummond (drummond_jastrow.cpp:27-80), or McMillan u = -(b/r)^n (c[0] = b,
+// num = n).
 struct DevRadial {
-    int kind = 0;   // 0 Padé, 1 Drummond
+    int kind = 0;   // 0 Padé, 1 Drummond, 2 McMillan
     int num = 0, den = 0, image = 0;
     double c[kMaxCoef] = {};
     double r_cut = 0, inv_delta = 0, u_inf = 0;
@@ -99,6 +101,14 @@ struct DevModel {
     int n_coul = 0;
     int coul_ga[kMaxCoulomb] = {}, coul_gb[kMaxCoulomb] = {};
     double coul_coeff[kMaxCoulomb] = {};
+
+    // inverse-power pair terms C / r^p at the minimum image; ipow_int = p
+    // when p is an integer <= 16 (repeated product), else 0 (pow); the
+    // continuum tails of all terms summed in ipow_tail
+    int n_ipow = 0;
+    int ipow_ga[kMaxInversePower] = {}, ipow_gb[kMaxInversePower] = {}, ipow_int[kMaxInversePower] = {};
+    double ipow_c[kMaxInversePower] = {}, ipow_p[kMaxInversePower] = {};
+    double ipow_tail = 0.0;
 };
 
 // Per-walker state arrays (device pointers).  Strides are per walker.

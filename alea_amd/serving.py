@@ -1,8 +1,9 @@
 """Continuous-batching serving loop for VMC jobs on one GPU.
 
-A request is a VMC job on the 2-D paramagnetic electron gas: N electrons at
-density rs with a twist, n_walkers walkers, n_blocks blocks of
-sweeps_per_block sweeps.  Requests arrive over time; the server admits them,
+A request is a VMC job: on the 2-D paramagnetic electron gas (N electrons at
+density rs with a twist; Slater-Jastrow-Ewald), or on the 2-D bosonic dipole
+gas (N dipoles at density n r0^2; McMillan Jastrow, 1/r^3); n_walkers
+walkers, n_blocks blocks of sweeps_per_block sweeps.  Requests arrive over time; the server admits them,
 in arrival order, as soon as their device footprint fits the HBM budget and an
 active slot is free; admitted jobs live on the GPU as their own walker batch
 (their own alea_system, whose work runs on its own non-blocking stream), so
@@ -27,7 +28,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .system import heg_spec, uniform_positions
+from .system import dipole_spec, heg_spec, uniform_positions
 from .walkers import FP64, MIXED, gpu_walkers, seed as seed_walkers
 
 
@@ -43,6 +44,7 @@ class Job:
     sweeps_per_block: int = 50
     seed: int = 0
     precision: int = MIXED
+    kind: str = "heg"              # "heg" or "dipoles" (rs is then 1/sqrt(pi n r0^2))
     # filled in by the server
     admitted: float | None = None
     finished: float | None = None
@@ -60,34 +62,48 @@ class Job:
 
 
 def footprint_bytes(n_particles: int, n_walkers: int, precision: int, n_kj: int = 40,
-                    complex_orbitals: bool = True) -> int:
+                    complex_orbitals: bool = True, slater: bool = True) -> int:
     """Device bytes of one job's walker batch, as the C-ABI allocates them
     (alea_system::build): positions, k-space densities, log parts, the two
     determinant inverses (rows padded to 16) in the determinant precision,
     row sums, condition estimates, rebuild workspace (FP64), RNG states and
     the per-walker scalars."""
-    nb = n_particles // 2
+    nb = n_particles // 2 if slater else 0
     rows = (nb + 15) // 16 * 16
     e = 2 if complex_orbitals else 1
     t = 4 if precision == MIXED else 8
     inv = 2 * e * rows * nb
     inv += inv & 1
-    per_walker = (16 * n_particles + 16 * 2 * n_kj + 32 + inv * t + 16 * nb + 16 + 8
-                  + 8 * 2 * e * nb * nb + 32 + 16 + 16 + 24 + 8 + 4 + 8 + 4)
+    per_walker = (16 * n_particles + 16 * 2 * n_kj + 32 + inv * t + 16 * max(nb, 1) + 16 + 8
+                  + 8 * max(1, 2 * e * nb * nb) + 32 + 16 + 16 + 24 + 8 + 4 + 8 + 4)
     return per_walker * n_walkers + (1 << 20)
+
+
+def job_footprint(j) -> int:
+    return footprint_bytes(j.n_particles, j.n_walkers, j.precision, slater=j.kind == "heg")
 
 
 def poisson_jobs(n_jobs: int, rate: float, seed: int = 4, sizes=(26, 58, 74, 122, 242),
                  rs_choices=(30.0, 40.0, 60.0), walkers=(512, 8192), n_blocks: int = 20,
-                 sweeps_per_block: int = 50, precision: int = MIXED) -> list:
+                 sweeps_per_block: int = 50, precision: int = MIXED, dipole_fraction: float = 0.0,
+                 dipoles=(256, 1.0)) -> list:
     """A stream of requests: Poisson arrivals at `rate` per second; N, rs and
     the twist (uniform in the first Brillouin zone, redrawn while it splits a
     degenerate |k + theta| shell at the Fermi level) and a log-uniform walker
-    count rounded to 64, all from `seed`."""
+    count rounded to 64, all from `seed`.  A `dipole_fraction` of the jobs are
+    dipole-gas jobs of dipoles = (N, n r0^2)."""
     rng = np.random.default_rng(seed)
     jobs, t = [], 0.0
     for i in range(n_jobs):
         t += rng.exponential(1.0 / rate)
+        if dipole_fraction > 0.0 and rng.random() < dipole_fraction:
+            lo, hi = math.log2(walkers[0]), math.log2(walkers[1])
+            nw = int(round(2.0 ** rng.uniform(lo, hi) / 64.0)) * 64
+            nd, dens = dipoles
+            jobs.append(Job(i, t, nd, 1.0 / math.sqrt(math.pi * dens), (0.0, 0.0), max(64, nw), n_blocks,
+                            sweeps_per_block, seed=int(rng.integers(1 << 62)), precision=precision,
+                            kind="dipoles"))
+            continue
         n = int(rng.choice(sizes))
         rs = float(rng.choice(rs_choices))
         while True:
@@ -112,7 +128,10 @@ class GpuRunner:
     """One admitted job on the GPU: its walker batch and the block in flight."""
 
     def __init__(self, job: Job):
-        spec = heg_spec(job.n_particles // 2, job.n_particles // 2, job.rs, twist=job.twist)
+        if job.kind == "dipoles":
+            spec = dipole_spec(job.n_particles, density=1.0 / (math.pi * job.rs * job.rs))
+        else:
+            spec = heg_spec(job.n_particles // 2, job.n_particles // 2, job.rs, twist=job.twist)
         self.job = job
         self.w = gpu_walkers(spec, job.n_walkers, job.precision)
         self.w.set_positions(uniform_positions(spec, job.n_walkers, np.random.default_rng(job.seed)))
@@ -157,7 +176,7 @@ class Server:
     def run(self, jobs: list) -> dict:
         pending = sorted(jobs, key=lambda j: (j.arrival, j.job_id))
         for j in pending:
-            if footprint_bytes(j.n_particles, j.n_walkers, j.precision) > self.budget:
+            if job_footprint(j) > self.budget:
                 raise ValueError(f"job {j.job_id} needs more than the whole HBM budget")
         active = []          # (runner, bytes, block_start)
         used = 0
@@ -168,7 +187,7 @@ class Server:
             # admit in arrival order while memory and slots allow
             while pending and pending[0].arrival <= now and len(active) < self.max_active:
                 j = pending[0]
-                need = footprint_bytes(j.n_particles, j.n_walkers, j.precision)
+                need = job_footprint(j)
                 if used + need > self.budget:
                     break
                 pending.pop(0)

@@ -151,6 +151,8 @@ class Spec:
     slater_gauss: list = field(default_factory=list)    # (first, last, centers, alphas)
     ewald: dict | None = None
     coulomb: list = field(default_factory=list)         # (ga, gb, eps)
+    inverse_power: list = field(default_factory=list)   # (ga, gb, strength, exponent)
+    mcmillan: list = field(default_factory=list)        # (ga, gb, b, n)
     name: str = ""
 
 
@@ -187,6 +189,22 @@ def heg_spec(n_up: int, n_dn: int, rs: float, twist=None, slater=True, jastrow=T
                 s.slater_twisted.append((first, last, twisted_wavevectors(lat, n, twist)))
     if ewald:
         s.ewald = cell["ewald"]
+    return s
+
+
+def dipole_spec(n: int, density: float = 1.0, b: float | None = None, n_exp: int = 5,
+                lattice: str = "hexagonal") -> Spec:
+    """2-D bosonic dipoles perpendicular to the plane (configs/
+    dipoles_2d_gas_n256.yaml): one group of n bosons of mass 1, H = -1/2 sum
+    nabla^2 + sum_{i<j} 1/r^3 in units of r0, density n r0^2 = `density`
+    (rs = 1/sqrt(pi density)), and a Jastrow-only trial: McMillan
+    u = -(b/r)^5 (b defaults to the mean spacing rs)."""
+    rs = 1.0 / math.sqrt(math.pi * density)
+    lat = hexagonal_lattice(n, rs) if lattice == "hexagonal" else square_lattice(n, rs)
+    s = Spec(n_particles=n, lattice=lat, name=f"dipoles N={n} n r0^2={density:g}")
+    s.groups.append((0, n, 0.0, 1.0))
+    s.mcmillan.append((0, 0, rs if b is None else b, n_exp))
+    s.inverse_power.append((0, 0, 1.0, 3.0))
     return s
 
 

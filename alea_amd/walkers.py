@@ -85,6 +85,10 @@ class Walkers:
                 e["n_knots"], ptr(sp), e["r_cut"], e["inv_delta"], e["v_long_zero"], e["sr_k0"])
         for ga, gb, eps in s.coulomb:
             api("add_coulomb", h, ga, gb, eps)
+        for ga, gb, c, p in s.inverse_power:
+            api("add_inverse_power", h, ga, gb, c, p)
+        for ga, gb, b, n in s.mcmillan:
+            api("add_mcmillan_jastrow", h, ga, gb, b, n)
 
     # ── state ──────────────────────────────────────────────────────────────
     def set_positions(self, pos):

@@ -124,7 +124,8 @@ def cpu_baseline(tau, walkers_per_core=24):
 
 
 SERVING = dict(n_jobs=16, rate_per_s=4.0, seed=4, n_blocks=4, sweeps_per_block=10,
-               walkers=(512, 8192), sizes=(26, 58, 74, 122, 242), rs=(30.0, 40.0, 60.0))
+               walkers=(512, 8192), sizes=(26, 58, 74, 122, 242), rs=(30.0, 40.0, 60.0),
+               dipole_fraction=0.2, dipoles=(256, 1.0))
 
 
 def serving_leg(precision=1):
@@ -136,7 +137,8 @@ def serving_leg(precision=1):
     c = SERVING
     jobs = serving.poisson_jobs(c["n_jobs"], c["rate_per_s"], seed=c["seed"], sizes=c["sizes"],
                                 rs_choices=c["rs"], walkers=c["walkers"], n_blocks=c["n_blocks"],
-                                sweeps_per_block=c["sweeps_per_block"], precision=precision)
+                                sweeps_per_block=c["sweeps_per_block"], precision=precision,
+                                dipole_fraction=c["dipole_fraction"], dipoles=c["dipoles"])
     def progress(j):
         print(f"bench: serving job {j.job_id} (N={j.n_particles}, {j.n_walkers} walkers) done at "
               f"{j.finished:.1f} s", file=sys.stderr, flush=True)

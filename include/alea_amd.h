@@ -97,6 +97,19 @@ int alea_add_ewald(alea_system* s, int n_k, const int* n1, const int* n2, const 
 /* CoulombTerm(group_a, group_b, epsilon) — src/hamiltonian/coulomb.h:28 (open cell). */
 int alea_add_coulomb(alea_system* s, int group_a, int group_b, double epsilon);
 
+/* InversePowerTerm(group_a, group_b, strength C, exponent p, cell) —
+ * src/hamiltonian/inverse_power.h:56-60: sum of C / r^p over the pairs at the
+ * minimum image, plus in a periodic cell the continuum tail outside the
+ * Wigner-Seitz cell (inverse_power.cpp:43-83).  p = 3, C = 1: dipoles
+ * perpendicular to the plane. */
+int alea_add_inverse_power(alea_system* s, int group_a, int group_b, double strength,
+                           double exponent);
+
+/* McMillan pair Jastrow u(r) = -(b/r)^n at the minimum image (n = 5 is
+ * McMillan's form for helium and the dipole gas).  No counterpart in the
+ * reference: the dipole jobs of the serving stream name it. */
+int alea_add_mcmillan_jastrow(alea_system* s, int group_a, int group_b, double b, int n);
+
 /* WalkerSet::load_all (walker_set.cpp:51-70): positions are wrapped into the cell. */
 int alea_set_positions(alea_system* s, const double* pos);
 int alea_get_positions(alea_system* s, double* pos);

@@ -401,6 +401,7 @@ struct Spline {
 // (drummond_jastrow.cpp:27-80).
 struct Radial {
     bool drummond = false;
+    bool mcmillan = false;   // u = -(b/r)^n (no counterpart in the reference), b = c[0], n = num
     int num = 0, den = 0;
     std::vector<double> c;
     bool image = false;
@@ -421,6 +422,16 @@ struct Radial {
         }
     }
     void eval(double r, double& u, double& du, double& d2u) const {
+        if (mcmillan) {
+            const double x = c[0] / r;
+            double xn = 1.0;
+            for (int i = 0; i < num; ++i) xn *= x;
+            const double n = static_cast<double>(num);
+            u = -xn;
+            du = n * xn / r;
+            d2u = -n * (n + 1.0) * xn / (r * r);
+            return;
+        }
         if (drummond) {
             auto ipow = [](double b, int e) { double x = 1.0; for (int i = 0; i < e; ++i) x *= b; return x; };
             const double t = r - L;
@@ -513,6 +524,13 @@ struct Ewald {
     int n1_max = 0, n2_max = 0;
 };
 
+// InversePowerTerm (inverse_power.h:17-95): C / r^p over the pairs at the
+// minimum image plus the continuum tail outside the Wigner-Seitz cell.
+struct InversePower {
+    int ga = 0, gb = 0, p_int = 0;
+    double c = 0.0, p = 0.0, tail = 0.0;
+};
+
 struct CoulombPair {
     int ga, gb;
     double inv_eps;
@@ -527,6 +545,8 @@ struct Sys {
     std::vector<std::unique_ptr<Component>> comps;
     Ewald ewald;
     std::vector<CoulombPair> coulomb;
+    std::vector<InversePower> ipow;
+    double lattice[4]{};           // primitive vectors as given
     std::vector<double> inv_mass;  // [np]
     std::vector<double> e_loc, e_loc_im;
     // DMC population state (walker_set.h:135-173): weights, E_L before the
@@ -1522,6 +1542,56 @@ void coulomb_energy(Sys& s, const CoulombPair& c, std::vector<double>& le) {
             }
 }
 
+// InversePowerTerm::evaluate_potential (inverse_power.cpp:130-168).
+void inverse_power_energy(Sys& s, const InversePower& t, std::vector<double>& le) {
+    const Group& a = s.groups[t.ga];
+    const Group& b = s.groups[t.gb];
+    const bool intra = t.ga == t.gb;
+    for (std::size_t i = a.first; i < a.last; ++i)
+        for (std::size_t j = intra ? i + 1 : b.first; j < (intra ? a.last : b.last); ++j)
+            for (std::size_t w = 0; w < s.nw; ++w) {
+                const double r = s.D(i, j, w);
+                if (t.p_int > 0) {
+                    const double inv_r = 1.0 / r;
+                    double inv_rp = inv_r;
+                    for (int k = 1; k < t.p_int; ++k) inv_rp *= inv_r;
+                    le[w] += t.c * inv_rp;
+                } else {
+                    le[w] += t.c * std::pow(r, -t.p);
+                }
+            }
+    if (t.tail != 0.0)
+        for (std::size_t w = 0; w < s.nw; ++w) le[w] += t.tail;
+}
+
+// outside_wigner_seitz_integral in 2-D (inverse_power.cpp:28-41,
+// wigner_seitz_exterior.h:58-83): (1/(p - 2)) times the midpoint rule on
+// 2^18 angles of rho^(2 - p), rho from the bisectors of the 24 translations
+// s a1 + t a2 with |s|, |t| <= 2.
+double ws_exterior_integral(const double* a, double p) {
+    std::vector<double> lx, ly;
+    for (int si = -2; si <= 2; ++si)
+        for (int ti = -2; ti <= 2; ++ti) {
+            if (si == 0 && ti == 0) continue;
+            lx.push_back(si * a[0] + ti * a[2]);
+            ly.push_back(si * a[1] + ti * a[3]);
+        }
+    constexpr std::size_t n_theta = std::size_t{1} << 18;
+    const double dtheta = 2.0 * kPi / static_cast<double>(n_theta);
+    double sum = 0.0;
+    for (std::size_t k = 0; k < n_theta; ++k) {
+        const double theta = (static_cast<double>(k) + 0.5) * dtheta;
+        const double ux = std::cos(theta), uy = std::sin(theta);
+        double rho = std::numeric_limits<double>::infinity();
+        for (std::size_t c = 0; c < lx.size(); ++c) {
+            const double proj = ux * lx[c] + uy * ly[c];
+            if (proj > 0.0) rho = std::min(rho, 0.5 * (lx[c] * lx[c] + ly[c] * ly[c]) / proj);
+        }
+        sum += std::pow(rho, 2.0 - p);
+    }
+    return sum * dtheta / (p - 2.0);
+}
+
 void totals(Sys& s, std::vector<double>& g, std::vector<double>& l, std::vector<double>* pg,
             std::vector<double>* pl) {
     const std::size_t n = s.np * 2 * s.nw;
@@ -1568,6 +1638,7 @@ void local_energy(Sys& s) {
             }
     if (s.ewald.active) ewald_energy(s, s.e_loc);
     for (const auto& c : s.coulomb) coulomb_energy(s, c, s.e_loc);
+    for (const auto& t : s.ipow) inverse_power_energy(s, t, s.e_loc);
 }
 
 // TrialWavefunction::ratio_batch (trial_wavefunction.cpp:46-78).
@@ -1648,6 +1719,7 @@ int oracle_create(oracle_system** out, int nw, int np, int dim, const double* la
         s->np = static_cast<std::size_t>(np);
         s->dim = 2;
         s->cell.init(2, lattice);
+        if (lattice) std::copy(lattice, lattice + 4, s->lattice);
         s->alloc();
         s->inv_mass.assign(s->np, 1.0);
         *out = reinterpret_cast<oracle_system*>(s);
@@ -1735,6 +1807,54 @@ int oracle_add_drummond_jastrow(oracle_system* h, int ga, int gb, int C, double 
         for (int i = 0; i < C; ++i) ipw *= -L;
         c->rad.alpha1 = gamma / ipw + coeffs[0] * static_cast<double>(C) / L;
         s.comps.push_back(std::move(c));
+    });
+}
+
+int oracle_add_mcmillan_jastrow(oracle_system* h, int ga, int gb, double b, int n) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        check_index(s, ga);
+        check_index(s, gb);
+        if (!(b > 0.0) || !std::isfinite(b)) throw std::invalid_argument("oracle: mcmillan b must be positive");
+        if (n < 1 || n > 16) throw std::invalid_argument("oracle: mcmillan exponent must be 1..16");
+        auto c = std::make_unique<PairJastrow>();
+        c->ga = s.groups[ga];
+        c->gb = s.groups[gb];
+        c->same = ga == gb;
+        c->rad.mcmillan = true;
+        c->rad.num = n;
+        c->rad.c = {b};
+        s.comps.push_back(std::move(c));
+    });
+}
+
+// InversePowerTerm constructor (inverse_power.cpp:43-83).
+int oracle_add_inverse_power(oracle_system* h, int ga, int gb, double strength, double exponent) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        check_index(s, ga);
+        check_index(s, gb);
+        if (!std::isfinite(exponent) || exponent <= 0.0)
+            throw std::invalid_argument("oracle: inverse_power exponent must be finite and positive");
+        if (!std::isfinite(strength)) throw std::invalid_argument("oracle: inverse_power strength must be finite");
+        if (strength < 0.0 && exponent >= 2.0)
+            throw std::invalid_argument("oracle: inverse_power attractive core with exponent >= 2");
+        InversePower t;
+        t.ga = ga;
+        t.gb = gb;
+        t.c = strength;
+        t.p = exponent;
+        const double rounded = std::round(exponent);
+        if (rounded == exponent && rounded <= 16.0) t.p_int = static_cast<int>(rounded);
+        if (s.cell.periodic) {
+            if (exponent <= 2.0) throw std::invalid_argument("oracle: inverse_power in a periodic cell needs p > 2");
+            const double na = static_cast<double>(s.groups[ga].last - s.groups[ga].first);
+            const double nb = static_cast<double>(s.groups[gb].last - s.groups[gb].first);
+            const double pairs = ga == gb ? 0.5 * na * na : na * nb;
+            const double area = std::fabs(s.lattice[0] * s.lattice[3] - s.lattice[1] * s.lattice[2]);
+            t.tail = pairs / area * strength * ws_exterior_integral(s.lattice, exponent);
+        }
+        s.ipow.push_back(t);
     });
 }
 

@@ -52,6 +52,8 @@ int oracle_add_ewald(oracle_system* s, int n_k, const int* n1, const int* n2,
                      const double* spline, double r_cut, double inv_delta,
                      double v_long_zero, double sr_k0);
 int oracle_add_coulomb(oracle_system* s, int ga, int gb, double epsilon);
+int oracle_add_inverse_power(oracle_system* s, int ga, int gb, double strength, double exponent);
+int oracle_add_mcmillan_jastrow(oracle_system* s, int ga, int gb, double b, int n);
 
 int oracle_set_positions(oracle_system* s, const double* pos);
 int oracle_get_positions(oracle_system* s, double* pos);

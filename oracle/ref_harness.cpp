@@ -27,6 +27,7 @@
 #include "hamiltonian/coulomb.h"
 #include "hamiltonian/ewald_term.h"
 #include "hamiltonian/hamiltonian.h"
+#include "hamiltonian/inverse_power.h"
 #include "hamiltonian/kinetic.h"
 #include "numerics/ewald_breakup.h"
 #include "numerics/long_range_breakup.h"
@@ -73,6 +74,8 @@ struct RefSystem {
     bool use_ewald = false;
     struct CoulombPair { int a, b; double eps; };
     std::vector<CoulombPair> coulomb;
+    struct InversePowerSpec { int a, b; double c, p; };
+    std::vector<InversePowerSpec> ipow;
 };
 
 alea::ParticleGroup make_group(std::size_t first, std::size_t last, double charge,
@@ -98,6 +101,10 @@ void build_hamiltonian(RefSystem& s) {
         s.ham.add_component(std::make_unique<alea::CoulombTerm>(
             s.groups[static_cast<std::size_t>(c.a)], s.groups[static_cast<std::size_t>(c.b)],
             c.eps));
+    for (const auto& t : s.ipow)
+        s.ham.add_component(std::make_unique<alea::InversePowerTerm>(
+            s.groups[static_cast<std::size_t>(t.a)], s.groups[static_cast<std::size_t>(t.b)], t.c,
+            t.p, s.cell));
     s.ham_built = true;
 }
 
@@ -302,6 +309,13 @@ int ref_add_coulomb(void* h, int ga, int gb, double eps) {
 // Positions in the reference SoA layout [p*dim*nw + d*nw + w]; they are wrapped
 // into the cell and the distance tables rebuilt (WalkerSet::load_all,
 // src/population/walker_set.cpp:51-70), then ψ is evaluated.
+int ref_add_inverse_power(void* h, int ga, int gb, double c, double p) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        s.ipow.push_back({ga, gb, c, p});
+    });
+}
+
 int ref_set_positions(void* h, const double* pos) {
     return guarded([&] {
         auto& s = *static_cast<RefSystem*>(h);

@@ -96,6 +96,7 @@ def ref_lib():
                                          C.c_int, C.c_double, C.c_double, C.c_int, D, D,
                                          C.POINTER(C.c_int), C.c_int, D, D, D, C.POINTER(C.c_int), D, D]),
             "ref_robust_spread": (C.c_double, [D, C.c_int]),
+            "ref_add_inverse_power": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(lib, name)
@@ -134,6 +135,10 @@ class RefWalkers:
             self._chk(lib.ref_add_ewald(h, ewald_tol or spec.ewald.get("tol", 1e-6), 1.0))
         for ga, gb, eps in spec.coulomb:
             self._chk(lib.ref_add_coulomb(h, ga, gb, eps))
+        for ga, gb, c, p in spec.inverse_power:
+            self._chk(lib.ref_add_inverse_power(h, ga, gb, c, p))
+        if spec.mcmillan:
+            raise ValueError("the reference has no McMillan Jastrow")
 
     def _chk(self, rc):
         if rc:

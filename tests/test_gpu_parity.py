@@ -9,7 +9,7 @@ decisions on at least 98% of walkers of the N=122 system.
 import numpy as np
 import pytest
 
-from alea_amd.system import exciton_spec, heg_spec, hexagonal_lattice, uniform_positions
+from alea_amd.system import dipole_spec, exciton_spec, heg_spec, hexagonal_lattice, uniform_positions
 from alea_amd.walkers import FP64, MIXED, gpu_walkers, seed
 from backends import oracle_tape, oracle_walkers, rel_err
 
@@ -39,6 +39,9 @@ def specs():
     s.name = "drummond+nosanow N=10"
     out.append(s)
     s = exciton_spec()
+    out.append(s)
+    s = dipole_spec(16, density=4.0)
+    s.name = "dipoles N=16 mcmillan + 1/r^3"
     out.append(s)
     return out
 

@@ -30,7 +30,12 @@ def test_job_results_same_alone_and_in_mix(precision):
 
 
 def test_poisson_stream_completes():
-    jobs = serving.poisson_jobs(6, rate=50.0, seed=4, walkers=(512, 2048), n_blocks=2, sweeps_per_block=3)
+    jobs = serving.poisson_jobs(6, rate=50.0, seed=4, walkers=(512, 2048), n_blocks=2, sweeps_per_block=3,
+                                dipole_fraction=0.5, dipoles=(64, 1.0))
+    assert {j.kind for j in jobs} == {"heg", "dipoles"}
     st = serving.Server(max_active=4).run(jobs)
     assert st["jobs"] == 6 and st["p99_block_ms"] >= st["p50_block_ms"] > 0
-    assert all(abs(e) < 1.0 for j in jobs for e in j.block_energy)
+    assert all(abs(e) < 1.0 for j in jobs if j.kind == "heg" for e in j.block_energy)
+    # a few sweeps from a uniform start: the McMillan core still dominates
+    # E_L of close pairs, so only finiteness is a property here
+    assert all(np.isfinite(e) for j in jobs if j.kind == "dipoles" for e in j.block_energy)

@@ -20,7 +20,7 @@ import math
 import numpy as np
 import pytest
 
-from alea_amd.system import Spec, exciton_spec, heg_spec, hexagonal_lattice, load_tables, uniform_positions
+from alea_amd.system import dipole_spec, Spec, exciton_spec, heg_spec, hexagonal_lattice, load_tables, uniform_positions
 from backends import RefWalkers, oracle_lib, oracle_tape, oracle_walkers, ref_lib, rel_err
 import ctypes as C
 
@@ -203,6 +203,14 @@ def _jastrow_ewald_specs():
     dr.drummond.append(dict(ga=0, gb=0, C=3, gamma=1 / 3, L=L, coeffs=[2e-3, 1e-4]))
     dr.nosanow.append(dict(group=1, alpha=0.6, sites=np.random.default_rng(9).uniform(-5, 5, 10)))
     specs.append(dr)
+    # dipoles: inverse-power 1/r^3 with its Wigner-Seitz continuum tail, with a
+    # plain (minimum-image) Padé Jastrow since the reference has no McMillan
+    dp = dipole_spec(16, density=4.0)
+    dp.mcmillan = []
+    rs = 1.0 / np.sqrt(np.pi * 4.0)
+    dp.pade.append(dict(ga=0, gb=0, num=1, den=1, coeffs=[0.3, 1.0 / rs]))
+    dp.name = "dipoles N=16 inverse power"
+    specs.append(dp)
     return specs
 
 
@@ -239,3 +247,79 @@ def test_oracle_matches_reference(spec):
     assert np.array_equal(o.positions(), r.positions())
     assert rel_err(o.log_psi(), r.log_psi()) < 1e-11
     assert rel_err(o.local_energy()[0], r.local_energy()[0]) < 1e-10
+
+
+def test_mcmillan_closed_form():
+    """u = -(b/r)^5: two bosons in an open cell give log psi = -(b/r)^5 and
+    the analytic gradient and Laplacian, with no cusp term."""
+    from alea_amd.system import Spec
+    s = Spec(n_particles=2, lattice=None, name="mcmillan pair")
+    s.groups = [(0, 2, 0.0, 1.0)]
+    s.mcmillan = [(0, 0, 1.3, 5)]
+    nw = 3
+    w = oracle_walkers(s, nw)
+    pos = np.random.default_rng(2).uniform(-2, 2, 4 * nw)
+    w.set_positions(pos)
+    w.evaluate()
+    P = pos.reshape(2, 2, nw)
+    d = P[0] - P[1]
+    r = np.sqrt((d ** 2).sum(0))
+    assert np.allclose(w.log_psi(), -(1.3 / r) ** 5, rtol=1e-13)
+    g, lap = w.gradients()
+    du = 5 * (1.3 / r) ** 5 / r                  # d/dr of -(b/r)^5
+    d2u = -30 * (1.3 / r) ** 5 / r ** 2
+    assert np.allclose(g.reshape(2, 2, nw)[0], du * d / r, rtol=1e-12)
+    # per-dimension second derivative: u'' (d/r)^2 + (u'/r)(1 - (d/r)^2)
+    lap0 = d2u * (d / r) ** 2 + du / r * (1 - (d / r) ** 2)
+    assert np.allclose(lap.reshape(2, 2, nw)[0], lap0, rtol=1e-11)
+
+
+def test_dipole_gradients_match_finite_differences():
+    s = dipole_spec(16, density=4.0)
+    nw = 2
+    w = oracle_walkers(s, nw)
+    w.set_positions(uniform_positions(s, nw, np.random.default_rng(6)))
+    w.evaluate()
+    g, lap = w.gradients()
+    pos = w.positions()
+    h = 1e-6
+    for p in (0, 9):
+        for dd in (0, 1):
+            lps = []
+            for sgn in (-1, 0, 1):
+                q = pos.copy()
+                q[(p * 2 + dd) * nw:(p * 2 + dd + 1) * nw] += sgn * h
+                w2 = oracle_walkers(s, nw)
+                w2.set_positions(q)
+                w2.evaluate()
+                lps.append(w2.log_psi())
+            fd = (lps[2] - lps[0]) / (2 * h)
+            idx = slice((p * 2 + dd) * nw, (p * 2 + dd + 1) * nw)
+            assert np.allclose(g[idx], fd, rtol=1e-5, atol=1e-6 * max(1, np.abs(fd).max()))
+
+
+def test_inverse_power_tail_on_a_triangular_crystal():
+    """A perfect triangular crystal of N = 64 dipoles: minimum image plus the
+    Wigner-Seitz continuum tail falls short of the exact lattice sum by the
+    relative -6.9e-4 the reference quotes for N = 64 (inverse_power.h:44-46)."""
+    n = 64
+    s = dipole_spec(n, density=1.0)
+    s.mcmillan = []
+    L = s.lattice
+    a1, a2 = np.array(L[:2]) / 8, np.array(L[2:]) / 8
+    pts = np.array([i * a1 + j * a2 for i in range(8) for j in range(8)])
+    w = oracle_walkers(s, 1)
+    w.set_positions(pts.reshape(n, 2, 1).reshape(-1))
+    w.evaluate()
+    e = w.local_energy()[0][0] / n
+    # exact: (1/2) sum over the infinite triangular lattice of 1/|R|^3, summed
+    # inside a disk of radius Rc and continued by the uniform remainder
+    # (1/2) 2 pi n / Rc (its error falls like Rc^-3)
+    ij = np.arange(-120, 121)
+    I, J = np.meshgrid(ij, ij)
+    R = np.stack([I.ravel(), J.ravel()], 1) @ np.stack([a1, a2])
+    d = np.linalg.norm(R, axis=1)
+    Rc = 100 * np.linalg.norm(a1)
+    d = d[(d > 0) & (d < Rc)]
+    exact = 0.5 * (np.sum(d ** -3.0) + 2 * np.pi * 1.0 / Rc)
+    assert -7.0e-4 < e / exact - 1 < -6.8e-4
