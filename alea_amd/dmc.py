@@ -154,7 +154,10 @@ class DMC:
         self.branch_state = xoshiro_state(stream_seed(seed, K_DMC_BRANCH_STREAM, 0))
         self.step_index = 0
         e, _ = walkers.local_energy()
-        self.pc.start(float(np.sum(e)) / e.size)    # dmc.cpp:628-635, summed in walker order
+        e_sum = 0.0
+        for x in e.tolist():                        # dmc.cpp:628-635, left to right in walker order
+            e_sum += x
+        self.pc.start(e_sum / e.size)
         self.refresh_energy_spread(e)
         self.history = []
 

@@ -1,0 +1,514 @@
+// alea_amd C++20 host interface over the C-ABI (alea_amd.h).
+//
+// The reference's host-side interfaces for the walker sweep, kept so that its
+// drivers can run on the device batch unchanged in shape:
+//   WalkerSet           src/population/walker_set.h:30-223 (the batch, its SoA
+//                       planes, apply_permutation, branch)
+//   TrialWavefunction   src/wavefunction/trial_wavefunction.h:17-130
+//                       (add_component, evaluate_batch, ratio_batch,
+//                       apply_move_batch, refresh_derivatives)
+//   Hamiltonian         src/hamiltonian/hamiltonian.h:20-40 (add_component,
+//                       evaluate_batch)
+//   PopulationControl   src/qmc/population_control.h:150-231
+//   DMC::run_one_step   src/qmc/dmc.cpp:693-739 over advance_measure (bare
+//                       reweighting, dmc.cpp:339-419), one crowd
+// One WalkerSet is one device system: its TrialWavefunction and Hamiltonian
+// register their factors on it, and every batch call runs HIP kernels.
+// Status codes become the reference's exceptions: ALEA_ERR_INVALID ->
+// std::invalid_argument, others -> std::runtime_error.
+//
+// Host arrays use the reference's walker-fastest SoA layouts:
+//   positions, gradients, laplacians   [p*dim*nw + d*nw + w]
+//   new_pos, new_grad, new_lap         [d*nw + w]
+#ifndef ALEA_AMD_HPP
+#define ALEA_AMD_HPP
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <limits>
+#include <span>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "alea_amd.h"
+
+namespace alea_amd {
+
+inline void check(int rc) {
+    if (rc == ALEA_OK) return;
+    const std::string msg = alea_last_error();
+    if (rc == ALEA_ERR_INVALID) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+enum class Precision { fp64 = ALEA_FP64, mixed = ALEA_MIXED };
+
+// ParticleGroup (src/population/particle_group.h:17), isotropic mass.
+struct ParticleGroup {
+    std::size_t first = 0, last = 0;
+    double charge = -1.0;
+    double mass = 1.0;
+};
+
+// ── factor descriptors: the reference's constructor arguments ─────────────
+// Breakup tables of an image-summed factor or of the Ewald term: the k set,
+// weights, quintic spline (3 per knot), r_cut, 1/delta.
+struct KSpaceTable {
+    std::vector<int> n1, n2;
+    std::vector<double> kx, ky, weights, spline;
+    double r_cut = 0.0, inv_delta = 0.0;
+};
+
+// PadeJastrow(num_order, den_order, group_a, group_b, params) — pade_jastrow.h:60;
+// params = [a_1..a_N, b_1..b_M]; image set: PeriodicPairSum (periodic_pair_sum.h:45)
+struct PadeJastrow {
+    int num_order = 1, den_order = 1;
+    int group_a = 0, group_b = 0;
+    std::vector<double> params;
+    bool image_sum = false;
+    KSpaceTable image;
+    double u_inf = 0.0;
+};
+// DrummondJastrow(C, gamma, group_a, group_b, params) — drummond_jastrow.h:77
+struct DrummondJastrow {
+    int C = 3;
+    double gamma = 0.0, L = 0.0;
+    int group_a = 0, group_b = 0;
+    std::vector<double> params;
+};
+// NosanowJastrow(first, sites, {alpha}) — nosanow_jastrow.h:55
+struct NosanowJastrow {
+    int group = 0;
+    double alpha = 0.0;
+    std::vector<double> sites;
+};
+// McMillan u = -(b/r)^n (no counterpart in the reference)
+struct McMillanJastrow {
+    int group_a = 0, group_b = 0;
+    double b = 1.0;
+    int n = 5;
+};
+// SlaterDeterminant + PlaneWaveOrbitalSet / GaussianOrbitalSet, and
+// ComplexSlaterDeterminant + TwistedPlaneWaveOrbitalSet (orbital_set.h,
+// complex_orbital_set.h) on particles [first, last)
+struct PlaneWaveSlater {
+    std::size_t first = 0, last = 0;
+    std::vector<double> kvecs;
+    std::vector<unsigned char> is_sin;
+};
+struct GaussianSlater {
+    std::size_t first = 0, last = 0;
+    std::vector<double> centers, alphas;
+};
+struct TwistedSlater {
+    std::size_t first = 0, last = 0;
+    std::vector<double> q;
+};
+// EwaldTerm with its EwaldSetup (ewald_term.h:70, ewald_breakup.h:186)
+struct EwaldTerm {
+    KSpaceTable table;
+    double v_long_zero = 0.0, sr_k0 = 0.0;
+};
+// CoulombTerm(group_a, group_b, epsilon) — coulomb.h:28
+struct CoulombTerm {
+    int group_a = 0, group_b = 0;
+    double epsilon = 1.0;
+};
+// InversePowerTerm(group_a, group_b, strength, exponent, cell) — inverse_power.h:56
+struct InversePowerTerm {
+    int group_a = 0, group_b = 0;
+    double strength = 1.0, exponent = 3.0;
+};
+
+// xoshiro256++ state of a stream seeded as XoshiroRandom(seed)
+// (xoshiro_random.h:26-29) from stream_seed(seed, stream, index) (random.h:52-56).
+struct XoshiroState {
+    std::array<std::uint64_t, 4> s{};
+    static constexpr std::uint64_t splitmix64(std::uint64_t x) {
+        x += 0x9e3779b97f4a7c15ULL;
+        x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+        return x ^ (x >> 31);
+    }
+    static XoshiroState from_seed(std::uint64_t seed) {
+        XoshiroState r;
+        for (std::uint64_t i = 0; i < 4; ++i) r.s[i] = splitmix64(seed + i * 0x9e3779b97f4a7c15ULL);
+        return r;
+    }
+    static XoshiroState stream(std::uint64_t base, std::uint64_t stream, std::uint64_t index) {
+        return from_seed(splitmix64(splitmix64(splitmix64(base) ^ stream) ^ index));
+    }
+};
+
+// ── WalkerSet ──────────────────────────────────────────────────────────────
+class WalkerSet {
+public:
+    // WalkerSet(nw, np, dim, cell): lattice = dim*dim primitive vectors, or
+    // empty for an open cell.  nw is the capacity (DMC populations move below it).
+    WalkerSet(std::size_t nw, std::size_t np, std::size_t dim, std::span<const double> lattice = {},
+              Precision precision = Precision::fp64)
+        : np_(np), dim_(dim) {
+        if (!lattice.empty() && lattice.size() != dim * dim)
+            throw std::invalid_argument("WalkerSet: lattice needs dim*dim components");
+        check(alea_create(&h_, static_cast<int>(nw), static_cast<int>(np), static_cast<int>(dim),
+                          lattice.empty() ? nullptr : lattice.data(), static_cast<int>(precision)));
+        new_pos_.assign(dim * nw, 0.0);
+        capacity_ = nw;
+    }
+    ~WalkerSet() { alea_destroy(h_); }
+    WalkerSet(const WalkerSet&) = delete;
+    WalkerSet& operator=(const WalkerSet&) = delete;
+    WalkerSet(WalkerSet&& o) noexcept { *this = std::move(o); }
+    WalkerSet& operator=(WalkerSet&& o) noexcept {
+        std::swap(h_, o.h_);
+        np_ = o.np_;
+        dim_ = o.dim_;
+        capacity_ = o.capacity_;
+        new_pos_ = std::move(o.new_pos_);
+        log_ratios_ = std::move(o.log_ratios_);
+        phase_diffs_ = std::move(o.phase_diffs_);
+        new_grad_ = std::move(o.new_grad_);
+        new_lap_ = std::move(o.new_lap_);
+        return *this;
+    }
+
+    std::size_t num_walkers() const {
+        int n = 0;
+        check(alea_get_num_walkers(h_, &n));
+        return static_cast<std::size_t>(n);
+    }
+    std::size_t n_particles() const { return np_; }
+    std::size_t dim() const { return dim_; }
+    std::size_t capacity() const { return capacity_; }
+
+    void add_group(const ParticleGroup& g) {
+        check(alea_add_group(h_, static_cast<int>(g.first), static_cast<int>(g.last), g.charge, 1.0 / g.mass));
+    }
+    // WalkerSet::load_all (walker_set.cpp:51-70): positions wrapped into the cell
+    void load_positions(std::span<const double> soa) {
+        if (soa.size() != np_ * dim_ * num_walkers()) throw std::invalid_argument("load_positions: wrong size");
+        check(alea_set_positions(h_, soa.data()));
+    }
+    std::vector<double> positions() const {
+        std::vector<double> out(np_ * dim_ * num_walkers());
+        check(alea_get_positions(h_, out.data()));
+        return out;
+    }
+    std::vector<double> log_psi() const {
+        std::vector<double> out(num_walkers());
+        check(alea_get_log_psi(h_, out.data()));
+        return out;
+    }
+    std::vector<double> weights() const {
+        std::vector<double> out(num_walkers());
+        check(alea_get_weights(h_, out.data()));
+        return out;
+    }
+    void set_weights(std::span<const double> w) { check(alea_set_weights(h_, w.data())); }
+
+    // ParticleBatch::propose_move_batch (particle_batch.cpp:43-57): the
+    // proposed position of p, [dim*nw], read by ratio_batch
+    std::span<double> new_pos() {
+        new_pos_.resize(dim_ * num_walkers());
+        return new_pos_;
+    }
+    void propose_move_batch(std::size_t p, std::span<const double> np) {
+        proposed_ = p;
+        std::copy(np.begin(), np.end(), new_pos().begin());
+    }
+    std::span<const double> log_ratios() const { return log_ratios_; }
+    std::span<const double> phase_diffs() const { return phase_diffs_; }
+    std::span<const double> new_grad() const { return new_grad_; }
+    std::span<const double> new_lap() const { return new_lap_; }
+
+    // WalkerSet::apply_permutation (walker_set.cpp:109-178)
+    void apply_permutation(std::span<const std::size_t> src) {
+        std::vector<int> s(src.begin(), src.end());
+        check(alea_permute(h_, static_cast<int>(s.size()), s.data()));
+    }
+    // WalkerSet::branch (walker_set.cpp:187-239)
+    void branch(XoshiroState& rng, std::size_t target, std::vector<std::size_t>& src) {
+        std::vector<int> s(2 * target + 1);
+        int n = 0;
+        check(alea_branch(h_, rng.s.data(), static_cast<int>(target), &n, s.data()));
+        src.assign(s.begin(), s.begin() + n);
+    }
+
+    alea_system* handle() const { return h_; }
+
+private:
+    friend class TrialWavefunction;
+    alea_system* h_ = nullptr;
+    std::size_t np_ = 0, dim_ = 0, capacity_ = 0, proposed_ = 0;
+    std::vector<double> new_pos_, log_ratios_, phase_diffs_, new_grad_, new_lap_;
+};
+
+// ── TrialWavefunction ──────────────────────────────────────────────────────
+class TrialWavefunction {
+public:
+    explicit TrialWavefunction(WalkerSet& ws) : h_(ws.handle()) {}
+
+    void add_component(const PadeJastrow& c) {
+        const KSpaceTable& t = c.image;
+        const int nk = c.image_sum ? static_cast<int>(t.weights.size()) : 0;
+        check(alea_add_pade_jastrow(h_, c.group_a, c.group_b, c.num_order, c.den_order, c.params.data(), nk,
+                                    nk ? t.n1.data() : nullptr, nk ? t.n2.data() : nullptr,
+                                    nk ? t.kx.data() : nullptr, nk ? t.ky.data() : nullptr,
+                                    nk ? t.weights.data() : nullptr, nk ? static_cast<int>(t.spline.size() / 3) : 0,
+                                    nk ? t.spline.data() : nullptr, t.r_cut, t.inv_delta, c.u_inf));
+    }
+    void add_component(const DrummondJastrow& c) {
+        check(alea_add_drummond_jastrow(h_, c.group_a, c.group_b, c.C, c.gamma, c.L,
+                                        static_cast<int>(c.params.size()), c.params.data()));
+    }
+    void add_component(const NosanowJastrow& c) {
+        check(alea_add_nosanow_jastrow(h_, c.group, c.alpha, c.sites.data()));
+    }
+    void add_component(const McMillanJastrow& c) {
+        check(alea_add_mcmillan_jastrow(h_, c.group_a, c.group_b, c.b, c.n));
+    }
+    void add_component(const PlaneWaveSlater& c) {
+        check(alea_add_slater_plane_wave(h_, static_cast<int>(c.first), static_cast<int>(c.last), c.kvecs.data(),
+                                         c.is_sin.data()));
+    }
+    void add_component(const GaussianSlater& c) {
+        check(alea_add_slater_gaussian(h_, static_cast<int>(c.first), static_cast<int>(c.last), c.centers.data(),
+                                       c.alphas.data()));
+    }
+    void add_component(const TwistedSlater& c) {
+        check(alea_add_slater_twisted(h_, static_cast<int>(c.first), static_cast<int>(c.last), c.q.data()));
+    }
+
+    // evaluate_batch (trial_wavefunction.cpp:19-35)
+    void evaluate_batch(WalkerSet& ws) {
+        (void)ws;
+        check(alea_evaluate(h_));
+    }
+    // ratio_batch (trial_wavefunction.cpp:46-78) for the move proposed on ws:
+    // fills ws.log_ratios(), phase_diffs(), new_grad(), new_lap()
+    void ratio_batch(WalkerSet& ws, std::size_t p) {
+        const std::size_t nw = ws.num_walkers(), d = ws.dim();
+        if (p != ws.proposed_) throw std::invalid_argument("ratio_batch: no move of this particle was proposed");
+        ws.log_ratios_.resize(nw);
+        ws.phase_diffs_.resize(nw);
+        ws.new_grad_.resize(d * nw);
+        ws.new_lap_.resize(d * nw);
+        check(alea_ratio(h_, static_cast<int>(p), ws.new_pos_.data(), ws.log_ratios_.data(), ws.phase_diffs_.data(),
+                         ws.new_grad_.data(), ws.new_lap_.data()));
+    }
+    // apply_move_batch + WalkerSet::apply_move (vmc.cpp:147-152)
+    void apply_move_batch(WalkerSet& ws, std::size_t p, std::span<const bool> mask) {
+        std::vector<unsigned char> m(mask.begin(), mask.end());
+        if (m.size() != ws.num_walkers()) throw std::invalid_argument("apply_move_batch: mask size");
+        check(alea_accept(h_, static_cast<int>(p), m.data()));
+    }
+    // refresh_derivatives: the gradient and Laplacian planes of log|psi|
+    void refresh_derivatives(WalkerSet& ws, std::vector<double>& grad, std::vector<double>& lap) {
+        const std::size_t n = ws.n_particles() * ws.dim() * ws.num_walkers();
+        grad.resize(n);
+        lap.resize(n);
+        check(alea_get_gradients(h_, grad.data(), lap.data()));
+    }
+
+private:
+    alea_system* h_;
+};
+
+// ── Hamiltonian ────────────────────────────────────────────────────────────
+class Hamiltonian {
+public:
+    explicit Hamiltonian(WalkerSet& ws) : h_(ws.handle()) {}
+    void add_component(const EwaldTerm& c) {
+        const KSpaceTable& t = c.table;
+        check(alea_add_ewald(h_, static_cast<int>(t.weights.size()), t.n1.data(), t.n2.data(), t.kx.data(),
+                             t.ky.data(), t.weights.data(), static_cast<int>(t.spline.size() / 3), t.spline.data(),
+                             t.r_cut, t.inv_delta, c.v_long_zero, c.sr_k0));
+    }
+    void add_component(const CoulombTerm& c) { check(alea_add_coulomb(h_, c.group_a, c.group_b, c.epsilon)); }
+    void add_component(const InversePowerTerm& c) {
+        check(alea_add_inverse_power(h_, c.group_a, c.group_b, c.strength, c.exponent));
+    }
+    // evaluate_batch (hamiltonian.cpp:18-29): E_L of every walker (and Im E_L
+    // of a complex trial when im is given)
+    std::vector<double> evaluate_batch(WalkerSet& ws, std::vector<double>* im = nullptr) {
+        std::vector<double> e(ws.num_walkers());
+        if (im) im->resize(e.size());
+        check(alea_local_energy(h_, e.data(), im ? im->data() : nullptr));
+        return e;
+    }
+
+private:
+    alea_system* h_;
+};
+
+// ── VMC sweeps ─────────────────────────────────────────────────────────────
+// VMC::sweep_walkers (vmc.cpp:69-155): n fused sweeps, symmetric proposals of
+// width sqrt(tau/m); tape in draw_sweep_randoms order or the device streams
+// seeded here with stream_seed(seed, "vmc-thrd", walker).
+class VMC {
+public:
+    VMC(WalkerSet& ws, double tau) : ws_(ws), tau_(tau) {}
+    void seed(std::uint64_t seed, std::uint64_t stream = 0x766d632d74687264ULL) {
+        check(alea_seed(ws_.handle(), seed, stream));
+    }
+    long long sweep(int n_sweeps, const double* tape = nullptr) {
+        long long acc = 0;
+        check(alea_vmc_sweeps(ws_.handle(), n_sweeps, tau_, tape, &acc));
+        return acc;
+    }
+
+private:
+    WalkerSet& ws_;
+    double tau_;
+};
+
+// ── PopulationControl (population_control.cpp:14-108) ────────────────────
+enum class BranchCutoff { universal, drv, none };
+
+class PopulationControl {
+public:
+    PopulationControl(double tau, int n_substeps, double generations, std::size_t target_walkers,
+                      BranchCutoff cutoff = BranchCutoff::universal, double alpha = 2.0,
+                      std::size_t n_particles = 1)
+        : tau_(tau), cutoff_(cutoff), alpha_(alpha), n_particles_(static_cast<double>(n_particles)),
+          target_walkers_(target_walkers) {
+        if (cutoff == BranchCutoff::universal && !(std::isfinite(alpha) && alpha > 0.0))
+            throw std::invalid_argument("dmc: the universal branch cutoff needs a finite alpha > 0");
+        if (n_particles == 0) throw std::invalid_argument("dmc: the branch cutoff needs at least one particle");
+        const double t_step = tau * static_cast<double>(std::max(1, n_substeps));
+        feedback_rate_ = (t_step > 0.0 && generations > 0.0) ? 1.0 / (generations * t_step) : 0.0;
+        if (cutoff == BranchCutoff::drv && tau > 0.0) set_cutoff(2.0 / std::sqrt(tau));
+    }
+    void start(double energy) {
+        etrial_ = eref_ = energy;
+        eref_sum_ = eref_wsum_ = 0.0;
+    }
+    void set_energy_spread(double sigma) {
+        sigma_ = sigma;
+        if (cutoff_ != BranchCutoff::universal) return;
+        if (!(std::isfinite(sigma) && sigma > 0.0 && tau_ > 0.0)) {
+            set_cutoff(kInf);
+            return;
+        }
+        set_cutoff(alpha_ * std::sqrt(sigma * std::sqrt(n_particles_) / tau_));
+    }
+    double branch_weight(double e_new, double e_old) const {
+        double half_tau = 0.5 * tau_;
+        const double x = std::max(eref_ - e_new, eref_ - e_old);
+        if (x > e_max_)
+            half_tau = 0.0;
+        else if (x > e_cut_)
+            half_tau *= 1.0 - (x - e_cut_) / (e_max_ - e_cut_);
+        return std::exp(half_tau * (2.0 * etrial_ - e_new - e_old));
+    }
+    void update(double ensemble_energy, double total_weight, bool accumulate) {
+        if (accumulate) {
+            eref_sum_ += ensemble_energy * total_weight;
+            eref_wsum_ += total_weight;
+            if (eref_wsum_ > 0.0) eref_ = eref_sum_ / eref_wsum_;
+        } else {
+            eref_ = ensemble_energy;
+        }
+        if (total_weight > 0.0)
+            etrial_ = eref_ - feedback_rate_ * std::log(total_weight / static_cast<double>(target_walkers_));
+    }
+    void begin_accumulation() { eref_sum_ = eref_wsum_ = 0.0; }
+    double etrial() const { return etrial_; }
+    double eref() const { return eref_; }
+    double energy_cutoff() const { return e_cut_; }
+    double e_max() const { return e_max_; }
+    double tau() const { return tau_; }
+    std::size_t target_walkers() const { return target_walkers_; }
+
+private:
+    static constexpr double kInf = std::numeric_limits<double>::infinity();
+    void set_cutoff(double e) {
+        e_cut_ = e;
+        e_max_ = 1.5 * e;
+    }
+    double tau_;
+    BranchCutoff cutoff_;
+    double alpha_, n_particles_;
+    std::size_t target_walkers_;
+    double feedback_rate_ = 0.0, sigma_ = 0.0;
+    double e_cut_ = kInf, e_max_ = kInf;
+    double etrial_ = 0.0, eref_ = 0.0, eref_sum_ = 0.0, eref_wsum_ = 0.0;
+};
+
+// robust_spread (statistics.cpp:297-315): 1.4826 x the median absolute deviation
+inline double robust_spread(std::vector<double> x) {
+    if (x.empty()) return 0.0;
+    auto median = [](std::vector<double>& v) {
+        const std::size_t n = v.size(), mid = n / 2;
+        std::nth_element(v.begin(), v.begin() + static_cast<std::ptrdiff_t>(mid), v.end());
+        const double upper = v[mid];
+        if (n % 2 == 1) return upper;
+        const double lower = *std::max_element(v.begin(), v.begin() + static_cast<std::ptrdiff_t>(mid));
+        return 0.5 * (lower + upper);
+    };
+    std::vector<double> s = x;
+    const double centre = median(s);
+    for (std::size_t i = 0; i < x.size(); ++i) s[i] = std::abs(x[i] - centre);
+    return 1.4826 * median(s);
+}
+
+// ── DMC step ───────────────────────────────────────────────────────────────
+// DMC::run_one_step (dmc.cpp:693-739) with one crowd over advance_measure
+// (dmc.cpp:339-419, bare reweighting): n_substeps drift-diffusion sweeps each
+// followed by E_L and the branch factor, the step's (sum w E_L, sum w), the
+// integer branch from stream_seed(seed, "dmc-brch", 0) and the E_T update.
+class DMC {
+public:
+    DMC(WalkerSet& ws, Hamiltonian& ham, double tau, std::size_t target, int n_substeps = 1,
+        double generations = 10.0, BranchCutoff cutoff = BranchCutoff::universal, double alpha = 2.0,
+        std::uint64_t seed = 42)
+        : ws_(ws), tau_(tau), target_(target), n_substeps_(n_substeps),
+          pc_(tau, n_substeps, generations, target, cutoff, alpha, ws.n_particles()),
+          branch_rng_(XoshiroState::stream(seed, 0x646d632d62726368ULL, 0)) {
+        const std::vector<double> e = ham.evaluate_batch(ws);
+        double sum = 0.0;
+        for (double x : e) sum += x;
+        pc_.start(sum / static_cast<double>(e.size()));   // dmc.cpp:628-635
+        pc_.set_energy_spread(robust_spread(e));          // dmc.cpp:640-644
+    }
+    // tapes: one [np][dim+1][stride] block per substep, or nullptr for the
+    // device streams.  Returns (sum w E_L, sum w) of the step.
+    std::pair<double, double> run_one_step(bool measure, const double* const* tapes = nullptr,
+                                           std::size_t stride = 0) {
+        alea_system* h = ws_.handle();
+        check(alea_dmc_begin(h));
+        long long acc = 0;
+        for (int s = 0; s < n_substeps_; ++s) {
+            const double* tape = tapes ? tapes[s] : nullptr;
+            check(alea_dmc_sweeps(h, 1, tau_, tape, tape ? static_cast<int>(stride) : 0, nullptr, nullptr, &acc));
+            check(alea_dmc_weights(h, pc_.etrial(), pc_.eref(), pc_.energy_cutoff(), pc_.e_max(), pc_.tau()));
+        }
+        double e = 0.0, w = 0.0;
+        long long n_cut = 0;
+        check(alea_dmc_sums(h, &e, &w, &n_cut));
+        const double ens_e = w > 0.0 ? e / w : pc_.etrial();
+        ws_.branch(branch_rng_, target_, src_);
+        pc_.update(ens_e, w, measure);
+        return {e, w};
+    }
+    const PopulationControl& population() const { return pc_; }
+    const std::vector<std::size_t>& last_branch() const { return src_; }
+
+private:
+    WalkerSet& ws_;
+    double tau_;
+    std::size_t target_;
+    int n_substeps_;
+    PopulationControl pc_;
+    XoshiroState branch_rng_;
+    std::vector<std::size_t> src_;
+};
+
+}  // namespace alea_amd
+
+#endif
