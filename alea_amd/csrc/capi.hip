@@ -28,7 +28,7 @@ void launch_evaluate(const DevModel* m, const DevState& s, int nw, bool mixed, b
 void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, int nb_max, bool mixed,
                     bool cplx, size_t lds, hipStream_t st);
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
-                  bool cplx, size_t lds, hipStream_t st);
+                  bool cplx, int rows_max, size_t lds, hipStream_t st);
 void launch_call(const DevModel* m, const DevState& s, const DevCall& c, int p, int mode, int nw,
                  bool mixed, bool cplx, size_t lds, hipStream_t st);
 void launch_measure(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, size_t lds,
@@ -191,6 +191,7 @@ struct alea_system {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;   // brackets the last sweep launch
     bool timed = false;
     size_t walker_lds = 0, refresh_lds = 0;
+    int rows_max = 0;             // largest padded block row count (0: no determinant)
     int refresh_every = 2;        // sweeps between batched inverse refreshes
     long long since_refresh = 0;  // sweeps (or np accept calls) since the last refresh
 
@@ -205,7 +206,7 @@ struct alea_system {
         // block membership
         for (int i = 0; i < np; ++i) m.block_of[i] = -1;
         int nb_max = 0, inv_elems = 0;
-        int rows_max = 1;
+        rows_max = 0;
         for (int b = 0; b < m.n_blocks; ++b) {
             for (int i = m.blk_first[b]; i < m.blk_first[b] + m.blk_n[b]; ++i) m.block_of[i] = b;
             nb_max = std::max(nb_max, m.blk_n[b]);
@@ -220,6 +221,18 @@ struct alea_system {
             const int a = m.pair[f].ga, b = m.pair[f].gb;
             m.pair_list[a][b][m.pair_list_n[a][b]++] = f;
             if (a != b) m.pair_list[b][a][m.pair_list_n[b][a]++] = f;
+        }
+        // the groups partition the particles (ParticleGroup ranges,
+        // particle_group.h:21-27): the kernels walk pair partners group by group
+        {
+            std::vector<int> owner(np, -1);
+            for (int g = 0; g < m.ng; ++g)
+                for (int i = m.g_first[g]; i < m.g_last[g]; ++i) {
+                    if (owner[i] >= 0) throw InvalidArg("particle groups overlap");
+                    owner[i] = g;
+                }
+            for (int i = 0; i < np; ++i)
+                if (owner[i] < 0) throw InvalidArg("particle " + std::to_string(i) + " is in no group");
         }
         for (int g = 0; g < m.ng; ++g)
             for (int i = m.g_first[g]; i < m.g_last[g]; ++i) m.group_of[i] = g;
@@ -266,12 +279,11 @@ struct alea_system {
         st.rr = rr.p;
         st.status = status.p;
         st.n_accepted = nacc.p;
-        int nbk = 1;
-        for (int b = 0; b < m.n_blocks; ++b) nbk = std::max(nbk, m.blk_n[b]);
-        // LDS of one walker: positions, k-space densities, v', column j, and
-        // the Sherman-Morrison operands (carve() in kernels.hip)
-        walker_lds = sizeof(double) * (2 * static_cast<size_t>(np) + 2 * m.ng * m.nkj + 4 * nbk +
-                                       4 * static_cast<size_t>(rows_max));
+        // LDS of one walker: positions, k-space densities, v', column j, the
+        // Sherman-Morrison operands and the resident block's row sums, the
+        // last four over the padded rows (carve() in kernels.hip)
+        walker_lds = sizeof(double) * (2 * static_cast<size_t>(np) + 2 * m.ng * m.nkj +
+                                       9 * static_cast<size_t>(std::max(rows_max, kRowPad)));
         {
             const size_t e = cplx ? 2 : 1, n = static_cast<size_t>(nb_max);
             refresh_lds = 8 * (e * n * n + e * n + 8) + 4 * (8 + n + (n & 1)) + 8 * n;
@@ -1020,7 +1032,7 @@ static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* t
         Pi.n_sweeps = 1;
         if (P.tape) Pi.tape = P.tape + i * tape_step;
         hip_check(hipEventRecord(s->ev0, s->stream), "event record");
-        launch_sweep(s->dmodel, s->st, Pi, s->nw, s->mixed(), s->cplx, s->walker_lds, s->stream);
+        launch_sweep(s->dmodel, s->st, Pi, s->nw, s->mixed(), s->cplx, s->rows_max, s->walker_lds, s->stream);
         hip_check(hipGetLastError(), "launch sweep");
         hip_check(hipEventRecord(s->ev1, s->stream), "event record");
         if (++s->since_refresh >= s->refresh_every) s->refresh();

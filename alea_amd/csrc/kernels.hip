@@ -53,15 +53,47 @@ namespace {
 
 constexpr int W = 64;   // wavefront
 
-__device__ inline double wsum(double v) {
+// Wavefront reductions (called with all 64 lanes active).  Within each row
+// of 16 lanes, DPP moves (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror,
+// row_mirror) combine partners in registers, leaving every lane of the row
+// with the row's total; the four row totals are then read from lanes 0, 16,
+// 32, 48 into scalar registers and combined in a fixed order, so every lane
+// gets the same bits.
+template <int CTRL>
+__device__ inline double dpp_mov(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double lane_value(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+template <int N>
+__device__ inline void wsum_n(double (&v)[N]) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
-    return v;
+    for (int i = 0; i < N; ++i) v[i] += dpp_mov<0xB1>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += dpp_mov<0x4E>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += dpp_mov<0x141>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += dpp_mov<0x140>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        v[i] = (lane_value(v[i], 0) + lane_value(v[i], 16)) + (lane_value(v[i], 32) + lane_value(v[i], 48));
+}
+__device__ inline double wsum(double v) {
+    double a[1] = {v};
+    wsum_n(a);
+    return a[0];
 }
 __device__ inline double wmax(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, W));
-    return v;
+    v = fmax(v, dpp_mov<0xB1>(v));
+    v = fmax(v, dpp_mov<0x4E>(v));
+    v = fmax(v, dpp_mov<0x141>(v));
+    v = fmax(v, dpp_mov<0x140>(v));
+    return fmax(fmax(lane_value(v, 0), lane_value(v, 16)), fmax(lane_value(v, 32), lane_value(v, 48)));
 }
 
 // ── cell ────────────────────────────────────────────────────────────────────
@@ -394,29 +426,25 @@ __device__ inline void store_elem_impl(T* row, int c, C2<T> v) {
 struct Shared {
     double* pos;   // [np*2]
     double* rho;   // [ng*nkj*2]
-    double* v;     // [2*nb] proposed orbital values (stored in the determinant precision T)
-    double* u;     // [2*nb] column j of A (old)
-    double* f;     // [4*nb]: u / R, then v again, both as C2<T> (determinant precision)
+    double* v;     // [2*rows] proposed orbital values (stored in the determinant precision T)
+    double* u;     // [2*rows] column j of A (old)
+    double* f;     // [4*rows]: u / R, then v again, both as C2<T> (determinant precision)
+    double* rabs;  // [rows] row sums |D| of the block held in registers (register path)
 };
 
 __device__ inline Shared carve(const DevModel& M, double* base) {
-    int nb = 1;
-    for (int b = 0; b < M.n_blocks; ++b) nb = M.blk_n[b] > nb ? M.blk_n[b] : nb;
+    int nr = kRowPad;
+    for (int b = 0; b < M.n_blocks; ++b) nr = M.blk_rows[b] > nr ? M.blk_rows[b] : nr;
     Shared sh;
     sh.pos = base;
     sh.rho = sh.pos + 2 * M.np;
     sh.v = sh.rho + 2 * M.ng * M.nkj;
-    sh.u = sh.v + 2 * nb;
-    sh.f = sh.u + 2 * nb;
+    sh.u = sh.v + 2 * nr;
+    sh.f = sh.u + 2 * nr;
+    sh.rabs = sh.f + 4 * nr;
     return sh;
 }
 
-// Sum reduction of n lane values at once (n <= 16) through LDS-free shuffles.
-template <int N>
-__device__ inline void wsum_n(double (&v)[N]) {
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = wsum(v[i]);
-}
 
 // Condition estimate of a freshly computed inverse, as stored: above 1e8 it
 // is the block's own conditioning, which another rebuild cannot lower, and it
@@ -426,8 +454,15 @@ __device__ inline void wsum_n(double (&v)[N]) {
 __device__ inline double fresh_cond(double kappa) { return kappa > 1e8 ? -kappa : kappa; }
 
 // ── the per-walker engine ──────────────────────────────────────────────────
-template <typename T, bool CPLX>
+// NR = 0: the determinant inverses stay in HBM and every access goes there.
+// NR > 0 (a multiple of kRowPad, >= every block's padded row count): the
+// block being swept is held in registers, lane c holding column c (NR
+// entries), from its first move in a sweep until a move of another block or
+// the end of the kernel; HBM then sees one load and one store of each block
+// per sweep instead of a read-modify-write of the block per accepted move.
+template <typename T, bool CPLX, int NR = 0>
 struct Walker {
+    static_assert(NR % kRowPad == 0, "register tile is a whole number of row chunks");
     const DevModel& M;
     const DevState& S;
     Shared sh;
@@ -435,6 +470,60 @@ struct Walker {
     double log_j;            // Jastrow + one-body part of log|psi|
     double log_b0 = 0.0, log_b1 = 0.0;   // determinant blocks (kMaxBlocks = 2)
     __device__ double& log_b(int b) { return b == 0 ? log_b0 : log_b1; }
+    C2<T> col[NR > 0 ? NR : 1];   // column `lane` of the resident block (NR > 0)
+    int res_b = -1;               // resident block, -1 for none
+    // the resident block's condition estimate and update count (its row
+    // sums are in sh.rabs); HBM holds them again after spill()
+    double res_cond = 0.0;
+    int res_upd = 0;
+
+    // Load column `lane` of block b into registers (rows past the block's
+    // padded count, and lanes past its orbital count, hold zeros).
+    __device__ void fill(int b) {
+        const InvRef<T, CPLX> A = inv(b);
+        const int n = M.blk_n[b], nch = M.blk_rows[b] / kRowPad;
+#pragma unroll
+        for (int ch = 0; ch < NR / kRowPad; ++ch) {
+#pragma unroll
+            for (int i = 0; i < kRowPad; ++i) col[ch * kRowPad + i] = C2<T>{T(0), T(0)};
+            if (ch < nch && lane < n) {
+#pragma unroll
+                for (int i = 0; i < kRowPad; ++i) col[ch * kRowPad + i] = A.get(ch * kRowPad + i, lane);
+            }
+        }
+        const double* ra = S.row_abs + (static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max;
+        for (int kr = lane; kr < n; kr += W) sh.rabs[kr] = ra[kr];
+        res_cond = S.cond[w * kMaxBlocks + b];
+        res_upd = S.updates[w * kMaxBlocks + b];
+        res_b = b;
+        __syncthreads();
+    }
+    // Store the resident block back to HBM.
+    __device__ void spill() {
+        if (NR == 0 || res_b < 0) return;
+        const InvRef<T, CPLX> A = inv(res_b);
+        const int n = M.blk_n[res_b], nch = M.blk_rows[res_b] / kRowPad;
+        if (lane < n) {
+#pragma unroll
+            for (int ch = 0; ch < NR / kRowPad; ++ch)
+                if (ch < nch) {
+#pragma unroll
+                    for (int i = 0; i < kRowPad; ++i) A.set(ch * kRowPad + i, lane, col[ch * kRowPad + i]);
+                }
+        }
+        double* ra = S.row_abs + (static_cast<size_t>(w) * kMaxBlocks + res_b) * S.nb_max;
+        for (int kr = lane; kr < n; kr += W) ra[kr] = sh.rabs[kr];
+        if (lane == 0) {
+            S.cond[w * kMaxBlocks + res_b] = res_cond;
+            S.updates[w * kMaxBlocks + res_b] = res_upd;
+        }
+        res_b = -1;
+    }
+    __device__ void make_resident(int b) {
+        if (res_b == b) return;
+        spill();
+        fill(b);
+    }
 
     __device__ Walker(const DevModel& m, const DevState& s, const Shared& shm, int walker)
         : M(m), S(s), sh(shm), w(walker), lane(threadIdx.x) {}
@@ -480,22 +569,26 @@ struct Walker {
         double dlog;
         double og[2], ng[2], nl[2];
     };
+    // Lane partial sums of the pair and k-space parts (reduced by the caller).
     template <bool derivs>
-    __device__ void jastrow_move(int p, double x0, double x1, double nx0, double nx1, JOut& out) {
+    __device__ void jastrow_lanes(int p, double x0, double x1, double nx0, double nx1, double (&acc)[7]) {
         const int gp = M.group_of[p];
-        double acc[7] = {0, 0, 0, 0, 0, 0, 0};   // dlog, og0, og1, ng0, ng1, nl0, nl1
-        for (int k = lane; k < M.np; k += W) {
+        for (int i = 0; i < 7; ++i) acc[i] = 0.0;   // dlog, og0, og1, ng0, ng1, nl0, nl1
+        // partners group by group (the groups partition the particles), so
+        // the pair factors, and the loads of their parameters, are uniform
+        // across the wavefront
+        for (int h = 0; h < M.ng; ++h) {
+          const int nf = M.pair_list_n[gp][h];
+          if (nf == 0) continue;
+          for (int k = M.g_first[h] + lane; k < M.g_last[h]; k += W) {
             if (k == p) continue;
-            const int gk = M.group_of[k];
-            const int nf = M.pair_list_n[gp][gk];
-            if (nf == 0) continue;
             const double xk0 = sh.pos[2 * k], xk1 = sh.pos[2 * k + 1];
             double o0 = x0 - xk0, o1 = x1 - xk1;
             const double ro = dist(M.cell, o0, o1);
             double n0 = nx0 - xk0, n1 = nx1 - xk1;
             const double rn = dist(M.cell, n0, n1);
             for (int fi = 0; fi < nf; ++fi) {
-                const DevRadial& R = M.pair[M.pair_list[gp][gk][fi]].rad;
+                const DevRadial& R = M.pair[M.pair_list[gp][h][fi]].rad;
                 double uo, duo, d2uo, un, dun, d2un;
                 radial_eval(R, ro, uo, duo, d2uo);
                 radial_eval(R, rn, un, dun, d2un);
@@ -512,6 +605,7 @@ struct Walker {
                     acc[6] += l1;
                 }
             }
+          }
         }
         // k-space: lanes over k
         if (M.nkj > 0 && M.kj_active[gp]) {
@@ -557,9 +651,9 @@ struct Walker {
                 }
             }
         }
-        if constexpr (derivs) wsum_n(acc);
-        else acc[0] = wsum(acc[0]);
-        // Nosanow one-body (uniform across lanes)
+    }
+    // The one-body part (uniform across lanes), added to the reduced sums.
+    __device__ void jastrow_onebody(int p, double x0, double x1, double nx0, double nx1, double (&acc)[7]) {
         if (M.has_nosanow && M.nos_alpha[p] != 0.0) {
             const double a = M.nos_alpha[p];
             double o0 = x0 - M.nos_site[p][0], o1 = x1 - M.nos_site[p][1];
@@ -573,10 +667,21 @@ struct Walker {
             acc[5] += -2.0 * a;
             acc[6] += -2.0 * a;
         }
+    }
+    static __device__ void jastrow_out(const double (&acc)[7], JOut& out) {
         out.dlog = acc[0];
         out.og[0] = acc[1]; out.og[1] = acc[2];
         out.ng[0] = acc[3]; out.ng[1] = acc[4];
         out.nl[0] = acc[5]; out.nl[1] = acc[6];
+    }
+    template <bool derivs>
+    __device__ void jastrow_move(int p, double x0, double x1, double nx0, double nx1, JOut& out) {
+        double acc[7];
+        jastrow_lanes<derivs>(p, x0, x1, nx0, nx1, acc);
+        if constexpr (derivs) wsum_n(acc);
+        else acc[0] = wsum(acc[0]);
+        jastrow_onebody(p, x0, x1, nx0, nx1, acc);
+        jastrow_out(acc, out);
     }
 
     // Commit the k-space densities of an accepted move of p.
@@ -603,14 +708,35 @@ struct Walker {
         double nl[2][2];
         double rowabs;     // sum_m |v_m|
     };
+    // Lane partial sums of the ratio of moving p (column j of block b), the
+    // proposed orbital values v' (sh.v) and column j (sh.u) in LDS; reduced
+    // by the caller, and sh.v / sh.u are read only after a later sync.
     template <bool derivs, bool old_derivs>
-    __device__ void det_ratio(int b, int j, double x0, double x1, double nx0, double nx1, DOut& o) {
+    __device__ void det_lanes(int b, int j, double x0, double x1, double nx0, double nx1, double (&acc)[20]) {
         const InvRef<T, CPLX> A = inv(b);
         const int n = M.blk_n[b];
-        double acc[20];
+        if constexpr (NR > 0) {
+            // column j lives in lane j: stage it through LDS for the lanes over m
+            make_resident(b);
+            const int nch = M.blk_rows[b] / kRowPad;
+            if (lane == j) {
+#pragma unroll
+                for (int ch = 0; ch < NR / kRowPad; ++ch)
+                    if (ch < nch) {
+#pragma unroll
+                        for (int i = 0; i < kRowPad; ++i) {
+                            sh.u[2 * (ch * kRowPad + i)] = static_cast<double>(col[ch * kRowPad + i].re);
+                            sh.u[2 * (ch * kRowPad + i) + 1] = static_cast<double>(col[ch * kRowPad + i].im);
+                        }
+                    }
+            }
+            __syncthreads();
+        }
         for (int i = 0; i < 20; ++i) acc[i] = 0.0;
         for (int m = lane; m < n; m += W) {
-            const C2<T> a = A.get(m, j);
+            C2<T> a;
+            if constexpr (NR > 0) a = C2<T>{static_cast<T>(sh.u[2 * m]), static_cast<T>(sh.u[2 * m + 1])};
+            else a = A.get(m, j);
             const double ar = static_cast<double>(a.re), ai = static_cast<double>(a.im);
             Orb on;
             orbital(M, b, m, nx0, nx1, on);
@@ -618,8 +744,10 @@ struct Walker {
             const T vr = static_cast<T>(on.vr), vi = static_cast<T>(on.vi);
             sh.v[2 * m] = static_cast<double>(vr);
             sh.v[2 * m + 1] = static_cast<double>(vi);
-            sh.u[2 * m] = ar;
-            sh.u[2 * m + 1] = ai;
+            if constexpr (NR == 0) {
+                sh.u[2 * m] = ar;
+                sh.u[2 * m + 1] = ai;
+            }
             const double tr = ar * static_cast<double>(vr) - ai * static_cast<double>(vi);
             const double ti = ar * static_cast<double>(vi) + ai * static_cast<double>(vr);
             acc[0] += tr;
@@ -645,14 +773,8 @@ struct Walker {
                 }
             }
         }
-        if constexpr (derivs || old_derivs) {
-            wsum_n(acc);
-        } else {
-            acc[0] = wsum(acc[0]);
-            acc[1] = wsum(acc[1]);
-            acc[2] = wsum(acc[2]);
-            acc[19] = wsum(acc[19]);
-        }
+    }
+    static __device__ void det_out(const double (&acc)[20], DOut& o) {
         o.R[0] = acc[0];
         o.R[1] = acc[1];
         o.scale = acc[2];
@@ -663,6 +785,19 @@ struct Walker {
             o.og[d][0] = acc[11 + 2 * d]; o.og[d][1] = acc[12 + 2 * d];
             o.ol[d][0] = acc[15 + 2 * d]; o.ol[d][1] = acc[16 + 2 * d];
         }
+    }
+    template <bool derivs, bool old_derivs>
+    __device__ void det_ratio(int b, int j, double x0, double x1, double nx0, double nx1, DOut& o) {
+        double acc[20];
+        det_lanes<derivs, old_derivs>(b, j, x0, x1, nx0, nx1, acc);
+        if constexpr (derivs || old_derivs) {
+            wsum_n(acc);
+        } else {
+            double r[4] = {acc[0], acc[1], acc[2], acc[19]};
+            wsum_n(r);
+            acc[0] = r[0]; acc[1] = r[1]; acc[2] = r[2]; acc[19] = r[3];
+        }
+        det_out(acc, o);
         __syncthreads();
     }
 
@@ -822,9 +957,9 @@ struct Walker {
         const double absR = CPLX ? sqrt(Rr * Rr + Ri * Ri) : fabs(Rr);
         const double eps = sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
         int* upd = S.updates + w * kMaxBlocks + b;
-        const int updates = *upd + 1;
-        double* rabs = S.row_abs + (static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max;
-        const double cond_stored = S.cond[w * kMaxBlocks + b];
+        const int updates = (NR > 0 ? res_upd : *upd) + 1;
+        double* rabs = NR > 0 ? sh.rabs : S.row_abs + (static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max;
+        const double cond_stored = NR > 0 ? res_cond : S.cond[w * kMaxBlocks + b];
         const bool latched = cond_stored < 0.0;    // see fresh_cond
         const double cond_old = fabs(cond_stored);
         __syncthreads();
@@ -865,7 +1000,36 @@ struct Walker {
             constexpr int CH = kRowPad;   // rows per chunk; rows padded with zeros to a multiple
             const int nch = M.blk_rows[b] / CH;
             double colmax = 0.0;
-            for (int c = lane; c < n; c += W) {
+            if constexpr (NR > 0) {
+                // the same two passes on the column held in registers; lanes
+                // past n hold a zero column and stay zero
+                C2<T> r{T(0), T(0)};
+#pragma unroll
+                for (int ch = 0; ch < NR / CH; ++ch)
+                    if (ch < nch) {
+#pragma unroll
+                        for (int i = 0; i < CH; ++i) {
+                            const C2<T> pr = cmul(vT[ch * CH + i], col[ch * CH + i]);
+                            r.re += pr.re;
+                            r.im += pr.im;
+                        }
+                    }
+                if (lane == j) r.re -= T(1);
+                T cs = T(0);
+#pragma unroll
+                for (int ch = NR / CH - 1; ch >= 0; --ch)
+                    if (ch < nch) {
+#pragma unroll
+                        for (int i = 0; i < CH; ++i) {
+                            const C2<T> pr = cmul(fT[ch * CH + i], r);
+                            col[ch * CH + i].re -= pr.re;
+                            col[ch * CH + i].im -= pr.im;
+                            cs += fabs(col[ch * CH + i].re) + (CPLX ? fabs(col[ch * CH + i].im) : T(0));
+                        }
+                    }
+                colmax = static_cast<double>(cs);
+            }
+            for (int c = lane; NR == 0 && c < n; c += W) {
                 C2<T> a[CH];
                 C2<T> r{T(0), T(0)};
                 for (int ch = 0; ch < nch; ++ch) {
@@ -912,13 +1076,19 @@ struct Walker {
                 rebuild_now = true;
             } else {
                 log_b(b) += log(absR);
-                if (lane == 0) {
+                if constexpr (NR > 0) {
+                    res_upd = updates;
+                    res_cond = latched ? -cond : cond;
+                } else if (lane == 0) {
                     *upd = updates;
                     S.cond[w * kMaxBlocks + b] = latched ? -cond : cond;
                 }
             }
         }
-        if (rebuild_now) log_b(b) = rebuild(b);
+        if (rebuild_now) {
+            log_b(b) = rebuild(b);
+            if constexpr (NR > 0) fill(b);   // the rebuild wrote the block to HBM
+        }
         __syncthreads();
         return rebuild_now;
     }
@@ -933,7 +1103,15 @@ struct Walker {
         sincos_bounded(M.orb_k[b][0][0] * shift0 + M.orb_k[b][0][1] * shift1, &s, &c);
         const InvRef<T, CPLX> A = inv(b);
         const C2<T> ph{static_cast<T>(c), static_cast<T>(s)};
-        for (int m = lane; m < M.blk_n[b]; m += W) A.set(m, j, cmul(A.get(m, j), ph));
+        if constexpr (NR > 0) {
+            // resident since this move's ratio; column j is lane j's registers
+            if (lane == j) {
+#pragma unroll
+                for (int m = 0; m < NR; ++m) col[m] = cmul(col[m], ph);
+            }
+        } else {
+            for (int m = lane; m < M.blk_n[b]; m += W) A.set(m, j, cmul(A.get(m, j), ph));
+        }
         __syncthreads();
     }
 
@@ -945,16 +1123,39 @@ struct Walker {
                          double tau, bool dmc, double drift_scale, const double* gold,
                          double& rr_acc) {
         const double x0 = sh.pos[2 * p], x1 = sh.pos[2 * p + 1];
+        const int b = M.block_of[p];
+        // Jastrow and determinant lane sums, then one reduction for both
+        double ja[7], da[20];
+        jastrow_lanes<derivs>(p, x0, x1, nx0, nx1, ja);
+        if (b >= 0) {
+            det_lanes<derivs, false>(b, p - M.blk_first[b], x0, x1, nx0, nx1, da);
+        } else {
+            for (int i = 0; i < 20; ++i) da[i] = 0.0;
+        }
+        if constexpr (derivs) {
+            double r[19];
+            for (int i = 0; i < 7; ++i) r[i] = ja[i];
+            for (int i = 0; i < 11; ++i) r[7 + i] = da[i];
+            r[18] = da[19];
+            wsum_n(r);
+            for (int i = 0; i < 7; ++i) ja[i] = r[i];
+            for (int i = 0; i < 11; ++i) da[i] = r[7 + i];
+            da[19] = r[18];
+        } else {
+            double r[5] = {ja[0], da[0], da[1], da[2], da[19]};
+            wsum_n(r);
+            ja[0] = r[0]; da[0] = r[1]; da[1] = r[2]; da[2] = r[3]; da[19] = r[4];
+        }
+        jastrow_onebody(p, x0, x1, nx0, nx1, ja);
         JOut J;
-        jastrow_move<derivs>(p, x0, x1, nx0, nx1, J);
+        jastrow_out(ja, J);
         double lr = J.dlog, phase = 0.0;
         double ng[2] = {J.ng[0], J.ng[1]}, nl[2] = {J.nl[0], J.nl[1]};
         double npg[2] = {0, 0}, npl[2] = {0, 0};
-        const int b = M.block_of[p];
         DOut D{};
         if (b >= 0) {
-            const int j = p - M.blk_first[b];
-            det_ratio<derivs, false>(b, j, x0, x1, nx0, nx1, D);
+            det_out(da, D);
+            __syncthreads();
             const double absR = sqrt(D.R[0] * D.R[0] + D.R[1] * D.R[1]);
             lr += log(fmax(absR, 1e-300));
             if (CPLX) {
@@ -1530,15 +1731,16 @@ __global__ __launch_bounds__(GX * 16) void k_refresh_reg(const DevModel* __restr
     }
 }
 
-// Fused sweeps: every particle of the walker, n_sweeps times.
-template <typename T, bool CPLX, bool DMC>
+// Fused sweeps: every particle of the walker, n_sweeps times.  NR > 0 holds
+// the block being swept in registers (see Walker).
+template <typename T, bool CPLX, bool DMC, int NR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_sweep(const DevModel* __restrict__ Mp, DevState S,
                                               SweepParams P) {
     const DevModel& M = *Mp;
     extern __shared__ double lds_[];
     Shared sh = carve(M, lds_);
     const int w = blockIdx.x, lane = threadIdx.x;
-    Walker<T, CPLX> wk(M, S, sh, w);
+    Walker<T, CPLX, NR> wk(M, S, sh, w);
     wk.load();
     Rng rng;
     if (!P.tape)
@@ -1579,6 +1781,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             if (accepted) ++nacc;
         }
     }
+    wk.spill();
     wk.store();
     if (lane == 0) {
         S.n_accepted[w] += nacc;
@@ -1893,24 +2096,59 @@ void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, 
     else
         DISPATCH(k_refresh, dim3(nw * n_blocks), dim3(256), lds, st, m, s);
 }
+// The register tile of the sweep: the smallest NR in {16, 32, 64} that
+// covers every block's padded rows, within 128 32-bit registers per lane for
+// the column; larger blocks (complex FP64 past 32 rows, anything past 64)
+// take the HBM path (NR = 0).
+template <typename T, bool CPLX>
+constexpr int kColWords = (CPLX ? 2 : 1) * static_cast<int>(sizeof(T) / 4);
+
+int sweep_tile_rows(bool mixed, bool cplx, int rows_max) {
+    if (rows_max <= 0) return 0;
+    const int words = (cplx ? 2 : 1) * (mixed ? 1 : 2);
+    for (int nr = 16; nr <= 64; nr *= 2)
+        if (rows_max <= nr) return nr * words <= 128 ? nr : 0;
+    return 0;
+}
+
+template <typename T, bool CPLX, bool DMC>
+static void launch_sweep_t(const DevModel* m, const DevState& s, const SweepParams& p, int nw, int nr,
+                           size_t lds, hipStream_t st) {
+    const dim3 g(nw), t(64);
+    switch (nr) {
+        case 16: hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 16>), g, t, lds, st, m, s, p); break;
+        case 32: hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 32>), g, t, lds, st, m, s, p); break;
+        case 64:
+            // the drift-diffusion move's extra live state leaves no room for
+            // a 128-register column: DMC sweeps of such blocks stay in HBM
+            if constexpr (64 * kColWords<T, CPLX> <= 128 && (!DMC || 64 * kColWords<T, CPLX> <= 64)) {
+                hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 64>), g, t, lds, st, m, s, p);
+                break;
+            }
+            [[fallthrough]];
+        default: hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 0>), g, t, lds, st, m, s, p); break;
+    }
+}
+
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
-                  bool cplx, size_t lds, hipStream_t st) {
+                  bool cplx, int rows_max, size_t lds, hipStream_t st) {
+    const int nr = sweep_tile_rows(mixed, cplx, rows_max);
     if (p.dmc) {
         if (mixed) {
-            if (cplx) hipLaunchKernelGGL((k_sweep<float, true, true>), dim3(nw), dim3(64), lds, st, m, s, p);
-            else hipLaunchKernelGGL((k_sweep<float, false, true>), dim3(nw), dim3(64), lds, st, m, s, p);
+            if (cplx) launch_sweep_t<float, true, true>(m, s, p, nw, nr, lds, st);
+            else launch_sweep_t<float, false, true>(m, s, p, nw, nr, lds, st);
         } else {
-            if (cplx) hipLaunchKernelGGL((k_sweep<double, true, true>), dim3(nw), dim3(64), lds, st, m, s, p);
-            else hipLaunchKernelGGL((k_sweep<double, false, true>), dim3(nw), dim3(64), lds, st, m, s, p);
+            if (cplx) launch_sweep_t<double, true, true>(m, s, p, nw, nr, lds, st);
+            else launch_sweep_t<double, false, true>(m, s, p, nw, nr, lds, st);
         }
         return;
     }
     if (mixed) {
-        if (cplx) hipLaunchKernelGGL((k_sweep<float, true, false>), dim3(nw), dim3(64), lds, st, m, s, p);
-        else hipLaunchKernelGGL((k_sweep<float, false, false>), dim3(nw), dim3(64), lds, st, m, s, p);
+        if (cplx) launch_sweep_t<float, true, false>(m, s, p, nw, nr, lds, st);
+        else launch_sweep_t<float, false, false>(m, s, p, nw, nr, lds, st);
     } else {
-        if (cplx) hipLaunchKernelGGL((k_sweep<double, true, false>), dim3(nw), dim3(64), lds, st, m, s, p);
-        else hipLaunchKernelGGL((k_sweep<double, false, false>), dim3(nw), dim3(64), lds, st, m, s, p);
+        if (cplx) launch_sweep_t<double, true, false>(m, s, p, nw, nr, lds, st);
+        else launch_sweep_t<double, false, false>(m, s, p, nw, nr, lds, st);
     }
 }
 void launch_call(const DevModel* m, const DevState& s, const DevCall& c, int p, int mode, int nw,

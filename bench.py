@@ -111,6 +111,8 @@ def cpu_baseline(tau, walkers_per_core=24):
     once.  Per-walker CPU cost does not depend on the walker count."""
     cores = max(1, min(16, os.cpu_count() or 1))
     jobs = [(walkers_per_core, tau, 100 + i) for i in range(cores)]
+    print(f"bench: CPU baseline, {cores} processes x {walkers_per_core} walkers on the oracle", file=sys.stderr,
+          flush=True)
     t0 = time.perf_counter()
     with mp.get_context("spawn").Pool(cores) as pool:
         res = pool.map(_cpu_worker, jobs)
