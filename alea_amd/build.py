@@ -10,7 +10,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared",
-    # packed FP32 is disabled for the device code ("Build")
+    # no packed FP32 (v_pk_*) in the device code: those forms have given
+    # wrong results on this hardware without any error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]

@@ -1513,6 +1513,18 @@ __global__ __launch_bounds__(256) void k_refresh(const DevModel* __restrict__ Mp
 // FP64 throughout; A is stored in the determinant precision with log|det D|
 // (floored at 1e-300), the row sums of |D|, the condition estimate and a zero
 // update count, like k_refresh.
+// One DPP step of the pivot search: the larger value wins, ties to the
+// first row.
+template <int CTRL>
+__device__ inline void pivot_combine(double& best, int& bi) {
+    const double ob = dpp_mov<CTRL>(best);
+    const int oi = __builtin_amdgcn_update_dpp(bi, bi, CTRL, 0xF, 0xF, false);
+    if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+    }
+}
+
 template <typename T, bool CPLX, int NP, int GX>
 __global__ __launch_bounds__(GX * 16) void k_refresh_reg(const DevModel* __restrict__ Mp, DevState S) {
     constexpr int GY = 16;                 // owners of a column: 16 adjacent lanes
@@ -1570,22 +1582,21 @@ __global__ __launch_bounds__(GX * 16) void k_refresh_reg(const DevModel* __restr
 #pragma unroll
                 for (int j = 0; j < TC; ++j)
                     if (j == kj) {
+                        // |z|^2 orders the candidates as |z| does, without a sqrt
                         const double re = a[i][j][0], im = CPLX ? a[i][j][E - 1] : 0.0;
-                        const double v = CPLX ? sqrt(re * re + im * im) : fabs(re);
+                        const double v = CPLX ? re * re + im * im : fabs(re);
                         if (v > best) {
                             best = v;
                             bi = r;
                         }
                     }
             }
-            for (int o = GY / 2; o > 0; o >>= 1) {
-                const double ob = __shfl_xor(best, o, GY);
-                const int oi = __shfl_xor(bi, o, GY);
-                if (ob > best || (ob == best && oi < bi)) {
-                    best = ob;
-                    bi = oi;
-                }
-            }
+            // the GY = 16 owners of column k are one DPP row: combine
+            // (value, first row) pairs in registers
+            pivot_combine<0xB1>(best, bi);
+            pivot_combine<0x4E>(best, bi);
+            pivot_combine<0x141>(best, bi);
+            pivot_combine<0x140>(best, bi);
             if (ty == 0) {
                 piv_sh[buf] = bi;
                 best_sh[buf] = best;
@@ -1598,7 +1609,7 @@ __global__ __launch_bounds__(GX * 16) void k_refresh_reg(const DevModel* __restr
             singular = true;
             break;
         }
-        if (tid == 0) log_det += log(best);   // only thread 0 stores it
+        if (tid == 0) log_det += CPLX ? 0.5 * log(best) : log(best);   // only thread 0 stores it
         // the pivot row goes to LDS with 1 in column k (so every thread
         // normalizes it without a select), column k goes as it is, and its
         // owners then zero it: in place, column k of the other rows becomes

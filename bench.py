@@ -1,7 +1,7 @@
 """Headline benchmark: walker-moves/s of the batched VMC sweep on the N=122
 twisted 2-D electron gas, mixed precision vs plain FP64, on one MI355X.
 
-Workload (the north-star configuration that fits one GPU):
+Workload (the largest of the published configurations that fits one GPU):
   2-D paramagnetic HEG, N = 122 (61 up, 61 down), rs = 30, square cell
   L = sqrt(pi N) rs; complex plane-wave orbitals at a twist drawn uniformly in
   the first Brillouin zone (seed 3); image-summed Padé Jastrow
@@ -104,7 +104,7 @@ def _cpu_worker(args):
     return nw * spec.n_particles, time.perf_counter() - t0
 
 
-def cpu_baseline(tau, walkers_per_core=24):
+def cpu_baseline(tau, walkers_per_core=384):
     """The oracle (the FP64 CPU restatement, which follows the reference's
     full complex re-inversion per accepted move) on the host cores: one
     single-threaded process per core, each sweeping `walkers_per_core` walkers

@@ -512,4 +512,26 @@ double ref_robust_spread(const double* x, int n) {
     return alea::numerics::robust_spread(std::span<const double>(x, static_cast<std::size_t>(n)), scratch);
 }
 
+// numerics::reblock / reblock_chains (n_chains = 0: reblock).  out[7] =
+// mean, naive_error, error, tau_int, error_on_error, block_length, level;
+// iout[3] = plateau_found, selected_level, selected_n_blocks.
+int ref_reblock(const double* x, int n, int n_chains, double* out, int* iout) {
+    return guarded([&] {
+        const std::span<const double> s(x, static_cast<std::size_t>(n));
+        const alea::numerics::ReblockResult r =
+            n_chains > 0 ? alea::numerics::reblock_chains(s, static_cast<std::size_t>(n_chains))
+                         : alea::numerics::reblock(s);
+        out[0] = r.mean;
+        out[1] = r.naive_error;
+        out[2] = r.error;
+        out[3] = r.tau_int;
+        out[4] = r.error_on_error;
+        out[5] = r.block_length;
+        out[6] = r.level;
+        iout[0] = r.plateau_found ? 1 : 0;
+        iout[1] = r.selected_level;
+        iout[2] = static_cast<int>(r.selected_n_blocks);
+    });
+}
+
 }  // extern "C"
