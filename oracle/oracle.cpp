@@ -24,6 +24,14 @@
 //   Eigen stand-in          -> the initial inverse uses the reference's own
 //                               pivoted Gauss-Jordan (slater_determinant.cpp:
 //                               905-993) in place of Eigen's D.inverse().
+//   inverse power           -> oracle/_ref InversePowerTerm (dipole system);
+//                               the Wigner-Seitz tail against a disk-cut
+//                               lattice sum plus its continuum remainder
+//   DMC weights, branch     -> oracle/_ref WalkerSet::branch and
+//                               PopulationControl, bit-exact populations
+//   McMillan Jastrow        -> no counterpart in the reference: closed form
+//                               u = -(b/r)^n, u', u'' and finite differences of
+//                               log psi ("parity pinned by closed form")
 
 #include "oracle.h"
 
