@@ -1525,15 +1525,17 @@ __device__ inline void pivot_combine(double& best, int& bi) {
     }
 }
 
+// Up to 64 orbitals the tile fits 128 registers: four waves per SIMD hide the
+// two barriers of every elimination step better than three.
 template <typename T, bool CPLX, int NP, int GX>
-__global__ __launch_bounds__(GX * 16) void k_refresh_reg(const DevModel* __restrict__ Mp, DevState S) {
+__global__ __launch_bounds__(GX * 16) __attribute__((amdgpu_waves_per_eu(NP <= 64 ? 4 : 1, 8))) void k_refresh_reg(const DevModel* __restrict__ Mp, DevState S) {
     constexpr int GY = 16;                 // owners of a column: 16 adjacent lanes
     constexpr int TR = NP / GY, TC = NP / GX;   // tile: TR rows x TC columns per thread
     constexpr int E = CPLX ? 2 : 1;
     constexpr int NWORDS = (NP + 63) / 64;
     constexpr int NT = GX * GY, NWAVES = NT / W;
     const DevModel& M = *Mp;
-    __shared__ double prow[2][NP * E], fcol[2][NP * E];
+    __shared__ double prow[2][NP * E], fcol[2][NP * E], pval[2][2];
     __shared__ double part[GX][NP];
     __shared__ double best_sh[2], red[2][NWAVES];
     __shared__ int piv_sh[2], perm[NP], pinv[NP];
@@ -1568,118 +1570,131 @@ __global__ __launch_bounds__(GX * 16) void k_refresh_reg(const DevModel* __restr
     unsigned long long used[NWORDS];
 #pragma unroll
     for (int q = 0; q < NWORDS; ++q) used[q] = 0ull;
-    double log_det = 0.0;
+    // log|det| as a running mantissa and binary exponent (one log at the end)
+    double det_m = 1.0;
+    int det_e = 0;
     bool singular = false;
-    for (int k = 0; k < NP; ++k) {
-        const int buf = k & 1, kb = k / TC, kj = k % TC;
-        if (tx == kb) {
-            double best = -1.0;
-            int bi = NP;
+    // Step k eliminates column k = kb*TC + kj; the TC steps of a column
+    // block are unrolled so that column kj of the tile is a static register.
+    // Only the n real columns are eliminated: the padding rows and columns
+    // are identity and stay so.
+    const int n_kb = (n + TC - 1) / TC;
+    for (int kb = 0; kb < n_kb && !singular; ++kb) {
+#pragma unroll
+        for (int kj = 0; kj < TC; ++kj) {
+            const int k = kb * TC + kj;
+            if (k >= n) break;
+            const int buf = k & 1;
+            if (tx == kb) {
+                double best = -1.0;
+                int bi = NP;
+#pragma unroll
+                for (int i = 0; i < TR; ++i) {
+                    const int r = ty * TR + i;
+                    if ((used[r >> 6] >> (r & 63)) & 1ull) continue;
+                    // |z|^2 orders the candidates as |z| does, without a sqrt
+                    const double re = a[i][kj][0], im = CPLX ? a[i][kj][E - 1] : 0.0;
+                    const double v = CPLX ? re * re + im * im : fabs(re);
+                    if (v > best) {
+                        best = v;
+                        bi = r;
+                    }
+                }
+                // the GY = 16 owners of column k are one DPP row: combine
+                // (value, first row) pairs in registers
+                pivot_combine<0xB1>(best, bi);
+                pivot_combine<0x4E>(best, bi);
+                pivot_combine<0x141>(best, bi);
+                pivot_combine<0x140>(best, bi);
+                if (ty == 0) {
+                    piv_sh[buf] = bi;
+                    best_sh[buf] = best;
+                }
+            }
+            __syncthreads();
+            const int p = piv_sh[buf];
+            const double best = best_sh[buf];
+            if (!(best > 0.0) || !isfinite(best) || p >= NP) {   // uniform: every thread reads the same
+                singular = true;
+                break;
+            }
+            {
+                int e;
+                det_m = frexp(det_m * best, &e);
+                det_e += e;
+            }
+            // The pivot row goes to LDS with 1 in column k, its value to
+            // pval, and its owners zero it in registers; the owners of column
+            // k publish it with -1 in the pivot row and zero it.  One update
+            // a -= f_r * (prow / pivot) then gives every row its new values:
+            // the pivot row 0 + prow/pivot, column k -f_r/pivot (1/pivot in
+            // the pivot row), the rest the rank-1 elimination.  LDS slots are
+            // ordered by owner thread (column j*GX + tx, row i*GY + ty), so a
+            // wavefront's reads fall in distinct banks.
+            if (ty == p / TR) {
+#pragma unroll
+                for (int i = 0; i < TR; ++i)
+                    if (i == p % TR) {
+#pragma unroll
+                        for (int j = 0; j < TC; ++j) {
+                            const bool kk = j == kj && tx == kb;
+                            if (kk) {
+                                pval[buf][0] = a[i][j][0];
+                                pval[buf][1] = CPLX ? a[i][j][E - 1] : 0.0;
+                            }
+                            prow[buf][(j * GX + tx) * E] = kk ? 1.0 : a[i][j][0];
+                            if (CPLX) prow[buf][(j * GX + tx) * E + E - 1] = kk ? 0.0 : a[i][j][E - 1];
+#pragma unroll
+                            for (int q = 0; q < E; ++q) a[i][j][q] = 0.0;
+                        }
+                    }
+            }
+            if (tx == kb) {
+#pragma unroll
+                for (int i = 0; i < TR; ++i) {
+                    const bool pr = ty * TR + i == p;
+                    fcol[buf][(i * GY + ty) * E] = pr ? -1.0 : a[i][kj][0];
+                    if (CPLX) fcol[buf][(i * GY + ty) * E + E - 1] = pr ? 0.0 : a[i][kj][E - 1];
+#pragma unroll
+                    for (int q = 0; q < E; ++q) a[i][kj][q] = 0.0;
+                }
+            }
+            __syncthreads();
+            const double dr = pval[buf][0], di = pval[buf][1];
+            double ir, ii;
+            if (CPLX) {
+                const double inv_d2 = 1.0 / (dr * dr + di * di);
+                ir = dr * inv_d2;
+                ii = -di * inv_d2;
+            } else {
+                ir = 1.0 / dr;
+                ii = 0.0;
+            }
+            double nr[TC][E];
+#pragma unroll
+            for (int j = 0; j < TC; ++j) {
+                const double xr = prow[buf][(j * GX + tx) * E];
+                const double xi = CPLX ? prow[buf][(j * GX + tx) * E + E - 1] : 0.0;
+                nr[j][0] = CPLX ? xr * ir - xi * ii : xr * ir;
+                if (CPLX) nr[j][E - 1] = xr * ii + xi * ir;
+            }
 #pragma unroll
             for (int i = 0; i < TR; ++i) {
-                const int r = ty * TR + i;
-                if ((used[r >> 6] >> (r & 63)) & 1ull) continue;
-#pragma unroll
-                for (int j = 0; j < TC; ++j)
-                    if (j == kj) {
-                        // |z|^2 orders the candidates as |z| does, without a sqrt
-                        const double re = a[i][j][0], im = CPLX ? a[i][j][E - 1] : 0.0;
-                        const double v = CPLX ? re * re + im * im : fabs(re);
-                        if (v > best) {
-                            best = v;
-                            bi = r;
-                        }
-                    }
-            }
-            // the GY = 16 owners of column k are one DPP row: combine
-            // (value, first row) pairs in registers
-            pivot_combine<0xB1>(best, bi);
-            pivot_combine<0x4E>(best, bi);
-            pivot_combine<0x141>(best, bi);
-            pivot_combine<0x140>(best, bi);
-            if (ty == 0) {
-                piv_sh[buf] = bi;
-                best_sh[buf] = best;
-            }
-        }
-        __syncthreads();
-        const int p = piv_sh[buf];
-        const double best = best_sh[buf];
-        if (!(best > 0.0) || !isfinite(best) || p >= NP) {   // uniform: every thread reads the same
-            singular = true;
-            break;
-        }
-        if (tid == 0) log_det += CPLX ? 0.5 * log(best) : log(best);   // only thread 0 stores it
-        // the pivot row goes to LDS with 1 in column k (so every thread
-        // normalizes it without a select), column k goes as it is, and its
-        // owners then zero it: in place, column k of the other rows becomes
-        // -f_r / pivot and of the pivot row 1 / pivot
-        if (ty == p / TR) {
-#pragma unroll
-            for (int i = 0; i < TR; ++i)
-                if (i == p % TR)
-#pragma unroll
-                    for (int j = 0; j < TC; ++j) {
-                        const bool kk = tx * TC + j == k;
-                        prow[buf][(tx * TC + j) * E] = kk ? 1.0 : a[i][j][0];
-                        if (CPLX) prow[buf][(tx * TC + j) * E + E - 1] = kk ? 0.0 : a[i][j][E - 1];
-                    }
-        }
-        if (tx == kb) {
-#pragma unroll
-            for (int j = 0; j < TC; ++j)
-                if (j == kj)
-#pragma unroll
-                    for (int i = 0; i < TR; ++i) {
-#pragma unroll
-                        for (int q = 0; q < E; ++q) {
-                            fcol[buf][(ty * TR + i) * E + q] = a[i][j][q];
-                            a[i][j][q] = 0.0;
-                        }
-                    }
-        }
-        __syncthreads();
-        const double dr = fcol[buf][p * E], di = CPLX ? fcol[buf][p * E + E - 1] : 0.0;
-        double ir, ii;
-        if (CPLX) {
-            const double inv_d2 = 1.0 / (dr * dr + di * di);
-            ir = dr * inv_d2;
-            ii = -di * inv_d2;
-        } else {
-            ir = 1.0 / dr;
-            ii = 0.0;
-        }
-        double nr[TC][E];
-#pragma unroll
-        for (int j = 0; j < TC; ++j) {
-            const double xr = prow[buf][(tx * TC + j) * E];
-            const double xi = CPLX ? prow[buf][(tx * TC + j) * E + E - 1] : 0.0;
-            nr[j][0] = CPLX ? xr * ir - xi * ii : xr * ir;
-            if (CPLX) nr[j][E - 1] = xr * ii + xi * ir;
-        }
-#pragma unroll
-        for (int i = 0; i < TR; ++i) {
-            const int r = ty * TR + i;
-            if (r == p) {
-#pragma unroll
-                for (int j = 0; j < TC; ++j)
-#pragma unroll
-                    for (int q = 0; q < E; ++q) a[i][j][q] = nr[j][q];
-            } else {
-                const double fr = fcol[buf][r * E], fi = CPLX ? fcol[buf][r * E + E - 1] : 0.0;
+                const double fr = fcol[buf][(i * GY + ty) * E], fi = CPLX ? fcol[buf][(i * GY + ty) * E + E - 1] : 0.0;
 #pragma unroll
                 for (int j = 0; j < TC; ++j) {
                     a[i][j][0] -= CPLX ? fr * nr[j][0] - fi * nr[j][E - 1] : fr * nr[j][0];
                     if (CPLX) a[i][j][E - 1] -= fr * nr[j][E - 1] + fi * nr[j][0];
                 }
             }
-        }
-        used[p >> 6] |= 1ull << (p & 63);
-        if (tid == 0) {
-            perm[k] = p;
-            pinv[p] = k;
+            used[p >> 6] |= 1ull << (p & 63);
+            if (tid == 0) {
+                perm[k] = p;
+                pinv[p] = k;
+            }
         }
     }
+    const double log_det = (CPLX ? 0.5 : 1.0) * (log(det_m) + det_e * 0.69314718055994530942);
     if (singular) {
         if (tid == 0) S.status[w] = 1;
         return;
