@@ -222,6 +222,10 @@ struct alea_system {
             m.pair_list[a][b][m.pair_list_n[a][b]++] = f;
             if (a != b) m.pair_list[b][a][m.pair_list_n[b][a]++] = f;
         }
+        // the sweep copies every image factor's spline to LDS at this stride
+        m.spl_stride = 0;
+        for (int f = 0; f < m.n_pair; ++f)
+            if (m.pair[f].rad.image) m.spl_stride = std::max(m.spl_stride, 3 * m.pair[f].rad.n_knots);
         // the groups partition the particles (ParticleGroup ranges,
         // particle_group.h:21-27): the kernels walk pair partners group by group
         {
@@ -280,10 +284,13 @@ struct alea_system {
         st.status = status.p;
         st.n_accepted = nacc.p;
         // LDS of one walker: positions, k-space densities, v', column j, the
-        // Sherman-Morrison operands and the resident block's row sums, the
-        // last four over the padded rows (carve() in kernels.hip)
+        // Sherman-Morrison operands and the resident block's row sums (these
+        // four over the padded rows), then copies of the pair splines and the
+        // k-space Jastrow vectors and weights (carve() in kernels.hip)
         walker_lds = sizeof(double) * (2 * static_cast<size_t>(np) + 2 * m.ng * m.nkj +
-                                       9 * static_cast<size_t>(std::max(rows_max, kRowPad)));
+                                       9 * static_cast<size_t>(std::max(rows_max, kRowPad)) +
+                                       static_cast<size_t>(m.n_pair) * m.spl_stride +
+                                       (2 + static_cast<size_t>(m.ng) * m.ng) * m.nkj);
         {
             const size_t e = cplx ? 2 : 1, n = static_cast<size_t>(nb_max);
             refresh_lds = 8 * (e * n * n + e * n + 8) + 4 * (8 + n + (n & 1)) + 8 * n;

@@ -203,7 +203,9 @@ __device__ inline void spline_eval(const double* sc, int n_knots, double r_cut, 
           (a[0] * ddp0 + a[1] * ddp1 + a[2] * ddp2 + b[0] * ddq0 - b[1] * ddq1 + b[2] * ddq2);
 }
 
-__device__ inline void radial_eval(const DevRadial& R, double r, double& u, double& du, double& d2u) {
+// spline: the factor's image-sum spline (R.spline, or a copy of it in LDS)
+__device__ inline void radial_eval(const DevRadial& R, double r, double& u, double& du, double& d2u,
+                                   const double* spline) {
     if (R.kind == 2) {   // McMillan: u = -(b/r)^n
         const double x = R.c[0] / r;
         double xn = 1.0;
@@ -269,11 +271,14 @@ __device__ inline void radial_eval(const DevRadial& R, double r, double& u, doub
     if (R.image) {
         if (r >= R.r_cut) { u = du = d2u = 0.0; return; }
         double f, df, d2f;
-        spline_eval(R.spline, R.n_knots, R.r_cut, R.inv_delta, r, f, df, d2f);
+        spline_eval(spline, R.n_knots, R.r_cut, R.inv_delta, r, f, df, d2f);
         u -= R.u_inf + f;
         du -= df;
         d2u -= d2f;
     }
+}
+__device__ inline void radial_eval(const DevRadial& R, double r, double& u, double& du, double& d2u) {
+    radial_eval(R, r, u, du, d2u, R.spline);
 }
 
 // Pair contribution of a radial u to the gradient and per-dimension
@@ -430,6 +435,8 @@ struct Shared {
     double* u;     // [2*rows] column j of A (old)
     double* f;     // [4*rows]: u / R, then v again, both as C2<T> (determinant precision)
     double* rabs;  // [rows] row sums |D| of the block held in registers (register path)
+    double* spl;   // [n_pair][spl_stride] pair-factor splines (copies of M.pair[f].rad.spline)
+    double* kj;    // [nkj][2] k-space Jastrow vectors, then kjw [ng][ng][nkj]
 };
 
 __device__ inline Shared carve(const DevModel& M, double* base) {
@@ -442,6 +449,8 @@ __device__ inline Shared carve(const DevModel& M, double* base) {
     sh.u = sh.v + 2 * nr;
     sh.f = sh.u + 2 * nr;
     sh.rabs = sh.f + 4 * nr;
+    sh.spl = sh.rabs + nr;
+    sh.kj = sh.spl + M.n_pair * M.spl_stride;
     return sh;
 }
 
@@ -470,6 +479,7 @@ struct Walker {
     double log_j;            // Jastrow + one-body part of log|psi|
     double log_b0 = 0.0, log_b1 = 0.0;   // determinant blocks (kMaxBlocks = 2)
     __device__ double& log_b(int b) { return b == 0 ? log_b0 : log_b1; }
+    double kdr = 0.0, kdi = 0.0;  // this lane's k-space phase change of the last move
     C2<T> col[NR > 0 ? NR : 1];   // column `lane` of the resident block (NR > 0)
     int res_b = -1;               // resident block, -1 for none
     // the resident block's condition estimate and update count (its row
@@ -547,6 +557,18 @@ struct Walker {
         log_j = S.logs[w * 4 + 0];
         if (M.n_blocks > 0) log_b0 = S.logs[w * 4 + 1];
         if (M.n_blocks > 1) log_b1 = S.logs[w * 4 + 2];
+        // the tables every move gathers from: pair splines and k vectors
+        for (int f = 0; f < M.n_pair; ++f) {
+            const DevRadial& R = M.pair[f].rad;
+            if (!R.image) continue;
+            for (int i = lane; i < 3 * R.n_knots; i += W) sh.spl[f * M.spl_stride + i] = R.spline[i];
+        }
+        for (int k = lane; k < M.nkj; k += W) {
+            sh.kj[2 * k] = M.kjx[k];
+            sh.kj[2 * k + 1] = M.kjy[k];
+            for (int g = 0; g < M.ng; ++g)
+                for (int h = 0; h < M.ng; ++h) sh.kj[2 * M.nkj + (g * M.ng + h) * M.nkj + k] = M.kjw[g][h][k];
+        }
         __syncthreads();
     }
     __device__ void store() {
@@ -588,10 +610,12 @@ struct Walker {
             double n0 = nx0 - xk0, n1 = nx1 - xk1;
             const double rn = dist(M.cell, n0, n1);
             for (int fi = 0; fi < nf; ++fi) {
-                const DevRadial& R = M.pair[M.pair_list[gp][h][fi]].rad;
+                const int f = M.pair_list[gp][h][fi];
+                const DevRadial& R = M.pair[f].rad;
+                const double* spl = sh.spl + f * M.spl_stride;   // LDS copy (image factors)
                 double uo, duo, d2uo, un, dun, d2un;
-                radial_eval(R, ro, uo, duo, d2uo);
-                radial_eval(R, rn, un, dun, d2un);
+                radial_eval(R, ro, uo, duo, d2uo, spl);
+                radial_eval(R, rn, un, dun, d2un, spl);
                 acc[0] += un - uo;
                 if constexpr (derivs) {
                     double g0, g1, l0, l1;
@@ -609,18 +633,20 @@ struct Walker {
         }
         // k-space: lanes over k
         if (M.nkj > 0 && M.kj_active[gp]) {
-            for (int k = lane; k < M.nkj; k += W) {
-                const double kx = M.kjx[k], ky = M.kjy[k];
+            for (int k = lane; k < M.nkj; k += W) {   // one k per lane (nkj <= 64)
+                const double kx = sh.kj[2 * k], ky = sh.kj[2 * k + 1];
                 double so, co, sn, cn;
                 sincos_bounded(kx * x0 + ky * x1, &so, &co);
                 sincos_bounded(kx * nx0 + ky * nx1, &sn, &cn);
                 const double dr = cn - co, di = sn - so;
+                kdr = dr;   // kept for jastrow_commit of this move
+                kdi = di;
                 double dl = 0.0;
                 // moved particle's gradient at the old / new position
                 double gold = 0.0, gnew = 0.0, lnew = 0.0;
                 for (int h = 0; h < M.ng; ++h) {
                     if (!M.kj_active[h]) continue;
-                    const double wgh = M.kjw[gp][h][k];
+                    const double wgh = sh.kj[2 * M.nkj + (gp * M.ng + h) * M.nkj + k];
                     if (wgh == 0.0) continue;
                     const double rr = sh.rho[(h * M.nkj + k) * 2], ri = sh.rho[(h * M.nkj + k) * 2 + 1];
                     if (h == gp) {
@@ -684,17 +710,13 @@ struct Walker {
         jastrow_out(acc, out);
     }
 
-    // Commit the k-space densities of an accepted move of p.
-    __device__ void jastrow_commit(int p, double x0, double x1, double nx0, double nx1) {
+    // Commit the k-space densities of the accepted move of p, with the phase
+    // change jastrow_lanes computed for it (one k per lane).
+    __device__ void jastrow_commit(int p) {
         const int gp = M.group_of[p];
-        if (M.nkj > 0 && M.kj_active[gp]) {
-            for (int k = lane; k < M.nkj; k += W) {
-                double so, co, sn, cn;
-                sincos_bounded(M.kjx[k] * x0 + M.kjy[k] * x1, &so, &co);
-                sincos_bounded(M.kjx[k] * nx0 + M.kjy[k] * nx1, &sn, &cn);
-                sh.rho[(gp * M.nkj + k) * 2] += cn - co;
-                sh.rho[(gp * M.nkj + k) * 2 + 1] += sn - so;
-            }
+        if (M.nkj > 0 && M.kj_active[gp] && lane < M.nkj) {
+            sh.rho[(gp * M.nkj + lane) * 2] += kdr;
+            sh.rho[(gp * M.nkj + lane) * 2 + 1] += kdi;
         }
     }
 
@@ -719,14 +741,21 @@ struct Walker {
             // column j lives in lane j: stage it through LDS for the lanes over m
             make_resident(b);
             const int nch = M.blk_rows[b] / kRowPad;
+            // (in the determinant precision, 16 bytes per store)
             if (lane == j) {
 #pragma unroll
                 for (int ch = 0; ch < NR / kRowPad; ++ch)
                     if (ch < nch) {
+                        const int m0 = ch * kRowPad;
+                        if constexpr (sizeof(T) == 4) {
+                            float4* s4 = reinterpret_cast<float4*>(sh.u);
 #pragma unroll
-                        for (int i = 0; i < kRowPad; ++i) {
-                            sh.u[2 * (ch * kRowPad + i)] = static_cast<double>(col[ch * kRowPad + i].re);
-                            sh.u[2 * (ch * kRowPad + i) + 1] = static_cast<double>(col[ch * kRowPad + i].im);
+                            for (int i = 0; i < kRowPad / 2; ++i)
+                                s4[m0 / 2 + i] = make_float4(col[m0 + 2 * i].re, col[m0 + 2 * i].im,
+                                                             col[m0 + 2 * i + 1].re, col[m0 + 2 * i + 1].im);
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < kRowPad; ++i) reinterpret_cast<C2<T>*>(sh.u)[m0 + i] = col[m0 + i];
                         }
                     }
             }
@@ -735,7 +764,7 @@ struct Walker {
         for (int i = 0; i < 20; ++i) acc[i] = 0.0;
         for (int m = lane; m < n; m += W) {
             C2<T> a;
-            if constexpr (NR > 0) a = C2<T>{static_cast<T>(sh.u[2 * m]), static_cast<T>(sh.u[2 * m + 1])};
+            if constexpr (NR > 0) a = reinterpret_cast<const C2<T>*>(sh.u)[m];
             else a = A.get(m, j);
             const double ar = static_cast<double>(a.re), ai = static_cast<double>(a.im);
             Orb on;
@@ -744,10 +773,7 @@ struct Walker {
             const T vr = static_cast<T>(on.vr), vi = static_cast<T>(on.vi);
             sh.v[2 * m] = static_cast<double>(vr);
             sh.v[2 * m + 1] = static_cast<double>(vi);
-            if constexpr (NR == 0) {
-                sh.u[2 * m] = ar;
-                sh.u[2 * m + 1] = ai;
-            }
+            if constexpr (NR == 0) reinterpret_cast<C2<T>*>(sh.u)[m] = a;
             const double tr = ar * static_cast<double>(vr) - ai * static_cast<double>(vi);
             const double ti = ar * static_cast<double>(vi) + ai * static_cast<double>(vr);
             acc[0] += tr;
@@ -983,7 +1009,7 @@ struct Walker {
             const C2<T>* vT = reinterpret_cast<const C2<T>*>(sh.f) + rows;   // packed after f
             for (int m = lane; m < rows; m += W) {
                 if (m < n) {
-                    const C2<T> um{static_cast<T>(sh.u[2 * m]), static_cast<T>(sh.u[2 * m + 1])};
+                    const C2<T> um = reinterpret_cast<const C2<T>*>(sh.u)[m];
                     fT[m] = cmul(um, C2<T>{invRr, invRi});
                     reinterpret_cast<C2<T>*>(sh.f)[rows + m] =
                         C2<T>{static_cast<T>(sh.v[2 * m]), static_cast<T>(sh.v[2 * m + 1])};
@@ -1223,7 +1249,7 @@ struct Walker {
         }
         if (!acc) return false;
         log_j += J.dlog;
-        jastrow_commit(p, x0, x1, nx0, nx1);
+        jastrow_commit(p);
         __syncthreads();
         if (lane == 0) {
             double w0 = nx0, w1 = nx1;

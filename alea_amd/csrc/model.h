@@ -67,6 +67,7 @@ struct DevModel {
     DevPair pair[kMaxPair];
     int pair_list_n[kMaxGroups][kMaxGroups] = {};
     int pair_list[kMaxGroups][kMaxGroups][kMaxPair] = {};
+    int spl_stride = 0;   // doubles per factor of the splines the sweep copies to LDS
 
     // merged k-space Jastrow: U = sum_k sum_{g<=h} w_gh(k) T_gh
     int nkj = 0;
