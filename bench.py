@@ -206,6 +206,17 @@ def main():
             t = json.load(f)
         if t.get("walkers") == args.walkers and t.get("precision") == "mixed":
             traffic = t.get("hbm_bytes_per_launch")
+    # the sweep holds the swept block in registers, so HBM does not bound it:
+    # its VALU issue time (SQ_INSTS_VALU from rocprofv3, profiles/) against
+    # the kernel time measured here
+    valu = None
+    vf = os.path.join(ROOT, "profiles", "sweep_counters.json")
+    if os.path.exists(vf):
+        with open(vf) as f:
+            v = json.load(f)
+        if v.get("walkers") == args.walkers and v.get("precision") == "mixed":
+            valu = {"insts_per_launch": v["valu_insts_per_launch"], "issue_ms": v["valu_issue_ms"],
+                    "issue_frac": v["valu_issue_ms"] / mixed["kernel_ms"], "wait_frac": v["wait_fraction"]}
     line = {
         "metric": "walker-moves/s, 2D HEG N=122, mixed vs FP64 GPU",
         "value": mixed["moves_per_s"],
@@ -231,8 +242,12 @@ def main():
         "im_e_per_particle": mixed["im_e_per_particle"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_sweep<float,true>", "kernel_ms": mixed["kernel_ms"],
-                     "bytes_per_move": b_move},
+                     "kernel": "k_sweep<float,true,false,64>", "kernel_ms": mixed["kernel_ms"],
+                     "bytes_per_move": b_move,
+                     "model": "achieved = the algorithmic bytes per move x moves / kernel time; with the "
+                              "swept block in registers the kernel moves `traffic` bytes and is bound by "
+                              "VALU issue and latency (`valu`)",
+                     "valu": valu},
         "serving": None if args.no_serving else serving_leg(),
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(tau),
     }
