@@ -2167,18 +2167,25 @@ template <typename T, bool CPLX, bool DMC>
 static void launch_sweep_t(const DevModel* m, const DevState& s, const SweepParams& p, int nw, int nr,
                            size_t lds, hipStream_t st) {
     const dim3 g(nw), t(64);
-    switch (nr) {
-        case 16: hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 16>), g, t, lds, st, m, s, p); break;
-        case 32: hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 32>), g, t, lds, st, m, s, p); break;
-        case 64:
-            // the drift-diffusion move's extra live state leaves no room for
-            // a 128-register column: DMC sweeps of such blocks stay in HBM
-            if constexpr (64 * kColWords<T, CPLX> <= 128 && (!DMC || 64 * kColWords<T, CPLX> <= 64)) {
-                hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 64>), g, t, lds, st, m, s, p);
-                break;
-            }
-            [[fallthrough]];
-        default: hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 0>), g, t, lds, st, m, s, p); break;
+    if constexpr (DMC) {
+        // The drift-diffusion move keeps gradients, Laplacians and the drift
+        // live across the whole move: with a register tile beside them the
+        // kernel spills (73 registers at 32 real rows, 245 at 64), and the
+        // HBM path measured faster, so DMC sweeps stream the inverse.
+        hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 0>), g, t, lds, st, m, s, p);
+        return;
+    } else {
+        switch (nr) {
+            case 16: hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 16>), g, t, lds, st, m, s, p); break;
+            case 32: hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 32>), g, t, lds, st, m, s, p); break;
+            case 64:
+                if constexpr (64 * kColWords<T, CPLX> <= 128) {
+                    hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 64>), g, t, lds, st, m, s, p);
+                    break;
+                }
+                [[fallthrough]];
+            default: hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 0>), g, t, lds, st, m, s, p); break;
+        }
     }
 }
 

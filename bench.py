@@ -149,6 +149,92 @@ def serving_leg(precision=1):
     return st
 
 
+DMC_CFG = dict(n_up=37, n_dn=37, rs=40.0, target=16384, tau=2.0, steps=500, equilibration=100,
+               vmc_sweeps=200, drummond_order=8, drummond_sigma=1e-3, drummond_seed=2, pos_seed=1,
+               seed=7, generations=10.0, alpha=2.0)
+
+
+def dmc_spec(c=DMC_CFG):
+    """The N=74, rs=40 paramagnetic HEG of the published DMC case: plane-wave
+    Slater determinants, the image-summed Pade pair Jastrow with the Kato
+    cusps, plus a Drummond polynomial (order 8, C = 3, cutoff L/2) on each
+    spin pair with no cusp of its own, so the Pade factor alone sets the
+    cusp; 2-D Ewald.  The Drummond polynomial is in r itself
+    (drummond_jastrow.cpp:47-64), so coefficients of size 1e-3 would make
+    u ~ 1e20 at r ~ 300: they are drawn N(0, sigma) in units of the cutoff,
+    alpha_l = xi_l / L^(C + l) for the term in r^l, so each term is ~sigma at
+    r ~ L."""
+    from alea_amd.system import heg_spec, square_lattice
+    n = c["n_up"] + c["n_dn"]
+    half = square_lattice(n, c["rs"])[0] / 2.0
+    rng = np.random.default_rng(c["drummond_seed"])
+    C = 3
+    # coeffs[0] multiplies r^0, coeffs[k >= 1] multiplies r^(k+1)
+    powers = [0] + list(range(2, c["drummond_order"] + 1))
+    dr = []
+    for ga, gb in ((0, 0), (1, 1), (0, 1)):
+        xi = rng.normal(0.0, c["drummond_sigma"], c["drummond_order"])
+        dr.append(dict(ga=ga, gb=gb, C=C, gamma=0.0, L=half,
+                       coeffs=[float(x) / half ** (C + l) for x, l in zip(xi, powers)]))
+    return heg_spec(c["n_up"], c["n_dn"], c["rs"], drummond=dr)
+
+
+def dmc_leg(c=DMC_CFG):
+    """Fixed-node DMC on the N=74, rs=40 HEG in FP64 and in mixed form from
+    the same walkers and seeds: VMC equilibration, then `steps` DMC steps
+    (one drift-diffusion sweep each, bare reweighting, integer branching,
+    universal cutoff); E/N is the weighted mean of the steps after
+    `equilibration`, with a reblocked error bar."""
+    from alea_amd import dmc, statistics
+    from alea_amd.system import uniform_positions
+    from alea_amd.walkers import gpu_walkers, seed as seed_walkers
+    spec = dmc_spec(c)
+    n = spec.n_particles
+    cap = 2 * c["target"] + 1
+    pos = uniform_positions(spec, cap, np.random.default_rng(c["pos_seed"]))
+    out = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        print(f"bench: DMC N={n} rs={c['rs']:g} {name}", file=sys.stderr, flush=True)
+        w = gpu_walkers(spec, cap, prec)
+        w.set_positions(pos)
+        w.evaluate()
+        w.permute(np.arange(c["target"]))
+        seed_walkers(w, c["seed"])
+        w.vmc_sweeps(c["vmc_sweeps"], (0.5 * c["rs"]) ** 2)
+        seed_walkers(w, c["seed"], dmc.K_DMC_THREAD_STREAM)
+        d = dmc.DMC(w, c["tau"], c["target"], n_substeps=1, generations=c["generations"],
+                    cutoff="universal", alpha=c["alpha"], seed=c["seed"])
+        series, pops, moves = [], [], 0
+        t0 = time.perf_counter()
+        for step in range(c["steps"]):
+            moves += w.nw * n
+            e_sum, w_sum = d.step(measure=step >= c["equilibration"])
+            if step >= c["equilibration"]:
+                series.append(e_sum / w_sum / n)
+                pops.append(w.nw)
+            if step % 100 == 99:
+                print(f"bench: DMC {name} step {step + 1}, population {w.nw}", file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        rb = statistics.reblock(series)
+        out[name] = dict(e_per_particle=rb.mean, error=rb.error, tau_int=rb.tau_int,
+                         plateau_found=rb.plateau_found, mean_population=float(np.mean(pops)),
+                         walker_moves_per_s=moves / wall, wall_s=wall)
+        out[name]["_series"] = series
+        w.close()
+    de = out["mixed"]["e_per_particle"] - out["fp64"]["e_per_particle"]
+    sigma = math.hypot(out["mixed"]["error"], out["fp64"]["error"])
+    out["mixed_minus_fp64"] = de
+    out["combined_error"] = sigma
+    out["within_1e-5"] = abs(de) <= 1e-5
+    out["within_1_sigma"] = abs(de) <= sigma
+    # the two runs share every random stream, so their per-step difference
+    # is a correlated-sampling estimate of the precision bias
+    diff = statistics.reblock(np.subtract(out["mixed"].pop("_series"), out["fp64"].pop("_series")))
+    out["correlated_difference"] = dict(mean=diff.mean, error=diff.error, plateau_found=diff.plateau_found)
+    out["config"] = dict(c, system="N=74 (37+37) paramagnetic HEG, rs=40, plane-wave nodes, Pade + Drummond, Ewald")
+    return out
+
+
 def copy_calibration(n_bytes, reps=3):
     """A plain device-to-device copy of n_bytes, reps times: the known-size
     reference that scales the FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
@@ -175,6 +261,8 @@ def main():
                     help="run a plain copy of BYTES bytes (counter calibration) and exit")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--serving-only", action="store_true", help="run the serving leg alone, print its dict")
+    ap.add_argument("--no-dmc", action="store_true", help="skip the DMC accuracy leg")
+    ap.add_argument("--dmc-only", action="store_true", help="run the DMC leg alone, print its dict")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
@@ -185,6 +273,9 @@ def main():
         return
     if args.serving_only:
         print(json.dumps(serving_leg()), flush=True)
+        return
+    if args.dmc_only:
+        print(json.dumps(dmc_leg()), flush=True)
         return
     tau = (0.5 * RS) ** 2   # sigma = sqrt(tau/m) = 0.5 rs
     if args.only:
@@ -249,6 +340,7 @@ def main():
                               "VALU issue and latency (`valu`)",
                      "valu": valu},
         "serving": None if args.no_serving else serving_leg(),
+        "dmc": None if args.no_dmc else dmc_leg(),
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(tau),
     }
     print(json.dumps(line), flush=True)
