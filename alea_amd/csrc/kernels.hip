@@ -383,6 +383,24 @@ template <typename T>
 __device__ inline C2<T> cmul(C2<T> a, C2<T> b) {
     return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
+__device__ inline float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// Entries m and m+1 (m even) of a complex LDS vector, in 16-byte reads: one
+// read for two single-precision entries, one per double-precision entry.
+// The vector starts 16-byte aligned.
+template <typename T>
+__device__ inline void lds_pair(const C2<T>* v, int m, C2<T> (&out)[2]) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 q = reinterpret_cast<const float4*>(v)[m / 2];
+        out[0] = C2<T>{q.x, q.y};
+        out[1] = C2<T>{q.z, q.w};
+    } else {
+        const double2 q0 = reinterpret_cast<const double2*>(v)[m];
+        const double2 q1 = reinterpret_cast<const double2*>(v)[m + 1];
+        out[0] = C2<T>{q0.x, q0.y};
+        out[1] = C2<T>{q1.x, q1.y};
+    }
+}
 
 // Determinant inverse A = D^-1 (A[m][c]: orbital m, particle c), row-major
 // per walker as in the reference (slater_determinant.cpp:363-365).  Lane c
@@ -1028,16 +1046,27 @@ struct Walker {
             double colmax = 0.0;
             if constexpr (NR > 0) {
                 // the same two passes on the column held in registers; lanes
-                // past n hold a zero column and stay zero
+                // past n hold a zero column and stay zero.  v' and u/R are
+                // LDS broadcasts, read 16 bytes at a time; each complex
+                // multiply-accumulate is four FMAs.
                 C2<T> r{T(0), T(0)};
 #pragma unroll
                 for (int ch = 0; ch < NR / CH; ++ch)
                     if (ch < nch) {
 #pragma unroll
-                        for (int i = 0; i < CH; ++i) {
-                            const C2<T> pr = cmul(vT[ch * CH + i], col[ch * CH + i]);
-                            r.re += pr.re;
-                            r.im += pr.im;
+                        for (int i = 0; i < CH; i += 2) {
+                            C2<T> v[2];
+                            lds_pair(vT, ch * CH + i, v);
+#pragma unroll
+                            for (int q = 0; q < 2; ++q) {
+                                const C2<T>& a = col[ch * CH + i + q];
+                                r.re = fmadd(v[q].re, a.re, r.re);
+                                if constexpr (CPLX) {
+                                    r.re = fmadd(-v[q].im, a.im, r.re);
+                                    r.im = fmadd(v[q].re, a.im, r.im);
+                                    r.im = fmadd(v[q].im, a.re, r.im);
+                                }
+                            }
                         }
                     }
                 if (lane == j) r.re -= T(1);
@@ -1046,11 +1075,20 @@ struct Walker {
                 for (int ch = NR / CH - 1; ch >= 0; --ch)
                     if (ch < nch) {
 #pragma unroll
-                        for (int i = 0; i < CH; ++i) {
-                            const C2<T> pr = cmul(fT[ch * CH + i], r);
-                            col[ch * CH + i].re -= pr.re;
-                            col[ch * CH + i].im -= pr.im;
-                            cs += fabs(col[ch * CH + i].re) + (CPLX ? fabs(col[ch * CH + i].im) : T(0));
+                        for (int i = 0; i < CH; i += 2) {
+                            C2<T> f[2];
+                            lds_pair(fT, ch * CH + i, f);
+#pragma unroll
+                            for (int q = 0; q < 2; ++q) {
+                                C2<T>& a = col[ch * CH + i + q];
+                                a.re = fmadd(-f[q].re, r.re, a.re);
+                                if constexpr (CPLX) {
+                                    a.re = fmadd(f[q].im, r.im, a.re);
+                                    a.im = fmadd(-f[q].re, r.im, a.im);
+                                    a.im = fmadd(-f[q].im, r.re, a.im);
+                                }
+                                cs += fabs(a.re) + (CPLX ? fabs(a.im) : T(0));
+                            }
                         }
                     }
                 colmax = static_cast<double>(cs);
