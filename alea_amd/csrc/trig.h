@@ -7,9 +7,14 @@
 // 1980): x = q pi/2 + r with q = rint(x 2/pi) and pi/2 split into three
 // doubles of 33 significant bits (computed in exact decimal arithmetic), so
 // q*P1 and q*P2 are exact for |q| < 2^20; then the Taylor series of sin and
-// cos on |r| <= pi/4 (truncation error below 1e-19 at degree 17/18) and the
-// quadrant swap.  Accurate to a few ulp for |x| < 1e6; beyond that the
-// reduction degrades (tested against libm in tests/test_trig.py).
+// cos on |r| <= pi/4 (degree 15 and 16: truncation below 5e-17 and 2e-18,
+// under half an ulp) and the quadrant swap.  Accurate to a few ulp for
+// |x| < 1e6; beyond that the reduction degrades (tested against libm in
+// tests/test_trig.py).
+//
+// In device code each coefficient is pinned to scalar registers where it is
+// used (kc): hoisted out of the sweep's loops, the seventeen FP64 constants
+// would hold 34 vector registers for the whole kernel.
 #pragma once
 
 #if defined(__HIPCC__)
@@ -22,37 +27,42 @@
 
 namespace alea_amd {
 
+ALEA_HD inline double kc(double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(c));
+#endif
+    return c;
+}
+
 ALEA_HD inline void sincos_bounded(double x, double* s_out, double* c_out) {
     constexpr double kTwoOverPi = 0.6366197723675814;
     constexpr double kP1 = 1.5707963267341256;
     constexpr double kP2 = 6.077100506303966e-11;
     constexpr double kP3 = 2.0222662487959506e-21;
-    const double q = rint(x * kTwoOverPi);
-    double r = x - q * kP1;
-    r = r - q * kP2;
-    r = r - q * kP3;
+    const double q = rint(x * kc(kTwoOverPi));
+    double r = x - q * kc(kP1);
+    r = r - q * kc(kP2);
+    r = r - q * kc(kP3);
     const double r2 = r * r;
-    // sin r = r + r^3 P(r^2), P(z) = -1/3! + z/5! - z^2/7! + ... + z^7/17!
-    double ps = 1.0 / 355687428096000.0;             //  1/17!
-    ps = ps * r2 - 1.0 / 1307674368000.0;            // -1/15!
-    ps = ps * r2 + 1.0 / 6227020800.0;               //  1/13!
-    ps = ps * r2 - 1.0 / 39916800.0;                 // -1/11!
-    ps = ps * r2 + 1.0 / 362880.0;                   //  1/9!
-    ps = ps * r2 - 1.0 / 5040.0;                     // -1/7!
-    ps = ps * r2 + 1.0 / 120.0;                      //  1/5!
-    ps = ps * r2 - 1.0 / 6.0;                        // -1/3!
+    // sin r = r + r^3 P(r^2), P(z) = -1/3! + z/5! - z^2/7! + ... - z^6/15!
+    double ps = kc(-1.0 / 1307674368000.0);          // -1/15!
+    ps = ps * r2 + kc(1.0 / 6227020800.0);           //  1/13!
+    ps = ps * r2 - kc(1.0 / 39916800.0);             // -1/11!
+    ps = ps * r2 + kc(1.0 / 362880.0);               //  1/9!
+    ps = ps * r2 - kc(1.0 / 5040.0);                 // -1/7!
+    ps = ps * r2 + kc(1.0 / 120.0);                  //  1/5!
+    ps = ps * r2 - kc(1.0 / 6.0);                    // -1/3!
     const double sr = r + r * r2 * ps;
-    // cos r = 1 - r^2/2! + r^4/4! - ... + r^18/18!
-    double pc = 1.0 / 6402373705728000.0;            //  1/18!
-    pc = pc * r2 - 1.0 / 20922789888000.0;           // -1/16!
-    pc = pc * r2 + 1.0 / 87178291200.0;              //  1/14!
-    pc = pc * r2 - 1.0 / 479001600.0;                // -1/12!
-    pc = pc * r2 + 1.0 / 3628800.0;                  //  1/10!
-    pc = pc * r2 - 1.0 / 40320.0;                    // -1/8!
-    pc = pc * r2 + 1.0 / 720.0;                      //  1/6!
-    pc = pc * r2 - 1.0 / 24.0;                       // -1/4!
-    pc = pc * r2 + 0.5;                              //  1/2!
-    const double cr = 1.0 - r2 * pc;
+    // cos r = 1 - r^2/2! + r^4/4! - ... + r^16/16!
+    double pc = kc(1.0 / 20922789888000.0);          //  1/16!
+    pc = pc * r2 - kc(1.0 / 87178291200.0);          // -1/14!
+    pc = pc * r2 + kc(1.0 / 479001600.0);            //  1/12!
+    pc = pc * r2 - kc(1.0 / 3628800.0);              // -1/10!
+    pc = pc * r2 + kc(1.0 / 40320.0);                //  1/8!
+    pc = pc * r2 - kc(1.0 / 720.0);                  // -1/6!
+    pc = pc * r2 + kc(1.0 / 24.0);                   //  1/4!
+    pc = pc * r2 - 0.5;                              // -1/2!
+    const double cr = 1.0 + r2 * pc;
     const long long qi = static_cast<long long>(q);
     const int quad = static_cast<int>(qi & 3);
     double s = sr, c = cr;
