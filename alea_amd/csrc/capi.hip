@@ -29,6 +29,7 @@ void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, 
                     bool cplx, size_t lds, hipStream_t st);
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
                   bool cplx, int rows_max, size_t lds, hipStream_t st);
+void launch_draw(const DevState& s, int np, int nw, double* out, int stride, hipStream_t st);
 void launch_call(const DevModel* m, const DevState& s, const DevCall& c, int p, int mode, int nw,
                  bool mixed, bool cplx, size_t lds, hipStream_t st);
 void launch_measure(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, size_t lds,
@@ -188,6 +189,7 @@ struct alea_system {
     DevBuf<unsigned char> call_mask;
     DevBuf<double> grad_buf;       // 4 * np * 2 * nw
     DevBuf<double> tape;
+    DevBuf<double> dtape;          // one sweep of device-stream numbers, [np][3][nw]
     hipEvent_t ev0 = nullptr, ev1 = nullptr;   // brackets the last sweep launch
     bool timed = false;
     size_t walker_lds = 0, refresh_lds = 0;
@@ -1022,6 +1024,11 @@ static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* t
         hip_check(copy_on(s->stream, s->tape.p, tape, len * sizeof(double), hipMemcpyHostToDevice), "tape");
         P.tape = s->tape.p;
         P.tape_stride = stride;
+    } else if (n_sweeps > 0) {
+        // device streams: k_draw writes each sweep's numbers before it
+        const size_t len = static_cast<size_t>(s->np) * 3 * s->nw;
+        if (s->dtape.n < len) s->dtape.alloc(static_cast<size_t>(s->np) * 3 * s->capacity, s->stream);
+        stride = s->nw;
     }
     hip_check(hipMemsetAsync(s->nacc.p, 0, s->nw * sizeof(long long), s->stream), "memset");
     s->set_model_nw(s->nw);
@@ -1030,15 +1037,23 @@ static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* t
         hip_check(hipEventCreate(&s->ev0), "event");
         hip_check(hipEventCreate(&s->ev1), "event");
     }
-    // One launch per sweep; the batched inverse refresh runs every
-    // refresh_every sweeps (at most refresh_every * Nb Sherman-Morrison
-    // updates between rebuilds).  The events bracket the sweep launches only.
+    // One launch per sweep (after its k_draw on the device streams); the
+    // batched inverse refresh runs every refresh_every sweeps (at most
+    // refresh_every * Nb Sherman-Morrison updates between rebuilds).  The
+    // events bracket the draw and sweep launches only.
     const size_t tape_step = static_cast<size_t>(s->np) * 3 * stride;
     for (int i = 0; i < n_sweeps; ++i) {
         SweepParams Pi = P;
         Pi.n_sweeps = 1;
-        if (P.tape) Pi.tape = P.tape + i * tape_step;
         hip_check(hipEventRecord(s->ev0, s->stream), "event record");
+        if (P.tape) {
+            Pi.tape = P.tape + i * tape_step;
+        } else {
+            launch_draw(s->st, s->np, s->nw, s->dtape.p, stride, s->stream);
+            hip_check(hipGetLastError(), "launch draw");
+            Pi.tape = s->dtape.p;
+            Pi.tape_stride = stride;
+        }
         launch_sweep(s->dmodel, s->st, Pi, s->nw, s->mixed(), s->cplx, s->rows_max, s->walker_lds, s->stream);
         hip_check(hipGetLastError(), "launch sweep");
         hip_check(hipEventRecord(s->ev1, s->stream), "event record");

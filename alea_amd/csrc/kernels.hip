@@ -1832,24 +1832,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     const int w = blockIdx.x, lane = threadIdx.x;
     Walker<T, CPLX, NR> wk(M, S, sh, w);
     wk.load();
-    Rng rng;
-    if (!P.tape)
-        for (int i = 0; i < 4; ++i) rng.s[i] = S.rng[w * 4 + i];
     long long nacc = 0;
     double rr_p = 0.0, rr_a = 0.0;
     const int np = M.np;
+    // The random numbers come from a tape, [sweep][p][3][stride]: the
+    // caller's, or the one k_draw wrote from the walker's own stream.  The
+    // next move's three are loaded one move ahead, so the wait for them
+    // overlaps the current move.
+    const int ts = P.tape_stride;
+    auto tape_row = [&](int s, int p) { return P.tape + (static_cast<size_t>(s) * np + p) * 3 * ts; };
+    double nxt0 = tape_row(0, 0)[w], nxt1 = tape_row(0, 0)[ts + w], nxtu = tape_row(0, 0)[2 * ts + w];
     for (int s = 0; s < P.n_sweeps; ++s) {
         for (int p = 0; p < np; ++p) {
-            double xi0, xi1, uni;
-            if (P.tape) {
-                const double* row = P.tape + (static_cast<size_t>(s) * np + p) * 3 * P.tape_stride;
-                xi0 = row[w];
-                xi1 = row[P.tape_stride + w];
-                uni = row[2 * P.tape_stride + w];
-            } else {
-                xi0 = rng.gaussian();
-                xi1 = rng.gaussian();
-                uni = rng.uniform();
+            const double xi0 = nxt0, xi1 = nxt1, uni = nxtu;
+            {
+                const bool more = p + 1 < np || s + 1 < P.n_sweeps;
+                const double* row = more ? (p + 1 < np ? tape_row(s, p + 1) : tape_row(s + 1, 0)) : tape_row(s, p);
+                nxt0 = row[w];
+                nxt1 = row[ts + w];
+                nxtu = row[2 * ts + w];
             }
             const double tm = P.tau * M.inv_mass[p];
             const double sigma = sqrt(tm);
@@ -1879,9 +1880,29 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             S.rr[w * 2] += rr_p;
             S.rr[w * 2 + 1] += rr_a;
         }
-        if (!P.tape)
-            for (int i = 0; i < 4; ++i) S.rng[w * 4 + i] = rng.s[i];
     }
+}
+
+// One sweep of random numbers per walker from its xoshiro256++ stream, in
+// the order the sweep consumes them (draw_sweep_randoms, sweep_blocking.h:
+// 39-48, per walker): for each particle two Gaussians, then a uniform.
+// out: [p][3][stride]; one thread per walker.
+__global__ __launch_bounds__(64) void k_draw(DevState S, int np, int nw, double* __restrict__ out,
+                                             int stride) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    Rng rng;
+    for (int i = 0; i < 4; ++i) rng.s[i] = S.rng[w * 4 + i];
+    for (int p = 0; p < np; ++p) {
+        double* row = out + static_cast<size_t>(p) * 3 * stride;
+        const double g0 = rng.gaussian();
+        const double g1 = rng.gaussian();
+        const double u = rng.uniform();
+        row[w] = g0;
+        row[stride + w] = g1;
+        row[2 * stride + w] = u;
+    }
+    for (int i = 0; i < 4; ++i) S.rng[w * 4 + i] = rng.s[i];
 }
 
 // Per-call path: ratio of particle p at call.new_pos (mode 1) or commit of
@@ -2247,6 +2268,9 @@ void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, in
         if (cplx) launch_sweep_t<double, true, false>(m, s, p, nw, nr, lds, st);
         else launch_sweep_t<double, false, false>(m, s, p, nw, nr, lds, st);
     }
+}
+void launch_draw(const DevState& s, int np, int nw, double* out, int stride, hipStream_t st) {
+    hipLaunchKernelGGL(k_draw, dim3((nw + 63) / 64), dim3(64), 0, st, s, np, nw, out, stride);
 }
 void launch_call(const DevModel* m, const DevState& s, const DevCall& c, int p, int mode, int nw,
                  bool mixed, bool cplx, size_t lds, hipStream_t st) {
