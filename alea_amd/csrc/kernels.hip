@@ -1746,9 +1746,13 @@ __global__ __launch_bounds__(GX * 16) __attribute__((amdgpu_waves_per_eu(NP <= 6
             for (int i = 0; i < TR; ++i) {
                 const double fr = fcol[buf][(i * GY + ty) * E], fi = CPLX ? fcol[buf][(i * GY + ty) * E + E - 1] : 0.0;
 #pragma unroll
-                for (int j = 0; j < TC; ++j) {
-                    a[i][j][0] -= CPLX ? fr * nr[j][0] - fi * nr[j][E - 1] : fr * nr[j][0];
-                    if (CPLX) a[i][j][E - 1] -= fr * nr[j][E - 1] + fi * nr[j][0];
+                for (int j = 0; j < TC; ++j) {   // a -= f * nr, four FMAs per complex entry
+                    a[i][j][0] = fmadd(-fr, nr[j][0], a[i][j][0]);
+                    if constexpr (CPLX) {
+                        a[i][j][0] = fmadd(fi, nr[j][E - 1], a[i][j][0]);
+                        a[i][j][E - 1] = fmadd(-fr, nr[j][E - 1], a[i][j][E - 1]);
+                        a[i][j][E - 1] = fmadd(-fi, nr[j][0], a[i][j][E - 1]);
+                    }
                 }
             }
             used[p >> 6] |= 1ull << (p & 63);
