@@ -334,6 +334,30 @@ __device__ inline void orbital(const DevModel& M, int b, int m, double x, double
     o.gi[0] = o.gi[1] = o.li[0] = o.li[1] = 0.0;
 }
 
+// Value only, in single precision, for the mixed form's ratio: plane-wave
+// and twisted orbitals through sincos_bounded_f32 (the phase k.r reduced in
+// double); Gaussian orbitals through the double-precision path.
+__device__ inline void orbital_value_f32(const DevModel& M, int b, int m, double x, double y, float& vr,
+                                         float& vi) {
+    const int kind = M.blk_kind[b];
+    if (kind == kGaussian) {
+        Orb o;
+        orbital(M, b, m, x, y, o);
+        vr = static_cast<float>(o.vr);
+        vi = 0.0f;
+        return;
+    }
+    float s, c;
+    sincos_bounded_f32(M.orb_k[b][m][0] * x + M.orb_k[b][m][1] * y, &s, &c);
+    if (kind == kTwisted) {
+        vr = c;
+        vi = s;
+    } else {
+        vr = M.orb_sin[b][m] != 0 ? s : c;
+        vi = 0.0f;
+    }
+}
+
 // ── xoshiro256++ and the ziggurat, every lane stepping the same state ──────
 struct Rng {
     uint64_t s[4];
@@ -786,9 +810,16 @@ struct Walker {
             else a = A.get(m, j);
             const double ar = static_cast<double>(a.re), ai = static_cast<double>(a.im);
             Orb on;
-            orbital(M, b, m, nx0, nx1, on);
-            // values stored in the determinant precision
-            const T vr = static_cast<T>(on.vr), vi = static_cast<T>(on.vi);
+            T vr, vi;   // values in the determinant precision
+            if constexpr (sizeof(T) == 4 && !derivs && !old_derivs) {
+                orbital_value_f32(M, b, m, nx0, nx1, vr, vi);
+                on.vr = static_cast<double>(vr);
+                on.vi = static_cast<double>(vi);
+            } else {
+                orbital(M, b, m, nx0, nx1, on);
+                vr = static_cast<T>(on.vr);
+                vi = static_cast<T>(on.vi);
+            }
             sh.v[2 * m] = static_cast<double>(vr);
             sh.v[2 * m + 1] = static_cast<double>(vi);
             if constexpr (NR == 0) reinterpret_cast<C2<T>*>(sh.u)[m] = a;
@@ -796,8 +827,12 @@ struct Walker {
             const double ti = ar * static_cast<double>(vi) + ai * static_cast<double>(vr);
             acc[0] += tr;
             acc[1] += ti;
-            acc[2] += CPLX ? sqrt(tr * tr + ti * ti) : fabs(tr);
-            acc[19] += CPLX ? sqrt(on.vr * on.vr + on.vi * on.vi) : fabs(static_cast<double>(vr));
+            // the two magnitude sums only feed the rebuild thresholds (the
+            // cancellation bound, the condition estimate): a single-precision
+            // square root is accurate enough for both
+            acc[2] += CPLX ? static_cast<double>(__builtin_sqrtf(static_cast<float>(tr * tr + ti * ti))) : fabs(tr);
+            acc[19] += CPLX ? static_cast<double>(__builtin_sqrtf(static_cast<float>(on.vr * on.vr + on.vi * on.vi)))
+                            : fabs(static_cast<double>(vr));
             if constexpr (derivs) {
                 for (int d = 0; d < 2; ++d) {
                     acc[3 + 2 * d] += ar * on.gr[d] - ai * on.gi[d];
@@ -1217,10 +1252,12 @@ struct Walker {
         double ng[2] = {J.ng[0], J.ng[1]}, nl[2] = {J.nl[0], J.nl[1]};
         double npg[2] = {0, 0}, npl[2] = {0, 0};
         DOut D{};
+        double R2 = 1.0;   // |R|^2 of the determinant ratio
         if (b >= 0) {
             det_out(da, D);
             __syncthreads();
-            const double absR = sqrt(D.R[0] * D.R[0] + D.R[1] * D.R[1]);
+            R2 = D.R[0] * D.R[0] + D.R[1] * D.R[1];
+            const double absR = sqrt(R2);
             lr += log(fmax(absR, 1e-300));
             if (CPLX) {
                 phase += atan2(D.R[1], D.R[0]);
@@ -1263,7 +1300,9 @@ struct Walker {
         if (mode == 2) {
             acc = call->mask[w] != 0;
         } else if (!dmc) {
-            acc = uni < exp(2.0 * lr);
+            // uni < |psi'/psi|^2 = exp(2 lr), with the determinant's share as
+            // |R|^2 rather than through its log
+            acc = uni < exp(2.0 * J.dlog) * R2;
         } else {
             const double tm = tau * M.inv_mass[p];
             if (!CPLX && M.n_blocks > 0 && cos(phase) < 2.220446049250313e-16) {

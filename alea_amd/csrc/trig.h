@@ -73,4 +73,39 @@ ALEA_HD inline void sincos_bounded(double x, double* s_out, double* c_out) {
     *c_out = c;
 }
 
+// Single precision: sin and cos of x = k.r for single-precision orbital
+// values.  The reduction x = q pi/2 + r stays in double (|x| reaches ~1e2
+// and beyond), the polynomials on |r| <= pi/4 are float: sin to r^9
+// (truncation 7e-10), cos to r^8 (2.5e-8), below one float ulp.
+ALEA_HD inline float kcf(float c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(c));
+#endif
+    return c;
+}
+
+ALEA_HD inline void sincos_bounded_f32(double x, float* s_out, float* c_out) {
+    const double q = rint(x * kc(0.6366197723675814));
+    const double rd = (x - q * kc(1.5707963267341256)) - q * kc(6.077100506303966e-11);
+    const float r = static_cast<float>(rd);
+    const float r2 = r * r;
+    float ps = kcf(1.0f / 362880.0f);              //  1/9!
+    ps = ps * r2 - kcf(1.0f / 5040.0f);            // -1/7!
+    ps = ps * r2 + kcf(1.0f / 120.0f);             //  1/5!
+    ps = ps * r2 - kcf(1.0f / 6.0f);               // -1/3!
+    const float sr = r + r * r2 * ps;
+    float pc = kcf(1.0f / 40320.0f);               //  1/8!
+    pc = pc * r2 - kcf(1.0f / 720.0f);             // -1/6!
+    pc = pc * r2 + kcf(1.0f / 24.0f);              //  1/4!
+    pc = pc * r2 - 0.5f;                           // -1/2!
+    const float cr = 1.0f + r2 * pc;
+    const int quad = static_cast<int>(static_cast<long long>(q) & 3);
+    float s = sr, c = cr;
+    if (quad == 1) { s = cr; c = -sr; }
+    else if (quad == 2) { s = -sr; c = -cr; }
+    else if (quad == 3) { s = -cr; c = sr; }
+    *s_out = s;
+    *c_out = c;
+}
+
 }  // namespace alea_amd

@@ -1,5 +1,6 @@
 // Host build of alea_amd/csrc/trig.h for tests/test_trig.py: max ulp error of
-// sincos_bounded against libm over a sweep of arguments.
+// sincos_bounded, and max absolute error of sincos_bounded_f32, against libm
+// over a sweep of arguments.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -25,6 +26,15 @@ int main(int argc, char** argv) {
         if (std::fabs(rs) > 1e-3) worst_s = std::fmax(worst_s, ulp_err(s, rs));
         if (std::fabs(rc) > 1e-3) worst_c = std::fmax(worst_c, ulp_err(c, rc));
     }
-    std::printf("%.3f %.3f %.3e\n", worst_s, worst_c, abs_err);
+    // single-precision values (the mixed form's orbitals): absolute error
+    double abs_f32 = 0;
+    for (int i = 0; i < 4000000; ++i) {
+        st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+        const double x = ((st >> 11) * 0x1.0p-53 * 2.0 - 1.0) * lim;
+        float s, c;
+        alea_amd::sincos_bounded_f32(x, &s, &c);
+        abs_f32 = std::fmax(abs_f32, std::fmax(std::fabs(s - std::sin(x)), std::fabs(c - std::cos(x))));
+    }
+    std::printf("%.3f %.3f %.3e %.3e\n", worst_s, worst_c, abs_err, abs_f32);
     return 0;
 }
