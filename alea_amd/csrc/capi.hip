@@ -224,10 +224,28 @@ struct alea_system {
             m.pair_list[a][b][m.pair_list_n[a][b]++] = f;
             if (a != b) m.pair_list[b][a][m.pair_list_n[b][a]++] = f;
         }
-        // the sweep copies every image factor's spline to LDS at this stride
+        // The sweep evaluates every image factor's spline in monomial form
+        // from an LDS copy.  Panel i joins knots i and i+1, whose (f, f', f'')
+        // triples a and b (LongRangeSpline::scaled order) weight the quintic
+        // Hermite basis of spline_eval; expanded in powers of the panel
+        // coordinate x that basis gives the coefficients below.
         m.spl_stride = 0;
-        for (int f = 0; f < m.n_pair; ++f)
-            if (m.pair[f].rad.image) m.spl_stride = std::max(m.spl_stride, 3 * m.pair[f].rad.n_knots);
+        for (int f = 0; f < m.n_pair; ++f) {
+            DevRadial& R = m.pair[f].rad;
+            if (!R.image) continue;
+            for (int i = 0; i + 1 < R.n_knots; ++i) {
+                const double* a = R.spline + 3 * i;
+                const double* b = a + 3;
+                double* c = R.mono + 6 * i;
+                c[0] = a[0];
+                c[1] = a[1];
+                c[2] = 0.5 * a[2];
+                c[3] = -10.0 * a[0] - 6.0 * a[1] - 1.5 * a[2] + 10.0 * b[0] - 4.0 * b[1] + 0.5 * b[2];
+                c[4] = 15.0 * a[0] + 8.0 * a[1] + 1.5 * a[2] - 15.0 * b[0] + 7.0 * b[1] - b[2];
+                c[5] = -6.0 * a[0] - 3.0 * a[1] - 0.5 * a[2] + 6.0 * b[0] - 3.0 * b[1] + 0.5 * b[2];
+            }
+            m.spl_stride = std::max(m.spl_stride, 6 * (R.n_knots - 1));
+        }
         // the groups partition the particles (ParticleGroup ranges,
         // particle_group.h:21-27): the kernels walk pair partners group by group
         {

@@ -203,9 +203,35 @@ __device__ inline void spline_eval(const double* sc, int n_knots, double r_cut, 
           (a[0] * ddp0 + a[1] * ddp1 + a[2] * ddp2 + b[0] * ddq0 - b[1] * ddq1 + b[2] * ddq2);
 }
 
-// spline: the factor's image-sum spline (R.spline, or a copy of it in LDS)
+// 1/q from the hardware reciprocal and two Newton steps (error below 1 ulp),
+// in place of the IEEE division sequence.
+__device__ inline double rcp_refined(double q) {
+    double r = __builtin_amdgcn_rcp(q);
+    double e = __builtin_fma(-q, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-q, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// The spline of spline_eval as monomials per panel (DevRadial::mono): Horner
+// in the panel coordinate for the value and both derivatives.
+__device__ inline void spline_eval_mono(const double* mono, int n_knots, double r_cut, double inv_delta,
+                                        double r, double& f, double& df, double& d2f) {
+    if (r >= r_cut) { f = df = d2f = 0.0; return; }
+    const double t = r * inv_delta;
+    int i = static_cast<int>(t);
+    if (i > n_knots - 2) i = n_knots - 2;
+    const double x = t - i;
+    const double* c = mono + 6 * i;
+    f = ((((c[5] * x + c[4]) * x + c[3]) * x + c[2]) * x + c[1]) * x + c[0];
+    df = inv_delta * ((((5.0 * c[5] * x + 4.0 * c[4]) * x + 3.0 * c[3]) * x + 2.0 * c[2]) * x + c[1]);
+    d2f = inv_delta * inv_delta * (((20.0 * c[5] * x + 12.0 * c[4]) * x + 6.0 * c[3]) * x + 2.0 * c[2]);
+}
+
+// mono: an LDS copy of the factor's monomial spline (R.mono), or null for
+// the knot form in R.spline
 __device__ inline void radial_eval(const DevRadial& R, double r, double& u, double& du, double& d2u,
-                                   const double* spline) {
+                                   const double* mono) {
     if (R.kind == 2) {   // McMillan: u = -(b/r)^n
         const double x = R.c[0] / r;
         double xn = 1.0;
@@ -264,21 +290,22 @@ __device__ inline void radial_eval(const DevRadial& R, double r, double& u, doub
             r1 *= r;
         }
     }
-    const double iq = 1.0 / q;
+    const double iq = rcp_refined(q);
     u = p * iq;
     du = (dp * q - p * dq) * iq * iq;
     d2u = (d2p - (2.0 * dp * dq - 2.0 * p * dq * dq * iq + p * d2q) * iq) * iq;
     if (R.image) {
         if (r >= R.r_cut) { u = du = d2u = 0.0; return; }
         double f, df, d2f;
-        spline_eval(spline, R.n_knots, R.r_cut, R.inv_delta, r, f, df, d2f);
+        if (mono) spline_eval_mono(mono, R.n_knots, R.r_cut, R.inv_delta, r, f, df, d2f);
+        else spline_eval(R.spline, R.n_knots, R.r_cut, R.inv_delta, r, f, df, d2f);
         u -= R.u_inf + f;
         du -= df;
         d2u -= d2f;
     }
 }
 __device__ inline void radial_eval(const DevRadial& R, double r, double& u, double& du, double& d2u) {
-    radial_eval(R, r, u, du, d2u, R.spline);
+    radial_eval(R, r, u, du, d2u, nullptr);
 }
 
 // Pair contribution of a radial u to the gradient and per-dimension
@@ -477,7 +504,7 @@ struct Shared {
     double* u;     // [2*rows] column j of A (old)
     double* f;     // [4*rows]: u / R, then v again, both as C2<T> (determinant precision)
     double* rabs;  // [rows] row sums |D| of the block held in registers (register path)
-    double* spl;   // [n_pair][spl_stride] pair-factor splines (copies of M.pair[f].rad.spline)
+    double* spl;   // [n_pair][spl_stride] pair-factor splines (copies of M.pair[f].rad.mono)
     double* kj;    // [nkj][2] k-space Jastrow vectors, then kjw [ng][ng][nkj]
 };
 
@@ -603,7 +630,7 @@ struct Walker {
         for (int f = 0; f < M.n_pair; ++f) {
             const DevRadial& R = M.pair[f].rad;
             if (!R.image) continue;
-            for (int i = lane; i < 3 * R.n_knots; i += W) sh.spl[f * M.spl_stride + i] = R.spline[i];
+            for (int i = lane; i < 6 * (R.n_knots - 1); i += W) sh.spl[f * M.spl_stride + i] = R.mono[i];
         }
         for (int k = lane; k < M.nkj; k += W) {
             sh.kj[2 * k] = M.kjx[k];
@@ -654,7 +681,7 @@ struct Walker {
             for (int fi = 0; fi < nf; ++fi) {
                 const int f = M.pair_list[gp][h][fi];
                 const DevRadial& R = M.pair[f].rad;
-                const double* spl = sh.spl + f * M.spl_stride;   // LDS copy (image factors)
+                const double* spl = sh.spl + f * M.spl_stride;   // LDS monomial copy (image factors)
                 double uo, duo, d2uo, un, dun, d2un;
                 radial_eval(R, ro, uo, duo, d2uo, spl);
                 radial_eval(R, rn, un, dun, d2un, spl);

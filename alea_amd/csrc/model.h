@@ -45,6 +45,9 @@ struct DevRadial {
     int C = 3;
     double L = 0, alpha1 = 0;
     double spline[3 * kMaxKnots] = {};
+    // the same spline per panel as the monomial c0 + c1 x + ... + c5 x^5 in
+    // the panel coordinate x (filled at build), evaluated by Horner in the sweep
+    double mono[6 * (kMaxKnots - 1)] = {};
 };
 
 struct DevPair {
