@@ -267,7 +267,15 @@ __device__ inline void radial_eval(const DevRadial& R, double r, double& u, doub
         return;
     }
     double p = 0.0, dp = 0.0, d2p = 0.0;
-    {
+    double q = 1.0, dq = 0.0, d2q = 0.0;
+    if (R.num == 1 && R.den == 1) {
+        // u = a r / (1 + b r), the cusp Pade of every electron-gas input: the
+        // general loops below give the same values
+        p = R.c[0] * r;
+        dp = R.c[0];
+        q += R.c[1] * r;
+        dq = R.c[1];
+    } else {
         double r1 = 1.0, r2 = 0.0;
         for (int i = 0; i < R.num; ++i) {
             const int k = i + 1;
@@ -277,10 +285,8 @@ __device__ inline void radial_eval(const DevRadial& R, double r, double& u, doub
             r2 = r1;
             r1 *= r;
         }
-    }
-    double q = 1.0, dq = 0.0, d2q = 0.0;
-    {
-        double r1 = 1.0, r2 = 0.0;
+        r1 = 1.0;
+        r2 = 0.0;
         for (int i = 0; i < R.den; ++i) {
             const int k = i + 1;
             q += R.c[R.num + i] * r1 * r;
