@@ -233,16 +233,18 @@ struct alea_system {
         for (int f = 0; f < m.n_pair; ++f) {
             DevRadial& R = m.pair[f].rad;
             if (!R.image) continue;
-            for (int i = 0; i + 1 < R.n_knots; ++i) {
+            const int n_panels = R.n_knots - 1;   // coefficient k of panel i at k * n_panels + i
+            for (int i = 0; i < n_panels; ++i) {
                 const double* a = R.spline + 3 * i;
                 const double* b = a + 3;
-                double* c = R.mono + 6 * i;
+                double c[6];
                 c[0] = a[0];
                 c[1] = a[1];
                 c[2] = 0.5 * a[2];
                 c[3] = -10.0 * a[0] - 6.0 * a[1] - 1.5 * a[2] + 10.0 * b[0] - 4.0 * b[1] + 0.5 * b[2];
                 c[4] = 15.0 * a[0] + 8.0 * a[1] + 1.5 * a[2] - 15.0 * b[0] + 7.0 * b[1] - b[2];
                 c[5] = -6.0 * a[0] - 3.0 * a[1] - 0.5 * a[2] + 6.0 * b[0] - 3.0 * b[1] + 0.5 * b[2];
+                for (int k = 0; k < 6; ++k) R.mono[k * n_panels + i] = c[k];
             }
             m.spl_stride = std::max(m.spl_stride, 6 * (R.n_knots - 1));
         }

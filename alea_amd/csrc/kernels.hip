@@ -213,8 +213,9 @@ __device__ inline double rcp_refined(double q) {
     return __builtin_fma(r, e, r);
 }
 
-// The spline of spline_eval as monomials per panel (DevRadial::mono): Horner
-// in the panel coordinate for the value and both derivatives.
+// The spline of spline_eval as monomials per panel (DevRadial::mono, stored
+// coefficient-major): Horner in the panel coordinate for the value and both
+// derivatives.
 __device__ inline void spline_eval_mono(const double* mono, int n_knots, double r_cut, double inv_delta,
                                         double r, double& f, double& df, double& d2f) {
     if (r >= r_cut) { f = df = d2f = 0.0; return; }
@@ -222,10 +223,14 @@ __device__ inline void spline_eval_mono(const double* mono, int n_knots, double 
     int i = static_cast<int>(t);
     if (i > n_knots - 2) i = n_knots - 2;
     const double x = t - i;
-    const double* c = mono + 6 * i;
-    f = ((((c[5] * x + c[4]) * x + c[3]) * x + c[2]) * x + c[1]) * x + c[0];
-    df = inv_delta * ((((5.0 * c[5] * x + 4.0 * c[4]) * x + 3.0 * c[3]) * x + 2.0 * c[2]) * x + c[1]);
-    d2f = inv_delta * inv_delta * (((20.0 * c[5] * x + 12.0 * c[4]) * x + 6.0 * c[3]) * x + 2.0 * c[2]);
+    // coefficient-major: c_k of panel i at k * (n_knots - 1) + i, so lanes
+    // reading different panels touch different LDS banks
+    const int np_ = n_knots - 1;
+    const double c0 = mono[i], c1 = mono[np_ + i], c2 = mono[2 * np_ + i];
+    const double c3 = mono[3 * np_ + i], c4 = mono[4 * np_ + i], c5 = mono[5 * np_ + i];
+    f = ((((c5 * x + c4) * x + c3) * x + c2) * x + c1) * x + c0;
+    df = inv_delta * ((((5.0 * c5 * x + 4.0 * c4) * x + 3.0 * c3) * x + 2.0 * c2) * x + c1);
+    d2f = inv_delta * inv_delta * (((20.0 * c5 * x + 12.0 * c4) * x + 6.0 * c3) * x + 2.0 * c2);
 }
 
 // mono: an LDS copy of the factor's monomial spline (R.mono), or null for
