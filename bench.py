@@ -290,13 +290,18 @@ def main():
     b_move = bytes_per_move(4, mixed["acceptance"], spec)
     moves_per_launch = args.walkers * spec.n_particles
     achieved = b_move * moves_per_launch / (mixed["kernel_ms"] * 1e-3) / 1e9
-    traffic = None
+    traffic = traffic_fp64 = None
     tf = os.path.join(ROOT, "profiles", "sweep_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             t = json.load(f)
         if t.get("walkers") == args.walkers and t.get("precision") == "mixed":
             traffic = t.get("hbm_bytes_per_launch")
+            traffic_fp64 = t.get("fp64", {}).get("hbm_bytes_per_launch")
+    # the FP64 complex sweep streams the inverse through HBM on every accepted
+    # move: the survey's byte model is its roofline as it stands
+    b_move64 = bytes_per_move(8, fp64["acceptance"], spec)
+    achieved64 = b_move64 * moves_per_launch / (fp64["kernel_ms"] * 1e-3) / 1e9
     # the sweep holds the swept block in registers, so HBM does not bound it:
     # its VALU issue time (SQ_INSTS_VALU from rocprofv3, profiles/) against
     # the kernel time measured here
@@ -326,7 +331,10 @@ def main():
                    "step": "one sweep (every particle of every walker)", "tau": tau},
         "fp64": {"value": fp64["moves_per_s"], "ms_per_step": fp64["ms_per_step"],
                  "kernel_ms": fp64["kernel_ms"], "acceptance": fp64["acceptance"],
-                 "e_per_particle": fp64["e_per_particle"]},
+                 "e_per_particle": fp64["e_per_particle"],
+                 "roofline": {"bound": "hbm", "achieved": achieved64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": achieved64 / HBM_PEAK_GBS, "traffic": traffic_fp64,
+                              "kernel": "k_sweep<double,true,false,0>", "bytes_per_move": b_move64}},
         "mixed_over_fp64": mixed["moves_per_s"] / fp64["moves_per_s"],
         "acceptance": mixed["acceptance"],
         "e_per_particle": mixed["e_per_particle"],
@@ -335,9 +343,11 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_sweep<float,true,false,64>", "kernel_ms": mixed["kernel_ms"],
                      "bytes_per_move": b_move,
-                     "model": "achieved = the algorithmic bytes per move x moves / kernel time; with the "
-                              "swept block in registers the kernel moves `traffic` bytes and is bound by "
-                              "VALU issue and latency (`valu`)",
+                     "model": "achieved = the survey's algorithmic bytes per move (A read and written "
+                              "through HBM on every accepted move) x moves / kernel time.  This kernel "
+                              "keeps the swept block in registers, so those bytes never reach HBM "
+                              "(measured: `traffic` per launch) and frac can pass 1; its binding "
+                              "roofline is VALU issue (`valu.issue_frac`) and latency (`valu.wait_frac`)",
                      "valu": valu},
         "serving": None if args.no_serving else serving_leg(),
         "dmc": None if args.no_dmc else dmc_leg(),
