@@ -77,7 +77,31 @@ def cells():
     out.append((hexagonal_lattice(6, 2.0), [([1.0 / 3.0], [0.25]), ([1.0], [0.25])], None))
     # the reference's Madelung check cell (tests/test_ewald.cpp:54-59, tol 1e-8)
     out.append(([2.0, 0.0, 0.0, 2.0], [], (1e-8,)))
-    return out
+    return out + config_cells()
+
+
+def config_cells():
+    """The periodic cells of the reference configs in
+    tests/golden/build_characterization.json, with the Padé image sums and
+    the Ewald tolerance their builders ask for (alea_amd.config)."""
+    from alea_amd import config  # noqa: E402
+    path = os.path.join(ROOT, "tests", "golden", "build_characterization.json")
+    if not os.path.exists(path):
+        return []
+    with open(path, encoding="utf-8") as f:
+        cases = json.load(f)["cases"]
+    by_cell = {}
+    for case in cases.values():
+        req = []
+        config.build_spec(case["config"], requests=req)
+        for r in req:
+            lat = r[1]
+            entry = by_cell.setdefault(cell_key(lat), [lat, [], None])
+            if r[0] == "pade" and (r[2], r[3]) not in entry[1]:
+                entry[1].append((r[2], r[3]))
+            elif r[0] == "ewald":
+                entry[2] = (r[2],)
+    return [tuple(e) for e in by_cell.values()]
 
 
 def main():

@@ -15,12 +15,16 @@
 //                               sources compiled here) and the reference's
 //                               golden vectors (xoshiro state {1,2,3,4},
 //                               Madelung 1.6155426267, exciton E = -1)
-//   Slater real/complex     -> closed forms (N <= 2 determinants, plane-wave
-//                               eigenstates: E_L = sum |k|^2/2 exactly),
-//                               brute-force permutation-sum determinants,
-//                               finite differences; the reference's own
-//                               Slater files need Eigen (absent), so they are
-//                               not built: "parity pinned by closed forms".
+//   Slater real/complex     -> the reference's characterization record
+//                               (tests/golden/build_characterization.txt:
+//                               log|psi| and E_L of four walkers of each of
+//                               its configs the path holds), closed forms
+//                               (N <= 2 determinants, plane-wave eigenstates:
+//                               E_L = sum |k|^2/2 exactly), brute-force
+//                               permutation-sum determinants, finite
+//                               differences; the reference's own Slater files
+//                               need Eigen (absent), so they are not built.
+//   initial walkers         -> the same record; libstdc++'s own engine.
 //   Eigen stand-in          -> the initial inverse uses the reference's own
 //                               pivoted Gauss-Jordan (slater_determinant.cpp:
 //                               905-993) in place of Eigen's D.inverse().
@@ -46,6 +50,7 @@
 #include <limits>
 #include <memory>
 #include <numbers>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1706,6 +1711,54 @@ int oracle_sweep_tape(uint64_t seed, int np, int dim, int nw, int n_sweeps, doub
     Rng r(seed);
     const std::size_t len = static_cast<std::size_t>(np) * (dim + 1) * nw;
     for (int s = 0; s < n_sweeps; ++s) draw_sweep(r, np, dim, nw, 0, nw, out + s * len);
+    return 0;
+}
+
+// The reference's initial walkers (src/qmc/app.cpp:56-116, numerics::Random
+// of src/numerics/random.h:11-27): one std::mt19937_64 seeded with `seed`,
+// walker by walker and particle by particle; a periodic cell draws fractional
+// coordinates f_i = uniform(-1/2, 1/2) and maps them through the cell's
+// vectors() (pos_d = sum_i f_i a[i*2+d]): the basis after axis alignment and
+// Gauss reduction, which for a hexagonal cell (on the reduction's tie) depends
+// on rounding.  An open cell draws each coordinate in
+// [-L/2, L/2); particles flagged in at_origin (may be null) sit at the origin
+// and draw nothing.  The same libstdc++ engine and distribution as the
+// reference's build.  Output AoS [w][p][d].  Crystal anchors and pairing
+// partners (which also draw normals) are not restated.
+int oracle_initial_positions(uint64_t seed, int np, int nw, const double* lattice, double L,
+                             const unsigned char* at_origin, double* out) {
+    std::mt19937_64 engine(seed);
+    std::uniform_real_distribution<double> uni(0.0, 1.0);
+    auto uniform = [&](double lo, double hi) { return lo + (hi - lo) * uni(engine); };
+    Cell cell;
+    if (lattice) {
+        cell.init(2, lattice);
+        lattice = cell.a;
+    }
+    for (int w = 0; w < nw; ++w)
+        for (int p = 0; p < np; ++p) {
+            double* x = out + (static_cast<std::size_t>(w) * np + p) * 2;
+            if (at_origin && at_origin[p]) {
+                x[0] = x[1] = 0.0;
+            } else if (lattice) {
+                double f[2];
+                for (int i = 0; i < 2; ++i) f[i] = uniform(-0.5, 0.5);
+                for (int d = 0; d < 2; ++d) {
+                    double s = 0.0;
+                    for (int i = 0; i < 2; ++i) s += f[i] * lattice[i * 2 + d];
+                    x[d] = s;
+                }
+            } else {
+                for (int d = 0; d < 2; ++d) x[d] = uniform(-0.5 * L, 0.5 * L);
+            }
+        }
+    return 0;
+}
+
+// The 64-bit Mersenne Twister itself, n outputs from `seed`.
+int oracle_mt19937_64(uint64_t seed, int n, uint64_t* out) {
+    std::mt19937_64 engine(seed);
+    for (int i = 0; i < n; ++i) out[i] = engine();
     return 0;
 }
 

@@ -28,6 +28,9 @@ int oracle_xoshiro_state_u64(const uint64_t* state, int n, uint64_t* out);
 int oracle_xoshiro_draws(uint64_t seed, int kind, int n, double* out);
 int oracle_sweep_tape(uint64_t seed, int np, int dim, int nw, int n_sweeps, double* out);
 int oracle_ziggurat_tables(uint64_t* k, double* w, double* f, double* r);
+int oracle_initial_positions(uint64_t seed, int np, int nw, const double* lattice, double L,
+                             const unsigned char* at_origin, double* out);
+int oracle_mt19937_64(uint64_t seed, int n, uint64_t* out);
 
 int oracle_create(oracle_system** out, int nw, int np, int dim, const double* lattice,
                   int precision);
