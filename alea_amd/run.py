@@ -1,0 +1,186 @@
+"""End-to-end runs of a reference config on the GPU path: the `qmc:` stages in
+order, as the reference's main loop runs them (main.cpp:62-90), each stage
+starting where the previous one ended (carry_over_walkers,
+src/qmc/app.cpp:1066-1098).
+
+  VMC stage   VMC::run, src/qmc/vmc.cpp:157-420: equilibration blocks, then
+              accumulation blocks of steps_per_block steps; a step is
+              sub_steps Metropolis sweeps and one measurement of E_L over the
+              walkers; a block's energy is its steps' walker mean
+  DMC stage   DMC::run, src/qmc/dmc.cpp:736-830 through alea_amd.dmc.DMC
+              (run_one_step): equilibration blocks refresh the energy spread,
+              accumulation blocks give the weighted-mean block energies
+  errors      numerics::reblock (alea_amd.statistics) over the block energies
+
+The walkers draw from the device streams (one xoshiro256++ stream per
+walker, seeded from the stage seed), so a run reproduces the reference's
+statistics, not its trajectories (those are replayed through tapes in the
+tests).  Run settings the GPU path does not hold (langevin VMC moves,
+non-bare reweighting, forward walking, estimators, walker files, optimizer
+and PIMC stages) are refused with a ValueError.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import config, dmc, statistics
+from .walkers import FP64, gpu_walkers, seed as seed_walkers
+
+
+def _refuse(what: str):
+    raise ValueError(f"run: {what} is not on the GPU path")
+
+
+def _rows(pos_soa: np.ndarray, np_: int, nw: int) -> np.ndarray:
+    """SoA [p*2*nw + d*nw + w] -> rows [w][p*2 + d]."""
+    return pos_soa.reshape(np_, 2, nw).transpose(2, 0, 1).reshape(nw, np_ * 2)
+
+
+def _soa(rows: np.ndarray, np_: int) -> np.ndarray:
+    nw = rows.shape[0]
+    return np.ascontiguousarray(rows.reshape(nw, np_, 2).transpose(1, 2, 0).reshape(-1))
+
+
+def _seed_rows(rows: np.ndarray, nw: int) -> np.ndarray:
+    """seed_walker_rows: walker w takes row w mod n_rows (app.cpp:1066-1075)."""
+    return rows[np.arange(nw) % rows.shape[0]]
+
+
+def _check_stage(st: dict):
+    """Every refusal of a stage, before any stage runs."""
+    method = st.get("method")
+    if method not in ("vmc", "dmc"):
+        _refuse(f"method '{method}'")
+    for key in ("estimators", "walkers_in", "walkers_out", "forward_walking", "paths_in", "paths_out"):
+        if st.get(key):
+            _refuse(f"`{key}` of a {method} stage")
+    for key in ("samples", "accumulation_blocks", "steps_per_block", "sub_steps", "time_step"):
+        if not st.get(key):
+            raise ValueError(f"run: a {method} stage needs `{key}` > 0")
+    walk = st.get("random_walk") or ("metropolis" if method == "vmc" else "drift_diffusion")
+    if walk != ("metropolis" if method == "vmc" else "drift_diffusion"):
+        _refuse(f"random_walk '{walk}' in {method.upper()}")
+    if method == "dmc" and st.get("reweighting", "bare") != "bare":
+        _refuse(f"reweighting '{st.get('reweighting')}'")
+
+
+def run_vmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, nw=None) -> dict:
+    _check_stage(st)
+    nw = nw or int(st["samples"])
+    n = spec.n_particles
+    w = gpu_walkers(spec, nw, precision)
+    try:
+        w.set_positions(_soa(_seed_rows(rows, nw), n))
+        w.evaluate()
+        seed_walkers(w, stage_seed)
+        tau = float(st["time_step"])
+        n_sub, n_steps = int(st["sub_steps"]), int(st["steps_per_block"])
+        for _ in range(int(st.get("equilibration_blocks", 0))):
+            for _ in range(n_steps):
+                w.vmc_sweeps(n_sub, tau)
+        blocks, blocks_im = [], []
+        e_sum = e2_sum = 0.0
+        n_tot = acc = prop = 0
+        for _ in range(int(st["accumulation_blocks"])):
+            b_sum = b_im = 0.0
+            b_n = 0
+            for _ in range(n_steps):
+                acc += w.vmc_sweeps(n_sub, tau)
+                prop += n_sub * n * nw
+                e, e_im = w.local_energy()
+                b_sum += float(np.sum(e))
+                b_im += float(np.sum(e_im))
+                b_n += nw
+                e_sum += float(np.sum(e))
+                e2_sum += float(np.sum(e * e))
+                n_tot += nw
+            blocks.append(b_sum / b_n)
+            blocks_im.append(b_im / b_n)
+        rb = statistics.reblock(blocks)
+        mean = e_sum / n_tot
+        out = dict(method="vmc", walkers=nw, energy=rb.mean, error=rb.error, tau_int=rb.tau_int,
+                   variance=e2_sum / n_tot - mean * mean, acceptance=acc / prop,
+                   block_energies=blocks)
+        if spec.slater_twisted:
+            ri = statistics.reblock(blocks_im)
+            out.update(energy_imag=ri.mean, energy_imag_error=ri.error)
+        out["final_rows"] = _rows(w.positions(), n, nw)
+        return out
+    finally:
+        w.close()
+
+
+def run_dmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, target=None) -> dict:
+    _check_stage(st)
+    target = target or int(st["samples"])
+    n = spec.n_particles
+    cap = 2 * target + 1
+    w = gpu_walkers(spec, cap, precision)
+    try:
+        w.set_positions(_soa(_seed_rows(rows, cap), n))
+        w.evaluate()
+        w.permute(np.arange(target))
+        seed_walkers(w, stage_seed, dmc.K_DMC_THREAD_STREAM)
+        cutoff = st.get("branch_cutoff", "universal")
+        alpha = st.get("branch_cutoff_alpha")
+        d = dmc.DMC(w, float(st["time_step"]), target, n_substeps=int(st["sub_steps"]),
+                    generations=float(st.get("population_generations", 10.0)), cutoff=cutoff,
+                    alpha=dmc.K_DEFAULT_BRANCH_CUTOFF_ALPHA if alpha is None else float(alpha),
+                    seed=stage_seed)
+        n_steps = int(st["steps_per_block"])
+        eq_blocks = []
+        for _ in range(int(st.get("equilibration_blocks", 0))):
+            be = bw = 0.0
+            for _ in range(n_steps):
+                e, ws = d.step(measure=False)
+                be += e
+                bw += ws
+            eq_blocks.append(be / bw if bw > 0.0 else 0.0)
+            d.refresh_energy_spread()
+        d.pc.begin_accumulation()
+        blocks, pops = [], []
+        acc = att = 0
+        for _ in range(int(st["accumulation_blocks"])):
+            be = bw = 0.0
+            for _ in range(n_steps):
+                att += w.nw * n * int(st["sub_steps"])
+                e, ws = d.step(measure=True)
+                acc += d.history[-1]["n_accepted"]
+                be += e
+                bw += ws
+            blocks.append(be / bw if bw > 0.0 else 0.0)
+            pops.append(w.nw)
+        rb = statistics.reblock(blocks)
+        out = dict(method="dmc", target=target, energy=rb.mean, error=rb.error, tau_int=rb.tau_int,
+                   acceptance=acc / att if att else 0.0, mean_population=float(np.mean(pops)),
+                   e_trial=d.pc.etrial, block_energies=blocks, equilibration_block_energies=eq_blocks)
+        out["final_rows"] = _rows(w.positions(), n, w.nw)
+        return out
+    finally:
+        w.close()
+
+
+def run(cfg, precision=FP64, seed=None, walkers=None) -> list[dict]:
+    """Every `qmc:` stage of a config, in order; `walkers` overrides each
+    stage's `samples` (VMC walkers, DMC target)."""
+    spec = config.build_spec(cfg)
+    stages = list(cfg.get("qmc", []) or [])
+    if any(st.get("method") in ("optimizer", "pimc") for st in stages):
+        _refuse("an optimizer or PIMC stage")
+    for st in stages:
+        _check_stage(st)
+    base = int(seed if seed is not None else (cfg.get("project", {}) or {}).get("seed", 0))
+    rows, results = None, []
+    for i, st in enumerate(stages):
+        stage_seed = base + i + 1           # app.cpp:815-817, no optimizer or PIMC stage
+        nw = walkers or int(st["samples"])
+        if rows is None:
+            rows = _rows(config.initial_positions(cfg, nw, stage_seed), spec.n_particles, nw)
+        if st.get("method") == "vmc":
+            res = run_vmc(spec, st, rows, stage_seed, precision, nw)
+        else:
+            res = run_dmc(spec, st, rows, stage_seed, precision, nw)
+        rows = res.pop("final_rows")
+        res["stage"] = i
+        results.append(res)
+    return results

@@ -1,0 +1,89 @@
+"""End-to-end config runs (alea_amd.run): the stage loop, the carry-over of
+walkers between stages and the refusals on the CPU; on the GPU, the
+reference's exact-energy checks run as whole configs (tests/test_vmc_physics.cpp:
+86 for the 2-D exciton, E_L = -1 with zero variance; the plane-wave
+eigenstate of tests/configs/periodic_2d_pw_free.yaml, E_L = 8 pi sqrt(3) /
+(N rs^2) = 6.218741 at N = 7, rs = 1)."""
+import math
+
+import numpy as np
+import pytest
+
+from alea_amd import run
+
+EXCITON = {
+    "project": {"name": "exciton_2d_cusp_m1", "seed": 11},
+    "system": {"lattice": {"type": "open", "dimension": 2, "L": 1.0},
+               "particles": [{"name": "e", "group": "electron_group", "type": "fermions", "mass": 1.0,
+                              "charge": -1, "up": 1, "down": 0, "initialization": "uniform"},
+                             {"name": "h", "group": "hole_group", "type": "fermions", "mass": 1.0,
+                              "charge": 1, "up": 1, "down": 0, "initialization": "uniform"}]},
+    "wavefunction": [{"jastrow": {"type": "padé", "pair": ["electron_group", "hole_group"]}}],
+    "hamiltonian": {"terms": [{"type": "coulomb", "pair": ["electron_group", "hole_group"], "epsilon": 1.0}]},
+    "qmc": [{"method": "vmc", "random_walk": "metropolis", "samples": 256, "equilibration_blocks": 2,
+             "accumulation_blocks": 8, "steps_per_block": 5, "sub_steps": 2, "time_step": 0.5},
+            {"method": "dmc", "random_walk": "drift_diffusion", "samples": 256, "equilibration_blocks": 2,
+             "accumulation_blocks": 8, "steps_per_block": 5, "sub_steps": 2, "time_step": 0.02}],
+}
+
+PW_FREE = {
+    "project": {"name": "periodic_2d_pw_free", "seed": 42},
+    "system": {"lattice": {"type": "periodic", "dimension": 2, "shape": "hexagonal", "rs": 1.0},
+               "particles": [{"name": "e", "group": "electron_group", "type": "fermions", "mass": 1.0,
+                              "charge": -1, "up": 7, "down": 0, "initialization": "uniform"}]},
+    "wavefunction": [{"slater": {"group": "electron_group", "orbitals": {"type": "plane_wave"}}}],
+    "hamiltonian": {"terms": []},
+    "qmc": [{"method": "vmc", "random_walk": "metropolis", "samples": 64, "equilibration_blocks": 4,
+             "accumulation_blocks": 8, "steps_per_block": 10, "sub_steps": 2, "time_step": 0.5}],
+}
+
+
+def test_rows_round_trip_and_seeding():
+    n, nw = 3, 5
+    soa = np.arange(n * 2 * nw, dtype=float)
+    rows = run._rows(soa, n, nw)
+    assert rows.shape == (nw, n * 2)
+    assert rows[2, 1 * 2 + 1] == soa[1 * 2 * nw + 1 * nw + 2]
+    assert np.array_equal(run._soa(rows, n), soa)
+    # seed_walker_rows: walker w takes row w mod n_rows
+    seeded = run._seed_rows(rows, 12)
+    assert np.array_equal(seeded[7], rows[2]) and np.array_equal(seeded[11], rows[1])
+
+
+@pytest.mark.parametrize("edit,msg", [
+    (lambda c: c["qmc"][0].update(random_walk="langevin"), "random_walk"),
+    (lambda c: c["qmc"][1].update(reweighting="suppressed"), "reweighting"),
+    (lambda c: c["qmc"][0].update(estimators=[{"type": "gofr"}]), "estimators"),
+    (lambda c: c["qmc"].insert(0, {"method": "optimizer"}), "optimizer"),
+    (lambda c: c["qmc"][0].update(time_step=0), "time_step"),
+])
+def test_refusals(edit, msg, monkeypatch):
+    import copy
+    cfg = copy.deepcopy(EXCITON)
+    edit(cfg)
+    # nothing reaches the device: the walker factory fails the test if called
+    monkeypatch.setattr(run, "gpu_walkers", lambda *a, **k: pytest.fail("reached the device"))
+    with pytest.raises(ValueError, match=msg):
+        run.run(cfg)
+
+
+@pytest.mark.gpu
+def test_exciton_config_runs_exactly():
+    print("run: exciton VMC then DMC", flush=True)
+    vmc, dmc_ = run.run(EXCITON)
+    assert vmc["method"] == "vmc" and dmc_["method"] == "dmc"
+    assert all(abs(e + 1.0) < 1e-12 for e in vmc["block_energies"])
+    assert abs(vmc["variance"]) < 1e-12
+    assert 0.0 < vmc["acceptance"] < 1.0
+    assert all(abs(e + 1.0) < 1e-12 for e in dmc_["block_energies"])
+    assert abs(dmc_["e_trial"] + 1.0) < 1e-6
+    assert 0.9 < dmc_["acceptance"] <= 1.0
+
+
+@pytest.mark.gpu
+def test_plane_wave_eigenstate_config_runs_exactly():
+    print("run: periodic plane-wave VMC", flush=True)
+    (vmc,) = run.run(PW_FREE)
+    exact = 8.0 * math.pi * math.sqrt(3.0) / 7.0
+    assert abs(vmc["energy"] - exact) < 1e-9 * exact
+    assert abs(vmc["variance"]) < 1e-9
