@@ -42,6 +42,10 @@ void launch_dmc_weights(const double* e_new, double* e_old, double* weight, long
 void launch_branch(const double* weight, const double* xi, int* src, int src_cap, int nw,
                    long long* total, hipStream_t st);
 void launch_fill(double* a, int n, double v, hipStream_t st);
+void launch_estimators(const DevModel* m, const double* pos, const double* weights, int nw, int chunk,
+                       bool gofr, double rmax, double inv_delta, int n_bins, int nfb, bool sofk,
+                       const double* kvec, int nk, double* part, int part_stride, int n_entries, double* acc,
+                       hipStream_t st);
 }  // namespace alea_amd
 
 using namespace alea_amd;
@@ -196,6 +200,21 @@ struct alea_system {
     int rows_max = 0;             // largest padded block row count (0: no determinant)
     int refresh_every = 2;        // sweeps between batched inverse refreshes
     long long since_refresh = 0;  // sweeps (or np accept calls) since the last refresh
+
+    // Estimators (pair correlation, structure factor): the block's running
+    // sums on the device, [gofr: n_func*n_bins][sofk: n_pairs*nk][rho re:
+    // G*nk][rho im: G*nk][weight sum]
+    struct Estimators {
+        bool gofr = false, sofk = false;
+        double rmax = 0.0, inv_delta = 0.0;
+        int n_bins = 0, nfb = 0, nk = 0;
+        std::vector<double> kx, ky;
+        std::vector<int> shell;
+        DevBuf<double> acc, part, kvec;
+        int entries(int ng) const {
+            return nfb + (sofk ? (ng * (ng + 1) / 2 + 2 * ng) * nk : 0);
+        }
+    } est;
 
     bool mixed() const { return precision == ALEA_MIXED; }
     size_t tsize() const { return mixed() ? 4 : 8; }
@@ -1417,6 +1436,150 @@ int alea_branch(alea_system* s, uint64_t* rng_state, int target, int* new_nw, in
         s->sync();
         *new_nw = nn;
         if (src_out) std::copy(srci.begin(), srci.end(), src_out);
+    });
+}
+
+}  // extern "C"
+
+// ── Estimators ──────────────────────────────────────────────────────────────
+namespace {
+constexpr int kEstMaxChunks = 2048;
+
+void est_alloc(alea_system* s) {
+    auto& e = s->est;
+    const int n = e.entries(s->host.ng);
+    e.acc.alloc(static_cast<size_t>(n) + 1, s->stream);
+    e.part.alloc(static_cast<size_t>(std::min(s->capacity, kEstMaxChunks)) * std::max(n, 1), s->stream);
+}
+}  // namespace
+
+extern "C" {
+
+int alea_add_gofr(alea_system* s, double rmax, int n_bins) {
+    return guarded([&] {
+        Sys(s);
+        const int G = s->host.ng;
+        if (G <= 0) throw std::invalid_argument("alea_add_gofr: add the particle groups first");
+        if (!(rmax > 0.0) || n_bins <= 0) throw std::invalid_argument("alea_add_gofr: needs rmax > 0 and n_bins > 0");
+        if (G * (G + 1) / 2 * n_bins > kMaxGofrEntries)
+            throw std::invalid_argument("alea_add_gofr: group pairs x bins exceeds 4096");
+        if (s->host.cell.periodic && rmax > s->host.cell.ws)
+            throw std::invalid_argument("alea_add_gofr: rmax exceeds the cell's Wigner-Seitz inradius");
+        auto& e = s->est;
+        e.gofr = true;
+        e.rmax = rmax;
+        e.n_bins = n_bins;
+        e.inv_delta = static_cast<double>(n_bins) / rmax;
+        e.nfb = G * (G + 1) / 2 * n_bins;
+        est_alloc(s);
+    });
+}
+
+// The half-space reciprocal vectors of the reduced basis with |k| <= kmax,
+// sorted by |k|, in shells of equal |k| (structure_factor.cpp:16-54).
+int alea_add_sofk(alea_system* s, double kmax, int* n_k) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->host.cell.periodic) throw std::invalid_argument("alea_add_sofk: needs a periodic cell");
+        if (s->host.ng <= 0) throw std::invalid_argument("alea_add_sofk: add the particle groups first");
+        const double* b = s->host.cell.b;
+        const double b1 = std::hypot(b[0], b[1]), b2 = std::hypot(b[2], b[3]);
+        const int nmax = static_cast<int>(std::ceil(kmax / std::min(b1, b2))) + 1;
+        struct K { double kx, ky, k; };
+        std::vector<K> ks;
+        for (int i = 0; i <= nmax; ++i)
+            for (int j = -nmax; j <= nmax; ++j) {
+                if (i == 0 && j <= 0) continue;
+                const double kx = i * b[0] + j * b[2], ky = i * b[1] + j * b[3];
+                const double k = std::hypot(kx, ky);
+                if (k <= kmax) ks.push_back({kx, ky, k});
+            }
+        if (ks.empty()) throw std::invalid_argument("alea_add_sofk: kmax is below the shortest reciprocal vector");
+        if (static_cast<int>(ks.size()) > kMaxSofkVectors)
+            throw std::invalid_argument("alea_add_sofk: more than 1024 vectors below kmax");
+        std::sort(ks.begin(), ks.end(), [](const K& x, const K& y) { return x.k < y.k; });
+        auto& e = s->est;
+        e.sofk = true;
+        e.kx.clear();
+        e.ky.clear();
+        e.shell.clear();
+        const double tol = 1e-9 * std::min(b1, b2);
+        double last = 0.0;
+        int shell = -1;
+        for (const K& q : ks) {
+            if (shell < 0 || q.k - last > tol) {
+                ++shell;
+                last = q.k;
+            }
+            e.kx.push_back(q.kx);
+            e.ky.push_back(q.ky);
+            e.shell.push_back(shell);
+        }
+        e.nk = static_cast<int>(ks.size());
+        std::vector<double> kv(e.kx);
+        kv.insert(kv.end(), e.ky.begin(), e.ky.end());
+        e.kvec.alloc(kv.size(), s->stream);
+        hip_check(copy_on(s->stream, e.kvec.p, kv.data(), kv.size() * sizeof(double), hipMemcpyHostToDevice),
+                  "upload sofk vectors");
+        est_alloc(s);
+        *n_k = e.nk;
+    });
+}
+
+int alea_get_sofk_vectors(alea_system* s, double* kx, double* ky, int* shell) {
+    return guarded([&] {
+        Sys(s);
+        const auto& e = s->est;
+        if (!e.sofk) throw std::invalid_argument("alea_get_sofk_vectors: no structure factor");
+        std::copy(e.kx.begin(), e.kx.end(), kx);
+        std::copy(e.ky.begin(), e.ky.end(), ky);
+        std::copy(e.shell.begin(), e.shell.end(), shell);
+    });
+}
+
+int alea_estimators_start_block(alea_system* s) {
+    return guarded([&] {
+        Sys(s);
+        auto& e = s->est;
+        if (e.acc.p) launch_fill(e.acc.p, static_cast<int>(e.acc.n), 0.0, s->stream);
+    });
+}
+
+int alea_estimators_accumulate(alea_system* s, int use_weights) {
+    return guarded([&] {
+        Sys(s);
+        s->build();
+        auto& e = s->est;
+        if (!e.gofr && !e.sofk) throw std::invalid_argument("alea_estimators_accumulate: no estimators");
+        if (s->nw <= 0) return;
+        const int chunk = (s->nw + kEstMaxChunks - 1) / kEstMaxChunks;
+        const int n = e.entries(s->host.ng);
+        launch_estimators(s->dmodel, s->pos.p, use_weights ? s->weights.p : nullptr, s->nw, chunk, e.gofr,
+                          e.rmax, e.inv_delta, e.n_bins, e.nfb, e.sofk, e.kvec.p, e.nk, e.part.p, n, n,
+                          e.acc.p, s->stream);
+        hip_check(hipGetLastError(), "estimator launch");
+    });
+}
+
+int alea_estimators_read(alea_system* s, double* gofr, double* sofk, double* rho_re, double* rho_im,
+                         double* wsum) {
+    return guarded([&] {
+        Sys(s);
+        auto& e = s->est;
+        const int n = e.entries(s->host.ng);
+        std::vector<double> h(static_cast<size_t>(n) + 1, 0.0);
+        if (e.acc.p) {
+            hip_check(copy_on(s->stream, h.data(), e.acc.p, h.size() * sizeof(double), hipMemcpyDeviceToHost),
+                      "read estimators");
+            hip_check(hipStreamSynchronize(s->stream), "read estimators");
+        }
+        const int G = s->host.ng, npg = G * (G + 1) / 2;
+        if (gofr) std::copy(h.begin(), h.begin() + e.nfb, gofr);
+        const double* base = h.data() + e.nfb;
+        if (sofk && e.sofk) std::copy(base, base + npg * e.nk, sofk);
+        if (rho_re && e.sofk) std::copy(base + npg * e.nk, base + (npg + G) * e.nk, rho_re);
+        if (rho_im && e.sofk) std::copy(base + (npg + G) * e.nk, base + (npg + 2 * G) * e.nk, rho_im);
+        if (wsum) wsum[0] = wsum[1] = h[n];
     });
 }
 

@@ -2382,4 +2382,201 @@ void launch_fill(double* a, int n, double v, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, st, a, n, v);
 }
 
+// ── Estimators: pair correlation and structure factor ─────────────────────
+// One measurement of every walker: each kernel processes a contiguous chunk
+// of walkers per workgroup, in walker order, and writes the chunk's weighted
+// sums to its own row of `part`; k_est_reduce then adds the rows in chunk
+// order to the block's running sums.  The result does not depend on
+// scheduling: the same walkers give the same bits.
+namespace {
+
+constexpr int kEstThreads = 256;
+constexpr int kGofrPerThread = kMaxGofrEntries / kEstThreads;
+
+// PairCorrelation::accumulate (pair_correlation.cpp:74-105): every pair i < j
+// of a walker, the minimum-image distance of r_i - r_j as the distance table
+// holds it (uncontracted r^2, correctly rounded sqrt), binned by
+// int(r / delta) below rmax into function pid(g_i, g_j).  A walker's counts
+// are exact integers in LDS; they join the thread's partial sums times the
+// walker's weight.
+__global__ __launch_bounds__(kEstThreads) void k_gofr(const DevModel* __restrict__ Mp, const double* __restrict__ pos,
+                                                      const double* __restrict__ weights, int nw, int chunk,
+                                                      double rmax, double inv_delta, int n_bins,
+                                                      double* __restrict__ part, int part_stride) {
+    const DevModel& M = *Mp;
+    __shared__ int cnt[kMaxGofrEntries];
+    __shared__ double sp[2 * kMaxParticles];
+    __shared__ int gof[kMaxParticles];
+    const int np = M.np, G = M.ng, nfb = G * (G + 1) / 2 * n_bins, tid = threadIdx.x;
+    for (int k = tid; k < nfb; k += kEstThreads) cnt[k] = 0;
+    for (int p = tid; p < np; p += kEstThreads) gof[p] = M.group_of[p];
+    double acc[kGofrPerThread];
+#pragma unroll
+    for (int q = 0; q < kGofrPerThread; ++q) acc[q] = 0.0;
+    const int w0 = blockIdx.x * chunk, w1 = min(nw, w0 + chunk);
+    const int npairs = np * (np - 1) / 2, two_np = 2 * np;
+    for (int w = w0; w < w1; ++w) {
+        __syncthreads();   // the previous walker's positions and counts are done with
+        for (int k = tid; k < 2 * np; k += kEstThreads) sp[k] = pos[static_cast<size_t>(w) * np * 2 + k];
+        __syncthreads();
+        // thread tid takes the contiguous pairs [t0, t1) of the upper
+        // triangle in row order: row i holds start(i) .. start(i+1)-1 with
+        // start(i) = i (2 np - i - 1) / 2; the first pair is located once,
+        // the rest by stepping along the row
+        const int per = (npairs + kEstThreads - 1) / kEstThreads;
+        const int t0 = min(npairs, tid * per), t1 = min(npairs, t0 + per);
+        int i = 0, j = 1;
+        if (t0 < t1) {
+            const double disc = static_cast<double>(two_np - 1) * (two_np - 1) - 8.0 * t0;
+            i = static_cast<int>(0.5 * ((two_np - 1) - sqrt(fmax(disc, 0.0))));
+            i = min(max(i, 0), np - 2);
+            while (i > 0 && i * (two_np - i - 1) / 2 > t0) --i;
+            while ((i + 1) * (two_np - i - 2) / 2 <= t0) ++i;
+            j = t0 - i * (two_np - i - 1) / 2 + i + 1;
+        }
+        for (int t = t0; t < t1; ++t) {
+            double d0 = sp[2 * i] - sp[2 * j], d1 = sp[2 * i + 1] - sp[2 * j + 1];
+            min_image(M.cell, d0, d1);
+            const double r = sqrt(__dadd_rn(__dmul_rn(d0, d0), __dmul_rn(d1, d1)));
+            if (r < rmax) {
+                const int bin = static_cast<int>(r * inv_delta);
+                if (bin < n_bins) {
+                    int a = gof[i], b = gof[j];
+                    if (b < a) { const int x = a; a = b; b = x; }
+                    atomicAdd(&cnt[(a * G - a * (a - 1) / 2 + (b - a)) * n_bins + bin], 1);
+                }
+            }
+            if (++j == np) {
+                ++i;
+                j = i + 1;
+            }
+        }
+        __syncthreads();
+        const double wt = weights ? weights[w] : 1.0;
+#pragma unroll
+        for (int q = 0; q < kGofrPerThread; ++q) {
+            const int k = tid + q * kEstThreads;
+            if (k < nfb) {
+                acc[q] += wt * static_cast<double>(cnt[k]);
+                cnt[k] = 0;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kGofrPerThread; ++q) {
+        const int k = tid + q * kEstThreads;
+        if (k < nfb) part[static_cast<size_t>(blockIdx.x) * part_stride + k] = acc[q];
+    }
+}
+
+// StructureFactor::accumulate (structure_factor.cpp:62-105): per walker and
+// group, rho_g(k) = sum_i e^{i k.r_i}; the chunk sums w rho_g and
+// w Re(rho_a rho_b^*) per group pair a <= b.  One wave per chunk and tile of
+// 64 vectors, lanes over k.
+__global__ __launch_bounds__(64) void k_sofk(const DevModel* __restrict__ Mp, const double* __restrict__ pos,
+                                             const double* __restrict__ weights, int nw, int chunk,
+                                             const double* __restrict__ kvec, int nk, double* __restrict__ part,
+                                             int part_stride, int off) {
+    const DevModel& M = *Mp;
+    const int np = M.np, G = M.ng, lane = threadIdx.x;
+    const int w0 = blockIdx.x * chunk, w1 = min(nw, w0 + chunk);
+    constexpr int kPairs = kMaxGroups * (kMaxGroups + 1) / 2;
+    double* row = part + static_cast<size_t>(blockIdx.x) * part_stride + off;
+    const int npg = G * (G + 1) / 2;
+    {   // one wave per (chunk, tile of 64 vectors)
+        const int k = blockIdx.y * 64 + lane;
+        const double kx = k < nk ? kvec[k] : 0.0, ky = k < nk ? kvec[nk + k] : 0.0;
+        double as[kPairs], ar[kMaxGroups], ai[kMaxGroups];
+#pragma unroll
+        for (int q = 0; q < kPairs; ++q) as[q] = 0.0;
+#pragma unroll
+        for (int g = 0; g < kMaxGroups; ++g) ar[g] = ai[g] = 0.0;
+        for (int w = w0; w < w1; ++w) {
+            const double wt = weights ? weights[w] : 1.0;
+            const double* pw = pos + static_cast<size_t>(w) * np * 2;
+            double re[kMaxGroups], im[kMaxGroups];
+#pragma unroll
+            for (int g = 0; g < kMaxGroups; ++g) {
+                re[g] = im[g] = 0.0;
+                if (g < G)
+                    for (int i = M.g_first[g]; i < M.g_last[g]; ++i) {
+                        double sn, cs;
+                        sincos_bounded(kx * pw[2 * i] + ky * pw[2 * i + 1], &sn, &cs);
+                        re[g] += cs;
+                        im[g] += sn;
+                    }
+                ar[g] += wt * re[g];
+                ai[g] += wt * im[g];
+            }
+            int q = 0;
+#pragma unroll
+            for (int a = 0; a < kMaxGroups; ++a)
+#pragma unroll
+                for (int b = a; b < kMaxGroups; ++b, ++q)
+                    as[q] += wt * (re[a] * re[b] + im[a] * im[b]);
+        }
+        if (k < nk) {
+            // pair (a, b) of G groups sits at a G - a(a-1)/2 + (b - a)
+            int q = 0;
+#pragma unroll
+            for (int a = 0; a < kMaxGroups; ++a)
+#pragma unroll
+                for (int b = a; b < kMaxGroups; ++b, ++q)
+                    if (b < G) row[(a * G - a * (a - 1) / 2 + (b - a)) * nk + k] = as[q];
+#pragma unroll
+            for (int g = 0; g < kMaxGroups; ++g)
+                if (g < G) {
+                    row[npg * nk + g * nk + k] = ar[g];
+                    row[npg * nk + G * nk + g * nk + k] = ai[g];
+                }
+        }
+    }
+}
+
+// acc[e] += sum over chunks c (in order) of part[c][e]; the last workgroup
+// adds the walkers' weight sum (a fixed per-thread then tree order) to
+// acc[n_entries].
+__global__ __launch_bounds__(kEstThreads) void k_est_reduce(const double* __restrict__ part, int n_chunks,
+                                                            int part_stride, int n_entries,
+                                                            double* __restrict__ acc,
+                                                            const double* __restrict__ weights, int nw) {
+    const int tid = threadIdx.x;
+    const int e = blockIdx.x * kEstThreads + tid;
+    if (e < n_entries) {
+        double sum = 0.0;
+        for (int c = 0; c < n_chunks; ++c) sum += part[static_cast<size_t>(c) * part_stride + e];
+        acc[e] += sum;
+    }
+    if (blockIdx.x + 1 == gridDim.x) {
+        __shared__ double red[kEstThreads];
+        double sum = 0.0;
+        for (int w = tid; w < nw; w += kEstThreads) sum += weights ? weights[w] : 1.0;
+        red[tid] = sum;
+        __syncthreads();
+        for (int h = kEstThreads / 2; h > 0; h >>= 1) {
+            if (tid < h) red[tid] += red[tid + h];
+            __syncthreads();
+        }
+        if (tid == 0) acc[n_entries] += red[0];
+    }
+}
+
+}  // namespace
+
+void launch_estimators(const DevModel* m, const double* pos, const double* weights, int nw, int chunk,
+                       bool gofr, double rmax, double inv_delta, int n_bins, int nfb, bool sofk,
+                       const double* kvec, int nk, double* part, int part_stride, int n_entries, double* acc,
+                       hipStream_t st) {
+    const int n_chunks = (nw + chunk - 1) / chunk;
+    if (gofr)
+        hipLaunchKernelGGL(k_gofr, dim3(n_chunks), dim3(kEstThreads), 0, st, m, pos, weights, nw, chunk, rmax,
+                           inv_delta, n_bins, part, part_stride);
+    if (sofk)
+        hipLaunchKernelGGL(k_sofk, dim3(n_chunks, (nk + 63) / 64), dim3(64), 0, st, m, pos, weights, nw, chunk,
+                           kvec, nk, part, part_stride, nfb);
+    const int blocks = (n_entries + kEstThreads - 1) / kEstThreads + 1;
+    hipLaunchKernelGGL(k_est_reduce, dim3(blocks), dim3(kEstThreads), 0, st, part, n_chunks, part_stride,
+                       n_entries, acc, weights, nw);
+}
+
 }  // namespace alea_amd

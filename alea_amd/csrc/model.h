@@ -13,6 +13,8 @@ namespace alea_amd {
 
 constexpr int kMaxParticles = 256;
 constexpr int kMaxGroups = 4;
+constexpr int kMaxGofrEntries = 4096;   // pair-correlation functions x bins
+constexpr int kMaxSofkVectors = 1024;
 constexpr int kMaxPair = 8;        // pair-Jastrow factors
 constexpr int kMaxCoef = 12;       // Padé / Drummond coefficients
 constexpr int kMaxKnots = 64;      // breakup spline knots

@@ -167,8 +167,10 @@ class DMC:
             e, _ = self.w.local_energy()
         self.pc.set_energy_spread(robust_spread(e))
 
-    def step(self, measure=False):
-        """advance_measure -> branch -> E_T update; returns (sum w E_L, sum w)."""
+    def step(self, measure=False, on_measure=None):
+        """advance_measure -> branch -> E_T update; returns (sum w E_L, sum w).
+        on_measure() runs on measured steps before the branch, while the
+        walkers carry their DMC weights (the mixed estimators, dmc.cpp:702-707)."""
         w, pc = self.w, self.pc
         w.dmc_begin()
         n_acc = 0
@@ -177,6 +179,8 @@ class DMC:
             n_acc += w.dmc_sweeps(1, self.tau, tape, self.stride if tape is not None else 0)
             w.dmc_weights(pc.etrial, pc.eref, pc.e_cut, pc.e_max, pc.tau)
         e_sum, w_sum, n_cut = w.dmc_sums()
+        if measure and on_measure is not None:
+            on_measure()
         ens_e = e_sum / w_sum if w_sum > 0.0 else pc.etrial
         nw_before = w.nw
         src = w.branch(self.branch_state, self.target)

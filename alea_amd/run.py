@@ -15,15 +15,19 @@ src/qmc/app.cpp:1066-1098).
 The walkers draw from the device streams (one xoshiro256++ stream per
 walker, seeded from the stage seed), so a run reproduces the reference's
 statistics, not its trajectories (those are replayed through tapes in the
-tests).  Run settings the GPU path does not hold (langevin VMC moves,
-non-bare reweighting, forward walking, estimators, walker files, optimizer
-and PIMC stages) are refused with a ValueError.
+tests).  The `gofr` and `sofk` estimators of a stage are measured on the
+device (alea_amd.estimators) after each accumulation step, with the DMC
+weights before the branch in DMC.  Run settings the GPU path does not hold
+(langevin VMC moves, non-bare reweighting, forward walking, other
+estimators, walker files, optimizer and PIMC stages) are refused with a
+ValueError.
 """
 from __future__ import annotations
 
 import numpy as np
 
 from . import config, dmc, statistics
+from .estimators import Estimators
 from .walkers import FP64, gpu_walkers, seed as seed_walkers
 
 
@@ -51,9 +55,10 @@ def _check_stage(st: dict):
     method = st.get("method")
     if method not in ("vmc", "dmc"):
         _refuse(f"method '{method}'")
-    for key in ("estimators", "walkers_in", "walkers_out", "forward_walking", "paths_in", "paths_out"):
+    for key in ("walkers_in", "walkers_out", "forward_walking", "paths_in", "paths_out"):
         if st.get(key):
             _refuse(f"`{key}` of a {method} stage")
+    _estimator_args(st)
     for key in ("samples", "accumulation_blocks", "steps_per_block", "sub_steps", "time_step"):
         if not st.get(key):
             raise ValueError(f"run: a {method} stage needs `{key}` > 0")
@@ -64,7 +69,32 @@ def _check_stage(st: dict):
         _refuse(f"reweighting '{st.get('reweighting')}'")
 
 
-def run_vmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, nw=None) -> dict:
+def _estimator_args(st: dict) -> dict:
+    """The stage's `estimators:` as Estimators arguments: one `gofr` (rmax,
+    n_bins) and one `sofk` (kmax) at most (parser.cpp:148-171)."""
+    args = {}
+    for e in st.get("estimators", []) or []:
+        kind = e.get("type")
+        if kind not in ("gofr", "sofk"):
+            _refuse(f"estimator '{kind}'")
+        if e.get("name"):
+            _refuse(f"a named {kind} estimator")
+        if kind in args:
+            _refuse(f"a second {kind} estimator in one stage")
+        if kind == "gofr":
+            args["gofr"] = (float(e.get("rmax", 0.0)), int(e.get("n_bins", 0)))
+        else:
+            args["sofk"] = float(e.get("kmax", 0.0))
+    return args
+
+
+def _estimators(w, st, group_names):
+    args = _estimator_args(st)
+    return Estimators(w, group_names=group_names, **args) if args else None
+
+
+def run_vmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, nw=None,
+            group_names=None) -> dict:
     _check_stage(st)
     nw = nw or int(st["samples"])
     n = spec.n_particles
@@ -73,6 +103,7 @@ def run_vmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, n
         w.set_positions(_soa(_seed_rows(rows, nw), n))
         w.evaluate()
         seed_walkers(w, stage_seed)
+        est = _estimators(w, st, group_names)
         tau = float(st["time_step"])
         n_sub, n_steps = int(st["sub_steps"]), int(st["steps_per_block"])
         for _ in range(int(st.get("equilibration_blocks", 0))):
@@ -84,9 +115,13 @@ def run_vmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, n
         for _ in range(int(st["accumulation_blocks"])):
             b_sum = b_im = 0.0
             b_n = 0
+            if est:
+                est.start_block()
             for _ in range(n_steps):
                 acc += w.vmc_sweeps(n_sub, tau)
                 prop += n_sub * n * nw
+                if est:
+                    est.accumulate()
                 e, e_im = w.local_energy()
                 b_sum += float(np.sum(e))
                 b_im += float(np.sum(e_im))
@@ -96,6 +131,8 @@ def run_vmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, n
                 n_tot += nw
             blocks.append(b_sum / b_n)
             blocks_im.append(b_im / b_n)
+            if est:
+                est.stop_block()
         rb = statistics.reblock(blocks)
         mean = e_sum / n_tot
         out = dict(method="vmc", walkers=nw, energy=rb.mean, error=rb.error, tau_int=rb.tau_int,
@@ -104,13 +141,16 @@ def run_vmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, n
         if spec.slater_twisted:
             ri = statistics.reblock(blocks_im)
             out.update(energy_imag=ri.mean, energy_imag_error=ri.error)
+        if est:
+            out["estimators"] = est.finalize()
         out["final_rows"] = _rows(w.positions(), n, nw)
         return out
     finally:
         w.close()
 
 
-def run_dmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, target=None) -> dict:
+def run_dmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, target=None,
+            group_names=None) -> dict:
     _check_stage(st)
     target = target or int(st["samples"])
     n = spec.n_particles
@@ -121,6 +161,8 @@ def run_dmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, t
         w.evaluate()
         w.permute(np.arange(target))
         seed_walkers(w, stage_seed, dmc.K_DMC_THREAD_STREAM)
+        est = _estimators(w, st, group_names)
+        measure = (lambda: est.accumulate(use_weights=True)) if est else None
         cutoff = st.get("branch_cutoff", "universal")
         alpha = st.get("branch_cutoff_alpha")
         d = dmc.DMC(w, float(st["time_step"]), target, n_substeps=int(st["sub_steps"]),
@@ -142,18 +184,24 @@ def run_dmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, t
         acc = att = 0
         for _ in range(int(st["accumulation_blocks"])):
             be = bw = 0.0
+            if est:
+                est.start_block()
             for _ in range(n_steps):
                 att += w.nw * n * int(st["sub_steps"])
-                e, ws = d.step(measure=True)
+                e, ws = d.step(measure=True, on_measure=measure)
                 acc += d.history[-1]["n_accepted"]
                 be += e
                 bw += ws
             blocks.append(be / bw if bw > 0.0 else 0.0)
             pops.append(w.nw)
+            if est:
+                est.stop_block()
         rb = statistics.reblock(blocks)
         out = dict(method="dmc", target=target, energy=rb.mean, error=rb.error, tau_int=rb.tau_int,
                    acceptance=acc / att if att else 0.0, mean_population=float(np.mean(pops)),
                    e_trial=d.pc.etrial, block_energies=blocks, equilibration_block_energies=eq_blocks)
+        if est:
+            out["estimators"] = est.finalize()
         out["final_rows"] = _rows(w.positions(), n, w.nw)
         return out
     finally:
@@ -170,6 +218,7 @@ def run(cfg, precision=FP64, seed=None, walkers=None) -> list[dict]:
     for st in stages:
         _check_stage(st)
     base = int(seed if seed is not None else (cfg.get("project", {}) or {}).get("seed", 0))
+    names = [g["name"] for g in config.build_groups(cfg)]
     rows, results = None, []
     for i, st in enumerate(stages):
         stage_seed = base + i + 1           # app.cpp:815-817, no optimizer or PIMC stage
@@ -177,9 +226,9 @@ def run(cfg, precision=FP64, seed=None, walkers=None) -> list[dict]:
         if rows is None:
             rows = _rows(config.initial_positions(cfg, nw, stage_seed), spec.n_particles, nw)
         if st.get("method") == "vmc":
-            res = run_vmc(spec, st, rows, stage_seed, precision, nw)
+            res = run_vmc(spec, st, rows, stage_seed, precision, nw, names)
         else:
-            res = run_dmc(spec, st, rows, stage_seed, precision, nw)
+            res = run_dmc(spec, st, rows, stage_seed, precision, nw, names)
         rows = res.pop("final_rows")
         res["stage"] = i
         results.append(res)

@@ -235,6 +235,35 @@ def dmc_leg(c=DMC_CFG):
     return out
 
 
+def estimators_leg(nw=65536, reps=5):
+    """One measurement of g(r) (rmax = L/2, 100 bins) and S(k) (|k| <= 3 k_F)
+    of the N=122 workload's walkers on the device, `reps` times; milliseconds
+    per measurement from the wall clock around synchronized batches."""
+    from alea_amd.estimators import Estimators
+    from alea_amd.system import uniform_positions
+    from alea_amd.walkers import gpu_walkers
+    spec, _ = workload_spec()
+    w = gpu_walkers(spec, nw, 1)
+    try:
+        w.set_positions(uniform_positions(spec, nw, np.random.default_rng(POS_SEED)))
+        L = spec.lattice[0]
+        k_f = math.sqrt(4.0 * math.pi * N_UP / (L * L))
+        est = Estimators(w, gofr=(0.5 * L, 100), sofk=3.0 * k_f)
+        est.start_block()
+        est.accumulate()
+        est.stop_block()     # warm-up; reading the sums waits for the stream
+        est.start_block()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            est.accumulate()
+        est.stop_block()     # reads the sums back: waits for the stream
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        return dict(walkers=nw, n_particles=spec.n_particles, gofr_bins=100, sofk_vectors=est.nk,
+                    ms_per_measurement=ms)
+    finally:
+        w.close()
+
+
 def copy_calibration(n_bytes, reps=3):
     """A plain device-to-device copy of n_bytes, reps times: the known-size
     reference that scales the FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
@@ -263,6 +292,8 @@ def main():
     ap.add_argument("--serving-only", action="store_true", help="run the serving leg alone, print its dict")
     ap.add_argument("--no-dmc", action="store_true", help="skip the DMC accuracy leg")
     ap.add_argument("--dmc-only", action="store_true", help="run the DMC leg alone, print its dict")
+    ap.add_argument("--estimators-only", action="store_true",
+                    help="time one g(r) + S(k) measurement of the workload's walkers, print its dict")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
@@ -276,6 +307,9 @@ def main():
         return
     if args.dmc_only:
         print(json.dumps(dmc_leg()), flush=True)
+        return
+    if args.estimators_only:
+        print(json.dumps(estimators_leg(args.walkers)), flush=True)
         return
     tau = (0.5 * RS) ** 2   # sigma = sqrt(tau/m) = 0.5 rs
     if args.only:

@@ -212,6 +212,36 @@ int alea_set_weights(alea_system* s, const double* w);
  * new_nw source indices; 2 target + 1 must not exceed the capacity. */
 int alea_branch(alea_system* s, uint64_t* rng_state, int target, int* new_nw, int* src_out);
 
+/* ── Estimators (src/estimators/estimator.h:48-64: start_block, accumulate,
+ *    then the block's sums; stop_block's normalization and finalize run on
+ *    the host) ─────────────────────────────────────────────────────────── */
+/* PairCorrelation(groups, dim, rmax, n_bins, cell) —
+ * src/estimators/pair_correlation.h:29-33: one function per unordered group
+ * pair a <= b, pid = a G - a(a-1)/2 + (b - a), histogram [pid * n_bins + bin]
+ * of r < rmax at bin int(r n_bins / rmax).  A periodic cell needs rmax at most
+ * its Wigner-Seitz inradius; G(G+1)/2 * n_bins <= 4096. */
+int alea_add_gofr(alea_system* s, double rmax, int n_bins);
+/* StructureFactor(groups, cell, kmax) — src/estimators/structure_factor.h:52:
+ * the half space (n1 > 0, or n1 = 0 and n2 > 0) of the reduced basis'
+ * reciprocal lattice with |k| <= kmax, sorted by |k|; *n_k receives their
+ * number (at most 1024).  Periodic cells only. */
+int alea_add_sofk(alea_system* s, double kmax, int* n_k);
+/* The structure factor's vectors kx, ky [n_k] and shell index [n_k]. */
+int alea_get_sofk_vectors(alea_system* s, double* kx, double* ky, int* shell);
+/* Estimator::start_block of every estimator: the running sums to zero. */
+int alea_estimators_start_block(alea_system* s);
+/* Estimator::accumulate(wset) (pair_correlation.cpp:74-105,
+ * structure_factor.cpp:62-105): one measurement of the current walkers with
+ * weight 1 each (use_weights = 0, VMC) or the walkers' weights (DMC).
+ * Asynchronous on the system's stream; deterministic for given walkers. */
+int alea_estimators_accumulate(alea_system* s, int use_weights);
+/* The block's running sums (NULL skips): gofr [G(G+1)/2 * n_bins] sum of
+ * weights per bin; sofk [G(G+1)/2 * n_k] sum of w Re(rho_a rho_b^*); rho_re,
+ * rho_im [G * n_k] sum of w rho_g(k); wsum[2] the weight sum (for the pair
+ * correlation's and the structure factor's normalization). */
+int alea_estimators_read(alea_system* s, double* gofr, double* sofk, double* rho_re, double* rho_im,
+                         double* wsum);
+
 #ifdef __cplusplus
 }
 #endif

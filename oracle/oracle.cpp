@@ -567,6 +567,21 @@ struct Sys {
     std::vector<double> weights, e_prev, rr_p, rr_a;
     std::vector<long long> n_cut;
     bool has_phase = false;
+    // Estimators (PairCorrelation, StructureFactor): the block's running sums
+    struct Gofr {
+        bool on = false;
+        double rmax = 0.0, inv_delta = 0.0, wsum = 0.0;
+        std::size_t n_bins = 0, n_func = 0;
+        std::vector<double> hist;   // [pid * n_bins + bin]
+    } gofr;
+    struct Sofk {
+        bool on = false;
+        std::vector<int> n1, n2, shell;
+        std::vector<double> kx, ky;
+        int m1 = 0, m2 = 0;
+        double wsum = 0.0;
+        std::vector<double> acc_s, acc_re, acc_im;   // [pid][k], [g][k]
+    } sofk;
 
     double& P(std::size_t p, std::size_t d, std::size_t w) { return pos[(p * dim + d) * nw + w]; }
     double& D(std::size_t i, std::size_t j, std::size_t w) { return dist[(i * np + j) * nw + w]; }
@@ -2332,6 +2347,211 @@ int oracle_set_weights(oracle_system* h, const double* w) {
             if (!(w[i] >= 0.0) || !std::isfinite(w[i]))
                 throw std::invalid_argument("oracle: weights must be finite and non-negative");
         std::copy(w, w + s.nw, s.weights.begin());
+    });
+}
+
+// ── Estimators ─────────────────────────────────────────────────────────────
+// PairCorrelation (src/estimators/pair_correlation.cpp:16-64, 66-105): one
+// function per unordered group pair a <= b, pid = a G - a(a-1)/2 + (b-a); every
+// pair i < j of the distance table with r < rmax adds its walker's weight to
+// bin int(r / delta); the block's weight sum adds each call's walker sum.
+int oracle_add_gofr(oracle_system* h, double rmax, int n_bins) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        if (!(rmax > 0.0) || n_bins <= 0) throw std::invalid_argument("oracle: gofr needs rmax > 0, n_bins > 0");
+        if (s.cell.periodic && rmax > s.cell.ws)
+            throw std::invalid_argument("oracle: gofr rmax exceeds the Wigner-Seitz inradius");
+        const std::size_t G = s.groups.size();
+        auto& g = s.gofr;
+        g.on = true;
+        g.rmax = rmax;
+        g.n_bins = static_cast<std::size_t>(n_bins);
+        g.inv_delta = static_cast<double>(n_bins) / rmax;
+        g.n_func = G * (G + 1) / 2;
+        g.hist.assign(g.n_func * g.n_bins, 0.0);
+    });
+}
+
+// StructureFactor (src/estimators/structure_factor.cpp:16-54): the half space
+// n1 > 0, or n1 = 0 and n2 > 0, of the reciprocal lattice of the reduced
+// basis, |k| <= kmax, sorted by |k| (std::sort, as the reference), shells of
+// equal |k| within 1e-9 min|b|.
+int oracle_add_sofk(oracle_system* h, double kmax, int* n_k) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        if (!s.cell.periodic) throw std::invalid_argument("oracle: sofk needs a periodic cell");
+        const double* b = s.cell.b;
+        const double b1 = std::hypot(b[0], b[1]), b2 = std::hypot(b[2], b[3]);
+        const int nmax = static_cast<int>(std::ceil(kmax / std::min(b1, b2))) + 1;
+        struct K { int n1, n2; double kx, ky, k; };
+        std::vector<K> ks;
+        for (int i = 0; i <= nmax; ++i)
+            for (int j = -nmax; j <= nmax; ++j) {
+                if (i == 0 && j <= 0) continue;
+                const double kx = i * b[0] + j * b[2], ky = i * b[1] + j * b[3];
+                const double k = std::hypot(kx, ky);
+                if (k <= kmax) ks.push_back({i, j, kx, ky, k});
+            }
+        if (ks.empty()) throw std::invalid_argument("oracle: sofk kmax below the shortest reciprocal vector");
+        std::sort(ks.begin(), ks.end(), [](const K& x, const K& y) { return x.k < y.k; });
+        auto& f = s.sofk;
+        f = Sys::Sofk{};
+        f.on = true;
+        const double tol = 1e-9 * std::min(b1, b2);
+        double last = 0.0;
+        int shell = -1;
+        for (const K& q : ks) {
+            if (shell < 0 || q.k - last > tol) {
+                ++shell;
+                last = q.k;
+            }
+            f.shell.push_back(shell);
+            f.kx.push_back(q.kx);
+            f.ky.push_back(q.ky);
+            f.n1.push_back(q.n1);
+            f.n2.push_back(q.n2);
+            f.m1 = std::max(f.m1, std::abs(q.n1));
+            f.m2 = std::max(f.m2, std::abs(q.n2));
+        }
+        const std::size_t nk = f.kx.size(), G = s.groups.size();
+        f.acc_s.assign(G * (G + 1) / 2 * nk, 0.0);
+        f.acc_re.assign(G * nk, 0.0);
+        f.acc_im.assign(G * nk, 0.0);
+        *n_k = static_cast<int>(nk);
+    });
+}
+
+int oracle_get_sofk_vectors(oracle_system* h, double* kx, double* ky, int* shell) {
+    return guarded([&] {
+        const auto& f = S(h)->sofk;
+        if (!f.on) throw std::invalid_argument("oracle: no sofk estimator");
+        std::copy(f.kx.begin(), f.kx.end(), kx);
+        std::copy(f.ky.begin(), f.ky.end(), ky);
+        std::copy(f.shell.begin(), f.shell.end(), shell);
+    });
+}
+
+int oracle_estimators_start_block(oracle_system* h) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        std::fill(s.gofr.hist.begin(), s.gofr.hist.end(), 0.0);
+        s.gofr.wsum = 0.0;
+        std::fill(s.sofk.acc_s.begin(), s.sofk.acc_s.end(), 0.0);
+        std::fill(s.sofk.acc_re.begin(), s.sofk.acc_re.end(), 0.0);
+        std::fill(s.sofk.acc_im.begin(), s.sofk.acc_im.end(), 0.0);
+        s.sofk.wsum = 0.0;
+    });
+}
+
+namespace {
+// ReciprocalPhases::ladder (src/numerics/reciprocal_phases.h): e^{i n theta}
+// for n in [-m, m] by repeated multiplication.
+void phase_ladder(double theta, int m, std::vector<double>& re, std::vector<double>& im) {
+    re.assign(2 * m + 1, 0.0);
+    im.assign(2 * m + 1, 0.0);
+    double* r = re.data() + m;
+    double* i = im.data() + m;
+    const double c = std::cos(theta), sn = std::sin(theta);
+    r[0] = 1.0;
+    i[0] = 0.0;
+    for (int n = 1; n <= m; ++n) {
+        r[n] = r[n - 1] * c - i[n - 1] * sn;
+        i[n] = r[n - 1] * sn + i[n - 1] * c;
+        r[-n] = r[n];
+        i[-n] = -i[n];
+    }
+}
+}  // namespace
+
+// One measurement of the current walkers (Estimator::accumulate(wset)):
+// weights 1 (use_weights = 0) or the walkers' DMC weights.
+int oracle_estimators_accumulate(oracle_system* h, int use_weights) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        const std::size_t nw = s.nw, np = s.np, G = s.groups.size();
+        std::vector<double> wt(nw, 1.0);
+        if (use_weights) std::copy(s.weights.begin(), s.weights.begin() + nw, wt.begin());
+        std::vector<std::size_t> group_of(np, 0);
+        for (std::size_t g = 0; g < G; ++g)
+            for (std::size_t p = s.groups[g].first; p < s.groups[g].last; ++p) group_of[p] = g;
+        auto pair_id = [&](std::size_t a, std::size_t b) {
+            if (b < a) std::swap(a, b);
+            return a * G - a * (a - 1) / 2 + (b - a);
+        };
+        if (s.gofr.on) {   // pair_correlation.cpp:74-105
+            auto& g = s.gofr;
+            for (std::size_t i = 0; i < np; ++i)
+                for (std::size_t j = i + 1; j < np; ++j) {
+                    double* hst = g.hist.data() + pair_id(group_of[i], group_of[j]) * g.n_bins;
+                    for (std::size_t w = 0; w < nw; ++w) {
+                        const double r = s.D(i, j, w);
+                        if (r < g.rmax) {
+                            const std::size_t bin = static_cast<std::size_t>(r * g.inv_delta);
+                            if (bin < g.n_bins) hst[bin] += wt[w];
+                        }
+                    }
+                }
+            double ws = 0.0;
+            for (std::size_t w = 0; w < nw; ++w) ws += wt[w];
+            g.wsum += ws;
+        }
+        if (s.sofk.on) {   // structure_factor.cpp:62-105
+            auto& f = s.sofk;
+            const std::size_t nk = f.kx.size();
+            const double* b = s.cell.b;
+            std::vector<double> rre(G * nk), rim(G * nk), c(nk), sn(nk), e1r, e1i, e2r, e2i;
+            for (std::size_t w = 0; w < nw; ++w) {
+                for (std::size_t gg = 0; gg < G; ++gg) {
+                    double* re = rre.data() + gg * nk;
+                    double* im = rim.data() + gg * nk;
+                    std::fill_n(re, nk, 0.0);
+                    std::fill_n(im, nk, 0.0);
+                    for (std::size_t i = s.groups[gg].first; i < s.groups[gg].last; ++i) {
+                        const double x = s.P(i, 0, w), y = s.P(i, 1, w);
+                        phase_ladder(b[0] * x + b[1] * y, f.m1, e1r, e1i);
+                        phase_ladder(b[2] * x + b[3] * y, f.m2, e2r, e2i);
+                        for (std::size_t k = 0; k < nk; ++k) {
+                            const double ar = e1r[f.m1 + f.n1[k]], ai = e1i[f.m1 + f.n1[k]];
+                            const double br = e2r[f.m2 + f.n2[k]], bi = e2i[f.m2 + f.n2[k]];
+                            c[k] = ar * br - ai * bi;
+                            sn[k] = ar * bi + ai * br;
+                        }
+                        for (std::size_t k = 0; k < nk; ++k) {
+                            re[k] += c[k];
+                            im[k] += sn[k];
+                        }
+                    }
+                    for (std::size_t k = 0; k < nk; ++k) {
+                        f.acc_re[gg * nk + k] += wt[w] * re[k];
+                        f.acc_im[gg * nk + k] += wt[w] * im[k];
+                    }
+                }
+                std::size_t pid = 0;
+                for (std::size_t a = 0; a < G; ++a)
+                    for (std::size_t bb = a; bb < G; ++bb, ++pid)
+                        for (std::size_t k = 0; k < nk; ++k)
+                            f.acc_s[pid * nk + k] += wt[w] * (rre[a * nk + k] * rre[bb * nk + k] +
+                                                               rim[a * nk + k] * rim[bb * nk + k]);
+                f.wsum += wt[w];
+            }
+        }
+    });
+}
+
+// The block's running sums; wsum[0] the pair correlation's weight sum,
+// wsum[1] the structure factor's (each summed in its reference's order).
+int oracle_estimators_read(oracle_system* h, double* gofr, double* sofk, double* rho_re, double* rho_im,
+                           double* wsum) {
+    return guarded([&] {
+        const Sys& s = *S(h);
+        if (gofr) std::copy(s.gofr.hist.begin(), s.gofr.hist.end(), gofr);
+        if (sofk) std::copy(s.sofk.acc_s.begin(), s.sofk.acc_s.end(), sofk);
+        if (rho_re) std::copy(s.sofk.acc_re.begin(), s.sofk.acc_re.end(), rho_re);
+        if (rho_im) std::copy(s.sofk.acc_im.begin(), s.sofk.acc_im.end(), rho_im);
+        if (wsum) {
+            wsum[0] = s.gofr.wsum;
+            wsum[1] = s.sofk.wsum;
+        }
     });
 }
 

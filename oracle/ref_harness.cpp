@@ -24,6 +24,8 @@
 #include <string>
 #include <vector>
 
+#include "estimators/pair_correlation.h"
+#include "estimators/structure_factor.h"
 #include "hamiltonian/coulomb.h"
 #include "hamiltonian/ewald_term.h"
 #include "hamiltonian/hamiltonian.h"
@@ -39,6 +41,7 @@
 #include "population/walker.h"
 #include "population/walker_set.h"
 #include "qmc/population_control.h"
+#include "qmc/run_result.h"
 #include "qmc/sweep_blocking.h"
 #include "wavefunction/drummond_jastrow.h"
 #include "wavefunction/nosanow_jastrow.h"
@@ -531,6 +534,53 @@ int ref_reblock(const double* x, int n, int n_chains, double* out, int* iout) {
         iout[0] = r.plateau_found ? 1 : 0;
         iout[1] = r.selected_level;
         iout[2] = static_cast<int>(r.selected_n_blocks);
+    });
+}
+
+// PairCorrelation (rmax > 0) and StructureFactor (kmax > 0) over the
+// system's walkers: n_blocks blocks of `steps` measurements, measurement m
+// at positions pos + m * np*dim*nw (SoA, wrapped and tabulated as
+// ref_set_positions does) with weights w + m * nw; then finalize().  The
+// RunResult series come back flattened: names joined by '\n' into `names`,
+// values concatenated into `values`, lengths[n_series].
+int ref_estimators(void* h, double rmax, int n_bins, double kmax, int n_blocks, int steps,
+                   const double* pos, const double* w, char* names, int names_cap, double* values,
+                   int values_cap, int* lengths, int lengths_cap, int* n_series) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        std::vector<std::unique_ptr<alea::Estimator>> ests;
+        if (rmax > 0.0)
+            ests.push_back(std::make_unique<alea::PairCorrelation>(s.groups, s.dim, rmax, n_bins, s.cell));
+        if (kmax > 0.0)
+            ests.push_back(std::make_unique<alea::StructureFactor>(s.groups, s.cell, kmax));
+        const std::size_t len = s.np * s.dim * s.nw;
+        for (int b = 0; b < n_blocks; ++b) {
+            for (auto& e : ests) e->start_block();
+            for (int m = 0; m < steps; ++m) {
+                const std::size_t idx = static_cast<std::size_t>(b) * steps + m;
+                if (ref_set_positions(h, pos + idx * len)) throw std::runtime_error(g_error);
+                const std::span<const double> wt(w + idx * s.nw, s.nw);
+                for (auto& e : ests) e->accumulate(*s.ws, wt);
+            }
+            for (auto& e : ests) e->stop_block();
+        }
+        alea::RunResult rr;
+        for (auto& e : ests) e->finalize(rr);
+        std::string joined;
+        std::size_t nv = 0;
+        int ns = 0;
+        for (const auto& [name, vals] : rr.series) {
+            if (ns >= lengths_cap || nv + vals.size() > static_cast<std::size_t>(values_cap))
+                throw std::runtime_error("ref_estimators: output capacity");
+            joined += name + "\n";
+            std::copy(vals.begin(), vals.end(), values + nv);
+            nv += vals.size();
+            lengths[ns++] = static_cast<int>(vals.size());
+        }
+        if (joined.size() + 1 > static_cast<std::size_t>(names_cap))
+            throw std::runtime_error("ref_estimators: name capacity");
+        std::memcpy(names, joined.c_str(), joined.size() + 1);
+        *n_series = ns;
     });
 }
 

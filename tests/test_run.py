@@ -53,7 +53,8 @@ def test_rows_round_trip_and_seeding():
 @pytest.mark.parametrize("edit,msg", [
     (lambda c: c["qmc"][0].update(random_walk="langevin"), "random_walk"),
     (lambda c: c["qmc"][1].update(reweighting="suppressed"), "reweighting"),
-    (lambda c: c["qmc"][0].update(estimators=[{"type": "gofr"}]), "estimators"),
+    (lambda c: c["qmc"][0].update(estimators=[{"type": "density"}]), "density"),
+    (lambda c: c["qmc"][1].update(estimators=[{"type": "gofr", "rmax": 1, "n_bins": 4, "name": "x"}]), "named"),
     (lambda c: c["qmc"].insert(0, {"method": "optimizer"}), "optimizer"),
     (lambda c: c["qmc"][0].update(time_step=0), "time_step"),
 ])
@@ -78,6 +79,31 @@ def test_exciton_config_runs_exactly():
     assert all(abs(e + 1.0) < 1e-12 for e in dmc_["block_energies"])
     assert abs(dmc_["e_trial"] + 1.0) < 1e-6
     assert 0.9 < dmc_["acceptance"] <= 1.0
+
+
+@pytest.mark.gpu
+def test_config_estimators_run_on_the_device():
+    """A config's gofr and sofk estimators: VMC of the free plane-wave
+    eigenstate, g(r) and S(k) finite with block errors; the DMC stage of the
+    exciton measures g_eh(r) with its weights."""
+    import copy
+    cfg = copy.deepcopy(PW_FREE)
+    cfg["qmc"][0]["estimators"] = [{"type": "gofr", "rmax": 2.0, "n_bins": 20}, {"type": "sofk", "kmax": 12.0}]
+    print("run: plane-wave VMC with gofr and sofk", flush=True)
+    (vmc,) = run.run(cfg)
+    est = vmc["estimators"]
+    g = np.array(est["gofr_electron_group_electron_group"])
+    assert g.shape == (20,) and np.all(np.isfinite(g))
+    # Pauli exclusion: same-spin fermions avoid each other at short range
+    assert g[0] < 0.2 * g[-5:].mean()
+    s = np.array(est["sofk_electron_group_electron_group"])
+    assert s.size == len(est["sofk_k"]) and np.all(s >= 0.0)
+    ex = copy.deepcopy(EXCITON)
+    ex["qmc"][1]["estimators"] = [{"type": "gofr", "rmax": 3.0, "n_bins": 30}]
+    print("run: exciton DMC with gofr", flush=True)
+    _, d = run.run(ex)
+    geh = np.array(d["estimators"]["gofr_electron_group_hole_group"])
+    assert np.all(np.isfinite(geh)) and geh.sum() > 0.0
 
 
 @pytest.mark.gpu
