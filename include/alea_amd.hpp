@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <limits>
 #include <span>
 #include <stdexcept>
@@ -457,6 +458,221 @@ inline double robust_spread(std::vector<double> x) {
     return 1.4826 * median(s);
 }
 
+// numerics::block_mean_error (statistics.cpp:18-34): mean and standard error
+// of decorrelated block means, sums left to right.
+inline std::pair<double, double> block_mean_error(std::span<const double> x) {
+    const std::size_t m = x.size();
+    if (m == 0) return {0.0, 0.0};
+    double sum = 0.0;
+    for (double v : x) sum += v;
+    const double mean = sum / static_cast<double>(m);
+    if (m < 2) return {mean, 0.0};
+    double sq = 0.0;
+    for (double v : x) sq += (v - mean) * (v - mean);
+    return {mean, std::sqrt(sq / static_cast<double>(m - 1) / static_cast<double>(m))};
+}
+
+// ── Estimators ─────────────────────────────────────────────────────────────
+// PairCorrelation (pair_correlation.cpp) and StructureFactor
+// (structure_factor.cpp) behind the Estimator interface (estimator.h:48-64).
+// The device keeps each block's running sums (alea_estimators_*); stop_block
+// normalizes them as the reference does and finalize() returns the
+// reference's RunResult series: gofr_r, gofr_<A>_<B>[_error], sofk_k,
+// sofk_shell_count, sofk_<A>_<B>[_error], sofk_kx, sofk_ky, sofk_shell,
+// rhok_<A>_re/_im[_error].
+struct EstimatorGroup {
+    std::string name;
+    std::size_t first = 0, last = 0;
+};
+struct Series {
+    std::string name;
+    std::vector<double> values;
+};
+
+class Estimators {
+public:
+    // rmax > 0 adds g(r) with n_bins bins; kmax > 0 adds S(k).  lattice: the
+    // cell vectors as given to the WalkerSet (empty when open).
+    Estimators(WalkerSet& ws, std::vector<EstimatorGroup> groups, std::span<const double> lattice,
+               double rmax = 0.0, int n_bins = 0, double kmax = 0.0)
+        : h_(ws.handle()), groups_(std::move(groups)) {
+        G_ = groups_.size();
+        npg_ = G_ * (G_ + 1) / 2;
+        if (lattice.size() == 4) area_ = reduced_area(lattice);
+        if (rmax > 0.0) {
+            check(alea_add_gofr(h_, rmax, n_bins));
+            gofr_ = true;
+            n_bins_ = static_cast<std::size_t>(n_bins);
+            delta_ = rmax / static_cast<double>(n_bins);
+        }
+        if (kmax > 0.0) {
+            int nk = 0;
+            check(alea_add_sofk(h_, kmax, &nk));
+            sofk_ = true;
+            nk_ = static_cast<std::size_t>(nk);
+            kx_.resize(nk_);
+            ky_.resize(nk_);
+            shell_.resize(nk_);
+            check(alea_get_sofk_vectors(h_, kx_.data(), ky_.data(), shell_.data()));
+            for (std::size_t k = 0; k < nk_; ++k) {
+                if (static_cast<std::size_t>(shell_[k]) == shell_count_.size()) {
+                    shell_count_.push_back(0);
+                    shell_k_.push_back(std::hypot(kx_[k], ky_[k]));
+                }
+                ++shell_count_[static_cast<std::size_t>(shell_[k])];
+            }
+        }
+    }
+
+    void start_block() { check(alea_estimators_start_block(h_)); }
+    // one measurement: weight 1 per walker (VMC) or the walkers' DMC weights
+    void accumulate(bool use_weights = false) { check(alea_estimators_accumulate(h_, use_weights ? 1 : 0)); }
+
+    void stop_block() {
+        std::vector<double> g(npg_ * n_bins_), s(npg_ * nk_), re(G_ * nk_), im(G_ * nk_);
+        double wsum[2] = {0.0, 0.0};
+        check(alea_estimators_read(h_, gofr_ ? g.data() : nullptr, sofk_ ? s.data() : nullptr,
+                                   sofk_ ? re.data() : nullptr, sofk_ ? im.data() : nullptr, wsum));
+        if (gofr_) {   // pair_correlation.cpp:112-128
+            std::vector<double> blk(g.size(), 0.0);
+            if (wsum[0] > 0.0) {
+                const double inv_w = 1.0 / wsum[0];
+                for (std::size_t k = 0; k < g.size(); ++k) blk[k] = g[k] * inv_w;
+            }
+            hist_g_.push_back(std::move(blk));
+        }
+        if (sofk_) {   // structure_factor.cpp:113-141
+            const std::size_t ns = shell_count_.size();
+            std::vector<double> sb(npg_ * ns, 0.0), rb(G_ * nk_, 0.0), ib(G_ * nk_, 0.0);
+            if (wsum[1] > 0.0) {
+                const double inv_w = 1.0 / wsum[1];
+                std::size_t pid = 0;
+                for (std::size_t a = 0; a < G_; ++a)
+                    for (std::size_t b = a; b < G_; ++b, ++pid) {
+                        const double na = static_cast<double>(groups_[a].last - groups_[a].first);
+                        const double nb = static_cast<double>(groups_[b].last - groups_[b].first);
+                        const double norm = (na > 0 && nb > 0) ? 1.0 / std::sqrt(na * nb) : 0.0;
+                        for (std::size_t k = 0; k < nk_; ++k) {
+                            const auto q = static_cast<std::size_t>(shell_[k]);
+                            sb[pid * ns + q] += s[pid * nk_ + k] * inv_w * norm / static_cast<double>(shell_count_[q]);
+                        }
+                    }
+                for (std::size_t q = 0; q < G_ * nk_; ++q) {
+                    rb[q] = re[q] * inv_w;
+                    ib[q] = im[q] * inv_w;
+                }
+            }
+            hist_s_.push_back(std::move(sb));
+            hist_re_.push_back(std::move(rb));
+            hist_im_.push_back(std::move(ib));
+        }
+    }
+
+    std::vector<Series> finalize() const {
+        std::vector<Series> out;
+        std::vector<double> per;
+        auto stat = [&](const std::vector<std::vector<double>>& hist, std::size_t q) {
+            per.resize(hist.size());
+            for (std::size_t i = 0; i < hist.size(); ++i) per[i] = hist[i][q];
+            return block_mean_error(per);
+        };
+        if (gofr_ && !hist_g_.empty()) {   // pair_correlation.cpp:140-190
+            std::vector<double> r(n_bins_);
+            for (std::size_t b = 0; b < n_bins_; ++b) r[b] = (static_cast<double>(b) + 0.5) * delta_;
+            out.push_back({"gofr_r", std::move(r)});
+            std::size_t pid = 0;
+            for (std::size_t a = 0; a < G_; ++a)
+                for (std::size_t b = a; b < G_; ++b, ++pid) {
+                    const std::size_t na = groups_[a].last - groups_[a].first;
+                    const std::size_t nb = groups_[b].last - groups_[b].first;
+                    const std::size_t pc = a == b ? na * (na - 1) / 2 : na * nb;
+                    if (pc == 0) continue;
+                    std::vector<double> g(n_bins_), e(n_bins_);
+                    for (std::size_t bin = 0; bin < n_bins_; ++bin) {
+                        const double lo = static_cast<double>(bin) * delta_, hi = static_cast<double>(bin + 1) * delta_;
+                        const double shell = kPi * (hi * hi - lo * lo);
+                        const double dpc = static_cast<double>(pc);
+                        const double norm = area_ > 0.0 ? dpc * shell / area_ : dpc * shell;
+                        const auto [m, err] = stat(hist_g_, pid * n_bins_ + bin);
+                        g[bin] = m / norm;
+                        e[bin] = err / norm;
+                    }
+                    const std::string name = "gofr_" + groups_[a].name + "_" + groups_[b].name;
+                    out.push_back({name, std::move(g)});
+                    out.push_back({name + "_error", std::move(e)});
+                }
+        }
+        if (sofk_ && !hist_s_.empty()) {   // structure_factor.cpp:169-221
+            const std::size_t ns = shell_count_.size();
+            auto emit = [&](const std::string& name, const std::vector<std::vector<double>>& hist,
+                            std::size_t offset, std::size_t count) {
+                std::vector<double> m(count), e(count);
+                for (std::size_t q = 0; q < count; ++q) std::tie(m[q], e[q]) = stat(hist, offset + q);
+                out.push_back({name, std::move(m)});
+                out.push_back({name + "_error", std::move(e)});
+            };
+            out.push_back({"sofk_k", shell_k_});
+            out.push_back({"sofk_shell_count", std::vector<double>(shell_count_.begin(), shell_count_.end())});
+            std::size_t pid = 0;
+            for (std::size_t a = 0; a < G_; ++a)
+                for (std::size_t b = a; b < G_; ++b, ++pid)
+                    emit("sofk_" + groups_[a].name + "_" + groups_[b].name, hist_s_, pid * ns, ns);
+            out.push_back({"sofk_kx", kx_});
+            out.push_back({"sofk_ky", ky_});
+            out.push_back({"sofk_shell", std::vector<double>(shell_.begin(), shell_.end())});
+            for (std::size_t g = 0; g < G_; ++g) {
+                emit("rhok_" + groups_[g].name + "_re", hist_re_, g * nk_, nk_);
+                emit("rhok_" + groups_[g].name + "_im", hist_im_, g * nk_, nk_);
+            }
+        }
+        return out;
+    }
+
+    std::size_t n_vectors() const { return nk_; }
+
+private:
+    static constexpr double kPi = 3.14159265358979323846;
+    // |det| of the basis SimulationCell keeps (simulation_cell.cpp:16-36,
+    // 100-132): axis-aligned rows reordered, any other basis Gauss-reduced
+    static double reduced_area(std::span<const double> v) {
+        double a[4] = {v[0], v[1], v[2], v[3]};
+        const bool row0_axis = (a[0] != 0.0) != (a[1] != 0.0), row1_axis = (a[2] != 0.0) != (a[3] != 0.0);
+        const bool ortho = row0_axis && row1_axis && ((a[0] != 0.0) != (a[2] != 0.0));
+        if (ortho) {
+            if (a[0] == 0.0) {   // rows along y then x: put them in axis order
+                std::swap(a[0], a[2]);
+                std::swap(a[1], a[3]);
+            }
+        } else {
+            for (int guard = 0; guard < 64; ++guard) {
+                const double n0 = a[0] * a[0] + a[1] * a[1];
+                const double n1 = a[2] * a[2] + a[3] * a[3];
+                if (n1 < n0) {
+                    std::swap(a[0], a[2]);
+                    std::swap(a[1], a[3]);
+                    continue;
+                }
+                if (n0 == 0.0) break;
+                const double mu = std::nearbyint((a[0] * a[2] + a[1] * a[3]) / n0);
+                if (mu == 0.0) break;
+                a[2] -= mu * a[0];
+                a[3] -= mu * a[1];
+            }
+        }
+        return std::abs(a[0] * a[3] - a[1] * a[2]);
+    }
+
+    alea_system* h_;
+    std::vector<EstimatorGroup> groups_;
+    std::size_t G_ = 0, npg_ = 0, n_bins_ = 0, nk_ = 0;
+    bool gofr_ = false, sofk_ = false;
+    double delta_ = 0.0, area_ = 0.0;
+    std::vector<double> kx_, ky_, shell_k_;
+    std::vector<int> shell_;
+    std::vector<std::size_t> shell_count_;
+    std::vector<std::vector<double>> hist_g_, hist_s_, hist_re_, hist_im_;
+};
+
 // ── DMC step ───────────────────────────────────────────────────────────────
 // DMC::run_one_step (dmc.cpp:693-739) with one crowd over advance_measure
 // (dmc.cpp:339-419, bare reweighting): n_substeps drift-diffusion sweeps each
@@ -491,6 +707,9 @@ public:
         double e = 0.0, w = 0.0;
         long long n_cut = 0;
         check(alea_dmc_sums(h, &e, &w, &n_cut));
+        // the mixed estimators see the population before the branch, with
+        // its DMC weights (dmc.cpp:702-707)
+        if (measure && measure_hook_) measure_hook_();
         const double ens_e = w > 0.0 ? e / w : pc_.etrial();
         ws_.branch(branch_rng_, target_, src_);
         pc_.update(ens_e, w, measure);
@@ -498,6 +717,9 @@ public:
     }
     const PopulationControl& population() const { return pc_; }
     const std::vector<std::size_t>& last_branch() const { return src_; }
+    // DMC::set_measure_hook: called on measured steps before the branch,
+    // e.g. [&] { estimators.accumulate(true); }
+    void set_measure_hook(std::function<void()> hook) { measure_hook_ = std::move(hook); }
 
 private:
     WalkerSet& ws_;
@@ -507,6 +729,7 @@ private:
     PopulationControl pc_;
     XoshiroState branch_rng_;
     std::vector<std::size_t> src_;
+    std::function<void()> measure_hook_;
 };
 
 }  // namespace alea_amd

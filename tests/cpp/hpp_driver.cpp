@@ -10,6 +10,9 @@
 //       TrialWavefunction / Hamiltonian, then: evaluate_batch; ratio_batch +
 //       apply_move_batch over every particle; VMC sweeps from a tape; DMC
 //       run_one_step with tapes.  Writes every result as "name count values".
+//   hpp_driver estimators <spec> <input> <output>
+//       the Estimators class (g(r), S(k)) over given positions and weights;
+//       writes every finalized series.
 //
 // Text files are whitespace-separated tokens; arrays are "count v1 .. vn".
 #include <cstdio>
@@ -315,13 +318,59 @@ int run(const char* spec_path, const char* in_path, const char* out_path) {
     return 0;
 }
 
+// hpp_driver estimators <spec> <input> <output>: the spec's groups only;
+// input "rmax n_bins kmax n_blocks steps use_weights" then, per measurement,
+// positions (SoA) and weights; output every finalized series.
+int estimators(const char* spec_path, const char* in_path, const char* out_path) {
+    Reader sp(spec_path);
+    if (sp.word() != "system") throw std::runtime_error("spec must start with 'system'");
+    const auto nw = static_cast<std::size_t>(sp.integer());
+    const auto np = static_cast<std::size_t>(sp.integer());
+    const auto precision = static_cast<Precision>(sp.integer());
+    const std::vector<double> lattice = sp.vec();
+    WalkerSet ws(nw, np, 2, lattice, precision);
+    std::vector<EstimatorGroup> groups;
+    for (std::string key = sp.word(); key != "end"; key = sp.word()) {
+        if (key != "group") throw std::runtime_error("estimators: the spec holds groups only");
+        ParticleGroup g;
+        g.first = static_cast<std::size_t>(sp.integer());
+        g.last = static_cast<std::size_t>(sp.integer());
+        g.charge = sp.num();
+        g.mass = sp.num();
+        ws.add_group(g);
+        groups.push_back({"g" + std::to_string(g.first), g.first, g.last});
+    }
+    Reader in(in_path);
+    Writer out(out_path);
+    const double rmax = in.num();
+    const int n_bins = static_cast<int>(in.integer());
+    const double kmax = in.num();
+    const int n_blocks = static_cast<int>(in.integer());
+    const int steps = static_cast<int>(in.integer());
+    const bool use_weights = in.integer() != 0;
+    Estimators est(ws, groups, lattice, rmax, n_bins, kmax);
+    for (int b = 0; b < n_blocks; ++b) {
+        est.start_block();
+        for (int m = 0; m < steps; ++m) {
+            ws.load_positions(in.vec());
+            const std::vector<double> w = in.vec();
+            if (use_weights) ws.set_weights(w);
+            est.accumulate(use_weights);
+        }
+        est.stop_block();
+    }
+    for (const Series& s : est.finalize()) out.put(s.name, s.values);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
     try {
         if (argc == 2 && std::string(argv[1]) == "selftest") return selftest();
         if (argc == 5 && std::string(argv[1]) == "run") return run(argv[2], argv[3], argv[4]);
-        std::fprintf(stderr, "usage: hpp_driver selftest | run <spec> <input> <output>\n");
+        if (argc == 5 && std::string(argv[1]) == "estimators") return estimators(argv[2], argv[3], argv[4]);
+        std::fprintf(stderr, "usage: hpp_driver selftest | run|estimators <spec> <input> <output>\n");
         return 2;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "hpp_driver: %s\n", e.what());

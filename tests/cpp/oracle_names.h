@@ -35,3 +35,9 @@
 #define alea_get_weights oracle_get_weights
 #define alea_set_weights oracle_set_weights
 #define alea_branch oracle_branch
+#define alea_add_gofr oracle_add_gofr
+#define alea_add_sofk oracle_add_sofk
+#define alea_get_sofk_vectors oracle_get_sofk_vectors
+#define alea_estimators_start_block oracle_estimators_start_block
+#define alea_estimators_accumulate oracle_estimators_accumulate
+#define alea_estimators_read oracle_estimators_read

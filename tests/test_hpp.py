@@ -246,3 +246,62 @@ def test_cpp_interface_matches_python_binding(spec, precision, tmp_path):
     case = _case(spec, cap, 5)
     got = _run_driver(_driver(), spec, cap, precision, case, tmp_path)
     _compare(got, _python_run(lambda: gpu_walkers(spec, cap, precision), case), case[5]["n_steps"])
+
+
+# ── Estimators through the C++ class ───────────────────────────────────────
+def _est_cases():
+    from test_estimators import _spec
+    from alea_amd.system import hexagonal_lattice
+    return [pytest.param(_spec(hexagonal_lattice(10, 1.5), (6, 4), "hex"), (3.0, 24, 9.0), id="hex 6+4"),
+            pytest.param(_spec(None, (3, 2), "open"), (2.0, 20, 0.0), id="open 3+2")]
+
+
+def _est_compare(driver, make, spec, args, tmp_path, weighted):
+    from test_estimators import _inputs, _run
+    nw, blocks, steps = 11, 3, 2
+    rmax, nb, kmax = args
+    pos, wts = _inputs(spec, nw, blocks, steps, weighted, 21)
+    sp, inp, outp = tmp_path / "spec.txt", tmp_path / "in.txt", tmp_path / "out.txt"
+    lines = [f"system {nw} {spec.n_particles} 0 " + (_arr(spec.lattice) if spec.lattice is not None else "0")]
+    for first, last, q, im in spec.groups:
+        lines.append(f"group {first} {last} {q!r} {1.0 / im!r}")
+    sp.write_text("\n".join(lines + ["end"]) + "\n")
+    parts = [repr(rmax), str(nb), repr(kmax), str(blocks), str(steps), "1" if weighted else "0"]
+    for m in range(blocks * steps):
+        parts += [_arr(pos[m]), _arr(wts[m] if weighted else np.ones(nw))]
+    inp.write_text("\n".join(parts) + "\n")
+    print(f"{os.path.basename(driver)} estimators: {spec.name} weighted={weighted}", flush=True)
+    r = subprocess.run([driver, "estimators", str(sp), str(inp), str(outp)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = _read_out(outp)
+    kw = dict(gofr=(rmax, nb))
+    if kmax > 0:
+        kw["sofk"] = kmax
+    w = make(spec, nw)
+    try:
+        want = _run(w, spec, kw, pos, wts, blocks, steps)
+    finally:
+        w.close()
+    assert set(got) == set(want)
+    for k, v in want.items():
+        assert np.array_equal(got[k], np.asarray(v, dtype=np.float64)), k
+
+
+@pytest.mark.parametrize("weighted", [False, True], ids=["vmc", "dmc"])
+@pytest.mark.parametrize("spec,args", _est_cases())
+def test_cpp_estimators_on_the_oracle(spec, args, weighted, tmp_path):
+    """The C++ Estimators class gives the Python module's series bit for
+    bit on the same library (here the oracle, on the CPU)."""
+    from backends import oracle_walkers
+    driver = DRIVER + "_oracle"
+    if not os.path.exists(driver):
+        pytest.fail("tests/cpp/hpp_driver_oracle is not built: run __graft_entry__.build()")
+    _est_compare(driver, oracle_walkers, spec, args, tmp_path, weighted)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("weighted", [False, True], ids=["vmc", "dmc"])
+@pytest.mark.parametrize("spec,args", _est_cases())
+def test_cpp_estimators_match_python_binding(spec, args, weighted, tmp_path):
+    _est_compare(_driver(), lambda s, n: gpu_walkers(s, n), spec, args, tmp_path, weighted)
