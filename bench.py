@@ -151,7 +151,7 @@ def serving_leg(precision=1):
 
 DMC_CFG = dict(n_up=37, n_dn=37, rs=40.0, target=16384, tau=2.0, steps=500, equilibration=100,
                vmc_sweeps=200, drummond_order=8, drummond_sigma=1e-3, drummond_seed=2, pos_seed=1,
-               seed=7, generations=10.0, alpha=2.0)
+               seed=7, generations=10.0, alpha=2.0, replicas=6)
 
 
 def dmc_spec(c=DMC_CFG):
@@ -179,31 +179,27 @@ def dmc_spec(c=DMC_CFG):
     return heg_spec(c["n_up"], c["n_dn"], c["rs"], drummond=dr)
 
 
-def dmc_leg(c=DMC_CFG):
-    """Fixed-node DMC on the N=74, rs=40 HEG in FP64 and in mixed form from
-    the same walkers and seeds: VMC equilibration, then `steps` DMC steps
-    (one drift-diffusion sweep each, bare reweighting, integer branching,
-    universal cutoff); E/N is the weighted mean of the steps after
-    `equilibration`, with a reblocked error bar."""
-    from alea_amd import dmc, statistics
+def dmc_run(spec, c, prec, pos_seed, seed):
+    """One fixed-node DMC run: uniform walkers from pos_seed, `vmc_sweeps`
+    VMC sweeps, then `steps` DMC steps (one drift-diffusion sweep each, bare
+    reweighting, integer branching, universal cutoff) on the streams of
+    `seed`.  Returns the per-step ensemble E/N after `equilibration`, the
+    populations and the walker-moves/s."""
+    from alea_amd import dmc
     from alea_amd.system import uniform_positions
     from alea_amd.walkers import gpu_walkers, seed as seed_walkers
-    spec = dmc_spec(c)
     n = spec.n_particles
     cap = 2 * c["target"] + 1
-    pos = uniform_positions(spec, cap, np.random.default_rng(c["pos_seed"]))
-    out = {}
-    for name, prec in (("fp64", 0), ("mixed", 1)):
-        print(f"bench: DMC N={n} rs={c['rs']:g} {name}", file=sys.stderr, flush=True)
-        w = gpu_walkers(spec, cap, prec)
-        w.set_positions(pos)
+    w = gpu_walkers(spec, cap, prec)
+    try:
+        w.set_positions(uniform_positions(spec, cap, np.random.default_rng(pos_seed)))
         w.evaluate()
         w.permute(np.arange(c["target"]))
-        seed_walkers(w, c["seed"])
+        seed_walkers(w, seed)
         w.vmc_sweeps(c["vmc_sweeps"], (0.5 * c["rs"]) ** 2)
-        seed_walkers(w, c["seed"], dmc.K_DMC_THREAD_STREAM)
+        seed_walkers(w, seed, dmc.K_DMC_THREAD_STREAM)
         d = dmc.DMC(w, c["tau"], c["target"], n_substeps=1, generations=c["generations"],
-                    cutoff="universal", alpha=c["alpha"], seed=c["seed"])
+                    cutoff="universal", alpha=c["alpha"], seed=seed)
         series, pops, moves = [], [], 0
         t0 = time.perf_counter()
         for step in range(c["steps"]):
@@ -212,25 +208,50 @@ def dmc_leg(c=DMC_CFG):
             if step >= c["equilibration"]:
                 series.append(e_sum / w_sum / n)
                 pops.append(w.nw)
-            if step % 100 == 99:
-                print(f"bench: DMC {name} step {step + 1}, population {w.nw}", file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
-        rb = statistics.reblock(series)
-        out[name] = dict(e_per_particle=rb.mean, error=rb.error, tau_int=rb.tau_int,
-                         plateau_found=rb.plateau_found, mean_population=float(np.mean(pops)),
-                         walker_moves_per_s=moves / wall, wall_s=wall)
-        out[name]["_series"] = series
+        return series, pops, moves / wall
+    finally:
         w.close()
-    de = out["mixed"]["e_per_particle"] - out["fp64"]["e_per_particle"]
-    sigma = math.hypot(out["mixed"]["error"], out["fp64"]["error"])
-    out["mixed_minus_fp64"] = de
-    out["combined_error"] = sigma
-    out["within_1e-5"] = abs(de) <= 1e-5
-    out["within_1_sigma"] = abs(de) <= sigma
-    # the two runs share every random stream, so their per-step difference
-    # is a correlated-sampling estimate of the precision bias
-    diff = statistics.reblock(np.subtract(out["mixed"].pop("_series"), out["fp64"].pop("_series")))
-    out["correlated_difference"] = dict(mean=diff.mean, error=diff.error, plateau_found=diff.plateau_found)
+
+
+def _mean_err(xs):
+    """Mean of independent replica values and its standard error."""
+    xs = np.asarray(xs, dtype=np.float64)
+    return float(xs.mean()), float(xs.std(ddof=1) / math.sqrt(len(xs))) if len(xs) > 1 else float("nan")
+
+
+def dmc_leg(c=DMC_CFG):
+    """Fixed-node DMC on the N=74, rs=40 HEG in FP64 and in mixed form.
+    `replicas` independent runs, replica r on walker seed pos_seed + r and
+    stream seed seed + r; FP64 and mixed share each replica's walkers and
+    streams.  E/N of one run is the mean of its steps after `equilibration`
+    (with its reblocked error); a form's E/N is the mean over replicas, and
+    its error is the spread of the replica means, which stays honest where
+    population-control correlations outlast one run's reblocking.  The
+    mixed - FP64 difference is taken per replica, then averaged the same way."""
+    from alea_amd import statistics
+    spec = dmc_spec(c)
+    out = {"fp64": {}, "mixed": {}}
+    per = {"fp64": [], "mixed": []}
+    for r in range(c["replicas"]):
+        for name, prec in (("fp64", 0), ("mixed", 1)):
+            series, pops, rate = dmc_run(spec, c, prec, c["pos_seed"] + r, c["seed"] + r)
+            rb = statistics.reblock(series)
+            per[name].append(dict(e_per_particle=rb.mean, reblocked_error=rb.error, tau_int=rb.tau_int,
+                                  plateau_found=rb.plateau_found, mean_population=float(np.mean(pops)),
+                                  final_population=pops[-1], walker_moves_per_s=rate))
+            print(f"bench: DMC replica {r} {name}: E/N {rb.mean:.7f} +- {rb.error:.1e}, "
+                  f"final population {pops[-1]}", file=sys.stderr, flush=True)
+    for name in per:
+        m, e = _mean_err([x["e_per_particle"] for x in per[name]])
+        out[name] = dict(e_per_particle=m, error=e,
+                         walker_moves_per_s=float(np.mean([x["walker_moves_per_s"] for x in per[name]])),
+                         replicas=per[name])
+    d, de = _mean_err([a["e_per_particle"] - b["e_per_particle"] for a, b in zip(per["mixed"], per["fp64"])])
+    out["mixed_minus_fp64"] = d
+    out["error"] = de
+    out["within_1e-5"] = abs(d) <= 1e-5
+    out["within_1_sigma"] = abs(d) <= de
     out["config"] = dict(c, system="N=74 (37+37) paramagnetic HEG, rs=40, plane-wave nodes, Pade + Drummond, Ewald")
     return out
 
