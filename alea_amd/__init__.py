@@ -35,6 +35,7 @@ def load() -> CApi:
         for name, args in (("synchronize", [C.c_void_p]),
                            ("last_sweep_ms", [C.c_void_p, C.POINTER(C.c_double)]),
                            ("set_refresh_interval", [C.c_void_p, C.c_int]),
+                           ("set_sweep_graphs", [C.c_void_p, C.c_int]),
                            ("query", [C.c_void_p, C.POINTER(C.c_int)])):
             f = getattr(lib, "alea_" + name)
             f.restype = C.c_int

@@ -201,6 +201,27 @@ struct alea_system {
     int refresh_every = 2;        // sweeps between batched inverse refreshes
     long long since_refresh = 0;  // sweeps (or np accept calls) since the last refresh
 
+    // One captured block of device-stream sweeps (alea_set_sweep_graphs):
+    // the key holds everything its launches bake in, compared bytewise; a
+    // block whose key differs re-captures.
+    struct SweepGraphKey {
+        int n_sweeps, nw, np, dmc, refresh_every, rows_max, precision, cplx;
+        long long since_refresh;
+        double tau;
+        const double* dtape;
+        const DevModel* dmodel;
+        size_t walker_lds, refresh_lds;
+        DevState st;
+    };
+    bool sweep_graphs = false;
+    hipGraphExec_t graph_exec = nullptr;
+    SweepGraphKey graph_key{};
+    long long graph_since_after = 0;   // since_refresh after the captured block
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        graph_exec = nullptr;
+    }
+
     // Estimators (pair correlation, structure factor): the block's running
     // sums on the device, [gofr: n_func*n_bins][sofk: n_pairs*nk][rho re:
     // G*nk][rho im: G*nk][weight sum]
@@ -537,6 +558,7 @@ int alea_destroy(alea_system* s) {
     if (st) (void)hipStreamSynchronize(st);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
+    s->drop_graph();
     if (s->dmodel) (void)hipFreeAsync(s->dmodel, st);
     delete s;   // the buffers free on the stream, which must outlive them
     if (st) {
@@ -1049,6 +1071,82 @@ int alea_accept(alea_system* s, int p, const unsigned char* mask) {
     });
 }
 
+// One sweep of the loop below: its k_draw on the device streams, the sweep,
+// then the refresh when it is due.  With `events` the draw and sweep are
+// bracketed by the system's timing events (not inside a graph capture).
+static void launch_one_sweep(alea_system* s, const SweepParams& P, int i, int stride, bool events) {
+    SweepParams Pi = P;
+    Pi.n_sweeps = 1;
+    if (events) hip_check(hipEventRecord(s->ev0, s->stream), "event record");
+    if (P.tape) {
+        Pi.tape = P.tape + i * static_cast<size_t>(s->np) * 3 * stride;
+    } else {
+        launch_draw(s->st, s->np, s->nw, s->dtape.p, stride, s->stream);
+        hip_check(hipGetLastError(), "launch draw");
+        Pi.tape = s->dtape.p;
+        Pi.tape_stride = stride;
+    }
+    launch_sweep(s->dmodel, s->st, Pi, s->nw, s->mixed(), s->cplx, s->rows_max, s->walker_lds, s->stream);
+    hip_check(hipGetLastError(), "launch sweep");
+    if (events) hip_check(hipEventRecord(s->ev1, s->stream), "event record");
+    if (++s->since_refresh >= s->refresh_every) s->refresh();
+}
+
+static void run_sweeps_eager(alea_system* s, const SweepParams& P, int stride, int n_sweeps) {
+    for (int i = 0; i < n_sweeps; ++i) launch_one_sweep(s, P, i, stride, true);
+}
+
+// The same launches captured once into a hipGraph and replayed while nothing
+// they bake in changes (sweep count, step, walker count, refresh phase,
+// device buffers).  The capture itself launches nothing: since_refresh is
+// advanced by the capture as by the eager loop and restored on a replay.
+// Nothing inside the graph is a memset (the acceptance counters are cleared
+// before it, outside).
+static void run_sweep_graph(alea_system* s, const SweepParams& P, int stride, int n_sweeps) {
+    alea_system::SweepGraphKey k;
+    std::memset(&k, 0, sizeof k);
+    k.n_sweeps = n_sweeps;
+    k.nw = s->nw;
+    k.np = s->np;
+    k.dmc = P.dmc;
+    k.rows_max = s->rows_max;
+    k.precision = s->precision;
+    k.cplx = s->cplx ? 1 : 0;
+    k.refresh_every = s->refresh_every;
+    k.since_refresh = s->since_refresh;
+    k.tau = P.tau;
+    k.dtape = s->dtape.p;
+    k.dmodel = s->dmodel;
+    k.walker_lds = s->walker_lds;
+    k.refresh_lds = s->refresh_lds;
+    std::memcpy(&k.st, &s->st, sizeof(DevState));
+    if (!s->graph_exec || std::memcmp(&k, &s->graph_key, sizeof k) != 0) {
+        s->drop_graph();
+        hipGraph_t graph = nullptr;
+        hip_check(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed), "begin capture");
+        try {
+            for (int i = 0; i < n_sweeps; ++i) launch_one_sweep(s, P, i, stride, false);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            s->since_refresh = k.since_refresh;
+            throw;
+        }
+        hip_check(hipStreamEndCapture(s->stream, &graph), "end capture");
+        const hipError_t e = hipGraphInstantiate(&s->graph_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) s->graph_exec = nullptr;
+        hip_check(e, "graph instantiate");
+        s->graph_key = k;
+        s->graph_since_after = s->since_refresh;
+    }
+    hip_check(hipEventRecord(s->ev0, s->stream), "event record");
+    hip_check(hipGraphLaunch(s->graph_exec, s->stream), "graph launch");
+    hip_check(hipEventRecord(s->ev1, s->stream), "event record");
+    s->since_refresh = s->graph_since_after;
+}
+
 static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape, int stride,
                        bool dmc, long long* n_accepted) {
     if (!s->evaluated) throw InvalidArg("evaluate first");
@@ -1079,27 +1177,17 @@ static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* t
     // One launch per sweep (after its k_draw on the device streams); the
     // batched inverse refresh runs every refresh_every sweeps (at most
     // refresh_every * Nb Sherman-Morrison updates between rebuilds).  The
-    // events bracket the draw and sweep launches only.
-    const size_t tape_step = static_cast<size_t>(s->np) * 3 * stride;
-    for (int i = 0; i < n_sweeps; ++i) {
-        SweepParams Pi = P;
-        Pi.n_sweeps = 1;
-        hip_check(hipEventRecord(s->ev0, s->stream), "event record");
-        if (P.tape) {
-            Pi.tape = P.tape + i * tape_step;
-        } else {
-            launch_draw(s->st, s->np, s->nw, s->dtape.p, stride, s->stream);
-            hip_check(hipGetLastError(), "launch draw");
-            Pi.tape = s->dtape.p;
-            Pi.tape_stride = stride;
-        }
-        launch_sweep(s->dmodel, s->st, Pi, s->nw, s->mixed(), s->cplx, s->rows_max, s->walker_lds, s->stream);
-        hip_check(hipGetLastError(), "launch sweep");
-        hip_check(hipEventRecord(s->ev1, s->stream), "event record");
-        if (++s->since_refresh >= s->refresh_every) s->refresh();
+    // events bracket the draw and sweep launches only, or with sweep graphs
+    // the whole block.
+    if (!P.tape && s->sweep_graphs && n_sweeps > 0) {
+        run_sweep_graph(s, P, stride, n_sweeps);
+        s->timed = true;
+    } else {
+        run_sweeps_eager(s, P, stride, n_sweeps);
+        s->timed = n_sweeps > 0;
     }
-    s->timed = n_sweeps > 0;
     if (n_accepted) {
+
         s->sync();
         std::vector<long long> a(s->nw);
         hip_check(copy_on(s->stream, a.data(), s->nacc.p, s->nw * sizeof(long long), hipMemcpyDeviceToHost), "nacc");
@@ -1150,6 +1238,14 @@ int alea_set_refresh_interval(alea_system* s, int sweeps) {
         Sys(s);
         if (sweeps < 1) throw InvalidArg("refresh interval must be at least one sweep");
         s->refresh_every = sweeps;
+    });
+}
+
+int alea_set_sweep_graphs(alea_system* s, int on) {
+    return guarded([&] {
+        Sys(s);
+        s->sweep_graphs = on != 0;
+        if (!s->sweep_graphs) s->drop_graph();
     });
 }
 

@@ -9,7 +9,8 @@ active slot is free; admitted jobs live on the GPU as their own walker batch
 (their own alea_system, whose work runs on its own non-blocking stream), so
 the blocks of all active jobs are in flight on the device together and a new
 job starts while others are mid-run.  Each block is queued asynchronously
-(alea_vmc_sweeps with no acceptance read-back) and polled (alea_query); when
+(alea_vmc_sweeps with no acceptance read-back, replayed as one captured
+hipGraph per job) and polled (alea_query); when
 it completes, its local energy is measured and the job's next block is
 queued.  A finished job frees its memory and slot at once.
 
@@ -29,7 +30,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .system import dipole_spec, heg_spec, uniform_positions
-from .walkers import FP64, MIXED, gpu_walkers, seed as seed_walkers
+from .walkers import FP64, MIXED, gpu_walkers, seed as seed_walkers, set_sweep_graphs
 
 
 @dataclass
@@ -127,7 +128,7 @@ def poisson_jobs(n_jobs: int, rate: float, seed: int = 4, sizes=(26, 58, 74, 122
 class GpuRunner:
     """One admitted job on the GPU: its walker batch and the block in flight."""
 
-    def __init__(self, job: Job):
+    def __init__(self, job: Job, graphs: bool = True):
         if job.kind == "dipoles":
             spec = dipole_spec(job.n_particles, density=1.0 / (math.pi * job.rs * job.rs))
         else:
@@ -137,6 +138,8 @@ class GpuRunner:
         self.w.set_positions(uniform_positions(spec, job.n_walkers, np.random.default_rng(job.seed)))
         self.w.evaluate()
         seed_walkers(self.w, job.seed)
+        # every block has the same launches: one captured hipGraph per job
+        set_sweep_graphs(self.w, graphs)
 
     def start_block(self):
         self.w.api("vmc_sweeps", self.w.h, self.job.sweeps_per_block, self.job.tau, None, None)

@@ -217,3 +217,10 @@ def gpu_walkers(spec: Spec, nw: int, precision: int = FP64) -> Walkers:
 
 def seed(w: Walkers, seed_value: int, stream: int = 0x766d632d74687264):
     w.api("seed", w.h, seed_value, stream)
+
+
+def set_sweep_graphs(w: Walkers, on: bool = True):
+    """Device-stream sweeps of `w` replay one captured hipGraph per block of
+    sweeps (alea_set_sweep_graphs); results are bit-identical to the eager
+    launches."""
+    w.api("set_sweep_graphs", w.h, 1 if on else 0)

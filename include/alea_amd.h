@@ -159,6 +159,15 @@ int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape
  * estimate above 1e8) still rebuild inside the sweep. */
 int alea_set_refresh_interval(alea_system* s, int sweeps);
 
+/* on != 0: alea_vmc_sweeps / alea_dmc_sweeps on the device streams (tape
+ * NULL) replay one captured hipGraph per block of n_sweeps (draw, sweep and
+ * the periodic refresh of every sweep), re-captured when the sweep count,
+ * step, walker count, refresh phase or any device buffer changes.  Results
+ * are bit-identical to the eager launches; alea_last_sweep_ms then times the
+ * whole block.  No counterpart in the reference: the serving loop's launch
+ * path for repeated blocks. */
+int alea_set_sweep_graphs(alea_system* s, int on);
+
 /* Waits for the system's stream (hipStreamSynchronize). */
 int alea_synchronize(alea_system* s);
 /* Non-blocking: *done = 1 when all work queued on the system's stream has

@@ -39,3 +39,64 @@ def test_poisson_stream_completes():
     # a few sweeps from a uniform start: the McMillan core still dominates
     # E_L of close pairs, so only finiteness is a property here
     assert all(np.isfinite(e) for j in jobs if j.kind == "dipoles" for e in j.block_energy)
+
+
+def _graph_pair(precision, nw=256):
+    from alea_amd.system import heg_spec, uniform_positions
+    from alea_amd.walkers import gpu_walkers, seed, set_sweep_graphs
+    spec = heg_spec(13, 13, 30.0, twist=(0.13, 0.27))
+    pos = uniform_positions(spec, nw, np.random.default_rng(5))
+    pair = []
+    for graphs in (False, True):
+        w = gpu_walkers(spec, nw, precision)
+        w.set_positions(pos)
+        w.evaluate()
+        seed(w, 11)
+        set_sweep_graphs(w, graphs)
+        pair.append(w)
+    return spec, pair
+
+
+def _same_state(a, b):
+    assert np.array_equal(a.positions(), b.positions())
+    assert np.array_equal(a.log_psi(), b.log_psi())
+    ea, eb = a.local_energy()[0], b.local_energy()[0]
+    assert np.array_equal(ea, eb)
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_sweep_graph_replays_match_eager_sweeps(precision):
+    """Blocks of device-stream sweeps replayed from a captured hipGraph give
+    the eager launches' walkers bit for bit: repeated replays with eager work
+    (energies, reads) between them, blocks of 3 sweeps that shift the refresh
+    phase (refresh every 2 sweeps), and a permutation that halves the walker
+    count, all of which must re-capture or replay the right graph."""
+    spec, (eager, graph) = _graph_pair(precision)
+    tau = 15.0 ** 2
+    for block in range(4):
+        na = eager.vmc_sweeps(3, tau)
+        nb = graph.vmc_sweeps(3, tau)
+        assert na == nb > 0, (block, na, nb)
+        _same_state(eager, graph)
+    src = np.arange(0, eager.nw, 2)
+    for w in (eager, graph):
+        w.permute(src)
+    for block in range(2):
+        assert eager.vmc_sweeps(2, tau) == graph.vmc_sweeps(2, tau)
+        _same_state(eager, graph)
+    for w in (eager, graph):
+        w.close()
+
+
+def test_sweep_graph_dmc_sweeps_match_eager():
+    from alea_amd import dmc
+    from alea_amd.walkers import seed
+    spec, (eager, graph) = _graph_pair(FP64)
+    for w in (eager, graph):
+        seed(w, 11, dmc.K_DMC_THREAD_STREAM)
+        w.dmc_begin()
+    for block in range(3):
+        assert eager.dmc_sweeps(2, 2.0) == graph.dmc_sweeps(2, 2.0)
+        _same_state(eager, graph)
+    for w in (eager, graph):
+        w.close()
