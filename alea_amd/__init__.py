@@ -41,5 +41,9 @@ def load() -> CApi:
             f.restype = C.c_int
             f.argtypes = args
             api.fns[name] = f
+        f = lib.alea_vmc_sweeps_fused
+        f.restype = C.c_int
+        f.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_double)]
+        api.fns["vmc_sweeps_fused"] = f
         _api = api
     return _api

@@ -30,6 +30,9 @@ void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, 
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
                   bool cplx, int rows_max, size_t lds, hipStream_t st);
 void launch_draw(const DevState& s, int np, int nw, double* out, int stride, hipStream_t st);
+int sweep_tile_rows(bool mixed, bool cplx, int rows_max);
+void launch_sweep_fused(const FusedArgs& a, int total_nw, bool mixed, bool cplx, int rows_max, size_t lds,
+                        hipStream_t st);
 void launch_call(const DevModel* m, const DevState& s, const DevCall& c, int p, int mode, int nw,
                  bool mixed, bool cplx, size_t lds, hipStream_t st);
 void launch_measure(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, size_t lds,
@@ -222,6 +225,11 @@ struct alea_system {
         graph_exec = nullptr;
     }
 
+    // Fused launches (alea_vmc_sweeps_fused): recorded on a member's stream
+    // for the lead stream to wait on, or on the lead stream after the block
+    // for the members to wait on.
+    hipEvent_t fused_join = nullptr;
+
     // Estimators (pair correlation, structure factor): the block's running
     // sums on the device, [gofr: n_func*n_bins][sofk: n_pairs*nk][rho re:
     // G*nk][rho im: G*nk][weight sum]
@@ -372,9 +380,9 @@ struct alea_system {
 
     void sync() { hip_check(hipStreamSynchronize(stream), "kernel"); }
 
-    void refresh() {
+    void refresh(hipStream_t on = nullptr) {
         if (host.n_blocks == 0) return;
-        launch_refresh(dmodel, st, nw, host.n_blocks, st.nb_max, mixed(), cplx, refresh_lds, stream);
+        launch_refresh(dmodel, st, nw, host.n_blocks, st.nb_max, mixed(), cplx, refresh_lds, on ? on : stream);
         hip_check(hipGetLastError(), "launch refresh");
         since_refresh = 0;
     }
@@ -559,6 +567,7 @@ int alea_destroy(alea_system* s) {
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     s->drop_graph();
+    if (s->fused_join) (void)hipEventDestroy(s->fused_join);
     if (s->dmodel) (void)hipFreeAsync(s->dmodel, st);
     delete s;   // the buffers free on the stream, which must outlive them
     if (st) {
@@ -1238,6 +1247,78 @@ int alea_set_refresh_interval(alea_system* s, int sweeps) {
         Sys(s);
         if (sweeps < 1) throw InvalidArg("refresh interval must be at least one sweep");
         s->refresh_every = sweeps;
+    });
+}
+
+int alea_vmc_sweeps_fused(alea_system* const* systems, int n_systems, int n_sweeps, const double* taus) {
+    return guarded([&] {
+        if (!systems || !taus) throw InvalidArg("null systems or step array");
+        if (n_systems < 1 || n_systems > kMaxFused)
+            throw InvalidArg("a fused launch takes 1 to " + std::to_string(kMaxFused) + " systems");
+        if (n_sweeps < 0) throw InvalidArg("negative sweep count");
+        alea_system* lead = Sys(systems[0]);
+        const int nr = sweep_tile_rows(lead->mixed(), lead->cplx, lead->rows_max);
+        size_t lds = 0;
+        long long total = 0;
+        for (int i = 0; i < n_systems; ++i) {
+            alea_system* s = Sys(systems[i]);
+            for (int k = 0; k < i; ++k)
+                if (systems[k] == s) throw InvalidArg("a system appears twice in one fused launch");
+            if (!s->evaluated) throw InvalidArg("evaluate first");
+            if (s->precision != lead->precision || s->cplx != lead->cplx ||
+                sweep_tile_rows(s->mixed(), s->cplx, s->rows_max) != nr)
+                throw InvalidArg("fused systems must share the precision, the complex form and the register tile");
+            lds = std::max(lds, s->walker_lds);
+            total += s->nw;
+        }
+        if (total > std::numeric_limits<int>::max()) throw InvalidArg("too many walkers for one launch");
+        if (n_sweeps == 0 || total == 0) return;
+        hipStream_t st = lead->stream;
+        // every member's pending work (and its buffers' allocation) comes
+        // first: the lead stream waits for each member stream
+        FusedArgs args;
+        args.n = n_systems;
+        for (int i = 0; i < n_systems; ++i) {
+            alea_system* s = systems[i];
+            s->set_model_nw(s->nw);
+            const size_t len = static_cast<size_t>(s->np) * 3 * s->nw;
+            if (s->dtape.n < len) s->dtape.alloc(static_cast<size_t>(s->np) * 3 * s->capacity, s->stream);
+            if (!s->fused_join) hip_check(hipEventCreateWithFlags(&s->fused_join, hipEventDisableTiming), "event");
+            if (i > 0) {
+                hip_check(hipEventRecord(s->fused_join, s->stream), "event record");
+                hip_check(hipStreamWaitEvent(st, s->fused_join, 0), "stream wait");
+            }
+            FusedJob& J = args.job[i];
+            J.m = s->dmodel;
+            J.s = s->st;
+            J.p.n_sweeps = 1;
+            J.p.tau = taus[i];
+            J.p.tape = s->dtape.p;
+            J.p.tape_stride = s->nw;
+            J.p.dmc = 0;
+            J.first = i == 0 ? 0 : args.job[i - 1].first + systems[i - 1]->nw;
+            J.nw = s->nw;
+            s->energies_valid = false;
+        }
+        // per sweep: each member's draws, one fused sweep, each member's
+        // refresh when it is due
+        for (int k = 0; k < n_sweeps; ++k) {
+            for (int i = 0; i < n_systems; ++i) {
+                alea_system* s = systems[i];
+                launch_draw(s->st, s->np, s->nw, s->dtape.p, s->nw, st);
+                hip_check(hipGetLastError(), "launch draw");
+            }
+            launch_sweep_fused(args, static_cast<int>(total), lead->mixed(), lead->cplx, lead->rows_max, lds, st);
+            hip_check(hipGetLastError(), "launch fused sweep");
+            for (int i = 0; i < n_systems; ++i) {
+                alea_system* s = systems[i];
+                if (++s->since_refresh >= s->refresh_every) s->refresh(st);
+            }
+        }
+        // each member's later work waits for the fused block
+        hip_check(hipEventRecord(lead->fused_join, st), "event record");
+        for (int i = 1; i < n_systems; ++i)
+            hip_check(hipStreamWaitEvent(systems[i]->stream, lead->fused_join, 0), "stream wait");
     });
 }
 

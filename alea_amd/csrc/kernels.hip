@@ -1902,6 +1902,69 @@ __global__ __launch_bounds__(GX * 16) __attribute__((amdgpu_waves_per_eu(NP <= 6
     }
 }
 
+// k_sweep's body for one walker w of a given system, for the fused launch
+// below.  k_sweep keeps its own copy: compiled through this function its
+// register allocation changes (more spills in the 64-row tile).
+template <typename T, bool CPLX, bool DMC, int NR>
+__device__ __attribute__((always_inline)) inline void sweep_walker(const DevModel* __restrict__ Mp, const DevState& S,
+                                                                   const SweepParams& P, int w) {
+    const DevModel& M = *Mp;
+    extern __shared__ double lds_[];
+    Shared sh = carve(M, lds_);
+    const int lane = threadIdx.x;
+    Walker<T, CPLX, NR> wk(M, S, sh, w);
+    wk.load();
+    long long nacc = 0;
+    double rr_p = 0.0, rr_a = 0.0;
+    const int np = M.np;
+    // The random numbers come from a tape, [sweep][p][3][stride]: the
+    // caller's, or the one k_draw wrote from the walker's own stream.  The
+    // next move's three are loaded one move ahead, so the wait for them
+    // overlaps the current move.
+    const int ts = P.tape_stride;
+    auto tape_row = [&](int s, int p) { return P.tape + (static_cast<size_t>(s) * np + p) * 3 * ts; };
+    double nxt0 = tape_row(0, 0)[w], nxt1 = tape_row(0, 0)[ts + w], nxtu = tape_row(0, 0)[2 * ts + w];
+    for (int s = 0; s < P.n_sweeps; ++s) {
+        for (int p = 0; p < np; ++p) {
+            const double xi0 = nxt0, xi1 = nxt1, uni = nxtu;
+            {
+                const bool more = p + 1 < np || s + 1 < P.n_sweeps;
+                const double* row = more ? (p + 1 < np ? tape_row(s, p + 1) : tape_row(s + 1, 0)) : tape_row(s, p);
+                nxt0 = row[w];
+                nxt1 = row[ts + w];
+                nxtu = row[2 * ts + w];
+            }
+            const double tm = P.tau * M.inv_mass[p];
+            const double sigma = sqrt(tm);
+            const double x0 = wk.sh.pos[2 * p], x1 = wk.sh.pos[2 * p + 1];
+            double gold[2] = {0.0, 0.0}, scale = 0.0;
+            if constexpr (DMC) {
+                wk.grad_of(p, gold);
+                const double v2 = gold[0] * gold[0] + gold[1] * gold[1];
+                scale = v2 < 1e-10 ? tm : (-1.0 + sqrt(1.0 + 2.0 * tm * v2)) / v2;
+            }
+            const double nx0 = x0 + scale * gold[0] + sigma * xi0;
+            const double nx1 = x1 + scale * gold[1] + sigma * xi1;
+            if constexpr (DMC) {
+                const double d0 = nx0 - x0, d1 = nx1 - x1;
+                rr_p += tm * d0 * d0 + tm * d1 * d1;
+            }
+            const bool accepted =
+                wk.template move<DMC>(p, nx0, nx1, uni, 0, nullptr, P.tau, DMC, scale, gold, rr_a);
+            if (accepted) ++nacc;
+        }
+    }
+    wk.spill();
+    wk.store();
+    if (lane == 0) {
+        S.n_accepted[w] += nacc;
+        if constexpr (DMC) {
+            S.rr[w * 2] += rr_p;
+            S.rr[w * 2 + 1] += rr_a;
+        }
+    }
+}
+
 // Fused sweeps: every particle of the walker, n_sweeps times.  NR > 0 holds
 // the block being swept in registers (see Walker).
 template <typename T, bool CPLX, bool DMC, int NR>
@@ -1962,6 +2025,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             S.rr[w * 2 + 1] += rr_a;
         }
     }
+}
+
+// Fused VMC sweeps of several systems that share the determinant precision,
+// the complex form and the register tile: workgroup g runs walker
+// g - jobs[j].first of system j, with system j's model, state and
+// parameters, exactly as k_sweep would run it alone.
+template <typename T, bool CPLX, int NR>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_sweep_fused(
+    const FusedArgs A) {
+    const int g = blockIdx.x;
+    int j = 0;
+    while (j + 1 < A.n && g >= A.job[j + 1].first) ++j;
+    const FusedJob& J = A.job[j];
+    if (g - J.first >= J.nw) return;   // cannot happen: the grid is the sum of the walker counts
+    sweep_walker<T, CPLX, false, NR>(J.m, J.s, J.p, g - J.first);
 }
 
 // One sweep of random numbers per walker from its xoshiro256++ stream, in
@@ -2350,6 +2428,36 @@ void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, in
         else launch_sweep_t<double, false, false>(m, s, p, nw, nr, lds, st);
     }
 }
+static_assert(sizeof(FusedArgs) <= 4096, "the fused job table must fit the kernel arguments");
+
+template <typename T, bool CPLX>
+static void launch_sweep_fused_t(const FusedArgs& a, int total_nw, int nr, size_t lds, hipStream_t st) {
+    const dim3 g(total_nw), t(64);
+    switch (nr) {
+        case 16: hipLaunchKernelGGL((k_sweep_fused<T, CPLX, 16>), g, t, lds, st, a); break;
+        case 32: hipLaunchKernelGGL((k_sweep_fused<T, CPLX, 32>), g, t, lds, st, a); break;
+        case 64:
+            if constexpr (64 * kColWords<T, CPLX> <= 128) {
+                hipLaunchKernelGGL((k_sweep_fused<T, CPLX, 64>), g, t, lds, st, a);
+                break;
+            }
+            [[fallthrough]];
+        default: hipLaunchKernelGGL((k_sweep_fused<T, CPLX, 0>), g, t, lds, st, a); break;
+    }
+}
+
+void launch_sweep_fused(const FusedArgs& a, int total_nw, bool mixed, bool cplx, int rows_max, size_t lds,
+                        hipStream_t st) {
+    const int nr = sweep_tile_rows(mixed, cplx, rows_max);
+    if (mixed) {
+        if (cplx) launch_sweep_fused_t<float, true>(a, total_nw, nr, lds, st);
+        else launch_sweep_fused_t<float, false>(a, total_nw, nr, lds, st);
+    } else {
+        if (cplx) launch_sweep_fused_t<double, true>(a, total_nw, nr, lds, st);
+        else launch_sweep_fused_t<double, false>(a, total_nw, nr, lds, st);
+    }
+}
+
 void launch_draw(const DevState& s, int np, int nw, double* out, int stride, hipStream_t st) {
     hipLaunchKernelGGL(k_draw, dim3((nw + 63) / 64), dim3(64), 0, st, s, np, nw, out, stride);
 }

@@ -162,4 +162,20 @@ struct SweepParams {
     int dmc = 0;
 };
 
+// One system of a fused VMC sweep launch: its model, state and sweep
+// parameters, and the range of the launch's workgroups that are its walkers.
+struct FusedJob {
+    const DevModel* m = nullptr;
+    DevState s;
+    SweepParams p;
+    int first = 0, nw = 0;
+};
+constexpr int kMaxFused = 8;    // systems in one fused launch
+// The job table travels as the kernel argument (invariant kernel-argument
+// memory), about 1.9 KB.
+struct FusedArgs {
+    FusedJob job[kMaxFused];
+    int n = 0;
+};
+
 }  // namespace alea_amd

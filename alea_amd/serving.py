@@ -30,7 +30,9 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .system import dipole_spec, heg_spec, uniform_positions
-from .walkers import FP64, MIXED, gpu_walkers, seed as seed_walkers, set_sweep_graphs
+from .walkers import FP64, MIXED, gpu_walkers, seed as seed_walkers, set_sweep_graphs, vmc_sweeps_fused
+
+MAX_FUSED = 8    # systems in one fused launch (kMaxFused of the C-ABI)
 
 
 @dataclass
@@ -134,6 +136,7 @@ class GpuRunner:
         else:
             spec = heg_spec(job.n_particles // 2, job.n_particles // 2, job.rs, twist=job.twist)
         self.job = job
+        self.complex_orbitals = bool(spec.slater_twisted)
         self.w = gpu_walkers(spec, job.n_walkers, job.precision)
         self.w.set_positions(uniform_positions(spec, job.n_walkers, np.random.default_rng(job.seed)))
         self.w.evaluate()
@@ -143,6 +146,19 @@ class GpuRunner:
 
     def start_block(self):
         self.w.api("vmc_sweeps", self.w.h, self.job.sweeps_per_block, self.job.tau, None, None)
+
+    def bucket_key(self):
+        """Jobs with equal keys share fused launches: the same kind and
+        particle count (hence block sizes and register tile), precision,
+        orbital form and block length."""
+        j = self.job
+        return (j.kind, j.n_particles, j.precision, self.complex_orbitals, j.sweeps_per_block)
+
+    @staticmethod
+    def start_fused(runners):
+        """One block of every runner's job as fused launches
+        (alea_vmc_sweeps_fused), queued on the first runner's stream."""
+        vmc_sweeps_fused([r.w for r in runners], runners[0].job.sweeps_per_block, [r.job.tau for r in runners])
 
     def done(self) -> bool:
         import ctypes as C
@@ -163,10 +179,18 @@ def _pct(x, q):
 
 
 class Server:
-    """Admission by HBM footprint and free slots, FIFO by arrival."""
+    """Admission by HBM footprint and free slots, FIFO by arrival.
+
+    With fuse (and a runner that has bucket_key and start_fused), active jobs
+    of one size bucket form a group whose blocks run as fused launches: one
+    sweep kernel over the walkers of every member.  A job admitted while its
+    bucket's group is mid-block joins the group at the group's next block;
+    a finished member leaves at once.  Groups of different buckets run on
+    their own streams and overlap on the device."""
 
     def __init__(self, hbm_budget_bytes: int = 64 << 30, max_active: int = 8, runner=GpuRunner,
-                 clock=time.perf_counter, poll_interval: float = 1e-4, on_finish=None):
+                 clock=time.perf_counter, poll_interval: float = 1e-4, on_finish=None, fuse: bool = True):
+        self.fuse = fuse and hasattr(runner, "start_fused") and hasattr(runner, "bucket_key")
         self.budget = hbm_budget_bytes
         self.max_active = max_active
         self.runner = runner
@@ -174,9 +198,12 @@ class Server:
         self.poll_interval = poll_interval
         self.peak_bytes = 0
         self.peak_active = 0
+        self.peak_fused = 1     # most jobs in one fused launch
         self.on_finish = on_finish      # called with each finished job (progress reporting)
 
     def run(self, jobs: list) -> dict:
+        if self.fuse:
+            return self._run_fused(jobs)
         pending = sorted(jobs, key=lambda j: (j.arrival, j.job_id))
         for j in pending:
             if job_footprint(j) > self.budget:
@@ -228,7 +255,86 @@ class Server:
                         time.sleep(min(wait, 0.05))
                 else:
                     time.sleep(self.poll_interval)
-        wall = self.clock() - t0
+        return self._stats(done_jobs, self.clock() - t0)
+
+    def _run_fused(self, jobs: list) -> dict:
+        pending = sorted(jobs, key=lambda j: (j.arrival, j.job_id))
+        for j in pending:
+            if job_footprint(j) > self.budget:
+                raise ValueError(f"job {j.job_id} needs more than the whole HBM budget")
+        groups = []          # {"key", "members": [[runner, bytes, block_start]], "joining": [...]}
+        used = 0
+        n_active = 0
+        t0 = self.clock()
+        done_jobs = []
+
+        def launch(g):
+            g["members"] += g["joining"]
+            g["joining"] = []
+            now = self.clock() - t0
+            for e in g["members"]:
+                e[2] = now
+            self.peak_fused = max(self.peak_fused, len(g["members"]))
+            self.runner.start_fused([e[0] for e in g["members"]])
+
+        while pending or groups:
+            now = self.clock() - t0
+            while pending and pending[0].arrival <= now and n_active < self.max_active:
+                j = pending[0]
+                need = job_footprint(j)
+                if used + need > self.budget:
+                    break
+                pending.pop(0)
+                r = self.runner(j)
+                j.admitted = self.clock() - t0
+                entry = [r, need, None]
+                key = r.bucket_key()
+                g = next((g for g in groups if g["key"] == key
+                          and len(g["members"]) + len(g["joining"]) < MAX_FUSED), None)
+                if g is None:
+                    g = {"key": key, "members": [], "joining": [entry]}
+                    groups.append(g)
+                    launch(g)
+                else:
+                    g["joining"].append(entry)      # joins at the group's next block
+                used += need
+                n_active += 1
+                self.peak_bytes = max(self.peak_bytes, used)
+                self.peak_active = max(self.peak_active, n_active)
+            progressed = False
+            for g in list(groups):
+                if not g["members"][0][0].done():     # the block runs on the first member's stream
+                    continue
+                progressed = True
+                end = self.clock() - t0
+                for e in list(g["members"]):
+                    r, need, start = e
+                    j = r.job
+                    j.block_latency.append(end - start)
+                    j.block_energy.append(r.block_energy())
+                    if len(j.block_energy) >= j.n_blocks:
+                        j.finished = self.clock() - t0
+                        r.close()
+                        g["members"].remove(e)
+                        used -= need
+                        n_active -= 1
+                        done_jobs.append(j)
+                        if self.on_finish:
+                            self.on_finish(j)
+                if g["members"] or g["joining"]:
+                    launch(g)
+                else:
+                    groups.remove(g)
+            if not progressed:
+                if not groups and pending:
+                    wait = pending[0].arrival - (self.clock() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.05))
+                else:
+                    time.sleep(self.poll_interval)
+        return self._stats(done_jobs, self.clock() - t0)
+
+    def _stats(self, done_jobs, wall) -> dict:
         blocks = [x for j in done_jobs for x in j.block_latency]
         requests = [j.finished - j.arrival for j in done_jobs]
         waits = [j.admitted - j.arrival for j in done_jobs]
@@ -238,4 +344,4 @@ class Server:
                     p50_block_ms=1e3 * _pct(blocks, 50), p99_block_ms=1e3 * _pct(blocks, 99),
                     p50_request_s=_pct(requests, 50), p99_request_s=_pct(requests, 99),
                     p50_admission_wait_s=_pct(waits, 50), peak_active=self.peak_active,
-                    peak_bytes=self.peak_bytes, budget_bytes=self.budget)
+                    peak_fused=self.peak_fused, peak_bytes=self.peak_bytes, budget_bytes=self.budget)

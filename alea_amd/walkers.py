@@ -224,3 +224,18 @@ def set_sweep_graphs(w: Walkers, on: bool = True):
     sweeps (alea_set_sweep_graphs); results are bit-identical to the eager
     launches."""
     w.api("set_sweep_graphs", w.h, 1 if on else 0)
+
+
+def vmc_sweeps_fused(batches, n_sweeps: int, taus):
+    """n_sweeps VMC sweeps of several walker batches on their device streams
+    as fused launches (alea_vmc_sweeps_fused): one sweep kernel per sweep over
+    the walkers of all of them.  The batches share the precision, the complex
+    form and the block sizes; queued asynchronously on the first batch's
+    stream, which alea_query on that batch then covers."""
+    import ctypes as C
+    if not batches:
+        return
+    api = batches[0].api
+    hs = (C.c_void_p * len(batches))(*[b.h for b in batches])
+    ts = (C.c_double * len(batches))(*[float(t) for t in taus])
+    api("vmc_sweeps_fused", hs, len(batches), n_sweeps, ts)

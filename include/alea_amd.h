@@ -159,6 +159,21 @@ int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape
  * estimate above 1e8) still rebuild inside the sweep. */
 int alea_set_refresh_interval(alea_system* s, int sweeps);
 
+/* n_sweeps VMC sweeps (as alea_vmc_sweeps on the device streams, tape NULL,
+ * no acceptance read-back) of n_systems systems as fused launches: per sweep
+ * each system's random numbers, then ONE sweep kernel over the walkers of all
+ * of them (each workgroup runs one walker of one system with that system's
+ * model, state and step taus[i]), then each system's refresh when it is due.
+ * The systems must share the precision, the complex form and the register
+ * tile (same block sizes), and may differ in everything else (density, twist,
+ * factors, walker counts).  Every walker's result is bit-identical to
+ * alea_vmc_sweeps on its own system.  The launches go on systems[0]'s stream
+ * after the work already queued on every member's stream; each member's
+ * later work waits for them.  Returns once they are queued.  At most 8
+ * systems, none twice.  No counterpart in the reference: the serving loop's
+ * size-bucketed launch. */
+int alea_vmc_sweeps_fused(alea_system* const* systems, int n_systems, int n_sweeps, const double* taus);
+
 /* on != 0: alea_vmc_sweeps / alea_dmc_sweeps on the device streams (tape
  * NULL) replay one captured hipGraph per block of n_sweeps (draw, sweep and
  * the periodic refresh of every sweep), re-captured when the sweep count,

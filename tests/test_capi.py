@@ -35,7 +35,7 @@ def test_oracle_exports_the_same_entry_points():
     lib = C.CDLL(ORACLE_SO)
     # the oracle has no device RNG to seed and no stream to time or poll
     missing = [n for n in declared() if n not in ("alea_seed", "alea_synchronize", "alea_last_sweep_ms",
-                                                 "alea_set_refresh_interval", "alea_set_sweep_graphs",
+                                                 "alea_set_refresh_interval", "alea_set_sweep_graphs", "alea_vmc_sweeps_fused",
                                                  "alea_query")
                and not hasattr(lib, n.replace("alea_", "oracle_", 1))]
     assert not missing, missing

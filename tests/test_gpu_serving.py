@@ -100,3 +100,75 @@ def test_sweep_graph_dmc_sweeps_match_eager():
         _same_state(eager, graph)
     for w in (eager, graph):
         w.close()
+
+
+def _fused_systems(kind, precision, sizes, n):
+    """Systems of one size bucket that differ in density, twist and walker
+    count: HEG N=n at rs 30/40/60, or dipoles at three densities."""
+    from alea_amd.system import dipole_spec, heg_spec, uniform_positions
+    from alea_amd.walkers import gpu_walkers, seed
+    out = []
+    for i, (rs, nw) in enumerate(sizes):
+        if kind == "dipoles":
+            spec = dipole_spec(n, density=1.0 / (np.pi * rs * rs))
+        else:
+            spec = heg_spec(n // 2, n // 2, rs, twist=(0.1 * (i + 1), -0.07 * (i + 1)))
+        w = gpu_walkers(spec, nw, precision)
+        w.set_positions(uniform_positions(spec, nw, np.random.default_rng(20 + i)))
+        w.evaluate()
+        seed(w, 300 + i)
+        out.append((w, (0.5 * rs) ** 2))
+    return out
+
+
+@pytest.mark.parametrize("kind,precision,n", [
+    ("heg", MIXED, 26),      # 13-orbital blocks: 16-row register tile
+    ("heg", MIXED, 74),      # 37-orbital blocks: 64-row register tile
+    ("heg", FP64, 58),       # complex FP64, 29 orbitals: 32-row tile
+    ("heg", FP64, 242),      # 121 orbitals: the inverse streams from HBM
+    ("dipoles", FP64, 64),   # Jastrow only
+])
+def test_fused_sweeps_match_each_system_alone(kind, precision, n):
+    """alea_vmc_sweeps_fused over three systems of one bucket gives every
+    walker exactly what alea_vmc_sweeps gives on its own system: positions,
+    log psi and local energies bit for bit, over two blocks (refresh phases
+    included) with eager work between them."""
+    from alea_amd.walkers import vmc_sweeps_fused
+    sizes = [(30.0, 192), (40.0, 64), (60.0, 320)]
+    alone = _fused_systems(kind, precision, sizes, n)
+    fused = _fused_systems(kind, precision, sizes, n)
+    for block in range(2):
+        for w, tau in alone:
+            w.vmc_sweeps(3, tau)
+        vmc_sweeps_fused([w for w, _ in fused], 3, [tau for _, tau in fused])
+        for (a, _), (b, _) in zip(alone, fused):
+            _same_state(a, b)
+    for w, _ in alone + fused:
+        w.close()
+
+
+def test_fused_launch_refuses_mixed_buckets():
+    from alea_amd.walkers import vmc_sweeps_fused
+    (a, ta), = _fused_systems("heg", MIXED, [(30.0, 64)], 26)
+    (b, tb), = _fused_systems("heg", MIXED, [(30.0, 64)], 58)     # another register tile
+    (c, tc), = _fused_systems("heg", FP64, [(30.0, 64)], 26)      # another precision
+    for other, t in ((b, tb), (c, tc)):
+        with pytest.raises(RuntimeError, match="share the precision"):
+            vmc_sweeps_fused([a, other], 1, [ta, t])
+    with pytest.raises(RuntimeError, match="appears twice"):
+        vmc_sweeps_fused([a, a], 1, [ta, ta])
+    for w in (a, b, c):
+        w.close()
+
+
+def test_serving_fuses_a_size_bucket():
+    """Four N=26 jobs arriving together run as one fused group; each job's
+    block energies equal its run alone."""
+    twists = [(0.1, 0.2), (0.13, 0.27), (-0.21, 0.05), (0.3, -0.3)]
+    jobs = [_job(i, 26, 256 * (i + 1), arrival=0.0, twist=twists[i]) for i in range(4)]
+    st = serving.Server(max_active=8).run(jobs)
+    assert st["jobs"] == 4 and st["peak_fused"] >= 2
+    for i in range(4):
+        alone = _job(i, 26, 256 * (i + 1), arrival=0.0, twist=twists[i])
+        serving.Server(max_active=1).run([alone])
+        assert alone.block_energy == jobs[i].block_energy, i
