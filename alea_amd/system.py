@@ -153,6 +153,7 @@ class Spec:
     coulomb: list = field(default_factory=list)         # (ga, gb, eps)
     inverse_power: list = field(default_factory=list)   # (ga, gb, strength, exponent)
     mcmillan: list = field(default_factory=list)        # (ga, gb, b, n)
+    sites: object = None                                # crystal sites [N][2], if any
     name: str = ""
 
 
@@ -190,6 +191,58 @@ def heg_spec(n_up: int, n_dn: int, rs: float, twist=None, slater=True, jastrow=T
     if ewald:
         s.ewald = cell["ewald"]
     return s
+
+
+def wigner_crystal_spec(rs: float = 60.0, rows: int = 8, cols: int = 16, nosanow_alpha: float | None = None,
+                        drummond_order: int = 8, drummond_sigma: float = 1e-3, drummond_seed: int = 2,
+                        tables=None) -> Spec:
+    """The polarized 2-D Wigner crystal: N = rows x cols electrons of one spin
+    on a triangular lattice in the rectangular cell of area N pi rs^2 (cols
+    sites along x, rows along y, every other row shifted by half a spacing).
+    Trial: one real determinant of Gaussians exp(-c |r - R_j|^2) (minimum
+    image) centred on the sites with c = 0.1 / rs^1.5, a Nosanow one-body
+    term -alpha |r_i - R_i|^2 on each particle's own site (alpha defaults to
+    c), the image-summed Pade u = r/3 / (1 + r/(2 rs)) and a Drummond
+    polynomial (C = 3, cutoff the Wigner-Seitz radius, coefficients drawn
+    N(0, sigma) in units of the cutoff, as for the paramagnetic DMC case);
+    2-D Ewald.  Particle i sits on site i."""
+    n = rows * cols
+    a0 = rs * math.sqrt(2.0 * math.pi / math.sqrt(3.0))
+    lat = [cols * a0, 0.0, 0.0, rows * a0 * math.sqrt(3.0) / 2.0]
+    tables = tables or load_tables()
+    cell = tables["cells"].get(cell_key(lat))
+    if cell is None:
+        raise KeyError(f"no breakup tables for cell {cell_key(lat)}; run alea_amd/tables/make_tables.py")
+    sites = np.array([((c + 0.5 * (r % 2)) * a0, r * a0 * math.sqrt(3.0) / 2.0)
+                      for r in range(rows) for c in range(cols)])
+    sites -= 0.5 * np.array([lat[0], lat[3]])   # centred on the origin, like the cell
+    c = 0.1 / rs ** 1.5
+    s = Spec(n_particles=n, lattice=lat, name=f"wigner crystal N={n} rs={rs:g}")
+    s.groups.append((0, n, -1.0, 1.0))
+    b = [0.5 / rs]
+    s.pade.append(dict(ga=0, gb=0, num=1, den=1, coeffs=[1.0 / 3.0] + b,
+                       image=cell["pade"][pade_key([1.0 / 3.0], b)]))
+    half = 0.5 * min(lat[0], lat[3])
+    rng = np.random.default_rng(drummond_seed)
+    C = 3
+    powers = [0] + list(range(2, drummond_order + 1))
+    xi = rng.normal(0.0, drummond_sigma, drummond_order)
+    s.drummond.append(dict(ga=0, gb=0, C=C, gamma=0.0, L=half,
+                           coeffs=[float(x) / half ** (C + l) for x, l in zip(xi, powers)]))
+    s.nosanow.append(dict(group=0, alpha=c if nosanow_alpha is None else nosanow_alpha,
+                          sites=sites.reshape(-1).copy()))
+    s.slater_gauss.append((0, n, sites.copy(), np.full(n, c)))
+    s.ewald = cell["ewald"]
+    s.sites = sites
+    return s
+
+
+def site_positions(spec: Spec, nw: int, rng: np.random.Generator, spread: float):
+    """Walkers near the crystal sites: site + N(0, spread^2) per coordinate,
+    SoA [p*2*nw + d*nw + w]."""
+    sites = np.asarray(spec.sites)
+    pos = sites[:, :, None] + rng.normal(0.0, spread, size=(spec.n_particles, 2, nw))
+    return np.ascontiguousarray(pos.reshape(-1))
 
 
 def dipole_spec(n: int, density: float = 1.0, b: float | None = None, n_exp: int = 5,

@@ -180,3 +180,68 @@ def test_largest_complex_blocks_evaluate_like_the_oracle():
     assert rel_err(g.log_psi(), o.log_psi()) < TOL
     assert rel_err(g.gradients()[0], o.gradients()[0]) < 1e-9
     assert rel_err(g.local_energy()[0], o.local_energy()[0]) < TOL
+
+
+def _crystal(nw, seed_):
+    from alea_amd.system import site_positions, wigner_crystal_spec
+    spec = wigner_crystal_spec()
+    a0 = spec.lattice[0] / 16
+    return spec, site_positions(spec, nw, np.random.default_rng(seed_), spread=0.15 * a0)
+
+
+def test_wigner_crystal_parity():
+    """The polarized Wigner crystal at its full shape (N = 128 on an 8 x 16
+    triangular lattice, rs = 60, one 128-orbital real Gaussian determinant,
+    Nosanow, Pade + Drummond, Ewald in the elongated cell): the FP64 device
+    matches the oracle on evaluation, on two VMC sweeps and on two
+    drift-diffusion sweeps from one tape, to 1e-10 with identical accepts."""
+    nw = 8
+    spec, pos = _crystal(nw, 3)
+    g, o = gpu_walkers(spec, nw), oracle_walkers(spec, nw)
+    for x in (g, o):
+        x.set_positions(pos)
+        x.evaluate()
+    assert rel_err(g.log_psi(), o.log_psi()) < TOL
+    assert rel_err(g.local_energy()[0], o.local_energy()[0]) < TOL
+    tape = oracle_tape(5, spec.n_particles, nw, 2)
+    tau = 15.0 ** 2
+    ag, ao = g.vmc_sweeps(2, tau, tape), o.vmc_sweeps(2, tau, tape)
+    assert ag == ao and 0 < ag < 2 * nw * spec.n_particles
+    assert rel_err(g.positions(), o.positions()) < TOL
+    assert rel_err(g.log_psi(), o.log_psi()) < TOL
+    assert rel_err(g.local_energy()[0], o.local_energy()[0]) < TOL
+    tape = oracle_tape(6, spec.n_particles, nw, 2)
+    rg, ro = [np.zeros(nw), np.zeros(nw)], [np.zeros(nw), np.zeros(nw)]
+    ag, ao = g.dmc_sweeps(2, 2.0, tape, nw, *rg), o.dmc_sweeps(2, 2.0, tape, nw, *ro)
+    assert ag == ao and ag > 0
+    assert rel_err(g.positions(), o.positions()) < TOL
+    assert rel_err(rg[1], ro[1]) < TOL
+    assert rel_err(g.log_psi(), o.log_psi()) < TOL
+    assert rel_err(g.local_energy()[0], o.local_energy()[0]) < TOL
+
+
+def test_wigner_crystal_mixed_tracks_fp64():
+    """Mixed precision on the crystal's well-conditioned Gaussian block: over
+    four sweeps of the device streams at least 98% of walkers take FP64's
+    decisions (an FP32 ratio may flip one that sits within rounding of its
+    threshold), and on those log psi and E_L agree to 1e-7 relative."""
+    nw = 256
+    spec, pos = _crystal(nw, 4)
+    ws = []
+    for prec in (FP64, MIXED):
+        w = gpu_walkers(spec, nw, prec)
+        w.set_positions(pos)
+        w.evaluate()
+        seed(w, 17)
+        ws.append(w)
+    a = [w.vmc_sweeps(4, 15.0 ** 2) for w in ws]
+    assert a[0] > 0 and a[1] > 0
+    p0, p1 = (w.positions().reshape(-1, nw) for w in ws)
+    same = np.all(np.abs(p0 - p1) <= 1e-9 * np.abs(p0).max(), axis=0)
+    assert same.mean() >= 0.98, same.mean()
+    l0, l1 = ws[0].log_psi()[same], ws[1].log_psi()[same]
+    e0, e1 = ws[0].local_energy()[0][same], ws[1].local_energy()[0][same]
+    assert rel_err(l1, l0) < 1e-7
+    assert rel_err(e1, e0) < 1e-7
+    for w in ws:
+        w.close()
