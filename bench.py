@@ -55,6 +55,9 @@ def bytes_per_move(word, acceptance, spec):
     return word * words
 
 
+KERNEL_SAMPLES = 4   # single sweeps timed with HIP events after the timed steps
+
+
 def run_gpu(precision, nw, steps, warmup, tau):
     from alea_amd.system import uniform_positions
     from alea_amd.walkers import gpu_walkers, seed
@@ -69,16 +72,20 @@ def run_gpu(precision, nw, steps, warmup, tau):
     for _ in range(warmup):
         w.vmc_sweeps(1, tau)
     api("synchronize", w.h)
-    kernel_ms = []
+    # the timed steps are queued back to back, as a VMC run queues its
+    # sweeps (draw, sweep, and the refresh every second sweep), and read back
+    # once: the acceptance count of the same steps
     t0 = time.perf_counter()
-    for _ in range(steps):
-        w.vmc_sweeps(1, tau)
-        api("last_sweep_ms", w.h, C.byref(ms))   # waits for this sweep
-        kernel_ms.append(ms.value)
+    acc = w.vmc_sweeps(steps, tau)
     api("synchronize", w.h)
     wall = time.perf_counter() - t0
-    acc = w.vmc_sweeps(1, tau)                   # untimed: acceptance
-    acceptance = acc / (nw * spec.n_particles)
+    acceptance = acc / (nw * spec.n_particles * steps)
+    # untimed: the sweep kernel alone, HIP events around single sweeps
+    kernel_ms = []
+    for _ in range(KERNEL_SAMPLES):
+        w.vmc_sweeps(1, tau)
+        api("last_sweep_ms", w.h, C.byref(ms))
+        kernel_ms.append(ms.value)
     e, e_im = w.local_energy()
     out = dict(moves_per_s=nw * spec.n_particles * steps / wall, ms_per_step=1e3 * wall / steps,
                kernel_ms=float(np.mean(kernel_ms)), acceptance=acceptance,
@@ -301,7 +308,7 @@ def copy_calibration(n_bytes, reps=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--walkers", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
