@@ -47,8 +47,8 @@ void launch_branch(const double* weight, const double* xi, int* src, int src_cap
 void launch_fill(double* a, int n, double v, hipStream_t st);
 void launch_estimators(const DevModel* m, const double* pos, const double* weights, int nw, int chunk,
                        bool gofr, double rmax, double inv_delta, int n_bins, int nfb, bool sofk,
-                       const double* kvec, int nk, double* part, int part_stride, int n_entries, double* acc,
-                       hipStream_t st);
+                       const double* kvec, int nk, bool density, double d_rmax, double d_inv_delta, int d_bins,
+                       int d_off, double* part, int part_stride, int n_entries, double* acc, hipStream_t st);
 }  // namespace alea_amd
 
 using namespace alea_amd;
@@ -234,15 +234,19 @@ struct alea_system {
     // sums on the device, [gofr: n_func*n_bins][sofk: n_pairs*nk][rho re:
     // G*nk][rho im: G*nk][weight sum]
     struct Estimators {
-        bool gofr = false, sofk = false;
+        bool gofr = false, sofk = false, density = false;
         double rmax = 0.0, inv_delta = 0.0;
         int n_bins = 0, nfb = 0, nk = 0;
+        double d_rmax = 0.0, d_inv_delta = 0.0;   // density: window half-width (open cells), bins per length
+        int d_bins = 0;
         std::vector<double> kx, ky;
         std::vector<int> shell;
         DevBuf<double> acc, part, kvec;
-        int entries(int ng) const {
-            return nfb + (sofk ? (ng * (ng + 1) / 2 + 2 * ng) * nk : 0);
-        }
+        // [gofr][sofk, rho re, rho im][density]
+        int sofk_entries(int ng) const { return sofk ? (ng * (ng + 1) / 2 + 2 * ng) * nk : 0; }
+        int density_offset(int ng) const { return nfb + sofk_entries(ng); }
+        int density_entries(int ng) const { return density ? ng * d_bins * d_bins : 0; }
+        int entries(int ng) const { return density_offset(ng) + density_entries(ng); }
     } est;
 
     bool mixed() const { return precision == ALEA_MIXED; }
@@ -1652,6 +1656,29 @@ int alea_add_gofr(alea_system* s, double rmax, int n_bins) {
     });
 }
 
+// Density(groups, dim, rmax, n_bins, per_particle = false, cell)
+// (density.h:55, density.cpp:16-64): one n_bins x n_bins grid per group over
+// the periodic cell (rmax unused) or over [-rmax, rmax)^2 in an open cell.
+int alea_add_density(alea_system* s, double rmax, int n_bins) {
+    return guarded([&] {
+        Sys(s);
+        const int G = s->host.ng;
+        if (G <= 0) throw std::invalid_argument("alea_add_density: add the particle groups first");
+        if (n_bins <= 0) throw std::invalid_argument("alea_add_density: needs n_bins > 0");
+        if (!s->host.cell.periodic && !(rmax > 0.0))
+            throw std::invalid_argument("alea_add_density: an open cell needs rmax > 0");
+        if (static_cast<long long>(G) * n_bins * n_bins > kMaxDensityEntries)
+            throw std::invalid_argument("alea_add_density: groups x n_bins^2 exceeds " +
+                                        std::to_string(kMaxDensityEntries));
+        auto& e = s->est;
+        e.density = true;
+        e.d_rmax = rmax;
+        e.d_bins = n_bins;
+        e.d_inv_delta = static_cast<double>(n_bins) / (2.0 * rmax);
+        est_alloc(s);
+    });
+}
+
 // The half-space reciprocal vectors of the reduced basis with |k| <= kmax,
 // sorted by |k|, in shells of equal |k| (structure_factor.cpp:16-54).
 int alea_add_sofk(alea_system* s, double kmax, int* n_k) {
@@ -1727,14 +1754,30 @@ int alea_estimators_accumulate(alea_system* s, int use_weights) {
         Sys(s);
         s->build();
         auto& e = s->est;
-        if (!e.gofr && !e.sofk) throw std::invalid_argument("alea_estimators_accumulate: no estimators");
+        if (!e.gofr && !e.sofk && !e.density) throw std::invalid_argument("alea_estimators_accumulate: no estimators");
         if (s->nw <= 0) return;
         const int chunk = (s->nw + kEstMaxChunks - 1) / kEstMaxChunks;
         const int n = e.entries(s->host.ng);
         launch_estimators(s->dmodel, s->pos.p, use_weights ? s->weights.p : nullptr, s->nw, chunk, e.gofr,
-                          e.rmax, e.inv_delta, e.n_bins, e.nfb, e.sofk, e.kvec.p, e.nk, e.part.p, n, n,
-                          e.acc.p, s->stream);
+                          e.rmax, e.inv_delta, e.n_bins, e.nfb, e.sofk, e.kvec.p, e.nk, e.density, e.d_rmax,
+                          e.d_inv_delta, e.d_bins, e.density_offset(s->host.ng), e.part.p, n, n, e.acc.p,
+                          s->stream);
         hip_check(hipGetLastError(), "estimator launch");
+    });
+}
+
+int alea_estimators_read_density(alea_system* s, double* density, double* wsum) {
+    return guarded([&] {
+        Sys(s);
+        auto& e = s->est;
+        if (!e.density) throw std::invalid_argument("alea_estimators_read_density: no density estimator");
+        const int n = e.entries(s->host.ng), off = e.density_offset(s->host.ng);
+        std::vector<double> h(static_cast<size_t>(n) + 1, 0.0);
+        if (e.acc.p)
+            hip_check(copy_on(s->stream, h.data(), e.acc.p, h.size() * sizeof(double), hipMemcpyDeviceToHost),
+                      "read density");
+        if (density) std::copy(h.begin() + off, h.begin() + off + e.density_entries(s->host.ng), density);
+        if (wsum) *wsum = h[n];
     });
 }
 

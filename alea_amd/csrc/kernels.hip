@@ -2641,6 +2641,74 @@ __global__ __launch_bounds__(64) void k_sofk(const DevModel* __restrict__ Mp, co
     }
 }
 
+// Density::accumulate (density.cpp:66-117): each particle's cell of an
+// n_bins x n_bins grid per group, C row-major.  A periodic cell bins the
+// fractional coordinates of the wrapped position (wrap as SimulationCell::
+// wrap, f_i = sum_d ainv[i][d] r_d uncontracted, bin int((f + 1/2) n_bins)
+// clamped to the last bin); an open cell bins [-rmax, rmax)^2 and drops the
+// particles outside.  A walker's counts are exact integers in LDS; the thread
+// that opens a cell for the walker folds count x weight into the chunk's
+// FP64 histogram and clears the count, walker after walker, so the sums do
+// not depend on scheduling.
+__global__ __launch_bounds__(kEstThreads) void k_density(const DevModel* __restrict__ Mp, const double* __restrict__ pos,
+                                                         const double* __restrict__ weights, int nw, int chunk,
+                                                         double rmax, double inv_delta, int n_bins,
+                                                         double* __restrict__ part, int part_stride, int off) {
+    const DevModel& M = *Mp;
+    __shared__ int cnt[kMaxDensityEntries];
+    __shared__ double hist[kMaxDensityEntries];
+    const int np = M.np, n_cells = n_bins * n_bins, nden = M.ng * n_cells, tid = threadIdx.x;
+    for (int k = tid; k < nden; k += kEstThreads) {
+        cnt[k] = 0;
+        hist[k] = 0.0;
+    }
+    const double two_rmax = 2.0 * rmax, nb = static_cast<double>(n_bins);
+    const int w0 = blockIdx.x * chunk, w1 = min(nw, w0 + chunk);
+    for (int w = w0; w < w1; ++w) {
+        __syncthreads();   // the previous walker's counts are folded and cleared
+        // each thread's particles: tid, tid + 256, ... (np <= kMaxParticles)
+        int idx[kMaxParticles / kEstThreads + 1];
+#pragma unroll
+        for (int q = 0; q * kEstThreads < kMaxParticles; ++q) {
+            const int p = tid + q * kEstThreads;
+            idx[q] = -1;
+            if (p >= np) continue;
+            double r0 = pos[(static_cast<size_t>(w) * np + p) * 2], r1 = pos[(static_cast<size_t>(w) * np + p) * 2 + 1];
+            int cell = 0;
+            if (M.cell.periodic) {
+                wrap(M.cell, r0, r1);
+                const double* ai = M.cell.ainv;
+                const double f0 = __dadd_rn(__dadd_rn(0.0, __dmul_rn(ai[0], r0)), __dmul_rn(ai[1], r1));
+                const double f1 = __dadd_rn(__dadd_rn(0.0, __dmul_rn(ai[2], r0)), __dmul_rn(ai[3], r1));
+                const int b0 = min(static_cast<int>(__dmul_rn(__dadd_rn(f0, 0.5), nb)), n_bins - 1);
+                const int b1 = min(static_cast<int>(__dmul_rn(__dadd_rn(f1, 0.5), nb)), n_bins - 1);
+                cell = b0 * n_bins + b1;
+            } else {
+                const double x0 = __dadd_rn(r0, rmax), x1 = __dadd_rn(r1, rmax);
+                if (x0 < 0.0 || x0 >= two_rmax || x1 < 0.0 || x1 >= two_rmax) continue;
+                const int b0 = min(static_cast<int>(__dmul_rn(x0, inv_delta)), n_bins - 1);
+                const int b1 = min(static_cast<int>(__dmul_rn(x1, inv_delta)), n_bins - 1);
+                cell = b0 * n_bins + b1;
+            }
+            const int k = M.group_of[p] * n_cells + cell;
+            // the thread whose increment opens the cell folds it
+            idx[q] = atomicAdd(&cnt[k], 1) == 0 ? k : -1;
+        }
+        __syncthreads();
+        const double wt = weights ? weights[w] : 1.0;
+#pragma unroll
+        for (int q = 0; q * kEstThreads < kMaxParticles; ++q) {
+            const int k = idx[q];
+            if (k < 0) continue;
+            hist[k] += static_cast<double>(cnt[k]) * wt;
+            cnt[k] = 0;
+        }
+    }
+    __syncthreads();
+    double* row = part + static_cast<size_t>(blockIdx.x) * part_stride + off;
+    for (int k = tid; k < nden; k += kEstThreads) row[k] = hist[k];
+}
+
 // acc[e] += sum over chunks c (in order) of part[c][e]; the last workgroup
 // adds the walkers' weight sum (a fixed per-thread then tree order) to
 // acc[n_entries].
@@ -2673,9 +2741,12 @@ __global__ __launch_bounds__(kEstThreads) void k_est_reduce(const double* __rest
 
 void launch_estimators(const DevModel* m, const double* pos, const double* weights, int nw, int chunk,
                        bool gofr, double rmax, double inv_delta, int n_bins, int nfb, bool sofk,
-                       const double* kvec, int nk, double* part, int part_stride, int n_entries, double* acc,
-                       hipStream_t st) {
+                       const double* kvec, int nk, bool density, double d_rmax, double d_inv_delta, int d_bins,
+                       int d_off, double* part, int part_stride, int n_entries, double* acc, hipStream_t st) {
     const int n_chunks = (nw + chunk - 1) / chunk;
+    if (density)
+        hipLaunchKernelGGL(k_density, dim3(n_chunks), dim3(kEstThreads), 0, st, m, pos, weights, nw, chunk, d_rmax,
+                           d_inv_delta, d_bins, part, part_stride, d_off);
     if (gofr)
         hipLaunchKernelGGL(k_gofr, dim3(n_chunks), dim3(kEstThreads), 0, st, m, pos, weights, nw, chunk, rmax,
                            inv_delta, n_bins, part, part_stride);

@@ -14,6 +14,7 @@ namespace alea_amd {
 constexpr int kMaxParticles = 256;
 constexpr int kMaxGroups = 4;
 constexpr int kMaxGofrEntries = 4096;   // pair-correlation functions x bins
+constexpr int kMaxDensityEntries = 4096;   // density: groups x n_bins^2
 constexpr int kMaxSofkVectors = 1024;
 constexpr int kMaxPair = 8;        // pair-Jastrow factors
 constexpr int kMaxCoef = 12;       // Padé / Drummond coefficients

@@ -1,5 +1,5 @@
-"""Observables measured on the device: the pair correlation g(r) and the
-structure factor S(k) with the densities rho_k.
+"""Observables measured on the device: the pair correlation g(r), the
+structure factor S(k) with the densities rho_k, and the one-body density.
 
 The device holds each block's running sums (alea_add_gofr / alea_add_sofk,
 alea_estimators_start_block / _accumulate / _read); this module does what
@@ -9,6 +9,8 @@ the reference does on the host with them:
                weight sum), structure_factor.cpp:113-141 (S per shell: the sum
                over the shell's vectors of acc / W / sqrt(N_a N_b) / count;
                rho over W)
+               density.cpp:120-170 (the grid over W V_cell, streamed into
+               per-cell Welford means and second moments)
   finalize     pair_correlation.cpp:140-190 (bin centres; g = block mean /
                (pairs * shell area [/ cell area])), structure_factor.cpp:
                169-221 (series per group pair and group), with
@@ -16,7 +18,8 @@ the reference does on the host with them:
 
 Series names follow the reference's RunResult keys: gofr_r,
 gofr_<A>_<B>[_error], sofk_k, sofk_shell_count, sofk_<A>_<B>[_error],
-sofk_kx, sofk_ky, sofk_shell, rhok_<A>_re/_im[_error].
+sofk_kx, sofk_ky, sofk_shell, rhok_<A>_re/_im[_error], density_edges,
+density_<A>[_error] (density.cpp:220-266).
 """
 from __future__ import annotations
 
@@ -31,10 +34,12 @@ from .statistics import block_mean_error
 
 
 class Estimators:
-    """g(r) and/or S(k) of one Walkers batch (product or oracle library)."""
+    """g(r), S(k) and/or the density of one Walkers batch (product or oracle
+    library)."""
 
-    def __init__(self, walkers, gofr=None, sofk=None, group_names=None):
-        """gofr: (rmax, n_bins) or None; sofk: kmax or None; group_names:
+    def __init__(self, walkers, gofr=None, sofk=None, group_names=None, density=None):
+        """gofr: (rmax, n_bins) or None; sofk: kmax or None; density:
+        (rmax, n_bins) or None (rmax unused in a periodic cell); group_names:
         one name per group (default g<first>)."""
         self.w = walkers
         api, h = walkers.api, walkers.h
@@ -61,6 +66,19 @@ class Estimators:
             self.shell_count = np.bincount(self.shell, minlength=self.n_shells)
             first = [int(np.argmax(self.shell == q)) for q in range(self.n_shells)]
             self.shell_k = [math.hypot(self.kx[i], self.ky[i]) for i in first]
+        self.density = density is not None
+        if self.density:
+            self.d_rmax, self.d_bins = float(density[0]), int(density[1])
+            api("add_density", h, self.d_rmax, self.d_bins)
+            self.d_cells = self.d_bins * self.d_bins
+            if self.lattice is not None:
+                a = cell_vectors(self.lattice)
+                self.d_volume = abs(a[0] * a[3] - a[1] * a[2]) / float(self.d_cells)
+            else:
+                delta = 2.0 * self.d_rmax / float(self.d_bins)
+                self.d_volume = 1.0 * delta * delta
+            n = G * self.d_cells
+            self.d_mean, self.d_m2, self.d_blocks = [0.0] * n, [0.0] * n, 0
         self.hist_g, self.hist_s, self.hist_re, self.hist_im = [], [], [], []
 
     # ── per block ────────────────────────────────────────────────────────────
@@ -72,6 +90,10 @@ class Estimators:
         self.w.api("estimators_accumulate", self.w.h, 1 if use_weights else 0)
 
     def stop_block(self):
+        if self.density:
+            self._stop_density()
+        if not (self.gofr or self.sofk):
+            return
         G, npg = self.G, self.npg
         g = np.zeros(npg * self.n_bins) if self.gofr else None
         s = np.zeros(npg * self.nk) if self.sofk else None
@@ -108,9 +130,44 @@ class Estimators:
             self.hist_re.append(rb)
             self.hist_im.append(ib)
 
+    def _stop_density(self):
+        """density.cpp:120-170: x = counts / (W_block V_cell), then the
+        singleton Welford update of each cell's mean and second moment."""
+        n = self.G * self.d_cells
+        h = np.zeros(n)
+        wsum = np.zeros(1)
+        self.w.api("estimators_read_density", self.w.h, ptr(h), ptr(wsum))
+        w = float(wsum[0])
+        norm = 1.0 / (w * self.d_volume) if w > 0.0 else 0.0
+        self.d_blocks += 1
+        if self.d_blocks == 1:
+            self.d_mean = [float(x) * norm for x in h]
+            self.d_m2 = [0.0] * n
+            return
+        inv_n = 1.0 / float(self.d_blocks)
+        vw = float(self.d_blocks - 1) * inv_n
+        for k in range(n):
+            x = float(h[k]) * norm
+            d = x - self.d_mean[k]
+            self.d_mean[k] += d * inv_n
+            self.d_m2[k] += d * (d * vw)
+
     # ── results ──────────────────────────────────────────────────────────────
     def finalize(self) -> dict:
         out = {}
+        if self.density and self.d_blocks:
+            nb = self.d_bins
+            if self.lattice is not None:
+                out["density_edges"] = [-0.5 + float(b) / float(nb) for b in range(nb + 1)]
+            else:
+                delta = 2.0 * self.d_rmax / float(nb)
+                out["density_edges"] = [-self.d_rmax + float(b) * delta for b in range(nb + 1)]
+            n = float(self.d_blocks)
+            err_norm = 1.0 / ((n - 1.0) * n) if self.d_blocks >= 2 else 0.0
+            for g in range(self.G):
+                sl = slice(g * self.d_cells, (g + 1) * self.d_cells)
+                out[f"density_{self.names[g]}"] = list(self.d_mean[sl])
+                out[f"density_{self.names[g]}_error"] = [math.sqrt(m * err_norm) for m in self.d_m2[sl]]
         if self.gofr and self.hist_g:
             nb = self.n_bins
             out["gofr_r"] = [(b + 0.5) * self.delta for b in range(nb)]

@@ -15,7 +15,7 @@ src/qmc/app.cpp:1066-1098).
 The walkers draw from the device streams (one xoshiro256++ stream per
 walker, seeded from the stage seed), so a run reproduces the reference's
 statistics, not its trajectories (those are replayed through tapes in the
-tests).  The `gofr` and `sofk` estimators of a stage are measured on the
+tests).  The `gofr`, `density` and `sofk` estimators of a stage are measured on the
 device (alea_amd.estimators) after each accumulation step, with the DMC
 weights before the branch in DMC.  Run settings the GPU path does not hold
 (langevin VMC moves, non-bare reweighting, forward walking, other
@@ -71,18 +71,23 @@ def _check_stage(st: dict):
 
 def _estimator_args(st: dict) -> dict:
     """The stage's `estimators:` as Estimators arguments: one `gofr` (rmax,
-    n_bins) and one `sofk` (kmax) at most (parser.cpp:148-171)."""
+    n_bins), one `density` (rmax, n_bins) and one `sofk` (kmax) at most
+    (parser.cpp:148-171)."""
     args = {}
     for e in st.get("estimators", []) or []:
         kind = e.get("type")
-        if kind not in ("gofr", "sofk"):
+        if kind not in ("gofr", "sofk", "density"):
             _refuse(f"estimator '{kind}'")
+        if kind == "density" and e.get("per_particle"):
+            _refuse("a per-particle density")
         if e.get("name"):
             _refuse(f"a named {kind} estimator")
         if kind in args:
             _refuse(f"a second {kind} estimator in one stage")
         if kind == "gofr":
             args["gofr"] = (float(e.get("rmax", 0.0)), int(e.get("n_bins", 0)))
+        elif kind == "density":
+            args["density"] = (float(e.get("rmax", 0.0)), int(e.get("n_bins", 0)))
         else:
             args["sofk"] = float(e.get("kmax", 0.0))
     return args

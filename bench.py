@@ -216,9 +216,27 @@ def dmc_run(spec, c, prec, pos_seed, seed):
                 series.append(e_sum / w_sum / n)
                 pops.append(w.nw)
         wall = time.perf_counter() - t0
-        return series, pops, moves / wall
+        return series, pops, moves / wall, (w.nw, w.positions())
     finally:
         w.close()
+
+
+def same_configuration_energies(spec, nw, pos):
+    """E_L/N of the same walkers evaluated in FP64 and in mixed form (fresh
+    inverses, no trajectory): the mixed form's local-energy error alone."""
+    from alea_amd.walkers import gpu_walkers
+    e = []
+    for prec in (0, 1):
+        w = gpu_walkers(spec, nw, prec)
+        try:
+            w.set_positions(pos)
+            w.evaluate()
+            e.append(w.local_energy()[0] / spec.n_particles)
+        finally:
+            w.close()
+    d = e[1] - e[0]
+    return dict(walkers=nw, mean=float(np.mean(d)), max_abs=float(np.max(np.abs(d))),
+                fp64_e_per_particle=float(np.mean(e[0])))
 
 
 def _mean_err(xs):
@@ -235,14 +253,20 @@ def dmc_leg(c=DMC_CFG):
     (with its reblocked error); a form's E/N is the mean over replicas, and
     its error is the spread of the replica means, which stays honest where
     population-control correlations outlast one run's reblocking.  The
-    mixed - FP64 difference is taken per replica, then averaged the same way."""
+    mixed - FP64 difference is taken per replica, then averaged the same way.
+    Beside it, the local-energy difference of the two forms on identical
+    configurations (replica 0's final FP64 walkers), which no sampling
+    noise enters."""
     from alea_amd import statistics
     spec = dmc_spec(c)
     out = {"fp64": {}, "mixed": {}}
     per = {"fp64": [], "mixed": []}
+    final = None
     for r in range(c["replicas"]):
         for name, prec in (("fp64", 0), ("mixed", 1)):
-            series, pops, rate = dmc_run(spec, c, prec, c["pos_seed"] + r, c["seed"] + r)
+            series, pops, rate, last = dmc_run(spec, c, prec, c["pos_seed"] + r, c["seed"] + r)
+            if r == 0 and prec == 0:
+                final = last
             rb = statistics.reblock(series)
             per[name].append(dict(e_per_particle=rb.mean, reblocked_error=rb.error, tau_int=rb.tau_int,
                                   plateau_found=rb.plateau_found, mean_population=float(np.mean(pops)),
@@ -259,6 +283,8 @@ def dmc_leg(c=DMC_CFG):
     out["error"] = de
     out["within_1e-5"] = abs(d) <= 1e-5
     out["within_1_sigma"] = abs(d) <= de
+    # the final DMC walkers of replica 0's FP64 run, evaluated in both forms
+    out["same_configuration_difference"] = same_configuration_energies(spec, *final)
     out["config"] = dict(c, system="N=74 (37+37) paramagnetic HEG, rs=40, plane-wave nodes, Pade + Drummond, Ewald")
     return out
 

@@ -250,12 +250,18 @@ int alea_add_gofr(alea_system* s, double rmax, int n_bins);
  * reciprocal lattice with |k| <= kmax, sorted by |k|; *n_k receives their
  * number (at most 1024).  Periodic cells only. */
 int alea_add_sofk(alea_system* s, double kmax, int* n_k);
+/* Density(groups, dim, rmax, n_bins, per_particle = false, cell) —
+ * src/estimators/density.h:55: one n_bins x n_bins grid per group, C
+ * row-major [g * n_bins^2 + b0 * n_bins + b1]; a periodic cell bins the
+ * fractional coordinates of the wrapped position over the whole cell (rmax
+ * unused), an open cell the window [-rmax, rmax)^2.  G * n_bins^2 <= 4096. */
+int alea_add_density(alea_system* s, double rmax, int n_bins);
 /* The structure factor's vectors kx, ky [n_k] and shell index [n_k]. */
 int alea_get_sofk_vectors(alea_system* s, double* kx, double* ky, int* shell);
 /* Estimator::start_block of every estimator: the running sums to zero. */
 int alea_estimators_start_block(alea_system* s);
 /* Estimator::accumulate(wset) (pair_correlation.cpp:74-105,
- * structure_factor.cpp:62-105): one measurement of the current walkers with
+ * structure_factor.cpp:62-105, density.cpp:66-117): one measurement of the current walkers with
  * weight 1 each (use_weights = 0, VMC) or the walkers' weights (DMC).
  * Asynchronous on the system's stream; deterministic for given walkers. */
 int alea_estimators_accumulate(alea_system* s, int use_weights);
@@ -265,6 +271,9 @@ int alea_estimators_accumulate(alea_system* s, int use_weights);
  * correlation's and the structure factor's normalization). */
 int alea_estimators_read(alea_system* s, double* gofr, double* sofk, double* rho_re, double* rho_im,
                          double* wsum);
+/* The density's running sums: density [G * n_bins^2] sum of weights per
+ * cell (density.cpp:66-117), *wsum the weight sum. */
+int alea_estimators_read_density(alea_system* s, double* density, double* wsum);
 
 #ifdef __cplusplus
 }

@@ -582,6 +582,12 @@ struct Sys {
         double wsum = 0.0;
         std::vector<double> acc_s, acc_re, acc_im;   // [pid][k], [g][k]
     } sofk;
+    struct Dens {
+        bool on = false;
+        double rmax = 0.0, inv_delta = 0.0, wsum = 0.0;
+        std::size_t n_bins = 0, n_cells = 0;
+        std::vector<double> hist;   // [g * n_cells + cell]
+    } dens;
 
     double& P(std::size_t p, std::size_t d, std::size_t w) { return pos[(p * dim + d) * nw + w]; }
     double& D(std::size_t i, std::size_t j, std::size_t w) { return dist[(i * np + j) * nw + w]; }
@@ -2431,9 +2437,40 @@ int oracle_get_sofk_vectors(oracle_system* h, double* kx, double* ky, int* shell
     });
 }
 
+// Density (src/estimators/density.cpp:16-64, 66-117): one n_bins x n_bins
+// histogram per group, C row-major; a periodic cell bins the fractional
+// coordinates f + 1/2 of the wrapped position, an open cell the window
+// [-rmax, rmax)^2 (particles outside it drop); each call's walker weights add
+// to the block's weight sum.
+int oracle_add_density(oracle_system* h, double rmax, int n_bins) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        if (n_bins <= 0) throw std::invalid_argument("oracle: density needs n_bins > 0");
+        if (!s.cell.periodic && !(rmax > 0.0)) throw std::invalid_argument("oracle: density needs rmax > 0");
+        auto& d = s.dens;
+        d.on = true;
+        d.rmax = rmax;
+        d.n_bins = static_cast<std::size_t>(n_bins);
+        d.inv_delta = static_cast<double>(n_bins) / (2.0 * rmax);
+        d.n_cells = d.n_bins * d.n_bins;
+        d.hist.assign(s.groups.size() * d.n_cells, 0.0);
+    });
+}
+
+int oracle_estimators_read_density(oracle_system* h, double* density, double* wsum) {
+    return guarded([&] {
+        const Sys& s = *S(h);
+        if (!s.dens.on) throw std::invalid_argument("oracle: no density estimator");
+        if (density) std::copy(s.dens.hist.begin(), s.dens.hist.end(), density);
+        if (wsum) *wsum = s.dens.wsum;
+    });
+}
+
 int oracle_estimators_start_block(oracle_system* h) {
     return guarded([&] {
         Sys& s = *S(h);
+        std::fill(s.dens.hist.begin(), s.dens.hist.end(), 0.0);
+        s.dens.wsum = 0.0;
         std::fill(s.gofr.hist.begin(), s.gofr.hist.end(), 0.0);
         s.gofr.wsum = 0.0;
         std::fill(s.sofk.acc_s.begin(), s.sofk.acc_s.end(), 0.0);
@@ -2494,6 +2531,47 @@ int oracle_estimators_accumulate(oracle_system* h, int use_weights) {
             double ws = 0.0;
             for (std::size_t w = 0; w < nw; ++w) ws += wt[w];
             g.wsum += ws;
+        }
+        if (s.dens.on) {   // density.cpp:66-117
+            auto& d = s.dens;
+            const double two_rmax = 2.0 * d.rmax;
+            for (std::size_t gg = 0; gg < G; ++gg) {
+                double* hst = d.hist.data() + gg * d.n_cells;
+                for (std::size_t p = s.groups[gg].first; p < s.groups[gg].last; ++p)
+                    for (std::size_t w = 0; w < nw; ++w) {
+                        double r[2] = {s.P(p, 0, w), s.P(p, 1, w)};
+                        std::size_t cell = 0;
+                        if (s.cell.periodic) {
+                            s.cell.wrap(r[0], r[1]);
+                            for (std::size_t i = 0; i < 2; ++i) {
+                                double f = 0.0;
+                                f += s.cell.ainv[i * 2 + 0] * r[0];
+                                f += s.cell.ainv[i * 2 + 1] * r[1];
+                                const double x = f + 0.5;
+                                std::size_t bin = static_cast<std::size_t>(x * static_cast<double>(d.n_bins));
+                                if (bin >= d.n_bins) bin = d.n_bins - 1;
+                                cell = cell * d.n_bins + bin;
+                            }
+                        } else {
+                            bool in_box = true;
+                            for (std::size_t i = 0; i < 2; ++i) {
+                                const double x = r[i] + d.rmax;
+                                if (x < 0.0 || x >= two_rmax) {
+                                    in_box = false;
+                                    break;
+                                }
+                                std::size_t bin = static_cast<std::size_t>(x * d.inv_delta);
+                                if (bin >= d.n_bins) bin = d.n_bins - 1;
+                                cell = cell * d.n_bins + bin;
+                            }
+                            if (!in_box) continue;
+                        }
+                        hst[cell] += wt[w];
+                    }
+            }
+            double ws = 0.0;
+            for (std::size_t w = 0; w < nw; ++w) ws += wt[w];
+            d.wsum += ws;
         }
         if (s.sofk.on) {   // structure_factor.cpp:62-105
             auto& f = s.sofk;

@@ -24,6 +24,7 @@
 #include <string>
 #include <vector>
 
+#include "estimators/density.h"
 #include "estimators/pair_correlation.h"
 #include "estimators/structure_factor.h"
 #include "hamiltonian/coulomb.h"
@@ -543,16 +544,51 @@ int ref_reblock(const double* x, int n, int n_chains, double* out, int* iout) {
 // ref_set_positions does) with weights w + m * nw; then finalize().  The
 // RunResult series come back flattened: names joined by '\n' into `names`,
 // values concatenated into `values`, lengths[n_series].
+namespace {
+int run_estimators(void* h, std::vector<std::unique_ptr<alea::Estimator>>& ests, int n_blocks, int steps,
+                   const double* pos, const double* w, char* names, int names_cap, double* values,
+                   int values_cap, int* lengths, int lengths_cap, int* n_series);
+}
+
 int ref_estimators(void* h, double rmax, int n_bins, double kmax, int n_blocks, int steps,
                    const double* pos, const double* w, char* names, int names_cap, double* values,
                    int values_cap, int* lengths, int lengths_cap, int* n_series) {
-    return guarded([&] {
+    std::vector<std::unique_ptr<alea::Estimator>> ests;
+    const int rc = guarded([&] {
         auto& s = *static_cast<RefSystem*>(h);
-        std::vector<std::unique_ptr<alea::Estimator>> ests;
         if (rmax > 0.0)
             ests.push_back(std::make_unique<alea::PairCorrelation>(s.groups, s.dim, rmax, n_bins, s.cell));
         if (kmax > 0.0)
             ests.push_back(std::make_unique<alea::StructureFactor>(s.groups, s.cell, kmax));
+    });
+    if (rc) return rc;
+    return run_estimators(h, ests, n_blocks, steps, pos, w, names, names_cap, values, values_cap, lengths,
+                          lengths_cap, n_series);
+}
+
+// Density (density.h:55, per group, lab-frame grid or the periodic cell) over
+// the same measurement sequence as ref_estimators.
+int ref_density(void* h, double rmax, int n_bins, int n_blocks, int steps, const double* pos, const double* w,
+                char* names, int names_cap, double* values, int values_cap, int* lengths, int lengths_cap,
+                int* n_series) {
+    std::vector<std::unique_ptr<alea::Estimator>> ests;
+    const int rc = guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        ests.push_back(std::make_unique<alea::Density>(s.groups, s.dim, rmax, n_bins, false, s.cell));
+    });
+    if (rc) return rc;
+    return run_estimators(h, ests, n_blocks, steps, pos, w, names, names_cap, values, values_cap, lengths,
+                          lengths_cap, n_series);
+}
+
+}  // extern "C"
+
+namespace {
+int run_estimators(void* h, std::vector<std::unique_ptr<alea::Estimator>>& ests, int n_blocks, int steps,
+                   const double* pos, const double* w, char* names, int names_cap, double* values,
+                   int values_cap, int* lengths, int lengths_cap, int* n_series) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
         const std::size_t len = s.np * s.dim * s.nw;
         for (int b = 0; b < n_blocks; ++b) {
             for (auto& e : ests) e->start_block();
@@ -583,5 +619,4 @@ int ref_estimators(void* h, double rmax, int n_bins, double kmax, int n_blocks, 
         *n_series = ns;
     });
 }
-
-}  // extern "C"
+}  // namespace

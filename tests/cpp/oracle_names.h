@@ -36,6 +36,8 @@
 #define alea_set_weights oracle_set_weights
 #define alea_branch oracle_branch
 #define alea_add_gofr oracle_add_gofr
+#define alea_add_density oracle_add_density
+#define alea_estimators_read_density oracle_estimators_read_density
 #define alea_add_sofk oracle_add_sofk
 #define alea_get_sofk_vectors oracle_get_sofk_vectors
 #define alea_estimators_start_block oracle_estimators_start_block

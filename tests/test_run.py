@@ -53,7 +53,9 @@ def test_rows_round_trip_and_seeding():
 @pytest.mark.parametrize("edit,msg", [
     (lambda c: c["qmc"][0].update(random_walk="langevin"), "random_walk"),
     (lambda c: c["qmc"][1].update(reweighting="suppressed"), "reweighting"),
-    (lambda c: c["qmc"][0].update(estimators=[{"type": "density"}]), "density"),
+    (lambda c: c["qmc"][0].update(estimators=[{"type": "density", "rmax": 2, "n_bins": 8, "per_particle": True}]),
+     "per-particle"),
+    (lambda c: c["qmc"][0].update(estimators=[{"type": "bond_order"}]), "bond_order"),
     (lambda c: c["qmc"][1].update(estimators=[{"type": "gofr", "rmax": 1, "n_bins": 4, "name": "x"}]), "named"),
     (lambda c: c["qmc"].insert(0, {"method": "optimizer"}), "optimizer"),
     (lambda c: c["qmc"][0].update(time_step=0), "time_step"),
@@ -83,12 +85,14 @@ def test_exciton_config_runs_exactly():
 
 @pytest.mark.gpu
 def test_config_estimators_run_on_the_device():
-    """A config's gofr and sofk estimators: VMC of the free plane-wave
-    eigenstate, g(r) and S(k) finite with block errors; the DMC stage of the
+    """A config's gofr, sofk and density estimators: VMC of the free
+    plane-wave eigenstate, g(r) and S(k) finite with block errors and a
+    density that integrates to the particle count; the DMC stage of the
     exciton measures g_eh(r) with its weights."""
     import copy
     cfg = copy.deepcopy(PW_FREE)
-    cfg["qmc"][0]["estimators"] = [{"type": "gofr", "rmax": 2.0, "n_bins": 20}, {"type": "sofk", "kmax": 12.0}]
+    cfg["qmc"][0]["estimators"] = [{"type": "gofr", "rmax": 2.0, "n_bins": 20}, {"type": "sofk", "kmax": 12.0},
+                                   {"type": "density", "n_bins": 8}]
     print("run: plane-wave VMC with gofr and sofk", flush=True)
     (vmc,) = run.run(cfg)
     est = vmc["estimators"]
@@ -98,6 +102,10 @@ def test_config_estimators_run_on_the_device():
     assert g[0] < 0.2 * g[-5:].mean()
     s = np.array(est["sofk_electron_group_electron_group"])
     assert s.size == len(est["sofk_k"]) and np.all(s >= 0.0)
+    rho = np.array(est["density_electron_group"])
+    assert rho.shape == (64,) and len(est["density_edges"]) == 9 and np.all(rho >= 0.0)
+    # the cell's area is N pi rs^2 = 7 pi: the density integrates to N = 7
+    assert abs(rho.sum() * (7.0 * math.pi / 64.0) - 7.0) < 1e-9
     ex = copy.deepcopy(EXCITON)
     ex["qmc"][1]["estimators"] = [{"type": "gofr", "rmax": 3.0, "n_bins": 30}]
     print("run: exciton DMC with gofr", flush=True)
