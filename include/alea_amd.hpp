@@ -473,13 +473,14 @@ inline std::pair<double, double> block_mean_error(std::span<const double> x) {
 }
 
 // ── Estimators ─────────────────────────────────────────────────────────────
-// PairCorrelation (pair_correlation.cpp) and StructureFactor
-// (structure_factor.cpp) behind the Estimator interface (estimator.h:48-64).
+// PairCorrelation (pair_correlation.cpp), StructureFactor
+// (structure_factor.cpp) and Density (density.cpp) behind the Estimator
+// interface (estimator.h:48-64).
 // The device keeps each block's running sums (alea_estimators_*); stop_block
 // normalizes them as the reference does and finalize() returns the
 // reference's RunResult series: gofr_r, gofr_<A>_<B>[_error], sofk_k,
 // sofk_shell_count, sofk_<A>_<B>[_error], sofk_kx, sofk_ky, sofk_shell,
-// rhok_<A>_re/_im[_error].
+// rhok_<A>_re/_im[_error], density_edges, density_<A>[_error].
 struct EstimatorGroup {
     std::string name;
     std::size_t first = 0, last = 0;
@@ -524,11 +525,33 @@ public:
         }
     }
 
+    // Density(groups, dim, rmax, n_bins, per_particle = false, cell)
+    // (density.h:55): n_bins^2 cells per group over the periodic cell, or
+    // over [-rmax, rmax)^2 when the cell is open.
+    void add_density(double rmax, int n_bins) {
+        check(alea_add_density(h_, rmax, n_bins));
+        density_ = true;
+        d_rmax_ = rmax;
+        d_bins_ = static_cast<std::size_t>(n_bins);
+        d_cells_ = d_bins_ * d_bins_;
+        if (area_ > 0.0) {
+            d_volume_ = area_ / static_cast<double>(d_cells_);
+        } else {
+            const double delta = 2.0 * rmax / static_cast<double>(n_bins);
+            d_volume_ = 1.0 * delta * delta;
+        }
+        d_mean_.assign(G_ * d_cells_, 0.0);
+        d_m2_.assign(G_ * d_cells_, 0.0);
+        d_blocks_ = 0;
+    }
+
     void start_block() { check(alea_estimators_start_block(h_)); }
     // one measurement: weight 1 per walker (VMC) or the walkers' DMC weights
     void accumulate(bool use_weights = false) { check(alea_estimators_accumulate(h_, use_weights ? 1 : 0)); }
 
     void stop_block() {
+        if (density_) stop_density();
+        if (!gofr_ && !sofk_) return;
         std::vector<double> g(npg_ * n_bins_), s(npg_ * nk_), re(G_ * nk_), im(G_ * nk_);
         double wsum[2] = {0.0, 0.0};
         check(alea_estimators_read(h_, gofr_ ? g.data() : nullptr, sofk_ ? s.data() : nullptr,
@@ -570,6 +593,24 @@ public:
 
     std::vector<Series> finalize() const {
         std::vector<Series> out;
+        if (density_ && d_blocks_ > 0) {   // density.cpp:220-266
+            std::vector<double> edges(d_bins_ + 1);
+            for (std::size_t b = 0; b <= d_bins_; ++b)
+                edges[b] = area_ > 0.0 ? -0.5 + static_cast<double>(b) / static_cast<double>(d_bins_)
+                                       : -d_rmax_ + static_cast<double>(b) * (2.0 * d_rmax_ / static_cast<double>(d_bins_));
+            out.push_back({"density_edges", std::move(edges)});
+            const double n = static_cast<double>(d_blocks_);
+            const double err_norm = d_blocks_ >= 2 ? 1.0 / ((n - 1.0) * n) : 0.0;
+            for (std::size_t g = 0; g < G_; ++g) {
+                std::vector<double> m(d_cells_), e(d_cells_);
+                for (std::size_t c = 0; c < d_cells_; ++c) {
+                    m[c] = d_mean_[g * d_cells_ + c];
+                    e[c] = std::sqrt(d_m2_[g * d_cells_ + c] * err_norm);
+                }
+                out.push_back({"density_" + groups_[g].name, std::move(m)});
+                out.push_back({"density_" + groups_[g].name + "_error", std::move(e)});
+            }
+        }
         std::vector<double> per;
         auto stat = [&](const std::vector<std::vector<double>>& hist, std::size_t q) {
             per.resize(hist.size());
@@ -631,6 +672,31 @@ public:
     std::size_t n_vectors() const { return nk_; }
 
 private:
+    // density.cpp:120-170: x = counts / (W_block V_cell), then the singleton
+    // Welford update of each cell's mean and second moment
+    void stop_density() {
+        std::vector<double> h(G_ * d_cells_);
+        double w = 0.0;
+        check(alea_estimators_read_density(h_, h.data(), &w));
+        const double norm = w > 0.0 ? 1.0 / (w * d_volume_) : 0.0;
+        ++d_blocks_;
+        if (d_blocks_ == 1) {
+            for (std::size_t k = 0; k < h.size(); ++k) {
+                d_mean_[k] = h[k] * norm;
+                d_m2_[k] = 0.0;
+            }
+            return;
+        }
+        const double inv_n = 1.0 / static_cast<double>(d_blocks_);
+        const double vw = static_cast<double>(d_blocks_ - 1) * inv_n;
+        for (std::size_t k = 0; k < h.size(); ++k) {
+            const double x = h[k] * norm;
+            const double d = x - d_mean_[k];
+            d_mean_[k] += d * inv_n;
+            d_m2_[k] += d * (d * vw);
+        }
+    }
+
     static constexpr double kPi = 3.14159265358979323846;
     // |det| of the basis SimulationCell keeps (simulation_cell.cpp:16-36,
     // 100-132): axis-aligned rows reordered, any other basis Gauss-reduced
@@ -671,6 +737,10 @@ private:
     std::vector<int> shell_;
     std::vector<std::size_t> shell_count_;
     std::vector<std::vector<double>> hist_g_, hist_s_, hist_re_, hist_im_;
+    bool density_ = false;
+    double d_rmax_ = 0.0, d_volume_ = 0.0;
+    std::size_t d_bins_ = 0, d_cells_ = 0, d_blocks_ = 0;
+    std::vector<double> d_mean_, d_m2_;
 };
 
 // ── DMC step ───────────────────────────────────────────────────────────────

@@ -11,7 +11,7 @@
 //       apply_move_batch over every particle; VMC sweeps from a tape; DMC
 //       run_one_step with tapes.  Writes every result as "name count values".
 //   hpp_driver estimators <spec> <input> <output>
-//       the Estimators class (g(r), S(k)) over given positions and weights;
+//       the Estimators class (g(r), S(k), density) over given positions and weights;
 //       writes every finalized series.
 //
 // Text files are whitespace-separated tokens; arrays are "count v1 .. vn".
@@ -348,7 +348,10 @@ int estimators(const char* spec_path, const char* in_path, const char* out_path)
     const int n_blocks = static_cast<int>(in.integer());
     const int steps = static_cast<int>(in.integer());
     const bool use_weights = in.integer() != 0;
+    const double d_rmax = in.num();
+    const int d_bins = static_cast<int>(in.integer());
     Estimators est(ws, groups, lattice, rmax, n_bins, kmax);
+    if (d_bins > 0) est.add_density(d_rmax, d_bins);
     for (int b = 0; b < n_blocks; ++b) {
         est.start_block();
         for (int m = 0; m < steps; ++m) {

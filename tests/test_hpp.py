@@ -252,21 +252,22 @@ def test_cpp_interface_matches_python_binding(spec, precision, tmp_path):
 def _est_cases():
     from test_estimators import _spec
     from alea_amd.system import hexagonal_lattice
-    return [pytest.param(_spec(hexagonal_lattice(10, 1.5), (6, 4), "hex"), (3.0, 24, 9.0), id="hex 6+4"),
-            pytest.param(_spec(None, (3, 2), "open"), (2.0, 20, 0.0), id="open 3+2")]
+    return [pytest.param(_spec(hexagonal_lattice(10, 1.5), (6, 4), "hex"), (3.0, 24, 9.0, 0.0, 12), id="hex 6+4"),
+            pytest.param(_spec(None, (3, 2), "open"), (2.0, 20, 0.0, 1.2, 10), id="open 3+2")]
 
 
 def _est_compare(driver, make, spec, args, tmp_path, weighted):
     from test_estimators import _inputs, _run
     nw, blocks, steps = 11, 3, 2
-    rmax, nb, kmax = args
+    rmax, nb, kmax, d_rmax, d_bins = args
     pos, wts = _inputs(spec, nw, blocks, steps, weighted, 21)
     sp, inp, outp = tmp_path / "spec.txt", tmp_path / "in.txt", tmp_path / "out.txt"
     lines = [f"system {nw} {spec.n_particles} 0 " + (_arr(spec.lattice) if spec.lattice is not None else "0")]
     for first, last, q, im in spec.groups:
         lines.append(f"group {first} {last} {q!r} {1.0 / im!r}")
     sp.write_text("\n".join(lines + ["end"]) + "\n")
-    parts = [repr(rmax), str(nb), repr(kmax), str(blocks), str(steps), "1" if weighted else "0"]
+    parts = [repr(rmax), str(nb), repr(kmax), str(blocks), str(steps), "1" if weighted else "0", repr(d_rmax),
+             str(d_bins)]
     for m in range(blocks * steps):
         parts += [_arr(pos[m]), _arr(wts[m] if weighted else np.ones(nw))]
     inp.write_text("\n".join(parts) + "\n")
@@ -275,7 +276,7 @@ def _est_compare(driver, make, spec, args, tmp_path, weighted):
                        timeout=120)
     assert r.returncode == 0, r.stderr
     got = _read_out(outp)
-    kw = dict(gofr=(rmax, nb))
+    kw = dict(gofr=(rmax, nb), density=(d_rmax, d_bins))
     if kmax > 0:
         kw["sofk"] = kmax
     w = make(spec, nw)
