@@ -289,6 +289,64 @@ def dmc_leg(c=DMC_CFG):
     return out
 
 
+CRYSTAL_CFG = dict(rs=60.0, walkers=32768, steps=300, equilibration=60, tau=2.0, vmc_sweeps=20, seed=11,
+                   pos_seed=5, spread=0.15, generations=10.0, alpha=2.0)
+
+
+def crystal_leg(c=CRYSTAL_CFG):
+    """Fixed-node DMC of the polarized Wigner crystal (N = 128 on an 8 x 16
+    triangular lattice at rs = 60; one 128-orbital real Gaussian determinant,
+    Nosanow, Pade + Drummond, Ewald) in FP64 and in mixed form from the same
+    walkers and streams: walkers start at the sites plus N(0, (0.15 a)^2),
+    VMC sweeps, then `steps` DMC steps (one drift-diffusion sweep each) at a
+    target population of `walkers`.  E/N is the mean of the steps after
+    `equilibration` with a reblocked error; walker-moves/s over the DMC
+    steps."""
+    from alea_amd import dmc, statistics
+    from alea_amd.system import site_positions, wigner_crystal_spec
+    from alea_amd.walkers import gpu_walkers, seed as seed_walkers
+    spec = wigner_crystal_spec(c["rs"])
+    n = spec.n_particles
+    a0 = spec.lattice[0] / 16
+    target = c["walkers"]
+    cap = 2 * target + 1
+    pos = site_positions(spec, cap, np.random.default_rng(c["pos_seed"]), c["spread"] * a0)
+    out = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        print(f"bench: crystal DMC N={n} {name}", file=sys.stderr, flush=True)
+        w = gpu_walkers(spec, cap, prec)
+        try:
+            w.set_positions(pos)
+            w.evaluate()
+            w.permute(np.arange(target))
+            seed_walkers(w, c["seed"])
+            w.vmc_sweeps(c["vmc_sweeps"], (0.25 * a0) ** 2)
+            seed_walkers(w, c["seed"], dmc.K_DMC_THREAD_STREAM)
+            d = dmc.DMC(w, c["tau"], target, n_substeps=1, generations=c["generations"], cutoff="universal",
+                        alpha=c["alpha"], seed=c["seed"])
+            series, moves = [], 0
+            w.api("synchronize", w.h)
+            t0 = time.perf_counter()
+            for step in range(c["steps"]):
+                moves += w.nw * n
+                e_sum, w_sum = d.step(measure=step >= c["equilibration"])
+                if step >= c["equilibration"]:
+                    series.append(e_sum / w_sum / n)
+                if step % 100 == 99:
+                    print(f"bench: crystal {name} step {step + 1}, population {w.nw}", file=sys.stderr, flush=True)
+            wall = time.perf_counter() - t0
+            rb = statistics.reblock(series)
+            out[name] = dict(e_per_particle=rb.mean, error=rb.error, plateau_found=rb.plateau_found,
+                             walker_moves_per_s=moves / wall, wall_s=wall, final_population=w.nw)
+        finally:
+            w.close()
+    out["mixed_over_fp64"] = out["mixed"]["walker_moves_per_s"] / out["fp64"]["walker_moves_per_s"]
+    out["mixed_minus_fp64"] = out["mixed"]["e_per_particle"] - out["fp64"]["e_per_particle"]
+    out["config"] = dict(c, system="polarized Wigner crystal N=128 (8 x 16 triangular), rs=60, Gaussian orbitals "
+                                   "c=0.1/rs^1.5, Nosanow, Pade + Drummond, Ewald")
+    return out
+
+
 def estimators_leg(nw=65536, reps=5):
     """One measurement of g(r) (rmax = L/2, 100 bins) and S(k) (|k| <= 3 k_F)
     of the N=122 workload's walkers on the device, `reps` times; milliseconds
@@ -346,6 +404,8 @@ def main():
     ap.add_argument("--serving-only", action="store_true", help="run the serving leg alone, print its dict")
     ap.add_argument("--no-dmc", action="store_true", help="skip the DMC accuracy leg")
     ap.add_argument("--dmc-only", action="store_true", help="run the DMC leg alone, print its dict")
+    ap.add_argument("--crystal-only", action="store_true",
+                    help="run the Wigner-crystal DMC leg (FP64 and mixed) alone, print its dict")
     ap.add_argument("--estimators-only", action="store_true",
                     help="time one g(r) + S(k) measurement of the workload's walkers, print its dict")
     args = ap.parse_args()
@@ -361,6 +421,9 @@ def main():
         return
     if args.dmc_only:
         print(json.dumps(dmc_leg()), flush=True)
+        return
+    if args.crystal_only:
+        print(json.dumps(crystal_leg()), flush=True)
         return
     if args.estimators_only:
         print(json.dumps(estimators_leg(args.walkers)), flush=True)
