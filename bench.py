@@ -348,9 +348,10 @@ def crystal_leg(c=CRYSTAL_CFG):
 
 
 def estimators_leg(nw=65536, reps=5):
-    """One measurement of g(r) (rmax = L/2, 100 bins) and S(k) (|k| <= 3 k_F)
-    of the N=122 workload's walkers on the device, `reps` times; milliseconds
-    per measurement from the wall clock around synchronized batches."""
+    """One measurement of g(r) (rmax = L/2, 100 bins), S(k) (|k| <= 3 k_F)
+    and the density (32 x 32 cells per group) of the N=122 workload's
+    walkers on the device, `reps` times; milliseconds per measurement from
+    the wall clock around synchronized batches."""
     from alea_amd.estimators import Estimators
     from alea_amd.system import uniform_positions
     from alea_amd.walkers import gpu_walkers
@@ -360,7 +361,7 @@ def estimators_leg(nw=65536, reps=5):
         w.set_positions(uniform_positions(spec, nw, np.random.default_rng(POS_SEED)))
         L = spec.lattice[0]
         k_f = math.sqrt(4.0 * math.pi * N_UP / (L * L))
-        est = Estimators(w, gofr=(0.5 * L, 100), sofk=3.0 * k_f)
+        est = Estimators(w, gofr=(0.5 * L, 100), sofk=3.0 * k_f, density=(0.0, 32))
         est.start_block()
         est.accumulate()
         est.stop_block()     # warm-up; reading the sums waits for the stream
@@ -371,7 +372,7 @@ def estimators_leg(nw=65536, reps=5):
         est.stop_block()     # reads the sums back: waits for the stream
         ms = (time.perf_counter() - t0) * 1e3 / reps
         return dict(walkers=nw, n_particles=spec.n_particles, gofr_bins=100, sofk_vectors=est.nk,
-                    ms_per_measurement=ms)
+                    density_cells=32 * 32, ms_per_measurement=ms)
     finally:
         w.close()
 
