@@ -55,7 +55,7 @@ SIGNATURES = {
     "set_weights": (C.c_int, [_V, _P_D]),
     "branch": (C.c_int, [_V, _P_U64, C.c_int, _P_I, _P_I]),
     "add_gofr": (C.c_int, [_V, C.c_double, C.c_int]),
-    "add_density": (C.c_int, [_V, C.c_double, C.c_int]),
+    "add_density": (C.c_int, [_V, C.c_double, C.c_int, C.c_int]),
     "estimators_read_density": (C.c_int, [_V, _P_D, _P_D]),
     "add_sofk": (C.c_int, [_V, C.c_double, _P_I]),
     "get_sofk_vectors": (C.c_int, [_V, _P_D, _P_D, _P_I]),

@@ -48,7 +48,8 @@ void launch_fill(double* a, int n, double v, hipStream_t st);
 void launch_estimators(const DevModel* m, const double* pos, const double* weights, int nw, int chunk,
                        bool gofr, double rmax, double inv_delta, int n_bins, int nfb, bool sofk,
                        const double* kvec, int nk, bool density, double d_rmax, double d_inv_delta, int d_bins,
-                       int d_off, double* part, int part_stride, int n_entries, double* acc, hipStream_t st);
+                       bool d_per_particle, int d_off, double* part, int part_stride, int n_entries, double* acc,
+                       hipStream_t st);
 }  // namespace alea_amd
 
 using namespace alea_amd;
@@ -238,14 +239,15 @@ struct alea_system {
         double rmax = 0.0, inv_delta = 0.0;
         int n_bins = 0, nfb = 0, nk = 0;
         double d_rmax = 0.0, d_inv_delta = 0.0;   // density: window half-width (open cells), bins per length
-        int d_bins = 0;
+        int d_bins = 0, d_rows = 0;   // density: grids (groups, or particles when per particle)
+        bool d_per_particle = false;
         std::vector<double> kx, ky;
         std::vector<int> shell;
         DevBuf<double> acc, part, kvec;
         // [gofr][sofk, rho re, rho im][density]
         int sofk_entries(int ng) const { return sofk ? (ng * (ng + 1) / 2 + 2 * ng) * nk : 0; }
         int density_offset(int ng) const { return nfb + sofk_entries(ng); }
-        int density_entries(int ng) const { return density ? ng * d_bins * d_bins : 0; }
+        int density_entries(int) const { return density ? d_rows * d_bins * d_bins : 0; }
         int entries(int ng) const { return density_offset(ng) + density_entries(ng); }
     } est;
 
@@ -1659,7 +1661,7 @@ int alea_add_gofr(alea_system* s, double rmax, int n_bins) {
 // Density(groups, dim, rmax, n_bins, per_particle = false, cell)
 // (density.h:55, density.cpp:16-64): one n_bins x n_bins grid per group over
 // the periodic cell (rmax unused) or over [-rmax, rmax)^2 in an open cell.
-int alea_add_density(alea_system* s, double rmax, int n_bins) {
+int alea_add_density(alea_system* s, double rmax, int n_bins, int per_particle) {
     return guarded([&] {
         Sys(s);
         const int G = s->host.ng;
@@ -1667,11 +1669,14 @@ int alea_add_density(alea_system* s, double rmax, int n_bins) {
         if (n_bins <= 0) throw std::invalid_argument("alea_add_density: needs n_bins > 0");
         if (!s->host.cell.periodic && !(rmax > 0.0))
             throw std::invalid_argument("alea_add_density: an open cell needs rmax > 0");
-        if (static_cast<long long>(G) * n_bins * n_bins > kMaxDensityEntries)
-            throw std::invalid_argument("alea_add_density: groups x n_bins^2 exceeds " +
+        const int rows = per_particle ? s->np : G;
+        if (static_cast<long long>(rows) * n_bins * n_bins > kMaxDensityEntries)
+            throw std::invalid_argument("alea_add_density: grids x n_bins^2 exceeds " +
                                         std::to_string(kMaxDensityEntries));
         auto& e = s->est;
         e.density = true;
+        e.d_rows = rows;
+        e.d_per_particle = per_particle != 0;
         e.d_rmax = rmax;
         e.d_bins = n_bins;
         e.d_inv_delta = static_cast<double>(n_bins) / (2.0 * rmax);
@@ -1760,8 +1765,8 @@ int alea_estimators_accumulate(alea_system* s, int use_weights) {
         const int n = e.entries(s->host.ng);
         launch_estimators(s->dmodel, s->pos.p, use_weights ? s->weights.p : nullptr, s->nw, chunk, e.gofr,
                           e.rmax, e.inv_delta, e.n_bins, e.nfb, e.sofk, e.kvec.p, e.nk, e.density, e.d_rmax,
-                          e.d_inv_delta, e.d_bins, e.density_offset(s->host.ng), e.part.p, n, n, e.acc.p,
-                          s->stream);
+                          e.d_inv_delta, e.d_bins, e.d_per_particle,
+                          e.density_offset(s->host.ng), e.part.p, n, n, e.acc.p, s->stream);
         hip_check(hipGetLastError(), "estimator launch");
     });
 }

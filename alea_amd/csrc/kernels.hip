@@ -2646,18 +2646,20 @@ __global__ __launch_bounds__(64) void k_sofk(const DevModel* __restrict__ Mp, co
 // fractional coordinates of the wrapped position (wrap as SimulationCell::
 // wrap, f_i = sum_d ainv[i][d] r_d uncontracted, bin int((f + 1/2) n_bins)
 // clamped to the last bin); an open cell bins [-rmax, rmax)^2 and drops the
-// particles outside.  A walker's counts are exact integers in LDS; the thread
+// particles outside; per_particle gives every particle its own grid.  A
+// walker's counts are exact integers in LDS; the thread
 // that opens a cell for the walker folds count x weight into the chunk's
 // FP64 histogram and clears the count, walker after walker, so the sums do
 // not depend on scheduling.
 __global__ __launch_bounds__(kEstThreads) void k_density(const DevModel* __restrict__ Mp, const double* __restrict__ pos,
                                                          const double* __restrict__ weights, int nw, int chunk,
-                                                         double rmax, double inv_delta, int n_bins,
+                                                         double rmax, double inv_delta, int n_bins, int per_particle,
                                                          double* __restrict__ part, int part_stride, int off) {
     const DevModel& M = *Mp;
     __shared__ int cnt[kMaxDensityEntries];
     __shared__ double hist[kMaxDensityEntries];
-    const int np = M.np, n_cells = n_bins * n_bins, nden = M.ng * n_cells, tid = threadIdx.x;
+    const int np = M.np, n_cells = n_bins * n_bins, tid = threadIdx.x;
+    const int nden = (per_particle ? np : M.ng) * n_cells;
     for (int k = tid; k < nden; k += kEstThreads) {
         cnt[k] = 0;
         hist[k] = 0.0;
@@ -2690,7 +2692,7 @@ __global__ __launch_bounds__(kEstThreads) void k_density(const DevModel* __restr
                 const int b1 = min(static_cast<int>(__dmul_rn(x1, inv_delta)), n_bins - 1);
                 cell = b0 * n_bins + b1;
             }
-            const int k = M.group_of[p] * n_cells + cell;
+            const int k = (per_particle ? p : M.group_of[p]) * n_cells + cell;
             // the thread whose increment opens the cell folds it
             idx[q] = atomicAdd(&cnt[k], 1) == 0 ? k : -1;
         }
@@ -2742,11 +2744,12 @@ __global__ __launch_bounds__(kEstThreads) void k_est_reduce(const double* __rest
 void launch_estimators(const DevModel* m, const double* pos, const double* weights, int nw, int chunk,
                        bool gofr, double rmax, double inv_delta, int n_bins, int nfb, bool sofk,
                        const double* kvec, int nk, bool density, double d_rmax, double d_inv_delta, int d_bins,
-                       int d_off, double* part, int part_stride, int n_entries, double* acc, hipStream_t st) {
+                       bool d_per_particle, int d_off, double* part, int part_stride, int n_entries, double* acc,
+                       hipStream_t st) {
     const int n_chunks = (nw + chunk - 1) / chunk;
     if (density)
         hipLaunchKernelGGL(k_density, dim3(n_chunks), dim3(kEstThreads), 0, st, m, pos, weights, nw, chunk, d_rmax,
-                           d_inv_delta, d_bins, part, part_stride, d_off);
+                           d_inv_delta, d_bins, d_per_particle ? 1 : 0, part, part_stride, d_off);
     if (gofr)
         hipLaunchKernelGGL(k_gofr, dim3(n_chunks), dim3(kEstThreads), 0, st, m, pos, weights, nw, chunk, rmax,
                            inv_delta, n_bins, part, part_stride);

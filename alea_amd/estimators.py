@@ -39,8 +39,9 @@ class Estimators:
 
     def __init__(self, walkers, gofr=None, sofk=None, group_names=None, density=None):
         """gofr: (rmax, n_bins) or None; sofk: kmax or None; density:
-        (rmax, n_bins) or None (rmax unused in a periodic cell); group_names:
-        one name per group (default g<first>)."""
+        (rmax, n_bins) or (rmax, n_bins, per_particle) or None (rmax unused in
+        a periodic cell); group_names: one name per group (default
+        g<first>)."""
         self.w = walkers
         api, h = walkers.api, walkers.h
         spec = walkers.spec
@@ -69,15 +70,23 @@ class Estimators:
         self.density = density is not None
         if self.density:
             self.d_rmax, self.d_bins = float(density[0]), int(density[1])
-            api("add_density", h, self.d_rmax, self.d_bins)
+            per_particle = len(density) > 2 and bool(density[2])
+            api("add_density", h, self.d_rmax, self.d_bins, 1 if per_particle else 0)
             self.d_cells = self.d_bins * self.d_bins
+            # one grid per group, or per particle named <group>_p<i> with i
+            # local to the group (density.cpp:42-51)
+            if per_particle:
+                self.d_names = [f"{self.names[g]}_p{p - first}" for g, (first, last) in enumerate(self.groups)
+                                for p in range(first, last)]
+            else:
+                self.d_names = list(self.names)
             if self.lattice is not None:
                 a = cell_vectors(self.lattice)
                 self.d_volume = abs(a[0] * a[3] - a[1] * a[2]) / float(self.d_cells)
             else:
                 delta = 2.0 * self.d_rmax / float(self.d_bins)
                 self.d_volume = 1.0 * delta * delta
-            n = G * self.d_cells
+            n = len(self.d_names) * self.d_cells
             self.d_mean, self.d_m2, self.d_blocks = [0.0] * n, [0.0] * n, 0
         self.hist_g, self.hist_s, self.hist_re, self.hist_im = [], [], [], []
 
@@ -133,7 +142,7 @@ class Estimators:
     def _stop_density(self):
         """density.cpp:120-170: x = counts / (W_block V_cell), then the
         singleton Welford update of each cell's mean and second moment."""
-        n = self.G * self.d_cells
+        n = len(self.d_names) * self.d_cells
         h = np.zeros(n)
         wsum = np.zeros(1)
         self.w.api("estimators_read_density", self.w.h, ptr(h), ptr(wsum))
@@ -164,10 +173,10 @@ class Estimators:
                 out["density_edges"] = [-self.d_rmax + float(b) * delta for b in range(nb + 1)]
             n = float(self.d_blocks)
             err_norm = 1.0 / ((n - 1.0) * n) if self.d_blocks >= 2 else 0.0
-            for g in range(self.G):
+            for g, name in enumerate(self.d_names):
                 sl = slice(g * self.d_cells, (g + 1) * self.d_cells)
-                out[f"density_{self.names[g]}"] = list(self.d_mean[sl])
-                out[f"density_{self.names[g]}_error"] = [math.sqrt(m * err_norm) for m in self.d_m2[sl]]
+                out[f"density_{name}"] = list(self.d_mean[sl])
+                out[f"density_{name}_error"] = [math.sqrt(m * err_norm) for m in self.d_m2[sl]]
         if self.gofr and self.hist_g:
             nb = self.n_bins
             out["gofr_r"] = [(b + 0.5) * self.delta for b in range(nb)]

@@ -78,8 +78,6 @@ def _estimator_args(st: dict) -> dict:
         kind = e.get("type")
         if kind not in ("gofr", "sofk", "density"):
             _refuse(f"estimator '{kind}'")
-        if kind == "density" and e.get("per_particle"):
-            _refuse("a per-particle density")
         if e.get("name"):
             _refuse(f"a named {kind} estimator")
         if kind in args:
@@ -87,7 +85,7 @@ def _estimator_args(st: dict) -> dict:
         if kind == "gofr":
             args["gofr"] = (float(e.get("rmax", 0.0)), int(e.get("n_bins", 0)))
         elif kind == "density":
-            args["density"] = (float(e.get("rmax", 0.0)), int(e.get("n_bins", 0)))
+            args["density"] = (float(e.get("rmax", 0.0)), int(e.get("n_bins", 0)), bool(e.get("per_particle", False)))
         else:
             args["sofk"] = float(e.get("kmax", 0.0))
     return args

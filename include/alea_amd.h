@@ -250,12 +250,13 @@ int alea_add_gofr(alea_system* s, double rmax, int n_bins);
  * reciprocal lattice with |k| <= kmax, sorted by |k|; *n_k receives their
  * number (at most 1024).  Periodic cells only. */
 int alea_add_sofk(alea_system* s, double kmax, int* n_k);
-/* Density(groups, dim, rmax, n_bins, per_particle = false, cell) —
- * src/estimators/density.h:55: one n_bins x n_bins grid per group, C
- * row-major [g * n_bins^2 + b0 * n_bins + b1]; a periodic cell bins the
- * fractional coordinates of the wrapped position over the whole cell (rmax
- * unused), an open cell the window [-rmax, rmax)^2.  G * n_bins^2 <= 4096. */
-int alea_add_density(alea_system* s, double rmax, int n_bins);
+/* Density(groups, dim, rmax, n_bins, per_particle, cell) —
+ * src/estimators/density.h:55: one n_bins x n_bins grid per group (or per
+ * particle when per_particle != 0), C row-major [g * n_bins^2 + b0 * n_bins +
+ * b1]; a periodic cell bins the fractional coordinates of the wrapped
+ * position over the whole cell (rmax unused), an open cell the window
+ * [-rmax, rmax)^2.  (G or N) * n_bins^2 <= 4096. */
+int alea_add_density(alea_system* s, double rmax, int n_bins, int per_particle);
 /* The structure factor's vectors kx, ky [n_k] and shell index [n_k]. */
 int alea_get_sofk_vectors(alea_system* s, double* kx, double* ky, int* shell);
 /* Estimator::start_block of every estimator: the running sums to zero. */
@@ -271,7 +272,7 @@ int alea_estimators_accumulate(alea_system* s, int use_weights);
  * correlation's and the structure factor's normalization). */
 int alea_estimators_read(alea_system* s, double* gofr, double* sofk, double* rho_re, double* rho_im,
                          double* wsum);
-/* The density's running sums: density [G * n_bins^2] sum of weights per
+/* The density's running sums: density [(G or N) * n_bins^2] sum of weights per
  * cell (density.cpp:66-117), *wsum the weight sum. */
 int alea_estimators_read_density(alea_system* s, double* density, double* wsum);
 

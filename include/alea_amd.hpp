@@ -525,12 +525,21 @@ public:
         }
     }
 
-    // Density(groups, dim, rmax, n_bins, per_particle = false, cell)
-    // (density.h:55): n_bins^2 cells per group over the periodic cell, or
-    // over [-rmax, rmax)^2 when the cell is open.
-    void add_density(double rmax, int n_bins) {
-        check(alea_add_density(h_, rmax, n_bins));
+    // Density(groups, dim, rmax, n_bins, per_particle, cell) (density.h:55):
+    // n_bins^2 cells per group (per particle, named <group>_p<i>, when
+    // per_particle) over the periodic cell, or over [-rmax, rmax)^2 when the
+    // cell is open.
+    void add_density(double rmax, int n_bins, bool per_particle = false) {
+        check(alea_add_density(h_, rmax, n_bins, per_particle ? 1 : 0));
         density_ = true;
+        d_names_.clear();
+        for (const auto& g : groups_) {
+            if (!per_particle) {
+                d_names_.push_back(g.name);
+                continue;
+            }
+            for (std::size_t p = g.first; p < g.last; ++p) d_names_.push_back(g.name + "_p" + std::to_string(p - g.first));
+        }
         d_rmax_ = rmax;
         d_bins_ = static_cast<std::size_t>(n_bins);
         d_cells_ = d_bins_ * d_bins_;
@@ -540,8 +549,8 @@ public:
             const double delta = 2.0 * rmax / static_cast<double>(n_bins);
             d_volume_ = 1.0 * delta * delta;
         }
-        d_mean_.assign(G_ * d_cells_, 0.0);
-        d_m2_.assign(G_ * d_cells_, 0.0);
+        d_mean_.assign(d_names_.size() * d_cells_, 0.0);
+        d_m2_.assign(d_names_.size() * d_cells_, 0.0);
         d_blocks_ = 0;
     }
 
@@ -601,14 +610,14 @@ public:
             out.push_back({"density_edges", std::move(edges)});
             const double n = static_cast<double>(d_blocks_);
             const double err_norm = d_blocks_ >= 2 ? 1.0 / ((n - 1.0) * n) : 0.0;
-            for (std::size_t g = 0; g < G_; ++g) {
+            for (std::size_t g = 0; g < d_names_.size(); ++g) {
                 std::vector<double> m(d_cells_), e(d_cells_);
                 for (std::size_t c = 0; c < d_cells_; ++c) {
                     m[c] = d_mean_[g * d_cells_ + c];
                     e[c] = std::sqrt(d_m2_[g * d_cells_ + c] * err_norm);
                 }
-                out.push_back({"density_" + groups_[g].name, std::move(m)});
-                out.push_back({"density_" + groups_[g].name + "_error", std::move(e)});
+                out.push_back({"density_" + d_names_[g], std::move(m)});
+                out.push_back({"density_" + d_names_[g] + "_error", std::move(e)});
             }
         }
         std::vector<double> per;
@@ -675,7 +684,7 @@ private:
     // density.cpp:120-170: x = counts / (W_block V_cell), then the singleton
     // Welford update of each cell's mean and second moment
     void stop_density() {
-        std::vector<double> h(G_ * d_cells_);
+        std::vector<double> h(d_names_.size() * d_cells_);
         double w = 0.0;
         check(alea_estimators_read_density(h_, h.data(), &w));
         const double norm = w > 0.0 ? 1.0 / (w * d_volume_) : 0.0;
@@ -741,6 +750,7 @@ private:
     double d_rmax_ = 0.0, d_volume_ = 0.0;
     std::size_t d_bins_ = 0, d_cells_ = 0, d_blocks_ = 0;
     std::vector<double> d_mean_, d_m2_;
+    std::vector<std::string> d_names_;
 };
 
 // ── DMC step ───────────────────────────────────────────────────────────────

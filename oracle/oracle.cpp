@@ -586,7 +586,8 @@ struct Sys {
         bool on = false;
         double rmax = 0.0, inv_delta = 0.0, wsum = 0.0;
         std::size_t n_bins = 0, n_cells = 0;
-        std::vector<double> hist;   // [g * n_cells + cell]
+        bool per_particle = false;
+        std::vector<double> hist;   // [g * n_cells + cell], g a group or a particle
     } dens;
 
     double& P(std::size_t p, std::size_t d, std::size_t w) { return pos[(p * dim + d) * nw + w]; }
@@ -2442,7 +2443,7 @@ int oracle_get_sofk_vectors(oracle_system* h, double* kx, double* ky, int* shell
 // coordinates f + 1/2 of the wrapped position, an open cell the window
 // [-rmax, rmax)^2 (particles outside it drop); each call's walker weights add
 // to the block's weight sum.
-int oracle_add_density(oracle_system* h, double rmax, int n_bins) {
+int oracle_add_density(oracle_system* h, double rmax, int n_bins, int per_particle) {
     return guarded([&] {
         Sys& s = *S(h);
         if (n_bins <= 0) throw std::invalid_argument("oracle: density needs n_bins > 0");
@@ -2453,7 +2454,8 @@ int oracle_add_density(oracle_system* h, double rmax, int n_bins) {
         d.n_bins = static_cast<std::size_t>(n_bins);
         d.inv_delta = static_cast<double>(n_bins) / (2.0 * rmax);
         d.n_cells = d.n_bins * d.n_bins;
-        d.hist.assign(s.groups.size() * d.n_cells, 0.0);
+        d.per_particle = per_particle != 0;
+        d.hist.assign((d.per_particle ? s.np : s.groups.size()) * d.n_cells, 0.0);
     });
 }
 
@@ -2535,9 +2537,10 @@ int oracle_estimators_accumulate(oracle_system* h, int use_weights) {
         if (s.dens.on) {   // density.cpp:66-117
             auto& d = s.dens;
             const double two_rmax = 2.0 * d.rmax;
+            // per_particle: each particle is its own range (density.cpp:42-51)
             for (std::size_t gg = 0; gg < G; ++gg) {
-                double* hst = d.hist.data() + gg * d.n_cells;
-                for (std::size_t p = s.groups[gg].first; p < s.groups[gg].last; ++p)
+                for (std::size_t p = s.groups[gg].first; p < s.groups[gg].last; ++p) {
+                    double* hst = d.hist.data() + (d.per_particle ? p : gg) * d.n_cells;
                     for (std::size_t w = 0; w < nw; ++w) {
                         double r[2] = {s.P(p, 0, w), s.P(p, 1, w)};
                         std::size_t cell = 0;
@@ -2568,6 +2571,7 @@ int oracle_estimators_accumulate(oracle_system* h, int use_weights) {
                         }
                         hst[cell] += wt[w];
                     }
+                }
             }
             double ws = 0.0;
             for (std::size_t w = 0; w < nw; ++w) ws += wt[w];

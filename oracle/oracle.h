@@ -34,7 +34,7 @@ int oracle_mt19937_64(uint64_t seed, int n, uint64_t* out);
 int oracle_add_gofr(oracle_system* s, double rmax, int n_bins);
 int oracle_add_sofk(oracle_system* s, double kmax, int* n_k);
 int oracle_get_sofk_vectors(oracle_system* s, double* kx, double* ky, int* shell);
-int oracle_add_density(oracle_system* s, double rmax, int n_bins);
+int oracle_add_density(oracle_system* s, double rmax, int n_bins, int per_particle);
 int oracle_estimators_read_density(oracle_system* s, double* density, double* wsum);
 int oracle_estimators_start_block(oracle_system* s);
 int oracle_estimators_accumulate(oracle_system* s, int use_weights);

@@ -568,13 +568,14 @@ int ref_estimators(void* h, double rmax, int n_bins, double kmax, int n_blocks, 
 
 // Density (density.h:55, per group, lab-frame grid or the periodic cell) over
 // the same measurement sequence as ref_estimators.
-int ref_density(void* h, double rmax, int n_bins, int n_blocks, int steps, const double* pos, const double* w,
+int ref_density(void* h, double rmax, int n_bins, int per_particle, int n_blocks, int steps, const double* pos,
+                const double* w,
                 char* names, int names_cap, double* values, int values_cap, int* lengths, int lengths_cap,
                 int* n_series) {
     std::vector<std::unique_ptr<alea::Estimator>> ests;
     const int rc = guarded([&] {
         auto& s = *static_cast<RefSystem*>(h);
-        ests.push_back(std::make_unique<alea::Density>(s.groups, s.dim, rmax, n_bins, false, s.cell));
+        ests.push_back(std::make_unique<alea::Density>(s.groups, s.dim, rmax, n_bins, per_particle != 0, s.cell));
     });
     if (rc) return rc;
     return run_estimators(h, ests, n_blocks, steps, pos, w, names, names_cap, values, values_cap, lengths,

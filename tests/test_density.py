@@ -25,6 +25,9 @@ SYSTEMS = [
     # the open window [-1.2, 1.2)^2 inside positions drawn on [-1.5, 1.5)^2:
     # some particles fall outside and drop
     pytest.param(_spec(None, (3, 2), "open"), dict(density=(1.2, 10)), id="open 3+2"),
+    # one grid per particle, named <group>_p<i> (density.cpp:42-51)
+    pytest.param(_spec(hexagonal_lattice(10, 1.5), (6, 4), "hex"), dict(density=(0.0, 8, True)),
+                 id="hex 6+4 per particle"),
 ]
 
 
@@ -35,8 +38,8 @@ def _ref_density(spec, kw, pos_sets, weight_sets, blocks, steps, nw):
         D = C.POINTER(C.c_double)
         f = lib.ref_density
         f.restype = C.c_int
-        f.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, D, D, C.c_char_p, C.c_int, D, C.c_int,
-                      C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
+        f.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, D, D, C.c_char_p, C.c_int, D,
+                      C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
         pos = np.ascontiguousarray(np.concatenate(pos_sets))
         wts = np.ascontiguousarray(np.concatenate(weight_sets) if weight_sets is not None
                                    else np.ones(blocks * steps * nw))
@@ -44,8 +47,9 @@ def _ref_density(spec, kw, pos_sets, weight_sets, blocks, steps, nw):
         vals = np.empty(1 << 18)
         lens = np.zeros(4096, dtype=np.int32)
         ns = C.c_int()
-        rmax, nb = kw["density"]
-        rc = f(r.h, rmax, nb, blocks, steps, pos.ctypes.data_as(D), wts.ctypes.data_as(D), names, len(names),
+        rmax, nb = kw["density"][:2]
+        per_particle = 1 if len(kw["density"]) > 2 and kw["density"][2] else 0
+        rc = f(r.h, rmax, nb, per_particle, blocks, steps, pos.ctypes.data_as(D), wts.ctypes.data_as(D), names, len(names),
                vals.ctypes.data_as(D), vals.size, lens.ctypes.data_as(C.POINTER(C.c_int)), lens.size, C.byref(ns))
         assert rc == 0, lib.ref_last_error()
         out, off = {}, 0
@@ -112,14 +116,14 @@ def test_density_refusals():
     w = oracle_walkers(spec, 2)
     try:
         with pytest.raises(RuntimeError):
-            w.api("add_density", w.h, 0.0, 0)
+            w.api("add_density", w.h, 0.0, 0, 0)
     finally:
         w.close()
     spec = _spec(None, (3, 2), "open")
     w = oracle_walkers(spec, 2)
     try:
         with pytest.raises(RuntimeError):
-            w.api("add_density", w.h, 0.0, 8)     # an open cell needs a window
+            w.api("add_density", w.h, 0.0, 8, 0)     # an open cell needs a window
     finally:
         w.close()
 

@@ -53,8 +53,6 @@ def test_rows_round_trip_and_seeding():
 @pytest.mark.parametrize("edit,msg", [
     (lambda c: c["qmc"][0].update(random_walk="langevin"), "random_walk"),
     (lambda c: c["qmc"][1].update(reweighting="suppressed"), "reweighting"),
-    (lambda c: c["qmc"][0].update(estimators=[{"type": "density", "rmax": 2, "n_bins": 8, "per_particle": True}]),
-     "per-particle"),
     (lambda c: c["qmc"][0].update(estimators=[{"type": "bond_order"}]), "bond_order"),
     (lambda c: c["qmc"][1].update(estimators=[{"type": "gofr", "rmax": 1, "n_bins": 4, "name": "x"}]), "named"),
     (lambda c: c["qmc"].insert(0, {"method": "optimizer"}), "optimizer"),
