@@ -1308,9 +1308,12 @@ int alea_vmc_sweeps_fused(alea_system* const* systems, int n_systems, int n_swee
         }
         // per sweep: each member's draws, one fused sweep, each member's
         // refresh when it is due
+        // (a member with no walkers has no work: its draws and refresh would
+        // be empty launches, and the sweep kernel never reaches it)
         for (int k = 0; k < n_sweeps; ++k) {
             for (int i = 0; i < n_systems; ++i) {
                 alea_system* s = systems[i];
+                if (s->nw == 0) continue;
                 launch_draw(s->st, s->np, s->nw, s->dtape.p, s->nw, st);
                 hip_check(hipGetLastError(), "launch draw");
             }
@@ -1318,6 +1321,7 @@ int alea_vmc_sweeps_fused(alea_system* const* systems, int n_systems, int n_swee
             hip_check(hipGetLastError(), "launch fused sweep");
             for (int i = 0; i < n_systems; ++i) {
                 alea_system* s = systems[i];
+                if (s->nw == 0) continue;
                 if (++s->since_refresh >= s->refresh_every) s->refresh(st);
             }
         }
