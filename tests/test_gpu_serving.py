@@ -17,13 +17,14 @@ def _job(i, n, nw, arrival=0.0, precision=MIXED, twist=(0.1, 0.2)):
                        seed=1000 + i, precision=precision)
 
 
+@pytest.mark.parametrize("fuse", [True, False], ids=["fused", "per-job graphs"])
 @pytest.mark.parametrize("precision", [FP64, MIXED])
-def test_job_results_same_alone_and_in_mix(precision):
+def test_job_results_same_alone_and_in_mix(precision, fuse):
     alone = _job(0, 26, 512, precision=precision)
-    serving.Server(max_active=4).run([alone])
+    serving.Server(max_active=4, fuse=fuse).run([alone])
     mix = [_job(0, 26, 512, precision=precision), _job(1, 122, 2048, 0.0, precision, (0.13, 0.27)),
            _job(2, 58, 1024, 0.001, precision, (-0.21, 0.05)), _job(3, 26, 4096, 0.002, precision, (0.3, -0.3))]
-    st = serving.Server(max_active=4).run(mix)
+    st = serving.Server(max_active=4, fuse=fuse).run(mix)
     assert st["jobs"] == 4 and st["peak_active"] >= 2          # they did share the device
     assert alone.block_energy == mix[0].block_energy            # bit for bit
     assert all(len(j.block_energy) == 3 and np.all(np.isfinite(j.block_energy)) for j in mix)
