@@ -54,6 +54,12 @@ SIGNATURES = {
     "get_weights": (C.c_int, [_V, _P_D]),
     "set_weights": (C.c_int, [_V, _P_D]),
     "branch": (C.c_int, [_V, _P_U64, C.c_int, _P_I, _P_I]),
+    "rebuild_counts": (C.c_int, [_V, _P_LL, _P_I, C.c_int]),
+    "set_cancellation_factor": (C.c_int, [_V, C.c_double]),
+    "refresh_inverse": (C.c_int, [_V, C.c_int, C.c_int]),
+    "check_status": (C.c_int, [_V]),
+    "set_refresh_updates": (C.c_int, [_V, C.c_int]),
+    "sweep_kernel_ms": (C.c_int, [_V, _P_D, C.c_int, _P_I]),
     "add_gofr": (C.c_int, [_V, C.c_double, C.c_int]),
     "add_density": (C.c_int, [_V, C.c_double, C.c_int, C.c_int]),
     "estimators_read_density": (C.c_int, [_V, _P_D, _P_D]),

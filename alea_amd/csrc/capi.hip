@@ -26,7 +26,7 @@ int set_ziggurat(const uint64_t* k, const double* w, const double* f, double r);
 void launch_evaluate(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, size_t lds,
                      hipStream_t st);
 void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, int nb_max, bool mixed,
-                    bool cplx, size_t lds, hipStream_t st);
+                    bool cplx, int min_updates, hipStream_t st);
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
                   bool cplx, int rows_max, size_t lds, hipStream_t st);
 void launch_draw(const DevState& s, int np, int nw, double* out, int stride, hipStream_t st);
@@ -45,6 +45,7 @@ void launch_dmc_weights(const double* e_new, double* e_old, double* weight, long
 void launch_branch(const double* weight, const double* xi, int* src, int src_cap, int nw,
                    long long* total, hipStream_t st);
 void launch_fill(double* a, int n, double v, hipStream_t st);
+void launch_fill_int(int* a, int n, int v, hipStream_t st);
 void launch_estimators(const DevModel* m, const double* pos, const double* weights, int nw, int chunk,
                        bool gofr, double rmax, double inv_delta, int n_bins, int nfb, bool sofk,
                        const double* kvec, int nk, bool density, double d_rmax, double d_inv_delta, int d_bins,
@@ -174,6 +175,12 @@ struct DevBuf {
 
 }  // namespace
 
+// The mixed form's cancellation factor.  Its inverse is stored in single
+// precision, so a rebuild, which rounds the new inverse to single precision
+// again, cannot lower the rounding floor eps32 * sum|A_mj v_m| of a ratio;
+// the guard fires only where that floor reaches |R| (no correct digit).
+constexpr double kMixedCancelFactor = 1.0;
+
 struct alea_system {
     int capacity = 0, nw = 0, np = 0, precision = 0;
     double lattice[4] = {0, 0, 0, 0};   // primitive vectors as given (rows), periodic cells
@@ -187,7 +194,7 @@ struct alea_system {
     DevBuf<double> weights, e_prev, xi;   // DMC weights, E_L before the sweep, branch uniforms
     DevBuf<char> inv_bytes, inv_spare;    // inverse records; the gather target of a permutation
     DevBuf<double> perm_tmp;              // gather target of the other per-walker arrays
-    DevBuf<int> updates, status, src;
+    DevBuf<int> updates, status, src, rebuilds;
     DevBuf<uint64_t> rng;
     DevBuf<long long> nacc, ncut, branch_total;
     bool energies_valid = false;          // e_loc holds E_L of the current configurations
@@ -198,23 +205,31 @@ struct alea_system {
     DevBuf<double> grad_buf;       // 4 * np * 2 * nw
     DevBuf<double> tape;
     DevBuf<double> dtape;          // one sweep of device-stream numbers, [np][3][nw]
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;   // brackets the last sweep launch
-    bool timed = false;
-    size_t walker_lds = 0, refresh_lds = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;   // brackets the last sweep launch (or graph block)
+    // one pair per sweep of the last eager call: the sweep kernel alone
+    std::vector<hipEvent_t> sweep_ev;
+    int sweeps_timed = 0;
+    bool timed = false, graph_timed = false;
+    size_t walker_lds = 0;
     int rows_max = 0;             // largest padded block row count (0: no determinant)
-    int refresh_every = 2;        // sweeps between batched inverse refreshes
-    long long since_refresh = 0;  // sweeps (or np accept calls) since the last refresh
+    // After every sweep the batched refresh rebuilds each block whose update
+    // count reached refresh_updates (the reference rebuilds at 64,
+    // slater_determinant.cpp:788-790); refresh_every > 0 also rebuilds every
+    // block every refresh_every sweeps.
+    int refresh_updates = 64;
+    int refresh_every = 0;
+    long long since_refresh = 0;  // sweeps (or np accept calls) since the last full refresh
 
     // One captured block of device-stream sweeps (alea_set_sweep_graphs):
     // the key holds everything its launches bake in, compared bytewise; a
     // block whose key differs re-captures.
     struct SweepGraphKey {
-        int n_sweeps, nw, np, dmc, refresh_every, rows_max, precision, cplx;
+        int n_sweeps, nw, np, dmc, refresh_every, refresh_updates, rows_max, precision, cplx;
         long long since_refresh;
         double tau;
         const double* dtape;
         const DevModel* dmodel;
-        size_t walker_lds, refresh_lds;
+        size_t walker_lds;
         DevState st;
     };
     bool sweep_graphs = false;
@@ -343,6 +358,7 @@ struct alea_system {
         ncut.alloc(nw_, stream);
         branch_total.alloc(1, stream);
         status.alloc(nw_, stream);
+        rebuilds.alloc(nw_ * kRebuildCauses, stream);
         nacc.alloc(nw_, stream);
         src.alloc(nw_, stream);
         st.pos = pos.p;
@@ -358,6 +374,7 @@ struct alea_system {
         st.e_loc_im = e_loc_im.p;
         st.rr = rr.p;
         st.status = status.p;
+        st.rebuilds = rebuilds.p;
         st.n_accepted = nacc.p;
         // LDS of one walker: positions, k-space densities, v', column j, the
         // Sherman-Morrison operands and the resident block's row sums (these
@@ -367,13 +384,8 @@ struct alea_system {
                                        9 * static_cast<size_t>(std::max(rows_max, kRowPad)) +
                                        static_cast<size_t>(m.n_pair) * m.spl_stride +
                                        (2 + static_cast<size_t>(m.ng) * m.ng) * m.nkj);
-        {
-            const size_t e = cplx ? 2 : 1, n = static_cast<size_t>(nb_max);
-            refresh_lds = 8 * (e * n * n + e * n + 8) + 4 * (8 + n + (n & 1)) + 8 * n;
-            // blocks of up to 128 orbitals refresh in registers (k_refresh_reg);
-            // the LDS kernel is only sized, never launched, for them
-            if (m.n_blocks > 0 && nb_max > 128) throw InvalidArg("determinant block larger than 128 orbitals");
-        }
+        // the batched refresh holds a block in registers (k_refresh_reg)
+        if (m.n_blocks > 0 && nb_max > kMaxOrb) throw InvalidArg("determinant block larger than 128 orbitals");
         static bool zig_done = false;
         if (!zig_done) {
             const Zig z = make_zig();
@@ -386,11 +398,59 @@ struct alea_system {
 
     void sync() { hip_check(hipStreamSynchronize(stream), "kernel"); }
 
+    // The state of walkers [w0, capacity) as a batch of its own: every
+    // per-walker array is a walker-major record, so a walker range is a
+    // pointer offset (the SoA outputs of the measure kernel are not shifted).
+    DevState shifted(int w0) const {
+        DevState v = st;
+        const size_t w = static_cast<size_t>(w0);
+        const DevModel& m = host;
+        v.pos += w * m.np * 2;
+        v.rho += w * m.ng * m.nkj * 2;
+        v.logs += w * 4;
+        v.inv = static_cast<char*>(v.inv) + w * v.inv_stride * tsize();
+        v.row_abs += w * kMaxBlocks * v.nb_max;
+        v.cond += w * kMaxBlocks;
+        v.updates += w * kMaxBlocks;
+        v.work += w * v.work_stride;
+        v.rng += w * 4;
+        v.e_loc += w;
+        v.e_loc_im += w;
+        v.rr += w * 2;
+        v.status += w;
+        v.n_accepted += w;
+        v.rebuilds += w * kRebuildCauses;
+        v.grad_out = v.lap_out = v.pgrad_out = v.plap_out = nullptr;
+        return v;
+    }
+
+    // The walker range [w_begin, w_end) of a range call; w_end < 0 means nw
+    // (the reference's npos).
+    std::pair<int, int> range(int w_begin, int w_end) const {
+        if (w_end < 0) w_end = nw;
+        if (w_begin < 0 || w_begin > w_end || w_end > nw)
+            throw InvalidArg("walker range [" + std::to_string(w_begin) + ", " + std::to_string(w_end) +
+                             ") outside the batch of " + std::to_string(nw));
+        return {w_begin, w_end};
+    }
+
+    // Every block of every walker.
     void refresh(hipStream_t on = nullptr) {
-        if (host.n_blocks == 0) return;
-        launch_refresh(dmodel, st, nw, host.n_blocks, st.nb_max, mixed(), cplx, refresh_lds, on ? on : stream);
-        hip_check(hipGetLastError(), "launch refresh");
         since_refresh = 0;
+        if (host.n_blocks == 0) return;
+        launch_refresh(dmodel, st, nw, host.n_blocks, st.nb_max, mixed(), cplx, 0, on ? on : stream);
+        hip_check(hipGetLastError(), "launch refresh");
+    }
+    // After a sweep: the full refresh when refresh_every is due, else the
+    // blocks whose update count reached refresh_updates.
+    void refresh_after_sweep(hipStream_t on = nullptr) {
+        if (refresh_every > 0 && ++since_refresh >= refresh_every) {
+            refresh(on);
+            return;
+        }
+        if (host.n_blocks == 0) return;
+        launch_refresh(dmodel, st, nw, host.n_blocks, st.nb_max, mixed(), cplx, refresh_updates, on ? on : stream);
+        hip_check(hipGetLastError(), "launch refresh");
     }
 
     void check_status() {
@@ -555,6 +615,7 @@ int alea_create(alea_system** out, int n_walkers, int n_particles, int dim, cons
         s->capacity = s->nw = n_walkers;
         s->np = n_particles;
         s->precision = precision;
+        s->host.cancel_factor = precision == ALEA_MIXED ? kMixedCancelFactor : 1e4;
         if (lattice) {
             set_cell(s->host.cell, lattice);
             std::copy(lattice, lattice + 4, s->lattice);
@@ -572,6 +633,7 @@ int alea_destroy(alea_system* s) {
     if (st) (void)hipStreamSynchronize(st);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
+    for (hipEvent_t e : s->sweep_ev) (void)hipEventDestroy(e);
     s->drop_graph();
     if (s->fused_join) (void)hipEventDestroy(s->fused_join);
     if (s->dmodel) (void)hipFreeAsync(s->dmodel, st);
@@ -1031,6 +1093,17 @@ int alea_local_energy(alea_system* s, double* e_re, double* e_im) {
         hip_check(copy_on(s->stream, e_re, s->e_loc.p, s->nw * sizeof(double), hipMemcpyDeviceToHost), "e_loc");
         if (e_im)
             hip_check(copy_on(s->stream, e_im, s->e_loc_im.p, s->nw * sizeof(double), hipMemcpyDeviceToHost), "e_loc_im");
+        // energies of a walker whose determinant went singular are not reported
+        s->check_status();
+    });
+}
+
+int alea_check_status(alea_system* s) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->built) return;
+        s->sync();
+        s->check_status();
     });
 }
 
@@ -1079,20 +1152,28 @@ int alea_accept(alea_system* s, int p, const unsigned char* mask) {
         launch_call(s->dmodel, s->st, c, p, 2, s->nw, s->mixed(), s->cplx, s->walker_lds, s->stream);
         hip_check(hipGetLastError(), "launch accept");
         s->energies_valid = false;
-        // one sweep's worth of accept calls counts as a sweep for the refresh
-        if (++s->since_refresh >= static_cast<long long>(s->refresh_every) * s->np) s->refresh();
+        // blocks whose update count reached refresh_updates are rebuilt after
+        // each accept call (the reference rebuilds inside the call,
+        // slater_determinant.cpp:788-790); a forced full refresh counts one
+        // sweep's worth of calls as a sweep
+        if (s->refresh_every > 0 && ++s->since_refresh >= static_cast<long long>(s->refresh_every) * s->np) {
+            s->refresh();
+        } else if (s->host.n_blocks > 0) {
+            launch_refresh(s->dmodel, s->st, s->nw, s->host.n_blocks, s->st.nb_max, s->mixed(), s->cplx,
+                           s->refresh_updates, s->stream);
+            hip_check(hipGetLastError(), "launch refresh");
+        }
         s->sync();
         s->check_status();
     });
 }
 
 // One sweep of the loop below: its k_draw on the device streams, the sweep,
-// then the refresh when it is due.  With `events` the draw and sweep are
-// bracketed by the system's timing events (not inside a graph capture).
+// then the refresh.  With `events` the sweep kernel is bracketed by the i-th
+// pair of the system's timing events (not inside a graph capture).
 static void launch_one_sweep(alea_system* s, const SweepParams& P, int i, int stride, bool events) {
     SweepParams Pi = P;
     Pi.n_sweeps = 1;
-    if (events) hip_check(hipEventRecord(s->ev0, s->stream), "event record");
     if (P.tape) {
         Pi.tape = P.tape + i * static_cast<size_t>(s->np) * 3 * stride;
     } else {
@@ -1101,14 +1182,21 @@ static void launch_one_sweep(alea_system* s, const SweepParams& P, int i, int st
         Pi.tape = s->dtape.p;
         Pi.tape_stride = stride;
     }
+    if (events) hip_check(hipEventRecord(s->sweep_ev[2 * i], s->stream), "event record");
     launch_sweep(s->dmodel, s->st, Pi, s->nw, s->mixed(), s->cplx, s->rows_max, s->walker_lds, s->stream);
     hip_check(hipGetLastError(), "launch sweep");
-    if (events) hip_check(hipEventRecord(s->ev1, s->stream), "event record");
-    if (++s->since_refresh >= s->refresh_every) s->refresh();
+    if (events) hip_check(hipEventRecord(s->sweep_ev[2 * i + 1], s->stream), "event record");
+    s->refresh_after_sweep();
 }
 
 static void run_sweeps_eager(alea_system* s, const SweepParams& P, int stride, int n_sweeps) {
+    while (s->sweep_ev.size() < 2 * static_cast<size_t>(n_sweeps)) {
+        hipEvent_t e = nullptr;
+        hip_check(hipEventCreate(&e), "event");
+        s->sweep_ev.push_back(e);
+    }
     for (int i = 0; i < n_sweeps; ++i) launch_one_sweep(s, P, i, stride, true);
+    s->sweeps_timed = n_sweeps;
 }
 
 // The same launches captured once into a hipGraph and replayed while nothing
@@ -1128,12 +1216,12 @@ static void run_sweep_graph(alea_system* s, const SweepParams& P, int stride, in
     k.precision = s->precision;
     k.cplx = s->cplx ? 1 : 0;
     k.refresh_every = s->refresh_every;
+    k.refresh_updates = s->refresh_updates;
     k.since_refresh = s->since_refresh;
     k.tau = P.tau;
     k.dtape = s->dtape.p;
     k.dmodel = s->dmodel;
     k.walker_lds = s->walker_lds;
-    k.refresh_lds = s->refresh_lds;
     std::memcpy(&k.st, &s->st, sizeof(DevState));
     if (!s->graph_exec || std::memcmp(&k, &s->graph_key, sizeof k) != 0) {
         s->drop_graph();
@@ -1160,6 +1248,7 @@ static void run_sweep_graph(alea_system* s, const SweepParams& P, int stride, in
     hip_check(hipGraphLaunch(s->graph_exec, s->stream), "graph launch");
     hip_check(hipEventRecord(s->ev1, s->stream), "event record");
     s->since_refresh = s->graph_since_after;
+    s->sweeps_timed = 0;
 }
 
 static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape, int stride,
@@ -1197,9 +1286,11 @@ static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* t
     if (!P.tape && s->sweep_graphs && n_sweeps > 0) {
         run_sweep_graph(s, P, stride, n_sweeps);
         s->timed = true;
+        s->graph_timed = true;
     } else {
         run_sweeps_eager(s, P, stride, n_sweeps);
         s->timed = n_sweeps > 0;
+        s->graph_timed = false;
     }
     if (n_accepted) {
 
@@ -1248,11 +1339,47 @@ int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape
     });
 }
 
+int alea_refresh_inverse(alea_system* s, int w_begin, int w_end) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->evaluated) throw InvalidArg("evaluate first");
+        const auto [w0, w1] = s->range(w_begin, w_end);
+        if (s->host.n_blocks == 0 || w1 == w0) return;
+        launch_refresh(s->dmodel, s->shifted(w0), w1 - w0, s->host.n_blocks, s->st.nb_max, s->mixed(), s->cplx, 0,
+                       s->stream);
+        hip_check(hipGetLastError(), "launch refresh");
+        if (w0 == 0 && w1 == s->nw) s->since_refresh = 0;
+        s->energies_valid = false;
+    });
+}
+
+int alea_set_cancellation_factor(alea_system* s, double factor) {
+    return guarded([&] {
+        Sys(s);
+        if (!(factor >= 0.0) || !std::isfinite(factor)) throw InvalidArg("cancellation factor must be finite and >= 0");
+        s->host.cancel_factor = factor;
+        if (s->built)
+            hip_check(hipMemcpyAsync(&s->dmodel->cancel_factor, &s->host.cancel_factor, sizeof(double),
+                                     hipMemcpyHostToDevice, s->stream),
+                      "model cancellation factor");
+        s->sync();
+    });
+}
+
 int alea_set_refresh_interval(alea_system* s, int sweeps) {
     return guarded([&] {
         Sys(s);
-        if (sweeps < 1) throw InvalidArg("refresh interval must be at least one sweep");
+        if (sweeps < 0) throw InvalidArg("refresh interval must not be negative");
         s->refresh_every = sweeps;
+        s->since_refresh = 0;
+    });
+}
+
+int alea_set_refresh_updates(alea_system* s, int updates) {
+    return guarded([&] {
+        Sys(s);
+        if (updates < 1) throw InvalidArg("refresh update count must be at least one");
+        s->refresh_updates = updates;
     });
 }
 
@@ -1322,7 +1449,7 @@ int alea_vmc_sweeps_fused(alea_system* const* systems, int n_systems, int n_swee
             for (int i = 0; i < n_systems; ++i) {
                 alea_system* s = systems[i];
                 if (s->nw == 0) continue;
-                if (++s->since_refresh >= s->refresh_every) s->refresh(st);
+                s->refresh_after_sweep(st);
             }
         }
         // each member's later work waits for the fused block
@@ -1347,14 +1474,63 @@ int alea_synchronize(alea_system* s) {
     });
 }
 
+int alea_rebuild_counts(alea_system* s, long long* counts, int* per_walker, int reset) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->built) throw InvalidArg("no walkers yet: set positions first");
+        const size_t n = static_cast<size_t>(s->capacity) * kRebuildCauses;
+        std::vector<int> c(n);
+        hip_check(copy_on(s->stream, c.data(), s->rebuilds.p, n * sizeof(int), hipMemcpyDeviceToHost), "rebuilds");
+        if (counts)
+            for (int k = 0; k < kRebuildCauses; ++k) {
+                long long t = 0;
+                for (int w = 0; w < s->nw; ++w) t += c[static_cast<size_t>(w) * kRebuildCauses + k];
+                counts[k] = t;
+            }
+        if (per_walker)
+            for (int w = 0; w < s->nw; ++w) {
+                int t = 0;
+                for (int k = 0; k < kRebuildCauses; ++k) t += c[static_cast<size_t>(w) * kRebuildCauses + k];
+                per_walker[w] = t;
+            }
+        if (reset) {
+            // a fill kernel rather than a memset node, so a later graph capture never records one
+            launch_fill_int(s->rebuilds.p, static_cast<int>(n), 0, s->stream);
+            hip_check(hipGetLastError(), "launch fill");
+            s->sync();
+        }
+    });
+}
+
 int alea_last_sweep_ms(alea_system* s, double* ms) {
     return guarded([&] {
         Sys(s);
         if (!s->timed) throw InvalidArg("no sweep has been launched");
-        hip_check(hipEventSynchronize(s->ev1), "event sync");
+        hipEvent_t a = s->ev0, b = s->ev1;
+        if (!s->graph_timed) {
+            a = s->sweep_ev[2 * (s->sweeps_timed - 1)];
+            b = s->sweep_ev[2 * (s->sweeps_timed - 1) + 1];
+        }
+        hip_check(hipEventSynchronize(b), "event sync");
         float t = 0.0f;
-        hip_check(hipEventElapsedTime(&t, s->ev0, s->ev1), "event elapsed");
+        hip_check(hipEventElapsedTime(&t, a, b), "event elapsed");
         *ms = t;
+    });
+}
+
+int alea_sweep_kernel_ms(alea_system* s, double* ms, int capacity, int* n) {
+    return guarded([&] {
+        Sys(s);
+        if (!n || (!ms && capacity > 0)) throw InvalidArg("null output");
+        if (!s->timed || s->graph_timed) throw InvalidArg("no eager sweeps have been launched");
+        const int k = std::min(capacity, s->sweeps_timed);
+        for (int i = 0; i < k; ++i) {
+            hip_check(hipEventSynchronize(s->sweep_ev[2 * i + 1]), "event sync");
+            float t = 0.0f;
+            hip_check(hipEventElapsedTime(&t, s->sweep_ev[2 * i], s->sweep_ev[2 * i + 1]), "event elapsed");
+            ms[i] = t;
+        }
+        *n = s->sweeps_timed;
     });
 }
 

@@ -1082,15 +1082,20 @@ struct Walker {
         __syncthreads();
         // The reference also rebuilds after 64 updates (slater_determinant.cpp:
         // 788-790); here that periodic rebuild is the batched refresh kernel
-        // (k_refresh), which the host runs every few sweeps.  The numerical
-        // triggers stay in the sweep.  The cancellation guard rebuilds when
-        // the rounding bound eps * sum|A_mj v_m| on R exceeds 1/kCancel of
-        // |R|: 1e-4 in FP64 (the reference's constant, slater_determinant.cpp:
-        // 790); 1e-2 in FP32, whose rounding bound passes 1e-4 on about 1% of
-        // the accepted moves of a 61-orbital plane-wave block.
-        constexpr double kCancel = sizeof(T) == 8 ? 1e4 : 1e2;
-        bool rebuild_now = !isfinite(absR) || absR < 1e-5 || (!latched && cond_old > 1e8) ||
-                           absR <= kCancel * eps * d.scale;
+        // (k_refresh_reg), which the host runs every few sweeps.  A tiny or
+        // non-finite ratio, and the cancellation guard (the rounding bound
+        // eps * sum|A_mj v_m| on R above |R| / cancel_factor), rebuild here.
+        // A condition estimate above 1e8 latches the block instead (see
+        // fresh_cond) and leaves its rebuild to the next batched refresh: the
+        // estimate measures the configuration's own conditioning, which a
+        // rebuild does not lower.
+        int cause = -1;   // RebuildCause, uniform across the wave
+        bool latch_now = false;
+        if (!isfinite(absR) || absR < 1e-5) cause = kRebuildTinyRatio;
+        else if (absR <= M.cancel_factor * eps * d.scale) cause = kRebuildCancellation;
+        else if (!latched && cond_old > 1e8) latch_now = true;
+        bool rebuild_now = cause >= 0;
+        if (latch_now && lane == 0) S.rebuilds[w * kRebuildCauses + kRebuildCondBefore] += 1;
         if (!rebuild_now) {
             // pass 1: row product r_c = sum_m v'_m A[m][c] - delta_cj
             const T invRr = static_cast<T>(CPLX ? Rr / (absR * absR) : 1.0 / Rr);
@@ -1209,20 +1214,27 @@ struct Walker {
             for (int kr = lane; kr < n; kr += W) rowmax = fmax(rowmax, rabs[kr]);
             rowmax = wmax(rowmax);
             const double cond = rowmax * colmax;
-            if (!isfinite(cond) || (!latched && cond > 1e8)) {
+            if (!isfinite(cond)) {
                 rebuild_now = true;
+                cause = kRebuildCondAfter;
             } else {
+                if (!latched && !latch_now && cond > 1e8) {
+                    latch_now = true;
+                    if (lane == 0) S.rebuilds[w * kRebuildCauses + kRebuildCondAfter] += 1;
+                }
+                const bool lt = latched || latch_now;
                 log_b(b) += log(absR);
                 if constexpr (NR > 0) {
                     res_upd = updates;
-                    res_cond = latched ? -cond : cond;
+                    res_cond = lt ? -cond : cond;
                 } else if (lane == 0) {
                     *upd = updates;
-                    S.cond[w * kMaxBlocks + b] = latched ? -cond : cond;
+                    S.cond[w * kMaxBlocks + b] = lt ? -cond : cond;
                 }
             }
         }
         if (rebuild_now) {
+            if (lane == 0) S.rebuilds[w * kRebuildCauses + cause] += 1;
             log_b(b) = rebuild(b);
             if constexpr (NR > 0) fill(b);   // the rebuild wrote the block to HBM
         }
@@ -1464,198 +1476,6 @@ __global__ __launch_bounds__(64) void k_evaluate(const DevModel* __restrict__ Mp
     wk.store();
 }
 
-// Batched inverse refresh: one 256-thread workgroup per (walker, block)
-// recomputes D from the current positions and A = D^-1 by pivoted
-// Gauss-Jordan in place in LDS (partial pivoting on the reduced column, first
-// largest |.|, as slater_determinant.cpp:905-993; columns unscrambled in
-// reverse pivot order at the end), in FP64, then stores A in the determinant
-// precision with log|det D| (floored at 1e-300 like evaluate_batch), the row
-// sums of |D|, the condition estimate, and a zero update count.
-template <typename T, bool CPLX>
-__global__ __launch_bounds__(256) void k_refresh(const DevModel* __restrict__ Mp, DevState S) {
-    const DevModel& M = *Mp;
-    extern __shared__ double lds_[];
-    const int nbk = M.n_blocks;
-    const int w = blockIdx.x / nbk, b = blockIdx.x % nbk;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int n = M.blk_n[b], first = M.blk_first[b];
-    constexpr int e = CPLX ? 2 : 1;
-    double* a = lds_;                  // [n][n] (x e)
-    double* fcol = a + e * n * n;      // [n] (x e)
-    double* red_v = fcol + e * n;      // [8]
-    int* red_i = reinterpret_cast<int*>(red_v + 8);   // [8]
-    int* perm = red_i + 8;             // [n]
-    double* rsum = reinterpret_cast<double*>(perm + n + (n & 1));   // [n]
-    const double* pos = S.pos + static_cast<size_t>(w) * M.np * 2;
-    for (int idx = tid; idx < n * n; idx += nt) {
-        const int kr = idx / n, m = idx % n;
-        Orb o;
-        orbital(M, b, m, pos[2 * (first + kr)], pos[2 * (first + kr) + 1], o);
-        a[e * idx] = o.vr;
-        if (CPLX) a[e * idx + 1] = o.vi;
-    }
-    __syncthreads();
-    for (int kr = tid; kr < n; kr += nt) {
-        double rs = 0.0;
-        for (int m = 0; m < n; ++m) {
-            const double re = a[e * (kr * n + m)];
-            const double im = CPLX ? a[e * (kr * n + m) + 1] : 0.0;
-            rs += CPLX ? sqrt(re * re + im * im) : fabs(re);
-        }
-        rsum[kr] = rs;
-    }
-    double log_det = 0.0;
-    bool singular = false;
-    for (int k = 0; k < n; ++k) {
-        // pivot: first row r >= k of largest |a[r][k]|
-        double best = -1.0;
-        int piv = n;
-        for (int r = k + tid; r < n; r += nt) {
-            const double re = a[e * (r * n + k)];
-            const double im = CPLX ? a[e * (r * n + k) + 1] : 0.0;
-            const double v = CPLX ? sqrt(re * re + im * im) : fabs(re);
-            if (v > best) { best = v; piv = r; }
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            const double ob = __shfl_xor(best, o, W);
-            const int op = __shfl_xor(piv, o, W);
-            if (ob > best || (ob == best && op < piv)) { best = ob; piv = op; }
-        }
-        if ((tid & (W - 1)) == 0) {
-            red_v[tid / W] = best;
-            red_i[tid / W] = piv;
-        }
-        __syncthreads();
-        best = red_v[0];
-        piv = red_i[0];
-        for (int q = 1; q < nt / W; ++q)
-            if (red_v[q] > best || (red_v[q] == best && red_i[q] < piv)) { best = red_v[q]; piv = red_i[q]; }
-        if (!(best > 0.0) || !isfinite(best)) { singular = true; break; }
-        log_det += log(best);
-        if (tid == 0) perm[k] = piv;
-        if (piv != k)
-            for (int c = tid; c < n; c += nt)
-                for (int q = 0; q < e; ++q) {
-                    const double t = a[e * (k * n + c) + q];
-                    a[e * (k * n + c) + q] = a[e * (piv * n + c) + q];
-                    a[e * (piv * n + c) + q] = t;
-                }
-        __syncthreads();
-        const double dr = a[e * (k * n + k)];
-        const double di = CPLX ? a[e * (k * n + k) + 1] : 0.0;
-        double pr, pi;
-        if (CPLX) {
-            const double d2 = dr * dr + di * di;
-            pr = dr / d2;
-            pi = -di / d2;
-        } else {
-            pr = 1.0 / dr;
-            pi = 0.0;
-        }
-        __syncthreads();
-        for (int c = tid; c < n; c += nt) {
-            const double xr = c == k ? 1.0 : a[e * (k * n + c)];
-            const double xi = c == k ? 0.0 : (CPLX ? a[e * (k * n + c) + 1] : 0.0);
-            a[e * (k * n + c)] = CPLX ? xr * pr - xi * pi : xr * pr;
-            if (CPLX) a[e * (k * n + c) + 1] = xr * pi + xi * pr;
-        }
-        for (int r = tid; r < n; r += nt) {
-            fcol[e * r] = r == k ? 0.0 : a[e * (r * n + k)];
-            if (CPLX) fcol[e * r + 1] = r == k ? 0.0 : a[e * (r * n + k) + 1];
-        }
-        __syncthreads();
-        for (int idx = tid; idx < n * n; idx += nt) {
-            const int r = idx / n, c = idx % n;
-            if (r == k) continue;
-            const double fr = fcol[e * r], fi = CPLX ? fcol[e * r + 1] : 0.0;
-            const double kr_ = a[e * (k * n + c)], ki = CPLX ? a[e * (k * n + c) + 1] : 0.0;
-            double xr = c == k ? 0.0 : a[e * idx];
-            double xi = c == k ? 0.0 : (CPLX ? a[e * idx + 1] : 0.0);
-            xr -= fr * kr_ - fi * ki;
-            a[e * idx] = xr;
-            if (CPLX) {
-                xi -= fr * ki + fi * kr_;
-                a[e * idx + 1] = xi;
-            }
-        }
-        __syncthreads();
-    }
-    if (singular) {
-        if (tid == 0) S.status[w] = 1;
-        return;
-    }
-    for (int k = n - 1; k >= 0; --k) {
-        const int pk = perm[k];
-        if (pk != k)
-            for (int r = tid; r < n; r += nt)
-                for (int q = 0; q < e; ++q) {
-                    const double t = a[e * (r * n + k) + q];
-                    a[e * (r * n + k) + q] = a[e * (r * n + pk) + q];
-                    a[e * (r * n + pk) + q] = t;
-                }
-        __syncthreads();
-    }
-    // store A[m][c] (row m = orbital) and the norms
-    T* out = static_cast<T*>(S.inv) + static_cast<size_t>(w) * S.inv_stride;
-    for (int i = 0; i < b; ++i) out += e * M.blk_rows[i] * M.blk_n[i];
-    for (int idx = tid; idx < n * n; idx += nt) {
-        out[e * idx] = static_cast<T>(a[e * idx]);
-        if (CPLX) out[e * idx + 1] = static_cast<T>(a[e * idx + 1]);
-    }
-    for (int idx = n * n + tid; idx < M.blk_rows[b] * n; idx += nt) {   // zero padding rows
-        out[e * idx] = T(0);
-        if (CPLX) out[e * idx + 1] = T(0);
-    }
-    double colmax = 0.0;
-    for (int c = tid; c < n; c += nt) {
-        double cs = 0.0;
-        for (int m = 0; m < n; ++m) {
-            const double re = a[e * (m * n + c)];
-            const double im = CPLX ? a[e * (m * n + c) + 1] : 0.0;
-            cs += fabs(re) + fabs(im);
-        }
-        colmax = fmax(colmax, cs);
-    }
-    double rowmax = 0.0;
-    for (int kr = tid; kr < n; kr += nt) {
-        rowmax = fmax(rowmax, rsum[kr]);
-        S.row_abs[(static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max + kr] = rsum[kr];
-    }
-    colmax = wmax(colmax);
-    rowmax = wmax(rowmax);
-    __syncthreads();
-    if ((tid & (W - 1)) == 0) {
-        red_v[tid / W] = colmax;
-        fcol[tid / W] = rowmax;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        for (int q = 1; q < nt / W; ++q) {
-            colmax = fmax(colmax, red_v[q]);
-            rowmax = fmax(rowmax, fcol[q]);
-        }
-        S.cond[w * kMaxBlocks + b] = fresh_cond(rowmax * colmax);
-        S.updates[w * kMaxBlocks + b] = 0;
-        S.logs[w * 4 + 1 + b] = fmax(log_det, log(1e-300));
-    }
-}
-
-// Batched inverse refresh, register-tiled: one GX*16-thread workgroup per
-// (walker, block).  The block is padded to NP x NP with the identity, and
-// thread (tx, ty) = (tid / 16, tid % 16) of GX x 16 holds rows ty*TR.. and
-// columns tx*TC.. of the working matrix in registers (TR = NP / 16,
-// TC = NP / GX), so the 16 owners of a column are adjacent lanes of one wave.  Gauss-Jordan in place
-// with implicit partial pivoting: step k takes the unused row of largest
-// |M[r][k]| (first largest on ties, as slater_determinant.cpp:905-993 picks
-// among the rows not yet pivoted), records it, and only that row and column k
-// cross LDS (double-buffered: two barriers a step).  Column k of the pivot
-// row becomes 1/pivot and of the other rows 0 before the rank-1 update, so
-// at the end A[pinv[r]][perm[k]] = M[r][k] for the real rows and columns.
-// FP64 throughout; A is stored in the determinant precision with log|det D|
-// (floored at 1e-300), the row sums of |D|, the condition estimate and a zero
-// update count, like k_refresh.
-// One DPP step of the pivot search: the larger value wins, ties to the
-// first row.
 template <int CTRL>
 __device__ inline void pivot_combine(double& best, int& bi) {
     const double ob = dpp_mov<CTRL>(best);
@@ -1669,7 +1489,7 @@ __device__ inline void pivot_combine(double& best, int& bi) {
 // Up to 64 orbitals the tile fits 128 registers: four waves per SIMD hide the
 // two barriers of every elimination step better than three.
 template <typename T, bool CPLX, int NP, int GX>
-__global__ __launch_bounds__(GX * 16) __attribute__((amdgpu_waves_per_eu(NP <= 64 ? 4 : 1, 8))) void k_refresh_reg(const DevModel* __restrict__ Mp, DevState S) {
+__global__ __launch_bounds__(GX * 16) __attribute__((amdgpu_waves_per_eu(NP <= 64 ? 4 : 1, 8))) void k_refresh_reg(const DevModel* __restrict__ Mp, DevState S, int min_updates) {
     constexpr int GY = 16;                 // owners of a column: 16 adjacent lanes
     constexpr int TR = NP / GY, TC = NP / GX;   // tile: TR rows x TC columns per thread
     constexpr int E = CPLX ? 2 : 1;
@@ -1682,6 +1502,9 @@ __global__ __launch_bounds__(GX * 16) __attribute__((amdgpu_waves_per_eu(NP <= 6
     __shared__ int piv_sh[2], perm[NP], pinv[NP];
     const int nbk = M.n_blocks;
     const int w = blockIdx.x / nbk, b = blockIdx.x % nbk;
+    // selective refresh: only blocks whose update count reached min_updates
+    // (the whole workgroup reads the same count and leaves together)
+    if (min_updates > 0 && S.updates[w * kMaxBlocks + b] < min_updates) return;
     const int tid = threadIdx.x, tx = tid / GY, ty = tid % GY;
     const int n = M.blk_n[b], first = M.blk_first[b];
     const double* pos = S.pos + static_cast<size_t>(w) * M.np * 2;
@@ -2312,6 +2135,11 @@ __global__ void k_fill(double* __restrict__ a, int n, double v) {
     if (i < n) a[i] = v;
 }
 
+__global__ void k_fill_int(int* __restrict__ a, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 }  // namespace
 
 // ── host launch wrappers ───────────────────────────────────────────────────
@@ -2342,29 +2170,31 @@ void launch_evaluate(const DevModel* m, const DevState& s, int nw, bool mixed, b
 // to 128 (the most a block can have).
 template <int NP, int GX>
 static void launch_refresh_reg(const DevModel* m, const DevState& s, int nw, int n_blocks, bool mixed,
-                               bool cplx, hipStream_t st) {
+                               bool cplx, int min_updates, hipStream_t st) {
     const dim3 g(nw * n_blocks), t(GX * 16);
     if (mixed) {
-        if (cplx) hipLaunchKernelGGL((k_refresh_reg<float, true, NP, GX>), g, t, 0, st, m, s);
-        else hipLaunchKernelGGL((k_refresh_reg<float, false, NP, GX>), g, t, 0, st, m, s);
+        if (cplx) hipLaunchKernelGGL((k_refresh_reg<float, true, NP, GX>), g, t, 0, st, m, s, min_updates);
+        else hipLaunchKernelGGL((k_refresh_reg<float, false, NP, GX>), g, t, 0, st, m, s, min_updates);
     } else {
-        if (cplx) hipLaunchKernelGGL((k_refresh_reg<double, true, NP, GX>), g, t, 0, st, m, s);
-        else hipLaunchKernelGGL((k_refresh_reg<double, false, NP, GX>), g, t, 0, st, m, s);
+        if (cplx) hipLaunchKernelGGL((k_refresh_reg<double, true, NP, GX>), g, t, 0, st, m, s, min_updates);
+        else hipLaunchKernelGGL((k_refresh_reg<double, false, NP, GX>), g, t, 0, st, m, s, min_updates);
     }
 }
 
+// Blocks of up to 128 orbitals (the most the build accepts) refresh in
+// registers: 16 x 16 threads up to 64 orbitals, 32 x 16 up to 128.
+// min_updates > 0 rebuilds only the blocks whose update count reached it.
 void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, int nb_max, bool mixed,
-                    bool cplx, size_t lds, hipStream_t st) {
+                    bool cplx, int min_updates, hipStream_t st) {
+    if (nw == 0 || n_blocks == 0) return;
     if (nb_max <= 16)
-        launch_refresh_reg<16, 16>(m, s, nw, n_blocks, mixed, cplx, st);
+        launch_refresh_reg<16, 16>(m, s, nw, n_blocks, mixed, cplx, min_updates, st);
     else if (nb_max <= 32)
-        launch_refresh_reg<32, 16>(m, s, nw, n_blocks, mixed, cplx, st);
+        launch_refresh_reg<32, 16>(m, s, nw, n_blocks, mixed, cplx, min_updates, st);
     else if (nb_max <= 64)
-        launch_refresh_reg<64, 16>(m, s, nw, n_blocks, mixed, cplx, st);
-    else if (nb_max <= 128)
-        launch_refresh_reg<128, 32>(m, s, nw, n_blocks, mixed, cplx, st);   // 512 threads, 8 x 4 tiles
+        launch_refresh_reg<64, 16>(m, s, nw, n_blocks, mixed, cplx, min_updates, st);
     else
-        DISPATCH(k_refresh, dim3(nw * n_blocks), dim3(256), lds, st, m, s);
+        launch_refresh_reg<128, 32>(m, s, nw, n_blocks, mixed, cplx, min_updates, st);   // 512 threads, 8 x 4 tiles
 }
 // The register tile of the sweep: the smallest NR in {16, 32, 64} that
 // covers every block's padded rows, within 128 32-bit registers per lane for
@@ -2488,6 +2318,9 @@ void launch_branch(const double* weight, const double* xi, int* src, int src_cap
 
 void launch_fill(double* a, int n, double v, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, st, a, n, v);
+}
+void launch_fill_int(int* a, int n, int v, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, st, a, n, v);
 }
 
 // ── Estimators: pair correlation and structure factor ─────────────────────

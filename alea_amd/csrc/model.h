@@ -116,6 +116,12 @@ struct DevModel {
     int ipow_ga[kMaxInversePower] = {}, ipow_gb[kMaxInversePower] = {}, ipow_int[kMaxInversePower] = {};
     double ipow_c[kMaxInversePower] = {}, ipow_p[kMaxInversePower] = {};
     double ipow_tail = 0.0;
+
+    // The cancellation guard of an accepted move: rebuild when |R| <=
+    // cancel_factor * eps * sum_m |A_mj v'_m|, eps the unit roundoff of the
+    // determinant precision (the reference's 1e4 in FP64,
+    // slater_determinant.cpp:788-790).
+    double cancel_factor = 1e4;
 };
 
 // Per-walker state arrays (device pointers).  Strides are per walker.
@@ -138,9 +144,21 @@ struct DevState {
     double* plap_out = nullptr;
     int* status = nullptr;        // [w] nonzero: singular determinant
     long long* n_accepted = nullptr;   // [w]
+    int* rebuilds = nullptr;      // [w][4] in-sweep determinant rebuilds by trigger (RebuildCause)
     int inv_stride = 0;           // T elements per walker in inv
     int work_stride = 0;          // doubles per walker in work
     int nb_max = 0;
+};
+
+// Why an accepted move rebuilt a determinant inside the sweep
+// (slater_determinant.cpp:615-617,788-790,825-826): the index into
+// DevState::rebuilds.
+enum RebuildCause {
+    kRebuildTinyRatio = 0,     // |R| < 1e-5 or R not finite
+    kRebuildCondBefore = 1,    // condition estimate above 1e8 before the update
+    kRebuildCancellation = 2,  // |R| within the rounding bound of its terms
+    kRebuildCondAfter = 3,     // condition estimate above 1e8 (or not finite) after the update
+    kRebuildCauses = 4
 };
 
 // Per-call (parity path) buffers, walker-fastest like the reference.

@@ -156,6 +156,36 @@ class Walkers:
                  C.byref(acc))
         return acc.value
 
+    def refresh_inverse(self, w_begin=0, w_end=-1):
+        """SlaterDeterminant::rebuild_inverse of every block for the walkers
+        [w_begin, w_end) (w_end < 0: to nw)."""
+        self.api("refresh_inverse", self.h, w_begin, w_end)
+
+    def check_status(self):
+        """Raises if a walker's determinant went singular in a rebuild."""
+        self.api("check_status", self.h)
+
+    def set_cancellation_factor(self, factor):
+        self.api("set_cancellation_factor", self.h, float(factor))
+
+    def set_refresh_updates(self, updates):
+        self.api("set_refresh_updates", self.h, int(updates))
+
+    def rebuild_counts(self, reset=True):
+        """Rebuild triggers since the last reset: (counts[4], per-walker totals)."""
+        c = (C.c_longlong * 4)()
+        per = np.zeros(self.nw, dtype=np.int32)
+        self.api("rebuild_counts", self.h, c, ptr(per, _P_I), 1 if reset else 0)
+        return list(c), per
+
+    def sweep_kernel_ms(self):
+        """Sweep-kernel time of every sweep of the last eager sweep call."""
+        n = C.c_int()
+        self.api("sweep_kernel_ms", self.h, None, 0, C.byref(n))
+        out = np.zeros(n.value)
+        self.api("sweep_kernel_ms", self.h, ptr(out), n.value, C.byref(n))
+        return out
+
     def permute(self, src):
         s = i32(src)
         self.api("permute", self.h, len(s), ptr(s, _P_I))

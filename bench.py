@@ -55,44 +55,64 @@ def bytes_per_move(word, acceptance, spec):
     return word * words
 
 
-KERNEL_SAMPLES = 4   # single sweeps timed with HIP events after the timed steps
+EQUIL_SWEEPS = 150   # FP64 VMC sweeps from the uniform start before either form is timed
+EQUIL_SEED = 7
 
 
-def run_gpu(precision, nw, steps, warmup, tau):
+def equilibrated_positions(nw, tau, sweeps=EQUIL_SWEEPS):
+    """Uniform walkers (seed 1) relaxed by `sweeps` FP64 VMC sweeps: the
+    acceptance falls from 0.52 to a plateau of 0.27 and E/N settles by about
+    sweep 120 on this workload.  Both forms start from these walkers."""
     from alea_amd.system import uniform_positions
     from alea_amd.walkers import gpu_walkers, seed
-    import ctypes as C
+    spec, _ = workload_spec()
+    w = gpu_walkers(spec, nw, 0)
+    try:
+        w.set_positions(uniform_positions(spec, nw, np.random.default_rng(POS_SEED)))
+        w.evaluate()
+        seed(w, EQUIL_SEED)
+        for done in range(0, sweeps, 50):
+            acc = w.vmc_sweeps(min(50, sweeps - done), tau)
+            print(f"bench: equilibration sweep {min(done + 50, sweeps)}/{sweeps}, acceptance "
+                  f"{acc / (nw * spec.n_particles * min(50, sweeps - done)):.4f}", file=sys.stderr, flush=True)
+        return w.positions()
+    finally:
+        w.close()
+
+
+def run_gpu(precision, nw, steps, warmup, tau, pos):
+    from alea_amd.walkers import gpu_walkers, seed
     spec, _ = workload_spec()
     w = gpu_walkers(spec, nw, precision)
-    w.set_positions(uniform_positions(spec, nw, np.random.default_rng(POS_SEED)))
-    w.evaluate()
-    seed(w, 42 + precision)
-    api = w.api
-    ms = C.c_double()
-    for _ in range(warmup):
-        w.vmc_sweeps(1, tau)
-    api("synchronize", w.h)
-    # the timed steps are queued back to back, as a VMC run queues its
-    # sweeps (draw, sweep, and the refresh every second sweep), and read back
-    # once: the acceptance count of the same steps
-    t0 = time.perf_counter()
-    acc = w.vmc_sweeps(steps, tau)
-    api("synchronize", w.h)
-    wall = time.perf_counter() - t0
-    acceptance = acc / (nw * spec.n_particles * steps)
-    # untimed: the sweep kernel alone, HIP events around single sweeps
-    kernel_ms = []
-    for _ in range(KERNEL_SAMPLES):
-        w.vmc_sweeps(1, tau)
-        api("last_sweep_ms", w.h, C.byref(ms))
-        kernel_ms.append(ms.value)
-    e, e_im = w.local_energy()
-    out = dict(moves_per_s=nw * spec.n_particles * steps / wall, ms_per_step=1e3 * wall / steps,
-               kernel_ms=float(np.mean(kernel_ms)), acceptance=acceptance,
-               e_per_particle=float(np.mean(e)) / spec.n_particles,
-               e_per_particle_err=float(np.std(e) / math.sqrt(nw)) / spec.n_particles,
-               im_e_per_particle=float(np.mean(e_im)) / spec.n_particles)
-    w.close()
+    try:
+        w.set_positions(pos)
+        w.evaluate()
+        seed(w, 42)
+        for _ in range(warmup):
+            w.vmc_sweeps(1, tau)
+        w.api("synchronize", w.h)
+        w.rebuild_counts(reset=True)
+        # the timed steps are queued back to back, as a VMC run queues its
+        # sweeps (draw, sweep, then the batched refresh of the blocks that
+        # reached 64 updates), and read back once: the acceptance count of
+        # the same steps; each sweep kernel is bracketed by its own events
+        t0 = time.perf_counter()
+        acc = w.vmc_sweeps(steps, tau)
+        w.api("synchronize", w.h)
+        wall = time.perf_counter() - t0
+        kernel_ms = w.sweep_kernel_ms()
+        triggers, _ = w.rebuild_counts(reset=True)
+        acceptance = acc / (nw * spec.n_particles * steps)
+        e, e_im = w.local_energy()
+        out = dict(moves_per_s=nw * spec.n_particles * steps / wall, ms_per_step=1e3 * wall / steps,
+                   kernel_ms=float(np.mean(kernel_ms)), kernel_ms_min=float(np.min(kernel_ms)),
+                   kernel_ms_max=float(np.max(kernel_ms)), acceptance=acceptance,
+                   rebuild_triggers_per_sweep=[t / steps for t in triggers],
+                   e_per_particle=float(np.mean(e)) / spec.n_particles,
+                   e_per_particle_err=float(np.std(e) / math.sqrt(nw)) / spec.n_particles,
+                   im_e_per_particle=float(np.mean(e_im)) / spec.n_particles)
+    finally:
+        w.close()
     return out, spec
 
 
@@ -111,12 +131,17 @@ def _cpu_worker(args):
     return nw * spec.n_particles, time.perf_counter() - t0
 
 
+CPU_SHARE = 16   # host cores one GPU of the box is allotted
+
+
 def cpu_baseline(tau, walkers_per_core=384):
     """The oracle (the FP64 CPU restatement, which follows the reference's
     full complex re-inversion per accepted move) on the host cores: one
-    single-threaded process per core, each sweeping `walkers_per_core` walkers
-    once.  Per-walker CPU cost does not depend on the walker count."""
-    cores = max(1, min(16, os.cpu_count() or 1))
+    single-threaded process per core of the GPU's host-core share, each
+    sweeping `walkers_per_core` walkers once.  Per-walker CPU cost does not
+    depend on the walker count."""
+    nproc = os.cpu_count() or 1
+    cores = max(1, min(CPU_SHARE, nproc))
     jobs = [(walkers_per_core, tau, 100 + i) for i in range(cores)]
     print(f"bench: CPU baseline, {cores} processes x {walkers_per_core} walkers on the oracle", file=sys.stderr,
           flush=True)
@@ -126,10 +151,10 @@ def cpu_baseline(tau, walkers_per_core=384):
     wall = time.perf_counter() - t0
     moves = sum(r[0] for r in res)
     sweep_time = max(r[1] for r in res)
-    return dict(value=moves / sweep_time, unit="walker-moves/s", cores=cores, kind="port",
-                sample=f"{cores} processes x {walkers_per_core} walkers x 1 sweep of the N=122 "
-                       f"workload on the oracle (FP64, complex re-inversion per accept); "
-                       f"{wall:.1f} s wall incl. setup")
+    return dict(value=moves / sweep_time, unit="walker-moves/s", cores=cores, nproc=nproc, kind="port",
+                sample=f"{cores} processes (the {CPU_SHARE}-core host share of one GPU; nproc {nproc}) x "
+                       f"{walkers_per_core} walkers x 1 sweep of the N=122 workload from uniform walkers on "
+                       f"the oracle (FP64, complex re-inversion per accept); {wall:.1f} s wall incl. setup")
 
 
 SERVING = dict(n_jobs=16, rate_per_s=4.0, seed=4, n_blocks=4, sweeps_per_block=10,
@@ -393,8 +418,10 @@ def copy_calibration(n_bytes, reps=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--equilibration", type=int, default=EQUIL_SWEEPS,
+                    help="FP64 sweeps from the uniform start before either form is timed")
     ap.add_argument("--walkers", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
@@ -430,14 +457,15 @@ def main():
         print(json.dumps(estimators_leg(args.walkers)), flush=True)
         return
     tau = (0.5 * RS) ** 2   # sigma = sqrt(tau/m) = 0.5 rs
+    pos = equilibrated_positions(args.walkers, tau, args.equilibration)
     if args.only:
-        out, _ = run_gpu(0 if args.only == "fp64" else 1, args.walkers, args.steps, args.warmup, tau)
+        out, _ = run_gpu(0 if args.only == "fp64" else 1, args.walkers, args.steps, args.warmup, tau, pos)
         print(json.dumps(dict(form=args.only, **out)), flush=True)
         return
     print("bench: FP64 sweeps", file=sys.stderr, flush=True)
-    fp64, spec = run_gpu(0, args.walkers, args.steps, args.warmup, tau)
+    fp64, spec = run_gpu(0, args.walkers, args.steps, args.warmup, tau, pos)
     print(f"bench: FP64 {fp64['moves_per_s']:.4g} walker-moves/s; mixed sweeps", file=sys.stderr, flush=True)
-    mixed, _ = run_gpu(1, args.walkers, args.steps, args.warmup, tau)
+    mixed, _ = run_gpu(1, args.walkers, args.steps, args.warmup, tau, pos)
     print(f"bench: mixed {mixed['moves_per_s']:.4g} walker-moves/s", file=sys.stderr, flush=True)
     b_move = bytes_per_move(4, mixed["acceptance"], spec)
     moves_per_launch = args.walkers * spec.n_particles
@@ -477,19 +505,25 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32+fp64 (mixed)",
-        "data": "synthetic: uniform initial walkers (seed 1), twist from seed 3",
+        "data": f"synthetic: uniform initial walkers (seed 1) equilibrated by {args.equilibration} FP64 VMC "
+                "sweeps, shared by both forms; twist from seed 3",
         "config": {"workload": "twisted 2D HEG VMC sweep, N=122 (61+61), rs=30, Slater-Jastrow-Ewald",
                    "n_walkers": args.walkers, "n_particles": spec.n_particles,
                    "step": "one sweep (every particle of every walker)", "tau": tau},
         "fp64": {"value": fp64["moves_per_s"], "ms_per_step": fp64["ms_per_step"],
-                 "kernel_ms": fp64["kernel_ms"], "acceptance": fp64["acceptance"],
-                 "e_per_particle": fp64["e_per_particle"],
+                 "kernel_ms": fp64["kernel_ms"], "kernel_ms_range": [fp64["kernel_ms_min"], fp64["kernel_ms_max"]],
+                 "acceptance": fp64["acceptance"], "e_per_particle": fp64["e_per_particle"],
+                 "e_per_particle_err": fp64["e_per_particle_err"],
+                 "rebuild_triggers_per_sweep": fp64["rebuild_triggers_per_sweep"],
                  "roofline": {"bound": "hbm", "achieved": achieved64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": achieved64 / HBM_PEAK_GBS, "traffic": traffic_fp64,
                               "kernel": "k_sweep<double,true,false,0>", "bytes_per_move": b_move64}},
         "mixed_over_fp64": mixed["moves_per_s"] / fp64["moves_per_s"],
         "acceptance": mixed["acceptance"],
         "e_per_particle": mixed["e_per_particle"],
+        "e_per_particle_err": mixed["e_per_particle_err"],
+        "rebuild_triggers_per_sweep": mixed["rebuild_triggers_per_sweep"],
+        "kernel_ms_range": [mixed["kernel_ms_min"], mixed["kernel_ms_max"]],
         "im_e_per_particle": mixed["im_e_per_particle"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -151,12 +151,18 @@ int alea_vmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape
 int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape, int tape_stride,
                     double* rr_proposed, double* rr_accepted, long long* n_accepted);
 
-/* Sweeps between two batched recomputations of every determinant inverse
- * (default 2).  The reference rebuilds one walker's inverse after 64 accepted
- * updates (slater_determinant.cpp:788-790); here the periodic rebuild is one
- * batched kernel over all walkers, at most interval * Nb updates apart.  The
- * numerical triggers (tiny or non-finite ratio, cancellation, condition
- * estimate above 1e8) still rebuild inside the sweep. */
+/* The periodic rebuild of SlaterDeterminant::commit_walker
+ * (slater_determinant.cpp:788-790): the reference rebuilds one walker's
+ * inverse inside the move once its block has taken `updates` Sherman-Morrison
+ * updates (64).  Here, after every sweep (and after every alea_accept), one
+ * batched kernel rebuilds every block whose update count reached `updates`
+ * (default 64, both precisions), so a block carries at most updates + Nb - 1
+ * updates.  The numerical triggers (tiny or non-finite ratio, the
+ * cancellation guard) still rebuild inside the sweep. */
+int alea_set_refresh_updates(alea_system* s, int updates);
+
+/* sweeps > 0 also rebuilds every block of every walker after every `sweeps`
+ * sweeps (0, the default: only the update-count rule above). */
 int alea_set_refresh_interval(alea_system* s, int sweeps);
 
 /* n_sweeps VMC sweeps (as alea_vmc_sweeps on the device streams, tape NULL,
@@ -183,6 +189,14 @@ int alea_vmc_sweeps_fused(alea_system* const* systems, int n_systems, int n_swee
  * path for repeated blocks. */
 int alea_set_sweep_graphs(alea_system* s, int on);
 
+/* Waits for the system's stream, then fails with ALEA_ERR_RUNTIME and the
+ * reference's message (slater_determinant.cpp:927-928) if any walker's
+ * determinant became singular or non-finite in a rebuild since evaluate.  The
+ * asynchronous paths (sweeps with n_accepted NULL, fused sweeps, graph
+ * replays) leave this check to the caller; alea_local_energy and the
+ * synchronous sweeps make it themselves. */
+int alea_check_status(alea_system* s);
+
 /* Waits for the system's stream (hipStreamSynchronize). */
 int alea_synchronize(alea_system* s);
 /* Non-blocking: *done = 1 when all work queued on the system's stream has
@@ -191,10 +205,48 @@ int alea_synchronize(alea_system* s);
  * n_accepted NULL, alea_vmc_sweeps only queues its launches.  No counterpart
  * in the reference: the serving loop's completion poll. */
 int alea_query(alea_system* s, int* done);
-/* Device time of the last alea_vmc_sweeps / alea_dmc_sweeps launch, measured
- * with HIP events on the system's stream (no counterpart in the reference:
- * the measurement hook of the benchmark). */
+/* Device time of the sweep kernel of the last sweep of the last
+ * alea_vmc_sweeps / alea_dmc_sweeps call (or, for a replayed sweep graph, of
+ * the whole block), measured with HIP events on the system's stream (no
+ * counterpart in the reference: the measurement hook of the benchmark). */
 int alea_last_sweep_ms(alea_system* s, double* ms);
+/* The sweep-kernel time of every sweep of the last eager call, ms[i] for
+ * sweep i < min(capacity, *n); *n receives the call's sweep count (ms may be
+ * NULL with capacity 0 to ask for the count alone). */
+int alea_sweep_kernel_ms(alea_system* s, double* ms, int capacity, int* n);
+
+/* Rebuild triggers of accepted moves since the last reset, by kind
+ * (slater_determinant.cpp:615-617,788-790,825-826): counts[0] ratio below
+ * 1e-5 or not finite, [1] condition estimate above 1e8 before the update,
+ * [2] the cancellation guard, [3] condition estimate above 1e8 (or not
+ * finite) after the update.  Kinds 0 and 2, and a non-finite estimate,
+ * rebuild the inverse inside the sweep; an estimate above 1e8 latches the
+ * block and leaves its rebuild to the next batched refresh.  per_walker [nw]
+ * (may be NULL) receives each walker slot's total (a permutation does not
+ * move the counts).  reset != 0 clears them.  Synchronous.  No counterpart in
+ * the reference, which does not count its rebuilds: the measurement hook for
+ * the rebuild rate. */
+int alea_rebuild_counts(alea_system* s, long long* counts, int* per_walker, int reset);
+
+/* SlaterDeterminant::rebuild_inverse (slater_determinant.cpp:905-993, declared
+ * at slater_determinant.h:142) for every determinant block of the walkers
+ * [w_begin, w_end) (w_end < 0: to nw, the reference's npos): D from the
+ * current positions, A = D^-1 by pivoted Gauss-Jordan in FP64, stored in the
+ * determinant precision with log|det|, the row sums and the condition
+ * estimate; update counts to zero.  Asynchronous on the system's stream; a
+ * singular block sets the walker's status (reported by the next synchronous
+ * sweep or alea_check_status). */
+int alea_refresh_inverse(alea_system* s, int w_begin, int w_end);
+
+/* The cancellation guard of SlaterDeterminant::commit_walker
+ * (slater_determinant.cpp:788-790): an accepted move rebuilds the inverse
+ * when |R| <= factor * eps * sum_m |A_mj v'_m|, eps the unit roundoff of the
+ * precision that stores A.  The reference's factor is 1e4 (FP64), the
+ * default of ALEA_FP64.  ALEA_MIXED defaults to 1: its inverse is rounded to
+ * FP32 again by any rebuild, so below that floor a rebuild buys nothing, and
+ * the guard fires only where the ratio has no correct digit.  0 turns the
+ * guard off. */
+int alea_set_cancellation_factor(alea_system* s, double factor);
 
 /* Per-walker xoshiro256++ streams: walker w gets stream_seed(seed, stream, w)
  * (random.h:52-56) expanded by splitmix64 (xoshiro_random.h:29-32). */

@@ -497,6 +497,9 @@ struct Component {
     virtual void ratio(Sys& s, std::size_t p) = 0;
     virtual void accept(Sys& s, std::size_t p, const std::vector<double>& mask) = 0;
     virtual void refresh_particle(Sys&, std::size_t) {}
+    // rebuild_inverse of every block for the walkers [w0, w1)
+    // (slater_determinant.cpp:905-993); factors without an inverse do nothing
+    virtual void refresh_inverse(Sys&, std::size_t, std::size_t) {}
     virtual void permute(const std::vector<std::size_t>& src, std::size_t old_nw) = 0;
     virtual bool complex() const { return false; }
     std::vector<double> log_psi_c, grad_c, lap_c, pgrad_c, plap_c;
@@ -1337,6 +1340,30 @@ struct RealSlater : Component {
             particle_derivs(s, B, j, w);
         }
     }
+    // rebuild_inverse (slater_determinant.cpp:905-993) of every block of the
+    // walkers [w0, w1): log|det| floored as evaluate_batch floors it, the
+    // condition estimate and the particles' derivatives from the new inverse
+    void refresh_inverse(Sys& s, std::size_t w0, std::size_t w1) override {
+        for (std::size_t w = w0; w < w1; ++w) {
+            double total = 0.0;
+            for (auto& B : blocks) {
+                const std::size_t N = B.N;
+                B.logd[w] = std::max(rebuild(B, w), std::log(kDetFloor));
+                total += B.logd[w];
+                const double* A = B.A.data() + w * N * N;
+                double norm_a = 0.0, norm_d = 0.0;
+                for (std::size_t a = 0; a < N; ++a) {
+                    double rs = 0.0;
+                    for (std::size_t c = 0; c < N; ++c) rs += std::abs(A[a * N + c]);
+                    norm_a = std::max(norm_a, rs);
+                    norm_d = std::max(norm_d, B.row_abs[a * s.nw + w]);
+                }
+                B.cond[w] = norm_d * norm_a;
+                for (std::size_t kr = 0; kr < N; ++kr) particle_derivs(s, B, kr, w);
+            }
+            log_psi_c[w] = total;
+        }
+    }
     void refresh_particle(Sys& s, std::size_t p) override {
         const std::size_t b = block_of(p);
         if (b == SIZE_MAX) return;
@@ -1424,6 +1451,15 @@ struct ComplexSlater : Component {
                 pgrad_c[idx] = g.imag();
                 plap_c[idx] = l.imag();
             }
+        }
+    }
+    // a fresh inverse of every block of the walkers [w0, w1) (the rebuild of
+    // complex_slater_determinant.cpp:245-253), log|det| floored as
+    // evaluate_batch floors it
+    void refresh_inverse(Sys& s, std::size_t w0, std::size_t w1) override {
+        for (std::size_t w = w0; w < w1; ++w) {
+            log_psi_c[w] = 0.0;
+            for (auto& B : blocks) full(s, B, w, SIZE_MAX, true);
         }
     }
     void evaluate(Sys& s) override {
@@ -2282,6 +2318,22 @@ int oracle_permute(oracle_system* h, int new_nw, const int* src) {
         s.rr_a = ra;
         s.n_cut = nc;
     });
+}
+
+int oracle_refresh_inverse(oracle_system* h, int w_begin, int w_end) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        const long long nw = static_cast<long long>(s.nw);
+        const long long w1 = w_end < 0 ? nw : w_end;
+        if (w_begin < 0 || w_begin > w1 || w1 > nw) throw std::invalid_argument("walker range outside the batch");
+        for (auto& c : s.comps) c->refresh_inverse(s, static_cast<std::size_t>(w_begin), static_cast<std::size_t>(w1));
+    });
+}
+
+// The oracle throws at the singular rebuild itself (slater_determinant.cpp:
+// 927-928), so nothing is pending here.
+int oracle_check_status(oracle_system* h) {
+    return guarded([&] { S(h); });
 }
 
 int oracle_get_num_walkers(oracle_system* h, int* nw) {

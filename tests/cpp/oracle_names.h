@@ -43,3 +43,5 @@
 #define alea_estimators_start_block oracle_estimators_start_block
 #define alea_estimators_accumulate oracle_estimators_accumulate
 #define alea_estimators_read oracle_estimators_read
+#define alea_refresh_inverse oracle_refresh_inverse
+#define alea_check_status oracle_check_status

@@ -148,7 +148,11 @@ def test_mixed_trajectories_track_fp64_on_the_n122_workload():
     """The mixed form follows the FP64 form's Metropolis decisions on the
     benchmark system (twisted N=122, rs=30, 61-orbital complex blocks) from the
     same tape: at least 98% of walkers take identical trajectories over four
-    sweeps, and on those the local energy per particle agrees to 1e-7 Ha."""
+    sweeps.  On those, the local energy per particle read from the swept
+    single-precision inverse (rounded, then carried through the
+    Sherman-Morrison updates) agrees to 1e-5 Ha per walker and to 1e-7 Ha in
+    the mean; after refresh_inverse (a fresh FP64 Gauss-Jordan of the same
+    positions, rounded once) it agrees to 1e-7 Ha per walker."""
     s = heg_spec(61, 61, 30.0, twist=tuple(np.random.default_rng(3).uniform(-0.5, 0.5, 2)))
     nw, n_sw = 256, 4
     pos = _pos(s, nw, 8)
@@ -159,12 +163,39 @@ def test_mixed_trajectories_track_fp64_on_the_n122_workload():
         x.set_positions(pos)
         x.evaluate()
         x.vmc_sweeps(n_sw, (0.5 * 30.0) ** 2, tape)
-        out.append((x.positions(), x.local_energy()[0]))
+        swept = x.local_energy()[0]
+        x.refresh_inverse()
+        out.append((x.positions(), swept, x.local_energy()[0]))
         x.close()
     same = np.all(np.abs(out[0][0] - out[1][0]).reshape(-1, nw) < 1e-9, axis=0)
     assert same.mean() >= 0.98
-    de = np.abs(out[0][1] - out[1][1])[same] / s.n_particles
-    assert de.max() < 1e-7
+    de = (out[1][1] - out[0][1])[same] / s.n_particles
+    assert np.abs(de).max() < 1e-5
+    assert abs(de.mean()) < 1e-7
+    de_fresh = np.abs(out[1][2] - out[0][2])[same] / s.n_particles
+    assert de_fresh.max() < 1e-7
+
+
+def test_refresh_inverse_over_a_walker_range_matches_the_oracle():
+    """rebuild_inverse on the walkers [w_begin, w_end) only
+    (slater_determinant.cpp:905-993): after sweeps, the device and the oracle
+    refresh the same range; log|psi| and the local energies agree to 1e-10,
+    and walkers outside the range keep their swept inverses (the next sweep
+    from one tape still agrees)."""
+    s = heg_spec(5, 5, 2.0, twist=(0.13, 0.27))
+    nw = 24
+    pos = _pos(s, nw, 31)
+    g, o = gpu_walkers(s, nw), oracle_walkers(s, nw)
+    for x in (g, o):
+        x.set_positions(pos)
+        x.evaluate()
+        x.vmc_sweeps(2, 1.0, oracle_tape(5, s.n_particles, nw, 2))
+        x.refresh_inverse(7, 19)
+    assert rel_err(g.log_psi(), o.log_psi()) < TOL
+    assert rel_err(g.local_energy()[0], o.local_energy()[0]) < TOL
+    tape = oracle_tape(6, s.n_particles, nw, 1)
+    assert g.vmc_sweeps(1, 1.0, tape) == o.vmc_sweeps(1, 1.0, tape)
+    assert rel_err(g.log_psi(), o.log_psi()) < TOL
 
 
 def test_largest_complex_blocks_evaluate_like_the_oracle():
@@ -245,3 +276,27 @@ def test_wigner_crystal_mixed_tracks_fp64():
     assert rel_err(e1, e0) < 1e-7
     for w in ws:
         w.close()
+
+
+def test_a_singular_commit_is_reported():
+    """A move committed onto another same-spin particle makes the block
+    singular: the rebuild that the tiny ratio triggers fails, alea_accept
+    raises the reference's runtime_error (slater_determinant.cpp:927-928),
+    and so do alea_check_status and alea_local_energy afterwards (the serving
+    loop reads its block energies through the latter)."""
+    s = heg_spec(5, 5, 2.0)
+    nw = 4
+    pos = _pos(s, nw, 2).reshape(s.n_particles, 2, nw)
+    g = gpu_walkers(s, nw)
+    g.set_positions(pos.reshape(-1))
+    g.evaluate()
+    new = pos[0].copy()           # particle 1 proposed onto particle 0 (same spin)
+    g.ratio(1, new.reshape(-1))
+    mask = np.zeros(nw, dtype=np.uint8)
+    mask[2] = 1
+    with pytest.raises(RuntimeError, match="singular"):
+        g.accept(1, mask)
+    with pytest.raises(RuntimeError, match="singular"):
+        g.check_status()
+    with pytest.raises(RuntimeError, match="singular"):
+        g.local_energy()

@@ -323,3 +323,28 @@ def test_inverse_power_tail_on_a_triangular_crystal():
     d = d[(d > 0) & (d < Rc)]
     exact = 0.5 * (np.sum(d ** -3.0) + 2 * np.pi * 1.0 / Rc)
     assert -7.0e-4 < e / exact - 1 < -6.8e-4
+
+
+@pytest.mark.parametrize("twist", [None, (0.13, 0.27)])
+def test_refresh_inverse_equals_a_fresh_evaluation(twist):
+    """rebuild_inverse (slater_determinant.cpp:905-993) on a walker range: after
+    sweeps, the refreshed walkers' log|psi| and local energies equal those of a
+    fresh evaluation of the same positions (1e-10), real and complex
+    determinants; walkers outside the range are left as they were."""
+    s = heg_spec(5, 5, 2.0, twist=twist)
+    nw = 16
+    o = oracle_walkers(s, nw)
+    o.set_positions(uniform_positions(s, nw, np.random.default_rng(4)))
+    o.evaluate()
+    o.vmc_sweeps(3, 1.0, oracle_tape(9, s.n_particles, nw, 3))
+    before = o.log_psi()
+    o.refresh_inverse(4, 12)
+    after = o.log_psi()
+    f = oracle_walkers(s, nw)
+    f.set_positions(o.positions())
+    f.evaluate()
+    assert rel_err(after[4:12], f.log_psi()[4:12]) < 1e-10
+    assert rel_err(o.local_energy()[0][4:12], f.local_energy()[0][4:12]) < 1e-10
+    assert np.array_equal(after[:4], before[:4]) and np.array_equal(after[12:], before[12:])
+    with pytest.raises(RuntimeError, match="range"):
+        o.refresh_inverse(10, nw + 1)
