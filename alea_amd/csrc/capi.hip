@@ -12,6 +12,7 @@
 #include <exception>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <numbers>
 #include <stdexcept>
 #include <string>
@@ -1302,6 +1303,72 @@ static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* t
         *n_accepted = t;
         s->check_status();
     }
+}
+
+// ── VMC sweeps that allocate nothing ─────────────────────────────────────
+// The device-stream sweep stages each sweep's random numbers (np * 3 per
+// walker) between k_draw and k_sweep.  alea_vmc_sweeps keeps that tape in a
+// buffer it allocates on first use; alea_vmc_sweeps_static stages it in this
+// file-scope array instead, a chunk of walkers at a time.  Each GPU holds its
+// own copy of the array until the process ends.
+namespace {
+constexpr size_t kStaticTapeDoubles = size_t(4) << 20;   // 32 MiB per GPU
+__device__ __align__(256) double scratch[kStaticTapeDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this GPU's copy of `scratch` (the host-side name is
+// only a stand-in): looked up once per device, with that device current.
+double* device_scratch() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw DeviceError("device index out of range for the static tape");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+int alea_vmc_sweeps_static(alea_system* s, int n_sweeps, double tau) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->evaluated) throw InvalidArg("evaluate first");
+        if (n_sweeps < 0) throw InvalidArg("negative sweep count");
+        if (!s->seeded) throw InvalidArg("seed the device streams first (alea_seed)");
+        const size_t per_walker = static_cast<size_t>(s->np) * 3;
+        if (per_walker > kStaticTapeDoubles) throw InvalidArg("too many particles for the static tape");
+        if (n_sweeps == 0 || s->nw == 0) return;
+        double* tape = device_scratch();
+        // walkers per chunk: as many as the array holds (positive: np <= 256)
+        const int chunk = static_cast<int>(std::min<size_t>(kStaticTapeDoubles / per_walker, s->nw));
+        s->energies_valid = false;
+        s->set_model_nw(s->nw);
+        SweepParams P;
+        P.n_sweeps = 1;
+        P.tau = tau;
+        P.dmc = 0;
+        P.tape = tape;
+        for (int k = 0; k < n_sweeps; ++k) {
+            for (int w0 = 0; w0 < s->nw; w0 += chunk) {
+                const int n = std::min(chunk, s->nw - w0);
+                const DevState st = s->shifted(w0);
+                // k_draw writes every number the sweep then reads: nothing
+                // in the array needs clearing between calls
+                launch_draw(st, s->np, n, tape, n, s->stream);
+                hip_check(hipGetLastError(), "launch draw");
+                P.tape_stride = n;
+                launch_sweep(s->dmodel, st, P, n, s->mixed(), s->cplx, s->rows_max, s->walker_lds, s->stream);
+                hip_check(hipGetLastError(), "launch sweep");
+            }
+            s->refresh_after_sweep();
+        }
+    });
 }
 
 int alea_vmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape,

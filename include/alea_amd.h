@@ -142,6 +142,19 @@ int alea_accept(alea_system* s, int p, const unsigned char* mask);
  * NULL the call returns once the sweeps are queued (see alea_query). */
 int alea_vmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape,
                     long long* n_accepted);
+/* alea_vmc_sweeps on the device streams (tape NULL, no acceptance read-back)
+ * that allocates nothing: the random numbers of each sweep are staged in a
+ * fixed 32 MiB device array of the library (one per GPU, held until the
+ * process ends) instead of a buffer allocated on first use, a chunk of
+ * walkers at a time (4 Mi doubles / (3 np) walkers per chunk).  Results are
+ * bit-identical to alea_vmc_sweeps.  Returns once the launches are queued on
+ * the system's stream.  Calls on one GPU share the array: calls that overlap
+ * in time on one GPU (systems on different streams) race and must be
+ * ordered by the caller.  The system must be evaluated and seeded.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops). */
+int alea_vmc_sweeps_static(alea_system* s, int n_sweeps, double tau);
+
 /* DMC::sweep_walkers (dmc.cpp:203-337): Umrigar-scaled drift, node gate for a
  * real trial, Metropolis-Hastings with the drift Green's function.
  * rr_proposed and rr_accepted [nw] are accumulated into the given host arrays;

@@ -30,7 +30,8 @@ def oracle_lib():
         if not os.path.exists(ORACLE_SO):
             pytest.fail("oracle/liboracle.so is not built: run __graft_entry__.build()")
         lib = C.CDLL(ORACLE_SO)
-        _oracle = (lib, CApi(lib, "oracle_", optional=("rebuild_counts", "set_cancellation_factor", "set_refresh_updates", "sweep_kernel_ms")))
+        _oracle = (lib, CApi(lib, "oracle_", optional=("rebuild_counts", "set_cancellation_factor", "set_refresh_updates", "sweep_kernel_ms",
+                                                      "vmc_sweeps_static")))
         for name, res, args in [
             ("oracle_stream_seed", C.c_uint64, [C.c_uint64] * 3),
             ("oracle_xoshiro_state_u64", C.c_int, [_P_U64, C.c_int, _P_U64]),
