@@ -35,7 +35,7 @@ import os
 
 import numpy as np
 
-from .system import Spec, cell_key, load_tables, pade_key, plane_wave_shells, twisted_wavevectors
+from .system import Spec, ewald_table, pade_table, plane_wave_shells, twisted_wavevectors
 
 _U64 = (1 << 64) - 1
 
@@ -303,9 +303,10 @@ def build_spec(cfg, tables=None, requests: list | None = None) -> Spec:
     the Hamiltonian, with every derived value (cusps, k-points, breakup
     tables) the reference's builders derive.
 
-    With `requests` (a list), the breakup tables the config needs are
-    appended to it as ("pade", lattice, a, b) and ("ewald", lattice, tol)
-    instead of being looked up; the Spec then holds no tables."""
+    The breakup tables come from alea_amd.breakup, or from `tables` (one
+    cell's {"ewald", "pade"}) when given.  With `requests` (a list), the
+    tables the config needs are appended to it as ("pade", lattice, a, b) and
+    ("ewald", lattice, tol) instead; the Spec then holds no tables."""
     if any(s.get("method") == "pimc" for s in _stages(cfg)) or cfg.get("action"):
         _refuse("a PIMC run")
     groups = build_groups(cfg)
@@ -324,17 +325,7 @@ def build_spec(cfg, tables=None, requests: list | None = None) -> Spec:
     gindex = {g["name"]: i for i, g in enumerate(groups)}
     for g in groups:
         s.groups.append((g["first"], g["last"], g["charge"], 1.0 / g["mass"]))
-    cell = None
-    if periodic and requests is None:
-        tables = tables or load_tables()
-        cell = tables["cells"].get(cell_key(lattice))
     ewald = periodic and any(t[0] == "coulomb" for t in terms)
-
-    def need_cell():
-        if cell is None:
-            raise KeyError(f"no breakup tables for cell {cell_key(lattice)}; add it to "
-                           "alea_amd/tables/make_tables.py and run it")
-        return cell
 
     # ── wavefunction ──
     for item in cfg.get("wavefunction", []) or []:
@@ -360,10 +351,7 @@ def build_spec(cfg, tables=None, requests: list | None = None) -> Spec:
                     if requests is not None:
                         requests.append(("pade", lattice, num, den))
                     else:
-                        key = pade_key(num, den)
-                        image = need_cell()["pade"].get(key)
-                        if image is None:
-                            raise KeyError(f"no image-sum table {key} for cell {cell_key(lattice)}")
+                        image = pade_table(lattice, num, den, tables)
                 s.pade.append(dict(ga=gindex[a_name], gb=gindex[b_name], num=len(num), den=len(den),
                                    coeffs=num + den, image=image))
             elif jt == "drummond":
@@ -431,10 +419,9 @@ def build_spec(cfg, tables=None, requests: list | None = None) -> Spec:
         if requests is not None:
             requests.append(("ewald", lattice, tol))
         else:
-            e = need_cell()["ewald"]
+            e = ewald_table(lattice, tables, tol)
             if abs(e["tol"] - tol) > 1e-12 * tol:
-                raise KeyError(f"the Ewald table of cell {cell_key(lattice)} has tolerance {e['tol']}, "
-                               f"the config asks {tol}")
+                raise ValueError(f"the given Ewald table has tolerance {e['tol']}, the config asks {tol}")
             s.ewald = e
     for t in terms:
         if t[0] == "coulomb" and not periodic:

@@ -29,7 +29,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .system import dipole_spec, heg_spec, uniform_positions
+from .system import dipole_spec, heg_spec, square_lattice, twisted_wavevectors, uniform_positions
 from .walkers import FP64, MIXED, gpu_walkers, seed as seed_walkers, set_sweep_graphs, vmc_sweeps_fused
 
 MAX_FUSED = 8    # systems in one fused launch (kMaxFused of the C-ABI)
@@ -111,11 +111,13 @@ def poisson_jobs(n_jobs: int, rate: float, seed: int = 4, sizes=(26, 58, 74, 122
         rs = float(rng.choice(rs_choices))
         while True:
             twist = tuple(float(x) for x in rng.uniform(-0.5, 0.5, 2))
-            try:
-                heg_spec(n // 2, n // 2, rs, twist=twist)
+            try:   # a twist that splits a degenerate |k + theta| shell is drawn again
+                twisted_wavevectors(square_lattice(n, rs), n // 2, twist)
                 break
             except ValueError:
                 continue
+        # the cell's breakup tables, built once here rather than at admission
+        heg_spec(n // 2, n // 2, rs, twist=twist)
         lo, hi = math.log2(walkers[0]), math.log2(walkers[1])
         nw = int(round(2.0 ** rng.uniform(lo, hi) / 64.0)) * 64
         # FP32 inverses only for blocks of up to 64 orbitals: the larger

@@ -11,39 +11,53 @@ What is mirrored (reference file:line):
   Ewald                   src/builders/ewald.cpp:40-66 (tolerance 1e-6)
 
 The Natoli-Ceperley breakup tables (Ewald and the image-summed Padé) are
-setup-time inputs.  They are read from alea_amd/tables/cells.json, which the
-reference's own breakup produced (alea_amd/tables/make_tables.py); a drop-in
-caller passes the tables its own builders computed.
+built on the host by alea_amd.breakup for any cell; a caller may pass tables
+of its own instead (`tables=`: {"ewald": ..., "pade": {pade_key: ...}}), as a
+drop-in caller passes the tables its builders computed.
 """
 from __future__ import annotations
 
-import json
 import math
-import os
 from dataclasses import dataclass, field
 
 import numpy as np
 
-TABLES = os.path.join(os.path.dirname(__file__), "tables", "cells.json")
-
-
-def cell_key(lattice) -> str:
-    return ",".join(f"{x:.12g}" for x in lattice)
+from . import breakup
 
 
 def pade_key(a, b) -> str:
     return "a=" + ",".join(f"{x:.15g}" for x in a) + ";b=" + ",".join(f"{x:.15g}" for x in b)
 
 
-_tables_cache = None
+def ewald_table(lattice, tables=None, tol=1e-6):
+    """The Ewald set-up of a cell: from `tables` when given, else the
+    product's own breakup at tolerance tol (the builders' default 1e-6)."""
+    if tables is not None:
+        return tables["ewald"]
+    return breakup.ewald_tables(lattice, tol)
 
 
-def load_tables():
-    global _tables_cache
-    if _tables_cache is None:
-        with open(TABLES) as f:
-            _tables_cache = json.load(f)
-    return _tables_cache
+def pade_table(lattice, a, b, tables=None):
+    """The image-sum split of the Padé (a, b) in a cell: from `tables` when
+    given, else the product's own breakup."""
+    if tables is not None:
+        return tables["pade"][pade_key(a, b)]
+    return breakup.pade_tables(lattice, a, b)
+
+
+def ewald_table_tightest(lattice, tables=None, tols=(1e-6, 1e-5, 1e-4, 1e-3, 1e-2, 3e-2)):
+    """The Ewald set-up at the tightest tolerance of `tols` the cell's
+    breakup reaches (an elongated cell, whose Wigner-Seitz inradius leaves
+    the fit little room, does not reach 1e-6)."""
+    if tables is not None:
+        return tables["ewald"]
+    last = None
+    for tol in tols:
+        try:
+            return breakup.ewald_tables(lattice, tol)
+        except ValueError as e:
+            last = e
+    raise last
 
 
 def square_lattice(n_particles: int, rs: float):
@@ -159,13 +173,10 @@ class Spec:
 
 def heg_spec(n_up: int, n_dn: int, rs: float, twist=None, slater=True, jastrow=True,
              ewald=True, tables=None, lattice=None, drummond=None) -> Spec:
-    """The 2-D paramagnetic (or polarized) electron gas of the HEG inputs."""
+    """The 2-D paramagnetic (or polarized) electron gas of the HEG inputs;
+    `tables` (one cell's {"ewald", "pade"}) replaces the product's breakup."""
     N = n_up + n_dn
     lat = lattice or square_lattice(N, rs)
-    tables = tables or load_tables()
-    cell = tables["cells"].get(cell_key(lat))
-    if cell is None and (jastrow or ewald):
-        raise KeyError(f"no breakup tables for cell {cell_key(lat)}; run alea_amd/tables/make_tables.py")
     s = Spec(n_particles=N, lattice=lat, name=f"heg N={N} rs={rs}")
     s.groups.append((0, n_up, -1.0, 1.0))
     if n_dn:
@@ -176,7 +187,7 @@ def heg_spec(n_up: int, n_dn: int, rs: float, twist=None, slater=True, jastrow=T
         if n_dn:
             pairs += [(1, 1, 1.0 / 3.0), (0, 1, 1.0)]
         for ga, gb, a1 in pairs:
-            tab = cell["pade"][pade_key([a1], b)]
+            tab = pade_table(lat, [a1], b, tables)
             s.pade.append(dict(ga=ga, gb=gb, num=1, den=1, coeffs=[a1] + b, image=tab))
     if drummond is not None:
         s.drummond.extend(drummond)
@@ -189,7 +200,7 @@ def heg_spec(n_up: int, n_dn: int, rs: float, twist=None, slater=True, jastrow=T
             else:
                 s.slater_twisted.append((first, last, twisted_wavevectors(lat, n, twist)))
     if ewald:
-        s.ewald = cell["ewald"]
+        s.ewald = ewald_table(lat, tables)
     return s
 
 
@@ -209,10 +220,6 @@ def wigner_crystal_spec(rs: float = 60.0, rows: int = 8, cols: int = 16, nosanow
     n = rows * cols
     a0 = rs * math.sqrt(2.0 * math.pi / math.sqrt(3.0))
     lat = [cols * a0, 0.0, 0.0, rows * a0 * math.sqrt(3.0) / 2.0]
-    tables = tables or load_tables()
-    cell = tables["cells"].get(cell_key(lat))
-    if cell is None:
-        raise KeyError(f"no breakup tables for cell {cell_key(lat)}; run alea_amd/tables/make_tables.py")
     sites = np.array([((c + 0.5 * (r % 2)) * a0, r * a0 * math.sqrt(3.0) / 2.0)
                       for r in range(rows) for c in range(cols)])
     sites -= 0.5 * np.array([lat[0], lat[3]])   # centred on the origin, like the cell
@@ -221,7 +228,7 @@ def wigner_crystal_spec(rs: float = 60.0, rows: int = 8, cols: int = 16, nosanow
     s.groups.append((0, n, -1.0, 1.0))
     b = [0.5 / rs]
     s.pade.append(dict(ga=0, gb=0, num=1, den=1, coeffs=[1.0 / 3.0] + b,
-                       image=cell["pade"][pade_key([1.0 / 3.0], b)]))
+                       image=pade_table(lat, [1.0 / 3.0], b, tables)))
     half = 0.5 * min(lat[0], lat[3])
     rng = np.random.default_rng(drummond_seed)
     C = 3
@@ -232,7 +239,7 @@ def wigner_crystal_spec(rs: float = 60.0, rows: int = 8, cols: int = 16, nosanow
     s.nosanow.append(dict(group=0, alpha=c if nosanow_alpha is None else nosanow_alpha,
                           sites=sites.reshape(-1).copy()))
     s.slater_gauss.append((0, n, sites.copy(), np.full(n, c)))
-    s.ewald = cell["ewald"]
+    s.ewald = ewald_table_tightest(lat, tables)
     s.sites = sites
     return s
 

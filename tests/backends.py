@@ -22,6 +22,20 @@ ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libalea_ref.so")
 
 _oracle = None
+_fixture_cells = None
+
+
+def fixture_tables(lattice):
+    """One cell's breakup tables as the reference's own set-up code wrote
+    them (tests/golden/cells.json, by tests/golden/make_tables.py):
+    {"ewald": ..., "pade": {pade_key: ...}}, or None for a cell with no tables
+    (no Coulomb term)."""
+    global _fixture_cells
+    if _fixture_cells is None:
+        import json
+        with open(os.path.join(ROOT, "tests", "golden", "cells.json")) as f:
+            _fixture_cells = json.load(f)["cells"]
+    return _fixture_cells.get(",".join(f"{x:.12g}" for x in lattice))
 
 
 def oracle_lib():

@@ -1,5 +1,6 @@
-"""Writes alea_amd/tables/cells.json: the setup-time breakup tables of every
-cell the workloads and tests use.
+"""Writes tests/golden/cells.json: the setup-time breakup tables the
+reference's own code produces for every cell the workloads and tests use,
+the fixtures that pin the product's own breakup (alea_amd/breakup.py).
 
 The tables are what the reference's own setup code produces for each cell:
 numerics::build_ewald (src/numerics/ewald_breakup.cpp:205-276, tolerance 1e-6,
@@ -7,7 +8,7 @@ Coulomb, epsilon 1) and the image-sum split of each Padé u = a r/(1 + b r)
 (src/wavefunction/periodic_pair_sum.cpp:16-64).  They are computed by running
 that setup code, compiled from the reference sources by oracle/Makefile
 (target `ref`), through oracle/ref_harness.cpp.  Run in a tree that has
-/root/reference:  make -C oracle ref && python alea_amd/tables/make_tables.py
+/root/reference:  make -C oracle ref && python tests/golden/make_tables.py
 """
 import ctypes as C
 import json
@@ -18,7 +19,11 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
-from alea_amd.system import cell_key, hexagonal_lattice, pade_key, square_lattice  # noqa: E402
+from alea_amd.system import hexagonal_lattice, pade_key, square_lattice  # noqa: E402
+
+
+def cell_key(lattice) -> str:
+    return ",".join(f"{x:.12g}" for x in lattice)
 
 CAP_K, CAP_KNOTS = 1024, 256
 

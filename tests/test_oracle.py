@@ -20,8 +20,9 @@ import math
 import numpy as np
 import pytest
 
-from alea_amd.system import dipole_spec, Spec, exciton_spec, heg_spec, hexagonal_lattice, load_tables, uniform_positions
-from backends import RefWalkers, oracle_lib, oracle_tape, oracle_walkers, ref_lib, rel_err
+from alea_amd import breakup
+from alea_amd.system import dipole_spec, Spec, exciton_spec, heg_spec, hexagonal_lattice, uniform_positions
+from backends import RefWalkers, fixture_tables, oracle_lib, oracle_tape, oracle_walkers, ref_lib, rel_err
 import ctypes as C
 
 XOSHIRO_1234 = [41943041, 58720359, 3588806011781223, 3591011842654386,
@@ -66,8 +67,13 @@ def test_rng_matches_reference_build():
     assert np.array_equal(ta, tb)
 
 
-def test_madelung_square():
-    t = load_tables()["cells"]["2,0,0,2"]["ewald"]
+@pytest.mark.parametrize("source", ["breakup", "reference"])
+def test_madelung_square(source):
+    """The square-lattice Madelung constant 1.6155426267 of the reference's
+    own check (tests/test_ewald.cpp:212-224), from the product's breakup and
+    from the reference's tables (fixture), at tolerance 1e-8."""
+    t = breakup.ewald_tables([2.0, 0, 0, 2.0], 1e-8) if source == "breakup" \
+        else fixture_tables([2.0, 0, 0, 2.0])["ewald"]
     s = Spec(n_particles=4, lattice=[2.0, 0, 0, 2.0], name="madelung")
     s.groups = [(0, 2, 1.0, 1.0), (2, 4, -1.0, 1.0)]
     s.ewald = t
@@ -82,6 +88,15 @@ def test_madelung_square():
     e, _ = w.local_energy()   # no kinetic contribution: no wavefunction factors
     M = -2.0 * e[0] / 4.0
     assert abs(M - 1.6155426267) < 1e-6
+    return M
+
+
+def test_madelung_agrees_between_the_two_breakups():
+    """Two valid fits of the 1/r split give one energy up to their truncation
+    (residual 7e-10 here): the constant from the product's tables and from
+    the reference's agree to 1e-9, well inside the 7e-8 by which both differ
+    from the quoted 10-digit value."""
+    assert abs(test_madelung_square("breakup") - test_madelung_square("reference")) < 1e-9
 
 
 def test_exciton_exact():
@@ -191,12 +206,21 @@ def test_gradients_match_finite_differences():
 
 # ── against the reference's own code ──────────────────────────────────────
 
+def _ref_tables(n, rs, lattice=None):
+    from alea_amd.system import square_lattice
+    return fixture_tables(lattice or square_lattice(n, rs))
+
+
 def _jastrow_ewald_specs():
-    specs = [heg_spec(5, 5, 2.0, slater=False), heg_spec(13, 13, 30.0, slater=False)]
-    hexa = heg_spec(3, 3, 2.0, slater=False, lattice=hexagonal_lattice(6, 2.0))
+    # the reference's code builds its own image sums and Ewald sum, so the
+    # oracle takes the reference's tables here (tests/golden/cells.json)
+    specs = [heg_spec(5, 5, 2.0, slater=False, tables=_ref_tables(10, 2.0)),
+             heg_spec(13, 13, 30.0, slater=False, tables=_ref_tables(26, 30.0))]
+    hl = hexagonal_lattice(6, 2.0)
+    hexa = heg_spec(3, 3, 2.0, slater=False, lattice=hl, tables=_ref_tables(6, 2.0, hl))
     hexa.name = "hexagonal N=6"
     specs.append(hexa)
-    dr = heg_spec(5, 5, 2.0, slater=False, jastrow=False)
+    dr = heg_spec(5, 5, 2.0, slater=False, jastrow=False, tables=_ref_tables(10, 2.0))
     dr.name = "drummond+nosanow N=10"
     L = dr.lattice[0] / 2
     dr.drummond.append(dict(ga=0, gb=1, C=3, gamma=1.0, L=L, coeffs=[1e-3, -2e-4, 3e-5]))
