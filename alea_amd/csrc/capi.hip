@@ -1601,6 +1601,22 @@ int alea_sweep_kernel_ms(alea_system* s, double* ms, int capacity, int* n) {
     });
 }
 
+int alea_draw_tape(alea_system* s, int n_sweeps, double* out) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->seeded) throw InvalidArg("seed the device streams first (alea_seed)");
+        if (n_sweeps < 0 || (n_sweeps > 0 && !out)) throw InvalidArg("bad sweep count or null output");
+        const size_t len = static_cast<size_t>(s->np) * 3 * s->nw;
+        if (s->dtape.n < len) s->dtape.alloc(static_cast<size_t>(s->np) * 3 * s->capacity, s->stream);
+        for (int k = 0; k < n_sweeps; ++k) {
+            launch_draw(s->st, s->np, s->nw, s->dtape.p, s->nw, s->stream);
+            hip_check(hipGetLastError(), "launch draw");
+            hip_check(copy_on(s->stream, out + k * len, s->dtape.p, len * sizeof(double), hipMemcpyDeviceToHost),
+                      "draw tape");
+        }
+    });
+}
+
 int alea_seed(alea_system* s, uint64_t seed, uint64_t stream) {
     return guarded([&] {
         Sys(s);

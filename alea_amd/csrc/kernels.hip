@@ -414,6 +414,8 @@ struct Rng {
         return static_cast<double>(static_cast<int64_t>(next() >> 11)) * 0x1.0p-53;
     }
     __device__ double gaussian() {
+        // the wedge test rounds as the host's does: no fused multiply-adds
+#pragma clang fp contract(off)
         constexpr uint64_t mask = (uint64_t{1} << 52) - 1;
         for (;;) {
             const uint64_t bits = next();

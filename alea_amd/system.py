@@ -204,6 +204,33 @@ def heg_spec(n_up: int, n_dn: int, rs: float, twist=None, slater=True, jastrow=T
     return s
 
 
+def heg_drummond_spec(n_up: int, n_dn: int, rs: float, order: int = 8, sigma: float = 1e-3, seed: int = 2,
+                      tables=None) -> Spec:
+    """The paramagnetic HEG of the published DMC case (N = 74, rs = 40 there):
+    plane-wave Slater determinants, the image-summed Padé pair Jastrow with
+    the Kato cusps, plus a Drummond polynomial (order `order`, C = 3, cutoff
+    L/2) on each spin pair with no cusp of its own, so the Padé factor alone
+    sets the cusp; 2-D Ewald.  The Drummond polynomial is in r itself
+    (drummond_jastrow.cpp:47-64), so coefficients of size 1e-3 would make
+    u ~ 1e20 at r ~ 300: they are drawn N(0, sigma) from `seed` in units of
+    the cutoff, alpha_l = xi_l / L^(C + l) for the term in r^l, so each term
+    is ~sigma at r ~ L."""
+    n = n_up + n_dn
+    half = square_lattice(n, rs)[0] / 2.0
+    rng = np.random.default_rng(seed)
+    C = 3
+    # coeffs[0] multiplies r^0, coeffs[k >= 1] multiplies r^(k+1)
+    powers = [0] + list(range(2, order + 1))
+    dr = []
+    for ga, gb in ((0, 0), (1, 1), (0, 1)):
+        xi = rng.normal(0.0, sigma, order)
+        dr.append(dict(ga=ga, gb=gb, C=C, gamma=0.0, L=half,
+                       coeffs=[float(x) / half ** (C + l) for x, l in zip(xi, powers)]))
+    s = heg_spec(n_up, n_dn, rs, drummond=dr, tables=tables)
+    s.name = f"heg N={n} rs={rs:g} pade+drummond{order}"
+    return s
+
+
 def wigner_crystal_spec(rs: float = 60.0, rows: int = 8, cols: int = 16, nosanow_alpha: float | None = None,
                         drummond_order: int = 8, drummond_sigma: float = 1e-3, drummond_seed: int = 2,
                         tables=None) -> Spec:

@@ -178,6 +178,13 @@ class Walkers:
         self.api("rebuild_counts", self.h, c, ptr(per, _P_I), 1 if reset else 0)
         return list(c), per
 
+    def draw_tape(self, n_sweeps):
+        """The next n_sweeps sweeps' device-stream numbers [sweep][np][3][nw],
+        advancing the streams as the sweeps would (alea_draw_tape)."""
+        out = np.empty(n_sweeps * self.np * 3 * self.nw)
+        self.api("draw_tape", self.h, n_sweeps, ptr(out))
+        return out
+
     def sweep_kernel_ms(self):
         """Sweep-kernel time of every sweep of the last eager sweep call."""
         n = C.c_int()

@@ -187,28 +187,11 @@ DMC_CFG = dict(n_up=37, n_dn=37, rs=40.0, target=16384, tau=2.0, steps=500, equi
 
 
 def dmc_spec(c=DMC_CFG):
-    """The N=74, rs=40 paramagnetic HEG of the published DMC case: plane-wave
-    Slater determinants, the image-summed Pade pair Jastrow with the Kato
-    cusps, plus a Drummond polynomial (order 8, C = 3, cutoff L/2) on each
-    spin pair with no cusp of its own, so the Pade factor alone sets the
-    cusp; 2-D Ewald.  The Drummond polynomial is in r itself
-    (drummond_jastrow.cpp:47-64), so coefficients of size 1e-3 would make
-    u ~ 1e20 at r ~ 300: they are drawn N(0, sigma) in units of the cutoff,
-    alpha_l = xi_l / L^(C + l) for the term in r^l, so each term is ~sigma at
-    r ~ L."""
-    from alea_amd.system import heg_spec, square_lattice
-    n = c["n_up"] + c["n_dn"]
-    half = square_lattice(n, c["rs"])[0] / 2.0
-    rng = np.random.default_rng(c["drummond_seed"])
-    C = 3
-    # coeffs[0] multiplies r^0, coeffs[k >= 1] multiplies r^(k+1)
-    powers = [0] + list(range(2, c["drummond_order"] + 1))
-    dr = []
-    for ga, gb in ((0, 0), (1, 1), (0, 1)):
-        xi = rng.normal(0.0, c["drummond_sigma"], c["drummond_order"])
-        dr.append(dict(ga=ga, gb=gb, C=C, gamma=0.0, L=half,
-                       coeffs=[float(x) / half ** (C + l) for x, l in zip(xi, powers)]))
-    return heg_spec(c["n_up"], c["n_dn"], c["rs"], drummond=dr)
+    """The N=74, rs=40 paramagnetic HEG of the published DMC case
+    (alea_amd.system.heg_drummond_spec)."""
+    from alea_amd.system import heg_drummond_spec
+    return heg_drummond_spec(c["n_up"], c["n_dn"], c["rs"], order=c["drummond_order"],
+                             sigma=c["drummond_sigma"], seed=c["drummond_seed"])
 
 
 def dmc_run(spec, c, prec, pos_seed, seed):

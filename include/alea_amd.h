@@ -265,6 +265,13 @@ int alea_set_cancellation_factor(alea_system* s, double factor);
  * (random.h:52-56) expanded by splitmix64 (xoshiro_random.h:29-32). */
 int alea_seed(alea_system* s, uint64_t seed, uint64_t stream);
 
+/* The numbers the next n_sweeps device-stream sweeps would draw, into out
+ * [sweep][np][dim+1][nw] (draw_sweep_randoms order, sweep_blocking.h:39-48,
+ * over each walker's own stream: per particle two Gaussians then a uniform),
+ * advancing the streams exactly as those sweeps would.  A run recorded this
+ * way replays on any tape-driven path (the oracle's, say).  Synchronous. */
+int alea_draw_tape(alea_system* s, int n_sweeps, double* out);
+
 /* WalkerSet::apply_permutation + TrialWavefunction::permute_caches
  * (walker_set.cpp:109-178): new walker j takes old walker src[j]'s state;
  * new_nw <= capacity. */

@@ -1765,6 +1765,24 @@ int oracle_xoshiro_draws(uint64_t seed, int kind, int n, double* out) {
     return 0;
 }
 
+// The per-walker device streams: walker w draws from its own stream
+// stream_seed(seed, stream, w) (random.h:52-56), per sweep and per particle
+// two Gaussians then a uniform (draw_sweep_randoms, sweep_blocking.h:39-48,
+// over a one-walker range); out [sweep][np][3][nw].
+int oracle_device_tape(uint64_t seed, uint64_t stream, int np, int nw, int n_sweeps, double* out) {
+    for (int w = 0; w < nw; ++w) {
+        Rng r(stream_seed(seed, stream, static_cast<uint64_t>(w)));
+        for (int sw = 0; sw < n_sweeps; ++sw)
+            for (int p = 0; p < np; ++p) {
+                double* row = out + (static_cast<std::size_t>(sw) * np + p) * 3 * nw;
+                row[w] = r.gaussian();
+                row[nw + w] = r.gaussian();
+                row[2 * nw + w] = r.uniform();
+            }
+    }
+    return 0;
+}
+
 int oracle_sweep_tape(uint64_t seed, int np, int dim, int nw, int n_sweeps, double* out) {
     Rng r(seed);
     const std::size_t len = static_cast<std::size_t>(np) * (dim + 1) * nw;

@@ -84,6 +84,7 @@ int oracle_dmc_sweeps(oracle_system* s, int n_sweeps, double tau, const double* 
                       long long* n_accepted);
 int oracle_permute(oracle_system* s, int new_nw, const int* src);
 int oracle_get_num_walkers(oracle_system* s, int* nw);
+int oracle_device_tape(uint64_t seed, uint64_t stream, int np, int nw, int n_sweeps, double* out);
 int oracle_refresh_inverse(oracle_system* s, int w_begin, int w_end);
 int oracle_check_status(oracle_system* s);
 int oracle_dmc_begin(oracle_system* s);

@@ -45,13 +45,15 @@ def oracle_lib():
             pytest.fail("oracle/liboracle.so is not built: run __graft_entry__.build()")
         lib = C.CDLL(ORACLE_SO)
         _oracle = (lib, CApi(lib, "oracle_", optional=("rebuild_counts", "set_cancellation_factor", "set_refresh_updates", "sweep_kernel_ms",
-                                                      "vmc_sweeps_static")))
+                                                      "vmc_sweeps_static", "draw_tape")))
         for name, res, args in [
             ("oracle_stream_seed", C.c_uint64, [C.c_uint64] * 3),
             ("oracle_xoshiro_state_u64", C.c_int, [_P_U64, C.c_int, _P_U64]),
             ("oracle_xoshiro_draws", C.c_int, [C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_double)]),
             ("oracle_sweep_tape", C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int,
                                             C.POINTER(C.c_double)]),
+            ("oracle_device_tape", C.c_int, [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int,
+                                             C.POINTER(C.c_double)]),
         ]:
             f = getattr(lib, name)
             f.restype, f.argtypes = res, args
@@ -60,6 +62,21 @@ def oracle_lib():
 
 def oracle_walkers(spec: Spec, nw: int) -> Walkers:
     return Walkers(oracle_lib()[1], spec, nw)
+
+
+def pos_err(a, b):
+    """Largest difference of two position arrays over their largest
+    coordinate: a coordinate's own size depends on where the origin sits, so
+    positions are compared on the cell's scale."""
+    return rel_err(a, b, rel_floor=1.0)
+
+
+def oracle_device_tape(seed: int, stream: int, np_: int, nw: int, n_sweeps: int):
+    """The per-walker device streams' numbers, restated on the CPU."""
+    lib = oracle_lib()[0]
+    out = np.empty(n_sweeps * np_ * 3 * nw)
+    lib.oracle_device_tape(seed, stream, np_, nw, n_sweeps, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
 
 
 def oracle_tape(seed: int, np_: int, nw: int, n_sweeps: int):
@@ -215,7 +232,15 @@ class RefWalkers:
             pass
 
 
-def rel_err(a, b, floor=1e-300):
+def rel_err(a, b, floor=1e-300, rel_floor=1e-3):
+    """Largest elementwise relative difference of a from b: each component
+    is held relative to itself, with a floor of rel_floor times the array's
+    largest magnitude for components that are themselves the small result
+    of a cancellation (a gradient component near zero is the sum of terms of
+    the array's size, so its rounding is on that scale)."""
     a, b = np.asarray(a, float), np.asarray(b, float)
-    scale = np.maximum(np.max(np.abs(b)), floor)
-    return float(np.max(np.abs(a - b)) / scale)
+    if b.size == 0:
+        return 0.0
+    big = max(float(np.max(np.abs(b))), floor)
+    den = np.maximum(np.abs(b), rel_floor * big)
+    return float(np.max(np.abs(a - b) / den))

@@ -10,9 +10,9 @@ import numpy as np
 import pytest
 
 from alea_amd import dmc
-from alea_amd.system import exciton_spec, heg_spec
+from alea_amd.system import exciton_spec, heg_drummond_spec, heg_spec
 from alea_amd.walkers import gpu_walkers, seed
-from backends import oracle_tape, oracle_walkers, rel_err
+from backends import oracle_tape, oracle_walkers, pos_err, rel_err
 from test_gpu_parity import _pos, specs
 
 pytestmark = pytest.mark.gpu
@@ -41,7 +41,7 @@ def test_dmc_sweeps_parity(spec):
     # (b/r)^5 amplifies a 1e-12 position difference through three drifted
     # sweeps to about 1e-10 in log psi, so that system is held to 1e-9.
     tol = 1e-9 if spec.mcmillan else 1e-10
-    assert rel_err(g.positions(), o.positions()) < tol
+    assert pos_err(g.positions(), o.positions()) < tol
     assert rel_err(rg[0], ro[0]) < tol and rel_err(rg[1], ro[1]) < tol
     assert rel_err(g.log_psi(), o.log_psi()) < tol
     assert rel_err(g.local_energy()[0], o.local_energy()[0]) < tol
@@ -79,7 +79,39 @@ def test_dmc_steps_identical_populations(spec):
         assert abs(hg["etrial"] - ho["etrial"]) < 1e-10 * abs(ho["etrial"])
     assert len(set(h["nw"] for h in dg.history)) > 1          # the population did move
     assert np.array_equal(dg.branch_state, do.branch_state)
-    assert rel_err(g.positions(), o.positions()) < 1e-12
+    assert pos_err(g.positions(), o.positions()) < 1e-12
+
+
+def test_c1_dmc_identical_populations():
+    """The published DMC case's system (N = 74 paramagnetic HEG at rs = 40,
+    plane-wave nodes, image-summed Padé plus an order-8 Drummond polynomial,
+    Ewald) at tau = 2: target 32 in a capacity of 65, five steps of two
+    drift-diffusion sub-steps from one tape per sub-step and one branch
+    stream.  Device and oracle keep identical populations, source maps,
+    accept and cutoff counts at every step."""
+    spec = heg_drummond_spec(37, 37, 40.0)
+    cap, target, n_steps = 65, 32, 5
+    pos = _pos(spec, cap, 23)
+    tapes = lambda step, sub, n: oracle_tape(2000 + 10 * step + sub, spec.n_particles, cap, 1)
+    runs = []
+    for make in (gpu_walkers, oracle_walkers):
+        x = make(spec, cap)
+        x.set_positions(pos)
+        x.evaluate()
+        x.permute(np.arange(target))
+        d = dmc.DMC(x, 2.0, target, n_substeps=2, seed=19, tapes=tapes, stride=cap)
+        for i in range(n_steps):
+            d.step(measure=i >= 2)
+        runs.append((d, x))
+    (dg, g), (do, o) = runs
+    assert [h["nw"] for h in dg.history] == [h["nw"] for h in do.history]
+    for hg, ho in zip(dg.history, do.history):
+        assert list(hg["src"]) == list(ho["src"])
+        assert hg["n_accepted"] == ho["n_accepted"] and hg["n_cut"] == ho["n_cut"]
+        assert rel_err(hg["e_sum"], ho["e_sum"]) < 1e-10 and rel_err(hg["w_sum"], ho["w_sum"]) < 1e-10
+    assert len(set(h["nw"] for h in dg.history)) > 1
+    assert np.array_equal(dg.branch_state, do.branch_state)
+    assert pos_err(g.positions(), o.positions()) < 1e-10
 
 
 def test_dmc_exciton_population_exact_on_device():

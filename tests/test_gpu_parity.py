@@ -11,7 +11,7 @@ import pytest
 
 from alea_amd.system import dipole_spec, exciton_spec, heg_spec, hexagonal_lattice, uniform_positions
 from alea_amd.walkers import FP64, MIXED, gpu_walkers, seed
-from backends import oracle_tape, oracle_walkers, rel_err
+from backends import oracle_tape, oracle_walkers, pos_err, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -63,7 +63,10 @@ def test_evaluate_parity(spec):
     assert rel_err(g.log_psi(), o.log_psi()) < TOL
     gg, gl = g.gradients()
     og, ol = o.gradients()
-    assert rel_err(gg, og) < TOL and rel_err(gl, ol) < TOL
+    # a Laplacian component is a sum of second derivatives of the array's size
+    # that can cancel far below it: 1e-9 of itself (floor 1e-3 of the largest),
+    # as the per-call ratio's new Laplacian below
+    assert rel_err(gg, og) < TOL and rel_err(gl, ol) < 1e-9
     ge, gi = g.local_energy()
     oe, oi = o.local_energy()
     assert rel_err(ge, oe) < TOL
@@ -81,6 +84,10 @@ def test_ratio_accept_parity(spec):
     rng = np.random.default_rng(4)
     P = o.positions().reshape(spec.n_particles, 2, nw)
     scale = 0.5 if spec.lattice is None else spec.lattice[0] / 10
+    # log|psi| is a running sum of the accepted log-ratios, so its rounding is
+    # on the scale of the largest value it passed through (McMillan's
+    # (b/r)^5 puts some uniform starts near -1e10 before a move relaxes them)
+    history = np.abs(o.log_psi())
     for p in range(spec.n_particles):
         newp = (P[p] + rng.normal(scale=scale, size=(2, nw))).reshape(-1)
         a, b = g.ratio(p, newp), o.ratio(p, newp)
@@ -91,7 +98,8 @@ def test_ratio_accept_parity(spec):
         g.accept(p, mask)
         o.accept(p, mask)
         P = o.positions().reshape(spec.n_particles, 2, nw)
-    assert rel_err(g.log_psi(), o.log_psi()) < TOL
+        history = np.maximum(history, np.abs(o.log_psi()))
+    assert np.all(np.abs(g.log_psi() - o.log_psi()) <= TOL * np.maximum(history, 1.0))
     assert rel_err(g.local_energy()[0], o.local_energy()[0]) < TOL
 
 
@@ -109,7 +117,7 @@ def test_vmc_trajectory_parity(spec):
     ag = g.vmc_sweeps(n_sw, tau, tape)
     ao = o.vmc_sweeps(n_sw, tau, tape)
     assert ag == ao
-    assert rel_err(g.positions(), o.positions()) < 1e-12
+    assert pos_err(g.positions(), o.positions()) < 1e-12
     assert rel_err(g.log_psi(), o.log_psi()) < TOL
     assert rel_err(g.local_energy()[0], o.local_energy()[0]) < TOL
 
@@ -176,6 +184,32 @@ def test_mixed_trajectories_track_fp64_on_the_n122_workload():
     assert de_fresh.max() < 1e-7
 
 
+def test_n122_twisted_fp64_sweeps_match_the_oracle():
+    """The benchmark system in FP64 (twisted N = 122, rs = 30, two 61-orbital
+    complex blocks, image-summed Padé, Ewald): the streamed-inverse sweep
+    (k_sweep<double, true, false, 0>, four 16-row chunks per column) against
+    the oracle, which re-inverts the complex determinant on every accept as
+    the reference does.  16 walkers, two sweeps from one tape: identical
+    accept counts, then positions, log|psi| and E_L to 1e-10."""
+    s = heg_spec(61, 61, 30.0, twist=tuple(np.random.default_rng(3).uniform(-0.5, 0.5, 2)))
+    nw, n_sw = 16, 2
+    pos = _pos(s, nw, 41)
+    tape = oracle_tape(43, s.n_particles, nw, n_sw)
+    tau = (0.5 * 30.0) ** 2
+    g, o = gpu_walkers(s, nw), oracle_walkers(s, nw)
+    for x in (g, o):
+        x.set_positions(pos)
+        x.evaluate()
+    assert rel_err(g.log_psi(), o.log_psi()) < TOL
+    for k in range(n_sw):
+        one = tape[k * s.n_particles * 3 * nw:(k + 1) * s.n_particles * 3 * nw]
+        ag, ao = g.vmc_sweeps(1, tau, one), o.vmc_sweeps(1, tau, one)
+        assert ag == ao and ag > 0
+        assert pos_err(g.positions(), o.positions()) < TOL
+        assert rel_err(g.log_psi(), o.log_psi()) < TOL
+    assert rel_err(g.local_energy()[0], o.local_energy()[0]) < TOL
+
+
 def test_refresh_inverse_over_a_walker_range_matches_the_oracle():
     """rebuild_inverse on the walkers [w_begin, w_end) only
     (slater_determinant.cpp:905-993): after sweeps, the device and the oracle
@@ -238,14 +272,14 @@ def test_wigner_crystal_parity():
     tau = 15.0 ** 2
     ag, ao = g.vmc_sweeps(2, tau, tape), o.vmc_sweeps(2, tau, tape)
     assert ag == ao and 0 < ag < 2 * nw * spec.n_particles
-    assert rel_err(g.positions(), o.positions()) < TOL
+    assert pos_err(g.positions(), o.positions()) < TOL
     assert rel_err(g.log_psi(), o.log_psi()) < TOL
     assert rel_err(g.local_energy()[0], o.local_energy()[0]) < TOL
     tape = oracle_tape(6, spec.n_particles, nw, 2)
     rg, ro = [np.zeros(nw), np.zeros(nw)], [np.zeros(nw), np.zeros(nw)]
     ag, ao = g.dmc_sweeps(2, 2.0, tape, nw, *rg), o.dmc_sweeps(2, 2.0, tape, nw, *ro)
     assert ag == ao and ag > 0
-    assert rel_err(g.positions(), o.positions()) < TOL
+    assert pos_err(g.positions(), o.positions()) < TOL
     assert rel_err(rg[1], ro[1]) < TOL
     assert rel_err(g.log_psi(), o.log_psi()) < TOL
     assert rel_err(g.local_energy()[0], o.local_energy()[0]) < TOL
