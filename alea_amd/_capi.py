@@ -44,6 +44,8 @@ SIGNATURES = {
     "local_energy": (C.c_int, [_V, _P_D, _P_D]),
     "ratio": (C.c_int, [_V, C.c_int, _P_D, _P_D, _P_D, _P_D, _P_D]),
     "accept": (C.c_int, [_V, C.c_int, _P_U8]),
+    "ratio_range": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_D, _P_D, _P_D, _P_D, _P_D]),
+    "accept_range": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_U8]),
     "vmc_sweeps": (C.c_int, [_V, C.c_int, C.c_double, _P_D, _P_LL]),
     "dmc_sweeps": (C.c_int, [_V, C.c_int, C.c_double, _P_D, C.c_int, _P_D, _P_D, _P_LL]),
     "permute": (C.c_int, [_V, C.c_int, _P_I]),

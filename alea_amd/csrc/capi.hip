@@ -362,6 +362,9 @@ struct alea_system {
         rebuilds.alloc(nw_ * kRebuildCauses, stream);
         nacc.alloc(nw_, stream);
         src.alloc(nw_, stream);
+        // the per-call path's buffers (new_pos, outputs; mask), at the capacity
+        call_buf.alloc(12 * nw_, stream);
+        call_mask.alloc(nw_, stream);
         st.pos = pos.p;
         st.rho = rho.p;
         st.logs = logs.p;
@@ -454,11 +457,13 @@ struct alea_system {
         hip_check(hipGetLastError(), "launch refresh");
     }
 
-    void check_status() {
-        std::vector<int> s(nw);
-        hip_check(copy_on(stream, s.data(), status.p, nw * sizeof(int), hipMemcpyDeviceToHost), "status");
-        for (int w = 0; w < nw; ++w)
-            if (s[w]) throw std::runtime_error("Slater determinant: accepted matrix is singular or nonfinite");
+    void check_status(int w0 = 0, int w1 = -1) {
+        if (w1 < 0) w1 = nw;
+        if (w1 <= w0) return;
+        std::vector<int> s(static_cast<size_t>(w1 - w0));
+        hip_check(copy_on(stream, s.data(), status.p + w0, s.size() * sizeof(int), hipMemcpyDeviceToHost), "status");
+        for (int v : s)
+            if (v) throw std::runtime_error("Slater determinant: accepted matrix is singular or nonfinite");
     }
 
     // The model's nw is the walker stride of the per-call SoA buffers.
@@ -1108,64 +1113,108 @@ int alea_check_status(alea_system* s) {
     });
 }
 
+// The per-call path over the walkers [w0, w1): every array has the batch's
+// walker stride nw, and only the range's entries are read or written (the
+// reference's component contract, wavefunction_component.h:96-110).  The
+// kernels see the range as a batch of its own (shifted state, offset call
+// buffers); the call buffers are sized by the capacity at build, so range
+// calls allocate nothing, and calls on disjoint ranges touch disjoint
+// memory.
+static void ratio_range(alea_system* s, int p, int w0, int w1, const double* new_pos, double* log_ratio,
+                        double* phase, double* new_grad, double* new_lap) {
+    if (!s->evaluated) throw InvalidArg("evaluate first");
+    if (p < 0 || p >= s->np) throw InvalidArg("particle index out of range");
+    if (!new_pos || !log_ratio) throw InvalidArg("null proposal or output");
+    const size_t nw = s->nw, n = static_cast<size_t>(w1 - w0);
+    if (n == 0) return;
+    double* b = s->call_buf.p;
+    auto rows = [&](double* dev, const double* host, int n_rows, hipMemcpyKind kind, const char* what) {
+        for (int d = 0; d < n_rows; ++d) {
+            const size_t off = d * nw + w0;
+            if (kind == hipMemcpyHostToDevice)
+                hip_check(copy_on(s->stream, dev + off, host + off, n * sizeof(double), kind), what);
+            else
+                hip_check(copy_on(s->stream, const_cast<double*>(host) + off, dev + off, n * sizeof(double), kind), what);
+        }
+    };
+    rows(b, new_pos, 2, hipMemcpyHostToDevice, "new_pos");
+    DevCall c;
+    c.new_pos = b + w0;
+    c.log_ratio = b + 2 * nw + w0;
+    c.phase = b + 3 * nw + w0;
+    c.new_grad = b + 4 * nw + w0;
+    c.new_lap = b + 6 * nw + w0;
+    c.new_pgrad = b + 8 * nw + w0;
+    c.new_plap = b + 10 * nw + w0;
+    s->set_model_nw(s->nw);
+    launch_call(s->dmodel, s->shifted(w0), c, p, 1, w1 - w0, s->mixed(), s->cplx, s->walker_lds, s->stream);
+    hip_check(hipGetLastError(), "launch ratio");
+    rows(b + 2 * nw, log_ratio, 1, hipMemcpyDeviceToHost, "log_ratio");
+    if (phase) rows(b + 3 * nw, phase, 1, hipMemcpyDeviceToHost, "phase");
+    if (new_grad) rows(b + 4 * nw, new_grad, 2, hipMemcpyDeviceToHost, "new_grad");
+    if (new_lap) rows(b + 6 * nw, new_lap, 2, hipMemcpyDeviceToHost, "new_lap");
+}
+
+static void accept_range(alea_system* s, int p, int w0, int w1, const unsigned char* mask) {
+    if (!s->evaluated) throw InvalidArg("evaluate first");
+    if (p < 0 || p >= s->np) throw InvalidArg("particle index out of range");
+    if (!mask) throw InvalidArg("null mask");
+    const int n = w1 - w0;
+    if (n == 0) return;
+    hip_check(copy_on(s->stream, s->call_mask.p + w0, mask + w0, static_cast<size_t>(n), hipMemcpyHostToDevice), "mask");
+    DevCall c;
+    c.new_pos = s->call_buf.p + w0;
+    c.mask = s->call_mask.p + w0;
+    s->set_model_nw(s->nw);
+    const DevState st = s->shifted(w0);
+    launch_call(s->dmodel, st, c, p, 2, n, s->mixed(), s->cplx, s->walker_lds, s->stream);
+    hip_check(hipGetLastError(), "launch accept");
+    s->energies_valid = false;
+    // blocks of the range whose update count reached refresh_updates are
+    // rebuilt after each accept call (the reference rebuilds inside the call,
+    // slater_determinant.cpp:788-790); a forced full refresh (whole batch)
+    // counts one sweep's worth of calls as a sweep
+    if (s->refresh_every > 0 && w0 == 0 && w1 == s->nw &&
+        ++s->since_refresh >= static_cast<long long>(s->refresh_every) * s->np) {
+        s->refresh();
+    } else if (s->host.n_blocks > 0) {
+        launch_refresh(s->dmodel, st, n, s->host.n_blocks, s->st.nb_max, s->mixed(), s->cplx, s->refresh_updates,
+                       s->stream);
+        hip_check(hipGetLastError(), "launch refresh");
+    }
+    s->sync();
+    s->check_status(w0, w1);
+}
+
 int alea_ratio(alea_system* s, int p, const double* new_pos, double* log_ratio, double* phase,
                double* new_grad, double* new_lap) {
     return guarded([&] {
         Sys(s);
-        if (!s->evaluated) throw InvalidArg("evaluate first");
-        if (p < 0 || p >= s->np) throw InvalidArg("particle index out of range");
-        const size_t nw = s->nw;
-        if (s->call_buf.n < 12 * nw) s->call_buf.alloc(12 * nw, s->stream);
-        double* b = s->call_buf.p;
-        hip_check(copy_on(s->stream, b, new_pos, 2 * nw * sizeof(double), hipMemcpyHostToDevice), "new_pos");
-        DevCall c;
-        c.new_pos = b;
-        c.log_ratio = b + 2 * nw;
-        c.phase = b + 3 * nw;
-        c.new_grad = b + 4 * nw;
-        c.new_lap = b + 6 * nw;
-        c.new_pgrad = b + 8 * nw;
-        c.new_plap = b + 10 * nw;
-        s->set_model_nw(s->nw);
-        launch_call(s->dmodel, s->st, c, p, 1, s->nw, s->mixed(), s->cplx, s->walker_lds, s->stream);
-        hip_check(hipGetLastError(), "launch ratio");
-        s->sync();
-        hip_check(copy_on(s->stream, log_ratio, c.log_ratio, nw * sizeof(double), hipMemcpyDeviceToHost), "lr");
-        if (phase) hip_check(copy_on(s->stream, phase, c.phase, nw * sizeof(double), hipMemcpyDeviceToHost), "ph");
-        if (new_grad) hip_check(copy_on(s->stream, new_grad, c.new_grad, 2 * nw * sizeof(double), hipMemcpyDeviceToHost), "ng");
-        if (new_lap) hip_check(copy_on(s->stream, new_lap, c.new_lap, 2 * nw * sizeof(double), hipMemcpyDeviceToHost), "nl");
+        ratio_range(s, p, 0, s->nw, new_pos, log_ratio, phase, new_grad, new_lap);
     });
 }
 
 int alea_accept(alea_system* s, int p, const unsigned char* mask) {
     return guarded([&] {
         Sys(s);
-        if (!s->evaluated) throw InvalidArg("evaluate first");
-        if (p < 0 || p >= s->np) throw InvalidArg("particle index out of range");
-        const size_t nw = s->nw;
-        if (s->call_buf.n < 12 * nw) throw InvalidArg("accept needs a preceding ratio call");
-        if (s->call_mask.n < nw) s->call_mask.alloc(nw, s->stream);
-        hip_check(copy_on(s->stream, s->call_mask.p, mask, nw, hipMemcpyHostToDevice), "mask");
-        DevCall c;
-        c.new_pos = s->call_buf.p;
-        c.mask = s->call_mask.p;
-        s->set_model_nw(s->nw);
-        launch_call(s->dmodel, s->st, c, p, 2, s->nw, s->mixed(), s->cplx, s->walker_lds, s->stream);
-        hip_check(hipGetLastError(), "launch accept");
-        s->energies_valid = false;
-        // blocks whose update count reached refresh_updates are rebuilt after
-        // each accept call (the reference rebuilds inside the call,
-        // slater_determinant.cpp:788-790); a forced full refresh counts one
-        // sweep's worth of calls as a sweep
-        if (s->refresh_every > 0 && ++s->since_refresh >= static_cast<long long>(s->refresh_every) * s->np) {
-            s->refresh();
-        } else if (s->host.n_blocks > 0) {
-            launch_refresh(s->dmodel, s->st, s->nw, s->host.n_blocks, s->st.nb_max, s->mixed(), s->cplx,
-                           s->refresh_updates, s->stream);
-            hip_check(hipGetLastError(), "launch refresh");
-        }
-        s->sync();
-        s->check_status();
+        accept_range(s, p, 0, s->nw, mask);
+    });
+}
+
+int alea_ratio_range(alea_system* s, int p, int w_begin, int w_end, const double* new_pos, double* log_ratio,
+                     double* phase, double* new_grad, double* new_lap) {
+    return guarded([&] {
+        Sys(s);
+        const auto [w0, w1] = s->range(w_begin, w_end);
+        ratio_range(s, p, w0, w1, new_pos, log_ratio, phase, new_grad, new_lap);
+    });
+}
+
+int alea_accept_range(alea_system* s, int p, int w_begin, int w_end, const unsigned char* mask) {
+    return guarded([&] {
+        Sys(s);
+        const auto [w0, w1] = s->range(w_begin, w_end);
+        accept_range(s, p, w0, w1, mask);
     });
 }
 

@@ -141,6 +141,21 @@ class Walkers:
         m = u8(mask)
         self.api("accept", self.h, p, ptr(m, _P_U8))
 
+    # ── per-call path over a walker range (crowds) ─────────────────────────
+    def ratio_range(self, p, w_begin, w_end, new_pos, out=None):
+        """ratio_batch(ws, p, w_begin, w_end): the range's entries of out
+        (log_ratio, phase, new_grad, new_lap; batch stride) are written."""
+        npos = f64(new_pos)
+        if out is None:
+            out = (np.zeros(self.nw), np.zeros(self.nw), np.zeros(2 * self.nw), np.zeros(2 * self.nw))
+        lr, ph, ng, nl = out
+        self.api("ratio_range", self.h, p, w_begin, w_end, ptr(npos), ptr(lr), ptr(ph), ptr(ng), ptr(nl))
+        return out
+
+    def accept_range(self, p, w_begin, w_end, mask):
+        m = u8(mask)
+        self.api("accept_range", self.h, p, w_begin, w_end, ptr(m, _P_U8))
+
     # ── drivers ────────────────────────────────────────────────────────────
     def vmc_sweeps(self, n_sweeps, tau, tape=None):
         acc = C.c_longlong()

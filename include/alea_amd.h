@@ -134,6 +134,16 @@ int alea_local_energy(alea_system* s, double* e_re, double* e_im);
 int alea_ratio(alea_system* s, int p, const double* new_pos, double* log_ratio, double* phase,
                double* new_grad, double* new_lap);
 int alea_accept(alea_system* s, int p, const unsigned char* mask);
+/* The same two calls over the walkers [w_begin, w_end) (w_end < 0: to nw,
+ * the reference's npos): TrialWavefunction::ratio_batch(ws, p, w_begin,
+ * w_end) and apply_move_batch(ws, p, accept, w_begin, w_end)
+ * (trial_wavefunction.cpp:46-138, wavefunction_component.h:96-110).  Every
+ * array keeps the batch's walker stride nw; only the range's entries are
+ * read or written.  Nothing is allocated, and calls on disjoint ranges touch
+ * disjoint memory, so DMC crowds can share one system. */
+int alea_ratio_range(alea_system* s, int p, int w_begin, int w_end, const double* new_pos,
+                     double* log_ratio, double* phase, double* new_grad, double* new_lap);
+int alea_accept_range(alea_system* s, int p, int w_begin, int w_end, const unsigned char* mask);
 
 /* Fused sweeps: VMC::sweep / sweep_walkers (vmc.cpp:69-155) with symmetric
  * Metropolis proposals of width sqrt(tau/m).  tape: [sweep][np][dim+1][nw]

@@ -308,6 +308,29 @@ public:
         if (m.size() != ws.num_walkers()) throw std::invalid_argument("apply_move_batch: mask size");
         check(alea_accept(h_, static_cast<int>(p), m.data()));
     }
+
+    // The same over the walkers [w_begin, w_end) with the batch's walker
+    // stride (wavefunction_component.h:96-110; npos: to the end): only the
+    // range's entries of the log-ratio, phase and new-derivative planes are
+    // written, so disjoint ranges (DMC crowds) fill one batch between them.
+    static constexpr std::size_t npos = static_cast<std::size_t>(-1);
+    void ratio_batch(WalkerSet& ws, std::size_t p, std::size_t w_begin, std::size_t w_end = npos) {
+        const std::size_t nw = ws.num_walkers(), d = ws.dim();
+        if (p != ws.proposed_) throw std::invalid_argument("ratio_batch: no move of this particle was proposed");
+        if (ws.log_ratios_.size() != nw) ws.log_ratios_.assign(nw, 0.0);
+        if (ws.phase_diffs_.size() != nw) ws.phase_diffs_.assign(nw, 0.0);
+        if (ws.new_grad_.size() != d * nw) ws.new_grad_.assign(d * nw, 0.0);
+        if (ws.new_lap_.size() != d * nw) ws.new_lap_.assign(d * nw, 0.0);
+        check(alea_ratio_range(h_, static_cast<int>(p), range_int(w_begin), range_int(w_end), ws.new_pos_.data(),
+                               ws.log_ratios_.data(), ws.phase_diffs_.data(), ws.new_grad_.data(),
+                               ws.new_lap_.data()));
+    }
+    void apply_move_batch(WalkerSet& ws, std::size_t p, std::span<const bool> mask, std::size_t w_begin,
+                          std::size_t w_end = npos) {
+        std::vector<unsigned char> m(mask.begin(), mask.end());
+        if (m.size() != ws.num_walkers()) throw std::invalid_argument("apply_move_batch: mask size");
+        check(alea_accept_range(h_, static_cast<int>(p), range_int(w_begin), range_int(w_end), m.data()));
+    }
     // refresh_derivatives: the gradient and Laplacian planes of log|psi|
     void refresh_derivatives(WalkerSet& ws, std::vector<double>& grad, std::vector<double>& lap) {
         const std::size_t n = ws.n_particles() * ws.dim() * ws.num_walkers();
@@ -317,6 +340,7 @@ public:
     }
 
 private:
+    static int range_int(std::size_t w) { return w == npos ? -1 : static_cast<int>(w); }
     alea_system* h_;
 };
 

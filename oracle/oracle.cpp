@@ -2197,6 +2197,49 @@ int oracle_accept(oracle_system* h, int p, const unsigned char* mask) {
     });
 }
 
+// ratio_batch / apply_move_batch over the walkers [w_begin, w_end)
+// (trial_wavefunction.cpp:46-138): arrays keep the batch's walker stride and
+// only the range's entries are read or written.  The restatement recomputes
+// the whole batch with the other walkers' pending proposals left as they
+// were, so calls for interleaved ranges (crowds) see their own proposals.
+static std::pair<std::size_t, std::size_t> oracle_range(const Sys& s, int w_begin, int w_end) {
+    const long long nw = static_cast<long long>(s.nw);
+    const long long w1 = w_end < 0 ? nw : w_end;
+    if (w_begin < 0 || w_begin > w1 || w1 > nw) throw std::invalid_argument("walker range outside the batch");
+    return {static_cast<std::size_t>(w_begin), static_cast<std::size_t>(w1)};
+}
+
+int oracle_ratio_range(oracle_system* h, int p, int w_begin, int w_end, const double* new_pos, double* log_ratio,
+                       double* phase, double* new_grad, double* new_lap) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        if (p < 0 || static_cast<std::size_t>(p) >= s.np) throw std::invalid_argument("oracle: particle out of range");
+        const auto [w0, w1] = oracle_range(s, w_begin, w_end);
+        const std::size_t nw = s.nw;
+        for (std::size_t d = 0; d < 2; ++d)
+            for (std::size_t w = w0; w < w1; ++w) s.new_pos[d * nw + w] = new_pos[d * nw + w];
+        ratio_all(s, static_cast<std::size_t>(p));
+        for (std::size_t w = w0; w < w1; ++w) {
+            log_ratio[w] = s.log_ratios[w];
+            if (phase) phase[w] = s.phase_diffs[w];
+            for (std::size_t d = 0; d < 2; ++d) {
+                if (new_grad) new_grad[d * nw + w] = s.new_grad[d * nw + w];
+                if (new_lap) new_lap[d * nw + w] = s.new_lap[d * nw + w];
+            }
+        }
+    });
+}
+
+int oracle_accept_range(oracle_system* h, int p, int w_begin, int w_end, const unsigned char* mask) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        const auto [w0, w1] = oracle_range(s, w_begin, w_end);
+        std::vector<double> m(s.nw, 0.0);
+        for (std::size_t w = w0; w < w1; ++w) m[w] = mask[w] ? 1.0 : 0.0;
+        accept_all(s, static_cast<std::size_t>(p), m);
+    });
+}
+
 // VMC::sweep_walkers with symmetric proposals (vmc.cpp:82-155).
 int oracle_vmc_sweeps(oracle_system* h, int n_sweeps, double tau, const double* tape,
                       long long* n_accepted) {

@@ -85,6 +85,9 @@ int oracle_dmc_sweeps(oracle_system* s, int n_sweeps, double tau, const double* 
 int oracle_permute(oracle_system* s, int new_nw, const int* src);
 int oracle_get_num_walkers(oracle_system* s, int* nw);
 int oracle_device_tape(uint64_t seed, uint64_t stream, int np, int nw, int n_sweeps, double* out);
+int oracle_ratio_range(oracle_system* s, int p, int w_begin, int w_end, const double* new_pos, double* log_ratio,
+                       double* phase, double* new_grad, double* new_lap);
+int oracle_accept_range(oracle_system* s, int p, int w_begin, int w_end, const unsigned char* mask);
 int oracle_refresh_inverse(oracle_system* s, int w_begin, int w_end);
 int oracle_check_status(oracle_system* s);
 int oracle_dmc_begin(oracle_system* s);

@@ -8,7 +8,7 @@
 //   hpp_driver run <spec> <input> <output>
 //       builds a walker batch from a text spec through WalkerSet /
 //       TrialWavefunction / Hamiltonian, then: evaluate_batch; ratio_batch +
-//       apply_move_batch over every particle; VMC sweeps from a tape; DMC
+//       apply_move_batch over every particle (odd ones as two crowds); VMC sweeps from a tape; DMC
 //       run_one_step with tapes.  Writes every result as "name count values".
 //   hpp_driver estimators <spec> <input> <output>
 //       the Estimators class (g(r), S(k), density) over given positions and weights;
@@ -261,10 +261,19 @@ int run(const char* spec_path, const char* in_path, const char* out_path) {
     out.put("lap0", lap);
     out.put("e0", ham.evaluate_batch(ws, &e_im));
     out.put("e0_im", e_im);
-    // per-particle ratio_batch + apply_move_batch
+    // per-particle ratio_batch + apply_move_batch; odd particles go through
+    // two crowds, [0, nw/2) and [nw/2, nw), which must fill the batch exactly
+    // as one whole-batch call does
     for (std::size_t p = 0; p < np; ++p) {
         ws.propose_move_batch(p, in.vec());
-        psi.ratio_batch(ws, p);
+        const std::size_t half = ws.num_walkers() / 2;
+        const bool crowds = p % 2 == 1;
+        if (crowds) {
+            psi.ratio_batch(ws, p, 0, half);
+            psi.ratio_batch(ws, p, half);
+        } else {
+            psi.ratio_batch(ws, p);
+        }
         const std::string tag = std::to_string(p);
         out.put("lr" + tag, ws.log_ratios());
         out.put("ph" + tag, ws.phase_diffs());
@@ -273,7 +282,13 @@ int run(const char* spec_path, const char* in_path, const char* out_path) {
         const std::vector<double> m = in.vec();
         std::unique_ptr<bool[]> mask(new bool[m.size()]);
         for (std::size_t w = 0; w < m.size(); ++w) mask[w] = m[w] != 0.0;
-        psi.apply_move_batch(ws, p, std::span<const bool>(mask.get(), m.size()));
+        const std::span<const bool> ms(mask.get(), m.size());
+        if (crowds) {
+            psi.apply_move_batch(ws, p, ms, half, TrialWavefunction::npos);
+            psi.apply_move_batch(ws, p, ms, 0, half);
+        } else {
+            psi.apply_move_batch(ws, p, ms);
+        }
     }
     out.put("log_psi1", ws.log_psi());
     out.put("e1", ham.evaluate_batch(ws));

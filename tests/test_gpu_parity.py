@@ -334,3 +334,39 @@ def test_a_singular_commit_is_reported():
         g.check_status()
     with pytest.raises(RuntimeError, match="singular"):
         g.local_energy()
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED])
+def test_crowds_equal_the_whole_batch(prec):
+    """ratio_batch / apply_move_batch over disjoint walker ranges on one
+    shared system (the DMC crowd contract, wavefunction_component.h:96-110,
+    the reference's tests/test_pade_jastrow.cpp:254): two crowds, interleaved
+    (ratio of A, ratio of B, accept of B, accept of A), give bit for bit what
+    the whole-batch calls give, through every particle of the twisted
+    Slater-Jastrow-Ewald system."""
+    s = heg_spec(5, 5, 2.0, twist=(0.13, 0.27))
+    nw, half = 40, 17
+    pos = _pos(s, nw, 11)
+    whole, crowd = gpu_walkers(s, nw, prec), gpu_walkers(s, nw, prec)
+    for x in (whole, crowd):
+        x.set_positions(pos)
+        x.evaluate()
+    rng = np.random.default_rng(12)
+    P = whole.positions().reshape(s.n_particles, 2, nw)
+    for p in range(s.n_particles):
+        newp = (P[p] + rng.normal(scale=0.5, size=(2, nw))).reshape(-1)
+        mask = rng.random(nw) < 0.6
+        ref = whole.ratio(p, newp)
+        out = crowd.ratio_range(p, 0, half, newp)
+        crowd.ratio_range(p, half, -1, newp, out)
+        for a, b in zip(out, ref):
+            assert np.array_equal(a, b)
+        whole.accept(p, mask)
+        crowd.accept_range(p, half, -1, mask)
+        crowd.accept_range(p, 0, half, mask)
+        P = whole.positions().reshape(s.n_particles, 2, nw)
+    assert np.array_equal(crowd.positions(), whole.positions())
+    assert np.array_equal(crowd.log_psi(), whole.log_psi())
+    assert np.array_equal(crowd.local_energy()[0], whole.local_energy()[0])
+    with pytest.raises(RuntimeError, match="range"):
+        crowd.ratio_range(0, 5, nw + 1, np.zeros(2 * nw))
