@@ -47,6 +47,7 @@ SIGNATURES = {
     "ratio_range": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_D, _P_D, _P_D, _P_D, _P_D]),
     "accept_range": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_U8]),
     "vmc_sweeps": (C.c_int, [_V, C.c_int, C.c_double, _P_D, _P_LL]),
+    "vmc_drift_sweeps": (C.c_int, [_V, C.c_int, C.c_double, _P_D, _P_LL]),
     "dmc_sweeps": (C.c_int, [_V, C.c_int, C.c_double, _P_D, C.c_int, _P_D, _P_D, _P_LL]),
     "permute": (C.c_int, [_V, C.c_int, _P_I]),
     "get_num_walkers": (C.c_int, [_V, _P_I]),

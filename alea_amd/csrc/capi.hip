@@ -1302,13 +1302,13 @@ static void run_sweep_graph(alea_system* s, const SweepParams& P, int stride, in
 }
 
 static void run_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape, int stride,
-                       bool dmc, long long* n_accepted) {
+                       SweepKind kind, long long* n_accepted) {
     if (!s->evaluated) throw InvalidArg("evaluate first");
     if (n_sweeps < 0) throw InvalidArg("negative sweep count");
     SweepParams P;
     P.n_sweeps = n_sweeps;
     P.tau = tau;
-    P.dmc = dmc ? 1 : 0;
+    P.dmc = kind;
     if (tape) {
         const size_t len = static_cast<size_t>(n_sweeps) * s->np * 3 * stride;
         if (s->tape.n < len) s->tape.alloc(len, s->stream);
@@ -1401,7 +1401,7 @@ int alea_vmc_sweeps_static(alea_system* s, int n_sweeps, double tau) {
         SweepParams P;
         P.n_sweeps = 1;
         P.tau = tau;
-        P.dmc = 0;
+        P.dmc = kSweepVmc;
         P.tape = tape;
         for (int k = 0; k < n_sweeps; ++k) {
             for (int w0 = 0; w0 < s->nw; w0 += chunk) {
@@ -1424,7 +1424,15 @@ int alea_vmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape
                     long long* n_accepted) {
     return guarded([&] {
         Sys(s);
-        run_sweeps(s, n_sweeps, tau, tape, s->nw, false, n_accepted);
+        run_sweeps(s, n_sweeps, tau, tape, s->nw, kSweepVmc, n_accepted);
+    });
+}
+
+int alea_vmc_drift_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape,
+                          long long* n_accepted) {
+    return guarded([&] {
+        Sys(s);
+        run_sweeps(s, n_sweeps, tau, tape, s->nw, kSweepLangevin, n_accepted);
     });
 }
 
@@ -1443,7 +1451,7 @@ int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape
             }
             hip_check(copy_on(s->stream, s->rr.p, rr.data(), rr.size() * sizeof(double), hipMemcpyHostToDevice), "rr");
         }
-        run_sweeps(s, n_sweeps, tau, tape, tape ? tape_stride : s->nw, true, n_accepted);
+        run_sweeps(s, n_sweeps, tau, tape, tape ? tape_stride : s->nw, kSweepDmc, n_accepted);
         if (rr_proposed) {
             s->sync();
             hip_check(copy_on(s->stream, rr.data(), s->rr.p, rr.size() * sizeof(double), hipMemcpyDeviceToHost), "rr");
@@ -1544,7 +1552,7 @@ int alea_vmc_sweeps_fused(alea_system* const* systems, int n_systems, int n_swee
             J.p.tau = taus[i];
             J.p.tape = s->dtape.p;
             J.p.tape_stride = s->nw;
-            J.p.dmc = 0;
+            J.p.dmc = kSweepVmc;
             J.first = i == 0 ? 0 : args.job[i - 1].first + systems[i - 1]->nw;
             J.nw = s->nw;
             s->energies_valid = false;

@@ -1269,9 +1269,10 @@ struct Walker {
     // ── one Metropolis / drift-diffusion move of particle p ────────────────
     // mode 0: sweep (decide from the random numbers); 1: ratio only (per-call
     // path, writes DevCall outputs); 2: commit where call.mask is set.
+    // kind (mode 0): kSweepVmc, kSweepDmc or kSweepLangevin.
     template <bool derivs>
     __device__ bool move(int p, double nx0, double nx1, double uni, int mode, const DevCall* call,
-                         double tau, bool dmc, double drift_scale, const double* gold,
+                         double tau, int kind, double drift_scale, const double* gold,
                          double& rr_acc) {
         const double x0 = sh.pos[2 * p], x1 = sh.pos[2 * p + 1];
         const int b = M.block_of[p];
@@ -1351,10 +1352,22 @@ struct Walker {
         bool acc;
         if (mode == 2) {
             acc = call->mask[w] != 0;
-        } else if (!dmc) {
+        } else if (kind == kSweepVmc) {
             // uni < |psi'/psi|^2 = exp(2 lr), with the determinant's share as
             // |R|^2 rather than through its log
             acc = uni < exp(2.0 * J.dlog) * R2;
+        } else if (kind == kSweepLangevin) {
+            // VMC with the bare drift (tau/m) g: no node gate, the Green's
+            // function log-ratio per dimension -delta (g + g') + (tau/m)/2
+            // (g^2 - g'^2) (numerics/green_function.h:20-26)
+            const double tm = tau * M.inv_mass[p];
+            double lc = 0.0;
+            for (int d = 0; d < 2; ++d) {
+                const double delta = (d == 0 ? nx0 - x0 : nx1 - x1);
+                lc += -delta * (gold[d] + ng[d]) + 0.5 * tm * (gold[d] * gold[d] - ng[d] * ng[d]);
+            }
+            const double la = 2.0 * lr + lc;
+            acc = la >= 0.0 || uni < exp(la);
         } else {
             const double tm = tau * M.inv_mass[p];
             if (!CPLX && M.n_blocks > 0 && cos(phase) < 2.220446049250313e-16) {
@@ -1765,8 +1778,9 @@ __device__ __attribute__((always_inline)) inline void sweep_walker(const DevMode
             double gold[2] = {0.0, 0.0}, scale = 0.0;
             if constexpr (DMC) {
                 wk.grad_of(p, gold);
+                // DMC: the Umrigar-scaled drift; Langevin VMC: the bare tau/m
                 const double v2 = gold[0] * gold[0] + gold[1] * gold[1];
-                scale = v2 < 1e-10 ? tm : (-1.0 + sqrt(1.0 + 2.0 * tm * v2)) / v2;
+                scale = (P.dmc == kSweepLangevin || v2 < 1e-10) ? tm : (-1.0 + sqrt(1.0 + 2.0 * tm * v2)) / v2;
             }
             const double nx0 = x0 + scale * gold[0] + sigma * xi0;
             const double nx1 = x1 + scale * gold[1] + sigma * xi1;
@@ -1775,7 +1789,7 @@ __device__ __attribute__((always_inline)) inline void sweep_walker(const DevMode
                 rr_p += tm * d0 * d0 + tm * d1 * d1;
             }
             const bool accepted =
-                wk.template move<DMC>(p, nx0, nx1, uni, 0, nullptr, P.tau, DMC, scale, gold, rr_a);
+                wk.template move<DMC>(p, nx0, nx1, uni, 0, nullptr, P.tau, DMC ? P.dmc : kSweepVmc, scale, gold, rr_a);
             if (accepted) ++nacc;
         }
     }
@@ -1783,7 +1797,7 @@ __device__ __attribute__((always_inline)) inline void sweep_walker(const DevMode
     wk.store();
     if (lane == 0) {
         S.n_accepted[w] += nacc;
-        if constexpr (DMC) {
+        if (DMC && P.dmc == kSweepDmc) {
             S.rr[w * 2] += rr_p;
             S.rr[w * 2 + 1] += rr_a;
         }
@@ -1827,8 +1841,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             double gold[2] = {0.0, 0.0}, scale = 0.0;
             if constexpr (DMC) {
                 wk.grad_of(p, gold);
+                // DMC: the Umrigar-scaled drift; Langevin VMC: the bare tau/m
                 const double v2 = gold[0] * gold[0] + gold[1] * gold[1];
-                scale = v2 < 1e-10 ? tm : (-1.0 + sqrt(1.0 + 2.0 * tm * v2)) / v2;
+                scale = (P.dmc == kSweepLangevin || v2 < 1e-10) ? tm : (-1.0 + sqrt(1.0 + 2.0 * tm * v2)) / v2;
             }
             const double nx0 = x0 + scale * gold[0] + sigma * xi0;
             const double nx1 = x1 + scale * gold[1] + sigma * xi1;
@@ -1837,7 +1852,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
                 rr_p += tm * d0 * d0 + tm * d1 * d1;
             }
             const bool accepted =
-                wk.template move<DMC>(p, nx0, nx1, uni, 0, nullptr, P.tau, DMC, scale, gold, rr_a);
+                wk.template move<DMC>(p, nx0, nx1, uni, 0, nullptr, P.tau, DMC ? P.dmc : kSweepVmc, scale, gold, rr_a);
             if (accepted) ++nacc;
         }
     }
@@ -1845,7 +1860,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     wk.store();
     if (lane == 0) {
         S.n_accepted[w] += nacc;
-        if constexpr (DMC) {
+        if (DMC && P.dmc == kSweepDmc) {
             S.rr[w * 2] += rr_p;
             S.rr[w * 2 + 1] += rr_a;
         }
@@ -1901,7 +1916,7 @@ __global__ __launch_bounds__(64) void k_call(const DevModel* __restrict__ Mp, De
     Walker<T, CPLX> wk(M, S, sh, w);
     wk.load();
     double rr = 0.0;
-    wk.template move<true>(p, call.new_pos[w], call.new_pos[M.nw + w], 0.0, mode, &call, 0.0, false, 0.0,
+    wk.template move<true>(p, call.new_pos[w], call.new_pos[M.nw + w], 0.0, mode, &call, 0.0, kSweepVmc, 0.0,
                            nullptr, rr);
     if (mode == 2) wk.store();
 }

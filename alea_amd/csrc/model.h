@@ -173,12 +173,18 @@ struct DevCall {
     double* new_plap = nullptr;
 };
 
+// The move a sweep makes: Metropolis VMC (vmc.cpp:109-135 without drift),
+// drift-diffusion DMC with the Umrigar-scaled drift and node gate
+// (dmc.cpp:203-337), or VMC with the bare Langevin drift and its
+// Metropolis-Hastings correction (vmc.cpp:98-129 with use_drift).
+enum SweepKind : int { kSweepVmc = 0, kSweepDmc = 1, kSweepLangevin = 2 };
+
 struct SweepParams {
     int n_sweeps = 1;
     double tau = 0.0;
     const double* tape = nullptr;   // [sweep][np][3][stride] or null (device RNG)
     int tape_stride = 0;
-    int dmc = 0;
+    int dmc = kSweepVmc;            // a SweepKind
 };
 
 // One system of a fused VMC sweep launch: its model, state and sweep

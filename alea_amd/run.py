@@ -62,9 +62,12 @@ def _check_stage(st: dict):
     for key in ("samples", "accumulation_blocks", "steps_per_block", "sub_steps", "time_step"):
         if not st.get(key):
             raise ValueError(f"run: a {method} stage needs `{key}` > 0")
+    # parser.cpp:1184-1197: VMC walks by 'metropolis' or 'langevin' (the
+    # drift proposal, app.cpp:162); DMC by 'drift_diffusion' or its synonym
+    # 'langevin'
     walk = st.get("random_walk") or ("metropolis" if method == "vmc" else "drift_diffusion")
-    if walk != ("metropolis" if method == "vmc" else "drift_diffusion"):
-        _refuse(f"random_walk '{walk}' in {method.upper()}")
+    if walk not in (("metropolis", "langevin") if method == "vmc" else ("drift_diffusion", "langevin")):
+        raise ValueError(f"run: {method}: random_walk '{walk}' is not one the reference accepts")
     if method == "dmc" and st.get("reweighting", "bare") != "bare":
         _refuse(f"reweighting '{st.get('reweighting')}'")
 
@@ -108,10 +111,11 @@ def run_vmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, n
         seed_walkers(w, stage_seed)
         est = _estimators(w, st, group_names)
         tau = float(st["time_step"])
+        drift = st.get("random_walk") == "langevin"
         n_sub, n_steps = int(st["sub_steps"]), int(st["steps_per_block"])
         for _ in range(int(st.get("equilibration_blocks", 0))):
             for _ in range(n_steps):
-                w.vmc_sweeps(n_sub, tau)
+                w.vmc_sweeps(n_sub, tau, drift=drift)
         blocks, blocks_im = [], []
         e_sum = e2_sum = 0.0
         n_tot = acc = prop = 0
@@ -121,7 +125,7 @@ def run_vmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, n
             if est:
                 est.start_block()
             for _ in range(n_steps):
-                acc += w.vmc_sweeps(n_sub, tau)
+                acc += w.vmc_sweeps(n_sub, tau, drift=drift)
                 prop += n_sub * n * nw
                 if est:
                     est.accumulate()

@@ -157,10 +157,12 @@ class Walkers:
         self.api("accept_range", self.h, p, w_begin, w_end, ptr(m, _P_U8))
 
     # ── drivers ────────────────────────────────────────────────────────────
-    def vmc_sweeps(self, n_sweeps, tau, tape=None):
+    def vmc_sweeps(self, n_sweeps, tau, tape=None, drift=False):
+        """VMC::sweep_walkers (vmc.cpp:82-155); drift=True is its use_drift
+        form, the Langevin proposal with the Metropolis-Hastings correction."""
         acc = C.c_longlong()
         t = None if tape is None else f64(tape)
-        self.api("vmc_sweeps", self.h, n_sweeps, tau, ptr(t), C.byref(acc))
+        self.api("vmc_drift_sweeps" if drift else "vmc_sweeps", self.h, n_sweeps, tau, ptr(t), C.byref(acc))
         return acc.value
 
     def dmc_sweeps(self, n_sweeps, tau, tape=None, stride=0, rr_p=None, rr_a=None):

@@ -152,6 +152,13 @@ int alea_accept_range(alea_system* s, int p, int w_begin, int w_end, const unsig
  * NULL the call returns once the sweeps are queued (see alea_query). */
 int alea_vmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape,
                     long long* n_accepted);
+/* VMC::sweep_walkers with VMCConfig::use_drift (vmc.cpp:98-139): each
+ * particle's gradient of log|psi| refreshed, the proposal
+ * r + (tau/m) g + sqrt(tau/m) xi, acceptance min(1, |psi'/psi|^2 T(r|r')/T(r'|r))
+ * with the Langevin Green's function (numerics/green_function.h:20-26).  No
+ * node gate and no rr accounting.  Arguments as alea_vmc_sweeps. */
+int alea_vmc_drift_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape,
+                          long long* n_accepted);
 /* alea_vmc_sweeps on the device streams (tape NULL, no acceptance read-back)
  * that allocates nothing: the random numbers of each sweep are staged in a
  * fixed 32 MiB device array of the library (one per GPU, held until the

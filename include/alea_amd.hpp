@@ -373,23 +373,29 @@ private:
 
 // ── VMC sweeps ─────────────────────────────────────────────────────────────
 // VMC::sweep_walkers (vmc.cpp:69-155): n fused sweeps, symmetric proposals of
-// width sqrt(tau/m); tape in draw_sweep_randoms order or the device streams
-// seeded here with stream_seed(seed, "vmc-thrd", walker).
+// width sqrt(tau/m), or with use_drift (VMCConfig::use_drift, random_walk:
+// langevin) the Langevin proposal and its Metropolis-Hastings correction;
+// tape in draw_sweep_randoms order or the device streams seeded here with
+// stream_seed(seed, "vmc-thrd", walker).
 class VMC {
 public:
-    VMC(WalkerSet& ws, double tau) : ws_(ws), tau_(tau) {}
+    VMC(WalkerSet& ws, double tau, bool use_drift = false) : ws_(ws), tau_(tau), use_drift_(use_drift) {}
     void seed(std::uint64_t seed, std::uint64_t stream = 0x766d632d74687264ULL) {
         check(alea_seed(ws_.handle(), seed, stream));
     }
     long long sweep(int n_sweeps, const double* tape = nullptr) {
         long long acc = 0;
-        check(alea_vmc_sweeps(ws_.handle(), n_sweeps, tau_, tape, &acc));
+        if (use_drift_)
+            check(alea_vmc_drift_sweeps(ws_.handle(), n_sweeps, tau_, tape, &acc));
+        else
+            check(alea_vmc_sweeps(ws_.handle(), n_sweeps, tau_, tape, &acc));
         return acc;
     }
 
 private:
     WalkerSet& ws_;
     double tau_;
+    bool use_drift_;
 };
 
 // ── PopulationControl (population_control.cpp:14-108) ────────────────────

@@ -2269,6 +2269,50 @@ int oracle_vmc_sweeps(oracle_system* h, int n_sweeps, double tau, const double* 
     });
 }
 
+// VMC::sweep_walkers with use_drift (vmc.cpp:98-139): the particle's
+// derivatives refreshed, the bare Langevin drift (tau/m) grad log|psi| in the
+// proposal, and Metropolis-Hastings with the per-dimension Green's function
+// log-ratio -delta (g + g') + (tau/m)/2 (g^2 - g'^2)
+// (numerics/green_function.h:20-26).  No node gate: the reference's VMC has
+// none.
+int oracle_vmc_drift_sweeps(oracle_system* h, int n_sweeps, double tau, const double* tape,
+                            long long* n_accepted) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        const std::size_t nw = s.nw, np = s.np;
+        long long acc = 0;
+        std::vector<double> m(nw), g, l;
+        for (int sw = 0; sw < n_sweeps; ++sw) {
+            const double* randoms = tape + static_cast<std::size_t>(sw) * np * 3 * nw;
+            for (std::size_t p = 0; p < np; ++p) {
+                for (auto& c : s.comps) c->refresh_particle(s, p);
+                totals(s, g, l, nullptr, nullptr);
+                const double tm = tau * s.inv_mass[p];
+                const double sigma = std::sqrt(tm);
+                const double* gauss = randoms + p * 3 * nw;
+                const double* uni = gauss + 2 * nw;
+                for (std::size_t d = 0; d < 2; ++d)
+                    for (std::size_t w = 0; w < nw; ++w)
+                        s.new_pos[d * nw + w] = s.P(p, d, w) + tm * g[(p * 2 + d) * nw + w] + sigma * gauss[d * nw + w];
+                ratio_all(s, p);
+                for (std::size_t w = 0; w < nw; ++w) {
+                    double lc = 0.0;
+                    for (std::size_t d = 0; d < 2; ++d) {
+                        const double delta = s.new_pos[d * nw + w] - s.P(p, d, w);
+                        const double go = g[(p * 2 + d) * nw + w], gn = s.new_grad[d * nw + w];
+                        lc += -delta * (go + gn) + 0.5 * tm * (go * go - gn * gn);
+                    }
+                    const double la = 2.0 * s.log_ratios[w] + lc;
+                    m[w] = (la >= 0.0 || uni[w] < std::exp(la)) ? 1.0 : 0.0;
+                    if (m[w] != 0.0) ++acc;
+                }
+                accept_all(s, p, m);
+            }
+        }
+        *n_accepted = acc;
+    });
+}
+
 // DMC::sweep_walkers (dmc.cpp:203-337): Umrigar-scaled drift, fixed-node
 // gate (real trials) or fixed phase (complex), Metropolis-Hastings with the
 // drift Green's function, rr accounting.  tape rows have stride tape_stride.

@@ -79,6 +79,8 @@ int oracle_ratio(oracle_system* s, int p, const double* new_pos, double* log_rat
 int oracle_accept(oracle_system* s, int p, const unsigned char* mask);
 int oracle_vmc_sweeps(oracle_system* s, int n_sweeps, double tau, const double* tape,
                       long long* n_accepted);
+int oracle_vmc_drift_sweeps(oracle_system* s, int n_sweeps, double tau, const double* tape,
+                            long long* n_accepted);
 int oracle_dmc_sweeps(oracle_system* s, int n_sweeps, double tau, const double* tape,
                       int tape_stride, double* rr_proposed, double* rr_accepted,
                       long long* n_accepted);

@@ -33,6 +33,7 @@
 #include "hamiltonian/inverse_power.h"
 #include "hamiltonian/kinetic.h"
 #include "numerics/ewald_breakup.h"
+#include "numerics/green_function.h"
 #include "numerics/long_range_breakup.h"
 #include "numerics/pair_radial.h"
 #include "numerics/random.h"
@@ -445,6 +446,58 @@ int ref_vmc_sweeps(void* h, int n_sweeps, double tau, const double* tape,
                 const auto lr = s.ws->log_ratios();
                 for (std::size_t w = 0; w < nw; ++w) {
                     mask[w] = uniform[w] < std::exp(2.0 * lr[w]);
+                    if (mask[w]) ++acc;
+                }
+                s.psi.apply_move_batch_deferred(*s.ws, p, mask);
+                s.ws->apply_move(p, new_pos, s.ws->new_grad(), s.ws->new_lap(), mask);
+            }
+        }
+        *n_accepted = acc;
+    });
+}
+
+// VMC::sweep_walkers with use_drift (vmc.cpp:98-139) restated around the
+// reference's own wavefunction and walker set: refresh_particle_derivatives,
+// the drift proposal, and numerics::langevin_log_ratio_d for the acceptance.
+int ref_vmc_drift_sweeps(void* h, int n_sweeps, double tau, const double* tape,
+                         long long* n_accepted) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        const std::size_t nw = s.nw, np = s.np, dim = s.dim;
+        std::vector<double> tm(np * dim);
+        for (const auto& g : s.groups)
+            for (std::size_t i = g.first; i < g.last; ++i)
+                for (std::size_t d = 0; d < dim; ++d) tm[i * dim + d] = tau * g.inv_mass[d];
+        long long acc = 0;
+        for (int sw = 0; sw < n_sweeps; ++sw) {
+            const double* randoms = tape + static_cast<std::size_t>(sw) * np * (dim + 1) * nw;
+            auto new_pos = s.ws->new_pos();
+            auto mask = s.ws->accept_mask();
+            for (std::size_t p = 0; p < np; ++p) {
+                s.psi.refresh_particle_derivatives(*s.ws, p);
+                const double* gaussian = randoms + p * (dim + 1) * nw;
+                const double* uniform = gaussian + dim * nw;
+                const auto pos = s.ws->positions();
+                const auto grad = s.ws->gradients();
+                for (std::size_t d = 0; d < dim; ++d)
+                    for (std::size_t w = 0; w < nw; ++w) {
+                        const std::size_t off = p * dim * nw + d * nw + w;
+                        new_pos[d * nw + w] = pos[off] + tm[p * dim + d] * grad[off] +
+                                              std::sqrt(tm[p * dim + d]) * gaussian[d * nw + w];
+                    }
+                s.ws->propose_move_batch(p, new_pos);
+                s.psi.ratio_batch(*s.ws, p);
+                const auto lr = s.ws->log_ratios();
+                const auto ng = s.ws->new_grad();
+                for (std::size_t w = 0; w < nw; ++w) {
+                    double lc = 0.0;
+                    for (std::size_t d = 0; d < dim; ++d) {
+                        const std::size_t off = p * dim * nw + d * nw + w;
+                        lc += alea::numerics::langevin_log_ratio_d(new_pos[d * nw + w] - pos[off], grad[off],
+                                                                   ng[d * nw + w], tm[p * dim + d]);
+                    }
+                    const double la = 2.0 * lr[w] + lc;
+                    mask[w] = la >= 0.0 || uniform[w] < std::exp(la);
                     if (mask[w]) ++acc;
                 }
                 s.psi.apply_move_batch_deferred(*s.ws, p, mask);

@@ -122,6 +122,7 @@ def ref_lib():
             "ref_ratio": (C.c_int, [C.c_void_p, C.c_int, D, D, D, D]),
             "ref_accept": (C.c_int, [C.c_void_p, C.c_int, _P_U8]),
             "ref_vmc_sweeps": (C.c_int, [C.c_void_p, C.c_int, C.c_double, D, C.POINTER(C.c_longlong)]),
+            "ref_vmc_drift_sweeps": (C.c_int, [C.c_void_p, C.c_int, C.c_double, D, C.POINTER(C.c_longlong)]),
             "ref_branch": (C.c_int, [D, C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int), D]),
             "ref_popctl_run": (C.c_int, [C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
@@ -214,10 +215,11 @@ class RefWalkers:
         m = u8(mask)
         self._chk(self.lib.ref_accept(self.h, p, ptr(m, _P_U8)))
 
-    def vmc_sweeps(self, n_sweeps, tau, tape):
+    def vmc_sweeps(self, n_sweeps, tau, tape, drift=False):
         acc = C.c_longlong()
         t = f64(tape)
-        self._chk(self.lib.ref_vmc_sweeps(self.h, n_sweeps, tau, ptr(t), C.byref(acc)))
+        f = self.lib.ref_vmc_drift_sweeps if drift else self.lib.ref_vmc_sweeps
+        self._chk(f(self.h, n_sweeps, tau, ptr(t), C.byref(acc)))
         return acc.value
 
     def close(self):

@@ -330,6 +330,13 @@ int run(const char* spec_path, const char* in_path, const char* out_path) {
     }
     out.put("pos3", ws.positions());
     out.put("weights3", ws.weights());
+    // Langevin VMC sweeps (use_drift) of the final population from the VMC
+    // tape, read with the walker count as its row stride (the tape holds
+    // capacity >= walker-count rows)
+    VMC langevin(ws, tau, true);
+    out.put("langevin_accepted", static_cast<double>(langevin.sweep(n_sweeps, tape.data())));
+    out.put("pos_l", ws.positions());
+    out.put("log_psi_l", ws.log_psi());
     return 0;
 }
 

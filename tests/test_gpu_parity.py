@@ -122,6 +122,49 @@ def test_vmc_trajectory_parity(spec):
     assert rel_err(g.local_energy()[0], o.local_energy()[0]) < TOL
 
 
+@pytest.mark.parametrize("spec", specs(), ids=lambda s: s.name)
+def test_langevin_vmc_trajectory_parity(spec):
+    """VMC with use_drift (vmc.cpp:98-139) from one tape: identical
+    Metropolis-Hastings decisions, drifted positions at the drift's 1e-10
+    bar (1e-9 for the McMillan core, as in the DMC sweeps)."""
+    nw, n_sw = 32, 3
+    pos = _pos(spec, nw, 5)
+    g, o = gpu_walkers(spec, nw), oracle_walkers(spec, nw)
+    for x in (g, o):
+        x.set_positions(pos)
+        x.evaluate()
+    tape = oracle_tape(18, spec.n_particles, nw, n_sw)
+    # the DMC sweeps' step, (L/40)^2: the drift (tau/m) g multiplies the
+    # gradients' summation-order rounding by tau, and at (L/12)^2 = 510 in
+    # the rs = 30 gas three sweeps carry it to 1.4e-10 of the cell
+    tau = 0.1 if spec.lattice is None else (spec.lattice[0] / 40) ** 2
+    ag = g.vmc_sweeps(n_sw, tau, tape, drift=True)
+    ao = o.vmc_sweeps(n_sw, tau, tape, drift=True)
+    assert ag == ao and 0 < ag < n_sw * nw * spec.n_particles
+    tol = 1e-9 if spec.mcmillan else TOL
+    assert pos_err(g.positions(), o.positions()) < tol
+    assert rel_err(g.log_psi(), o.log_psi()) < tol
+    assert rel_err(g.local_energy()[0], o.local_energy()[0]) < tol
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED], ids=["fp64", "mixed"])
+def test_langevin_vmc_exact_on_device_streams(precision):
+    """The drift of the exact plane-wave eigenstate (N = 10, no Jastrow, no
+    interaction) keeps E_L at sum |k|^2 / 2 on the device's own streams;
+    the mixed form holds it to its FP32 inverse."""
+    s = heg_spec(5, 5, 2.0, jastrow=False, ewald=False)
+    nw = 128
+    g = gpu_walkers(s, nw, precision)
+    g.set_positions(_pos(s, nw, 8))
+    g.evaluate()
+    seed(g, 21)
+    exact = 0.5 * np.sum(np.concatenate([b[2] for b in s.slater_pw]) ** 2)
+    acc = g.vmc_sweeps(10, (s.lattice[0] / 12) ** 2, drift=True)
+    assert 0 < acc < 10 * nw * s.n_particles
+    e, _ = g.local_energy()
+    assert np.allclose(e, exact, rtol=1e-9 if precision == FP64 else 1e-4)
+
+
 def test_exciton_exact_on_device():
     s = exciton_spec()
     nw = 256

@@ -193,6 +193,9 @@ def _python_run(make, case):
             out[f"dmc_src{step}"] = np.asarray(h["src"], dtype=np.float64)
         out["pos3"] = w.positions()
         out["weights3"] = w.weights()
+        out["langevin_accepted"] = np.array([float(w.vmc_sweeps(n_sw, tau, tape, drift=True))])
+        out["pos_l"] = w.positions()
+        out["log_psi_l"] = w.log_psi()
     finally:
         w.close()
     return out

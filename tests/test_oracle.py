@@ -22,7 +22,7 @@ import pytest
 
 from alea_amd import breakup
 from alea_amd.system import dipole_spec, Spec, exciton_spec, heg_spec, hexagonal_lattice, uniform_positions
-from backends import RefWalkers, fixture_tables, oracle_lib, oracle_tape, oracle_walkers, ref_lib, rel_err
+from backends import RefWalkers, fixture_tables, oracle_lib, oracle_tape, oracle_walkers, pos_err, ref_lib, rel_err
 import ctypes as C
 
 XOSHIRO_1234 = [41943041, 58720359, 3588806011781223, 3591011842654386,
@@ -269,6 +269,14 @@ def test_oracle_matches_reference(spec):
     tau = (spec.lattice[0] / 8) ** 2
     assert o.vmc_sweeps(n_sw, tau, tape) == r.vmc_sweeps(n_sw, tau, tape)
     assert np.array_equal(o.positions(), r.positions())
+    assert rel_err(o.log_psi(), r.log_psi()) < 1e-11
+    assert rel_err(o.local_energy()[0], r.local_energy()[0]) < 1e-10
+    # Langevin VMC (vmc.cpp:98-139) from the same tape: identical decisions;
+    # the drift carries the gradients' rounding into the positions
+    tape = oracle_tape(22, spec.n_particles, nw, n_sw)
+    na = o.vmc_sweeps(n_sw, tau, tape, drift=True)
+    assert na == r.vmc_sweeps(n_sw, tau, tape, drift=True) and 0 < na < n_sw * nw * spec.n_particles
+    assert pos_err(o.positions(), r.positions()) < 1e-12
     assert rel_err(o.log_psi(), r.log_psi()) < 1e-11
     assert rel_err(o.local_energy()[0], r.local_energy()[0]) < 1e-10
 

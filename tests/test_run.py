@@ -51,7 +51,8 @@ def test_rows_round_trip_and_seeding():
 
 
 @pytest.mark.parametrize("edit,msg", [
-    (lambda c: c["qmc"][0].update(random_walk="langevin"), "random_walk"),
+    (lambda c: c["qmc"][0].update(random_walk="heat_bath"), "random_walk"),
+    (lambda c: c["qmc"][1].update(random_walk="metropolis"), "random_walk"),
     (lambda c: c["qmc"][1].update(reweighting="suppressed"), "reweighting"),
     (lambda c: c["qmc"][0].update(estimators=[{"type": "bond_order"}]), "bond_order"),
     (lambda c: c["qmc"][1].update(estimators=[{"type": "gofr", "rmax": 1, "n_bins": 4, "name": "x"}]), "named"),
@@ -79,6 +80,24 @@ def test_exciton_config_runs_exactly():
     assert all(abs(e + 1.0) < 1e-12 for e in dmc_["block_energies"])
     assert abs(dmc_["e_trial"] + 1.0) < 1e-6
     assert 0.9 < dmc_["acceptance"] <= 1.0
+
+
+@pytest.mark.gpu
+def test_exciton_langevin_vmc_runs_exactly():
+    """random_walk: langevin (app.cpp:162) drifts the exact exciton along its
+    own gradient: E_L stays -1 with zero variance, and the drift raises the
+    acceptance above the Metropolis walk's at the same time step."""
+    import copy
+    cfg = copy.deepcopy(EXCITON)
+    cfg["qmc"] = [cfg["qmc"][0]]
+    print("run: exciton Metropolis VMC", flush=True)
+    (plain,) = run.run(cfg)
+    cfg["qmc"][0]["random_walk"] = "langevin"
+    print("run: exciton Langevin VMC", flush=True)
+    (vmc,) = run.run(cfg)
+    assert all(abs(e + 1.0) < 1e-12 for e in vmc["block_energies"])
+    assert abs(vmc["variance"]) < 1e-12
+    assert plain["acceptance"] < vmc["acceptance"] < 1.0
 
 
 @pytest.mark.gpu
