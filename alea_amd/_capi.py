@@ -47,6 +47,10 @@ SIGNATURES = {
     "ratio_range": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_D, _P_D, _P_D, _P_D, _P_D]),
     "accept_range": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_U8]),
     "vmc_sweeps": (C.c_int, [_V, C.c_int, C.c_double, _P_D, _P_LL]),
+    "snapshots_create": (C.c_int, [_V, C.c_int, C.POINTER(C.c_void_p)]),
+    "snapshots_destroy": (C.c_int, [_V]),
+    "snapshots_record": (C.c_int, [_V, C.c_int]),
+    "snapshots_load": (C.c_int, [_V, C.c_int, _V, _P_I, _P_D, C.c_int]),
     "vmc_drift_sweeps": (C.c_int, [_V, C.c_int, C.c_double, _P_D, _P_LL]),
     "dmc_sweeps": (C.c_int, [_V, C.c_int, C.c_double, _P_D, C.c_int, _P_D, _P_D, _P_LL]),
     "permute": (C.c_int, [_V, C.c_int, _P_I]),

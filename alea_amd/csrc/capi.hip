@@ -1761,6 +1761,94 @@ int alea_get_num_walkers(alea_system* s, int* nw) {
     });
 }
 
+// ── Forward walking's snapshots (forward_walking.cpp:99-108, 148-150) ─────
+// A ring of position snapshots of one population, kept on its device: a
+// slot is a copy of the walker-major position records of the population's
+// walkers at one step.  The ancestry (integers) stays with the caller; a
+// load gathers the kept ancestors' configurations into a measurement batch.
+struct alea_snapshots {
+    alea_system* pop = nullptr;
+    int n_slots = 0;
+    size_t slot_doubles = 0;        // capacity * np * 2
+    std::vector<int> nw;            // walkers recorded in each slot (-1: none)
+    DevBuf<double> pos;             // [slot][capacity][np][2]
+    DevBuf<int> idx;                // [capacity] a load's gather indices
+};
+
+int alea_snapshots_create(alea_system* population, int n_slots, alea_snapshots** out) {
+    return guarded([&] {
+        Sys(population);
+        if (!out) throw InvalidArg("null output handle");
+        if (!population->built) throw InvalidArg("positions were never set");
+        if (n_slots < 1) throw InvalidArg("a snapshot ring needs at least one slot");
+        auto r = std::make_unique<alea_snapshots>();
+        r->pop = population;
+        r->n_slots = n_slots;
+        r->slot_doubles = static_cast<size_t>(population->capacity) * population->np * 2;
+        r->nw.assign(n_slots, -1);
+        r->pos.alloc(r->slot_doubles * n_slots, population->stream);
+        r->idx.alloc(population->capacity, population->stream);
+        population->sync();
+        *out = r.release();
+    });
+}
+
+int alea_snapshots_destroy(alea_snapshots* r) {
+    return guarded([&] {
+        if (!r) return;
+        r->pop->sync();
+        delete r;
+    });
+}
+
+int alea_snapshots_record(alea_snapshots* r, int slot) {
+    return guarded([&] {
+        if (!r) throw InvalidArg("null snapshot handle");
+        if (slot < 0 || slot >= r->n_slots) throw InvalidArg("snapshot slot out of range");
+        alea_system* s = r->pop;
+        const size_t n = static_cast<size_t>(s->nw) * s->np * 2;
+        hip_check(hipMemcpyAsync(r->pos.p + slot * r->slot_doubles, s->pos.p, n * sizeof(double),
+                                 hipMemcpyDeviceToDevice, s->stream),
+                  "record snapshot");
+        r->nw[slot] = s->nw;
+    });
+}
+
+int alea_snapshots_load(alea_snapshots* r, int slot, alea_system* batch, const int* kept, const double* weights,
+                        int n_kept) {
+    return guarded([&] {
+        if (!r) throw InvalidArg("null snapshot handle");
+        Sys(batch);
+        if (slot < 0 || slot >= r->n_slots || r->nw[slot] < 0) throw InvalidArg("snapshot slot never recorded");
+        alea_system* s = r->pop;
+        if (batch->np != s->np) throw InvalidArg("the batch has another particle count");
+        if (n_kept < 1 || n_kept > batch->capacity) throw InvalidArg("kept count out of the batch's capacity");
+        if (!kept || !weights) throw InvalidArg("null kept indices or weights");
+        for (int c = 0; c < n_kept; ++c)
+            if (kept[c] < 0 || kept[c] >= r->nw[slot]) throw InvalidArg("kept index outside the snapshot");
+        for (int c = 0; c < n_kept; ++c)
+            if (!(weights[c] >= 0.0) || !std::isfinite(weights[c]))
+                throw InvalidArg("weights must be finite and non-negative");
+        batch->build();
+        hip_check(copy_on(s->stream, r->idx.p, kept, n_kept * sizeof(int), hipMemcpyHostToDevice), "kept");
+        launch_gather(r->pos.p + slot * r->slot_doubles, batch->pos.p, s->np * 2, r->idx.p, n_kept, s->stream);
+        hip_check(hipGetLastError(), "launch gather");
+        s->sync();
+        batch->nw = n_kept;
+        hip_check(copy_on(batch->stream, batch->weights.p, weights, n_kept * sizeof(double), hipMemcpyHostToDevice),
+                  "weights");
+        batch->set_model_nw(n_kept);
+        launch_evaluate(batch->dmodel, batch->st, n_kept, batch->mixed(), batch->cplx, batch->walker_lds,
+                        batch->stream);
+        hip_check(hipGetLastError(), "launch evaluate");
+        batch->refresh();
+        batch->sync();
+        batch->check_status();
+        batch->evaluated = true;
+        batch->energies_valid = false;
+    });
+}
+
 // DMC::advance_measure's set-up of a step (dmc.cpp:349-354): rr sums to zero,
 // weights to one, E_old = the current local energies.
 int alea_dmc_begin(alea_system* s) {

@@ -142,8 +142,12 @@ class DMC:
     """
 
     def __init__(self, walkers, tau, target, n_substeps=1, generations=10.0, cutoff="universal",
-                 alpha=K_DEFAULT_BRANCH_CUTOFF_ALPHA, seed=42, tapes=None, stride=None):
+                 alpha=K_DEFAULT_BRANCH_CUTOFF_ALPHA, seed=42, tapes=None, stride=None, forward_walking=None):
+        """forward_walking: a prepared alea_amd.forward_walking.ForwardWalking;
+        each step it observes the reweighted population before the branch
+        and follows the branch's source map (dmc.cpp:711, 735)."""
         self.w = walkers
+        self.fw = forward_walking
         self.tau = tau
         self.target = target
         self.n_substeps = n_substeps
@@ -181,9 +185,13 @@ class DMC:
         e_sum, w_sum, n_cut = w.dmc_sums()
         if measure and on_measure is not None:
             on_measure()
+        if self.fw is not None:
+            self.fw.observe(self.step_index)
         ens_e = e_sum / w_sum if w_sum > 0.0 else pc.etrial
         nw_before = w.nw
         src = w.branch(self.branch_state, self.target)
+        if self.fw is not None:
+            self.fw.branch(src)
         pc.update(ens_e, w_sum, measure)
         self.history.append(dict(step=self.step_index, nw_before=nw_before, nw=w.nw, e_sum=e_sum,
                                  w_sum=w_sum, n_cut=n_cut, n_accepted=n_acc, etrial=pc.etrial,

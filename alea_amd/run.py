@@ -12,22 +12,29 @@ src/qmc/app.cpp:1066-1098).
               accumulation blocks give the weighted-mean block energies
   errors      numerics::reblock (alea_amd.statistics) over the block energies
 
+A VMC stage walks by Metropolis moves or, with random_walk: langevin, by
+the drift proposal (app.cpp:162).  A DMC stage's forward_walking {lags,
+interval} adds the pure estimators (alea_amd.forward_walking): the pure
+potential and the stage's estimators again for every lag.
+
 The walkers draw from the device streams (one xoshiro256++ stream per
 walker, seeded from the stage seed), so a run reproduces the reference's
 statistics, not its trajectories (those are replayed through tapes in the
 tests).  The `gofr`, `density` and `sofk` estimators of a stage are measured on the
 device (alea_amd.estimators) after each accumulation step, with the DMC
 weights before the branch in DMC.  Run settings the GPU path does not hold
-(langevin VMC moves, non-bare reweighting, forward walking, other
-estimators, walker files, optimizer and PIMC stages) are refused with a
-ValueError.
+(non-bare reweighting, other estimators, walker files, optimizer and PIMC
+stages) are refused with a ValueError.
 """
 from __future__ import annotations
+
+import math
 
 import numpy as np
 
 from . import config, dmc, statistics
 from .estimators import Estimators
+from .forward_walking import ForwardWalking
 from .walkers import FP64, gpu_walkers, seed as seed_walkers
 
 
@@ -55,9 +62,11 @@ def _check_stage(st: dict):
     method = st.get("method")
     if method not in ("vmc", "dmc"):
         _refuse(f"method '{method}'")
-    for key in ("walkers_in", "walkers_out", "forward_walking", "paths_in", "paths_out"):
+    for key in ("walkers_in", "walkers_out", "paths_in", "paths_out"):
         if st.get(key):
             _refuse(f"`{key}` of a {method} stage")
+    if st.get("forward_walking") and method != "dmc":
+        raise ValueError(f"run: {method}: forward_walking belongs to a dmc stage")
     _estimator_args(st)
     for key in ("samples", "accumulation_blocks", "steps_per_block", "sub_steps", "time_step"):
         if not st.get(key):
@@ -70,6 +79,46 @@ def _check_stage(st: dict):
         raise ValueError(f"run: {method}: random_walk '{walk}' is not one the reference accepts")
     if method == "dmc" and st.get("reweighting", "bare") != "bare":
         _refuse(f"reweighting '{st.get('reweighting')}'")
+    if method == "dmc":
+        _forward_walking(st)
+
+
+def _forward_walking(st: dict):
+    """The stage's forward_walking {lags, interval} as (lag steps, interval),
+    or None (parser.cpp:1310-1336, dmc.cpp:102-139): every lag a whole number
+    of steps of time_step * sub_steps, strictly increasing; the interval
+    dividing steps_per_block; the largest lag within the equilibration."""
+    fw = st.get("forward_walking")
+    if not fw:
+        return None
+    if not isinstance(fw, dict) or set(fw) != {"lags", "interval"}:
+        raise ValueError("dmc: forward_walking must be a map {lags: [...], interval: n}")
+    lags = fw["lags"]
+    if not isinstance(lags, (list, tuple)) or not lags:
+        raise ValueError("dmc: forward_walking needs `lags`, a nonempty list of imaginary times")
+    interval = int(fw["interval"])
+    step_time = float(st["time_step"]) * int(st["sub_steps"])
+    n_steps = int(st["steps_per_block"])
+    if interval < 1 or n_steps % interval != 0:
+        raise ValueError(f"dmc: forward_walking interval ({interval}) must be >= 1 and divide steps_per_block "
+                         f"({n_steps})")
+    out = []
+    for i, lag in enumerate(float(x) for x in lags):
+        if not (math.isfinite(lag) and lag >= 0.0):
+            raise ValueError("dmc: forward_walking lags must be finite and >= 0")
+        if i > 0 and not lag > float(lags[i - 1]):
+            raise ValueError("dmc: forward_walking lags must be strictly increasing")
+        steps = lag / step_time
+        whole = round(steps)
+        if abs(steps - whole) > 1e-9 * max(1.0, steps):
+            raise ValueError(f"dmc: forward_walking lag {lag} is not a whole number of steps of "
+                             f"time_step * sub_steps = {step_time}")
+        out.append(int(whole))
+    eq_steps = max(0, int(st.get("equilibration_blocks", 0))) * n_steps
+    if out[-1] > eq_steps:
+        raise ValueError(f"dmc: the largest forward_walking lag ({out[-1]} steps) exceeds the equilibration "
+                         f"({eq_steps} steps): snapshots start that many steps before accumulation")
+    return out, interval
 
 
 def _estimator_args(st: dict) -> dict:
@@ -163,6 +212,7 @@ def run_dmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, t
     n = spec.n_particles
     cap = 2 * target + 1
     w = gpu_walkers(spec, cap, precision)
+    fw = None
     try:
         w.set_positions(_soa(_seed_rows(rows, cap), n))
         w.evaluate()
@@ -172,11 +222,19 @@ def run_dmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, t
         measure = (lambda: est.accumulate(use_weights=True)) if est else None
         cutoff = st.get("branch_cutoff", "universal")
         alpha = st.get("branch_cutoff_alpha")
+        n_steps = int(st["steps_per_block"])
+        sched = _forward_walking(st)
+        if sched is not None:
+            # the stage's estimators again for every lag, labelled fw<i>
+            # (builders/estimators.cpp:158-195), and the pure potential
+            args = _estimator_args(st)
+            fw = ForwardWalking(sched[0], float(st["time_step"]) * int(st["sub_steps"]), sched[1],
+                                [args or None] * len(sched[0]))
+            fw.prepare(w, int(st.get("equilibration_blocks", 0)) * n_steps, group_names)
         d = dmc.DMC(w, float(st["time_step"]), target, n_substeps=int(st["sub_steps"]),
                     generations=float(st.get("population_generations", 10.0)), cutoff=cutoff,
                     alpha=dmc.K_DEFAULT_BRANCH_CUTOFF_ALPHA if alpha is None else float(alpha),
-                    seed=stage_seed)
-        n_steps = int(st["steps_per_block"])
+                    seed=stage_seed, forward_walking=fw)
         eq_blocks = []
         for _ in range(int(st.get("equilibration_blocks", 0))):
             be = bw = 0.0
@@ -193,6 +251,8 @@ def run_dmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, t
             be = bw = 0.0
             if est:
                 est.start_block()
+            if fw:
+                fw.start_block()
             for _ in range(n_steps):
                 att += w.nw * n * int(st["sub_steps"])
                 e, ws = d.step(measure=True, on_measure=measure)
@@ -203,15 +263,21 @@ def run_dmc(spec, st: dict, rows: np.ndarray, stage_seed: int, precision=FP64, t
             pops.append(w.nw)
             if est:
                 est.stop_block()
+            if fw:
+                fw.stop_block()
         rb = statistics.reblock(blocks)
         out = dict(method="dmc", target=target, energy=rb.mean, error=rb.error, tau_int=rb.tau_int,
                    acceptance=acc / att if att else 0.0, mean_population=float(np.mean(pops)),
                    e_trial=d.pc.etrial, block_energies=blocks, equilibration_block_energies=eq_blocks)
         if est:
             out["estimators"] = est.finalize()
+        if fw:
+            out["forward_walking"] = fw.results()
         out["final_rows"] = _rows(w.positions(), n, w.nw)
         return out
     finally:
+        if fw:
+            fw.close()
         w.close()
 
 

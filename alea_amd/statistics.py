@@ -17,6 +17,7 @@ import math
 from dataclasses import dataclass
 
 K_MIN_BLOCKS = 16
+K_MIN_EFFECTIVE_BLOCKS = 50     # qmc/reblock_diagnostics.h:28 kMinEffectiveBlocks
 
 
 @dataclass
@@ -128,6 +129,19 @@ def reblock(series) -> ReblockResult:
     if len(x) < 2:
         return res
     return _assemble(res, _table(x))
+
+
+def reblock_diagnostics(name: str, res: ReblockResult, n_samples: int,
+                        min_effective_blocks: int = K_MIN_EFFECTIVE_BLOCKS) -> dict:
+    """add_reblock_diagnostics (qmc/reblock_diagnostics.h:30-66): the
+    reliability figures written beside a reblocked mean."""
+    eff = n_samples / (1.0 + 2.0 * res.tau_int)
+    reliable = res.plateau_found and res.selected_n_blocks >= K_MIN_BLOCKS and eff >= min_effective_blocks
+    return {name + "_effective_blocks": eff, name + "_tau_int": res.tau_int,
+            name + "_error_naive": res.naive_error, name + "_reblock_plateau": 1.0 if res.plateau_found else 0.0,
+            name + "_error_reliable": 1.0 if reliable else 0.0, name + "_reblock_level": float(res.selected_level),
+            name + "_reblock_n_blocks": float(res.selected_n_blocks), name + "_reblock_onset_level": float(res.level),
+            name + "_error_on_error": res.error_on_error}
 
 
 def reblock_chains(chains, n_chains: int) -> ReblockResult:

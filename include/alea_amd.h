@@ -296,6 +296,27 @@ int alea_permute(alea_system* s, int new_nw, const int* src);
 /* WalkerSet::num_walkers (walker_set.h): the current population. */
 int alea_get_num_walkers(alea_system* s, int* nw);
 
+/* ── Forward walking's snapshots (ForwardWalking, forward_walking.h:60-147).
+ * The ring of ForwardWalking::Snapshot::positions (forward_walking.h:
+ * 116-121), held on the population's device: alea_snapshots_record is
+ * ForwardWalking::record (forward_walking.cpp:99-108) without the ancestor
+ * indices, which the caller keeps and carries through each branch's source
+ * map (ForwardWalking::branch, forward_walking.cpp:175-182, integers on the
+ * host).  alea_snapshots_load is ConfigurationBatch::load of the kept
+ * ancestors (forward_walking.cpp:148-150): the batch's walkers become
+ * configurations kept[0..n_kept) of the slot with the given weights (the
+ * descendant weights D[a]), and the batch is evaluated, so its local energy
+ * and estimators measure them (with no wavefunction factors in the batch,
+ * E_L is Hamiltonian::evaluate_potential).  The batch must have the
+ * population's particle count and capacity >= n_kept.  Destroy the ring
+ * before its population. */
+typedef struct alea_snapshots alea_snapshots;
+int alea_snapshots_create(alea_system* population, int n_slots, alea_snapshots** out);
+int alea_snapshots_destroy(alea_snapshots* r);
+int alea_snapshots_record(alea_snapshots* r, int slot);
+int alea_snapshots_load(alea_snapshots* r, int slot, alea_system* batch, const int* kept, const double* weights,
+                        int n_kept);
+
 /* ── DMC population step (DMC::advance_measure, dmc.cpp:339-419, bare
  * reweighting; DMC::run_one_step, dmc.cpp:693-739).  A step is:
  *   alea_dmc_begin; n_substeps x { alea_dmc_sweeps(rr NULL); alea_dmc_weights };

@@ -2445,6 +2445,64 @@ int oracle_get_num_walkers(oracle_system* h, int* nw) {
     return guarded([&] { *nw = static_cast<int>(S(h)->nw); });
 }
 
+// ForwardWalking's snapshot ring (forward_walking.cpp:99-108): the
+// population's positions at a step, [slot][p][d][nw of that step], and
+// ConfigurationBatch::load of the kept ancestors (forward_walking.cpp:148-150)
+// into a measurement batch, evaluated with the descendant weights.
+struct oracle_snapshots {
+    Sys* pop = nullptr;
+    std::vector<std::vector<double>> pos;
+    std::vector<long long> nw;
+};
+
+int oracle_snapshots_create(oracle_system* population, int n_slots, oracle_snapshots** out) {
+    return guarded([&] {
+        if (n_slots < 1) throw std::invalid_argument("oracle: a snapshot ring needs at least one slot");
+        auto* r = new oracle_snapshots;
+        r->pop = S(population);
+        r->pos.resize(n_slots);
+        r->nw.assign(n_slots, -1);
+        *out = r;
+    });
+}
+
+int oracle_snapshots_destroy(oracle_snapshots* r) {
+    delete r;
+    return 0;
+}
+
+int oracle_snapshots_record(oracle_snapshots* r, int slot) {
+    return guarded([&] {
+        if (slot < 0 || static_cast<std::size_t>(slot) >= r->pos.size())
+            throw std::invalid_argument("oracle: snapshot slot out of range");
+        r->pos[slot] = r->pop->pos;
+        r->nw[slot] = static_cast<long long>(r->pop->nw);
+    });
+}
+
+int oracle_snapshots_load(oracle_snapshots* r, int slot, oracle_system* batch, const int* kept,
+                          const double* weights, int n_kept) {
+    return guarded([&] {
+        if (slot < 0 || static_cast<std::size_t>(slot) >= r->pos.size() || r->nw[slot] < 0)
+            throw std::invalid_argument("oracle: snapshot slot never recorded");
+        Sys& b = *S(batch);
+        if (b.np != r->pop->np) throw std::invalid_argument("oracle: the batch has another particle count");
+        if (n_kept < 1) throw std::invalid_argument("oracle: nothing to load");
+        const std::vector<double>& src = r->pos[slot];
+        const std::size_t snw = static_cast<std::size_t>(r->nw[slot]), n = static_cast<std::size_t>(n_kept);
+        for (std::size_t c = 0; c < n; ++c)
+            if (kept[c] < 0 || static_cast<std::size_t>(kept[c]) >= snw)
+                throw std::invalid_argument("oracle: kept index outside the snapshot");
+        b.nw = n;
+        b.alloc();
+        for (std::size_t k = 0; k < b.np * b.dim; ++k)
+            for (std::size_t c = 0; c < n; ++c) b.pos[k * n + c] = src[k * snw + static_cast<std::size_t>(kept[c])];
+        b.build_dist();
+        std::copy(weights, weights + n, b.weights.begin());
+        for (auto& c : b.comps) c->evaluate(b);
+    });
+}
+
 // DMC::advance_measure set-up (dmc.cpp:349-354).
 int oracle_dmc_begin(oracle_system* h) {
     return guarded([&] {

@@ -85,6 +85,12 @@ int oracle_dmc_sweeps(oracle_system* s, int n_sweeps, double tau, const double* 
                       int tape_stride, double* rr_proposed, double* rr_accepted,
                       long long* n_accepted);
 int oracle_permute(oracle_system* s, int new_nw, const int* src);
+typedef struct oracle_snapshots oracle_snapshots;
+int oracle_snapshots_create(oracle_system* population, int n_slots, oracle_snapshots** out);
+int oracle_snapshots_destroy(oracle_snapshots* r);
+int oracle_snapshots_record(oracle_snapshots* r, int slot);
+int oracle_snapshots_load(oracle_snapshots* r, int slot, oracle_system* batch, const int* kept,
+                          const double* weights, int n_kept);
 int oracle_get_num_walkers(oracle_system* s, int* nw);
 int oracle_device_tape(uint64_t seed, uint64_t stream, int np, int nw, int n_sweeps, double* out);
 int oracle_ratio_range(oracle_system* s, int p, int w_begin, int w_end, const double* new_pos, double* log_ratio,

@@ -42,6 +42,7 @@
 #include "population/simulation_cell.h"
 #include "population/walker.h"
 #include "population/walker_set.h"
+#include "qmc/forward_walking.h"
 #include "qmc/population_control.h"
 #include "qmc/run_result.h"
 #include "qmc/sweep_blocking.h"
@@ -633,6 +634,80 @@ int ref_density(void* h, double rmax, int n_bins, int per_particle, int n_blocks
     if (rc) return rc;
     return run_estimators(h, ests, n_blocks, steps, pos, w, names, names_cap, values, values_cap, lengths,
                           lengths_cap, n_series);
+}
+
+// ForwardWalking (forward_walking.cpp) over a scripted population history on
+// the harness's walker set.  Step t (numbered from 0) sets the population's
+// nw_t walkers to positions pos_t [np*dim*nw_t] and weights w_t [nw_t], runs
+// ForwardWalking::observe, then passes the step's source map src_t
+// [nw_{t+1}] to ForwardWalking::branch.  Blocks of steps_per_block steps start at
+// first_measured.  Every lag measures the potential energy, and a pair
+// correlation when gofr_bins > 0.  Output: write_results' series, and its
+// scalars as one-value series, in the layout of run_estimators.
+int ref_forward_walking(void* h, int n_lags, const int* lags, double step_time, int interval, int first_measured,
+                        int n_steps, int steps_per_block, double gofr_rmax, int gofr_bins, const int* nw_of_step,
+                        const double* pos, const double* w, const int* src, char* names, int names_cap,
+                        double* values, int values_cap, int* lengths, int lengths_cap, int* n_series) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        build_hamiltonian(s);
+        std::vector<std::vector<std::unique_ptr<alea::Estimator>>> per_lag(static_cast<std::size_t>(n_lags));
+        if (gofr_bins > 0)
+            for (auto& set : per_lag) {
+                set.push_back(std::make_unique<alea::PairCorrelation>(s.groups, s.dim, gofr_rmax, gofr_bins, s.cell));
+                set.back()->set_output_label("fw" + std::to_string(&set - per_lag.data()));
+            }
+        std::vector<std::size_t> lag_steps(lags, lags + n_lags);
+        alea::ForwardWalking fw(lag_steps, step_time, static_cast<std::size_t>(interval), std::move(per_lag));
+        std::size_t capacity = 0;
+        for (int t = 0; t < n_steps; ++t) capacity = std::max(capacity, static_cast<std::size_t>(nw_of_step[t]));
+        fw.prepare(capacity, s.np, s.dim, s.cell, static_cast<std::size_t>(first_measured));
+        std::size_t pos_off = 0, w_off = 0, src_off = 0;
+        for (int t = 0; t < n_steps; ++t) {
+            // every step sets every position and weight, so the population is
+            // a fresh walker set of the step's size (the harness's sets carry
+            // no Walker handles for apply_permutation to move)
+            const std::size_t nw = static_cast<std::size_t>(nw_of_step[t]);
+            s.ws = std::make_unique<alea::WalkerSet>(nw, s.np, s.dim, s.cell);
+            s.nw = nw;
+            if (ref_set_positions(h, pos + pos_off)) throw std::runtime_error(g_error);
+            pos_off += s.np * s.dim * nw;
+            auto wt = s.ws->weights();
+            std::copy(w + w_off, w + w_off + nw, wt.begin());
+            w_off += nw;
+            const bool measured = t >= first_measured;
+            const bool opens = measured && (t - first_measured) % steps_per_block == 0;
+            const bool closes = measured && (t - first_measured + 1) % steps_per_block == 0;
+            if (opens) fw.start_block();
+            fw.observe(*s.ws, static_cast<std::size_t>(t), s.ham);
+            if (closes) fw.stop_block();
+            if (t + 1 < n_steps) {
+                const std::size_t nn = static_cast<std::size_t>(nw_of_step[t + 1]);
+                std::vector<std::size_t> map(src + src_off, src + src_off + nn);
+                src_off += nn;
+                fw.branch(map);
+            }
+        }
+        alea::RunResult rr;
+        fw.write_results(rr);
+        for (const auto& [name, value] : rr.scalars) rr.series.emplace_back(name, std::vector<double>{value});
+        std::string joined;
+        std::size_t nv = 0;
+        int ns = 0;
+        for (const auto& [name, vals] : rr.series) {
+            if (ns >= lengths_cap || nv + vals.size() > static_cast<std::size_t>(values_cap))
+                throw std::runtime_error("ref_forward_walking: output capacity");
+            joined += name + "\n";
+            std::copy(vals.begin(), vals.end(), values + nv);
+            nv += vals.size();
+            lengths[ns++] = static_cast<int>(vals.size());
+        }
+        if (joined.size() + 1 > static_cast<std::size_t>(names_cap))
+            throw std::runtime_error("ref_forward_walking: name capacity");
+        std::copy(joined.begin(), joined.end(), names);
+        names[joined.size()] = '\0';
+        *n_series = ns;
+    });
 }
 
 }  // extern "C"

@@ -58,6 +58,12 @@ def test_rows_round_trip_and_seeding():
     (lambda c: c["qmc"][1].update(estimators=[{"type": "gofr", "rmax": 1, "n_bins": 4, "name": "x"}]), "named"),
     (lambda c: c["qmc"].insert(0, {"method": "optimizer"}), "optimizer"),
     (lambda c: c["qmc"][0].update(time_step=0), "time_step"),
+    (lambda c: c["qmc"][0].update(forward_walking={"lags": [0.1], "interval": 1}), "belongs to a dmc stage"),
+    (lambda c: c["qmc"][1].update(forward_walking={"lags": [0.03], "interval": 1}), "whole number of steps"),
+    (lambda c: c["qmc"][1].update(forward_walking={"lags": [0.08, 0.04], "interval": 1}), "strictly increasing"),
+    (lambda c: c["qmc"][1].update(forward_walking={"lags": [0.04], "interval": 2}), "divide steps_per_block"),
+    (lambda c: c["qmc"][1].update(forward_walking={"lags": [0.44], "interval": 1}), "exceeds the equilibration"),
+    (lambda c: c["qmc"][1].update(forward_walking={"lags": [0.04]}), "must be a map"),
 ])
 def test_refusals(edit, msg, monkeypatch):
     import copy
@@ -98,6 +104,28 @@ def test_exciton_langevin_vmc_runs_exactly():
     assert all(abs(e + 1.0) < 1e-12 for e in vmc["block_energies"])
     assert abs(vmc["variance"]) < 1e-12
     assert plain["acceptance"] < vmc["acceptance"] < 1.0
+
+
+@pytest.mark.gpu
+def test_exciton_dmc_with_forward_walking():
+    """forward_walking {lags, interval} on the exact exciton's DMC stage: the
+    pure potential at every lag, with the lag's g(r) labelled fw<i>, and the
+    virial theorem's <V> = 2E = -2 within the reblocked errors."""
+    import copy
+    cfg = copy.deepcopy(EXCITON)
+    st = cfg["qmc"][1]
+    st.update(forward_walking={"lags": [0.0, 0.2, 0.4], "interval": 5}, equilibration_blocks=4,
+              accumulation_blocks=20, estimators=[{"type": "gofr", "rmax": 3.0, "n_bins": 12}])
+    print("run: exciton DMC with forward walking", flush=True)
+    _, d = run.run(cfg)
+    fw = d["forward_walking"]
+    assert fw["fw_lag_times"] == pytest.approx([0.0, 0.2, 0.4])
+    assert [fw[f"fw{i}_lag_steps"] for i in range(3)] == [0.0, 5.0, 10.0]
+    for i in range(3):
+        assert fw[f"fw{i}_measurements"] == 20.0        # one snapshot per 5 steps, 5 steps per block
+        v, e = fw[f"potential_fw{i}"], fw[f"potential_fw{i}_error"]
+        assert e > 0.0 and abs(v + 2.0) < 5.0 * e + 2e-2, (i, v, e)
+        assert len(fw[f"gofr_fw{i}_electron_group_hole_group"]) == 12
 
 
 @pytest.mark.gpu
