@@ -1491,6 +1491,76 @@ __global__ __launch_bounds__(64) void k_evaluate(const DevModel* __restrict__ Mp
     wk.store();
 }
 
+// The end of a batched rebuild: the inverse (held as the eliminated tile a,
+// rows and columns in pivot order) written to the walker's record in the
+// sweep's precision, the row sums of |M| (from part), the condition
+// estimate, the cleared update count and log|det|.
+template <typename T, bool CPLX, int NP, int GX>
+__device__ __attribute__((always_inline)) inline void refresh_store(const DevModel& M, const DevState& S, int w, int b, int n, int tid,
+                                     const double (&a)[NP / 16][NP / GX][CPLX ? 2 : 1], double (*part)[NP],
+                                     const int* perm, const int* pinv, double (*red)[GX * 16 / W], double log_det) {
+    constexpr int GY = 16;
+    constexpr int TR = NP / GY, TC = NP / GX;
+    constexpr int E = CPLX ? 2 : 1;
+    constexpr int NT = GX * GY, NWAVES = NT / W;
+    const int tx = tid / GY, ty = tid % GY;
+    // column sums of |A| (a permutation of M's columns' sums), max over the
+    // real columns; |re| + |im| as in the sweep's update
+    double colmax = 0.0;
+#pragma unroll
+    for (int j = 0; j < TC; ++j) {
+        double cs = 0.0;
+#pragma unroll
+        for (int i = 0; i < TR; ++i) cs += fabs(a[i][j][0]) + (CPLX ? fabs(a[i][j][E - 1]) : 0.0);
+        for (int o = GY / 2; o > 0; o >>= 1) cs += __shfl_xor(cs, o, GY);
+        if (tx * TC + j < n) colmax = fmax(colmax, cs);
+    }
+    T* out = static_cast<T*>(S.inv) + static_cast<size_t>(w) * S.inv_stride;
+    for (int i = 0; i < b; ++i) out += E * M.blk_rows[i] * M.blk_n[i];
+#pragma unroll
+    for (int i = 0; i < TR; ++i) {
+        const int r = ty * TR + i;
+        if (r >= n) continue;
+        const int m = pinv[r];
+        if (m >= n) continue;   // cannot happen for a non-singular block
+#pragma unroll
+        for (int j = 0; j < TC; ++j) {
+            const int k = tx * TC + j;
+            if (k >= n || perm[k] >= n) continue;
+            const size_t at = (static_cast<size_t>(m) * n + perm[k]) * E;
+            out[at] = static_cast<T>(a[i][j][0]);
+            if (CPLX) out[at + 1] = static_cast<T>(a[i][j][E - 1]);
+        }
+    }
+    for (int idx = n * n + tid; idx < M.blk_rows[b] * n; idx += NT) {   // zero padding rows
+        out[E * idx] = T(0);
+        if (CPLX) out[E * idx + 1] = T(0);
+    }
+    double rowmax = 0.0;
+    if (tid < n) {
+        double rs = 0.0;
+        for (int q = 0; q < GX; ++q) rs += part[q][tid];
+        S.row_abs[(static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max + tid] = rs;
+        rowmax = rs;
+    }
+    colmax = wmax(colmax);
+    rowmax = wmax(rowmax);
+    if ((tid & (W - 1)) == 0) {
+        red[0][tid / W] = colmax;
+        red[1][tid / W] = rowmax;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int q = 1; q < NWAVES; ++q) {
+            colmax = fmax(colmax, red[0][q]);
+            rowmax = fmax(rowmax, red[1][q]);
+        }
+        S.cond[w * kMaxBlocks + b] = fresh_cond(rowmax * colmax);
+        S.updates[w * kMaxBlocks + b] = 0;
+        S.logs[w * 4 + 1 + b] = fmax(log_det, log(1e-300));
+    }
+}
+
 template <int CTRL>
 __device__ inline void pivot_combine(double& best, int& bi) {
     const double ob = dpp_mov<CTRL>(best);
@@ -1683,61 +1753,7 @@ __global__ __launch_bounds__(GX * 16) __attribute__((amdgpu_waves_per_eu(NP <= 6
         return;
     }
     __syncthreads();   // perm, pinv
-    // column sums of |A| (a permutation of M's columns' sums), max over the
-    // real columns; |re| + |im| as in the sweep's update
-    double colmax = 0.0;
-#pragma unroll
-    for (int j = 0; j < TC; ++j) {
-        double cs = 0.0;
-#pragma unroll
-        for (int i = 0; i < TR; ++i) cs += fabs(a[i][j][0]) + (CPLX ? fabs(a[i][j][E - 1]) : 0.0);
-        for (int o = GY / 2; o > 0; o >>= 1) cs += __shfl_xor(cs, o, GY);
-        if (tx * TC + j < n) colmax = fmax(colmax, cs);
-    }
-    T* out = static_cast<T*>(S.inv) + static_cast<size_t>(w) * S.inv_stride;
-    for (int i = 0; i < b; ++i) out += E * M.blk_rows[i] * M.blk_n[i];
-#pragma unroll
-    for (int i = 0; i < TR; ++i) {
-        const int r = ty * TR + i;
-        if (r >= n) continue;
-        const int m = pinv[r];
-        if (m >= n) continue;   // cannot happen for a non-singular block
-#pragma unroll
-        for (int j = 0; j < TC; ++j) {
-            const int k = tx * TC + j;
-            if (k >= n || perm[k] >= n) continue;
-            const size_t at = (static_cast<size_t>(m) * n + perm[k]) * E;
-            out[at] = static_cast<T>(a[i][j][0]);
-            if (CPLX) out[at + 1] = static_cast<T>(a[i][j][E - 1]);
-        }
-    }
-    for (int idx = n * n + tid; idx < M.blk_rows[b] * n; idx += NT) {   // zero padding rows
-        out[E * idx] = T(0);
-        if (CPLX) out[E * idx + 1] = T(0);
-    }
-    double rowmax = 0.0;
-    if (tid < n) {
-        double rs = 0.0;
-        for (int q = 0; q < GX; ++q) rs += part[q][tid];
-        S.row_abs[(static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max + tid] = rs;
-        rowmax = rs;
-    }
-    colmax = wmax(colmax);
-    rowmax = wmax(rowmax);
-    if ((tid & (W - 1)) == 0) {
-        red[0][tid / W] = colmax;
-        red[1][tid / W] = rowmax;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        for (int q = 1; q < NWAVES; ++q) {
-            colmax = fmax(colmax, red[0][q]);
-            rowmax = fmax(rowmax, red[1][q]);
-        }
-        S.cond[w * kMaxBlocks + b] = fresh_cond(rowmax * colmax);
-        S.updates[w * kMaxBlocks + b] = 0;
-        S.logs[w * 4 + 1 + b] = fmax(log_det, log(1e-300));
-    }
+    refresh_store<T, CPLX, NP, GX>(M, S, w, b, n, tid, a, part, perm, pinv, red, log_det);
 }
 
 // k_sweep's body for one walker w of a given system, for the fused launch

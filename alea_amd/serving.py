@@ -191,7 +191,8 @@ class Server:
     their own streams and overlap on the device."""
 
     def __init__(self, hbm_budget_bytes: int = 64 << 30, max_active: int = 8, runner=GpuRunner,
-                 clock=time.perf_counter, poll_interval: float = 1e-4, on_finish=None, fuse: bool = True):
+                 clock=time.perf_counter, poll_interval: float = 1e-4, on_finish=None, fuse: bool = True,
+                 on_block=None):
         self.fuse = fuse and hasattr(runner, "start_fused") and hasattr(runner, "bucket_key")
         self.budget = hbm_budget_bytes
         self.max_active = max_active
@@ -202,6 +203,7 @@ class Server:
         self.peak_active = 0
         self.peak_fused = 1     # most jobs in one fused launch
         self.on_finish = on_finish      # called with each finished job (progress reporting)
+        self.on_block = on_block        # called with a job after each of its blocks
 
     def run(self, jobs: list) -> dict:
         if self.fuse:
@@ -239,6 +241,8 @@ class Server:
                 j = r.job
                 j.block_latency.append(self.clock() - t0 - start)
                 j.block_energy.append(r.block_energy())
+                if self.on_block:
+                    self.on_block(j)
                 if len(j.block_energy) < j.n_blocks:
                     entry[2] = self.clock() - t0
                     r.start_block()
@@ -314,6 +318,8 @@ class Server:
                     j = r.job
                     j.block_latency.append(end - start)
                     j.block_energy.append(r.block_energy())
+                    if self.on_block:
+                        self.on_block(j)
                     if len(j.block_energy) >= j.n_blocks:
                         j.finished = self.clock() - t0
                         r.close()

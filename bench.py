@@ -162,13 +162,18 @@ SERVING = dict(n_jobs=16, rate_per_s=4.0, seed=4, n_blocks=4, sweeps_per_block=1
                dipole_fraction=0.2, dipoles=(256, 1.0))
 
 
-def serving_leg(precision=1):
-    """A bounded sample of the mixed-size request stream: Poisson arrivals of
-    VMC jobs (N, rs, twist, walker count drawn from the seed), continuously
-    batched on the GPU.  p50/p99 block and request latency, aggregate
-    walker-moves/s."""
+# the stream's own job size: 20 blocks of 50 sweeps per job
+SERVING_FULL = dict(SERVING, n_blocks=20, sweeps_per_block=50)
+
+
+def serving_leg(precision=1, full=False):
+    """The mixed-size request stream: Poisson arrivals of VMC jobs (N, rs,
+    twist, walker count drawn from the seed), continuously batched on the
+    GPU.  p50/p99 block and request latency, aggregate walker-moves/s.  The
+    default is a reduced stream (4 blocks of 10 sweeps per job) that fits
+    the bench's few minutes; full=True runs every job at 20 x 50."""
     from alea_amd import serving
-    c = SERVING
+    c = SERVING_FULL if full else SERVING
     jobs = serving.poisson_jobs(c["n_jobs"], c["rate_per_s"], seed=c["seed"], sizes=c["sizes"],
                                 rs_choices=c["rs"], walkers=c["walkers"], n_blocks=c["n_blocks"],
                                 sweeps_per_block=c["sweeps_per_block"], precision=precision,
@@ -176,8 +181,20 @@ def serving_leg(precision=1):
     def progress(j):
         print(f"bench: serving job {j.job_id} (N={j.n_particles}, {j.n_walkers} walkers) done at "
               f"{j.finished:.1f} s", file=sys.stderr, flush=True)
-    st = serving.Server(hbm_budget_bytes=96 << 30, max_active=8, on_finish=progress).run(jobs)
-    st["config"] = dict(c, note="each job 4 blocks x 10 sweeps (the full stream: 20 x 50)")
+    last = [time.perf_counter()]
+
+    def block_done(j):
+        # a line at most every 20 s while blocks finish (the full stream runs minutes)
+        if time.perf_counter() - last[0] > 20.0:
+            last[0] = time.perf_counter()
+            print(f"bench: serving job {j.job_id} block {len(j.block_energy)}/{j.n_blocks}", file=sys.stderr,
+                  flush=True)
+    st = serving.Server(hbm_budget_bytes=96 << 30, max_active=8, on_finish=progress,
+                        on_block=block_done).run(jobs)
+    st["config"] = dict(c, reduced=not full,
+                        note="each job 20 blocks x 50 sweeps" if full else
+                        "reduced stream: each job 4 blocks x 10 sweeps (the stream's jobs are 20 x 50: "
+                        "bench.py --serving-full)")
     return st
 
 
@@ -413,6 +430,8 @@ def main():
                     help="run a plain copy of BYTES bytes (counter calibration) and exit")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--serving-only", action="store_true", help="run the serving leg alone, print its dict")
+    ap.add_argument("--serving-full", action="store_true",
+                    help="run the serving stream at its full job size (20 blocks x 50 sweeps) alone, print its dict")
     ap.add_argument("--no-dmc", action="store_true", help="skip the DMC accuracy leg")
     ap.add_argument("--dmc-only", action="store_true", help="run the DMC leg alone, print its dict")
     ap.add_argument("--crystal-only", action="store_true",
@@ -427,8 +446,8 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
-    if args.serving_only:
-        print(json.dumps(serving_leg()), flush=True)
+    if args.serving_only or args.serving_full:
+        print(json.dumps(serving_leg(full=args.serving_full)), flush=True)
         return
     if args.dmc_only:
         print(json.dumps(dmc_leg()), flush=True)
