@@ -315,7 +315,7 @@ def dmc_leg(c=DMC_CFG):
 
 
 CRYSTAL_CFG = dict(rs=60.0, walkers=32768, steps=300, equilibration=60, tau=2.0, vmc_sweeps=20, seed=11,
-                   pos_seed=5, spread=0.15, generations=10.0, alpha=2.0)
+                   pos_seed=5, spread=0.15, generations=10.0, alpha=2.0, refresh_updates=32)
 
 
 def crystal_leg(c=CRYSTAL_CFG):
@@ -341,6 +341,9 @@ def crystal_leg(c=CRYSTAL_CFG):
         print(f"bench: crystal DMC N={n} {name}", file=sys.stderr, flush=True)
         w = gpu_walkers(spec, cap, prec)
         try:
+            # the config's inverse refresh every 32 moves: a block is rebuilt
+            # once it has taken 32 Sherman-Morrison updates
+            w.set_refresh_updates(c["refresh_updates"])
             w.set_positions(pos)
             w.evaluate()
             w.permute(np.arange(target))
