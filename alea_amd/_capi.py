@@ -68,6 +68,7 @@ SIGNATURES = {
     "vmc_sweeps_static": (C.c_int, [_V, C.c_int, C.c_double]),
     "draw_tape": (C.c_int, [_V, C.c_int, _P_D]),
     "set_refresh_updates": (C.c_int, [_V, C.c_int]),
+    "set_refresh_kernel": (C.c_int, [_V, C.c_int]),
     "sweep_kernel_ms": (C.c_int, [_V, _P_D, C.c_int, _P_I]),
     "add_gofr": (C.c_int, [_V, C.c_double, C.c_int]),
     "add_density": (C.c_int, [_V, C.c_double, C.c_int, C.c_int]),

@@ -27,7 +27,7 @@ int set_ziggurat(const uint64_t* k, const double* w, const double* f, double r);
 void launch_evaluate(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, size_t lds,
                      hipStream_t st);
 void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, int nb_max, bool mixed,
-                    bool cplx, int min_updates, hipStream_t st);
+                    bool cplx, int min_updates, hipStream_t st, bool mfma);
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
                   bool cplx, int rows_max, size_t lds, hipStream_t st);
 void launch_draw(const DevState& s, int np, int nw, double* out, int stride, hipStream_t st);
@@ -219,13 +219,16 @@ struct alea_system {
     // block every refresh_every sweeps.
     int refresh_updates = 64;
     int refresh_every = 0;
+    // blocks of 33 to 64 orbitals refresh on the matrix cores (k_refresh_mfma)
+    // unless this is cleared (alea_set_refresh_kernel)
+    bool refresh_mfma = true;
     long long since_refresh = 0;  // sweeps (or np accept calls) since the last full refresh
 
     // One captured block of device-stream sweeps (alea_set_sweep_graphs):
     // the key holds everything its launches bake in, compared bytewise; a
     // block whose key differs re-captures.
     struct SweepGraphKey {
-        int n_sweeps, nw, np, dmc, refresh_every, refresh_updates, rows_max, precision, cplx;
+        int n_sweeps, nw, np, dmc, refresh_every, refresh_updates, refresh_mfma, rows_max, precision, cplx;
         long long since_refresh;
         double tau;
         const double* dtape;
@@ -442,7 +445,7 @@ struct alea_system {
     void refresh(hipStream_t on = nullptr) {
         since_refresh = 0;
         if (host.n_blocks == 0) return;
-        launch_refresh(dmodel, st, nw, host.n_blocks, st.nb_max, mixed(), cplx, 0, on ? on : stream);
+        launch_refresh(dmodel, st, nw, host.n_blocks, st.nb_max, mixed(), cplx, 0, on ? on : stream, refresh_mfma);
         hip_check(hipGetLastError(), "launch refresh");
     }
     // After a sweep: the full refresh when refresh_every is due, else the
@@ -453,7 +456,8 @@ struct alea_system {
             return;
         }
         if (host.n_blocks == 0) return;
-        launch_refresh(dmodel, st, nw, host.n_blocks, st.nb_max, mixed(), cplx, refresh_updates, on ? on : stream);
+        launch_refresh(dmodel, st, nw, host.n_blocks, st.nb_max, mixed(), cplx, refresh_updates, on ? on : stream,
+                       refresh_mfma);
         hip_check(hipGetLastError(), "launch refresh");
     }
 
@@ -1179,7 +1183,7 @@ static void accept_range(alea_system* s, int p, int w0, int w1, const unsigned c
         s->refresh();
     } else if (s->host.n_blocks > 0) {
         launch_refresh(s->dmodel, st, n, s->host.n_blocks, s->st.nb_max, s->mixed(), s->cplx, s->refresh_updates,
-                       s->stream);
+                       s->stream, s->refresh_mfma);
         hip_check(hipGetLastError(), "launch refresh");
     }
     s->sync();
@@ -1267,6 +1271,7 @@ static void run_sweep_graph(alea_system* s, const SweepParams& P, int stride, in
     k.cplx = s->cplx ? 1 : 0;
     k.refresh_every = s->refresh_every;
     k.refresh_updates = s->refresh_updates;
+    k.refresh_mfma = s->refresh_mfma ? 1 : 0;
     k.since_refresh = s->since_refresh;
     k.tau = P.tau;
     k.dtape = s->dtape.p;
@@ -1470,7 +1475,7 @@ int alea_refresh_inverse(alea_system* s, int w_begin, int w_end) {
         const auto [w0, w1] = s->range(w_begin, w_end);
         if (s->host.n_blocks == 0 || w1 == w0) return;
         launch_refresh(s->dmodel, s->shifted(w0), w1 - w0, s->host.n_blocks, s->st.nb_max, s->mixed(), s->cplx, 0,
-                       s->stream);
+                       s->stream, s->refresh_mfma);
         hip_check(hipGetLastError(), "launch refresh");
         if (w0 == 0 && w1 == s->nw) s->since_refresh = 0;
         s->energies_valid = false;
@@ -1496,6 +1501,14 @@ int alea_set_refresh_interval(alea_system* s, int sweeps) {
         if (sweeps < 0) throw InvalidArg("refresh interval must not be negative");
         s->refresh_every = sweeps;
         s->since_refresh = 0;
+    });
+}
+
+int alea_set_refresh_kernel(alea_system* s, int kind) {
+    return guarded([&] {
+        Sys(s);
+        if (kind != 0 && kind != 1) throw InvalidArg("refresh kernel must be 0 (vector units) or 1 (matrix cores)");
+        s->refresh_mfma = kind == 1;
     });
 }
 

@@ -1756,6 +1756,339 @@ __global__ __launch_bounds__(GX * 16) __attribute__((amdgpu_waves_per_eu(NP <= 6
     refresh_store<T, CPLX, NP, GX>(M, S, w, b, n, tid, a, part, perm, pinv, red, log_det);
 }
 
+// ── the batched inverse recompute on the matrix cores ──────────────────────
+//
+// Blocks of up to 64 orbitals: the implicit-pivot Gauss-Jordan exchange of
+// k_refresh_reg, blocked by 16 columns.  Four waves hold the 64 x 64 tile as
+// four column panels in the FP64 MFMA accumulator layout.  For each panel in
+// turn its owner wave sweeps the panel's pivots with one row per lane (pivot
+// search by a wave max and a ballot, the pivot row through LDS, no workgroup
+// barrier), and the other three waves apply the panel's exchange to their
+// columns as one rank-16 product on v_mfma_f64_16x16x4f64:
+//     X <- X + P' Y,   P' = P - sum_s e_{p_s} e_s^T,   Y = rows p_s of X,
+// where P is the swept panel and p_s the pivot row of its step s; the -1 at
+// (p_s, s) removes the old pivot rows, which the exchange replaces.
+//
+// v_mfma_f64_16x16x4f64 lane layout (lane l, checked on gfx950): A[i][k] at
+// l = i + 16 k, B[k][j] at l = j + 16 k, D[i][j] at l = j + 16 (i % 4) in
+// register i / 4.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// Sum over the 16 lanes of one DPP row, left in every lane of the row.
+__device__ inline double row16_sum(double v) {
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0x141>(v);
+    v += dpp_mov<0x140>(v);
+    return v;
+}
+
+// Largest unsigned value over the wave, as a uniform scalar: DPP row
+// maxima, then the four rows' maxima.
+__device__ inline unsigned wave_max_u32(unsigned v) {
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0xB1, 0xF, 0xF, false)));
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x4E, 0xF, 0xF, false)));
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x141, 0xF, 0xF, false)));
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x140, 0xF, 0xF, false)));
+    const unsigned a = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 0));
+    const unsigned b = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 16));
+    const unsigned c = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 32));
+    const unsigned d = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 48));
+    return max(max(a, b), max(c, d));
+}
+
+// Partial pivoting over one wave (lane = row): the first row, among those
+// not yet pivots, of the largest |z|^2 (|z|^2 orders as |z| does, without a
+// sqrt), as a uniform lane index.  Two integer maxima over the IEEE bits of
+// |z|^2 (non-negative, so they order as the values): the high words, then,
+// among the rows that hold the largest high word, the low words with the
+// row in their last six bits (ties within 2^-46 go to the first row).
+__device__ inline int pivot_row(double re, double im, unsigned long long used, int lane) {
+    const double val = re * re + im * im;
+    const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(val));
+    const bool free_row = !((used >> lane) & 1ull);
+    const unsigned hi = free_row ? (static_cast<unsigned>(bits >> 32) | 0x80000000u) : 0u;
+    const unsigned top = wave_max_u32(hi);
+    const unsigned lo = hi == top ? ((static_cast<unsigned>(bits) & ~63u) | static_cast<unsigned>(63 - lane)) : 0u;
+    return 63 - static_cast<int>(wave_max_u32(lo) & 63u);
+}
+
+// Orders the LDS accesses of one wave (no other wave takes part).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T, bool CPLX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_refresh_mfma(const DevModel* __restrict__ Mp, DevState S, int min_updates) {
+    constexpr int NP = 64, PB = 8, E = CPLX ? 2 : 1, NXI = CPLX ? 4 : 1;
+    const DevModel& M = *Mp;
+    // rows padded by one entry: a lane per row then reads across the banks
+    __shared__ double pan[NP][17][E];       // the panel owner's 16-column tile, row-major
+    __shared__ double ybuf[PB][NP + 1][E];  // the panel's pivot rows (before its exchange), every column
+    __shared__ double part[4][NP];       // |D| row sums, per wave
+    __shared__ double red[2][4];
+    __shared__ double det_sh[2];         // log|det| as running mantissa and binary exponent
+    __shared__ int perm[NP], pinv[NP], piv[PB], flag;
+    const int nbk = M.n_blocks;
+    const int w = blockIdx.x / nbk, b = blockIdx.x % nbk;
+    // selective refresh: the whole workgroup reads the same count and leaves together
+    if (min_updates > 0 && S.updates[w * kMaxBlocks + b] < min_updates) return;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & (W - 1), j = lane & 15, g = lane >> 4;
+    const int n = M.blk_n[b], first = M.blk_first[b];
+    const int col = 16 * wave + j;       // this lane's column of the tile
+    const double* pos = S.pos + static_cast<size_t>(w) * M.np * 2;
+
+    // D (identity past n): register v of row tile rt holds row 16 rt + 4 v + g
+    dbl4 xr[4], xi[NXI];
+    double av[4][4];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = 16 * rt + 4 * v + g;
+            double vr = r == col ? 1.0 : 0.0, vi = 0.0, a = 0.0;
+            if (r < n && col < n) {
+                Orb o;
+                orbital(M, b, col, pos[2 * (first + r)], pos[2 * (first + r) + 1], o);
+                vr = o.vr;
+                vi = o.vi;
+                a = CPLX ? sqrt(vr * vr + vi * vi) : fabs(vr);
+            }
+            xr[rt][v] = vr;
+            if constexpr (CPLX) xi[rt][v] = vi;
+            av[rt][v] = a;
+        }
+    }
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) av[rt][v] = row16_sum(av[rt][v]);
+    if (j == 0) {
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) part[wave][16 * rt + 4 * v + g] = av[rt][v];
+    }
+    if (tid == 0) {
+        det_sh[0] = 1.0;
+        det_sh[1] = 0.0;
+        flag = 0;
+    }
+    __syncthreads();
+
+    unsigned long long used = 0ull;   // rows already pivots (uniform)
+    const int n_panels = (n + PB - 1) / PB;
+    for (int t = 0; t < n_panels; ++t) {
+        const int owner = t / (16 / PB), c0 = PB * (t % (16 / PB));   // the panel's wave and its first tile column
+        if (wave == owner) {
+            // the whole tile to LDS (its registers are dead during the
+            // steps), then one row of the panel per lane
+#pragma unroll
+            for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = 16 * rt + 4 * v + g;
+                    pan[r][j][0] = xr[rt][v];
+                    if constexpr (CPLX) pan[r][j][E - 1] = xi[rt][v];
+                }
+            wave_sync();
+            double pr[PB][E];   // row `lane` of the panel
+#pragma unroll
+            for (int c = 0; c < PB; ++c)
+#pragma unroll
+                for (int q = 0; q < E; ++q) pr[c][q] = pan[lane][c0 + c][q];
+            double det_m = det_sh[0];
+            int det_e = static_cast<int>(det_sh[1]);
+            bool singular = false;
+            // PB steps in every panel, past n too: the padding is the
+            // identity, whose pivots are exact ones that change nothing, and
+            // no real row is free by then.  The steps are pipelined: a step
+            // updates the next pivot column first and starts the next pivot
+            // search, which overlaps its updates of the other columns.  The
+            // pivot row reaches the lanes by readlane (uniform p), so a step
+            // has no LDS round trip and no branch.
+            int p = pivot_row(pr[0][0], CPLX ? pr[0][E - 1] : 0.0, used, lane);
+#pragma unroll
+            for (int s = 0; s < PB; ++s) {
+                const double re = pr[s][0], im = CPLX ? pr[s][E - 1] : 0.0;
+                const double dr = lane_value(re, p), di = CPLX ? lane_value(im, p) : 0.0;
+                const double d2 = CPLX ? dr * dr + di * di : fabs(dr);
+                singular = singular || !(d2 > 0.0) || !isfinite(d2);
+                {
+                    int e;
+                    det_m = frexp(det_m * d2, &e);
+                    det_e += e;
+                }
+                double ir, ii;
+                if (CPLX) {
+                    const double inv_d2 = rcp_refined(d2);
+                    ir = dr * inv_d2;
+                    ii = -di * inv_d2;
+                } else {
+                    ir = rcp_refined(dr);
+                    ii = 0.0;
+                }
+                // Every column c != s takes x - h y_c, y = the pivot row:
+                // h = x_s / d off the pivot row gives the elimination, and
+                // h = 1 - 1/d on it (where x = y) gives y / d; column s
+                // becomes -h off the pivot row and 1/d on it.
+                const bool is_p = lane == p;
+                const double hr = is_p ? 1.0 - ir : (CPLX ? re * ir - im * ii : re * ir);
+                const double hi = CPLX ? (is_p ? -ii : re * ii + im * ir) : 0.0;
+                auto update = [&](int c) {
+                    if (c == s) {
+                        pr[c][0] = is_p ? ir : -hr;
+                        if constexpr (CPLX) pr[c][E - 1] = is_p ? ii : -hi;
+                        return;
+                    }
+                    const double yr = lane_value(pr[c][0], p);
+                    double xr_ = fmadd(-hr, yr, pr[c][0]);
+                    if constexpr (CPLX) {
+                        const double yi = lane_value(pr[c][E - 1], p);
+                        xr_ = fmadd(hi, yi, xr_);
+                        double xi_ = fmadd(-hr, yi, pr[c][E - 1]);
+                        xi_ = fmadd(-hi, yr, xi_);
+                        pr[c][E - 1] = xi_;
+                    }
+                    pr[c][0] = xr_;
+                };
+                used |= 1ull << p;
+                if (lane == 0) {
+                    piv[s] = p;
+                    perm[PB * t + s] = p;
+                    pinv[p] = PB * t + s;
+                }
+                int p_next = 0;
+                if (s + 1 < PB) {
+                    update(s + 1);
+                    p_next = pivot_row(pr[s + 1][0], CPLX ? pr[s + 1][E - 1] : 0.0, used, lane);
+                }
+#pragma unroll
+                for (int c = 0; c < PB; ++c)
+                    if (c != s + 1) update(c);
+                p = p_next;
+                __builtin_amdgcn_sched_barrier(0);   // one step at a time in registers
+            }
+#pragma unroll
+            for (int c = 0; c < PB; ++c)
+#pragma unroll
+                for (int q = 0; q < E; ++q) pan[lane][c0 + c][q] = pr[c][q];
+            wave_sync();
+            // the tile back, with the swept panel in it
+#pragma unroll
+            for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = 16 * rt + 4 * v + g;
+                    xr[rt][v] = pan[r][j][0];
+                    if constexpr (CPLX) xi[rt][v] = pan[r][j][E - 1];
+                }
+            if (lane == 0) {
+                det_sh[0] = det_m;
+                det_sh[1] = static_cast<double>(det_e);
+                if (singular) flag = 1;
+            }
+        }
+        __syncthreads();
+        if (flag) break;   // uniform: read after the barrier
+        unsigned long long pm = 0ull;
+        for (int s = 0; s < PB; ++s) pm |= 1ull << piv[s];
+        used |= pm;
+        // Y: this wave's columns of the pivot rows, before the exchange (the
+        // owner's panel columns, already final, take no update: B = 0 there)
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 16 * rt + 4 * v + g;
+                if ((pm >> r) & 1ull) {
+                    const int s = pinv[r] - PB * t;
+                    ybuf[s][col][0] = xr[rt][v];
+                    if constexpr (CPLX) ybuf[s][col][E - 1] = xi[rt][v];
+                }
+            }
+        __syncthreads();
+        {
+            // X += P' Y over the panel's PB pivots (K = PB in steps of 4)
+            const bool own = wave == owner && (j - c0) >= 0 && (j - c0) < PB;
+#pragma unroll
+            for (int ks = 0; ks < PB / 4; ++ks) {
+                const int kk = 4 * ks + g;   // this lane's k in both operands
+                const double br = own ? 0.0 : ybuf[kk][col][0];
+                const double bi = (CPLX && !own) ? ybuf[kk][col][E - 1] : 0.0;
+                const int pk = piv[kk];
+#pragma unroll
+                for (int rt = 0; rt < 4; ++rt) {
+                    const int ra = 16 * rt + j;   // this lane's row of A
+                    double ar = pan[ra][c0 + kk][0];
+                    const double ai = CPLX ? pan[ra][c0 + kk][E - 1] : 0.0;
+                    if (ra == pk) ar -= 1.0;
+                    xr[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, xr[rt], 0, 0, 0);
+                    if constexpr (CPLX) {
+                        xr[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, xr[rt], 0, 0, 0);
+                        xi[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, xi[rt], 0, 0, 0);
+                        xi[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, xi[rt], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        __syncthreads();   // pan, ybuf and piv are rewritten by the next panel
+    }
+    if (flag) {
+        if (tid == 0) S.status[w] = 1;
+        return;
+    }
+    const double log_det = (CPLX ? 0.5 : 1.0) * (log(det_sh[0]) + det_sh[1] * 0.69314718055994530942);
+
+    // the inverse: A[pinv[r]][perm[c]] = X[r][c], in the sweep's precision
+    T* out = static_cast<T*>(S.inv) + static_cast<size_t>(w) * S.inv_stride;
+    for (int i = 0; i < b; ++i) out += E * M.blk_rows[i] * M.blk_n[i];
+    double cs = 0.0;   // column sums of |X| (a permutation of A's), |re| + |im| as in the sweep
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = 16 * rt + 4 * v + g;
+            cs += fabs(xr[rt][v]) + (CPLX ? fabs(xi[rt][v]) : 0.0);
+            if (r < n && col < n) {
+                const size_t at = (static_cast<size_t>(pinv[r]) * n + perm[col]) * E;
+                out[at] = static_cast<T>(xr[rt][v]);
+                if constexpr (CPLX) out[at + 1] = static_cast<T>(xi[rt][v]);
+            }
+        }
+    cs += __shfl_xor(cs, 16, W);
+    cs += __shfl_xor(cs, 32, W);
+    double colmax = col < n ? cs : 0.0;
+    for (int idx = n * n + tid; idx < M.blk_rows[b] * n; idx += 256) {   // zero padding rows
+        out[E * idx] = T(0);
+        if (CPLX) out[E * idx + 1] = T(0);
+    }
+    double rowmax = 0.0;
+    if (tid < n) {
+        const double rs = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+        S.row_abs[(static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max + tid] = rs;
+        rowmax = rs;
+    }
+    colmax = wmax(colmax);
+    rowmax = wmax(rowmax);
+    if (lane == 0) {
+        red[0][wave] = colmax;
+        red[1][wave] = rowmax;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int q = 1; q < 4; ++q) {
+            colmax = fmax(colmax, red[0][q]);
+            rowmax = fmax(rowmax, red[1][q]);
+        }
+        S.cond[w * kMaxBlocks + b] = fresh_cond(rowmax * colmax);
+        S.updates[w * kMaxBlocks + b] = 0;
+        S.logs[w * 4 + 1 + b] = fmax(log_det, log(1e-300));
+    }
+}
+
 // k_sweep's body for one walker w of a given system, for the fused launch
 // below.  k_sweep keeps its own copy: compiled through this function its
 // register allocation changes (more spills in the 64-row tile).
@@ -2215,11 +2548,24 @@ static void launch_refresh_reg(const DevModel* m, const DevState& s, int nw, int
 }
 
 // Blocks of up to 128 orbitals (the most the build accepts) refresh in
-// registers: 16 x 16 threads up to 64 orbitals, 32 x 16 up to 128.
-// min_updates > 0 rebuilds only the blocks whose update count reached it.
+// registers: 16 x 16 threads up to 64 orbitals, 32 x 16 up to 128.  With
+// mfma, blocks of 33 to 64 orbitals take the blocked exchange on the matrix
+// cores (k_refresh_mfma) instead.  min_updates > 0 rebuilds only the blocks
+// whose update count reached it.
 void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, int nb_max, bool mixed,
-                    bool cplx, int min_updates, hipStream_t st) {
+                    bool cplx, int min_updates, hipStream_t st, bool mfma) {
     if (nw == 0 || n_blocks == 0) return;
+    if (mfma && nb_max > 32 && nb_max <= 64) {
+        const dim3 g(nw * n_blocks), t(256);
+        if (mixed) {
+            if (cplx) hipLaunchKernelGGL((k_refresh_mfma<float, true>), g, t, 0, st, m, s, min_updates);
+            else hipLaunchKernelGGL((k_refresh_mfma<float, false>), g, t, 0, st, m, s, min_updates);
+        } else {
+            if (cplx) hipLaunchKernelGGL((k_refresh_mfma<double, true>), g, t, 0, st, m, s, min_updates);
+            else hipLaunchKernelGGL((k_refresh_mfma<double, false>), g, t, 0, st, m, s, min_updates);
+        }
+        return;
+    }
     if (nb_max <= 16)
         launch_refresh_reg<16, 16>(m, s, nw, n_blocks, mixed, cplx, min_updates, st);
     else if (nb_max <= 32)

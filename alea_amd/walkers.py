@@ -188,6 +188,10 @@ class Walkers:
     def set_refresh_updates(self, updates):
         self.api("set_refresh_updates", self.h, int(updates))
 
+    def set_refresh_kernel(self, kind):
+        """1: blocks of 33-64 orbitals rebuild on the matrix cores; 0: vector units."""
+        self.api("set_refresh_kernel", self.h, int(kind))
+
     def rebuild_counts(self, reset=True):
         """Rebuild triggers since the last reset: (counts[4], per-walker totals)."""
         c = (C.c_longlong * 4)()

@@ -191,6 +191,14 @@ int alea_dmc_sweeps(alea_system* s, int n_sweeps, double tau, const double* tape
  * cancellation guard) still rebuild inside the sweep. */
 int alea_set_refresh_updates(alea_system* s, int updates);
 
+/* Which kernel recomputes the inverses (the batched rebuild of
+ * SlaterDeterminant::rebuild_inverse, slater_determinant.cpp:864-1060, the
+ * same pivoted Gauss-Jordan either way): 1 (the default) runs blocks of 33 to
+ * 64 orbitals on the FP64 matrix cores, as a Gauss-Jordan exchange blocked by
+ * 16 columns whose rank-16 updates are MFMA products; 0 keeps every block on
+ * the vector units.  Other block sizes always take the vector units. */
+int alea_set_refresh_kernel(alea_system* s, int kind);
+
 /* sweeps > 0 also rebuilds every block of every walker after every `sweeps`
  * sweeps (0, the default: only the update-count rule above). */
 int alea_set_refresh_interval(alea_system* s, int sweeps);

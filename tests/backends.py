@@ -44,7 +44,8 @@ def oracle_lib():
         if not os.path.exists(ORACLE_SO):
             pytest.fail("oracle/liboracle.so is not built: run __graft_entry__.build()")
         lib = C.CDLL(ORACLE_SO)
-        _oracle = (lib, CApi(lib, "oracle_", optional=("rebuild_counts", "set_cancellation_factor", "set_refresh_updates", "sweep_kernel_ms",
+        _oracle = (lib, CApi(lib, "oracle_", optional=("rebuild_counts", "set_cancellation_factor", "set_refresh_updates", "set_refresh_kernel",
+                                                      "sweep_kernel_ms",
                                                       "vmc_sweeps_static", "draw_tape")))
         for name, res, args in [
             ("oracle_stream_seed", C.c_uint64, [C.c_uint64] * 3),

@@ -39,7 +39,7 @@ def test_oracle_exports_the_same_entry_points():
     missing = [n for n in declared() if n not in ("alea_seed", "alea_synchronize", "alea_last_sweep_ms",
                                                  "alea_set_refresh_interval", "alea_set_sweep_graphs", "alea_vmc_sweeps_fused",
                                                  "alea_query", "alea_rebuild_counts", "alea_set_cancellation_factor",
-                                                 "alea_set_refresh_updates", "alea_sweep_kernel_ms", "alea_vmc_sweeps_static",
+                                                 "alea_set_refresh_updates", "alea_set_refresh_kernel", "alea_sweep_kernel_ms", "alea_vmc_sweeps_static",
                                                  "alea_draw_tape")
                and not hasattr(lib, n.replace("alea_", "oracle_", 1))]
     assert not missing, missing
