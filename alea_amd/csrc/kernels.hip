@@ -674,7 +674,9 @@ struct Walker {
         double og[2], ng[2], nl[2];
     };
     // Lane partial sums of the pair and k-space parts (reduced by the caller).
-    template <bool derivs>
+    // old_only: just the gradient at the current position (acc[1], acc[2]),
+    // for a drift move's old gradient; nx is then ignored.
+    template <bool derivs, bool old_only = false>
     __device__ void jastrow_lanes(int p, double x0, double x1, double nx0, double nx1, double (&acc)[7]) {
         const int gp = M.group_of[p];
         for (int i = 0; i < 7; ++i) acc[i] = 0.0;   // dlog, og0, og1, ng0, ng1, nl0, nl1
@@ -689,6 +691,18 @@ struct Walker {
             const double xk0 = sh.pos[2 * k], xk1 = sh.pos[2 * k + 1];
             double o0 = x0 - xk0, o1 = x1 - xk1;
             const double ro = dist(M.cell, o0, o1);
+            if constexpr (old_only) {
+                for (int fi = 0; fi < nf; ++fi) {
+                    const int f = M.pair_list[gp][h][fi];
+                    double uo, duo, d2uo;
+                    radial_eval(M.pair[f].rad, ro, uo, duo, d2uo, sh.spl + f * M.spl_stride);
+                    double g0, g1, l0, l1;
+                    pair_terms(duo, d2uo, ro, o0, o1, g0, g1, l0, l1);
+                    acc[1] += g0;
+                    acc[2] += g1;
+                }
+                continue;
+            }
             double n0 = nx0 - xk0, n1 = nx1 - xk1;
             const double rn = dist(M.cell, n0, n1);
             for (int fi = 0; fi < nf; ++fi) {
@@ -719,6 +733,19 @@ struct Walker {
                 const double kx = sh.kj[2 * k], ky = sh.kj[2 * k + 1];
                 double so, co, sn, cn;
                 sincos_bounded(kx * x0 + ky * x1, &so, &co);
+                if constexpr (old_only) {
+                    double gold = 0.0;
+                    for (int h = 0; h < M.ng; ++h) {
+                        if (!M.kj_active[h]) continue;
+                        const double wgh = sh.kj[2 * M.nkj + (gp * M.ng + h) * M.nkj + k];
+                        if (wgh == 0.0) continue;
+                        const double rr = sh.rho[(h * M.nkj + k) * 2], ri = sh.rho[(h * M.nkj + k) * 2 + 1];
+                        gold += 2.0 * wgh * (co * ri - so * rr);
+                    }
+                    acc[1] += kx * gold;
+                    acc[2] += ky * gold;
+                    continue;
+                }
                 sincos_bounded(kx * nx0 + ky * nx1, &sn, &cn);
                 const double dr = cn - co, di = sn - so;
                 kdr = dr;   // kept for jastrow_commit of this move
@@ -782,15 +809,6 @@ struct Walker {
         out.ng[0] = acc[3]; out.ng[1] = acc[4];
         out.nl[0] = acc[5]; out.nl[1] = acc[6];
     }
-    template <bool derivs>
-    __device__ void jastrow_move(int p, double x0, double x1, double nx0, double nx1, JOut& out) {
-        double acc[7];
-        jastrow_lanes<derivs>(p, x0, x1, nx0, nx1, acc);
-        if constexpr (derivs) wsum_n(acc);
-        else acc[0] = wsum(acc[0]);
-        jastrow_onebody(p, x0, x1, nx0, nx1, acc);
-        jastrow_out(acc, out);
-    }
 
     // Commit the k-space densities of the accepted move of p, with the phase
     // change jastrow_lanes computed for it (one k per lane).
@@ -805,12 +823,10 @@ struct Walker {
     // ── determinant: ratio of moving p (column j of block b) ───────────────
     struct DOut {
         double R[2];       // ratio (complex)
-        double scale;      // sum |A_mj v_m|
         double og[2][2];   // old gradient of p, complex per dim: [d][re/im]
         double ol[2][2];
         double ng[2][2];   // Sum_m A_mj grad phi_m(r') (before the 1/R)
         double nl[2][2];
-        double rowabs;     // sum_m |v_m|
     };
     // Lane partial sums of the ratio of moving p (column j of block b), the
     // proposed orbital values v' (sh.v) and column j (sh.u) in LDS; reduced
@@ -849,6 +865,17 @@ struct Walker {
             if constexpr (NR > 0) a = reinterpret_cast<const C2<T>*>(sh.u)[m];
             else a = A.get(m, j);
             const double ar = static_cast<double>(a.re), ai = static_cast<double>(a.im);
+            if constexpr (!derivs && old_derivs) {
+                // the old gradient alone (a drift move's, r' = r): one
+                // orbital evaluation, Sum_m A_mj grad phi_m(r) in acc[11..14]
+                Orb oo;
+                orbital(M, b, m, x0, x1, oo);
+                for (int d = 0; d < 2; ++d) {
+                    acc[11 + 2 * d] += ar * oo.gr[d] - ai * oo.gi[d];
+                    acc[12 + 2 * d] += ar * oo.gi[d] + ai * oo.gr[d];
+                }
+                continue;
+            }
             Orb on;
             T vr, vi;   // values in the determinant precision
             if constexpr (sizeof(T) == 4 && !derivs && !old_derivs) {
@@ -867,12 +894,8 @@ struct Walker {
             const double ti = ar * static_cast<double>(vi) + ai * static_cast<double>(vr);
             acc[0] += tr;
             acc[1] += ti;
-            // the two magnitude sums only feed the rebuild thresholds (the
-            // cancellation bound, the condition estimate): a single-precision
-            // square root is accurate enough for both
-            acc[2] += CPLX ? static_cast<double>(__builtin_sqrtf(static_cast<float>(tr * tr + ti * ti))) : fabs(tr);
-            acc[19] += CPLX ? static_cast<double>(__builtin_sqrtf(static_cast<float>(on.vr * on.vr + on.vi * on.vi)))
-                            : fabs(static_cast<double>(vr));
+            // (the magnitude sums behind the rebuild thresholds are taken in
+            // det_commit, for accepted moves only)
             if constexpr (derivs) {
                 for (int d = 0; d < 2; ++d) {
                     acc[3 + 2 * d] += ar * on.gr[d] - ai * on.gi[d];
@@ -896,28 +919,12 @@ struct Walker {
     static __device__ void det_out(const double (&acc)[20], DOut& o) {
         o.R[0] = acc[0];
         o.R[1] = acc[1];
-        o.scale = acc[2];
-        o.rowabs = acc[19];
         for (int d = 0; d < 2; ++d) {
             o.ng[d][0] = acc[3 + 2 * d]; o.ng[d][1] = acc[4 + 2 * d];
             o.nl[d][0] = acc[7 + 2 * d]; o.nl[d][1] = acc[8 + 2 * d];
             o.og[d][0] = acc[11 + 2 * d]; o.og[d][1] = acc[12 + 2 * d];
             o.ol[d][0] = acc[15 + 2 * d]; o.ol[d][1] = acc[16 + 2 * d];
         }
-    }
-    template <bool derivs, bool old_derivs>
-    __device__ void det_ratio(int b, int j, double x0, double x1, double nx0, double nx1, DOut& o) {
-        double acc[20];
-        det_lanes<derivs, old_derivs>(b, j, x0, x1, nx0, nx1, acc);
-        if constexpr (derivs || old_derivs) {
-            wsum_n(acc);
-        } else {
-            double r[4] = {acc[0], acc[1], acc[2], acc[19]};
-            wsum_n(r);
-            acc[0] = r[0]; acc[1] = r[1]; acc[2] = r[2]; acc[19] = r[3];
-        }
-        det_out(acc, o);
-        __syncthreads();
     }
 
     // Pivoted Gauss-Jordan rebuild of block b from the current positions
@@ -1073,6 +1080,21 @@ struct Walker {
         const InvRef<T, CPLX> A = inv(b);
         const int n = M.blk_n[b];
         const double Rr = d.R[0], Ri = d.R[1];
+        // The two magnitude sums behind the rebuild thresholds, from column j
+        // (sh.u) and v' (sh.v) as the ratio used them: the cancellation scale
+        // Sum_m |A_mj v'_m| and the row sum Sum_m |v'_m| of D.  They feed
+        // thresholds only, so a single-precision square root is enough.
+        double mags[2] = {0.0, 0.0};
+        for (int m = lane; m < n; m += W) {
+            const C2<T> a = reinterpret_cast<const C2<T>*>(sh.u)[m];
+            const double ar = static_cast<double>(a.re), ai = static_cast<double>(a.im);
+            const double vr = sh.v[2 * m], vi = sh.v[2 * m + 1];
+            const double tr = ar * vr - ai * vi, ti = ar * vi + ai * vr;
+            mags[0] += CPLX ? static_cast<double>(__builtin_sqrtf(static_cast<float>(tr * tr + ti * ti))) : fabs(tr);
+            mags[1] += CPLX ? static_cast<double>(__builtin_sqrtf(static_cast<float>(vr * vr + vi * vi))) : fabs(vr);
+        }
+        wsum_n(mags);
+        const double scale = mags[0], rowabs = mags[1];
         const double absR = CPLX ? sqrt(Rr * Rr + Ri * Ri) : fabs(Rr);
         const double eps = sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
         int* upd = S.updates + w * kMaxBlocks + b;
@@ -1094,7 +1116,7 @@ struct Walker {
         int cause = -1;   // RebuildCause, uniform across the wave
         bool latch_now = false;
         if (!isfinite(absR) || absR < 1e-5) cause = kRebuildTinyRatio;
-        else if (absR <= M.cancel_factor * eps * d.scale) cause = kRebuildCancellation;
+        else if (absR <= M.cancel_factor * eps * scale) cause = kRebuildCancellation;
         else if (!latched && cond_old > 1e8) latch_now = true;
         bool rebuild_now = cause >= 0;
         if (latch_now && lane == 0) S.rebuilds[w * kRebuildCauses + kRebuildCondBefore] += 1;
@@ -1210,7 +1232,7 @@ struct Walker {
                 colmax = fmax(colmax, static_cast<double>(cs));
             }
             colmax = wmax(colmax);
-            if (lane == 0) rabs[j] = d.rowabs;
+            if (lane == 0) rabs[j] = rowabs;
             __syncthreads();
             double rowmax = 0.0;
             for (int kr = lane; kr < n; kr += W) rowmax = fmax(rowmax, rabs[kr]);
@@ -1285,18 +1307,21 @@ struct Walker {
             for (int i = 0; i < 20; ++i) da[i] = 0.0;
         }
         if constexpr (derivs) {
-            double r[19];
+            // Jastrow dlog, gradients and Laplacians; R; Sum A grad/lap phi(r')
+            double r[17];
             for (int i = 0; i < 7; ++i) r[i] = ja[i];
-            for (int i = 0; i < 11; ++i) r[7 + i] = da[i];
-            r[18] = da[19];
+            r[7] = da[0];
+            r[8] = da[1];
+            for (int i = 0; i < 8; ++i) r[9 + i] = da[3 + i];
             wsum_n(r);
             for (int i = 0; i < 7; ++i) ja[i] = r[i];
-            for (int i = 0; i < 11; ++i) da[i] = r[7 + i];
-            da[19] = r[18];
+            da[0] = r[7];
+            da[1] = r[8];
+            for (int i = 0; i < 8; ++i) da[3 + i] = r[9 + i];
         } else {
-            double r[5] = {ja[0], da[0], da[1], da[2], da[19]};
+            double r[3] = {ja[0], da[0], da[1]};
             wsum_n(r);
-            ja[0] = r[0]; da[0] = r[1]; da[1] = r[2]; da[2] = r[3]; da[19] = r[4];
+            ja[0] = r[0]; da[0] = r[1]; da[1] = r[2];
         }
         jastrow_onebody(p, x0, x1, nx0, nx1, ja);
         JOut J;
@@ -1408,20 +1433,26 @@ struct Walker {
         return true;
     }
 
-    // Old gradient of p (DMC drift), all factors.
+    // Old gradient of p (the drift of DMC and Langevin moves), all factors:
+    // only the sums it needs are evaluated and reduced.
     __device__ void grad_of(int p, double g[2]) {
         const double x0 = sh.pos[2 * p], x1 = sh.pos[2 * p + 1];
-        JOut J;
-        jastrow_move<true>(p, x0, x1, x0, x1, J);
-        g[0] = J.og[0];
-        g[1] = J.og[1];
+        double ja[7];
+        jastrow_lanes<true, true>(p, x0, x1, x0, x1, ja);
+        double r[4] = {ja[1], ja[2], 0.0, 0.0};
         const int b = M.block_of[p];
         if (b >= 0) {
-            DOut D{};
-            det_ratio<false, true>(b, p - M.blk_first[b], x0, x1, x0, x1, D);
-            // at r' = r the ratio is 1 up to rounding; G = sum A grad phi (old)
-            for (int d = 0; d < 2; ++d) g[d] += D.og[d][0];
+            double da[20];
+            det_lanes<false, true>(b, p - M.blk_first[b], x0, x1, x0, x1, da);
+            r[2] = da[11];   // Re Sum_m A_mj d_x phi_m(r): at r' = r the ratio is 1
+            r[3] = da[13];
         }
+        wsum_n(r);
+        double acc[7] = {0.0, r[0], r[1], 0.0, 0.0, 0.0, 0.0};
+        jastrow_onebody(p, x0, x1, x0, x1, acc);
+        g[0] = acc[1] + r[2];
+        g[1] = acc[2] + r[3];
+        __syncthreads();
     }
 };
 
