@@ -162,9 +162,27 @@ __device__ inline void wrap(const DevCell& c, double& x0, double& x1) {
     x1 -= s1;
 }
 
+// sqrt of x >= 0 for a normal or zero x: the hardware reciprocal square
+// root refined by Goldschmidt's iteration (three FMA corrections), the
+// sequence of the library's correctly rounded sqrt without its rescaling of
+// inputs below 2^-767 (an interparticle r^2 never is one), so equal to it
+// bit for bit on every normal input.
+__device__ inline double sqrt_normal(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return x > 0.0 ? g : 0.0;
+}
+
 __device__ inline double dist(const DevCell& c, double& d0, double& d1) {
     min_image(c, d0, d1);
-    return sqrt(d0 * d0 + d1 * d1);
+    return sqrt_normal(d0 * d0 + d1 * d1);
 }
 
 // ── radial functions ───────────────────────────────────────────────────────
