@@ -69,6 +69,15 @@ __device__ inline double lane_value(double v, int l) {
     return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
                             __builtin_amdgcn_readlane(__double2loint(v), l));
 }
+// A row broadcast (row_bcast:15 into rows 1 and 3, row_bcast:31 into rows 2
+// and 3); the rows it does not write hold no defined value, and only lane
+// 63's sum is read afterwards.
+template <int CTRL, int ROWS>
+__device__ inline double dpp_bcast(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, ROWS, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
 template <int N>
 __device__ inline void wsum_n(double (&v)[N]) {
 #pragma unroll
@@ -79,9 +88,14 @@ __device__ inline void wsum_n(double (&v)[N]) {
     for (int i = 0; i < N; ++i) v[i] += dpp_mov<0x141>(v[i]);
 #pragma unroll
     for (int i = 0; i < N; ++i) v[i] += dpp_mov<0x140>(v[i]);
+    // rows r0..r3 -> lane 63 holds (r3 + r2) + (r1 + r0), which IEEE
+    // addition makes equal to (r0 + r1) + (r2 + r3) bit for bit
 #pragma unroll
-    for (int i = 0; i < N; ++i)
-        v[i] = (lane_value(v[i], 0) + lane_value(v[i], 16)) + (lane_value(v[i], 32) + lane_value(v[i], 48));
+    for (int i = 0; i < N; ++i) v[i] += dpp_bcast<0x142, 0xA>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += dpp_bcast<0x143, 0xC>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = lane_value(v[i], 63);
 }
 __device__ inline double wsum(double v) {
     double a[1] = {v};
@@ -1397,8 +1411,13 @@ struct Walker {
             acc = call->mask[w] != 0;
         } else if (kind == kSweepVmc) {
             // uni < |psi'/psi|^2 = exp(2 lr), with the determinant's share as
-            // |R|^2 rather than through its log
-            acc = uni < exp(2.0 * J.dlog) * R2;
+            // |R|^2 rather than through its log; the mixed form, whose ratio
+            // is single precision, takes the exponential in single precision
+            // (its argument bounded to the float range)
+            if constexpr (sizeof(T) == 4)
+                acc = uni < static_cast<double>(expf(static_cast<float>(fmin(fmax(2.0 * J.dlog, -87.0), 88.0)))) * R2;
+            else
+                acc = uni < exp(2.0 * J.dlog) * R2;
         } else if (kind == kSweepLangevin) {
             // VMC with the bare drift (tau/m) g: no node gate, the Green's
             // function log-ratio per dimension -delta (g + g') + (tau/m)/2
