@@ -46,6 +46,7 @@ SIGNATURES = {
     "accept": (C.c_int, [_V, C.c_int, _P_U8]),
     "ratio_range": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_D, _P_D, _P_D, _P_D, _P_D]),
     "accept_range": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_U8]),
+    "delta_potential": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_D, _P_D]),
     "vmc_sweeps": (C.c_int, [_V, C.c_int, C.c_double, _P_D, _P_LL]),
     "snapshots_create": (C.c_int, [_V, C.c_int, C.POINTER(C.c_void_p)]),
     "snapshots_destroy": (C.c_int, [_V]),

@@ -28,6 +28,8 @@ void launch_evaluate(const DevModel* m, const DevState& s, int nw, bool mixed, b
                      hipStream_t st);
 void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, int nb_max, bool mixed,
                     bool cplx, int min_updates, hipStream_t st, bool mfma);
+void launch_delta_potential(const DevModel* m, const DevState& s, int n, int p, const double* prop, int stride,
+                            double* out, hipStream_t st);
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
                   bool cplx, int rows_max, size_t lds, hipStream_t st);
 void launch_draw(const DevState& s, int np, int nw, double* out, int stride, hipStream_t st);
@@ -196,6 +198,8 @@ struct alea_system {
     DevBuf<char> inv_bytes, inv_spare;    // inverse records; the gather target of a permutation
     DevBuf<double> perm_tmp;              // gather target of the other per-walker arrays
     DevBuf<int> updates, status, src, rebuilds;
+    DevBuf<double> ew_rho, ew_snap;       // the Ewald structure factor kept per walker (ewald_rho)
+    DevBuf<int> ew_inc;
     DevBuf<uint64_t> rng;
     DevBuf<long long> nacc, ncut, branch_total;
     bool energies_valid = false;          // e_loc holds E_L of the current configurations
@@ -363,6 +367,11 @@ struct alea_system {
         branch_total.alloc(1, stream);
         status.alloc(nw_, stream);
         rebuilds.alloc(nw_ * kRebuildCauses, stream);
+        // the per-walker Ewald structure factors, none computed yet
+        ew_rho.alloc(m.ewald ? nw_ * m.nke * 2 : 1, stream);
+        ew_snap.alloc(m.ewald ? nw_ * np * 2 : 1, stream);
+        ew_inc.alloc(nw_, stream);
+        launch_fill_int(ew_inc.p, static_cast<int>(nw_), -1, stream);
         nacc.alloc(nw_, stream);
         src.alloc(nw_, stream);
         // the per-call path's buffers (new_pos, outputs; mask), at the capacity
@@ -383,6 +392,9 @@ struct alea_system {
         st.status = status.p;
         st.rebuilds = rebuilds.p;
         st.n_accepted = nacc.p;
+        st.ew_rho = ew_rho.p;
+        st.ew_snap = ew_snap.p;
+        st.ew_inc = ew_inc.p;
         // LDS of one walker: positions, k-space densities, v', column j, the
         // Sherman-Morrison operands and the resident block's row sums (these
         // four over the padded rows), then copies of the pair splines and the
@@ -427,6 +439,11 @@ struct alea_system {
         v.status += w;
         v.n_accepted += w;
         v.rebuilds += w * kRebuildCauses;
+        if (m.ewald) {
+            v.ew_rho += w * m.nke * 2;
+            v.ew_snap += w * m.np * 2;
+        }
+        v.ew_inc += w;
         v.grad_out = v.lap_out = v.pgrad_out = v.plap_out = nullptr;
         return v;
     }
@@ -1211,6 +1228,27 @@ int alea_ratio_range(alea_system* s, int p, int w_begin, int w_end, const double
         Sys(s);
         const auto [w0, w1] = s->range(w_begin, w_end);
         ratio_range(s, p, w0, w1, new_pos, log_ratio, phase, new_grad, new_lap);
+    });
+}
+
+int alea_delta_potential(alea_system* s, int p, int w_begin, int w_end, const double* new_pos, double* delta) {
+    return guarded([&] {
+        Sys(s);
+        const auto [w0, w1] = s->range(w_begin, w_end);
+        if (!s->evaluated) throw InvalidArg("evaluate first");
+        if (p < 0 || p >= s->np) throw InvalidArg("particle index out of range");
+        if (!new_pos || !delta) throw InvalidArg("null proposal or output");
+        const size_t nw = s->nw, n = static_cast<size_t>(w1 - w0);
+        if (n == 0) return;
+        double* b = s->call_buf.p;
+        for (int d = 0; d < 2; ++d)
+            hip_check(copy_on(s->stream, b + d * nw + w0, new_pos + d * nw + w0, n * sizeof(double),
+                              hipMemcpyHostToDevice), "new_pos");
+        launch_delta_potential(s->dmodel, s->shifted(w0), w1 - w0, p, b + w0, static_cast<int>(nw), b + 2 * nw + w0,
+                               s->stream);
+        hip_check(hipGetLastError(), "launch delta_potential");
+        hip_check(copy_on(s->stream, delta + w0, b + 2 * nw + w0, n * sizeof(double), hipMemcpyDeviceToHost),
+                  "delta");
     });
 }
 

@@ -2338,6 +2338,96 @@ __global__ __launch_bounds__(64) void k_call(const DevModel* __restrict__ Mp, De
     if (mode == 2) wk.store();
 }
 
+// ── the Ewald structure factor, kept up to date per walker ───────────────
+//
+// rho(k) = Sum_i q_i e^{ik.r_i} for the nke Ewald vectors, held per walker in
+// HBM (ew_rho) with the positions it describes (ew_snap), as the reference's
+// cache of delta_potential_particle does (EwaldTerm::sync_delta_cache,
+// ewald_term.cpp:675-747, ewald_term.h:217-241): a call compares the current
+// positions with the snapshot; up to `limit` changed particles are folded in
+// as rho += q_i (e^{ik.r_new} - e^{ik.r_old}), more (or no valid cache, or
+// 4096 increments since the last full sum, the bound on rounding drift)
+// recompute the full sum.  No move has to notify it, so it cannot go stale:
+// a permutation of the walkers or new positions from the host show up as
+// changed particles.  Lane l gets rho for k = l + 64 q in rr[q], ri[q].
+constexpr int kEwSlots = (kMaxKE + 63) / 64;
+constexpr int kEwRebuildEvery = 4096;
+__device__ void ewald_rho(const DevModel& M, const DevState& S, int w, const double* pos, int limit,
+                          double (&rr)[kEwSlots], double (&ri)[kEwSlots]) {
+    const int np = M.np, nke = M.nke, lane = threadIdx.x & (W - 1);
+    double* rho = S.ew_rho + static_cast<size_t>(w) * nke * 2;
+    double* snap = S.ew_snap + static_cast<size_t>(w) * np * 2;
+    const int inc = S.ew_inc[w];
+    unsigned long long chg[kMaxParticles / W];
+    int n_changed = 0;
+#pragma unroll
+    for (int c = 0; c < kMaxParticles / W; ++c) {
+        const int i = c * W + lane;
+        const bool moved = i < np && (snap[2 * i] != pos[2 * i] || snap[2 * i + 1] != pos[2 * i + 1]);
+        chg[c] = __ballot(moved);
+        n_changed += __popcll(chg[c]);
+    }
+    const bool built = inc >= 0;
+    const bool incremental = built && n_changed <= limit && inc + n_changed < kEwRebuildEvery;
+#pragma unroll
+    for (int q = 0; q < kEwSlots; ++q) {
+        rr[q] = ri[q] = 0.0;
+        const int k = lane + W * q;
+        if (incremental && k < nke) {
+            rr[q] = rho[2 * k];
+            ri[q] = rho[2 * k + 1];
+        }
+    }
+    if (built && n_changed == 0) return;
+    if (incremental) {
+#pragma unroll
+        for (int c = 0; c < kMaxParticles / W; ++c) {
+            for (unsigned long long m = chg[c]; m; m &= m - 1) {
+                const int i = c * W + __builtin_ctzll(m);
+                const double qi = M.charge[i];
+                if (qi == 0.0) continue;
+#pragma unroll
+                for (int q = 0; q < kEwSlots; ++q) {
+                    const int k = lane + W * q;
+                    if (k >= nke) continue;
+                    double so, co, sn, cn;
+                    sincos_bounded(M.kex[k] * snap[2 * i] + M.key[k] * snap[2 * i + 1], &so, &co);
+                    sincos_bounded(M.kex[k] * pos[2 * i] + M.key[k] * pos[2 * i + 1], &sn, &cn);
+                    rr[q] += qi * (cn - co);
+                    ri[q] += qi * (sn - so);
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kEwSlots; ++q) {
+            const int k = lane + W * q;
+            if (k >= nke) continue;
+            for (int i = 0; i < np; ++i) {
+                const double qi = M.charge[i];
+                if (qi == 0.0) continue;
+                double sn, cn;
+                sincos_bounded(M.kex[k] * pos[2 * i] + M.key[k] * pos[2 * i + 1], &sn, &cn);
+                rr[q] += qi * cn;
+                ri[q] += qi * sn;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kEwSlots; ++q) {
+        const int k = lane + W * q;
+        if (k < nke) {
+            rho[2 * k] = rr[q];
+            rho[2 * k + 1] = ri[q];
+        }
+    }
+    for (int i = lane; i < np; i += W) {
+        snap[2 * i] = pos[2 * i];
+        snap[2 * i + 1] = pos[2 * i + 1];
+    }
+    if (lane == 0) S.ew_inc[w] = incremental ? inc + n_changed : 0;
+}
+
 // Local energy: every particle's gradient and Laplacian from the current
 // state, kinetic (+ fixed-phase terms), Ewald, open Coulomb.
 template <typename T, bool CPLX>
@@ -2442,17 +2532,16 @@ __global__ __launch_bounds__(64) void k_measure(const DevModel* __restrict__ Mp,
                     pot += coeff * (M.ew_inv_eps / r - f);
                 }
             }
-        for (int k = lane; k < M.nke; k += W) {
-            double rr = 0.0, ri = 0.0;
-            for (int i = 0; i < np; ++i) {
-                const double q = M.charge[i];
-                if (q == 0.0) continue;
-                double s, c;
-                sincos_bounded(M.kex[k] * sh.pos[2 * i] + M.key[k] * sh.pos[2 * i + 1], &s, &c);
-                rr += q * c;
-                ri += q * s;
-            }
-            pot += M.kew[k] * (rr * rr + ri * ri);
+        // the reciprocal sum from the walker's kept structure factor, folding
+        // in the particles that moved since it was last brought up to date
+        // while that is cheaper than the full sum (two phases per moved
+        // particle against one per particle)
+        double rr[kEwSlots], ri[kEwSlots];
+        ewald_rho(M, S, w, sh.pos, np / 2, rr, ri);
+#pragma unroll
+        for (int q = 0; q < kEwSlots; ++q) {
+            const int k = lane + W * q;
+            if (k < M.nke) pot += M.kew[k] * (rr[q] * rr[q] + ri[q] * ri[q]);
         }
     }
     for (int cp = 0; cp < M.n_coul; ++cp) {
@@ -2491,6 +2580,101 @@ __global__ __launch_bounds__(64) void k_measure(const DevModel* __restrict__ Mp,
         S.e_loc[w] = e;
         S.e_loc_im[w] = ei;
     }
+}
+
+// Hamiltonian::delta_potential_particle (hamiltonian.cpp:105-117) for the
+// walkers of the batch: the potential change of moving particle p of walker
+// w to prop (wrapped), nothing committed.  Ewald
+// (EwaldTerm::delta_potential_particle, ewald_term.cpp:750-815): the moved
+// particle's real-space pairs, and the reciprocal part
+// Sum_k w_k [2 Re(delta rho*) + |delta|^2], delta = q_p (e^{ik.r'} - e^{ik.r}),
+// against the walker's kept structure factor (up to four moved particles
+// folded in, as the reference's cache does).  Open Coulomb and inverse
+// powers: the moved particle's pairs (radial_particle_delta,
+// hamiltonian_component.h:25-48).  The constants cancel.
+__global__ __launch_bounds__(64) void k_delta_potential(const DevModel* __restrict__ Mp, DevState S, int p,
+                                                        const double* __restrict__ prop, int stride,
+                                                        double* __restrict__ out) {
+    const DevModel& M = *Mp;
+    __shared__ double pos[2 * kMaxParticles];
+    const int w = blockIdx.x, lane = threadIdx.x;
+    const int np = M.np;
+    const double* src = S.pos + static_cast<size_t>(w) * np * 2;
+    for (int i = lane; i < 2 * np; i += W) pos[i] = src[i];
+    __syncthreads();
+    double c0 = prop[w], c1 = prop[stride + w];
+    wrap(M.cell, c0, c1);
+    const double o0 = pos[2 * p], o1 = pos[2 * p + 1];
+    const int gp = M.group_of[p];
+    double delta = 0.0;
+    if (M.ewald) {
+        const double qp = M.charge[p];
+        if (qp != 0.0) {
+            auto short_range = [&](double r) {
+                if (r >= M.ew_rcut) return 0.0;
+                double f, df, d2f;
+                spline_eval(M.ew_spline, M.ew_knots, M.ew_rcut, M.ew_inv_delta, r, f, df, d2f);
+                return M.ew_inv_eps / r - f;
+            };
+            for (int j = lane; j < np; j += W) {
+                const double qj = M.charge[j];
+                if (j == p || qj == 0.0) continue;
+                double n0 = c0 - pos[2 * j], n1 = c1 - pos[2 * j + 1];
+                double d0 = o0 - pos[2 * j], d1 = o1 - pos[2 * j + 1];
+                const double rn = dist(M.cell, n0, n1), ro = dist(M.cell, d0, d1);
+                delta += qp * qj * (short_range(rn) - short_range(ro));
+            }
+            double rr[kEwSlots], ri[kEwSlots];
+            ewald_rho(M, S, w, pos, 4, rr, ri);
+#pragma unroll
+            for (int q = 0; q < kEwSlots; ++q) {
+                const int k = lane + W * q;
+                if (k >= M.nke) continue;
+                double so, co, sn, cn;
+                sincos_bounded(M.kex[k] * o0 + M.key[k] * o1, &so, &co);
+                sincos_bounded(M.kex[k] * c0 + M.key[k] * c1, &sn, &cn);
+                const double dr = qp * (cn - co), di = qp * (sn - so);
+                delta += M.kew[k] * (2.0 * (dr * rr[q] + di * ri[q]) + dr * dr + di * di);
+            }
+        }
+    }
+    // the pairs of p in a term over groups (ga, gb)
+    auto pairs = [&](int ga, int gb, auto radial) {
+        auto one = [&](int j) {
+            if (j == p) return;
+            double n0 = c0 - pos[2 * j], n1 = c1 - pos[2 * j + 1];
+            double d0 = o0 - pos[2 * j], d1 = o1 - pos[2 * j + 1];
+            delta += radial(dist(M.cell, n0, n1)) - radial(dist(M.cell, d0, d1));
+        };
+        if (ga == gb) {
+            if (gp == ga)
+                for (int j = M.g_first[gb] + lane; j < M.g_last[gb]; j += W) one(j);
+        } else {
+            if (gp == ga)
+                for (int j = M.g_first[gb] + lane; j < M.g_last[gb]; j += W) one(j);
+            if (gp == gb)
+                for (int j = M.g_first[ga] + lane; j < M.g_last[ga]; j += W) one(j);
+        }
+    };
+    for (int cp = 0; cp < M.n_coul; ++cp) {
+        const double coeff = M.coul_coeff[cp];
+        pairs(M.coul_ga[cp], M.coul_gb[cp], [&](double r) { return coeff / r; });
+    }
+    for (int t = 0; t < M.n_ipow; ++t) {
+        const double c = M.ipow_c[t], pw = M.ipow_p[t];
+        const int pi = M.ipow_int[t];
+        pairs(M.ipow_ga[t], M.ipow_gb[t], [&](double r) {
+            if (pi > 0) {
+                const double inv_r = 1.0 / r;
+                double inv_rp = inv_r;
+                for (int q = 1; q < pi; ++q) inv_rp *= inv_r;
+                return c * inv_rp;
+            }
+            return c * pow(r, -pw);
+        });
+    }
+    delta = wsum(delta);
+    if (lane == 0) out[w] = delta;
 }
 
 // Branching gather: new walker j takes old walker src[j]'s record.
@@ -2745,6 +2929,10 @@ void launch_call(const DevModel* m, const DevState& s, const DevCall& c, int p, 
 void launch_measure(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, size_t lds,
                     hipStream_t st) {
     DISPATCH(k_measure, dim3(nw), dim3(64), lds, st, m, s);
+}
+void launch_delta_potential(const DevModel* m, const DevState& s, int n, int p, const double* prop, int stride,
+                            double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_delta_potential, dim3(n), dim3(64), 0, st, m, s, p, prop, stride, out);
 }
 void launch_gather(const double* src_d, double* dst_d, int stride, const int* src, int n_new,
                    hipStream_t st) {

@@ -145,6 +145,12 @@ struct DevState {
     int* status = nullptr;        // [w] nonzero: singular determinant
     long long* n_accepted = nullptr;   // [w]
     int* rebuilds = nullptr;      // [w][4] in-sweep determinant rebuilds by trigger (RebuildCause)
+    // Ewald structure factor rho(k) = Sum_i q_i e^{ik.r_i} of each walker,
+    // with the positions it was computed from and the increments folded in
+    // since its last full computation (-1: none yet); see ewald_rho
+    double* ew_rho = nullptr;     // [w][nke][2]
+    double* ew_snap = nullptr;    // [w][np][2]
+    int* ew_inc = nullptr;        // [w]
     int inv_stride = 0;           // T elements per walker in inv
     int work_stride = 0;          // doubles per walker in work
     int nb_max = 0;

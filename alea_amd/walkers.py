@@ -152,6 +152,14 @@ class Walkers:
         self.api("ratio_range", self.h, p, w_begin, w_end, ptr(npos), ptr(lr), ptr(ph), ptr(ng), ptr(nl))
         return out
 
+    def delta_potential(self, p, new_pos, w_begin=0, w_end=-1):
+        """Hamiltonian::delta_potential_particle for every walker of the range:
+        V with particle p at new_pos [2][nw] minus V now, nothing committed."""
+        npos = np.ascontiguousarray(new_pos, dtype=np.float64).reshape(-1)
+        out = np.zeros(self.nw)
+        self.api("delta_potential", self.h, p, w_begin, w_end, ptr(npos), ptr(out))
+        return out
+
     def accept_range(self, p, w_begin, w_end, mask):
         m = u8(mask)
         self.api("accept_range", self.h, p, w_begin, w_end, ptr(m, _P_U8))

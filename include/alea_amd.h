@@ -145,6 +145,18 @@ int alea_ratio_range(alea_system* s, int p, int w_begin, int w_end, const double
                      double* log_ratio, double* phase, double* new_grad, double* new_lap);
 int alea_accept_range(alea_system* s, int p, int w_begin, int w_end, const unsigned char* mask);
 
+/* Hamiltonian::delta_potential_particle (src/hamiltonian/hamiltonian.cpp:105-117),
+ * for the walkers [w_begin, w_end): delta[w] = V(p at new_pos[.][w], wrapped
+ * into the cell) - V(current positions), nothing committed.  new_pos is [2][nw]
+ * and delta [nw] at the batch stride; only the range's entries are read or
+ * written.  The Ewald part follows EwaldTerm::delta_potential_particle
+ * (ewald_term.cpp:750-815): the moved particle's real-space pairs and the
+ * reciprocal change against each walker's kept structure factor, which folds
+ * in up to four particles moved since it last saw the walker and is
+ * recomputed otherwise (sync_delta_cache, ewald_term.cpp:675-747); Coulomb and
+ * inverse powers take the moved particle's pairs. */
+int alea_delta_potential(alea_system* s, int p, int w_begin, int w_end, const double* new_pos, double* delta);
+
 /* Fused sweeps: VMC::sweep / sweep_walkers (vmc.cpp:69-155) with symmetric
  * Metropolis proposals of width sqrt(tau/m).  tape: [sweep][np][dim+1][nw]
  * random numbers in draw_sweep_randoms order (sweep_blocking.h:39-48), or NULL

@@ -366,6 +366,16 @@ public:
         check(alea_local_energy(h_, e.data(), im ? im->data() : nullptr));
         return e;
     }
+    // delta_potential_particle (hamiltonian.cpp:105-117), for every walker of
+    // [w_begin, w_end) at once: V with particle p at proposed ([dim][nw]) minus
+    // V now, nothing committed
+    std::vector<double> delta_potential_particle(WalkerSet& ws, int p, std::span<const double> proposed,
+                                                 int w_begin = 0, int w_end = -1) {
+        std::vector<double> d(ws.num_walkers());
+        if (proposed.size() < 2 * d.size()) throw std::invalid_argument("proposal must be [dim][nw]");
+        check(alea_delta_potential(h_, p, w_begin, w_end, proposed.data(), d.data()));
+        return d;
+    }
 
 private:
     alea_system* h_;

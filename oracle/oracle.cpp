@@ -2230,6 +2230,96 @@ int oracle_ratio_range(oracle_system* h, int p, int w_begin, int w_end, const do
     });
 }
 
+// Hamiltonian::delta_potential_particle (hamiltonian.cpp:105-117) for each
+// walker of [w_begin, w_end): the potential of the batch with particle p at
+// new_pos[.][w] (wrapped into the cell) minus the potential now.  Ewald as
+// EwaldTerm::delta_potential_particle (ewald_term.cpp:750-815): the moved
+// particle's real-space pairs against the distance table, and
+// Sum_k w_k [2 Re(delta rho*) + |delta|^2] with delta = q_p (e^{ik.r'} -
+// e^{ik.r}) against rho of the current positions (computed here afresh; the
+// reference keeps it in a cache that is exact by construction).  Coulomb
+// (coulomb.cpp:148-155) through radial_particle_delta
+// (hamiltonian_component.h:25-48); inverse powers, which the reference
+// evaluates before and after (hamiltonian_component.h:216-260), through the
+// same pair difference.
+int oracle_delta_potential(oracle_system* h, int p, int w_begin, int w_end, const double* new_pos, double* delta) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        if (p < 0 || static_cast<std::size_t>(p) >= s.np) throw std::invalid_argument("oracle: particle out of range");
+        const auto [w0, w1] = oracle_range(s, w_begin, w_end);
+        const std::size_t nw = s.nw, np = s.np, pp = static_cast<std::size_t>(p);
+        std::vector<double> q(np, 0.0);
+        std::vector<int> group_of(np, -1);
+        for (std::size_t g = 0; g < s.groups.size(); ++g)
+            for (std::size_t i = s.groups[g].first; i < s.groups[g].last; ++i) {
+                q[i] = s.groups[g].charge;
+                group_of[i] = static_cast<int>(g);
+            }
+        for (std::size_t w = w0; w < w1; ++w) {
+            double c0 = new_pos[w], c1 = new_pos[nw + w];
+            s.cell.wrap(c0, c1);
+            const double o0 = s.P(pp, 0, w), o1 = s.P(pp, 1, w);
+            auto r_new = [&](std::size_t j) {
+                double d0 = c0 - s.P(j, 0, w), d1 = c1 - s.P(j, 1, w);
+                s.cell.min_image(d0, d1);
+                return std::sqrt(d0 * d0 + d1 * d1);
+            };
+            double d = 0.0;
+            if (s.ewald.active && q[pp] != 0.0) {
+                const Ewald& e = s.ewald;
+                auto short_range = [&](double r) { return r < e.r_cut ? e.inv_eps / r - e.sp.value(r) : 0.0; };
+                for (std::size_t j = 0; j < np; ++j) {
+                    if (j == pp || q[j] == 0.0) continue;
+                    d += q[pp] * q[j] * (short_range(r_new(j)) - short_range(s.D(pp, j, w)));
+                }
+                for (std::size_t kk = 0; kk < e.w.size(); ++kk) {
+                    double rr = 0.0, ri = 0.0;
+                    for (std::size_t i = 0; i < np; ++i) {
+                        if (q[i] == 0.0) continue;
+                        const double ph = e.kx[kk] * s.P(i, 0, w) + e.ky[kk] * s.P(i, 1, w);
+                        rr += q[i] * std::cos(ph);
+                        ri += q[i] * std::sin(ph);
+                    }
+                    const double pn = e.kx[kk] * c0 + e.ky[kk] * c1, po = e.kx[kk] * o0 + e.ky[kk] * o1;
+                    const double dr = q[pp] * (std::cos(pn) - std::cos(po)), di = q[pp] * (std::sin(pn) - std::sin(po));
+                    d += e.w[kk] * (2.0 * (dr * rr + di * ri) + dr * dr + di * di);
+                }
+            }
+            auto pairs = [&](int ga, int gb, auto radial) {
+                auto one = [&](std::size_t j) {
+                    if (j != pp) d += radial(r_new(j)) - radial(s.D(pp, j, w));
+                };
+                const Group& a = s.groups[ga];
+                const Group& b = s.groups[gb];
+                if (ga == gb) {
+                    if (group_of[pp] == ga)
+                        for (std::size_t j = a.first; j < a.last; ++j) one(j);
+                } else {
+                    if (group_of[pp] == ga)
+                        for (std::size_t j = b.first; j < b.last; ++j) one(j);
+                    if (group_of[pp] == gb)
+                        for (std::size_t j = a.first; j < a.last; ++j) one(j);
+                }
+            };
+            for (const auto& c : s.coulomb) {
+                const double coeff = s.groups[c.ga].charge * s.groups[c.gb].charge * c.inv_eps;
+                pairs(c.ga, c.gb, [&](double r) { return coeff / std::sqrt(r * r + 0.0); });
+            }
+            for (const auto& t : s.ipow)
+                pairs(t.ga, t.gb, [&](double r) {
+                    if (t.p_int > 0) {
+                        const double inv_r = 1.0 / r;
+                        double inv_rp = inv_r;
+                        for (int k = 1; k < t.p_int; ++k) inv_rp *= inv_r;
+                        return t.c * inv_rp;
+                    }
+                    return t.c * std::pow(r, -t.p);
+                });
+            delta[w] = d;
+        }
+    });
+}
+
 int oracle_accept_range(oracle_system* h, int p, int w_begin, int w_end, const unsigned char* mask) {
     return guarded([&] {
         Sys& s = *S(h);

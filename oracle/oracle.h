@@ -96,6 +96,7 @@ int oracle_device_tape(uint64_t seed, uint64_t stream, int np, int nw, int n_swe
 int oracle_ratio_range(oracle_system* s, int p, int w_begin, int w_end, const double* new_pos, double* log_ratio,
                        double* phase, double* new_grad, double* new_lap);
 int oracle_accept_range(oracle_system* s, int p, int w_begin, int w_end, const unsigned char* mask);
+int oracle_delta_potential(oracle_system* s, int p, int w_begin, int w_end, const double* new_pos, double* delta);
 int oracle_refresh_inverse(oracle_system* s, int w_begin, int w_end);
 int oracle_check_status(oracle_system* s);
 int oracle_dmc_begin(oracle_system* s);

@@ -384,6 +384,22 @@ int ref_local_energy(void* h, double* e_re) {
     });
 }
 
+// Hamiltonian::delta_potential_particle (hamiltonian.cpp:105-117) for every
+// walker: new_pos [dim][nw], out [nw].
+int ref_delta_potential(void* h, int p, const double* new_pos, double* out) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        build_hamiltonian(s);
+        std::vector<double> scratch(s.nw);
+        double proposed[3];
+        for (std::size_t k = 0; k < s.nw; ++k) {
+            for (std::size_t d = 0; d < s.dim; ++d) proposed[d] = new_pos[d * s.nw + k];
+            out[k] = s.ham.delta_potential_particle(*s.ws, static_cast<std::size_t>(p), k,
+                                                    std::span<const double>(proposed, s.dim), scratch);
+        }
+    });
+}
+
 // Per-call path: propose particle p at new_pos ([dim*nw]) and return
 // TrialWavefunction::ratio_batch's outputs without committing.
 int ref_ratio(void* h, int p, const double* new_pos, double* log_ratio, double* new_grad,

@@ -131,6 +131,7 @@ def ref_lib():
                                          C.POINTER(C.c_int), C.c_int, D, D, D, C.POINTER(C.c_int), D, D]),
             "ref_robust_spread": (C.c_double, [D, C.c_int]),
             "ref_add_inverse_power": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double]),
+            "ref_delta_potential": (C.c_int, [C.c_void_p, C.c_int, D, D]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(lib, name)
@@ -215,6 +216,12 @@ class RefWalkers:
     def accept(self, p, mask):
         m = u8(mask)
         self._chk(self.lib.ref_accept(self.h, p, ptr(m, _P_U8)))
+
+    def delta_potential(self, p, new_pos):
+        npos = f64(new_pos)
+        out = np.empty(self.nw)
+        self._chk(self.lib.ref_delta_potential(self.h, p, ptr(npos), ptr(out)))
+        return out
 
     def vmc_sweeps(self, n_sweeps, tau, tape, drift=False):
         acc = C.c_longlong()

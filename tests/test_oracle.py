@@ -239,6 +239,47 @@ def _jastrow_ewald_specs():
 
 
 @pytest.mark.parametrize("spec", _jastrow_ewald_specs(), ids=lambda s: s.name)
+def test_delta_potential_matches_reference(spec):
+    """Hamiltonian::delta_potential_particle (hamiltonian.cpp:105-117) of the
+    oracle against the reference's own (Ewald's structure-factor cache,
+    ewald_term.cpp:675-815; Coulomb's and the inverse power's pair
+    differences): fresh walkers, then after commits of one particle (the
+    reference folds them into its cache) and after whole sweeps (it
+    recomputes), to 1e-10 relative.  A proposal onto the current position
+    changes nothing."""
+    nw = 6
+    pos = uniform_positions(spec, nw, np.random.default_rng(31))
+    o, r = oracle_walkers(spec, nw), RefWalkers(spec, nw)
+    for x in (o, r):
+        x.set_positions(pos)
+        x.evaluate()
+    rng = np.random.default_rng(32)
+    P = o.positions().reshape(spec.n_particles, 2, nw)
+    assert np.all(o.delta_potential(1, P[1].reshape(-1)) == 0.0)
+    for step in range(4):
+        for p in (0, spec.n_particles // 2, spec.n_particles - 1):
+            P = o.positions().reshape(spec.n_particles, 2, nw)
+            newp = (P[p] + rng.normal(scale=0.5, size=(2, nw))).reshape(-1)
+            do, dr = o.delta_potential(p, newp), r.delta_potential(p, newp)
+            assert rel_err(do, dr) < 1e-10, (step, p)
+        if step < 2:
+            # commit one particle on some walkers: the reference's cache
+            # folds the change in
+            P = o.positions().reshape(spec.n_particles, 2, nw)
+            q = 2 + step
+            newp = (P[q] + rng.normal(scale=0.5, size=(2, nw))).reshape(-1)
+            o.ratio(q, newp)
+            r.ratio(q, newp)
+            mask = rng.random(nw) < 0.6
+            o.accept(q, mask)
+            r.accept(q, mask)
+        else:
+            tape = oracle_tape(40 + step, spec.n_particles, nw, 1)
+            tau = (spec.lattice[0] / 8) ** 2
+            assert o.vmc_sweeps(1, tau, tape) == r.vmc_sweeps(1, tau, tape)
+
+
+@pytest.mark.parametrize("spec", _jastrow_ewald_specs(), ids=lambda s: s.name)
 def test_oracle_matches_reference(spec):
     nw = 8
     pos = uniform_positions(spec, nw, np.random.default_rng(12))
