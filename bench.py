@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
 N_UP = N_DN = 61
 RS = 30.0
 TWIST_SEED = 3
@@ -103,10 +104,16 @@ def run_gpu(precision, nw, steps, warmup, tau, pos):
         kernel_ms = w.sweep_kernel_ms()
         triggers, _ = w.rebuild_counts(reset=True)
         acceptance = acc / (nw * spec.n_particles * steps)
+        # one rebuild of every block (the batched inverse recompute on the
+        # matrix cores), timed on its own
+        t0 = time.perf_counter()
+        w.refresh_inverse()
+        w.api("synchronize", w.h)
+        refresh_ms = 1e3 * (time.perf_counter() - t0)
         e, e_im = w.local_energy()
         out = dict(moves_per_s=nw * spec.n_particles * steps / wall, ms_per_step=1e3 * wall / steps,
                    kernel_ms=float(np.mean(kernel_ms)), kernel_ms_min=float(np.min(kernel_ms)),
-                   kernel_ms_max=float(np.max(kernel_ms)), acceptance=acceptance,
+                   kernel_ms_max=float(np.max(kernel_ms)), acceptance=acceptance, refresh_ms=refresh_ms,
                    rebuild_triggers_per_sweep=[t / steps for t in triggers],
                    e_per_particle=float(np.mean(e)) / spec.n_particles,
                    e_per_particle_err=float(np.std(e) / math.sqrt(nw)) / spec.n_particles,
@@ -490,7 +497,7 @@ def main():
     # the sweep holds the swept block in registers, so HBM does not bound it:
     # its VALU issue time (SQ_INSTS_VALU from rocprofv3, profiles/) against
     # the kernel time measured here
-    valu = None
+    valu = refresh_roof = None
     vf = os.path.join(ROOT, "profiles", "sweep_counters.json")
     if os.path.exists(vf):
         with open(vf) as f:
@@ -498,6 +505,15 @@ def main():
         if v.get("walkers") == args.walkers and v.get("precision") == "mixed":
             valu = {"insts_per_launch": v["valu_insts_per_launch"], "issue_ms": v["valu_issue_ms"],
                     "issue_frac": v["valu_issue_ms"] / mixed["kernel_ms"], "wait_frac": v["wait_fraction"]}
+            rm = v.get("refresh_mfma")
+            if rm:
+                # the full rebuild's matrix-core flops (SQ_INSTS_VALU_MFMA_MOPS_F64,
+                # profiles/) over the rebuild timed here
+                tf_ = rm["mfma_flops"] / (mixed["refresh_ms"] * 1e-3) / 1e12
+                refresh_roof = {"kernel": rm["kernel"], "ms": mixed["refresh_ms"], "bound": "mfma",
+                                "achieved": tf_, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                "frac": tf_ / FP64_MFMA_PEAK_TFLOPS, "flops": rm["mfma_flops"],
+                                "fp64_form_ms": fp64["refresh_ms"]}
     line = {
         "metric": "walker-moves/s, 2D HEG N=122, mixed vs FP64 GPU",
         "value": mixed["moves_per_s"],
@@ -540,6 +556,7 @@ def main():
                               "(measured: `traffic` per launch) and frac can pass 1; its binding "
                               "roofline is VALU issue (`valu.issue_frac`) and latency (`valu.wait_frac`)",
                      "valu": valu},
+        "refresh_roofline": refresh_roof,
         "serving": None if args.no_serving else serving_leg(),
         "dmc": None if args.no_dmc else dmc_leg(),
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(tau),
