@@ -18,6 +18,12 @@ _P_LL = C.POINTER(C.c_longlong)
 _V = C.c_void_p
 
 # name -> (restype, argtypes); the handle is an opaque pointer.
+class AleaParam(C.Structure):
+    """alea_param: one free parameter of alea_parameter_derivatives."""
+    _fields_ = [("factor", C.c_int), ("index", C.c_int), ("weights", _P_D), ("spline", _P_D),
+                ("n_knots", C.c_int), ("r_cut", C.c_double), ("inv_delta", C.c_double), ("limit", C.c_double)]
+
+
 SIGNATURES = {
     "last_error": (C.c_char_p, []),
     "create": (C.c_int, [C.POINTER(_V), C.c_int, C.c_int, C.c_int, _P_D, C.c_int]),
@@ -47,6 +53,7 @@ SIGNATURES = {
     "ratio_range": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_D, _P_D, _P_D, _P_D, _P_D]),
     "accept_range": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_U8]),
     "delta_potential": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_D, _P_D]),
+    "parameter_derivatives": (C.c_int, [_V, C.POINTER(AleaParam), C.c_int, _P_D]),
     "vmc_sweeps": (C.c_int, [_V, C.c_int, C.c_double, _P_D, _P_LL]),
     "snapshots_create": (C.c_int, [_V, C.c_int, C.POINTER(C.c_void_p)]),
     "snapshots_destroy": (C.c_int, [_V]),

@@ -187,6 +187,30 @@ def _laplace_g(x):
     return out
 
 
+def _laplace_g2(x):
+    """int_0^inf e^{-xt} (1 - 1/sqrt(1 + t^2)) (1 - x t) dt = G(x) + x G'(x)
+    for complex x with Re x > 0, on the quadrature of _laplace_g: the
+    transform of 1/(r + c)^2 below is minus the c-derivative of that of
+    1/(r + c)."""
+    x = np.atleast_1d(np.asarray(x, dtype=complex))
+    out = np.zeros(x.shape, dtype=complex)
+    for i, xv in enumerate(x):
+        lo = 1e-3 * min(abs(xv), 1.0)
+        edges = [0.0, lo]
+        while edges[-1] < 80.0:
+            edges.append(2.0 * edges[-1])
+        e = np.array(edges)
+        a, b = e[:-1], e[1:]
+        s = (a[:, None] + (b - a)[:, None] * _GX[None, :]).ravel()
+        w = ((b - a)[:, None] * _GW[None, :]).ravel()
+        t = s / xv
+        q = t * t
+        root = np.sqrt(1.0 + q)
+        h = q / (root * (1.0 + root))
+        out[i] = np.sum(w * np.exp(-s) * h * (1.0 - s)) / xv
+    return out
+
+
 # ── the breakup ─────────────────────────────────────────────────────────────
 def _fit_rows(cell, k_cut, k_continuum, k_max, n_continuum):
     """|k| of the fitted rows and their weights: the lattice shells in
@@ -367,28 +391,37 @@ def _pade(lattice: tuple, a: tuple, b: tuple):
     u_inf = pade_limit(a, b)
     if not b:
         raise ValueError("padé jastrow: a periodic image sum needs a denominator")
-    # f = u - u_inf = R / Q with deg R < deg Q; over the simple roots rho = -c
-    # of Q, f = sum A_i / (r + c_i), A_i = R(rho_i) / Q'(rho_i)
+    # f = u - u_inf = R / Q with deg R < deg Q, by partial fractions over the
+    # roots rho = -c of Q: a simple root gives A / (r + c), A = R(rho)/Q'(rho);
+    # a double one (the squared denominators of a Padé's parameter
+    # derivatives, pade_jastrow.cpp:227-244) A / (r + c) + B / (r + c)^2 with
+    # Q = (r - rho)^2 S: B = R(rho)/S(rho), A = (R'(rho) S(rho) - R(rho) S'(rho)) / S(rho)^2
     pq = np.polynomial.Polynomial([1.0] + b)
     pr = np.polynomial.Polynomial([0.0] + a) - u_inf * pq
-    roots = pq.roots()
-    if len(roots) > 1 and np.min(np.abs(roots[:, None] - roots[None, :]) + np.eye(len(roots)) * 1e300) \
-            < 1e-8 * np.max(np.abs(roots)):
-        raise ValueError("padé jastrow: the denominator has a repeated root; no image-sum split for it")
-    if np.any(roots.real >= 0.0):
-        raise ValueError("padé jastrow: the denominator has a root with a non-negative real part; the "
-                         "image-sum split needs every root in the left half plane")
-    c = -roots
-    amp = pr(roots) / pq.deriv()(roots)
+    terms = []   # (c, A, B)
+    for rho, mult in _root_groups(pq):
+        if rho.real >= 0.0:
+            raise ValueError("padé jastrow: the denominator has a root with a non-negative real part; the "
+                             "image-sum split needs every root in the left half plane")
+        if mult == 1:
+            terms.append((-rho, pr(rho) / pq.deriv()(rho), 0.0))
+        else:
+            sq, rem = divmod(pq, np.polynomial.Polynomial([rho * rho, -2.0 * rho, 1.0]))
+            sv, dsv = sq(rho), sq.deriv()(rho)
+            terms.append((-rho, (pr.deriv()(rho) * sv - pr(rho) * dsv) / (sv * sv), pr(rho) / sv))
     ev = _rational(a, b)
     f = (lambda r: ev(r)[0] - u_inf, lambda r: ev(r)[1], lambda r: ev(r)[2])
 
     def target(ks):
-        # X(k) = -2 pi int_{r_c}^inf r J0(kr) f dr = -(full transform - inside r_c)
+        # X(k) = -2 pi int_{r_c}^inf r J0(kr) f dr = -(full transform - inside r_c);
+        # 2 pi int_0^inf r J0(kr)/(r + c) dr = 2 pi c G(kc), and for 1/(r + c)^2
+        # minus its c-derivative, -2 pi (G(kc) + kc G'(kc))
         ks = np.asarray(ks, dtype=float)
         full = np.zeros(len(ks))
-        for ci, ai in zip(c, amp):
+        for ci, ai, bi in terms:
             full += np.real(TWO_PI * ai * ci * _laplace_g(ks * ci))
+            if bi != 0.0:
+                full += np.real(-TWO_PI * bi * _laplace_g2(ks * ci))
         inner = _inner_transform(lambda r: ev(r)[0] - u_inf, r_cut, 19, ks)
         return -(full - inner)
 
@@ -400,6 +433,51 @@ def _pade(lattice: tuple, a: tuple, b: tuple):
     return dict(n1=br["n1"], n2=br["n2"], kx=br["kx"], ky=br["ky"], w=br["w"], spline=_spline(br),
                 n_knots=br["n_knots"], r_cut=r_cut, inv_delta=1.0 / br["delta"], u_inf=u_inf,
                 residual=br["residual"])
+
+
+def _root_groups(pq):
+    """The roots of the polynomial pq with their multiplicities (1 or 2):
+    roots closer than 1e-6 of their size pair up, polished as a root of pq'
+    by Newton; a triple root is refused."""
+    roots = list(pq.roots())
+    out = []
+    while roots:
+        r0 = roots.pop(0)
+        near = [i for i, r in enumerate(roots) if abs(r - r0) < 1e-6 * max(abs(r0), 1e-300)]
+        if not near:
+            out.append((r0, 1))
+            continue
+        if len(near) > 1:
+            raise ValueError("padé jastrow: the denominator has a root of multiplicity above two")
+        r1 = roots.pop(near[0])
+        rho = 0.5 * (r0 + r1)
+        d1, d2 = pq.deriv(), pq.deriv(2)
+        for _ in range(3):
+            rho = rho - d1(rho) / d2(rho)
+        out.append((rho, 2))
+    return out
+
+
+def pade_parameter_tables(lattice, a, b, q) -> dict:
+    """The image-sum tables of du/dp for parameter q of the Padé u = P/Q,
+    P = sum a_k r^k, Q = 1 + sum b_k r^k (q < len(a): a_{q+1}; else
+    b_{q-len(a)+1}), as PadeJastrow builds them for PeriodicPairSum
+    (pade_jastrow.cpp:227-245): du/da_k = r^k / Q, du/db_k = -P r^k / Q^2,
+    each broken up like u itself on the same k set (its weights, spline and
+    limit)."""
+    a = [float(x) for x in a]
+    b = [float(x) for x in b]
+    if q < len(a):
+        num = [0.0] * q + [1.0]
+        den = b
+    else:
+        k = q - len(a) + 1
+        num = [0.0] * (len(a) + k)
+        for j in range(len(a)):
+            num[j + k] = -a[j]
+        pq = np.polynomial.Polynomial([1.0] + b)
+        den = list((pq * pq).coef[1:])
+    return pade_tables(lattice, num, den)
 
 
 def pade_tables(lattice, a, b) -> dict:

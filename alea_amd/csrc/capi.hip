@@ -30,6 +30,8 @@ void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, 
                     bool cplx, int min_updates, hipStream_t st, bool mfma);
 void launch_delta_potential(const DevModel* m, const DevState& s, int n, int p, const double* prop, int stride,
                             double* out, hipStream_t st);
+void launch_param_derivs(const DevModel* m, const DevState& s, int nw, const DevParam* prm, int n_prm,
+                         const double* wts, const double* spl, double* out, int stride, hipStream_t st);
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
                   bool cplx, int rows_max, size_t lds, hipStream_t st);
 void launch_draw(const DevState& s, int np, int nw, double* out, int stride, hipStream_t st);
@@ -226,6 +228,9 @@ struct alea_system {
     // blocks of 33 to 64 orbitals refresh on the matrix cores (k_refresh_mfma)
     // unless this is cleared (alea_set_refresh_kernel)
     bool refresh_mfma = true;
+    // the wavefunction factors in the order they were added (the reference's
+    // component order): {0, pair factor}, {1, Nosanow group}, {2, block}
+    std::vector<std::pair<int, int>> factors;
     long long since_refresh = 0;  // sweeps (or np accept calls) since the last full refresh
 
     // One captured block of device-stream sweeps (alea_set_sweep_graphs):
@@ -734,6 +739,7 @@ int alea_add_pade_jastrow(alea_system* s, int ga, int gb, int num_order, int den
             load_spline(P.rad, n_knots, spline, r_cut, inv_delta);
             add_kspace(s, ga, gb, n_k, kx, ky, weights);
         }
+        s->factors.push_back({0, m.n_pair});
         ++m.n_pair;
     });
 }
@@ -761,6 +767,7 @@ int alea_add_drummond_jastrow(alea_system* s, int ga, int gb, int C, double gamm
         double ipw = 1.0;
         for (int i = 0; i < C; ++i) ipw *= -L;
         P.rad.alpha1 = gamma / ipw + coeffs[0] * static_cast<double>(C) / L;
+        s->factors.push_back({0, m.n_pair});
         ++m.n_pair;
     });
 }
@@ -779,6 +786,7 @@ int alea_add_nosanow_jastrow(alea_system* s, int group, double alpha, const doub
             m.nos_site[i][1] = sites[2 * (i - m.g_first[group]) + 1];
         }
         m.has_nosanow = 1;
+        s->factors.push_back({1, group});
     });
 }
 
@@ -811,6 +819,7 @@ int alea_add_slater_plane_wave(alea_system* s, int first, int last, const double
             m.orb_k[b][i][1] = kvecs[2 * i + 1];
             m.orb_sin[b][i] = is_sin[i] ? 1 : 0;
         }
+        s->factors.push_back({2, m.n_blocks});
         ++m.n_blocks;
     });
 }
@@ -828,6 +837,7 @@ int alea_add_slater_gaussian(alea_system* s, int first, int last, const double* 
             m.orb_k[b][i][1] = centers[2 * i + 1];
             m.orb_alpha[b][i] = alphas[i];
         }
+        s->factors.push_back({2, m.n_blocks});
         ++m.n_blocks;
     });
 }
@@ -844,6 +854,7 @@ int alea_add_slater_twisted(alea_system* s, int first, int last, const double* q
             m.orb_k[b][i][0] = q[2 * i];
             m.orb_k[b][i][1] = q[2 * i + 1];
         }
+        s->factors.push_back({2, m.n_blocks});
         ++m.n_blocks;
     });
 }
@@ -909,6 +920,7 @@ int alea_add_mcmillan_jastrow(alea_system* s, int ga, int gb, double b, int n) {
         if (m.n_pair >= kMaxPair) throw InvalidArg("too many pair Jastrow factors");
         if (!(b > 0.0) || !std::isfinite(b)) throw InvalidArg("mcmillan: b must be finite and positive");
         if (n < 1 || n > 16) throw InvalidArg("mcmillan: the exponent must be 1..16");
+        s->factors.push_back({0, m.n_pair});
         DevPair& P = m.pair[m.n_pair++];
         P.ga = ga;
         P.gb = gb;
@@ -1249,6 +1261,69 @@ int alea_delta_potential(alea_system* s, int p, int w_begin, int w_end, const do
         hip_check(hipGetLastError(), "launch delta_potential");
         hip_check(copy_on(s->stream, delta + w0, b + 2 * nw + w0, n * sizeof(double), hipMemcpyDeviceToHost),
                   "delta");
+    });
+}
+
+int alea_parameter_derivatives(alea_system* s, const alea_param* params, int n, double* derivs) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->evaluated) throw InvalidArg("evaluate first");
+        if (n < 0 || (n > 0 && (!params || !derivs))) throw InvalidArg("null parameters or output");
+        if (n == 0) return;
+        const DevModel& m = s->host;
+        std::vector<DevParam> dp(static_cast<size_t>(n));
+        std::vector<double> wts(1, 0.0), spl(1, 0.0);
+        for (int i = 0; i < n; ++i) {
+            const alea_param& q = params[i];
+            if (q.factor < 0 || q.factor >= static_cast<int>(s->factors.size()))
+                throw InvalidArg("factor index out of range");
+            const auto [kind, idx] = s->factors[static_cast<size_t>(q.factor)];
+            DevParam& d = dp[static_cast<size_t>(i)];
+            d.index = q.index;
+            if (kind == 0) {
+                const DevRadial& R = m.pair[idx].rad;
+                if (R.kind == 2) throw InvalidArg("the McMillan factor has no free parameters");
+                const int n_par = R.kind == 1 ? R.num : R.num + R.den;
+                if (q.index < 0 || q.index >= n_par) throw InvalidArg("parameter index out of range");
+                d.pair = idx;
+                if (R.image) {
+                    if (!q.weights || !q.spline) throw InvalidArg("a parameter of an image-summed factor needs its tables");
+                    if (q.n_knots < 2 || q.n_knots > kMaxKnots) throw InvalidArg("parameter spline: knot count out of range");
+                    d.wofs = static_cast<int>(wts.size());
+                    wts.insert(wts.end(), q.weights, q.weights + m.nkj);
+                    d.sofs = static_cast<int>(spl.size());
+                    spl.insert(spl.end(), q.spline, q.spline + 3 * q.n_knots);
+                    d.n_knots = q.n_knots;
+                    d.r_cut = q.r_cut;
+                    d.inv_delta = q.inv_delta;
+                    d.limit = q.limit;
+                }
+            } else if (kind == 1) {
+                if (q.index != 0) throw InvalidArg("the Nosanow factor has one parameter");
+                d.group = idx;
+            } else {
+                throw InvalidArg("this determinant has no free parameters");
+            }
+        }
+        const size_t nw = static_cast<size_t>(s->nw);
+        DevBuf<DevParam> d_prm;
+        DevBuf<double> d_w, d_s, d_out;
+        d_prm.alloc(dp.size(), s->stream);
+        d_w.alloc(wts.size(), s->stream);
+        d_s.alloc(spl.size(), s->stream);
+        d_out.alloc(static_cast<size_t>(n) * nw, s->stream);
+        hip_check(hipMemcpyAsync(d_prm.p, dp.data(), dp.size() * sizeof(DevParam), hipMemcpyHostToDevice, s->stream),
+                  "parameters");
+        hip_check(hipMemcpyAsync(d_w.p, wts.data(), wts.size() * sizeof(double), hipMemcpyHostToDevice, s->stream),
+                  "weights");
+        hip_check(hipMemcpyAsync(d_s.p, spl.data(), spl.size() * sizeof(double), hipMemcpyHostToDevice, s->stream),
+                  "splines");
+        s->set_model_nw(s->nw);
+        launch_param_derivs(s->dmodel, s->st, s->nw, d_prm.p, n, d_w.p, d_s.p, d_out.p, static_cast<int>(nw),
+                            s->stream);
+        hip_check(hipGetLastError(), "launch parameter derivatives");
+        hip_check(copy_on(s->stream, derivs, d_out.p, static_cast<size_t>(n) * nw * sizeof(double),
+                          hipMemcpyDeviceToHost), "derivatives");
     });
 }
 

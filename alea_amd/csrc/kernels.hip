@@ -2677,6 +2677,91 @@ __global__ __launch_bounds__(64) void k_delta_potential(const DevModel* __restri
     if (lane == 0) out[w] = delta;
 }
 
+// TrialWavefunction::compute_parameter_derivatives (trial_wavefunction.cpp:
+// 150-160) for the free parameters listed in prm: out[i * stride + w] =
+// d log|psi| / d p_i of walker w.  One wave per walker; the pair sums have
+// lanes over a factor's first particle, the reciprocal part lanes over k.
+// Padé (pade_jastrow.cpp:641-700): du/da_k = r^k / Q, du/db_k = -P r^k / Q^2,
+// summed over images as du/dp - limit - spline(r) inside r_cut plus
+// sum_k w_k |rho_a|^2 (one group) or 2 w_k Re(rho_a rho_b*) with the
+// parameter's weights (periodic_pair_sum.cpp:85-98,
+// kspace_pair_sum.cpp:303-313); Drummond (drummond_jastrow.cpp:200-300):
+// du/dalpha_0 = (r - L)^C (1 + C r / L), du/dalpha_l = (r - L)^C r^l inside
+// L; Nosanow: -sum_i |r_i - R_i|^2.
+__global__ __launch_bounds__(64) void k_param_derivs(const DevModel* __restrict__ Mp, DevState S,
+                                                     const DevParam* __restrict__ prm, int n_prm,
+                                                     const double* __restrict__ wts, const double* __restrict__ spl,
+                                                     double* __restrict__ out, int stride) {
+    const DevModel& M = *Mp;
+    __shared__ double pos[2 * kMaxParticles];
+    const int w = blockIdx.x, lane = threadIdx.x;
+    const int np = M.np;
+    const double* src = S.pos + static_cast<size_t>(w) * np * 2;
+    for (int i = lane; i < 2 * np; i += W) pos[i] = src[i];
+    __syncthreads();
+    const double* rho = S.rho + static_cast<size_t>(w) * M.ng * M.nkj * 2;
+    for (int i = 0; i < n_prm; ++i) {
+        const DevParam& q = prm[i];
+        double t = 0.0;
+        if (q.pair >= 0) {
+            const DevPair& F = M.pair[q.pair];
+            const DevRadial& R = F.rad;
+            auto dudp = [&](double r) {
+                if (R.kind == 1) {
+                    if (r >= R.L) return 0.0;
+                    double g = 1.0;
+                    for (int k = 0; k < R.C; ++k) g *= r - R.L;
+                    if (q.index == 0) return g * (1.0 + static_cast<double>(R.C) * r / R.L);
+                    double rl = 1.0;
+                    for (int k = 0; k <= q.index; ++k) rl *= r;
+                    return g * rl;
+                }
+                double P = 0.0, Q = 1.0, rk = 1.0;
+                for (int k = 0; k < max(R.num, R.den); ++k) {
+                    rk *= r;
+                    if (k < R.num) P += R.c[k] * rk;
+                    if (k < R.den) Q += R.c[R.num + k] * rk;
+                }
+                double v, rp = 1.0;
+                if (q.index < R.num) {
+                    for (int k = 0; k <= q.index; ++k) rp *= r;
+                    v = rp / Q;
+                } else {
+                    for (int k = 0; k <= q.index - R.num; ++k) rp *= r;
+                    v = -P * rp / (Q * Q);
+                }
+                if (R.image) {
+                    if (r >= q.r_cut) return 0.0;
+                    double f, df, d2f;
+                    spline_eval(spl + q.sofs, q.n_knots, q.r_cut, q.inv_delta, r, f, df, d2f);
+                    v -= q.limit + f;
+                }
+                return v;
+            };
+            const bool same = F.ga == F.gb;
+            for (int a = M.g_first[F.ga] + lane; a < M.g_last[F.ga]; a += W)
+                for (int b = same ? a + 1 : M.g_first[F.gb]; b < M.g_last[F.gb]; ++b) {
+                    double d0 = pos[2 * a] - pos[2 * b], d1 = pos[2 * a + 1] - pos[2 * b + 1];
+                    t += dudp(dist(M.cell, d0, d1));
+                }
+            if (R.image)
+                for (int k = lane; k < M.nkj; k += W) {
+                    const double ar = rho[(F.ga * M.nkj + k) * 2], ai = rho[(F.ga * M.nkj + k) * 2 + 1];
+                    const double br = rho[(F.gb * M.nkj + k) * 2], bi = rho[(F.gb * M.nkj + k) * 2 + 1];
+                    t += (same ? 1.0 : 2.0) * wts[q.wofs + k] * (ar * br + ai * bi);
+                }
+        } else {
+            for (int p = M.g_first[q.group] + lane; p < M.g_last[q.group]; p += W) {
+                double d0 = pos[2 * p] - M.nos_site[p][0], d1 = pos[2 * p + 1] - M.nos_site[p][1];
+                if (M.cell.periodic) min_image(M.cell, d0, d1);
+                t -= d0 * d0 + d1 * d1;
+            }
+        }
+        t = wsum(t);
+        if (lane == 0) out[static_cast<size_t>(i) * stride + w] = t;
+    }
+}
+
 // Branching gather: new walker j takes old walker src[j]'s record.
 __global__ void k_gather(const double* __restrict__ src_d, double* __restrict__ dst_d, int stride,
                          const int* __restrict__ src, int n_new) {
@@ -2933,6 +3018,10 @@ void launch_measure(const DevModel* m, const DevState& s, int nw, bool mixed, bo
 void launch_delta_potential(const DevModel* m, const DevState& s, int n, int p, const double* prop, int stride,
                             double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_delta_potential, dim3(n), dim3(64), 0, st, m, s, p, prop, stride, out);
+}
+void launch_param_derivs(const DevModel* m, const DevState& s, int nw, const DevParam* prm, int n_prm,
+                         const double* wts, const double* spl, double* out, int stride, hipStream_t st) {
+    hipLaunchKernelGGL(k_param_derivs, dim3(nw), dim3(64), 0, st, m, s, prm, n_prm, wts, spl, out, stride);
 }
 void launch_gather(const double* src_d, double* dst_d, int stride, const int* src, int n_new,
                    hipStream_t st) {

@@ -156,6 +156,16 @@ struct DevState {
     int nb_max = 0;
 };
 
+// One free parameter of alea_parameter_derivatives on the device: a pair
+// factor's coefficient (pair >= 0) or a Nosanow factor's alpha (group >= 0),
+// with the image-sum tables of its derivative for a Padé summed over images
+// (weights at wofs in the weights buffer, the knot-form spline at sofs).
+struct DevParam {
+    int pair = -1, group = -1, index = 0;
+    int n_knots = 0, wofs = 0, sofs = 0;
+    double r_cut = 0, inv_delta = 0, limit = 0;
+};
+
 // Why an accepted move rebuilt a determinant inside the sweep
 // (slater_determinant.cpp:615-617,788-790,825-826): the index into
 // DevState::rebuilds.

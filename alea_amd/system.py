@@ -315,3 +315,26 @@ def uniform_positions(spec: Spec, nw: int, rng: np.random.Generator, box=1.0):
     a = np.array(spec.lattice).reshape(2, 2)
     pos = np.einsum("pin,id->pdn", f, a)
     return np.ascontiguousarray(pos.reshape(-1))
+
+
+def free_parameters(spec: Spec, free, param_tables=None) -> list:
+    """The free parameters of `spec`'s wavefunction for
+    Walkers.parameter_derivatives: `free` lists (factor, index) pairs, factors
+    counted in the order Walkers registers them (Padé factors, Drummond,
+    Nosanow, then the determinants and McMillan), indices as
+    alea_parameter_derivatives takes them.  A Padé summed over images carries
+    the breakup of its du/dp (breakup.pade_parameter_tables, or
+    param_tables(factor, index) when given, e.g. another fit's tables)."""
+    out = []
+    n_pade = len(spec.pade)
+    for f, i in free:
+        q = {"factor": int(f), "index": int(i)}
+        if f < n_pade and spec.pade[f].get("image"):
+            j = spec.pade[f]
+            if param_tables is not None:
+                q["tables"] = param_tables(f, i)
+            else:
+                c = list(j["coeffs"])
+                q["tables"] = breakup.pade_parameter_tables(spec.lattice, c[:j["num"]], c[j["num"]:], i)
+        out.append(q)
+    return out

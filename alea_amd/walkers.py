@@ -152,6 +152,27 @@ class Walkers:
         self.api("ratio_range", self.h, p, w_begin, w_end, ptr(npos), ptr(lr), ptr(ph), ptr(ng), ptr(nl))
         return out
 
+    def parameter_derivatives(self, params):
+        """TrialWavefunction::compute_parameter_derivatives for the free
+        parameters `params` (system.free_parameters): [n][nw] of
+        d log|psi| / d p."""
+        from ._capi import AleaParam
+        n = len(params)
+        arr = (AleaParam * max(n, 1))()
+        keep = []
+        for i, q in enumerate(params):
+            arr[i].factor, arr[i].index = q["factor"], q["index"]
+            t = q.get("tables")
+            if t:
+                w, sp = f64(t["w"]), f64(t["spline"])
+                keep += [w, sp]
+                arr[i].weights, arr[i].spline = ptr(w), ptr(sp)
+                arr[i].n_knots, arr[i].r_cut, arr[i].inv_delta, arr[i].limit = (
+                    t["n_knots"], t["r_cut"], t["inv_delta"], t["u_inf"])
+        out = np.zeros(max(n, 1) * self.nw)
+        self.api("parameter_derivatives", self.h, arr, n, ptr(out))
+        return out[:n * self.nw].reshape(n, self.nw)
+
     def delta_potential(self, p, new_pos, w_begin=0, w_end=-1):
         """Hamiltonian::delta_potential_particle for every walker of the range:
         V with particle p at new_pos [2][nw] minus V now, nothing committed."""

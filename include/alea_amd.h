@@ -157,6 +157,32 @@ int alea_accept_range(alea_system* s, int p, int w_begin, int w_end, const unsig
  * inverse powers take the moved particle's pairs. */
 int alea_delta_potential(alea_system* s, int p, int w_begin, int w_end, const double* new_pos, double* delta);
 
+/* One free (optimizable) parameter of the trial wavefunction.  factor counts
+ * the wavefunction factors in the order they were added (the reference's
+ * component order); index is the parameter within it: a Padé's coefficients
+ * a_1..a_N then b_1..b_M, a Drummond's coefficients alpha_0, alpha_2, ...
+ * (as alea_add_drummond_jastrow takes them; L stays fixed), a Nosanow's 0
+ * (alpha).  For a Padé summed over images, the breakup of du/dp minus its
+ * limit on the factor's k vectors (PeriodicPairSum's param_minus_limit,
+ * periodic_pair_sum.cpp:16-64): weights [n_k], spline [3 n_knots] in the
+ * knot form of alea_add_pade_jastrow, r_cut, 1/delta and the limit;
+ * otherwise the pointers may be NULL. */
+typedef struct alea_param {
+    int factor, index;
+    const double* weights;
+    const double* spline;
+    int n_knots;
+    double r_cut, inv_delta, limit;
+} alea_param;
+
+/* TrialWavefunction::compute_parameter_derivatives
+ * (src/wavefunction/trial_wavefunction.cpp:150-160), for the n parameters
+ * listed: derivs[i * nw + w] = d log|psi| / d p_i of walker w, after
+ * evaluate or sweeps.  Per factor as PadeJastrow (pade_jastrow.cpp:641-700),
+ * DrummondJastrow (drummond_jastrow.cpp:200-300) and NosanowJastrow compute
+ * it; determinants have no free parameters here. */
+int alea_parameter_derivatives(alea_system* s, const alea_param* params, int n, double* derivs);
+
 /* Fused sweeps: VMC::sweep / sweep_walkers (vmc.cpp:69-155) with symmetric
  * Metropolis proposals of width sqrt(tau/m).  tape: [sweep][np][dim+1][nw]
  * random numbers in draw_sweep_randoms order (sweep_blocking.h:39-48), or NULL

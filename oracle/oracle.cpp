@@ -2320,6 +2320,113 @@ int oracle_delta_potential(oracle_system* h, int p, int w_begin, int w_end, cons
     });
 }
 
+// TrialWavefunction::compute_parameter_derivatives (trial_wavefunction.cpp:
+// 150-160) for the listed free parameters: derivs[i * nw + w] = d log|psi| /
+// d p_i of walker w.  A pair Jastrow's derivative sums du/dp over its pairs at
+// the minimum-image distance (PadeJastrow and DrummondJastrow::
+// compute_parameter_derivatives, pade_jastrow.cpp:641-700,
+// drummond_jastrow.cpp:200-300): Padé du/da_k = r^k/Q, du/db_k = -P r^k/Q^2;
+// Drummond du/dalpha_0 = (r - L)^C (1 + C r/L) (alpha_1 follows alpha_0
+// through the cusp), du/dalpha_l = (r - L)^C r^l, inside the cutoff.  Summed
+// over images (periodic_pair_sum.cpp:85-98, kspace_pair_sum.cpp:303-313) the
+// pair part takes du/dp - limit - spline(r) inside r_cut, and the
+// reciprocal part sum_k w_k |rho_a|^2 (one group) or 2 w_k Re(rho_a rho_b*)
+// with the parameter's weights.  Nosanow (nosanow_jastrow.cpp): d/dalpha =
+// -sum_i |r_i - R_i|^2.
+int oracle_parameter_derivatives(oracle_system* h, const oracle_param* params, int n, double* derivs) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        const std::size_t nw = s.nw;
+        for (int i = 0; i < n; ++i) {
+            const oracle_param& q = params[i];
+            if (q.factor < 0 || static_cast<std::size_t>(q.factor) >= s.comps.size())
+                throw std::invalid_argument("oracle: factor index out of range");
+            double* out = derivs + static_cast<std::size_t>(i) * nw;
+            std::fill(out, out + nw, 0.0);
+            Component* c = s.comps[static_cast<std::size_t>(q.factor)].get();
+            if (auto* pj = dynamic_cast<PairJastrow*>(c)) {
+                const Radial& R = pj->rad;
+                if (R.mcmillan) throw std::invalid_argument("oracle: the McMillan factor has no free parameters");
+                const int np_ = R.drummond ? static_cast<int>(R.c.size()) : R.num + R.den;
+                if (q.index < 0 || q.index >= np_) throw std::invalid_argument("oracle: parameter index out of range");
+                Spline sp;
+                if (R.image) {
+                    if (!q.weights || !q.spline) throw std::invalid_argument("oracle: image-summed parameter needs its tables");
+                    sp.scaled.assign(q.spline, q.spline + 3 * q.n_knots);
+                    sp.n_knots = q.n_knots;
+                    sp.r_cut = q.r_cut;
+                    sp.inv_delta = q.inv_delta;
+                }
+                auto dudp = [&](double r) {
+                    if (R.drummond) {
+                        if (r >= R.L) return 0.0;
+                        double g = 1.0;
+                        for (int k = 0; k < R.C; ++k) g *= r - R.L;
+                        if (q.index == 0) return g * (1.0 + static_cast<double>(R.C) * r / R.L);
+                        double rl = 1.0;
+                        for (int k = 0; k <= q.index; ++k) rl *= r;
+                        return g * rl;
+                    }
+                    double P = 0.0, Q = 1.0, rk = 1.0;
+                    for (int k = 0; k < std::max(R.num, R.den); ++k) {
+                        rk *= r;
+                        if (k < R.num) P += R.c[static_cast<std::size_t>(k)] * rk;
+                        if (k < R.den) Q += R.c[static_cast<std::size_t>(R.num + k)] * rk;
+                    }
+                    double v;
+                    if (q.index < R.num) {
+                        double rp = 1.0;
+                        for (int k = 0; k <= q.index; ++k) rp *= r;
+                        v = rp / Q;
+                    } else {
+                        double rp = 1.0;
+                        for (int k = 0; k <= q.index - R.num; ++k) rp *= r;
+                        v = -P * rp / (Q * Q);
+                    }
+                    if (R.image) v = r >= q.r_cut ? 0.0 : v - q.limit - sp.value(r);
+                    return v;
+                };
+                auto pair = [&](std::size_t a, std::size_t b) {
+                    for (std::size_t w = 0; w < nw; ++w) out[w] += dudp(s.D(a, b, w));
+                };
+                if (pj->same) {
+                    for (std::size_t a = pj->ga.first; a < pj->ga.last; ++a)
+                        for (std::size_t b = a + 1; b < pj->ga.last; ++b) pair(a, b);
+                } else {
+                    for (std::size_t a = pj->ga.first; a < pj->ga.last; ++a)
+                        for (std::size_t b = pj->gb.first; b < pj->gb.last; ++b) pair(a, b);
+                }
+                if (R.image) {
+                    const KSpace& k = pj->ks;
+                    const std::size_t K = k.nk();
+                    for (std::size_t w = 0; w < nw; ++w) {
+                        const double* ar = k.ra_re.data() + w * K;
+                        const double* ai = k.ra_im.data() + w * K;
+                        const double* br = k.same ? ar : k.rb_re.data() + w * K;
+                        const double* bi = k.same ? ai : k.rb_im.data() + w * K;
+                        double t = 0.0;
+                        for (std::size_t kk = 0; kk < K; ++kk)
+                            t += k.same ? q.weights[kk] * (ar[kk] * ar[kk] + ai[kk] * ai[kk])
+                                        : 2.0 * q.weights[kk] * (ar[kk] * br[kk] + ai[kk] * bi[kk]);
+                        out[w] += t;
+                    }
+                }
+            } else if (auto* ns = dynamic_cast<Nosanow*>(c)) {
+                if (q.index != 0) throw std::invalid_argument("oracle: the Nosanow factor has one parameter");
+                for (std::size_t p = ns->g.first; p < ns->g.last; ++p)
+                    for (std::size_t w = 0; w < nw; ++w) {
+                        double d0 = s.P(p, 0, w) - ns->sites[(p - ns->g.first) * 2];
+                        double d1 = s.P(p, 1, w) - ns->sites[(p - ns->g.first) * 2 + 1];
+                        if (s.cell.periodic) s.cell.min_image(d0, d1);
+                        out[w] -= d0 * d0 + d1 * d1;
+                    }
+            } else {
+                throw std::invalid_argument("oracle: this factor has no free parameters here");
+            }
+        }
+    });
+}
+
 int oracle_accept_range(oracle_system* h, int p, int w_begin, int w_end, const unsigned char* mask) {
     return guarded([&] {
         Sys& s = *S(h);

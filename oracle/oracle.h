@@ -97,6 +97,14 @@ int oracle_ratio_range(oracle_system* s, int p, int w_begin, int w_end, const do
                        double* phase, double* new_grad, double* new_lap);
 int oracle_accept_range(oracle_system* s, int p, int w_begin, int w_end, const unsigned char* mask);
 int oracle_delta_potential(oracle_system* s, int p, int w_begin, int w_end, const double* new_pos, double* delta);
+typedef struct oracle_param {
+    int factor, index;
+    const double* weights;
+    const double* spline;
+    int n_knots;
+    double r_cut, inv_delta, limit;
+} oracle_param;
+int oracle_parameter_derivatives(oracle_system* s, const oracle_param* params, int n, double* derivs);
 int oracle_refresh_inverse(oracle_system* s, int w_begin, int w_end);
 int oracle_check_status(oracle_system* s);
 int oracle_dmc_begin(oracle_system* s);

@@ -82,6 +82,11 @@ struct RefSystem {
     std::vector<CoulombPair> coulomb;
     struct InversePowerSpec { int a, b; double c, p; };
     std::vector<InversePowerSpec> ipow;
+    // free (optimizable) flags for the next factors' parameters, consumed in
+    // order by the add calls; parameters past the list stay fixed
+    std::vector<unsigned char> free_next;
+    std::size_t free_used = 0;
+    bool take_free() { return free_used < free_next.size() && free_next[free_used++] != 0; }
 };
 
 alea::ParticleGroup make_group(std::size_t first, std::size_t last, double charge,
@@ -263,7 +268,7 @@ int ref_add_pade_jastrow(void* h, int ga, int gb, int num_order, int den_order,
         auto& s = *static_cast<RefSystem*>(h);
         std::vector<alea::Parameter> params;
         for (int k = 0; k < num_order + den_order; ++k)
-            params.push_back(alea::Parameter{"c" + std::to_string(k), coeffs[k], true});
+            params.push_back(alea::Parameter{"c" + std::to_string(k), coeffs[k], !s.take_free()});
         auto c = std::make_unique<alea::PadeJastrow>(num_order, den_order,
                                                      s.groups[static_cast<std::size_t>(ga)],
                                                      s.groups[static_cast<std::size_t>(gb)],
@@ -279,7 +284,7 @@ int ref_add_drummond_jastrow(void* h, int ga, int gb, int C, double gamma, doubl
         auto& s = *static_cast<RefSystem*>(h);
         std::vector<alea::Parameter> params;
         for (int k = 0; k < n_coeffs; ++k)
-            params.push_back(alea::Parameter{"a" + std::to_string(k), coeffs[k], true});
+            params.push_back(alea::Parameter{"a" + std::to_string(k), coeffs[k], !s.take_free()});
         params.push_back(alea::Parameter{"L", L, true});
         s.psi.add_component(std::make_unique<alea::DrummondJastrow>(
             C, gamma, s.groups[static_cast<std::size_t>(ga)],
@@ -292,7 +297,7 @@ int ref_add_nosanow_jastrow(void* h, int g, double alpha, const double* sites) {
         auto& s = *static_cast<RefSystem*>(h);
         const auto& grp = s.groups[static_cast<std::size_t>(g)];
         std::vector<double> st(sites, sites + grp.size() * s.dim);
-        std::vector<alea::Parameter> params{alea::Parameter{"alpha", alpha, true}};
+        std::vector<alea::Parameter> params{alea::Parameter{"alpha", alpha, !s.take_free()}};
         s.psi.add_component(std::make_unique<alea::NosanowJastrow>(
             s.dim, grp.first, std::move(st), s.cell, std::move(params)));
     });
@@ -381,6 +386,28 @@ int ref_local_energy(void* h, double* e_re) {
         s.ham.evaluate_batch(*s.ws);
         auto le = s.ws->local_energies();
         std::copy(le.begin(), le.end(), e_re);
+    });
+}
+
+// Free flags for the parameters of the factors added next, in the order they
+// are added (Padé: its coefficients; Drummond: its coefficients, L stays
+// fixed; Nosanow: alpha).
+int ref_set_free(void* h, int n, const unsigned char* flags) {
+    auto& s = *static_cast<RefSystem*>(h);
+    s.free_next.assign(flags, flags + n);
+    s.free_used = 0;
+    return 0;
+}
+
+// TrialWavefunction::compute_parameter_derivatives (trial_wavefunction.cpp:
+// 150-160) after evaluate: derivs [n_opt][nw]; *n_opt the count.
+int ref_parameter_derivatives(void* h, double* derivs, int cap, int* n_opt) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        const std::size_t n = s.psi.total_optimizable_params();
+        *n_opt = static_cast<int>(n);
+        if (static_cast<int>(n * s.nw) > cap) throw std::runtime_error("derivs too small");
+        s.psi.compute_parameter_derivatives(*s.ws, std::span<double>(derivs, n * s.nw));
     });
 }
 

@@ -132,6 +132,8 @@ def ref_lib():
             "ref_robust_spread": (C.c_double, [D, C.c_int]),
             "ref_add_inverse_power": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double]),
             "ref_delta_potential": (C.c_int, [C.c_void_p, C.c_int, D, D]),
+            "ref_set_free": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_ubyte)]),
+            "ref_parameter_derivatives": (C.c_int, [C.c_void_p, D, C.c_int, C.POINTER(C.c_int)]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(lib, name)
@@ -144,13 +146,18 @@ class RefWalkers:
     """The reference's own objects behind the Walkers interface (Jastrow,
     Nosanow, Ewald and Coulomb factors; no Slater: Eigen is absent)."""
 
-    def __init__(self, spec: Spec, nw: int, ewald_tol: float | None = None):
+    def __init__(self, spec: Spec, nw: int, ewald_tol: float | None = None, free_flags=None):
         lib = ref_lib()
         self.lib, self.spec, self.nw, self.np = lib, spec, nw, spec.n_particles
         h = C.c_void_p()
         lat = None if spec.lattice is None else f64(spec.lattice)
         self._chk(lib.ref_create(C.byref(h), nw, self.np, 2, ptr(lat)))
         self.h = h
+        if free_flags is not None:
+            # free flags of the factors' parameters in the order they are
+            # added (Padé coefficients, Drummond coefficients, Nosanow alpha)
+            fl = u8(free_flags)
+            self._chk(lib.ref_set_free(h, len(fl), ptr(fl, _P_U8)))
         for first, last, q, im in spec.groups:
             lib.ref_add_group(h, first, last, q, im)
         for j in spec.pade:
@@ -216,6 +223,13 @@ class RefWalkers:
     def accept(self, p, mask):
         m = u8(mask)
         self._chk(self.lib.ref_accept(self.h, p, ptr(m, _P_U8)))
+
+    def parameter_derivatives(self):
+        n = C.c_int()
+        cap = 64 * self.nw
+        out = np.zeros(cap)
+        self._chk(self.lib.ref_parameter_derivatives(self.h, ptr(out), cap, C.byref(n)))
+        return out[:n.value * self.nw].reshape(n.value, self.nw)
 
     def delta_potential(self, p, new_pos):
         npos = f64(new_pos)

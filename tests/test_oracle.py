@@ -238,6 +238,142 @@ def _jastrow_ewald_specs():
     return specs
 
 
+def _derivative_num_den(a, b, q):
+    """du/dp of u = P/Q as (numerator, denominator) coefficient lists, as
+    PadeJastrow builds them (pade_jastrow.cpp:227-244)."""
+    if q < len(a):
+        return [0.0] * q + [1.0], list(b)
+    k = q - len(a) + 1
+    num = [0.0] * (len(a) + k)
+    for j in range(len(a)):
+        num[j + k] = -a[j]
+    pq = np.polynomial.Polynomial([1.0] + list(b))
+    return num, list((pq * pq).coef[1:])
+
+
+def _ref_param_tables(lattice, a, b, q):
+    """The reference's own breakup of du/dp (numerics::build_long_range_breakup
+    through oracle/_ref)."""
+    import ctypes as C
+    from backends import ref_lib
+    num, den = _derivative_num_den(a, b, q)
+    lib = ref_lib()
+    cap, capn = 512, 256
+    nk, nkn = C.c_int(), C.c_int()
+    n1, n2 = (C.c_int * cap)(), (C.c_int * cap)()
+    kx, ky, w = (C.c_double * cap)(), (C.c_double * cap)(), (C.c_double * cap)()
+    sc, scal = (C.c_double * (3 * capn))(), (C.c_double * 3)()
+    fn = lib.ref_pade_breakup
+    fn.restype = C.c_int
+    rc = fn((C.c_double * 4)(*lattice), len(num), (C.c_double * len(num))(*num), len(den),
+            (C.c_double * len(den))(*den), cap, C.byref(nk), n1, n2, kx, ky, w, capn, C.byref(nkn), sc, scal)
+    assert rc == 0
+    n, m = nk.value, nkn.value
+    return dict(w=list(w[:n]), spline=list(sc[:3 * m]), n_knots=m, r_cut=scal[0], inv_delta=scal[1],
+                u_inf=scal[2])
+
+
+def _free_cases():
+    """(spec, free (factor, index) pairs) with every parameter kind free: the
+    image-summed Padé's b (both spin pairs, the reference's tables), its a
+    too on one factor; Drummond alpha_0 and alpha_2 and Nosanow alpha; a
+    minimum-image Padé's a and b on the dipoles."""
+    out = []
+    s = heg_spec(5, 5, 2.0, slater=False, tables=_ref_tables(10, 2.0))
+    s.name = "HEG N=10 image-summed Padé"
+    out.append((s, [(0, 1), (1, 0), (1, 1)]))
+    dr = heg_spec(5, 5, 2.0, slater=False, jastrow=False, tables=_ref_tables(10, 2.0))
+    L = dr.lattice[0] / 2
+    dr.drummond.append(dict(ga=0, gb=1, C=3, gamma=1.0, L=L, coeffs=[1e-3, -2e-4, 3e-5]))
+    dr.nosanow.append(dict(group=1, alpha=0.6, sites=np.random.default_rng(9).uniform(-5, 5, 10)))
+    dr.name = "drummond+nosanow N=10"
+    out.append((dr, [(0, 0), (0, 2), (1, 0)]))
+    dp = dipole_spec(16, density=4.0)
+    dp.mcmillan = []
+    rs = 1.0 / np.sqrt(np.pi * 4.0)
+    dp.pade.append(dict(ga=0, gb=0, num=1, den=1, coeffs=[0.3, 1.0 / rs]))
+    dp.name = "dipoles N=16 minimum-image Padé"
+    out.append((dp, [(0, 0), (0, 1)]))
+    return out
+
+
+def _free_flags(spec, free):
+    """ref_set_free's flags in factor order (Padé coefficients, Drummond
+    coefficients, Nosanow alpha)."""
+    sizes = [j["num"] + j["den"] for j in spec.pade] + [len(d["coeffs"]) for d in spec.drummond] + \
+        [1 for _ in spec.nosanow]
+    flags = []
+    for f, n in enumerate(sizes):
+        flags += [1 if (f, i) in free else 0 for i in range(n)]
+    return flags
+
+
+@pytest.mark.parametrize("case", _free_cases(), ids=lambda c: c[0].name)
+def test_parameter_derivatives_match_reference(case):
+    """TrialWavefunction::compute_parameter_derivatives of the oracle against
+    the reference's own (PadeJastrow with PeriodicPairSum's parameter
+    breakups, DrummondJastrow, NosanowJastrow, compiled in oracle/_ref), on
+    fresh walkers and after sweeps, to 1e-10 relative; the image-summed
+    factors take the reference's breakups of du/dp."""
+    from alea_amd.system import free_parameters
+    spec, free = case
+    nw = 6
+    pos = uniform_positions(spec, nw, np.random.default_rng(33))
+    tables = lambda f, i: _ref_param_tables(spec.lattice, spec.pade[f]["coeffs"][:spec.pade[f]["num"]],
+                                            spec.pade[f]["coeffs"][spec.pade[f]["num"]:], i)
+    params = free_parameters(spec, sorted(free), tables)
+    o, r = oracle_walkers(spec, nw), RefWalkers(spec, nw, free_flags=_free_flags(spec, free))
+    for x in (o, r):
+        x.set_positions(pos)
+        x.evaluate()
+    do, dr_ = o.parameter_derivatives(params), r.parameter_derivatives()
+    assert do.shape == dr_.shape
+    assert rel_err(do, dr_) < 1e-10
+    tape = oracle_tape(51, spec.n_particles, nw, 2)
+    tau = (spec.lattice[0] / 8) ** 2
+    assert o.vmc_sweeps(2, tau, tape) == r.vmc_sweeps(2, tau, tape)
+    assert rel_err(o.parameter_derivatives(params), r.parameter_derivatives()) < 1e-10
+
+
+def test_parameter_derivatives_are_the_slope_of_log_psi():
+    """Without images the derivative is the slope of log|psi| itself: central
+    differences of the oracle's log|psi| in each Padé coefficient, Drummond
+    coefficient and the Nosanow alpha agree to 1e-6."""
+    from alea_amd.system import free_parameters
+    import copy
+    for spec, free in _free_cases()[1:]:
+        nw = 4
+        pos = uniform_positions(spec, nw, np.random.default_rng(34))
+        o = oracle_walkers(spec, nw)
+        o.set_positions(pos)
+        o.evaluate()
+        d = o.parameter_derivatives(free_parameters(spec, sorted(free)))
+        n_pade, n_dr = len(spec.pade), len(spec.drummond)
+        for row, (f, i) in enumerate(sorted(free)):
+            lp = []
+            for sgn in (1.0, -1.0):
+                t = copy.deepcopy(spec)
+                if f < n_pade:
+                    c = list(t.pade[f]["coeffs"])
+                    h = 1e-6 * max(abs(c[i]), 1.0)
+                    c[i] += sgn * h
+                    t.pade[f]["coeffs"] = c
+                elif f < n_pade + n_dr:
+                    c = list(t.drummond[f - n_pade]["coeffs"])
+                    h = 1e-6 * max(abs(c[i]), 1e-3)
+                    c[i] += sgn * h
+                    t.drummond[f - n_pade]["coeffs"] = c
+                else:
+                    h = 1e-6
+                    t.nosanow[f - n_pade - n_dr]["alpha"] += sgn * h
+                x = oracle_walkers(t, nw)
+                x.set_positions(pos)
+                x.evaluate()
+                lp.append(x.log_psi())
+            fd = (lp[0] - lp[1]) / (2 * h)
+            assert np.allclose(d[row], fd, rtol=1e-6, atol=1e-6 * np.abs(fd).max()), (spec.name, f, i)
+
+
 @pytest.mark.parametrize("spec", _jastrow_ewald_specs(), ids=lambda s: s.name)
 def test_delta_potential_matches_reference(spec):
     """Hamiltonian::delta_potential_particle (hamiltonian.cpp:105-117) of the
