@@ -54,6 +54,7 @@ SIGNATURES = {
     "accept_range": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_U8]),
     "delta_potential": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_D, _P_D]),
     "parameter_derivatives": (C.c_int, [_V, C.POINTER(AleaParam), C.c_int, _P_D]),
+    "local_energy_parameter_derivatives": (C.c_int, [_V, C.POINTER(AleaParam), C.c_int, _P_D, _P_D]),
     "vmc_sweeps": (C.c_int, [_V, C.c_int, C.c_double, _P_D, _P_LL]),
     "snapshots_create": (C.c_int, [_V, C.c_int, C.POINTER(C.c_void_p)]),
     "snapshots_destroy": (C.c_int, [_V]),

@@ -30,8 +30,13 @@ void launch_refresh(const DevModel* m, const DevState& s, int nw, int n_blocks, 
                     bool cplx, int min_updates, hipStream_t st, bool mfma);
 void launch_delta_potential(const DevModel* m, const DevState& s, int n, int p, const double* prop, int stride,
                             double* out, hipStream_t st);
-void launch_param_derivs(const DevModel* m, const DevState& s, int nw, const DevParam* prm, int n_prm,
-                         const double* wts, const double* spl, double* out, int stride, hipStream_t st);
+void launch_param_derivs(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, const DevParam* prm,
+                         int n_prm, const double* wts, const double* spl, double* out, int stride, hipStream_t st);
+void launch_param_eloc(const DevModel* m, const DevState& s, int nw, const DevParam* prm, int n_prm,
+                       const double* wts, const double* spl, const double* grad, const double* pgrad,
+                       double* out_re, double* out_im, int stride, hipStream_t st);
+void launch_gauss_eloc(const DevModel* m, const DevState& s, bool mixed, int b, double alpha, const double* grad,
+                       double* ybuf, int w0, int n_walkers, double* out, hipStream_t st);
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
                   bool cplx, int rows_max, size_t lds, hipStream_t st);
 void launch_draw(const DevState& s, int np, int nw, double* out, int stride, hipStream_t st);
@@ -767,6 +772,7 @@ int alea_add_drummond_jastrow(alea_system* s, int ga, int gb, int C, double gamm
         double ipw = 1.0;
         for (int i = 0; i < C; ++i) ipw *= -L;
         P.rad.alpha1 = gamma / ipw + coeffs[0] * static_cast<double>(C) / L;
+        P.rad.gamma = gamma;
         s->factors.push_back({0, m.n_pair});
         ++m.n_pair;
     });
@@ -1264,66 +1270,135 @@ int alea_delta_potential(alea_system* s, int p, int w_begin, int w_end, const do
     });
 }
 
+// The listed parameters resolved against the system's factors, packed for
+// the derivative kernels, with the breakup tables of image-summed ones
+// gathered into two device arrays.
+struct PackedParams {
+    std::vector<DevParam> host;
+    DevBuf<DevParam> prm;
+    DevBuf<double> w, spl;
+};
+
+static void pack_params(alea_system* s, const alea_param* params, int n, PackedParams& out) {
+    const DevModel& m = s->host;
+    std::vector<DevParam>& dp = out.host;
+    dp.assign(static_cast<size_t>(n), DevParam{});
+    std::vector<double> wts(1, 0.0), spl(1, 0.0);
+    for (int i = 0; i < n; ++i) {
+        const alea_param& q = params[i];
+        if (q.factor < 0 || q.factor >= static_cast<int>(s->factors.size()))
+            throw InvalidArg("factor index out of range");
+        const auto [kind, idx] = s->factors[static_cast<size_t>(q.factor)];
+        DevParam& d = dp[static_cast<size_t>(i)];
+        d.index = q.index;
+        if (kind == 0) {
+            const DevRadial& R = m.pair[idx].rad;
+            if (R.kind == 2) throw InvalidArg("the McMillan factor has no free parameters");
+            const int n_par = R.kind == 1 ? R.num + 1 : R.num + R.den;
+            if (q.index < 0 || q.index >= n_par) throw InvalidArg("parameter index out of range");
+            d.pair = idx;
+            if (R.kind == 1 && q.index == R.num) {
+                double ipw = 1.0;   // (-L)^(C+1)
+                for (int k = 0; k <= R.C; ++k) ipw *= -R.L;
+                d.da1_dl = R.gamma * R.C / ipw - R.c[0] * R.C / (R.L * R.L);
+            }
+            if (R.image) {
+                if (!q.weights || !q.spline) throw InvalidArg("a parameter of an image-summed factor needs its tables");
+                if (q.n_knots < 2 || q.n_knots > kMaxKnots) throw InvalidArg("parameter spline: knot count out of range");
+                d.wofs = static_cast<int>(wts.size());
+                wts.insert(wts.end(), q.weights, q.weights + m.nkj);
+                d.sofs = static_cast<int>(spl.size());
+                spl.insert(spl.end(), q.spline, q.spline + 3 * q.n_knots);
+                d.n_knots = q.n_knots;
+                d.r_cut = q.r_cut;
+                d.inv_delta = q.inv_delta;
+                d.limit = q.limit;
+            }
+        } else if (kind == 1) {
+            if (q.index != 0) throw InvalidArg("the Nosanow factor has one parameter");
+            d.group = idx;
+        } else {
+            // GaussianOrbitalSet's shared width (orbital_set.h:228-247): only a
+            // block whose Gaussians all have one width has it
+            bool uniform = m.blk_kind[idx] == kGaussian && !s->cplx;
+            for (int k = 1; uniform && k < m.blk_n[idx]; ++k) uniform = m.orb_alpha[idx][k] == m.orb_alpha[idx][0];
+            if (!uniform) throw InvalidArg("this determinant has no free parameters");
+            if (q.index != 0) throw InvalidArg("parameter index out of range");
+            d.block = idx;
+            d.alpha = m.orb_alpha[idx][0];
+        }
+    }
+    out.prm.alloc(dp.size(), s->stream);
+    out.w.alloc(wts.size(), s->stream);
+    out.spl.alloc(spl.size(), s->stream);
+    hip_check(hipMemcpyAsync(out.prm.p, dp.data(), dp.size() * sizeof(DevParam), hipMemcpyHostToDevice, s->stream),
+              "parameters");
+    hip_check(hipMemcpyAsync(out.w.p, wts.data(), wts.size() * sizeof(double), hipMemcpyHostToDevice, s->stream),
+              "weights");
+    hip_check(hipMemcpyAsync(out.spl.p, spl.data(), spl.size() * sizeof(double), hipMemcpyHostToDevice, s->stream),
+              "splines");
+}
+
 int alea_parameter_derivatives(alea_system* s, const alea_param* params, int n, double* derivs) {
     return guarded([&] {
         Sys(s);
         if (!s->evaluated) throw InvalidArg("evaluate first");
         if (n < 0 || (n > 0 && (!params || !derivs))) throw InvalidArg("null parameters or output");
         if (n == 0) return;
-        const DevModel& m = s->host;
-        std::vector<DevParam> dp(static_cast<size_t>(n));
-        std::vector<double> wts(1, 0.0), spl(1, 0.0);
-        for (int i = 0; i < n; ++i) {
-            const alea_param& q = params[i];
-            if (q.factor < 0 || q.factor >= static_cast<int>(s->factors.size()))
-                throw InvalidArg("factor index out of range");
-            const auto [kind, idx] = s->factors[static_cast<size_t>(q.factor)];
-            DevParam& d = dp[static_cast<size_t>(i)];
-            d.index = q.index;
-            if (kind == 0) {
-                const DevRadial& R = m.pair[idx].rad;
-                if (R.kind == 2) throw InvalidArg("the McMillan factor has no free parameters");
-                const int n_par = R.kind == 1 ? R.num : R.num + R.den;
-                if (q.index < 0 || q.index >= n_par) throw InvalidArg("parameter index out of range");
-                d.pair = idx;
-                if (R.image) {
-                    if (!q.weights || !q.spline) throw InvalidArg("a parameter of an image-summed factor needs its tables");
-                    if (q.n_knots < 2 || q.n_knots > kMaxKnots) throw InvalidArg("parameter spline: knot count out of range");
-                    d.wofs = static_cast<int>(wts.size());
-                    wts.insert(wts.end(), q.weights, q.weights + m.nkj);
-                    d.sofs = static_cast<int>(spl.size());
-                    spl.insert(spl.end(), q.spline, q.spline + 3 * q.n_knots);
-                    d.n_knots = q.n_knots;
-                    d.r_cut = q.r_cut;
-                    d.inv_delta = q.inv_delta;
-                    d.limit = q.limit;
-                }
-            } else if (kind == 1) {
-                if (q.index != 0) throw InvalidArg("the Nosanow factor has one parameter");
-                d.group = idx;
-            } else {
-                throw InvalidArg("this determinant has no free parameters");
-            }
-        }
+        PackedParams pk;
+        pack_params(s, params, n, pk);
         const size_t nw = static_cast<size_t>(s->nw);
-        DevBuf<DevParam> d_prm;
-        DevBuf<double> d_w, d_s, d_out;
-        d_prm.alloc(dp.size(), s->stream);
-        d_w.alloc(wts.size(), s->stream);
-        d_s.alloc(spl.size(), s->stream);
+        DevBuf<double> d_out;
         d_out.alloc(static_cast<size_t>(n) * nw, s->stream);
-        hip_check(hipMemcpyAsync(d_prm.p, dp.data(), dp.size() * sizeof(DevParam), hipMemcpyHostToDevice, s->stream),
-                  "parameters");
-        hip_check(hipMemcpyAsync(d_w.p, wts.data(), wts.size() * sizeof(double), hipMemcpyHostToDevice, s->stream),
-                  "weights");
-        hip_check(hipMemcpyAsync(d_s.p, spl.data(), spl.size() * sizeof(double), hipMemcpyHostToDevice, s->stream),
-                  "splines");
         s->set_model_nw(s->nw);
-        launch_param_derivs(s->dmodel, s->st, s->nw, d_prm.p, n, d_w.p, d_s.p, d_out.p, static_cast<int>(nw),
-                            s->stream);
+        launch_param_derivs(s->dmodel, s->st, s->nw, s->mixed(), s->cplx, pk.prm.p, n, pk.w.p, pk.spl.p, d_out.p,
+                            static_cast<int>(nw), s->stream);
         hip_check(hipGetLastError(), "launch parameter derivatives");
         hip_check(copy_on(s->stream, derivs, d_out.p, static_cast<size_t>(n) * nw * sizeof(double),
                           hipMemcpyDeviceToHost), "derivatives");
+    });
+}
+
+int alea_local_energy_parameter_derivatives(alea_system* s, const alea_param* params, int n, double* de_re,
+                                            double* de_im) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->evaluated) throw InvalidArg("evaluate first");
+        if (n < 0 || (n > 0 && (!params || !de_re))) throw InvalidArg("null parameters or output");
+        if (n == 0) return;
+        PackedParams pk;
+        pack_params(s, params, n, pk);
+        const size_t nw = static_cast<size_t>(s->nw), npnw = static_cast<size_t>(s->np) * 2 * nw;
+        // every particle's gradient (and phase gradient) of the whole trial
+        measure(s, true);
+        const double* grad = s->grad_buf.p;
+        const double* pgrad = s->cplx ? s->grad_buf.p + 2 * npnw : nullptr;
+        DevBuf<double> d_re, d_im;
+        d_re.alloc(static_cast<size_t>(n) * nw, s->stream);
+        d_im.alloc(static_cast<size_t>(n) * nw, s->stream);
+        launch_param_eloc(s->dmodel, s->st, s->nw, pk.prm.p, n, pk.w.p, pk.spl.p, grad, pgrad, d_re.p, d_im.p,
+                          static_cast<int>(nw), s->stream);
+        hip_check(hipGetLastError(), "launch local-energy parameter derivatives");
+        DevBuf<double> ybuf;
+        for (int i = 0; i < n; ++i) {
+            const DevParam& q = pk.host[static_cast<size_t>(i)];
+            if (q.block < 0) continue;
+            // dD A for a chunk of walkers at a time, n^2 doubles each, within 256 MiB
+            const size_t nb = static_cast<size_t>(s->host.blk_n[q.block]);
+            const int chunk = static_cast<int>(std::max<size_t>(1, std::min(nw, (size_t{256} << 20) / (nb * nb * 8))));
+            if (ybuf.n < static_cast<size_t>(chunk) * nb * nb) ybuf.alloc(static_cast<size_t>(chunk) * nb * nb, s->stream);
+            for (int w0 = 0; w0 < s->nw; w0 += chunk) {
+                const int cnt = std::min(chunk, s->nw - w0);
+                launch_gauss_eloc(s->dmodel, s->st, s->mixed(), q.block, q.alpha, grad, ybuf.p, w0, cnt,
+                                  d_re.p + static_cast<size_t>(i) * nw, s->stream);
+                hip_check(hipGetLastError(), "launch Gaussian width derivative");
+            }
+        }
+        hip_check(copy_on(s->stream, de_re, d_re.p, static_cast<size_t>(n) * nw * sizeof(double),
+                          hipMemcpyDeviceToHost), "dE/dp");
+        if (de_im)
+            hip_check(copy_on(s->stream, de_im, d_im.p, static_cast<size_t>(n) * nw * sizeof(double),
+                              hipMemcpyDeviceToHost), "dE_im/dp");
     });
 }
 

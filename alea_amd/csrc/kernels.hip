@@ -2677,17 +2677,138 @@ __global__ __launch_bounds__(64) void k_delta_potential(const DevModel* __restri
     if (lane == 0) out[w] = delta;
 }
 
+// ── optimizer derivatives ──────────────────────────────────────────────────
+// f = du/dp of a pair factor's parameter at distance r, with f' and f''.
+// Padé (PadeJastrow, pade_jastrow.cpp:641-700, 806-880): du/da_k = r^e/Q
+// (e = k+1), du/db_k = -P r^e/Q^2; summed over images the short-range part
+// takes f - limit - spline(r) inside r_cut (periodic_pair_sum.cpp:85-98).
+// Drummond (drummond_jastrow.cpp:201-420), inside the cutoff, with
+// t = r - L and g = t^C: du/dalpha_0 = g (1 + C r/L) (alpha_1 follows
+// alpha_0 through the cusp), du/dalpha_l = g r^l, du/dL = -C t^(C-1) P +
+// g r dalpha_1/dL.
+__device__ inline void param_radial(const DevRadial& R, const DevParam& q, const double* __restrict__ spl, double r,
+                                    double& f, double& f1, double& f2) {
+    f = f1 = f2 = 0.0;
+    auto ip = [](double x, int e) {
+        double y = 1.0;
+        for (int i = 0; i < e; ++i) y *= x;
+        return y;
+    };
+    if (R.kind == 1) {
+        if (r >= R.L) return;
+        const int C = R.C;
+        const double Cd = static_cast<double>(C), t = r - R.L;
+        const double g = ip(t, C), g1 = Cd * ip(t, C - 1), g2 = Cd * (C - 1) * ip(t, C - 2);
+        if (q.index == 0) {
+            const double k = Cd / R.L, m = 1.0 + k * r;
+            f = g * m;
+            f1 = g1 * m + g * k;
+            f2 = g2 * m + 2.0 * g1 * k;
+        } else if (q.index < R.num) {
+            const int l = q.index + 1;
+            const double m = ip(r, l), m1 = l * ip(r, l - 1), m2 = l * (l - 1) * ip(r, l - 2);
+            f = g * m;
+            f1 = g1 * m + g * m1;
+            f2 = g2 * m + 2.0 * g1 * m1 + g * m2;
+        } else {
+            double P = R.c[0] + R.alpha1 * r, P1 = R.alpha1, P2 = 0.0;
+            for (int k = 1; k < R.num; ++k) {
+                const int l = k + 1;
+                P += R.c[k] * ip(r, l);
+                P1 += R.c[k] * l * ip(r, l - 1);
+                P2 += R.c[k] * l * (l - 1) * ip(r, l - 2);
+            }
+            const double h = -Cd * ip(t, C - 1), h1 = -Cd * (C - 1) * ip(t, C - 2),
+                         h2 = C >= 3 ? -Cd * (C - 1) * (C - 2) * ip(t, C - 3) : 0.0;
+            const double a = q.da1_dl;
+            f = h * P + g * a * r;
+            f1 = h1 * P + h * P1 + g1 * a * r + g * a;
+            f2 = h2 * P + 2.0 * h1 * P1 + h * P2 + g2 * a * r + 2.0 * g1 * a;
+        }
+        return;
+    }
+    double P = 0.0, P1 = 0.0, P2 = 0.0, Q = 1.0, Q1 = 0.0, Q2 = 0.0;
+    {
+        double rk2 = 0.0, rk1 = 1.0;   // r^(k-2), r^(k-1) for power k
+        for (int i = 0; i < max(R.num, R.den); ++i) {
+            const int k = i + 1;
+            const double rk = rk1 * r;
+            if (i < R.num) { P += R.c[i] * rk; P1 += R.c[i] * k * rk1; P2 += R.c[i] * k * (k - 1) * rk2; }
+            if (i < R.den) {
+                const double c = R.c[R.num + i];
+                Q += c * rk; Q1 += c * k * rk1; Q2 += c * k * (k - 1) * rk2;
+            }
+            rk2 = rk1;
+            rk1 = rk;
+        }
+    }
+    const bool numer = q.index < R.num;
+    const int e = numer ? q.index + 1 : q.index - R.num + 1;
+    const double m = ip(r, e), m1 = e * ip(r, e - 1), m2 = e * (e - 1) * ip(r, e > 1 ? e - 2 : 0);
+    const double iq = 1.0 / Q, iq2 = iq * iq, iq3 = iq2 * iq;
+    if (numer) {
+        f = m * iq;
+        f1 = m1 * iq - m * Q1 * iq2;
+        f2 = m2 * iq - 2.0 * m1 * Q1 * iq2 - m * Q2 * iq2 + 2.0 * m * Q1 * Q1 * iq3;
+    } else {
+        const double h = P * m, h1 = P1 * m + P * m1, h2 = P2 * m + 2.0 * P1 * m1 + P * m2;
+        f = -h * iq2;
+        f1 = -h1 * iq2 + 2.0 * h * Q1 * iq3;
+        f2 = -h2 * iq2 + 4.0 * h1 * Q1 * iq3 + 2.0 * h * Q2 * iq3 - 6.0 * h * Q1 * Q1 * iq3 * iq;
+    }
+    if (R.image) {
+        if (r >= q.r_cut) { f = f1 = f2 = 0.0; return; }
+        double v, d1, d2;
+        spline_eval(spl + q.sofs, q.n_knots, q.r_cut, q.inv_delta, r, v, d1, d2);
+        f -= q.limit + v;
+        f1 -= d1;
+        f2 -= d2;
+    }
+}
+
+// d phi / d alpha of a Gaussian orbital at separation (d0, d1) from its
+// centre (GaussianOrbitalSet::evaluate_parameter_derivatives,
+// orbital_set.h:249-289): -r^2 phi; of its gradient -2 delta_d phi
+// (1 - alpha r^2); of its second derivatives phi [8 alpha delta_d^2 - 2 -
+// r^2 (4 alpha^2 delta_d^2 - 2 alpha)].  Also the orbital's own gradient and
+// second derivatives.
+struct GaussAlpha {
+    double v, ag[2], al[2], g[2], l[2];
+};
+__device__ inline GaussAlpha gauss_alpha(double a, double d0, double d1) {
+    GaussAlpha o;
+    const double r2 = d0 * d0 + d1 * d1, phi = exp(-a * r2);
+    o.v = -r2 * phi;
+    const double del[2] = {d0, d1};
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        o.ag[d] = -2.0 * del[d] * phi * (1.0 - a * r2);
+        o.al[d] = phi * (8.0 * a * del[d] * del[d] - 2.0 - r2 * (4.0 * a * a * del[d] * del[d] - 2.0 * a));
+        o.g[d] = phi * (-2.0 * a * del[d]);
+        o.l[d] = phi * (4.0 * a * a * del[d] * del[d] - 2.0 * a);
+    }
+    return o;
+}
+
+// The walker's block b inverse, A(m, c) at [m * n + c].
+template <typename T, bool CPLX>
+__device__ inline const T* block_inverse(const DevModel& M, const DevState& S, int w, int b) {
+    const T* base = static_cast<const T*>(S.inv) + static_cast<size_t>(w) * S.inv_stride;
+    for (int i = 0; i < b; ++i) base += (CPLX ? 2 : 1) * M.blk_rows[i] * M.blk_n[i];
+    return base;
+}
+
 // TrialWavefunction::compute_parameter_derivatives (trial_wavefunction.cpp:
 // 150-160) for the free parameters listed in prm: out[i * stride + w] =
-// d log|psi| / d p_i of walker w.  One wave per walker; the pair sums have
-// lanes over a factor's first particle, the reciprocal part lanes over k.
-// Padé (pade_jastrow.cpp:641-700): du/da_k = r^k / Q, du/db_k = -P r^k / Q^2,
-// summed over images as du/dp - limit - spline(r) inside r_cut plus
-// sum_k w_k |rho_a|^2 (one group) or 2 w_k Re(rho_a rho_b*) with the
-// parameter's weights (periodic_pair_sum.cpp:85-98,
-// kspace_pair_sum.cpp:303-313); Drummond (drummond_jastrow.cpp:200-300):
-// du/dalpha_0 = (r - L)^C (1 + C r / L), du/dalpha_l = (r - L)^C r^l inside
-// L; Nosanow: -sum_i |r_i - R_i|^2.
+// d log|psi| / d p_i of walker w.  One wave per walker.  A pair factor sums
+// du/dp (param_radial) over its pairs, lanes over the first particle, plus,
+// summed over images, the reciprocal part sum_k w_k |rho_a|^2 (one group) or
+// 2 w_k Re(rho_a rho_b*) with the parameter's weights
+// (kspace_pair_sum.cpp:303-313), lanes over k.  Nosanow
+// (nosanow_jastrow.cpp:176-190): -sum_i |r_i - R_i|^2.  A Gaussian block's
+// shared width (slater_determinant.cpp:98-157): sum_{c,m} A(m,c)
+// dphi_m(r_c)/dalpha, lanes over the block's particles (real blocks only).
+template <typename T, bool CPLX>
 __global__ __launch_bounds__(64) void k_param_derivs(const DevModel* __restrict__ Mp, DevState S,
                                                      const DevParam* __restrict__ prm, int n_prm,
                                                      const double* __restrict__ wts, const double* __restrict__ spl,
@@ -2705,60 +2826,233 @@ __global__ __launch_bounds__(64) void k_param_derivs(const DevModel* __restrict_
         double t = 0.0;
         if (q.pair >= 0) {
             const DevPair& F = M.pair[q.pair];
-            const DevRadial& R = F.rad;
-            auto dudp = [&](double r) {
-                if (R.kind == 1) {
-                    if (r >= R.L) return 0.0;
-                    double g = 1.0;
-                    for (int k = 0; k < R.C; ++k) g *= r - R.L;
-                    if (q.index == 0) return g * (1.0 + static_cast<double>(R.C) * r / R.L);
-                    double rl = 1.0;
-                    for (int k = 0; k <= q.index; ++k) rl *= r;
-                    return g * rl;
-                }
-                double P = 0.0, Q = 1.0, rk = 1.0;
-                for (int k = 0; k < max(R.num, R.den); ++k) {
-                    rk *= r;
-                    if (k < R.num) P += R.c[k] * rk;
-                    if (k < R.den) Q += R.c[R.num + k] * rk;
-                }
-                double v, rp = 1.0;
-                if (q.index < R.num) {
-                    for (int k = 0; k <= q.index; ++k) rp *= r;
-                    v = rp / Q;
-                } else {
-                    for (int k = 0; k <= q.index - R.num; ++k) rp *= r;
-                    v = -P * rp / (Q * Q);
-                }
-                if (R.image) {
-                    if (r >= q.r_cut) return 0.0;
-                    double f, df, d2f;
-                    spline_eval(spl + q.sofs, q.n_knots, q.r_cut, q.inv_delta, r, f, df, d2f);
-                    v -= q.limit + f;
-                }
-                return v;
-            };
             const bool same = F.ga == F.gb;
             for (int a = M.g_first[F.ga] + lane; a < M.g_last[F.ga]; a += W)
                 for (int b = same ? a + 1 : M.g_first[F.gb]; b < M.g_last[F.gb]; ++b) {
                     double d0 = pos[2 * a] - pos[2 * b], d1 = pos[2 * a + 1] - pos[2 * b + 1];
-                    t += dudp(dist(M.cell, d0, d1));
+                    double f, f1, f2;
+                    param_radial(F.rad, q, spl, dist(M.cell, d0, d1), f, f1, f2);
+                    t += f;
                 }
-            if (R.image)
+            if (F.rad.image)
                 for (int k = lane; k < M.nkj; k += W) {
                     const double ar = rho[(F.ga * M.nkj + k) * 2], ai = rho[(F.ga * M.nkj + k) * 2 + 1];
                     const double br = rho[(F.gb * M.nkj + k) * 2], bi = rho[(F.gb * M.nkj + k) * 2 + 1];
                     t += (same ? 1.0 : 2.0) * wts[q.wofs + k] * (ar * br + ai * bi);
                 }
-        } else {
+        } else if (q.group >= 0) {
             for (int p = M.g_first[q.group] + lane; p < M.g_last[q.group]; p += W) {
                 double d0 = pos[2 * p] - M.nos_site[p][0], d1 = pos[2 * p + 1] - M.nos_site[p][1];
                 if (M.cell.periodic) min_image(M.cell, d0, d1);
                 t -= d0 * d0 + d1 * d1;
             }
+        } else if constexpr (!CPLX) {
+            const int b = q.block, n = M.blk_n[b], first = M.blk_first[b];
+            const T* A = block_inverse<T, CPLX>(M, S, w, b);
+            for (int c = lane; c < n; c += W)
+                for (int m = 0; m < n; ++m) {
+                    double d0 = pos[2 * (first + c)] - M.orb_k[b][m][0];
+                    double d1 = pos[2 * (first + c) + 1] - M.orb_k[b][m][1];
+                    if (M.cell.periodic) min_image(M.cell, d0, d1);
+                    const double r2 = d0 * d0 + d1 * d1;
+                    t += static_cast<double>(A[m * n + c]) * (-r2 * exp(-q.alpha * r2));
+                }
         }
         t = wsum(t);
         if (lane == 0) out[static_cast<size_t>(i) * stride + w] = t;
+    }
+}
+
+// Hamiltonian::local_energy_parameter_derivatives (hamiltonian.cpp:189-196)
+// over TrialWavefunction::compute_parameter_gradient_derivatives
+// (trial_wavefunction.cpp:162-185) for the pair and one-body parameters of
+// prm (a determinant's rows are left to k_gauss_eloc): every particle's
+// dG and dL of log|psi| with respect to p, reduced on the spot by the kinetic
+// term (kinetic.cpp:176-200, 212-250):
+//   dE/dp = sum_{i,d} -1/2 (1/m_i) [dL + 2 G dG],  dE_im/dp = sum -1/2 (1/m_i) 2 dG v,
+// G and v the whole trial's gradient and phase gradient ([i][d][nw], from
+// k_measure).  A pair parameter's f = du/dp enters as u does (pade_jastrow.
+// cpp:806-880): dG_i += f' delta/r, dL_i += f'' (delta_d/r)^2 + (f'/r)(1 -
+// (delta_d/r)^2); summed over images its weights enter the reciprocal
+// gradient and Laplacian (kspace_pair_sum.cpp:315-320).  Nosanow
+// (nosanow_jastrow.cpp:192-210): dG = -2 delta, dL = -2.  One wave per
+// walker, lanes over the factor's particles, each summing over its partners.
+__global__ __launch_bounds__(64) void k_param_eloc(const DevModel* __restrict__ Mp, DevState S,
+                                                   const DevParam* __restrict__ prm, int n_prm,
+                                                   const double* __restrict__ wts, const double* __restrict__ spl,
+                                                   const double* __restrict__ grad, const double* __restrict__ pgrad,
+                                                   double* __restrict__ out_re, double* __restrict__ out_im,
+                                                   int stride) {
+    const DevModel& M = *Mp;
+    __shared__ double pos[2 * kMaxParticles];
+    const int w = blockIdx.x, lane = threadIdx.x;
+    const int np = M.np, nw = M.nw;
+    const double* src = S.pos + static_cast<size_t>(w) * np * 2;
+    for (int i = lane; i < 2 * np; i += W) pos[i] = src[i];
+    __syncthreads();
+    const double* rho = S.rho + static_cast<size_t>(w) * M.ng * M.nkj * 2;
+    for (int i = 0; i < n_prm; ++i) {
+        const DevParam& q = prm[i];
+        if (q.block >= 0) continue;
+        double er = 0.0, ei = 0.0;
+        auto reduce = [&](int p, const double dg[2], const double dl[2]) {
+            const double im = M.inv_mass[p];
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                const size_t o = static_cast<size_t>(p * 2 + d) * nw + w;
+                er += -0.5 * im * (dl[d] + 2.0 * grad[o] * dg[d]);
+                if (pgrad) ei += -0.5 * im * (2.0 * dg[d] * pgrad[o]);
+            }
+        };
+        if (q.pair >= 0) {
+            const DevPair& F = M.pair[q.pair];
+            const bool same = F.ga == F.gb;
+            // particle p of group gp against the factor's other group
+            auto particle = [&](int p, int h) {
+                double dg[2] = {0.0, 0.0}, dl[2] = {0.0, 0.0};
+                for (int j = M.g_first[h]; j < M.g_last[h]; ++j) {
+                    if (j == p) continue;
+                    double d0 = pos[2 * p] - pos[2 * j], d1 = pos[2 * p + 1] - pos[2 * j + 1];
+                    const double r = dist(M.cell, d0, d1);
+                    double f, f1, f2, g0, g1, l0, l1;
+                    param_radial(F.rad, q, spl, r, f, f1, f2);
+                    pair_terms(f1, f2, r, d0, d1, g0, g1, l0, l1);
+                    dg[0] += g0; dg[1] += g1; dl[0] += l0; dl[1] += l1;
+                }
+                if (F.rad.image) {
+                    const double self = same ? 1.0 : 0.0;
+                    for (int k = 0; k < M.nkj; ++k) {
+                        double sn, cs;
+                        sincos_bounded(M.kjx[k] * pos[2 * p] + M.kjy[k] * pos[2 * p + 1], &sn, &cs);
+                        const double rr = rho[(h * M.nkj + k) * 2], ri = rho[(h * M.nkj + k) * 2 + 1];
+                        const double wk = wts[q.wofs + k];
+                        const double gk = 2.0 * wk * (cs * ri - sn * rr);
+                        const double lk = 2.0 * wk * (self - (cs * rr + sn * ri));
+                        dg[0] += M.kjx[k] * gk; dg[1] += M.kjy[k] * gk;
+                        dl[0] += M.kjx[k] * M.kjx[k] * lk; dl[1] += M.kjy[k] * M.kjy[k] * lk;
+                    }
+                }
+                reduce(p, dg, dl);
+            };
+            for (int p = M.g_first[F.ga] + lane; p < M.g_last[F.ga]; p += W) particle(p, F.gb);
+            if (!same)
+                for (int p = M.g_first[F.gb] + lane; p < M.g_last[F.gb]; p += W) particle(p, F.ga);
+        } else {
+            for (int p = M.g_first[q.group] + lane; p < M.g_last[q.group]; p += W) {
+                double d0 = pos[2 * p] - M.nos_site[p][0], d1 = pos[2 * p + 1] - M.nos_site[p][1];
+                if (M.cell.periodic) min_image(M.cell, d0, d1);
+                const double dg[2] = {-2.0 * d0, -2.0 * d1}, dl[2] = {-2.0, -2.0};
+                reduce(p, dg, dl);
+            }
+        }
+        er = wsum(er);
+        ei = wsum(ei);
+        if (lane == 0) {
+            out_re[static_cast<size_t>(i) * stride + w] = er;
+            out_im[static_cast<size_t>(i) * stride + w] = ei;
+        }
+    }
+}
+
+// dE_L / d alpha for the shared width alpha of Gaussian block b
+// (SlaterDeterminant::compute_parameter_gradient_derivatives,
+// slater_determinant.cpp:159-251, reduced as in k_param_eloc).  The inverse
+// moves with alpha: dA = -A dD A with dD(c, m) = dphi_m(r_c)/dalpha, so
+//   dG_j = sum_m dA(m,j) dphi_m(r_j) + A(m,j) d(dphi_m(r_j))/dalpha,
+// the same for the second derivatives, and dL = dLraw - 2 G_det dG with the
+// block's own gradient G_det.  One workgroup of 256 per walker of the chunk
+// [w0, w0 + gridDim.x): thread t takes column j = t % n and row phase
+// r = t / n (R = 256 / n phases).  Pass 1 forms Y = dD A in ybuf
+// (n x n per walker of the chunk), R rows of dD at a time in LDS; pass 2
+// forms column j of dA = -A Y and folds it into the six sums of particle j,
+// which the R phases then add up.  O(n^3) per walker: the optimizer's path,
+// not the sweep's.
+constexpr int kGaussThreads = 256;
+template <typename T>
+__global__ __launch_bounds__(kGaussThreads) void k_gauss_eloc(const DevModel* __restrict__ Mp, DevState S, int b,
+                                                              double alpha, const double* __restrict__ grad,
+                                                              double* __restrict__ ybuf, int w0,
+                                                              double* __restrict__ out) {
+    const DevModel& M = *Mp;
+    __shared__ double px[kMaxOrb], py[kMaxOrb];
+    __shared__ double dd[kGaussThreads];
+    __shared__ double part[6][kGaussThreads];
+    __shared__ double esum[kGaussThreads / W];
+    const int w = w0 + blockIdx.x, t = threadIdx.x;
+    const int n = M.blk_n[b], first = M.blk_first[b], nw = M.nw;
+    const int R = kGaussThreads / n, j = t % n, r = t / n;
+    const bool on = r < R;
+    const double* src = S.pos + static_cast<size_t>(w) * M.np * 2;
+    for (int c = t; c < n; c += kGaussThreads) {
+        px[c] = src[2 * (first + c)];
+        py[c] = src[2 * (first + c) + 1];
+    }
+    const T* A = block_inverse<T, false>(M, S, w, b);
+    double* Y = ybuf + static_cast<size_t>(blockIdx.x) * n * n;
+    __syncthreads();
+    auto sep = [&](int c, int m, double& d0, double& d1) {
+        d0 = px[c] - M.orb_k[b][m][0];
+        d1 = py[c] - M.orb_k[b][m][1];
+        if (M.cell.periodic) min_image(M.cell, d0, d1);
+    };
+    // pass 1: Y(c, j) = sum_m dD(c, m) A(m, j)
+    for (int c0 = 0; c0 < n; c0 += R) {
+        const int c = c0 + r;
+        if (on && c < n) {
+            double d0, d1;
+            sep(c, j, d0, d1);   // thread (j, r) holds dD(c, m = j)
+            const double r2 = d0 * d0 + d1 * d1;
+            dd[r * n + j] = -r2 * exp(-alpha * r2);
+        }
+        __syncthreads();
+        if (on && c < n) {
+            double y = 0.0;
+            for (int m = 0; m < n; ++m) y += dd[r * n + m] * static_cast<double>(A[m * n + j]);
+            Y[c * n + j] = y;
+        }
+        __syncthreads();
+    }
+    // pass 2: column j of dA, folded into particle j's sums
+    double acc[6] = {0, 0, 0, 0, 0, 0};   // G_d, dG_d, dLraw_d
+    if (on)
+        for (int m = r; m < n; m += R) {
+            double da = 0.0;
+            for (int c = 0; c < n; ++c) da -= static_cast<double>(A[m * n + c]) * Y[c * n + j];
+            const double a = static_cast<double>(A[m * n + j]);
+            double d0, d1;
+            sep(j, m, d0, d1);
+            const GaussAlpha o = gauss_alpha(alpha, d0, d1);
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                acc[d] += a * o.g[d];
+                acc[2 + d] += da * o.g[d] + a * o.ag[d];
+                acc[4 + d] += da * o.l[d] + a * o.al[d];
+            }
+        }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) part[q][t] = acc[q];
+    __syncthreads();
+    double e = 0.0;
+    if (t < n) {
+        double s6[6] = {0, 0, 0, 0, 0, 0};
+        for (int rr = 0; rr < R; ++rr)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) s6[q] += part[q][rr * n + t];
+        const int p = first + t;
+        const double im = M.inv_mass[p];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const double G = s6[d], dG = s6[2 + d], dL = s6[4 + d] - 2.0 * G * dG;
+            e += -0.5 * im * (dL + 2.0 * grad[static_cast<size_t>(p * 2 + d) * nw + w] * dG);
+        }
+    }
+    e = wsum(e);
+    if ((t & (W - 1)) == 0) esum[t / W] = e;
+    __syncthreads();
+    if (t == 0) {
+        double tot = 0.0;
+        for (int k = 0; k < kGaussThreads / W; ++k) tot += esum[k];
+        out[w] = tot;
     }
 }
 
@@ -3019,9 +3313,24 @@ void launch_delta_potential(const DevModel* m, const DevState& s, int n, int p, 
                             double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_delta_potential, dim3(n), dim3(64), 0, st, m, s, p, prop, stride, out);
 }
-void launch_param_derivs(const DevModel* m, const DevState& s, int nw, const DevParam* prm, int n_prm,
-                         const double* wts, const double* spl, double* out, int stride, hipStream_t st) {
-    hipLaunchKernelGGL(k_param_derivs, dim3(nw), dim3(64), 0, st, m, s, prm, n_prm, wts, spl, out, stride);
+void launch_param_derivs(const DevModel* m, const DevState& s, int nw, bool mixed, bool cplx, const DevParam* prm,
+                         int n_prm, const double* wts, const double* spl, double* out, int stride, hipStream_t st) {
+    DISPATCH(k_param_derivs, dim3(nw), dim3(64), 0, st, m, s, prm, n_prm, wts, spl, out, stride);
+}
+void launch_param_eloc(const DevModel* m, const DevState& s, int nw, const DevParam* prm, int n_prm,
+                       const double* wts, const double* spl, const double* grad, const double* pgrad,
+                       double* out_re, double* out_im, int stride, hipStream_t st) {
+    hipLaunchKernelGGL(k_param_eloc, dim3(nw), dim3(64), 0, st, m, s, prm, n_prm, wts, spl, grad, pgrad, out_re,
+                       out_im, stride);
+}
+void launch_gauss_eloc(const DevModel* m, const DevState& s, bool mixed, int b, double alpha, const double* grad,
+                       double* ybuf, int w0, int n_walkers, double* out, hipStream_t st) {
+    if (mixed)
+        hipLaunchKernelGGL(k_gauss_eloc<float>, dim3(n_walkers), dim3(kGaussThreads), 0, st, m, s, b, alpha, grad,
+                           ybuf, w0, out);
+    else
+        hipLaunchKernelGGL(k_gauss_eloc<double>, dim3(n_walkers), dim3(kGaussThreads), 0, st, m, s, b, alpha, grad,
+                           ybuf, w0, out);
 }
 void launch_gather(const double* src_d, double* dst_d, int stride, const int* src, int n_new,
                    hipStream_t st) {

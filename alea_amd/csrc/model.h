@@ -46,7 +46,7 @@ struct DevRadial {
     double r_cut = 0, inv_delta = 0, u_inf = 0;
     int n_knots = 0;
     int C = 3;
-    double L = 0, alpha1 = 0;
+    double L = 0, alpha1 = 0, gamma = 0;
     double spline[3 * kMaxKnots] = {};
     // the same spline per panel as the monomial c0 + c1 x + ... + c5 x^5 in
     // the panel coordinate x (filled at build), evaluated by Horner in the sweep
@@ -161,9 +161,11 @@ struct DevState {
 // with the image-sum tables of its derivative for a Padé summed over images
 // (weights at wofs in the weights buffer, the knot-form spline at sofs).
 struct DevParam {
-    int pair = -1, group = -1, index = 0;
+    int pair = -1, group = -1, block = -1, index = 0;
     int n_knots = 0, wofs = 0, sofs = 0;
     double r_cut = 0, inv_delta = 0, limit = 0;
+    double da1_dl = 0;   // Drummond L: d alpha_1 / d L
+    double alpha = 0;    // Gaussian block: the shared width
 };
 
 // Why an accepted move rebuilt a determinant inside the sweep

@@ -152,10 +152,8 @@ class Walkers:
         self.api("ratio_range", self.h, p, w_begin, w_end, ptr(npos), ptr(lr), ptr(ph), ptr(ng), ptr(nl))
         return out
 
-    def parameter_derivatives(self, params):
-        """TrialWavefunction::compute_parameter_derivatives for the free
-        parameters `params` (system.free_parameters): [n][nw] of
-        d log|psi| / d p."""
+    @staticmethod
+    def _param_array(params):
         from ._capi import AleaParam
         n = len(params)
         arr = (AleaParam * max(n, 1))()
@@ -169,9 +167,29 @@ class Walkers:
                 arr[i].weights, arr[i].spline = ptr(w), ptr(sp)
                 arr[i].n_knots, arr[i].r_cut, arr[i].inv_delta, arr[i].limit = (
                     t["n_knots"], t["r_cut"], t["inv_delta"], t["u_inf"])
+        return arr, keep
+
+    def parameter_derivatives(self, params):
+        """TrialWavefunction::compute_parameter_derivatives for the free
+        parameters `params` (system.free_parameters): [n][nw] of
+        d log|psi| / d p."""
+        n = len(params)
+        arr, keep = self._param_array(params)
         out = np.zeros(max(n, 1) * self.nw)
         self.api("parameter_derivatives", self.h, arr, n, ptr(out))
         return out[:n * self.nw].reshape(n, self.nw)
+
+    def local_energy_parameter_derivatives(self, params):
+        """Hamiltonian::local_energy_parameter_derivatives over
+        TrialWavefunction::compute_parameter_gradient_derivatives for the free
+        parameters `params`: (re, im), each [n][nw], of dE_L / d p (im zero
+        for a real trial)."""
+        n = len(params)
+        arr, keep = self._param_array(params)
+        re = np.zeros(max(n, 1) * self.nw)
+        im = np.zeros(max(n, 1) * self.nw)
+        self.api("local_energy_parameter_derivatives", self.h, arr, n, ptr(re), ptr(im))
+        return re[:n * self.nw].reshape(n, self.nw), im[:n * self.nw].reshape(n, self.nw)
 
     def delta_potential(self, p, new_pos, w_begin=0, w_end=-1):
         """Hamiltonian::delta_potential_particle for every walker of the range:

@@ -159,10 +159,13 @@ int alea_delta_potential(alea_system* s, int p, int w_begin, int w_end, const do
 
 /* One free (optimizable) parameter of the trial wavefunction.  factor counts
  * the wavefunction factors in the order they were added (the reference's
- * component order); index is the parameter within it: a Padé's coefficients
- * a_1..a_N then b_1..b_M, a Drummond's coefficients alpha_0, alpha_2, ...
- * (as alea_add_drummond_jastrow takes them; L stays fixed), a Nosanow's 0
- * (alpha).  For a Padé summed over images, the breakup of du/dp minus its
+ * component order, each determinant block counted as one factor); index is
+ * the parameter within it: a Padé's coefficients a_1..a_N then b_1..b_M, a
+ * Drummond's coefficients alpha_0, alpha_2, ... (as
+ * alea_add_drummond_jastrow takes them) then L, a Nosanow's 0 (alpha), a
+ * Gaussian block's 0 (the shared width, GaussianOrbitalSet's one parameter,
+ * orbital_set.h:228-247; a block of several widths, and the plane-wave and
+ * twisted blocks, have none).  For a Padé summed over images, the breakup of du/dp minus its
  * limit on the factor's k vectors (PeriodicPairSum's param_minus_limit,
  * periodic_pair_sum.cpp:16-64): weights [n_k], spline [3 n_knots] in the
  * knot form of alea_add_pade_jastrow, r_cut, 1/delta and the limit;
@@ -179,9 +182,21 @@ typedef struct alea_param {
  * (src/wavefunction/trial_wavefunction.cpp:150-160), for the n parameters
  * listed: derivs[i * nw + w] = d log|psi| / d p_i of walker w, after
  * evaluate or sweeps.  Per factor as PadeJastrow (pade_jastrow.cpp:641-700),
- * DrummondJastrow (drummond_jastrow.cpp:200-300) and NosanowJastrow compute
- * it; determinants have no free parameters here. */
+ * DrummondJastrow (drummond_jastrow.cpp:201-320), NosanowJastrow
+ * (nosanow_jastrow.cpp:176-190) and SlaterDeterminant's Gaussian width
+ * (slater_determinant.cpp:98-157) compute it. */
 int alea_parameter_derivatives(alea_system* s, const alea_param* params, int n, double* derivs);
+
+/* Hamiltonian::local_energy_parameter_derivatives
+ * (src/hamiltonian/hamiltonian.cpp:189-196) fed by
+ * TrialWavefunction::compute_parameter_gradient_derivatives
+ * (trial_wavefunction.cpp:162-185), in one call: de_re[i * nw + w] =
+ * dE_L / d p_i of walker w, and for a complex trial de_im its imaginary part
+ * (complex_local_energy_parameter_derivatives, kinetic.cpp:212-250; zeros
+ * for a real trial; de_im may be NULL).  Only the kinetic term depends on
+ * the parameters.  The per-particle gradient derivatives stay on the device. */
+int alea_local_energy_parameter_derivatives(alea_system* s, const alea_param* params, int n, double* de_re,
+                                            double* de_im);
 
 /* Fused sweeps: VMC::sweep / sweep_walkers (vmc.cpp:69-155) with symmetric
  * Metropolis proposals of width sqrt(tau/m).  tape: [sweep][np][dim+1][nw]

@@ -338,6 +338,14 @@ public:
         lap.resize(n);
         check(alea_get_gradients(h_, grad.data(), lap.data()));
     }
+    // compute_parameter_derivatives (trial_wavefunction.cpp:150-160) for the
+    // listed free parameters (factors counted in the order they were added,
+    // each determinant block one factor): derivs [n][nw] of d log|psi| / d p
+    void compute_parameter_derivatives(WalkerSet& ws, std::span<const alea_param> params,
+                                       std::vector<double>& derivs) {
+        derivs.resize(params.size() * ws.num_walkers());
+        check(alea_parameter_derivatives(h_, params.data(), static_cast<int>(params.size()), derivs.data()));
+    }
 
 private:
     static int range_int(std::size_t w) { return w == npos ? -1 : static_cast<int>(w); }
@@ -365,6 +373,17 @@ public:
         if (im) im->resize(e.size());
         check(alea_local_energy(h_, e.data(), im ? im->data() : nullptr));
         return e;
+    }
+    // local_energy_parameter_derivatives (hamiltonian.cpp:189-196) with the
+    // wavefunction's gradient derivatives (trial_wavefunction.cpp:162-185)
+    // formed on the device in the same call: out [n][nw] of dE_L / d p, and
+    // its imaginary part for a complex trial in im when given
+    void local_energy_parameter_derivatives(WalkerSet& ws, std::span<const alea_param> params,
+                                            std::vector<double>& out, std::vector<double>* im = nullptr) {
+        out.resize(params.size() * ws.num_walkers());
+        if (im) im->resize(out.size());
+        check(alea_local_energy_parameter_derivatives(h_, params.data(), static_cast<int>(params.size()),
+                                                      out.data(), im ? im->data() : nullptr));
     }
     // delta_potential_particle (hamiltonian.cpp:105-117), for every walker of
     // [w_begin, w_end) at once: V with particle p at proposed ([dim][nw]) minus

@@ -421,7 +421,7 @@ struct Radial {
     Spline sp;
     double u_inf = 0.0;
     int C = 3;
-    double L = 0.0, alpha1 = 0.0;
+    double L = 0.0, alpha1 = 0.0, gamma = 0.0;
 
     static void poly(const double* c, int n, double r, double& v, double& dv, double& d2v) {
         double r_km1 = 1.0, r_km2 = 0.0;
@@ -559,6 +559,10 @@ struct Sys {
     std::vector<double> pos, dist, pending, new_pos;
     std::vector<double> log_ratios, phase_diffs, new_grad, new_lap, new_pgrad, new_plap;
     std::vector<std::unique_ptr<Component>> comps;
+    // the wavefunction's factors in the order they were added, as the
+    // product counts them: a pair or one-body factor, or one determinant
+    // block (component, block index)
+    std::vector<std::pair<Component*, int>> factors;
     Ewald ewald;
     std::vector<CoulombPair> coulomb;
     std::vector<InversePower> ipow;
@@ -1921,6 +1925,7 @@ int oracle_add_pade_jastrow(oracle_system* h, int ga, int gb, int num_order, int
             k.b1x = s.cell.b[0]; k.b1y = s.cell.b[1]; k.b2x = s.cell.b[2]; k.b2y = s.cell.b[3];
             k.n_cached = k.a.size() + (k.same ? 0 : k.b.size());
         }
+        s.factors.push_back({c.get(), -1});
         s.comps.push_back(std::move(c));
     });
 }
@@ -1943,6 +1948,8 @@ int oracle_add_drummond_jastrow(oracle_system* h, int ga, int gb, int C, double 
         double ipw = 1.0;
         for (int i = 0; i < C; ++i) ipw *= -L;
         c->rad.alpha1 = gamma / ipw + coeffs[0] * static_cast<double>(C) / L;
+        c->rad.gamma = gamma;
+        s.factors.push_back({c.get(), -1});
         s.comps.push_back(std::move(c));
     });
 }
@@ -1961,6 +1968,7 @@ int oracle_add_mcmillan_jastrow(oracle_system* h, int ga, int gb, double b, int 
         c->rad.mcmillan = true;
         c->rad.num = n;
         c->rad.c = {b};
+        s.factors.push_back({c.get(), -1});
         s.comps.push_back(std::move(c));
     });
 }
@@ -2007,6 +2015,7 @@ int oracle_add_nosanow_jastrow(oracle_system* h, int group, double alpha, const 
         c->g = s.groups[group];
         c->alpha = alpha;
         c->sites.assign(sites, sites + c->g.size() * 2);
+        s.factors.push_back({c.get(), -1});
         s.comps.push_back(std::move(c));
     });
 }
@@ -2030,6 +2039,7 @@ int oracle_add_slater_plane_wave(oracle_system* h, int first, int last, const do
             c = static_cast<RealSlater*>(s.comps.back().get());
         }
         c->blocks.push_back(std::move(B));
+        s.factors.push_back({c, static_cast<int>(c->blocks.size()) - 1});
     });
 }
 
@@ -2053,6 +2063,7 @@ int oracle_add_slater_gaussian(oracle_system* h, int first, int last, const doub
             c = static_cast<RealSlater*>(s.comps.back().get());
         }
         c->blocks.push_back(std::move(B));
+        s.factors.push_back({c, static_cast<int>(c->blocks.size()) - 1});
     });
 }
 
@@ -2072,6 +2083,7 @@ int oracle_add_slater_twisted(oracle_system* h, int first, int last, const doubl
             c = static_cast<ComplexSlater*>(s.comps.back().get());
         }
         c->blocks.push_back(std::move(B));
+        s.factors.push_back({c, static_cast<int>(c->blocks.size()) - 1});
         s.has_phase = true;
     });
 }
@@ -2320,84 +2332,195 @@ int oracle_delta_potential(oracle_system* h, int p, int w_begin, int w_end, cons
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// ── optimizer derivatives ──────────────────────────────────────────────────
+// f = du/dp of a pair factor's parameter q at distance r, with f' and f''.
+// Padé (PadeFunction's parameter derivatives behind pade_jastrow.cpp:641-700,
+// 806-880): du/da_k = r^e/Q (e = k+1), du/db_k = -P r^e/Q^2; summed over
+// images the short-range part takes f - limit - spline(r) inside r_cut
+// (periodic_pair_sum.cpp:85-98).  Drummond (drummond_jastrow.cpp:201-420),
+// inside the cutoff, with t = r - L and g = t^C: du/dalpha_0 = g (1 + C r/L)
+// (alpha_1 = gamma/(-L)^C + alpha_0 C/L follows alpha_0),
+// du/dalpha_l = g r^l, du/dL = -C t^(C-1) P + g r dalpha_1/dL.
+struct ParamRadial {
+    const Radial& R;
+    int index;
+    const Spline* sp;     // the parameter's spline (image-summed Padé)
+    double limit;
+    void eval(double r, double& f, double& f1, double& f2) const {
+        f = f1 = f2 = 0.0;
+        if (R.drummond) {
+            if (r >= R.L) return;
+            const int C = R.C;
+            const double Cd = static_cast<double>(C), t = r - R.L;
+            auto ip = [](double x, int e) { double y = 1.0; for (int i = 0; i < e; ++i) y *= x; return y; };
+            const double g = ip(t, C), g1 = Cd * ip(t, C - 1), g2 = Cd * (C - 1) * ip(t, C - 2);
+            const int nc = static_cast<int>(R.c.size());
+            if (index == 0) {
+                const double k = Cd / R.L;
+                f = g * (1.0 + k * r);
+                f1 = g1 * (1.0 + k * r) + g * k;
+                f2 = g2 * (1.0 + k * r) + 2.0 * g1 * k;
+            } else if (index < nc) {
+                const int l = index + 1;
+                const double m = ip(r, l), m1 = l * ip(r, l - 1), m2 = l * (l - 1) * ip(r, l - 2);
+                f = g * m;
+                f1 = g1 * m + g * m1;
+                f2 = g2 * m + 2.0 * g1 * m1 + g * m2;
+            } else {
+                double P = R.c[0] + R.alpha1 * r, P1 = R.alpha1, P2 = 0.0;
+                for (int k = 1; k < nc; ++k) {
+                    const int l = k + 1;
+                    P += R.c[static_cast<std::size_t>(k)] * ip(r, l);
+                    P1 += R.c[static_cast<std::size_t>(k)] * l * ip(r, l - 1);
+                    P2 += R.c[static_cast<std::size_t>(k)] * l * (l - 1) * ip(r, l - 2);
+                }
+                const double h = -Cd * ip(t, C - 1), h1 = -Cd * (C - 1) * ip(t, C - 2),
+                             h2 = C >= 3 ? -Cd * (C - 1) * (C - 2) * ip(t, C - 3) : 0.0;
+                const double a = R.gamma * Cd / ip(-R.L, C + 1) - R.c[0] * Cd / (R.L * R.L);
+                f = h * P + g * a * r;
+                f1 = h1 * P + h * P1 + g1 * a * r + g * a;
+                f2 = h2 * P + 2.0 * h1 * P1 + h * P2 + g2 * a * r + 2.0 * g1 * a;
+            }
+            return;
+        }
+        double P = 0.0, P1 = 0.0, P2 = 0.0, Q = 1.0, Q1 = 0.0, Q2 = 0.0;
+        Radial::poly(R.c.data(), R.num, r, P, P1, P2);
+        Radial::poly(R.c.data() + R.num, R.den, r, Q, Q1, Q2);
+        const bool num = index < R.num;
+        const int e = num ? index + 1 : index - R.num + 1;
+        double m = 1.0;
+        for (int i = 0; i < e; ++i) m *= r;
+        const double m1 = e * m / r, m2 = e * (e - 1) * m / (r * r);
+        const double iq = 1.0 / Q;
+        if (num) {
+            f = m * iq;
+            f1 = m1 * iq - m * Q1 * iq * iq;
+            f2 = m2 * iq - 2.0 * m1 * Q1 * iq * iq - m * Q2 * iq * iq + 2.0 * m * Q1 * Q1 * iq * iq * iq;
+        } else {
+            const double hh = P * m, h1 = P1 * m + P * m1, h2 = P2 * m + 2.0 * P1 * m1 + P * m2;
+            const double iq2 = iq * iq, iq3 = iq2 * iq;
+            f = -hh * iq2;
+            f1 = -h1 * iq2 + 2.0 * hh * Q1 * iq3;
+            f2 = -h2 * iq2 + 4.0 * h1 * Q1 * iq3 + 2.0 * hh * Q2 * iq3 - 6.0 * hh * Q1 * Q1 * iq3 * iq;
+        }
+        if (R.image) {
+            if (r >= sp->r_cut) { f = f1 = f2 = 0.0; return; }
+            double v, d1, d2;
+            sp->eval(r, v, d1, d2);
+            f -= limit + v;
+            f1 -= d1;
+            f2 -= d2;
+        }
+    }
+};
+
+// One listed parameter, resolved against the system's factors.
+struct ResolvedParam {
+    PairJastrow* pj = nullptr;
+    Nosanow* ns = nullptr;
+    RealSlater* det = nullptr;
+    int block = -1;
+    Spline sp;
+};
+
+ResolvedParam resolve_param(Sys& s, const oracle_param& q) {
+    if (q.factor < 0 || static_cast<std::size_t>(q.factor) >= s.factors.size())
+        throw std::invalid_argument("oracle: factor index out of range");
+    ResolvedParam out;
+    const auto [c, blk] = s.factors[static_cast<std::size_t>(q.factor)];
+    if (auto* pj = dynamic_cast<PairJastrow*>(c)) {
+        const Radial& R = pj->rad;
+        if (R.mcmillan) throw std::invalid_argument("oracle: the McMillan factor has no free parameters");
+        const int n_par = R.drummond ? static_cast<int>(R.c.size()) + 1 : R.num + R.den;
+        if (q.index < 0 || q.index >= n_par) throw std::invalid_argument("oracle: parameter index out of range");
+        if (R.image) {
+            if (!q.weights || !q.spline) throw std::invalid_argument("oracle: image-summed parameter needs its tables");
+            out.sp.scaled.assign(q.spline, q.spline + 3 * q.n_knots);
+            out.sp.n_knots = q.n_knots;
+            out.sp.r_cut = q.r_cut;
+            out.sp.inv_delta = q.inv_delta;
+        }
+        out.pj = pj;
+    } else if (auto* ns = dynamic_cast<Nosanow*>(c)) {
+        if (q.index != 0) throw std::invalid_argument("oracle: the Nosanow factor has one parameter");
+        out.ns = ns;
+    } else if (auto* rs = dynamic_cast<RealSlater*>(c)) {
+        const auto& B = rs->blocks[static_cast<std::size_t>(blk)];
+        bool uniform = B.orb.gaussian;
+        for (std::size_t m = 1; uniform && m < B.N; ++m) uniform = B.orb.alphas[m] == B.orb.alphas[0];
+        if (!uniform) throw std::invalid_argument("oracle: this determinant has no free parameters");
+        if (q.index != 0) throw std::invalid_argument("oracle: parameter index out of range");
+        out.det = rs;
+        out.block = blk;
+    } else {
+        throw std::invalid_argument("oracle: this determinant has no free parameters");
+    }
+    return out;
+}
+
+template <class F>
+void for_pairs(const PairJastrow& pj, F&& f) {
+    if (pj.same) {
+        for (std::size_t a = pj.ga.first; a < pj.ga.last; ++a)
+            for (std::size_t b = a + 1; b < pj.ga.last; ++b) f(a, b);
+    } else {
+        for (std::size_t a = pj.ga.first; a < pj.ga.last; ++a)
+            for (std::size_t b = pj.gb.first; b < pj.gb.last; ++b) f(a, b);
+    }
+}
+
+// dphi/dalpha of a Gaussian orbital at separation (d0, d1) from its centre
+// (GaussianOrbitalSet::evaluate_parameter_derivatives, orbital_set.h:249-289):
+// -r^2 phi, d/dalpha of its gradient -2 delta_d phi (1 - alpha r^2) and of its
+// second derivatives phi [8 alpha delta_d^2 - 2 - r^2 (4 alpha^2 delta_d^2 -
+// 2 alpha)].
+void gauss_alpha_terms(double a, double d0, double d1, double& v, double* g, double* l) {
+    const double r2 = d0 * d0 + d1 * d1, phi = std::exp(-a * r2);
+    v = -r2 * phi;
+    const double del[2] = {d0, d1};
+    for (int d = 0; d < 2; ++d) {
+        g[d] = -2.0 * del[d] * phi * (1.0 - a * r2);
+        l[d] = phi * (8.0 * a * del[d] * del[d] - 2.0 - r2 * (4.0 * a * a * del[d] * del[d] - 2.0 * a));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
 // TrialWavefunction::compute_parameter_derivatives (trial_wavefunction.cpp:
 // 150-160) for the listed free parameters: derivs[i * nw + w] = d log|psi| /
-// d p_i of walker w.  A pair Jastrow's derivative sums du/dp over its pairs at
-// the minimum-image distance (PadeJastrow and DrummondJastrow::
-// compute_parameter_derivatives, pade_jastrow.cpp:641-700,
-// drummond_jastrow.cpp:200-300): Padé du/da_k = r^k/Q, du/db_k = -P r^k/Q^2;
-// Drummond du/dalpha_0 = (r - L)^C (1 + C r/L) (alpha_1 follows alpha_0
-// through the cusp), du/dalpha_l = (r - L)^C r^l, inside the cutoff.  Summed
-// over images (periodic_pair_sum.cpp:85-98, kspace_pair_sum.cpp:303-313) the
-// pair part takes du/dp - limit - spline(r) inside r_cut, and the
+// d p_i of walker w.  A pair factor sums du/dp over its pairs at the
+// minimum-image distance (ParamRadial), plus, summed over images, the
 // reciprocal part sum_k w_k |rho_a|^2 (one group) or 2 w_k Re(rho_a rho_b*)
-// with the parameter's weights.  Nosanow (nosanow_jastrow.cpp): d/dalpha =
-// -sum_i |r_i - R_i|^2.
+// with the parameter's weights (kspace_pair_sum.cpp:303-313).  Nosanow
+// (nosanow_jastrow.cpp:176-190): -sum_i |r_i - R_i|^2.  A Gaussian block's
+// shared width (slater_determinant.cpp:98-157): sum_{c,a} A(a,c) dphi_a(r_c)/dalpha.
 int oracle_parameter_derivatives(oracle_system* h, const oracle_param* params, int n, double* derivs) {
     return guarded([&] {
         Sys& s = *S(h);
         const std::size_t nw = s.nw;
         for (int i = 0; i < n; ++i) {
             const oracle_param& q = params[i];
-            if (q.factor < 0 || static_cast<std::size_t>(q.factor) >= s.comps.size())
-                throw std::invalid_argument("oracle: factor index out of range");
+            ResolvedParam rp = resolve_param(s, q);
             double* out = derivs + static_cast<std::size_t>(i) * nw;
             std::fill(out, out + nw, 0.0);
-            Component* c = s.comps[static_cast<std::size_t>(q.factor)].get();
-            if (auto* pj = dynamic_cast<PairJastrow*>(c)) {
-                const Radial& R = pj->rad;
-                if (R.mcmillan) throw std::invalid_argument("oracle: the McMillan factor has no free parameters");
-                const int np_ = R.drummond ? static_cast<int>(R.c.size()) : R.num + R.den;
-                if (q.index < 0 || q.index >= np_) throw std::invalid_argument("oracle: parameter index out of range");
-                Spline sp;
-                if (R.image) {
-                    if (!q.weights || !q.spline) throw std::invalid_argument("oracle: image-summed parameter needs its tables");
-                    sp.scaled.assign(q.spline, q.spline + 3 * q.n_knots);
-                    sp.n_knots = q.n_knots;
-                    sp.r_cut = q.r_cut;
-                    sp.inv_delta = q.inv_delta;
-                }
-                auto dudp = [&](double r) {
-                    if (R.drummond) {
-                        if (r >= R.L) return 0.0;
-                        double g = 1.0;
-                        for (int k = 0; k < R.C; ++k) g *= r - R.L;
-                        if (q.index == 0) return g * (1.0 + static_cast<double>(R.C) * r / R.L);
-                        double rl = 1.0;
-                        for (int k = 0; k <= q.index; ++k) rl *= r;
-                        return g * rl;
+            if (rp.pj) {
+                const PairJastrow& pj = *rp.pj;
+                const ParamRadial pr{pj.rad, q.index, &rp.sp, q.limit};
+                for_pairs(pj, [&](std::size_t a, std::size_t b) {
+                    for (std::size_t w = 0; w < nw; ++w) {
+                        double f, f1, f2;
+                        pr.eval(s.D(a, b, w), f, f1, f2);
+                        out[w] += f;
                     }
-                    double P = 0.0, Q = 1.0, rk = 1.0;
-                    for (int k = 0; k < std::max(R.num, R.den); ++k) {
-                        rk *= r;
-                        if (k < R.num) P += R.c[static_cast<std::size_t>(k)] * rk;
-                        if (k < R.den) Q += R.c[static_cast<std::size_t>(R.num + k)] * rk;
-                    }
-                    double v;
-                    if (q.index < R.num) {
-                        double rp = 1.0;
-                        for (int k = 0; k <= q.index; ++k) rp *= r;
-                        v = rp / Q;
-                    } else {
-                        double rp = 1.0;
-                        for (int k = 0; k <= q.index - R.num; ++k) rp *= r;
-                        v = -P * rp / (Q * Q);
-                    }
-                    if (R.image) v = r >= q.r_cut ? 0.0 : v - q.limit - sp.value(r);
-                    return v;
-                };
-                auto pair = [&](std::size_t a, std::size_t b) {
-                    for (std::size_t w = 0; w < nw; ++w) out[w] += dudp(s.D(a, b, w));
-                };
-                if (pj->same) {
-                    for (std::size_t a = pj->ga.first; a < pj->ga.last; ++a)
-                        for (std::size_t b = a + 1; b < pj->ga.last; ++b) pair(a, b);
-                } else {
-                    for (std::size_t a = pj->ga.first; a < pj->ga.last; ++a)
-                        for (std::size_t b = pj->gb.first; b < pj->gb.last; ++b) pair(a, b);
-                }
-                if (R.image) {
-                    const KSpace& k = pj->ks;
+                });
+                if (pj.rad.image) {
+                    const KSpace& k = pj.ks;
                     const std::size_t K = k.nk();
                     for (std::size_t w = 0; w < nw; ++w) {
                         const double* ar = k.ra_re.data() + w * K;
@@ -2411,18 +2534,191 @@ int oracle_parameter_derivatives(oracle_system* h, const oracle_param* params, i
                         out[w] += t;
                     }
                 }
-            } else if (auto* ns = dynamic_cast<Nosanow*>(c)) {
-                if (q.index != 0) throw std::invalid_argument("oracle: the Nosanow factor has one parameter");
-                for (std::size_t p = ns->g.first; p < ns->g.last; ++p)
+            } else if (rp.ns) {
+                const Nosanow& ns = *rp.ns;
+                for (std::size_t p = ns.g.first; p < ns.g.last; ++p)
                     for (std::size_t w = 0; w < nw; ++w) {
-                        double d0 = s.P(p, 0, w) - ns->sites[(p - ns->g.first) * 2];
-                        double d1 = s.P(p, 1, w) - ns->sites[(p - ns->g.first) * 2 + 1];
+                        double d0 = s.P(p, 0, w) - ns.sites[(p - ns.g.first) * 2];
+                        double d1 = s.P(p, 1, w) - ns.sites[(p - ns.g.first) * 2 + 1];
                         if (s.cell.periodic) s.cell.min_image(d0, d1);
                         out[w] -= d0 * d0 + d1 * d1;
                     }
             } else {
-                throw std::invalid_argument("oracle: this factor has no free parameters here");
+                const auto& B = rp.det->blocks[static_cast<std::size_t>(rp.block)];
+                const std::size_t N = B.N;
+                const double alpha = B.orb.alphas[0];
+                for (std::size_t w = 0; w < nw; ++w) {
+                    const double* A = B.A.data() + w * N * N;
+                    double t = 0.0;
+                    for (std::size_t c = 0; c < N; ++c)
+                        for (std::size_t m = 0; m < N; ++m) {
+                            double d0 = s.P(B.first + c, 0, w) - B.orb.centers[m * 2];
+                            double d1 = s.P(B.first + c, 1, w) - B.orb.centers[m * 2 + 1];
+                            if (s.cell.periodic) s.cell.min_image(d0, d1);
+                            double v, g[2], l[2];
+                            gauss_alpha_terms(alpha, d0, d1, v, g, l);
+                            t += A[m * N + c] * v;
+                        }
+                    out[w] = t;
+                }
             }
+        }
+    });
+}
+
+// Hamiltonian::local_energy_parameter_derivatives (hamiltonian.cpp:189-196)
+// fed by TrialWavefunction::compute_parameter_gradient_derivatives
+// (trial_wavefunction.cpp:162-185): for each listed parameter the change of
+// every particle's gradient and Laplacian of log|psi|, reduced by the kinetic
+// term (kinetic.cpp:176-200, 212-250) as
+//   dE/dp = sum_{i,d} -1/2 (1/m_i) [dL + 2 G dG]   (real part)
+//   dE_im/dp = sum_{i,d} -1/2 (1/m_i) 2 dG v        (v the phase gradient)
+// with G the whole trial's gradient.  A pair factor's f = du/dp enters as u
+// does: dG_i += f' delta/r (- for the partner), dL += f'' (delta_d/r)^2 +
+// (f'/r)(1 - (delta_d/r)^2) for both (pade_jastrow.cpp:806-880); summed over
+// images the reciprocal part takes the parameter's weights in the
+// structure-factor gradient and Laplacian (kspace_pair_sum.cpp:315-320).
+// Nosanow: dG = -2 delta, dL = -2 per dimension.  A Gaussian block's width
+// (slater_determinant.cpp:159-251): dA = -A dD A, dG_j = sum_m dA(m,j)
+// dphi_m(r_j) + A(m,j) d(dphi_m)/dalpha, the same for the second
+// derivatives, and dL = dLraw - 2 G_det dG with the block's own gradient.
+int oracle_local_energy_parameter_derivatives(oracle_system* h, const oracle_param* params, int n,
+                                              double* de_re, double* de_im) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        const std::size_t nw = s.nw, np = s.np, stride = np * 2 * nw;
+        refresh_all(s);
+        std::vector<double> G, L, PG, PL;
+        totals(s, G, L, &PG, &PL);
+        std::vector<double> dg(stride), dl(stride);
+        for (int i = 0; i < n; ++i) {
+            const oracle_param& q = params[i];
+            ResolvedParam rp = resolve_param(s, q);
+            std::fill(dg.begin(), dg.end(), 0.0);
+            std::fill(dl.begin(), dl.end(), 0.0);
+            auto at = [&](std::size_t p, int d, std::size_t w) { return (p * 2 + d) * nw + w; };
+            if (rp.pj) {
+                const PairJastrow& pj = *rp.pj;
+                const ParamRadial pr{pj.rad, q.index, &rp.sp, q.limit};
+                for_pairs(pj, [&](std::size_t a, std::size_t b) {
+                    for (std::size_t w = 0; w < nw; ++w) {
+                        double d0 = s.P(a, 0, w) - s.P(b, 0, w), d1 = s.P(a, 1, w) - s.P(b, 1, w);
+                        if (s.cell.periodic) s.cell.min_image(d0, d1);
+                        const double r = s.D(a, b, w);
+                        double f, f1, f2;
+                        pr.eval(r, f, f1, f2);
+                        const double del[2] = {d0, d1};
+                        for (int d = 0; d < 2; ++d) {
+                            const double o = del[d] / r, gd = f1 * o, ld = f2 * o * o + f1 / r * (1.0 - o * o);
+                            dg[at(a, d, w)] += gd;
+                            dg[at(b, d, w)] -= gd;
+                            dl[at(a, d, w)] += ld;
+                            dl[at(b, d, w)] += ld;
+                        }
+                    }
+                });
+                if (pj.rad.image) {
+                    const KSpace& k = pj.ks;
+                    const std::size_t K = k.nk();
+                    std::vector<double> c(K), sn(K);
+                    for (std::size_t w = 0; w < nw; ++w) {
+                        const double* ar = k.ra_re.data() + w * K;
+                        const double* ai = k.ra_im.data() + w * K;
+                        const double* br = k.same ? ar : k.rb_re.data() + w * K;
+                        const double* bi = k.same ? ai : k.rb_im.data() + w * K;
+                        auto add = [&](std::size_t p, const double* pr_, const double* pi_, double self) {
+                            k.phases(s.P(p, 0, w), s.P(p, 1, w), c.data(), sn.data());
+                            for (std::size_t kk = 0; kk < K; ++kk) {
+                                const double g = 2.0 * q.weights[kk] * (c[kk] * pi_[kk] - sn[kk] * pr_[kk]);
+                                const double l = 2.0 * q.weights[kk] * (self - (c[kk] * pr_[kk] + sn[kk] * pi_[kk]));
+                                dg[at(p, 0, w)] += k.kx[kk] * g;
+                                dg[at(p, 1, w)] += k.ky[kk] * g;
+                                dl[at(p, 0, w)] += k.kx2[kk] * l;
+                                dl[at(p, 1, w)] += k.ky2[kk] * l;
+                            }
+                        };
+                        for (std::size_t p = k.a.first; p < k.a.last; ++p) add(p, br, bi, k.same ? 1.0 : 0.0);
+                        if (!k.same)
+                            for (std::size_t p = k.b.first; p < k.b.last; ++p) add(p, ar, ai, 0.0);
+                    }
+                }
+            } else if (rp.ns) {
+                const Nosanow& ns = *rp.ns;
+                for (std::size_t p = ns.g.first; p < ns.g.last; ++p)
+                    for (std::size_t w = 0; w < nw; ++w) {
+                        double d0 = s.P(p, 0, w) - ns.sites[(p - ns.g.first) * 2];
+                        double d1 = s.P(p, 1, w) - ns.sites[(p - ns.g.first) * 2 + 1];
+                        if (s.cell.periodic) s.cell.min_image(d0, d1);
+                        dg[at(p, 0, w)] = -2.0 * d0;
+                        dg[at(p, 1, w)] = -2.0 * d1;
+                        dl[at(p, 0, w)] = -2.0;
+                        dl[at(p, 1, w)] = -2.0;
+                    }
+            } else {
+                const auto& B = rp.det->blocks[static_cast<std::size_t>(rp.block)];
+                const std::size_t N = B.N;
+                const double alpha = B.orb.alphas[0];
+                std::vector<double> dD(N * N), Y(N * N), dA(N * N), val(N), og(N * 2), ol(N * 2);
+                for (std::size_t w = 0; w < nw; ++w) {
+                    const double* A = B.A.data() + w * N * N;
+                    auto sep = [&](std::size_t c, std::size_t m, double& d0, double& d1) {
+                        d0 = s.P(B.first + c, 0, w) - B.orb.centers[m * 2];
+                        d1 = s.P(B.first + c, 1, w) - B.orb.centers[m * 2 + 1];
+                        if (s.cell.periodic) s.cell.min_image(d0, d1);
+                    };
+                    for (std::size_t c = 0; c < N; ++c)
+                        for (std::size_t m = 0; m < N; ++m) {
+                            double d0, d1, v, g[2], l[2];
+                            sep(c, m, d0, d1);
+                            gauss_alpha_terms(alpha, d0, d1, v, g, l);
+                            dD[c * N + m] = v;   // dD(particle c, orbital m)
+                        }
+                    // Y = dD A (particle x particle), dA = -A Y (orbital x particle)
+                    for (std::size_t c = 0; c < N; ++c)
+                        for (std::size_t j = 0; j < N; ++j) {
+                            double t = 0.0;
+                            for (std::size_t m = 0; m < N; ++m) t += dD[c * N + m] * A[m * N + j];
+                            Y[c * N + j] = t;
+                        }
+                    for (std::size_t m = 0; m < N; ++m)
+                        for (std::size_t j = 0; j < N; ++j) {
+                            double t = 0.0;
+                            for (std::size_t c = 0; c < N; ++c) t += A[m * N + c] * Y[c * N + j];
+                            dA[m * N + j] = -t;
+                        }
+                    for (std::size_t j = 0; j < N; ++j) {
+                        B.orb.eval(s.cell, s.P(B.first + j, 0, w), s.P(B.first + j, 1, w), val.data(), og.data(),
+                                   ol.data());
+                        for (int d = 0; d < 2; ++d) {
+                            double Gd = 0.0, dG = 0.0, dLr = 0.0;
+                            for (std::size_t m = 0; m < N; ++m) {
+                                double d0, d1, v, g[2], l[2];
+                                sep(j, m, d0, d1);
+                                gauss_alpha_terms(alpha, d0, d1, v, g, l);
+                                const double a = A[m * N + j], da = dA[m * N + j];
+                                Gd += a * og[m * 2 + d];
+                                dG += da * og[m * 2 + d] + a * g[d];
+                                dLr += da * ol[m * 2 + d] + a * l[d];
+                            }
+                            dg[at(B.first + j, d, w)] = dG;
+                            dl[at(B.first + j, d, w)] = dLr - 2.0 * Gd * dG;
+                        }
+                    }
+                }
+            }
+            double* re = de_re + static_cast<std::size_t>(i) * nw;
+            std::fill(re, re + nw, 0.0);
+            double* im = de_im ? de_im + static_cast<std::size_t>(i) * nw : nullptr;
+            if (im) std::fill(im, im + nw, 0.0);
+            for (std::size_t p = 0; p < np; ++p)
+                for (int d = 0; d < 2; ++d) {
+                    const double mi = s.inv_mass[p];
+                    for (std::size_t w = 0; w < nw; ++w) {
+                        const std::size_t o = at(p, d, w);
+                        re[w] += -0.5 * mi * (dl[o] + 2.0 * G[o] * dg[o]);
+                        if (im && s.has_phase) im[w] += -0.5 * mi * (2.0 * dg[o] * PG[o]);
+                    }
+                }
         }
     });
 }

@@ -105,6 +105,8 @@ typedef struct oracle_param {
     double r_cut, inv_delta, limit;
 } oracle_param;
 int oracle_parameter_derivatives(oracle_system* s, const oracle_param* params, int n, double* derivs);
+int oracle_local_energy_parameter_derivatives(oracle_system* s, const oracle_param* params, int n,
+                                              double* de_re, double* de_im);
 int oracle_refresh_inverse(oracle_system* s, int w_begin, int w_end);
 int oracle_check_status(oracle_system* s);
 int oracle_dmc_begin(oracle_system* s);

@@ -285,7 +285,7 @@ int ref_add_drummond_jastrow(void* h, int ga, int gb, int C, double gamma, doubl
         std::vector<alea::Parameter> params;
         for (int k = 0; k < n_coeffs; ++k)
             params.push_back(alea::Parameter{"a" + std::to_string(k), coeffs[k], !s.take_free()});
-        params.push_back(alea::Parameter{"L", L, true});
+        params.push_back(alea::Parameter{"L", L, !s.take_free()});
         s.psi.add_component(std::make_unique<alea::DrummondJastrow>(
             C, gamma, s.groups[static_cast<std::size_t>(ga)],
             s.groups[static_cast<std::size_t>(gb)], std::move(params)));
@@ -390,8 +390,8 @@ int ref_local_energy(void* h, double* e_re) {
 }
 
 // Free flags for the parameters of the factors added next, in the order they
-// are added (Padé: its coefficients; Drummond: its coefficients, L stays
-// fixed; Nosanow: alpha).
+// are added (Padé: its coefficients; Drummond: its coefficients, then L;
+// Nosanow: alpha).
 int ref_set_free(void* h, int n, const unsigned char* flags) {
     auto& s = *static_cast<RefSystem*>(h);
     s.free_next.assign(flags, flags + n);
@@ -408,6 +408,29 @@ int ref_parameter_derivatives(void* h, double* derivs, int cap, int* n_opt) {
         *n_opt = static_cast<int>(n);
         if (static_cast<int>(n * s.nw) > cap) throw std::runtime_error("derivs too small");
         s.psi.compute_parameter_derivatives(*s.ws, std::span<double>(derivs, n * s.nw));
+    });
+}
+
+// TrialWavefunction::compute_parameter_gradient_derivatives
+// (trial_wavefunction.cpp:162-185) after refresh_derivatives, reduced by
+// Hamiltonian::local_energy_parameter_derivatives (hamiltonian.cpp:189-196):
+// de [n_opt][nw]; *n_opt the count.
+int ref_local_energy_parameter_derivatives(void* h, double* de, int cap, int* n_opt) {
+    return guarded([&] {
+        auto& s = *static_cast<RefSystem*>(h);
+        build_hamiltonian(s);
+        s.psi.refresh_derivatives(*s.ws);
+        const std::size_t n = s.psi.total_optimizable_params();
+        *n_opt = static_cast<int>(n);
+        if (static_cast<int>(n * s.nw) > cap) throw std::runtime_error("de too small");
+        const std::size_t block = s.np * s.dim * s.nw;
+        std::vector<double> dgrad(n * block), dlap(n * block);
+        std::vector<char> supported(n);
+        s.psi.compute_parameter_gradient_derivatives(*s.ws, dgrad, dlap, supported);
+        for (std::size_t k = 0; k < n; ++k)
+            if (!supported[k]) throw std::runtime_error("gradient derivatives not supported");
+        std::fill(de, de + n * s.nw, 0.0);
+        s.ham.local_energy_parameter_derivatives(*s.ws, dgrad, dlap, n, std::span<double>(de, n * s.nw));
     });
 }
 

@@ -134,6 +134,7 @@ def ref_lib():
             "ref_delta_potential": (C.c_int, [C.c_void_p, C.c_int, D, D]),
             "ref_set_free": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_ubyte)]),
             "ref_parameter_derivatives": (C.c_int, [C.c_void_p, D, C.c_int, C.POINTER(C.c_int)]),
+            "ref_local_energy_parameter_derivatives": (C.c_int, [C.c_void_p, D, C.c_int, C.POINTER(C.c_int)]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(lib, name)
@@ -229,6 +230,13 @@ class RefWalkers:
         cap = 64 * self.nw
         out = np.zeros(cap)
         self._chk(self.lib.ref_parameter_derivatives(self.h, ptr(out), cap, C.byref(n)))
+        return out[:n.value * self.nw].reshape(n.value, self.nw)
+
+    def local_energy_parameter_derivatives(self):
+        n = C.c_int()
+        cap = 64 * self.nw
+        out = np.zeros(cap)
+        self._chk(self.lib.ref_local_energy_parameter_derivatives(self.h, ptr(out), cap, C.byref(n)))
         return out[:n.value * self.nw].reshape(n.value, self.nw)
 
     def delta_potential(self, p, new_pos):

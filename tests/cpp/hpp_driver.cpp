@@ -162,6 +162,15 @@ int run(const char* spec_path, const char* in_path, const char* out_path) {
     WalkerSet ws(nw, np, 2, lattice, precision);
     TrialWavefunction psi(ws);
     Hamiltonian ham(ws);
+    // every parameter that needs no breakup tables: minimum-image Padé
+    // coefficients, Drummond coefficients and L, Nosanow alpha, a uniform
+    // Gaussian block's width
+    std::vector<alea_param> free;
+    int factor = 0;
+    auto take = [&](int n) {
+        for (int i = 0; i < n; ++i) free.push_back(alea_param{factor, i, nullptr, nullptr, 0, 0.0, 0.0, 0.0});
+        ++factor;
+    };
     for (std::string key = sp.word(); key != "end"; key = sp.word()) {
         if (key == "group") {
             ParticleGroup g;
@@ -183,6 +192,7 @@ int run(const char* spec_path, const char* in_path, const char* out_path) {
                 j.u_inf = sp.num();
             }
             psi.add_component(j);
+            take(j.image_sum ? 0 : j.num_order + j.den_order);
         } else if (key == "drummond") {
             DrummondJastrow j;
             j.group_a = static_cast<int>(sp.integer());
@@ -192,12 +202,14 @@ int run(const char* spec_path, const char* in_path, const char* out_path) {
             j.L = sp.num();
             j.params = sp.vec();
             psi.add_component(j);
+            take(static_cast<int>(j.params.size()) + 1);
         } else if (key == "nosanow") {
             NosanowJastrow j;
             j.group = static_cast<int>(sp.integer());
             j.alpha = sp.num();
             j.sites = sp.vec();
             psi.add_component(j);
+            take(1);
         } else if (key == "mcmillan") {
             McMillanJastrow j;
             j.group_a = static_cast<int>(sp.integer());
@@ -205,6 +217,7 @@ int run(const char* spec_path, const char* in_path, const char* out_path) {
             j.b = sp.num();
             j.n = static_cast<int>(sp.integer());
             psi.add_component(j);
+            take(0);
         } else if (key == "slater_pw") {
             PlaneWaveSlater d;
             d.first = static_cast<std::size_t>(sp.integer());
@@ -212,12 +225,14 @@ int run(const char* spec_path, const char* in_path, const char* out_path) {
             d.kvecs = sp.vec();
             for (int v : sp.ivec()) d.is_sin.push_back(static_cast<unsigned char>(v));
             psi.add_component(d);
+            take(0);
         } else if (key == "slater_twisted") {
             TwistedSlater d;
             d.first = static_cast<std::size_t>(sp.integer());
             d.last = static_cast<std::size_t>(sp.integer());
             d.q = sp.vec();
             psi.add_component(d);
+            take(0);
         } else if (key == "slater_gauss") {
             GaussianSlater d;
             d.first = static_cast<std::size_t>(sp.integer());
@@ -225,6 +240,7 @@ int run(const char* spec_path, const char* in_path, const char* out_path) {
             d.centers = sp.vec();
             d.alphas = sp.vec();
             psi.add_component(d);
+            take(std::all_of(d.alphas.begin(), d.alphas.end(), [&](double a) { return a == d.alphas[0]; }) ? 1 : 0);
         } else if (key == "ewald") {
             EwaldTerm e;
             e.table = sp.table();
@@ -292,6 +308,14 @@ int run(const char* spec_path, const char* in_path, const char* out_path) {
     }
     out.put("log_psi1", ws.log_psi());
     out.put("e1", ham.evaluate_batch(ws));
+    if (!free.empty()) {
+        std::vector<double> dlp, de, de_im;
+        psi.compute_parameter_derivatives(ws, free, dlp);
+        ham.local_energy_parameter_derivatives(ws, free, de, &de_im);
+        out.put("dlogpsi", dlp);
+        out.put("de", de);
+        out.put("de_im", de_im);
+    }
     // VMC sweeps from a tape
     const double tau = in.num();
     const int n_sweeps = static_cast<int>(in.integer());

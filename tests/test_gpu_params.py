@@ -1,19 +1,26 @@
-"""d log|psi| / d p on the device (alea_parameter_derivatives) against the
-oracle: TrialWavefunction::compute_parameter_derivatives
-(trial_wavefunction.cpp:200-236) over PadeJastrow (pade_jastrow.cpp:227-300,
-both the minimum-image and the image-summed form, the latter through
-PeriodicPairSum's per-parameter breakups), DrummondJastrow
-(drummond_jastrow.cpp:150-190) and NosanowJastrow (nosanow_jastrow.cpp:95-110).
+"""The optimizer's derivatives on the device against the oracle:
+d log|psi| / d p (alea_parameter_derivatives,
+TrialWavefunction::compute_parameter_derivatives, trial_wavefunction.cpp:
+150-160) and dE_L / d p (alea_local_energy_parameter_derivatives,
+Hamiltonian::local_energy_parameter_derivatives over
+compute_parameter_gradient_derivatives, hamiltonian.cpp:189-196,
+trial_wavefunction.cpp:162-185), over PadeJastrow (pade_jastrow.cpp:641-880,
+the minimum-image and the image-summed form, the latter through
+PeriodicPairSum's per-parameter breakups), DrummondJastrow (coefficients and
+L, drummond_jastrow.cpp:201-420), NosanowJastrow (nosanow_jastrow.cpp:176-210)
+and a Gaussian determinant's shared width (slater_determinant.cpp:98-251).
 
 Both sides take the same breakup tables of du/dp (breakup.pade_parameter_tables),
-so the sums agree to 1e-10 relative; the oracle itself is held to the
-reference's own derivatives and breakups in the CPU suite.
+so the FP64 results agree to 1e-10 relative; the oracle itself is held to the
+reference's own derivatives and breakups, and to finite differences of its
+own log|psi| and E_L, in the CPU suite.
 """
 import numpy as np
 import pytest
 
-from alea_amd.system import dipole_spec, free_parameters, heg_spec, uniform_positions
-from alea_amd.walkers import FP64, gpu_walkers
+from alea_amd.system import dipole_spec, free_parameters, heg_spec, site_positions, uniform_positions, \
+    wigner_crystal_spec
+from alea_amd.walkers import FP64, MIXED, gpu_walkers
 from backends import oracle_tape, oracle_walkers, rel_err
 
 pytestmark = pytest.mark.gpu
@@ -33,42 +40,89 @@ def _cases():
     d.drummond.append(dict(ga=0, gb=1, C=3, gamma=1.0, L=d.lattice[0] / 2, coeffs=[1e-3, -2e-4, 3e-5]))
     d.nosanow.append(dict(group=1, alpha=0.6, sites=np.random.default_rng(9).uniform(-5, 5, 10)))
     d.name = "drummond+nosanow N=10"
-    out.append((d, [(0, 0), (0, 1), (0, 2), (1, 0)], 2.0))
+    out.append((d, [(0, 0), (0, 1), (0, 2), (0, 3), (1, 0)], 2.0))
     p = dipole_spec(16, density=4.0)
     p.mcmillan = []
     p.pade.append(dict(ga=0, gb=0, num=1, den=1, coeffs=[0.3, 1.0 / np.sqrt(np.pi * 4.0)]))
     p.name = "dipoles N=16 minimum-image Padé"
     out.append((p, [(0, 0), (0, 1)], 0.3))
+    w = wigner_crystal_spec(rs=20.0, rows=2, cols=4, nosanow_alpha=0.05, drummond_sigma=0.1)
+    w.name = "wigner crystal N=8"
+    out.append((w, [(0, 1), (1, 0), (1, 8), (2, 0), (3, 0)], 2.0))
+    w = wigner_crystal_spec()
+    w.name = "wigner crystal N=128"
+    out.append((w, [(1, 8), (2, 0), (3, 0)], 3.0))
     return out
+
+
+def _positions(spec, nw, seed_):
+    rng = np.random.default_rng(seed_)
+    if spec.slater_gauss:
+        return site_positions(spec, nw, rng, 0.2 * np.sqrt(spec.lattice[0] * spec.lattice[3] / spec.n_particles))
+    return uniform_positions(spec, nw, rng)
 
 
 @pytest.mark.parametrize("case", _cases(), ids=lambda c: c[0].name)
 def test_parameter_derivatives_match_the_oracle(case):
     """Fresh walkers, then after two sweeps from one tape: every free
-    parameter's derivative on every walker to 1e-10 relative."""
+    parameter's d log|psi|/dp and dE_L/dp (both parts of a complex trial's)
+    on every walker to 1e-10 relative."""
     spec, free, rs = case
-    nw = 40
+    nw = 8 if spec.n_particles > 100 and spec.slater_gauss else 40
     params = free_parameters(spec, free)
-    pos = uniform_positions(spec, nw, np.random.default_rng(12))
+    pos = _positions(spec, nw, 12)
     g, o = gpu_walkers(spec, nw, FP64), oracle_walkers(spec, nw)
-    try:
-        for x in (g, o):
-            x.set_positions(pos)
-            x.evaluate()
+
+    def check():
         dg, do = g.parameter_derivatives(params), o.parameter_derivatives(params)
         assert dg.shape == (len(free), nw)
         assert rel_err(dg, do) < TOL
         assert np.all(np.abs(do).max(axis=1) > 0.0)
+        (eg, eg_im), (eo, eo_im) = g.local_energy_parameter_derivatives(params), \
+            o.local_energy_parameter_derivatives(params)
+        assert rel_err(eg, eo) < TOL
+        assert np.all(np.abs(eo).max(axis=1) > 0.0)
+        if spec.slater_twisted:
+            assert rel_err(eg_im, eo_im) < TOL and np.abs(eo_im).max() > 0.0
+        else:
+            assert np.all(eg_im == 0.0) and np.all(eo_im == 0.0)
+
+    try:
+        for x in (g, o):
+            x.set_positions(pos)
+            x.evaluate()
+        check()
         tape = oracle_tape(14, spec.n_particles, nw, 2)
-        tau = (0.5 * rs) ** 2
+        tau = (0.1 * rs) ** 2 if spec.slater_gauss else (0.5 * rs) ** 2
         assert g.vmc_sweeps(2, tau, tape) == o.vmc_sweeps(2, tau, tape)
-        assert rel_err(g.parameter_derivatives(params), o.parameter_derivatives(params)) < TOL
+        check()
     finally:
         g.close()
 
 
+def test_gaussian_width_derivatives_in_mixed_precision():
+    """The mixed form reads the single-precision inverse: on the full Wigner
+    crystal (one 128-orbital Gaussian block) d log|psi|/d alpha and
+    dE_L/d alpha agree with the FP64 device's to 1e-4 relative."""
+    spec = wigner_crystal_spec()
+    nw = 16
+    params = free_parameters(spec, [(3, 0), (2, 0)])
+    pos = _positions(spec, nw, 15)
+    out = []
+    for prec in (FP64, MIXED):
+        g = gpu_walkers(spec, nw, prec)
+        try:
+            g.set_positions(pos)
+            g.evaluate()
+            out.append((g.parameter_derivatives(params), g.local_energy_parameter_derivatives(params)[0]))
+        finally:
+            g.close()
+    assert rel_err(out[1][0], out[0][0]) < 1e-4
+    assert rel_err(out[1][1], out[0][1]) < 1e-4
+
+
 def test_parameter_derivatives_reject_what_has_no_parameter():
-    """The determinant and a McMillan factor have no free parameters, and
+    """A plane-wave determinant has no free parameters, and
     an image-summed Padé parameter needs its tables: each is refused with the
     library's error, nothing launched."""
     spec = heg_spec(13, 13, 30.0)
@@ -78,6 +132,8 @@ def test_parameter_derivatives_reject_what_has_no_parameter():
         g.evaluate()
         with pytest.raises(RuntimeError, match="no free parameters"):
             g.parameter_derivatives([{"factor": len(spec.pade), "index": 0}])
+        with pytest.raises(RuntimeError, match="no free parameters"):
+            g.local_energy_parameter_derivatives([{"factor": len(spec.pade), "index": 0}])
         with pytest.raises(RuntimeError, match="needs its tables"):
             g.parameter_derivatives([{"factor": 0, "index": 0}])
         with pytest.raises(RuntimeError, match="out of range"):

@@ -98,14 +98,15 @@ def spec_text(s, nw, precision):
         lines.append(f"drummond {d['ga']} {d['gb']} {d['C']} {float(d['gamma'])!r} {float(d['L'])!r} {_arr(d['coeffs'])}")
     for n in s.nosanow:
         lines.append(f"nosanow {n['group']} {float(n['alpha'])!r} {_arr(n['sites'])}")
-    for ga, gb, b, n in s.mcmillan:
-        lines.append(f"mcmillan {ga} {gb} {float(b)!r} {n}")
     for first, last, k, sn in s.slater_pw:
         lines.append(f"slater_pw {first} {last} {_arr(np.asarray(k, dtype=np.float64))} {_arr(np.asarray(sn, dtype=np.int64))}")
     for first, last, q in s.slater_twisted:
         lines.append(f"slater_twisted {first} {last} {_arr(np.asarray(q, dtype=np.float64))}")
     for first, last, cen, al in s.slater_gauss:
         lines.append(f"slater_gauss {first} {last} {_arr(np.asarray(cen, dtype=np.float64))} {_arr(np.asarray(al, dtype=np.float64))}")
+    # the factors in the order Walkers registers them, so that both count them alike
+    for ga, gb, b, n in s.mcmillan:
+        lines.append(f"mcmillan {ga} {gb} {float(b)!r} {n}")
     if s.ewald:
         e = s.ewald
         lines.append(f"ewald {_table(e)} {float(e['v_long_zero'])!r} {float(e['sr_k0'])!r}")
@@ -177,6 +178,11 @@ def _python_run(make, case):
             w.accept(p, mask)
         out["log_psi1"] = w.log_psi()
         out["e1"] = w.local_energy()[0]
+        free = _table_free_parameters(w.spec)
+        if free:
+            out["dlogpsi"] = w.parameter_derivatives(free).reshape(-1)
+            de, de_im = w.local_energy_parameter_derivatives(free)
+            out["de"], out["de_im"] = de.reshape(-1), de_im.reshape(-1)
         out["vmc_accepted"] = np.array([float(w.vmc_sweeps(n_sw, tau, tape))])
         out["pos2"] = w.positions()
         out["log_psi2"] = w.log_psi()
@@ -201,8 +207,31 @@ def _python_run(make, case):
     return out
 
 
+def _table_free_parameters(spec):
+    """The parameters the driver frees: those that need no breakup tables
+    (minimum-image Padé coefficients, Drummond coefficients and L, Nosanow
+    alpha, a uniform Gaussian block's width), factors in registration order."""
+    free, f = [], 0
+    for j in spec.pade:
+        free += [] if j.get("image") else [(f, i) for i in range(j["num"] + j["den"])]
+        f += 1
+    for d in spec.drummond:
+        free += [(f, i) for i in range(len(d["coeffs"]) + 1)]
+        f += 1
+    for _ in spec.nosanow:
+        free.append((f, 0))
+        f += 1
+    f += len(spec.slater_pw) + len(spec.slater_twisted)
+    for first, last, cen, al in spec.slater_gauss:
+        if np.all(np.asarray(al) == np.asarray(al)[0]):
+            free.append((f, 0))
+        f += 1
+    return [{"factor": a, "index": b} for a, b in free]
+
+
 def _specs():
     a = heg_spec(5, 5, 2.0, twist=(0.13, 0.27))
+    a.drummond.append(dict(ga=0, gb=1, C=3, gamma=1.0, L=a.lattice[0] / 2, coeffs=[1e-3, -2e-4, 3e-5]))
     a.name = "twisted N=10"
     b = dipole_spec(16, density=4.0)
     b.name = "dipoles N=16"

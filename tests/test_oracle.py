@@ -276,7 +276,7 @@ def _ref_param_tables(lattice, a, b, q):
 def _free_cases():
     """(spec, free (factor, index) pairs) with every parameter kind free: the
     image-summed Padé's b (both spin pairs, the reference's tables), its a
-    too on one factor; Drummond alpha_0 and alpha_2 and Nosanow alpha; a
+    too on one factor; Drummond alpha_0, alpha_2 and L and Nosanow alpha; a
     minimum-image Padé's a and b on the dipoles."""
     out = []
     s = heg_spec(5, 5, 2.0, slater=False, tables=_ref_tables(10, 2.0))
@@ -287,7 +287,7 @@ def _free_cases():
     dr.drummond.append(dict(ga=0, gb=1, C=3, gamma=1.0, L=L, coeffs=[1e-3, -2e-4, 3e-5]))
     dr.nosanow.append(dict(group=1, alpha=0.6, sites=np.random.default_rng(9).uniform(-5, 5, 10)))
     dr.name = "drummond+nosanow N=10"
-    out.append((dr, [(0, 0), (0, 2), (1, 0)]))
+    out.append((dr, [(0, 0), (0, 2), (0, 3), (1, 0)]))
     dp = dipole_spec(16, density=4.0)
     dp.mcmillan = []
     rs = 1.0 / np.sqrt(np.pi * 4.0)
@@ -299,8 +299,8 @@ def _free_cases():
 
 def _free_flags(spec, free):
     """ref_set_free's flags in factor order (Padé coefficients, Drummond
-    coefficients, Nosanow alpha)."""
-    sizes = [j["num"] + j["den"] for j in spec.pade] + [len(d["coeffs"]) for d in spec.drummond] + \
+    coefficients then L, Nosanow alpha)."""
+    sizes = [j["num"] + j["den"] for j in spec.pade] + [len(d["coeffs"]) + 1 for d in spec.drummond] + \
         [1 for _ in spec.nosanow]
     flags = []
     for f, n in enumerate(sizes):
@@ -335,43 +335,129 @@ def test_parameter_derivatives_match_reference(case):
     assert rel_err(o.parameter_derivatives(params), r.parameter_derivatives()) < 1e-10
 
 
-def test_parameter_derivatives_are_the_slope_of_log_psi():
-    """Without images the derivative is the slope of log|psi| itself: central
-    differences of the oracle's log|psi| in each Padé coefficient, Drummond
-    coefficient and the Nosanow alpha agree to 1e-6."""
+@pytest.mark.parametrize("case", _free_cases(), ids=lambda c: c[0].name)
+def test_local_energy_parameter_derivatives_match_reference(case):
+    """dE_L/dp of the oracle against the reference's
+    compute_parameter_gradient_derivatives reduced by
+    Hamiltonian::local_energy_parameter_derivatives (compiled in oracle/_ref),
+    fresh and after sweeps, to 1e-10 relative; the image-summed factors take
+    the reference's breakups of du/dp in both."""
     from alea_amd.system import free_parameters
+    spec, free = case
+    nw = 6
+    pos = uniform_positions(spec, nw, np.random.default_rng(35))
+    tables = lambda f, i: _ref_param_tables(spec.lattice, spec.pade[f]["coeffs"][:spec.pade[f]["num"]],
+                                            spec.pade[f]["coeffs"][spec.pade[f]["num"]:], i)
+    params = free_parameters(spec, sorted(free), tables)
+    o, r = oracle_walkers(spec, nw), RefWalkers(spec, nw, free_flags=_free_flags(spec, free))
+    for x in (o, r):
+        x.set_positions(pos)
+        x.evaluate()
+    de, im = o.local_energy_parameter_derivatives(params)
+    dr_ = r.local_energy_parameter_derivatives()
+    assert de.shape == dr_.shape
+    assert rel_err(de, dr_) < 1e-10
+    assert np.all(im == 0.0)
+    tape = oracle_tape(52, spec.n_particles, nw, 2)
+    tau = (spec.lattice[0] / 8) ** 2
+    assert o.vmc_sweeps(2, tau, tape) == r.vmc_sweeps(2, tau, tape)
+    assert rel_err(o.local_energy_parameter_derivatives(params)[0], r.local_energy_parameter_derivatives()) < 1e-10
+
+
+def _shifted(spec, f, i, h):
+    """A copy of spec with parameter (f, i) moved by h: factors counted as
+    the library registers them (Padé, Drummond, Nosanow, then the
+    determinant blocks); a Drummond index past its coefficients is L, a
+    Gaussian block's index 0 its shared width."""
     import copy
-    for spec, free in _free_cases()[1:]:
-        nw = 4
-        pos = uniform_positions(spec, nw, np.random.default_rng(34))
-        o = oracle_walkers(spec, nw)
-        o.set_positions(pos)
-        o.evaluate()
-        d = o.parameter_derivatives(free_parameters(spec, sorted(free)))
-        n_pade, n_dr = len(spec.pade), len(spec.drummond)
-        for row, (f, i) in enumerate(sorted(free)):
-            lp = []
-            for sgn in (1.0, -1.0):
-                t = copy.deepcopy(spec)
-                if f < n_pade:
-                    c = list(t.pade[f]["coeffs"])
-                    h = 1e-6 * max(abs(c[i]), 1.0)
-                    c[i] += sgn * h
-                    t.pade[f]["coeffs"] = c
-                elif f < n_pade + n_dr:
-                    c = list(t.drummond[f - n_pade]["coeffs"])
-                    h = 1e-6 * max(abs(c[i]), 1e-3)
-                    c[i] += sgn * h
-                    t.drummond[f - n_pade]["coeffs"] = c
-                else:
-                    h = 1e-6
-                    t.nosanow[f - n_pade - n_dr]["alpha"] += sgn * h
-                x = oracle_walkers(t, nw)
-                x.set_positions(pos)
-                x.evaluate()
-                lp.append(x.log_psi())
-            fd = (lp[0] - lp[1]) / (2 * h)
-            assert np.allclose(d[row], fd, rtol=1e-6, atol=1e-6 * np.abs(fd).max()), (spec.name, f, i)
+    t = copy.deepcopy(spec)
+    n_pade, n_dr, n_nos = len(t.pade), len(t.drummond), len(t.nosanow)
+    if f < n_pade:
+        c = list(t.pade[f]["coeffs"])
+        c[i] += h
+        t.pade[f]["coeffs"] = c
+    elif f < n_pade + n_dr:
+        d = t.drummond[f - n_pade]
+        if i < len(d["coeffs"]):
+            c = list(d["coeffs"])
+            c[i] += h
+            d["coeffs"] = c
+        else:
+            d["L"] += h
+    elif f < n_pade + n_dr + n_nos:
+        t.nosanow[f - n_pade - n_dr]["alpha"] += h
+    else:
+        b = f - n_pade - n_dr - n_nos - len(t.slater_pw) - len(t.slater_twisted)
+        first, last, cen, al = t.slater_gauss[b]
+        t.slater_gauss[b] = (first, last, cen, np.asarray(al) + h)
+    return t
+
+
+def _fd_cases():
+    """Systems whose parameters move without a new breakup (no image-summed
+    factor is varied): Drummond (coefficients and L) with Nosanow, real and
+    twisted (complex E_L); the minimum-image Padé on the dipoles; a small
+    Wigner crystal's Gaussian width, Drummond L and Nosanow alpha."""
+    from alea_amd.system import wigner_crystal_spec
+    out = []
+    c = _free_cases()
+    out.append((c[1][0], c[1][1], 1e-6))
+    tw = heg_spec(5, 5, 2.0, twist=(0.13, 0.27))
+    tw.pade = []
+    tw.drummond.append(dict(ga=0, gb=1, C=3, gamma=1.0, L=tw.lattice[0] / 2, coeffs=[1e-3, -2e-4, 3e-5]))
+    tw.name = "twisted N=10 drummond"
+    out.append((tw, [(0, 0), (0, 1), (0, 3)], 1e-6))
+    out.append((c[2][0], c[2][1], 1e-6))
+    wc = wigner_crystal_spec(rs=20.0, rows=2, cols=4, nosanow_alpha=0.05, drummond_sigma=0.1)
+    wc.name = "wigner crystal N=8"
+    out.append((wc, [(1, 8), (2, 0), (3, 0)], 1e-7))
+    return out
+
+
+@pytest.mark.parametrize("case", _fd_cases(), ids=lambda c: c[0].name)
+def test_parameter_derivatives_are_slopes(case):
+    """Central differences of the oracle's own log|psi| and E_L (both parts)
+    at fixed positions in each free parameter: d log|psi|/dp and dE_L/dp
+    agree to 1e-6 relative (Gaussian width, Drummond coefficients and L,
+    Nosanow alpha, minimum-image Padé coefficients)."""
+    from alea_amd.system import free_parameters, site_positions
+    spec, free, step = case
+    nw = 4
+    rng = np.random.default_rng(34)
+    pos = site_positions(spec, nw, rng, 0.3 * np.sqrt(spec.lattice[0] * spec.lattice[3] / spec.n_particles)) \
+        if spec.slater_gauss else uniform_positions(spec, nw, rng)
+    o = oracle_walkers(spec, nw)
+    o.set_positions(pos)
+    o.evaluate()
+    params = free_parameters(spec, free)
+    d = o.parameter_derivatives(params)
+    de, de_im = o.local_energy_parameter_derivatives(params)
+    for row, (f, i) in enumerate(free):
+        vals = []
+        h = step * max(1.0, abs(_param_value(spec, f, i)))
+        for sgn in (1.0, -1.0):
+            x = oracle_walkers(_shifted(spec, f, i, sgn * h), nw)
+            x.set_positions(pos)
+            x.evaluate()
+            e = x.local_energy()
+            vals.append((x.log_psi(), e[0], e[1]))
+        for got, k in ((d[row], 0), (de[row], 1), (de_im[row], 2)):
+            fd = (vals[0][k] - vals[1][k]) / (2 * h)
+            scale = max(np.abs(fd).max(), 1e-12)
+            assert np.allclose(got, fd, rtol=1e-6, atol=1e-6 * scale), (spec.name, f, i, k, got, fd)
+        assert np.abs(d[row]).max() > 0 and np.abs(de[row]).max() > 0
+
+
+def _param_value(spec, f, i):
+    n_pade, n_dr, n_nos = len(spec.pade), len(spec.drummond), len(spec.nosanow)
+    if f < n_pade:
+        return spec.pade[f]["coeffs"][i]
+    if f < n_pade + n_dr:
+        d = spec.drummond[f - n_pade]
+        return d["coeffs"][i] if i < len(d["coeffs"]) else d["L"]
+    if f < n_pade + n_dr + n_nos:
+        return spec.nosanow[f - n_pade - n_dr]["alpha"]
+    return float(np.asarray(spec.slater_gauss[0][3])[0])
 
 
 @pytest.mark.parametrize("spec", _jastrow_ewald_specs(), ids=lambda s: s.name)
