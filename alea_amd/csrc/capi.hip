@@ -2278,6 +2278,14 @@ int alea_add_density(alea_system* s, double rmax, int n_bins, int per_particle) 
         if (n_bins <= 0) throw std::invalid_argument("alea_add_density: needs n_bins > 0");
         if (!s->host.cell.periodic && !(rmax > 0.0))
             throw std::invalid_argument("alea_add_density: an open cell needs rmax > 0");
+        // per-particle grids are numbered in group declaration order
+        // (density.cpp:46-56) and indexed by particle on the device: the two
+        // agree only for groups declared in ascending particle order
+        if (per_particle)
+            for (int g = 0; g < G; ++g)
+                if (s->host.g_first[g] != (g == 0 ? 0 : s->host.g_last[g - 1]))
+                    throw std::invalid_argument(
+                        "alea_add_density: per-particle grids need the groups declared in ascending particle order");
         const int rows = per_particle ? s->np : G;
         if (static_cast<long long>(rows) * n_bins * n_bins > kMaxDensityEntries)
             throw std::invalid_argument("alea_add_density: grids x n_bins^2 exceeds " +

@@ -412,6 +412,59 @@ def estimators_leg(nw=65536, reps=5):
         w.close()
 
 
+def derivatives_leg(nw=65536, crystal_nw=8192, reps=3):
+    """The optimizer's derivatives on the device (d log|psi|/dp and dE_L/dp
+    of TrialWavefunction::compute_parameter_derivatives and
+    Hamiltonian::local_energy_parameter_derivatives): every Padé coefficient
+    of the N=122 workload (image-summed, complex trial) on `nw` walkers, and
+    the Wigner crystal's (N=128) Gaussian width, Drummond L and Nosanow alpha
+    on `crystal_nw` walkers.  Milliseconds per call from the wall clock
+    around each call (which reads the results back); the kernels' own times
+    come from a kernel trace of the same command."""
+    from alea_amd.system import free_parameters, site_positions, uniform_positions, wigner_crystal_spec
+    from alea_amd.walkers import gpu_walkers
+    out = {}
+    spec, _ = workload_spec()
+    free = [(f, i) for f, j in enumerate(spec.pade) for i in range(j["num"] + j["den"])]
+    params = free_parameters(spec, free)
+    w = gpu_walkers(spec, nw, 0)
+    try:
+        w.set_positions(uniform_positions(spec, nw, np.random.default_rng(POS_SEED)))
+        w.evaluate()
+        w.parameter_derivatives(params)
+        w.local_energy_parameter_derivatives(params)   # warm-up
+        t = []
+        for fn in (w.parameter_derivatives, w.local_energy_parameter_derivatives):
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn(params)
+            t.append((time.perf_counter() - t0) * 1e3 / reps)
+        out["heg122"] = dict(walkers=nw, parameters=len(params), dlogpsi_ms=t[0], de_ms=t[1])
+    finally:
+        w.close()
+    cs = wigner_crystal_spec()
+    nd = len(cs.drummond[0]["coeffs"])
+    params = free_parameters(cs, [(1, nd), (2, 0), (3, 0)])
+    a = math.sqrt(cs.lattice[0] * cs.lattice[3] / cs.n_particles)
+    w = gpu_walkers(cs, crystal_nw, 0)
+    try:
+        w.set_positions(site_positions(cs, crystal_nw, np.random.default_rng(POS_SEED), 0.15 * a))
+        w.evaluate()
+        w.local_energy_parameter_derivatives(params)   # warm-up
+        t = []
+        for fn in (w.parameter_derivatives, w.local_energy_parameter_derivatives):
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn(params)
+            t.append((time.perf_counter() - t0) * 1e3 / reps)
+        n = cs.n_particles
+        out["crystal128"] = dict(walkers=crystal_nw, parameters=len(params), dlogpsi_ms=t[0], de_ms=t[1],
+                                 gauss_width_flops_per_walker=4 * n ** 3)
+    finally:
+        w.close()
+    return out
+
+
 def copy_calibration(n_bytes, reps=3):
     """A plain device-to-device copy of n_bytes, reps times: the known-size
     reference that scales the FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
@@ -448,6 +501,9 @@ def main():
                     help="run the Wigner-crystal DMC leg (FP64 and mixed) alone, print its dict")
     ap.add_argument("--estimators-only", action="store_true",
                     help="time one g(r) + S(k) measurement of the workload's walkers, print its dict")
+    ap.add_argument("--derivatives-only", action="store_true",
+                    help="time the optimizer's parameter derivatives (N=122 Padé, Wigner-crystal width), "
+                         "print their dict")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
@@ -467,6 +523,9 @@ def main():
         return
     if args.estimators_only:
         print(json.dumps(estimators_leg(args.walkers)), flush=True)
+        return
+    if args.derivatives_only:
+        print(json.dumps(derivatives_leg(args.walkers)), flush=True)
         return
     tau = (0.5 * RS) ** 2   # sigma = sqrt(tau/m) = 0.5 rs
     pos = equilibrated_positions(args.walkers, tau, args.equilibration)

@@ -3156,6 +3156,10 @@ int oracle_add_density(oracle_system* h, double rmax, int n_bins, int per_partic
         Sys& s = *S(h);
         if (n_bins <= 0) throw std::invalid_argument("oracle: density needs n_bins > 0");
         if (!s.cell.periodic && !(rmax > 0.0)) throw std::invalid_argument("oracle: density needs rmax > 0");
+        if (per_particle)
+            for (std::size_t g = 0; g < s.groups.size(); ++g)
+                if (s.groups[g].first != (g == 0 ? 0 : s.groups[g - 1].last))
+                    throw std::invalid_argument("oracle: per-particle density needs the groups in ascending order");
         auto& d = s.dens;
         d.on = true;
         d.rmax = rmax;

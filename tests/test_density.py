@@ -111,7 +111,23 @@ def test_density_closed_forms():
         w.close()
 
 
+def _out_of_order(make):
+    """Per-particle grids are numbered in group declaration order
+    (density.cpp:46-56): with the groups declared out of particle order the
+    library refuses them, and keeps the per-group grids."""
+    spec = _spec(square_lattice(9, 2.0), (5, 4), "square")
+    spec.groups = spec.groups[::-1]
+    w = make(spec, 2)
+    try:
+        with pytest.raises(RuntimeError, match="ascending"):
+            w.api("add_density", w.h, 0.0, 8, 1)
+        w.api("add_density", w.h, 0.0, 8, 0)
+    finally:
+        w.close()
+
+
 def test_density_refusals():
+    _out_of_order(oracle_walkers)
     spec = _spec(square_lattice(9, 2.0), (9,), "square")
     w = oracle_walkers(spec, 2)
     try:
@@ -126,6 +142,12 @@ def test_density_refusals():
             w.api("add_density", w.h, 0.0, 8, 0)     # an open cell needs a window
     finally:
         w.close()
+
+
+@pytest.mark.gpu
+def test_gpu_density_refuses_out_of_order_groups():
+    from alea_amd.walkers import gpu_walkers
+    _out_of_order(gpu_walkers)
 
 
 @pytest.mark.gpu
