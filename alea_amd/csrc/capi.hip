@@ -1383,8 +1383,9 @@ int alea_local_energy_parameter_derivatives(alea_system* s, const alea_param* pa
         for (int i = 0; i < n; ++i) {
             const DevParam& q = pk.host[static_cast<size_t>(i)];
             if (q.block < 0) continue;
-            // dD A for a chunk of walkers at a time, n^2 doubles each, within 256 MiB
-            const size_t nb = static_cast<size_t>(s->host.blk_n[q.block]);
+            // dD A for a chunk of walkers at a time, n16^2 doubles each (the
+            // block padded to a multiple of 16), within 256 MiB
+            const size_t nb = static_cast<size_t>((s->host.blk_n[q.block] + 15) & ~15);
             const int chunk = static_cast<int>(std::max<size_t>(1, std::min(nw, (size_t{256} << 20) / (nb * nb * 8))));
             if (ybuf.n < static_cast<size_t>(chunk) * nb * nb) ybuf.alloc(static_cast<size_t>(chunk) * nb * nb, s->stream);
             for (int w0 = 0; w0 < s->nw; w0 += chunk) {

@@ -2960,94 +2960,165 @@ __global__ __launch_bounds__(64) void k_param_eloc(const DevModel* __restrict__ 
 // moves with alpha: dA = -A dD A with dD(c, m) = dphi_m(r_c)/dalpha, so
 //   dG_j = sum_m dA(m,j) dphi_m(r_j) + A(m,j) d(dphi_m(r_j))/dalpha,
 // the same for the second derivatives, and dL = dLraw - 2 G_det dG with the
-// block's own gradient G_det.  One workgroup of 256 per walker of the chunk
-// [w0, w0 + gridDim.x): thread t takes column j = t % n and row phase
-// r = t / n (R = 256 / n phases).  Pass 1 forms Y = dD A in ybuf
-// (n x n per walker of the chunk), R rows of dD at a time in LDS; pass 2
-// forms column j of dA = -A Y and folds it into the six sums of particle j,
-// which the R phases then add up.  O(n^3) per walker: the optimizer's path,
-// not the sweep's.
+// block's own gradient G_det.  One workgroup of four waves per walker of the
+// chunk [w0, w0 + gridDim.x), both products on v_mfma_f64_16x16x4f64 over the
+// block padded to n16 = 16 ceil(n/16): the waves split the n16 x n16 output
+// into quarters of 16 x 16 tiles, and the k dimension runs in panels of 16
+// staged in LDS.  G1: Y = dD A, dD's panel evaluated in place (every exp
+// once), Y written to ybuf (n16^2 per walker of the chunk); G2: dA = -A Y,
+// whose tiles meet the orbitals in the epilogue: each lane evaluates
+// orbital m at particle j for its own elements of dA and adds them into
+// particle j's six sums, which lanes and then the two row quarters add up.
 constexpr int kGaussThreads = 256;
+constexpr int kGaussPanel = 16;
 template <typename T>
-__global__ __launch_bounds__(kGaussThreads) void k_gauss_eloc(const DevModel* __restrict__ Mp, DevState S, int b,
+__global__ __launch_bounds__(kGaussThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_gauss_eloc(const DevModel* __restrict__ Mp, DevState S, int b,
                                                               double alpha, const double* __restrict__ grad,
                                                               double* __restrict__ ybuf, int w0,
                                                               double* __restrict__ out) {
     const DevModel& M = *Mp;
+    constexpr int KP = kGaussPanel;
     __shared__ double px[kMaxOrb], py[kMaxOrb];
-    __shared__ double dd[kGaussThreads];
-    __shared__ double part[6][kGaussThreads];
+    __shared__ double xp[kMaxOrb][KP + 1];      // left operand, columns k0 .. k0 + 15
+    __shared__ double zp[KP][kMaxOrb + 2];      // right operand, rows k0 .. k0 + 15
+    __shared__ double red[2][kMaxOrb][6];
     __shared__ double esum[kGaussThreads / W];
-    const int w = w0 + blockIdx.x, t = threadIdx.x;
+    const int w = w0 + blockIdx.x, t = threadIdx.x, lane = t & (W - 1), wv = t / W;
     const int n = M.blk_n[b], first = M.blk_first[b], nw = M.nw;
-    const int R = kGaussThreads / n, j = t % n, r = t / n;
-    const bool on = r < R;
+    const int n16 = (n + 15) & ~15, tn = n16 / 16, th = (tn + 1) / 2;
+    const int r0 = (wv >> 1) * th, c0 = (wv & 1) * th;   // this wave's first row and column tile
+    const int li = lane & 15, lk = lane >> 4;
     const double* src = S.pos + static_cast<size_t>(w) * M.np * 2;
     for (int c = t; c < n; c += kGaussThreads) {
         px[c] = src[2 * (first + c)];
         py[c] = src[2 * (first + c) + 1];
     }
     const T* A = block_inverse<T, false>(M, S, w, b);
-    double* Y = ybuf + static_cast<size_t>(blockIdx.x) * n * n;
-    __syncthreads();
+    double* Y = ybuf + static_cast<size_t>(blockIdx.x) * n16 * n16;
+    auto a_at = [&](int m, int c) { return m < n && c < n ? static_cast<double>(A[m * n + c]) : 0.0; };
     auto sep = [&](int c, int m, double& d0, double& d1) {
         d0 = px[c] - M.orb_k[b][m][0];
         d1 = py[c] - M.orb_k[b][m][1];
         if (M.cell.periodic) min_image(M.cell, d0, d1);
     };
-    // pass 1: Y(c, j) = sum_m dD(c, m) A(m, j)
-    for (int c0 = 0; c0 < n; c0 += R) {
-        const int c = c0 + r;
-        if (on && c < n) {
-            double d0, d1;
-            sep(c, j, d0, d1);   // thread (j, r) holds dD(c, m = j)
-            const double r2 = d0 * d0 + d1 * d1;
-            dd[r * n + j] = -r2 * exp(-alpha * r2);
-        }
-        __syncthreads();
-        if (on && c < n) {
-            double y = 0.0;
-            for (int m = 0; m < n; ++m) y += dd[r * n + m] * static_cast<double>(A[m * n + j]);
-            Y[c * n + j] = y;
-        }
-        __syncthreads();
-    }
-    // pass 2: column j of dA, folded into particle j's sums
-    double acc[6] = {0, 0, 0, 0, 0, 0};   // G_d, dG_d, dLraw_d
-    if (on)
-        for (int m = r; m < n; m += R) {
-            double da = 0.0;
-            for (int c = 0; c < n; ++c) da -= static_cast<double>(A[m * n + c]) * Y[c * n + j];
-            const double a = static_cast<double>(A[m * n + j]);
-            double d0, d1;
-            sep(j, m, d0, d1);
-            const GaussAlpha o = gauss_alpha(alpha, d0, d1);
+    __syncthreads();
+    dbl4 acc[4][4];
+    // acc = X Z over the k panels, fill(k0) staging X[:, k0:k0+16] in xp and
+    // Z[k0:k0+16, :] in zp
+    auto gemm = [&](auto fill) {
 #pragma unroll
-            for (int d = 0; d < 2; ++d) {
-                acc[d] += a * o.g[d];
-                acc[2 + d] += da * o.g[d] + a * o.ag[d];
-                acc[4 + d] += da * o.l[d] + a * o.al[d];
+        for (int I = 0; I < 4; ++I)
+#pragma unroll
+            for (int J = 0; J < 4; ++J) acc[I][J] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < n16; k0 += KP) {
+            fill(k0);
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < KP / 4; ++ks) {
+                double xa[4], zb[4];
+#pragma unroll
+                for (int I = 0; I < 4; ++I) xa[I] = r0 + I < tn ? xp[(r0 + I) * 16 + li][ks * 4 + lk] : 0.0;
+#pragma unroll
+                for (int J = 0; J < 4; ++J) zb[J] = c0 + J < tn ? zp[ks * 4 + lk][(c0 + J) * 16 + li] : 0.0;
+#pragma unroll
+                for (int I = 0; I < 4; ++I)
+#pragma unroll
+                    for (int J = 0; J < 4; ++J)
+                        if (I < th && J < th) acc[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[I], zb[J], acc[I][J], 0, 0, 0);
+            }
+            __syncthreads();
+        }
+    };
+    // G1: Y = dD A
+    gemm([&](int k0) {
+        for (int e = t; e < n16 * KP; e += kGaussThreads) {
+            const int c = e / KP, m = k0 + e % KP;
+            double v = 0.0;
+            if (c < n && m < n) {
+                double d0, d1;
+                sep(c, m, d0, d1);
+                const double r2 = d0 * d0 + d1 * d1;
+                v = -r2 * exp(-alpha * r2);
+            }
+            xp[c][e % KP] = v;
+        }
+        for (int e = t; e < KP * n16; e += kGaussThreads) zp[e / n16][e % n16] = a_at(k0 + e / n16, e % n16);
+    });
+#pragma unroll
+    for (int I = 0; I < 4; ++I)
+#pragma unroll
+        for (int J = 0; J < 4; ++J)
+            if (I < th && J < th && r0 + I < tn && c0 + J < tn)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    Y[((r0 + I) * 16 + 4 * q + lk) * n16 + (c0 + J) * 16 + li] = acc[I][J][q];
+    __syncthreads();   // Y is read by the other waves
+    // G2: dA = -(A Y)
+    gemm([&](int k0) {
+        for (int e = t; e < n16 * KP; e += kGaussThreads) xp[e / KP][e % KP] = a_at(e / KP, k0 + e % KP);
+        for (int e = t; e < KP * n16; e += kGaussThreads) zp[e / n16][e % n16] = Y[(k0 + e / n16) * n16 + e % n16];
+    });
+    double s6[4][6];
+#pragma unroll
+    for (int J = 0; J < 4; ++J)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) s6[J][q] = 0.0;
+#pragma unroll
+    for (int J = 0; J < 4; ++J) {
+        const int j = (c0 + J) * 16 + li;
+        if (J >= th || c0 + J >= tn || j >= n) continue;
+#pragma unroll
+        for (int I = 0; I < 4; ++I) {
+            if (I >= th || r0 + I >= tn) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = (r0 + I) * 16 + 4 * q + lk;
+                if (m >= n) continue;
+                const double da = -acc[I][J][q], a = static_cast<double>(A[m * n + j]);
+                double d0, d1;
+                sep(j, m, d0, d1);
+                const GaussAlpha o = gauss_alpha(alpha, d0, d1);
+#pragma unroll
+                for (int d = 0; d < 2; ++d) {
+                    s6[J][d] += a * o.g[d];
+                    s6[J][2 + d] += da * o.g[d] + a * o.ag[d];
+                    s6[J][4 + d] += da * o.l[d] + a * o.al[d];
+                }
             }
         }
+    }
+    // the four lanes of one column (li, li + 16, li + 32, li + 48), then the
+    // two row quarters through LDS
 #pragma unroll
-    for (int q = 0; q < 6; ++q) part[q][t] = acc[q];
+    for (int J = 0; J < 4; ++J)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            double v = s6[J][q];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            s6[J][q] = v;
+        }
+    if (lk == 0)
+#pragma unroll
+        for (int J = 0; J < 4; ++J)
+            if (J < th && c0 + J < tn)
+#pragma unroll
+                for (int q = 0; q < 6; ++q) red[wv >> 1][(c0 + J) * 16 + li][q] = s6[J][q];
     __syncthreads();
     double e = 0.0;
     if (t < n) {
-        double s6[6] = {0, 0, 0, 0, 0, 0};
-        for (int rr = 0; rr < R; ++rr)
-#pragma unroll
-            for (int q = 0; q < 6; ++q) s6[q] += part[q][rr * n + t];
         const int p = first + t;
         const double im = M.inv_mass[p];
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
-            const double G = s6[d], dG = s6[2 + d], dL = s6[4 + d] - 2.0 * G * dG;
-            e += -0.5 * im * (dL + 2.0 * grad[static_cast<size_t>(p * 2 + d) * nw + w] * dG);
+            const double G = red[0][t][d] + (th < tn ? red[1][t][d] : 0.0);
+            const double dG = red[0][t][2 + d] + (th < tn ? red[1][t][2 + d] : 0.0);
+            const double dLr = red[0][t][4 + d] + (th < tn ? red[1][t][4 + d] : 0.0);
+            e += -0.5 * im * (dLr - 2.0 * G * dG + 2.0 * grad[static_cast<size_t>(p * 2 + d) * nw + w] * dG);
         }
     }
     e = wsum(e);
-    if ((t & (W - 1)) == 0) esum[t / W] = e;
+    if (lane == 0) esum[wv] = e;
     __syncthreads();
     if (t == 0) {
         double tot = 0.0;

@@ -462,6 +462,44 @@ def derivatives_leg(nw=65536, crystal_nw=8192, reps=3):
                                  gauss_width_flops_per_walker=4 * n ** 3)
     finally:
         w.close()
+    out["cpu_baseline"] = cpu_baseline_derivatives()
+    return out
+
+
+def cpu_baseline_derivatives(heg_walkers=64, crystal_walkers=16):
+    """The oracle's d log|psi|/dp and dE_L/dp on one host core, on a sample
+    of each system's walkers (the same parameters as derivatives_leg); cost
+    per walker does not depend on the walker count, so the figures are ms
+    per 1,000 walkers."""
+    from alea_amd.system import free_parameters, site_positions, uniform_positions, wigner_crystal_spec
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+    from backends import oracle_walkers
+    out = dict(kind="port", cores=1)
+    spec, _ = workload_spec()
+    cs = wigner_crystal_spec()
+    a = math.sqrt(cs.lattice[0] * cs.lattice[3] / cs.n_particles)
+    nd = len(cs.drummond[0]["coeffs"])
+    for name, sp, free, nw, pos in (
+            ("heg122", spec, [(f, i) for f, j in enumerate(spec.pade) for i in range(j["num"] + j["den"])],
+             heg_walkers, lambda n: uniform_positions(spec, n, np.random.default_rng(POS_SEED))),
+            ("crystal128", cs, [(1, nd), (2, 0), (3, 0)], crystal_walkers,
+             lambda n: site_positions(cs, n, np.random.default_rng(POS_SEED), 0.15 * a))):
+        params = free_parameters(sp, free)
+        o = oracle_walkers(sp, nw)
+        try:
+            o.set_positions(pos(nw))
+            o.evaluate()
+            t0 = time.perf_counter()
+            o.parameter_derivatives(params)
+            t1 = time.perf_counter()
+            o.local_energy_parameter_derivatives(params)
+            t2 = time.perf_counter()
+        finally:
+            o.close()
+        out[name] = dict(walkers=nw, parameters=len(params), dlogpsi_ms_per_1k_walkers=(t1 - t0) * 1e6 / nw,
+                         de_ms_per_1k_walkers=(t2 - t1) * 1e6 / nw)
+    out["sample"] = (f"{heg_walkers} N=122 walkers (6 Padé parameters) and {crystal_walkers} Wigner-crystal "
+                     f"walkers (3 parameters) on the oracle, one thread")
     return out
 
 

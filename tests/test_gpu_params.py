@@ -49,6 +49,9 @@ def _cases():
     w = wigner_crystal_spec(rs=20.0, rows=2, cols=4, nosanow_alpha=0.05, drummond_sigma=0.1)
     w.name = "wigner crystal N=8"
     out.append((w, [(0, 1), (1, 0), (1, 8), (2, 0), (3, 0)], 2.0))
+    w = wigner_crystal_spec(rs=20.0, rows=4, cols=10, nosanow_alpha=0.05, drummond_sigma=0.1)
+    w.name = "wigner crystal N=40"   # 48 padded rows: three tiles, the matrix-core quarters partly empty
+    out.append((w, [(3, 0), (1, 8)], 2.0))
     w = wigner_crystal_spec()
     w.name = "wigner crystal N=128"
     out.append((w, [(1, 8), (2, 0), (3, 0)], 3.0))
