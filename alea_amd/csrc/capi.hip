@@ -200,7 +200,7 @@ struct alea_system {
     bool cplx = false;
     hipStream_t stream = nullptr;
     DevState st;
-    DevBuf<double> pos, rho, logs, row_abs, cond, work, e_loc, e_loc_im, rr;
+    DevBuf<double> pos, rho, logs, cond, work, e_loc, e_loc_im, rr;
     DevBuf<double> weights, e_prev, xi;   // DMC weights, E_L before the sweep, branch uniforms
     DevBuf<char> inv_bytes, inv_spare;    // inverse records; the gather target of a permutation
     DevBuf<double> perm_tmp;              // gather target of the other per-walker arrays
@@ -360,7 +360,6 @@ struct alea_system {
         st.inv_stride = inv_elems;
         st.nb_max = std::max(nb_max, 1);
         inv_bytes.alloc(std::max<size_t>(1, nw_ * inv_elems * tsize()), stream);
-        row_abs.alloc(nw_ * kMaxBlocks * st.nb_max, stream);
         cond.alloc(nw_ * kMaxBlocks, stream);
         updates.alloc(nw_ * kMaxBlocks, stream);
         st.work_stride = 2 * (cplx ? 2 : 1) * nb_max * nb_max;
@@ -391,7 +390,6 @@ struct alea_system {
         st.rho = rho.p;
         st.logs = logs.p;
         st.inv = inv_bytes.p;
-        st.row_abs = row_abs.p;
         st.cond = cond.p;
         st.updates = updates.p;
         st.work = work.p;
@@ -405,12 +403,12 @@ struct alea_system {
         st.ew_rho = ew_rho.p;
         st.ew_snap = ew_snap.p;
         st.ew_inc = ew_inc.p;
-        // LDS of one walker: positions, k-space densities, v', column j, the
-        // Sherman-Morrison operands and the resident block's row sums (these
-        // four over the padded rows), then copies of the pair splines and the
-        // k-space Jastrow vectors and weights (carve() in kernels.hip)
+        // LDS of one walker: positions, k-space densities, v', column j and
+        // the Sherman-Morrison operands (these three over the padded rows),
+        // then copies of the pair splines and the k-space Jastrow vectors and
+        // weights (carve() in kernels.hip)
         walker_lds = sizeof(double) * (2 * static_cast<size_t>(np) + 2 * m.ng * m.nkj +
-                                       9 * static_cast<size_t>(std::max(rows_max, kRowPad)) +
+                                       8 * static_cast<size_t>(std::max(rows_max, kRowPad)) +
                                        static_cast<size_t>(m.n_pair) * m.spl_stride +
                                        (2 + static_cast<size_t>(m.ng) * m.ng) * m.nkj);
         // the batched refresh holds a block in registers (k_refresh_reg)
@@ -438,7 +436,6 @@ struct alea_system {
         v.rho += w * m.ng * m.nkj * 2;
         v.logs += w * 4;
         v.inv = static_cast<char*>(v.inv) + w * v.inv_stride * tsize();
-        v.row_abs += w * kMaxBlocks * v.nb_max;
         v.cond += w * kMaxBlocks;
         v.updates += w * kMaxBlocks;
         v.work += w * v.work_stride;
@@ -1908,7 +1905,6 @@ static void permute_device(alea_system* s, int new_nw, const int* src_d) {
         {s->pos.p, sizeof(double) * m.np * 2},
         {s->rho.p, sizeof(double) * m.ng * m.nkj * 2},
         {s->logs.p, sizeof(double) * 4},
-        {s->row_abs.p, sizeof(double) * kMaxBlocks * s->st.nb_max},
         {s->cond.p, sizeof(double) * kMaxBlocks},
         {s->updates.p, sizeof(int) * kMaxBlocks},
         {s->e_loc.p, sizeof(double)},

@@ -548,7 +548,6 @@ struct Shared {
     double* v;     // [2*rows] proposed orbital values (stored in the determinant precision T)
     double* u;     // [2*rows] column j of A (old)
     double* f;     // [4*rows]: u / R, then v again, both as C2<T> (determinant precision)
-    double* rabs;  // [rows] row sums |D| of the block held in registers (register path)
     double* spl;   // [n_pair][spl_stride] pair-factor splines (copies of M.pair[f].rad.mono)
     double* kj;    // [nkj][2] k-space Jastrow vectors, then kjw [ng][ng][nkj]
 };
@@ -562,8 +561,7 @@ __device__ inline Shared carve(const DevModel& M, double* base) {
     sh.v = sh.rho + 2 * M.ng * M.nkj;
     sh.u = sh.v + 2 * nr;
     sh.f = sh.u + 2 * nr;
-    sh.rabs = sh.f + 4 * nr;
-    sh.spl = sh.rabs + nr;
+    sh.spl = sh.f + 4 * nr;
     sh.kj = sh.spl + M.n_pair * M.spl_stride;
     return sh;
 }
@@ -575,6 +573,12 @@ __device__ inline Shared carve(const DevModel& M, double* base) {
 // (slater_determinant.cpp:789, 825-826) stays quiet until the next batched
 // refresh instead of rebuilding on every accepted move.
 __device__ inline double fresh_cond(double kappa) { return kappa > 1e8 ? -kappa : kappa; }
+// The same for walker w's block, counting a latched fresh inverse among the
+// walker's rebuild triggers (kLatchedFresh).
+__device__ inline void store_fresh_cond(const DevState& S, int w, int b, double kappa) {
+    S.cond[w * kMaxBlocks + b] = fresh_cond(kappa);
+    if (kappa > 1e8) atomicAdd(&S.rebuilds[w * kRebuildCauses + kLatchedFresh], 1);
+}
 
 // ── the per-walker engine ──────────────────────────────────────────────────
 // NR = 0: the determinant inverses stay in HBM and every access goes there.
@@ -596,9 +600,7 @@ struct Walker {
     double kdr = 0.0, kdi = 0.0;  // this lane's k-space phase change of the last move
     C2<T> col[NR > 0 ? NR : 1];   // column `lane` of the resident block (NR > 0)
     int res_b = -1;               // resident block, -1 for none
-    // the resident block's condition estimate and update count (its row
-    // sums are in sh.rabs); HBM holds them again after spill()
-    double res_cond = 0.0;
+    // the resident block's update count; HBM holds it again after spill()
     int res_upd = 0;
 
     // Load column `lane` of block b into registers (rows past the block's
@@ -615,9 +617,6 @@ struct Walker {
                 for (int i = 0; i < kRowPad; ++i) col[ch * kRowPad + i] = A.get(ch * kRowPad + i, lane);
             }
         }
-        const double* ra = S.row_abs + (static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max;
-        for (int kr = lane; kr < n; kr += W) sh.rabs[kr] = ra[kr];
-        res_cond = S.cond[w * kMaxBlocks + b];
         res_upd = S.updates[w * kMaxBlocks + b];
         res_b = b;
         __syncthreads();
@@ -635,12 +634,7 @@ struct Walker {
                     for (int i = 0; i < kRowPad; ++i) A.set(ch * kRowPad + i, lane, col[ch * kRowPad + i]);
                 }
         }
-        double* ra = S.row_abs + (static_cast<size_t>(w) * kMaxBlocks + res_b) * S.nb_max;
-        for (int kr = lane; kr < n; kr += W) ra[kr] = sh.rabs[kr];
-        if (lane == 0) {
-            S.cond[w * kMaxBlocks + res_b] = res_cond;
-            S.updates[w * kMaxBlocks + res_b] = res_upd;
-        }
+        if (lane == 0) S.updates[w * kMaxBlocks + res_b] = res_upd;
         res_b = -1;
     }
     __device__ void make_resident(int b) {
@@ -1092,12 +1086,11 @@ struct Walker {
                 orbital(M, b, m, sh.pos[2 * (first + kr)], sh.pos[2 * (first + kr) + 1], o);
                 rs += CPLX ? sqrt(o.vr * o.vr + o.vi * o.vi) : fabs(o.vr);
             }
-            S.row_abs[(static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max + kr] = rs;
             rowmax = fmax(rowmax, rs);
         }
         rowmax = wmax(rowmax);
         if (lane == 0) {
-            S.cond[w * kMaxBlocks + b] = fresh_cond(rowmax * colmax);
+            store_fresh_cond(S, w, b, rowmax * colmax);
             S.updates[w * kMaxBlocks + b] = 0;
         }
         __syncthreads();
@@ -1109,60 +1102,54 @@ struct Walker {
     // triggers.  sh.v holds v' (orbitals at r'), sh.u column j of A, and
     // sh.pos already holds the committed position of the moved particle.
     __device__ bool det_commit(int b, int j, const DOut& d) {
-        const InvRef<T, CPLX> A = inv(b);
         const int n = M.blk_n[b];
         const double Rr = d.R[0], Ri = d.R[1];
-        // The two magnitude sums behind the rebuild thresholds, from column j
-        // (sh.u) and v' (sh.v) as the ratio used them: the cancellation scale
-        // Sum_m |A_mj v'_m| and the row sum Sum_m |v'_m| of D.  They feed
-        // thresholds only, so a single-precision square root is enough.
-        double mags[2] = {0.0, 0.0};
+        // The cancellation scale Sum_m |A_mj v'_m|, from column j (sh.u) and
+        // v' (sh.v) as the ratio used them.  It feeds a threshold only, so a
+        // single-precision square root is enough.
+        double scale = 0.0;
         for (int m = lane; m < n; m += W) {
             const C2<T> a = reinterpret_cast<const C2<T>*>(sh.u)[m];
             const double ar = static_cast<double>(a.re), ai = static_cast<double>(a.im);
             const double vr = sh.v[2 * m], vi = sh.v[2 * m + 1];
             const double tr = ar * vr - ai * vi, ti = ar * vi + ai * vr;
-            mags[0] += CPLX ? static_cast<double>(__builtin_sqrtf(static_cast<float>(tr * tr + ti * ti))) : fabs(tr);
-            mags[1] += CPLX ? static_cast<double>(__builtin_sqrtf(static_cast<float>(vr * vr + vi * vi))) : fabs(vr);
+            scale += CPLX ? static_cast<double>(__builtin_sqrtf(static_cast<float>(tr * tr + ti * ti))) : fabs(tr);
         }
-        wsum_n(mags);
-        const double scale = mags[0], rowabs = mags[1];
+        scale = wsum(scale);
         const double absR = CPLX ? sqrt(Rr * Rr + Ri * Ri) : fabs(Rr);
         const double eps = sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
         int* upd = S.updates + w * kMaxBlocks + b;
         const int updates = (NR > 0 ? res_upd : *upd) + 1;
-        double* rabs = NR > 0 ? sh.rabs : S.row_abs + (static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max;
-        const double cond_stored = NR > 0 ? res_cond : S.cond[w * kMaxBlocks + b];
-        const bool latched = cond_stored < 0.0;    // see fresh_cond
-        const double cond_old = fabs(cond_stored);
         __syncthreads();
         // The reference also rebuilds after 64 updates (slater_determinant.cpp:
-        // 788-790); here that periodic rebuild is the batched refresh kernel
-        // (k_refresh_reg), which the host runs every few sweeps.  A tiny or
-        // non-finite ratio, and the cancellation guard (the rounding bound
-        // eps * sum|A_mj v_m| on R above |R| / cancel_factor), rebuild here.
-        // A condition estimate above 1e8 latches the block instead (see
-        // fresh_cond) and leaves its rebuild to the next batched refresh: the
-        // estimate measures the configuration's own conditioning, which a
-        // rebuild does not lower.
+        // 788-790); here that periodic rebuild is the batched refresh kernel,
+        // which the host runs after every sweep.  A tiny or non-finite ratio,
+        // and the cancellation guard (the rounding bound eps * sum|A_mj v_m|
+        // on R above |R| / cancel_factor), rebuild here.  The condition
+        // estimate is taken when an inverse is rebuilt (see fresh_cond); an
+        // update rebuilds instead when its size bound max|r| max|f| is not
+        // finite or could carry A past the precision's range, which is where
+        // the reference's estimate turns non-finite (slater_determinant.cpp:
+        // 825-826).
         int cause = -1;   // RebuildCause, uniform across the wave
-        bool latch_now = false;
         if (!isfinite(absR) || absR < 1e-5) cause = kRebuildTinyRatio;
         else if (absR <= M.cancel_factor * eps * scale) cause = kRebuildCancellation;
-        else if (!latched && cond_old > 1e8) latch_now = true;
         bool rebuild_now = cause >= 0;
-        if (latch_now && lane == 0) S.rebuilds[w * kRebuildCauses + kRebuildCondBefore] += 1;
         if (!rebuild_now) {
-            // pass 1: row product r_c = sum_m v'_m A[m][c] - delta_cj
+            // r_c = sum_m v'_m A[m][c] - delta_cj, then A[m][c] -= f_m r_c
+            // with f = u / R
             const T invRr = static_cast<T>(CPLX ? Rr / (absR * absR) : 1.0 / Rr);
             const T invRi = static_cast<T>(CPLX ? -Ri / (absR * absR) : 0.0);
             const int rows = M.blk_rows[b];
             C2<T>* fT = reinterpret_cast<C2<T>*>(sh.f);
             const C2<T>* vT = reinterpret_cast<const C2<T>*>(sh.f) + rows;   // packed after f
+            T fmag = T(0);
             for (int m = lane; m < rows; m += W) {
                 if (m < n) {
                     const C2<T> um = reinterpret_cast<const C2<T>*>(sh.u)[m];
-                    fT[m] = cmul(um, C2<T>{invRr, invRi});
+                    const C2<T> fm = cmul(um, C2<T>{invRr, invRi});
+                    fT[m] = fm;
+                    fmag = fmax(fmag, fabs(fm.re) + fabs(fm.im));
                     reinterpret_cast<C2<T>*>(sh.f)[rows + m] =
                         C2<T>{static_cast<T>(sh.v[2 * m]), static_cast<T>(sh.v[2 * m + 1])};
                 } else {
@@ -1170,6 +1157,12 @@ struct Walker {
                     reinterpret_cast<C2<T>*>(sh.f)[rows + m] = C2<T>{T(0), T(0)};
                 }
             }
+            // |f| and |r| bound |f_m r_c| by max|f| max|r|: within this range
+            // neither 64 updates nor a non-finite product can leave A
+            // non-finite
+            const double f_max = wmax(static_cast<double>(fmag));
+            const double safe = sizeof(T) == 8 ? 1e300 : 1e30;
+            bool bad = false;
             __syncthreads();
             // Lane c owns column c and holds it in registers, CH rows per
             // chunk: pass 1 issues a whole chunk of loads at once, and pass 2
@@ -1177,7 +1170,6 @@ struct Walker {
             // <= CH rows is read once and written once per accepted move).
             constexpr int CH = kRowPad;   // rows per chunk; rows padded with zeros to a multiple
             const int nch = M.blk_rows[b] / CH;
-            double colmax = 0.0;
             if constexpr (NR > 0) {
                 // the same two passes on the column held in registers; lanes
                 // past n hold a zero column and stay zero.  v' and u/R are
@@ -1204,7 +1196,7 @@ struct Walker {
                         }
                     }
                 if (lane == j) r.re -= T(1);
-                T cs = T(0);
+                bad = !(static_cast<double>(fabs(r.re) + fabs(r.im)) * f_max <= safe);
 #pragma unroll
                 for (int ch = NR / CH - 1; ch >= 0; --ch)
                     if (ch < nch) {
@@ -1221,12 +1213,11 @@ struct Walker {
                                     a.im = fmadd(-f[q].re, r.im, a.im);
                                     a.im = fmadd(-f[q].im, r.re, a.im);
                                 }
-                                cs += fabs(a.re) + (CPLX ? fabs(a.im) : T(0));
                             }
                         }
                     }
-                colmax = static_cast<double>(cs);
             }
+            const InvRef<T, CPLX> A = inv(b);
             for (int c = lane; NR == 0 && c < n; c += W) {
                 C2<T> a[CH];
                 C2<T> r{T(0), T(0)};
@@ -1242,8 +1233,8 @@ struct Walker {
                     }
                 }
                 if (c == j) r.re -= T(1);
+                bad = bad || !(static_cast<double>(fabs(r.re) + fabs(r.im)) * f_max <= safe);
                 // pass 2: A[m][c] -= f_m r_c, last chunk first (in registers)
-                T cs = T(0);
                 for (int ch = nch - 1; ch >= 0; --ch) {
                     const int m0 = ch * CH;
                     if (ch != nch - 1) {
@@ -1256,37 +1247,17 @@ struct Walker {
                         a[i].re -= pr.re;
                         a[i].im -= pr.im;
                         store_elem(A.row(m0 + i), c, a[i]);
-                        // |Re| + |Im| bounds the modulus within sqrt(2); it
-                        // only feeds the condition estimate
-                        cs += fabs(a[i].re) + (CPLX ? fabs(a[i].im) : T(0));
                     }
                 }
-                colmax = fmax(colmax, static_cast<double>(cs));
             }
-            colmax = wmax(colmax);
-            if (lane == 0) rabs[j] = rowabs;
-            __syncthreads();
-            double rowmax = 0.0;
-            for (int kr = lane; kr < n; kr += W) rowmax = fmax(rowmax, rabs[kr]);
-            rowmax = wmax(rowmax);
-            const double cond = rowmax * colmax;
-            if (!isfinite(cond)) {
+            if (__builtin_amdgcn_ballot_w64(bad) != 0) {
+                // the rebuild below replaces the updated inverse
                 rebuild_now = true;
-                cause = kRebuildCondAfter;
+                cause = kRebuildUpdateBound;
             } else {
-                if (!latched && !latch_now && cond > 1e8) {
-                    latch_now = true;
-                    if (lane == 0) S.rebuilds[w * kRebuildCauses + kRebuildCondAfter] += 1;
-                }
-                const bool lt = latched || latch_now;
                 log_b(b) += log(absR);
-                if constexpr (NR > 0) {
-                    res_upd = updates;
-                    res_cond = lt ? -cond : cond;
-                } else if (lane == 0) {
-                    *upd = updates;
-                    S.cond[w * kMaxBlocks + b] = lt ? -cond : cond;
-                }
+                if constexpr (NR > 0) res_upd = updates;
+                else if (lane == 0) *upd = updates;
             }
         }
         if (rebuild_now) {
@@ -1608,7 +1579,6 @@ __device__ __attribute__((always_inline)) inline void refresh_store(const DevMod
     if (tid < n) {
         double rs = 0.0;
         for (int q = 0; q < GX; ++q) rs += part[q][tid];
-        S.row_abs[(static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max + tid] = rs;
         rowmax = rs;
     }
     colmax = wmax(colmax);
@@ -1623,7 +1593,7 @@ __device__ __attribute__((always_inline)) inline void refresh_store(const DevMod
             colmax = fmax(colmax, red[0][q]);
             rowmax = fmax(rowmax, red[1][q]);
         }
-        S.cond[w * kMaxBlocks + b] = fresh_cond(rowmax * colmax);
+        store_fresh_cond(S, w, b, rowmax * colmax);
         S.updates[w * kMaxBlocks + b] = 0;
         S.logs[w * 4 + 1 + b] = fmax(log_det, log(1e-300));
     }
@@ -2136,7 +2106,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     double rowmax = 0.0;
     if (tid < n) {
         const double rs = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
-        S.row_abs[(static_cast<size_t>(w) * kMaxBlocks + b) * S.nb_max + tid] = rs;
         rowmax = rs;
     }
     colmax = wmax(colmax);
@@ -2151,7 +2120,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             colmax = fmax(colmax, red[0][q]);
             rowmax = fmax(rowmax, red[1][q]);
         }
-        S.cond[w * kMaxBlocks + b] = fresh_cond(rowmax * colmax);
+        store_fresh_cond(S, w, b, rowmax * colmax);
         S.updates[w * kMaxBlocks + b] = 0;
         S.logs[w * 4 + 1 + b] = fmax(log_det, log(1e-300));
     }

@@ -130,7 +130,6 @@ struct DevState {
     double* rho = nullptr;        // [w][ng][nkj][2]  k-space Jastrow densities
     double* logs = nullptr;       // [w][4]: 0 Jastrow+one-body, 1.. determinant blocks
     void* inv = nullptr;          // [w][blocks][Nb*Nb] (T, complex: 2 T per entry)
-    double* row_abs = nullptr;    // [w][blocks][Nb]  |D| row sums
     double* cond = nullptr;       // [w][blocks]
     int* updates = nullptr;       // [w][blocks]
     double* work = nullptr;       // [w][2*Nb*Nb*cplx] Gauss-Jordan workspace (FP64)
@@ -169,13 +168,13 @@ struct DevParam {
 };
 
 // Why an accepted move rebuilt a determinant inside the sweep
-// (slater_determinant.cpp:615-617,788-790,825-826): the index into
-// DevState::rebuilds.
+// (slater_determinant.cpp:615-617,788-790,825-826), and the latched fresh
+// inverses: the index into DevState::rebuilds.
 enum RebuildCause {
     kRebuildTinyRatio = 0,     // |R| < 1e-5 or R not finite
-    kRebuildCondBefore = 1,    // condition estimate above 1e8 before the update
+    kLatchedFresh = 1,         // a rebuilt inverse whose condition estimate is above 1e8 (no rebuild)
     kRebuildCancellation = 2,  // |R| within the rounding bound of its terms
-    kRebuildCondAfter = 3,     // condition estimate above 1e8 (or not finite) after the update
+    kRebuildUpdateBound = 3,   // the update's size bound max|r| max|f| not finite or out of range
     kRebuildCauses = 4
 };
 

@@ -308,11 +308,11 @@ int alea_sweep_kernel_ms(alea_system* s, double* ms, int capacity, int* n);
 
 /* Rebuild triggers of accepted moves since the last reset, by kind
  * (slater_determinant.cpp:615-617,788-790,825-826): counts[0] ratio below
- * 1e-5 or not finite, [1] condition estimate above 1e8 before the update,
- * [2] the cancellation guard, [3] condition estimate above 1e8 (or not
- * finite) after the update.  Kinds 0 and 2, and a non-finite estimate,
- * rebuild the inverse inside the sweep; an estimate above 1e8 latches the
- * block and leaves its rebuild to the next batched refresh.  per_walker [nw]
+ * 1e-5 or not finite, [1] rebuilt inverses whose condition estimate is above
+ * 1e8 (latched: no further rebuild until the next periodic one), [2] the
+ * cancellation guard, [3] an update whose size bound max|r| max|f| is not
+ * finite or could carry the inverse out of range.  Kinds 0, 2 and 3 rebuild
+ * the inverse inside the sweep.  per_walker [nw]
  * (may be NULL) receives each walker slot's total (a permutation does not
  * move the counts).  reset != 0 clears them.  Synchronous.  No counterpart in
  * the reference, which does not count its rebuilds: the measurement hook for
