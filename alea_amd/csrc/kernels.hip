@@ -1403,7 +1403,9 @@ struct Walker {
             acc = la >= 0.0 || uni < exp(la);
         } else {
             const double tm = tau * M.inv_mass[p];
-            if (!CPLX && M.n_blocks > 0 && cos(phase) < 2.220446049250313e-16) {
+            // a real trial's node crossing: the phase change is 0 or pi, so
+            // cos(phase) < eps is the determinant ratio's sign
+            if (!CPLX && b >= 0 && D.R[0] < 0.0) {
                 acc = false;
             } else {
                 const double vn = ng[0] * ng[0] + ng[1] * ng[1];
