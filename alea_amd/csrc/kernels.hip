@@ -862,8 +862,8 @@ struct Walker {
         const InvRef<T, CPLX> A = inv(b);
         const int n = M.blk_n[b];
         if constexpr (NR > 0) {
-            // column j lives in lane j: stage it through LDS for the lanes over m
-            make_resident(b);
+            // column j lives in lane j (the sweep loop made block b resident
+            // before the move): stage it through LDS for the lanes over m
             const int nch = M.blk_rows[b] / kRowPad;
             // (in the determinant precision, 16 bytes per store)
             if (lane == j) {
@@ -2164,6 +2164,14 @@ __device__ __attribute__((always_inline)) inline void sweep_walker(const DevMode
             const double sigma = sqrt(tm);
             const double x0 = wk.sh.pos[2 * p], x1 = wk.sh.pos[2 * p + 1];
             double gold[2] = {0.0, 0.0}, scale = 0.0;
+            // the moved particle's block into registers once per move, before
+            // the old gradient and the ratio read it (one copy of the
+            // spill-and-fill code, which the two reads would otherwise each
+            // inline)
+            if constexpr (NR > 0) {
+                const int bp = M.block_of[p];
+                if (bp >= 0) wk.make_resident(bp);
+            }
             if constexpr (DMC) {
                 wk.grad_of(p, gold);
                 // DMC: the Umrigar-scaled drift; Langevin VMC: the bare tau/m
@@ -2227,6 +2235,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             const double sigma = sqrt(tm);
             const double x0 = wk.sh.pos[2 * p], x1 = wk.sh.pos[2 * p + 1];
             double gold[2] = {0.0, 0.0}, scale = 0.0;
+            // the moved particle's block into registers once per move, before
+            // the old gradient and the ratio read it (one copy of the
+            // spill-and-fill code, which the two reads would otherwise each
+            // inline)
+            if constexpr (NR > 0) {
+                const int bp = M.block_of[p];
+                if (bp >= 0) wk.make_resident(bp);
+            }
             if constexpr (DMC) {
                 wk.grad_of(p, gold);
                 // DMC: the Umrigar-scaled drift; Langevin VMC: the bare tau/m
@@ -3263,16 +3279,35 @@ int sweep_tile_rows(bool mixed, bool cplx, int rows_max) {
     return 0;
 }
 
+// Register tile rows of a drift (DMC or Langevin) sweep, 0 for the HBM path.
+// The drift move keeps the old and new gradients and Laplacians live across
+// the move besides the tile, so it takes tiles only where they stay within
+// the register file (spills of at most 8 registers): mixed blocks up to 48
+// rows, FP64 real blocks up to 32, FP64 complex blocks up to 16.
+int drift_tile_rows(bool mixed, bool cplx, int rows_max) {
+    if (rows_max <= 0) return 0;
+    const int cap = mixed ? 48 : (cplx ? 16 : 32);
+    for (int nr = 16; nr <= cap; nr += 16)
+        if (rows_max <= nr) return nr;
+    return 0;
+}
+
 template <typename T, bool CPLX, bool DMC>
 static void launch_sweep_t(const DevModel* m, const DevState& s, const SweepParams& p, int nw, int nr,
                            size_t lds, hipStream_t st) {
     const dim3 g(nw), t(64);
     if constexpr (DMC) {
-        // The drift-diffusion move keeps gradients, Laplacians and the drift
-        // live across the whole move: with a register tile beside them the
-        // kernel spills (73 registers at 32 real rows, 245 at 64), and the
-        // HBM path measured faster, so DMC sweeps stream the inverse.
-        hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 0>), g, t, lds, st, m, s, p);
+        switch (nr) {
+            case 16: hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 16>), g, t, lds, st, m, s, p); break;
+            case 32: hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 32>), g, t, lds, st, m, s, p); break;
+            case 48:
+                if constexpr (sizeof(T) == 4) {
+                    hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 48>), g, t, lds, st, m, s, p);
+                    break;
+                }
+                [[fallthrough]];
+            default: hipLaunchKernelGGL((k_sweep<T, CPLX, DMC, 0>), g, t, lds, st, m, s, p); break;
+        }
         return;
     } else {
         switch (nr) {
@@ -3291,8 +3326,9 @@ static void launch_sweep_t(const DevModel* m, const DevState& s, const SweepPara
 
 void launch_sweep(const DevModel* m, const DevState& s, const SweepParams& p, int nw, bool mixed,
                   bool cplx, int rows_max, size_t lds, hipStream_t st) {
-    const int nr = sweep_tile_rows(mixed, cplx, rows_max);
+    int nr = sweep_tile_rows(mixed, cplx, rows_max);
     if (p.dmc) {
+        nr = drift_tile_rows(mixed, cplx, rows_max);
         if (mixed) {
             if (cplx) launch_sweep_t<float, true, true>(m, s, p, nw, nr, lds, st);
             else launch_sweep_t<float, false, true>(m, s, p, nw, nr, lds, st);
