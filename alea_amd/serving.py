@@ -69,15 +69,15 @@ def footprint_bytes(n_particles: int, n_walkers: int, precision: int, n_kj: int 
     """Device bytes of one job's walker batch, as the C-ABI allocates them
     (alea_system::build): positions, k-space densities, log parts, the two
     determinant inverses (rows padded to 16) in the determinant precision,
-    row sums, condition estimates, rebuild workspace (FP64), RNG states and
-    the per-walker scalars."""
+    condition estimates, rebuild workspace (FP64), RNG states and the
+    per-walker scalars."""
     nb = n_particles // 2 if slater else 0
     rows = (nb + 15) // 16 * 16
     e = 2 if complex_orbitals else 1
     t = 4 if precision == MIXED else 8
     inv = 2 * e * rows * nb
     inv += inv & 1
-    per_walker = (16 * n_particles + 16 * 2 * n_kj + 32 + inv * t + 16 * max(nb, 1) + 16 + 8
+    per_walker = (16 * n_particles + 16 * 2 * n_kj + 32 + inv * t + 16 + 8
                   + 8 * max(1, 2 * e * nb * nb) + 32 + 16 + 16 + 24 + 8 + 4 + 8 + 4)
     return per_walker * n_walkers + (1 << 20)
 
