@@ -114,6 +114,52 @@ def test_c1_dmc_identical_populations():
     assert pos_err(g.positions(), o.positions()) < 1e-10
 
 
+def test_c1_mixed_drift_sweeps_track_fp64():
+    """Config 1's system in the mixed form, whose 37-orbital real blocks take
+    the 48-row register tile in drift sweeps (the FP64 form streams them from
+    HBM): from the same walkers and tape, three drift-diffusion sweeps leave at
+    least 95% of the walkers on the FP64 trajectory (positions within 1e-6 of
+    the cell scale: the drift reads the single-precision inverse), with the
+    accept counts within 2% and the local energy per particle on those walkers
+    within 1e-5 Ha; and the mixed walkers' inverse, read fresh after
+    refresh_inverse, gives the FP64 energies of the same positions to 1e-7 Ha
+    per particle."""
+    from alea_amd.walkers import FP64, MIXED
+    spec = heg_drummond_spec(37, 37, 40.0)
+    nw, n_sw = 128, 3
+    pos = _pos(spec, nw, 29)
+    tape = oracle_tape(37, spec.n_particles, nw, n_sw)
+    out = []
+    for prec in (FP64, MIXED):
+        x = gpu_walkers(spec, nw, prec)
+        try:
+            x.set_positions(pos)
+            x.evaluate()
+            r = [np.zeros(nw), np.zeros(nw)]
+            acc = x.dmc_sweeps(n_sw, 2.0, tape, nw, *r)
+            swept = x.local_energy()[0].copy()
+            p_ = x.positions().copy()
+            x.refresh_inverse()
+            out.append((acc, p_, swept, x.local_energy()[0].copy()))
+        finally:
+            x.close()
+    (a64, p64, e64, f64_), (a32, p32, e32, f32_) = out
+    assert abs(a32 - a64) <= 0.02 * a64 and a64 > 0
+    scale = np.abs(p64).max()
+    same = np.all(np.abs(p32 - p64).reshape(-1, nw) < 1e-6 * scale, axis=0)
+    assert same.mean() >= 0.95
+    n = spec.n_particles
+    assert np.abs((e32 - e64)[same]).max() / n < 1e-5
+    fresh = heg_drummond_spec(37, 37, 40.0)
+    g = gpu_walkers(fresh, nw, FP64)
+    try:
+        g.set_positions(p32)
+        g.evaluate()
+        assert np.abs(f32_ - g.local_energy()[0]).max() / n < 1e-7
+    finally:
+        g.close()
+
+
 def test_dmc_exciton_population_exact_on_device():
     s = exciton_spec()
     nw = 256
