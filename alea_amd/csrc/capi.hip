@@ -1157,7 +1157,8 @@ int alea_check_status(alea_system* s) {
 // calls allocate nothing, and calls on disjoint ranges touch disjoint
 // memory.
 static void ratio_range(alea_system* s, int p, int w0, int w1, const double* new_pos, double* log_ratio,
-                        double* phase, double* new_grad, double* new_lap) {
+                        double* phase, double* new_grad, double* new_lap, double* new_pgrad = nullptr,
+                        double* new_plap = nullptr) {
     if (!s->evaluated) throw InvalidArg("evaluate first");
     if (p < 0 || p >= s->np) throw InvalidArg("particle index out of range");
     if (!new_pos || !log_ratio) throw InvalidArg("null proposal or output");
@@ -1189,6 +1190,8 @@ static void ratio_range(alea_system* s, int p, int w0, int w1, const double* new
     if (phase) rows(b + 3 * nw, phase, 1, hipMemcpyDeviceToHost, "phase");
     if (new_grad) rows(b + 4 * nw, new_grad, 2, hipMemcpyDeviceToHost, "new_grad");
     if (new_lap) rows(b + 6 * nw, new_lap, 2, hipMemcpyDeviceToHost, "new_lap");
+    if (new_pgrad) rows(b + 8 * nw, new_pgrad, 2, hipMemcpyDeviceToHost, "new_pgrad");
+    if (new_plap) rows(b + 10 * nw, new_plap, 2, hipMemcpyDeviceToHost, "new_plap");
 }
 
 static void accept_range(alea_system* s, int p, int w0, int w1, const unsigned char* mask) {
@@ -1243,6 +1246,63 @@ int alea_ratio_range(alea_system* s, int p, int w_begin, int w_end, const double
         Sys(s);
         const auto [w0, w1] = s->range(w_begin, w_end);
         ratio_range(s, p, w0, w1, new_pos, log_ratio, phase, new_grad, new_lap);
+    });
+}
+
+int alea_ratio_range_phase(alea_system* s, int p, int w_begin, int w_end, const double* new_pos, double* log_ratio,
+                           double* phase, double* new_grad, double* new_lap, double* new_pgrad, double* new_plap) {
+    return guarded([&] {
+        Sys(s);
+        const auto [w0, w1] = s->range(w_begin, w_end);
+        ratio_range(s, p, w0, w1, new_pos, log_ratio, phase, new_grad, new_lap, new_pgrad, new_plap);
+    });
+}
+
+// log|psi| and the gradient planes of the walkers [w0, w1): k_measure on the
+// range (its grad_out planes offset to the range's first column, the batch's
+// walker stride kept), then the range's columns copied out.  Other walkers'
+// entries are neither read nor written.  The staging planes are shared by the
+// whole population: calls on one system are not made concurrently.
+int alea_get_derivatives_range(alea_system* s, int w_begin, int w_end, double* log_psi, double* grad, double* lap,
+                               double* pgrad, double* plap) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->evaluated) throw InvalidArg("evaluate first");
+        const auto [w0, w1] = s->range(w_begin, w_end);
+        const size_t nw = s->nw, n = static_cast<size_t>(w1 - w0), rows = static_cast<size_t>(s->np) * 2;
+        if (n == 0) return;
+        if (log_psi) {
+            std::vector<double> l(n * 4);
+            hip_check(copy_on(s->stream, l.data(), s->logs.p + w0 * 4, l.size() * sizeof(double),
+                              hipMemcpyDeviceToHost), "logs");
+            for (size_t k = 0; k < n; ++k) {
+                double t = l[k * 4];
+                for (int b = 0; b < s->host.n_blocks; ++b) t += l[k * 4 + 1 + b];
+                log_psi[w0 + k] = t;
+            }
+        }
+        if (!grad && !lap && !pgrad && !plap) return;
+        const size_t plane = rows * nw;
+        // sized by the capacity: after the first call a population change
+        // (alea_permute) never reallocates it under a later range call
+        const size_t staging = 4 * rows * static_cast<size_t>(s->capacity);
+        if (s->grad_buf.n < staging) s->grad_buf.alloc(staging, s->stream);
+        DevState st = s->shifted(w0);
+        st.grad_out = s->grad_buf.p + w0;
+        st.lap_out = s->grad_buf.p + plane + w0;
+        st.pgrad_out = s->grad_buf.p + 2 * plane + w0;
+        st.plap_out = s->grad_buf.p + 3 * plane + w0;
+        s->set_model_nw(s->nw);
+        launch_measure(s->dmodel, st, static_cast<int>(n), s->mixed(), s->cplx, s->walker_lds, s->stream);
+        hip_check(hipGetLastError(), "launch measure");
+        double* outs[4] = {grad, lap, pgrad, plap};
+        for (int q = 0; q < 4; ++q) {
+            if (!outs[q]) continue;
+            hip_check(hipMemcpy2DAsync(outs[q] + w0, nw * sizeof(double), s->grad_buf.p + q * plane + w0,
+                                       nw * sizeof(double), n * sizeof(double), rows, hipMemcpyDeviceToHost,
+                                       s->stream), "derivative planes");
+        }
+        s->sync();
     });
 }
 

@@ -144,6 +144,21 @@ int alea_accept(alea_system* s, int p, const unsigned char* mask);
 int alea_ratio_range(alea_system* s, int p, int w_begin, int w_end, const double* new_pos,
                      double* log_ratio, double* phase, double* new_grad, double* new_lap);
 int alea_accept_range(alea_system* s, int p, int w_begin, int w_end, const unsigned char* mask);
+/* alea_ratio_range plus the moved particle's new phase gradient and phase
+ * Laplacian [dim][nw] (WalkerSet::new_phase_grad / new_phase_lap, which a
+ * complex trial's ratio_batch fills, complex_slater_determinant.cpp:154-210;
+ * zeros for a real trial).  Any output but log_ratio may be NULL. */
+int alea_ratio_range_phase(alea_system* s, int p, int w_begin, int w_end, const double* new_pos, double* log_ratio,
+                           double* phase, double* new_grad, double* new_lap, double* new_pgrad, double* new_plap);
+/* log|psi| [nw] and the gradient, Laplacian, phase-gradient and
+ * phase-Laplacian planes [np][dim][nw] of the walkers [w_begin, w_end) only
+ * (WavefunctionComponent::refresh_derivatives over a crowd's range,
+ * wavefunction_component.h:249-251), with the batch's walker stride: other
+ * walkers' entries are neither read nor written.  Any output may be NULL.
+ * The first call allocates device staging for the capacity; the calls share
+ * it, so calls on one system are serialized by the caller. */
+int alea_get_derivatives_range(alea_system* s, int w_begin, int w_end, double* log_psi, double* grad, double* lap,
+                               double* pgrad, double* plap);
 
 /* Hamiltonian::delta_potential_particle (src/hamiltonian/hamiltonian.cpp:105-117),
  * for the walkers [w_begin, w_end): delta[w] = V(p at new_pos[.][w], wrapped

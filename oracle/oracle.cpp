@@ -2242,6 +2242,55 @@ int oracle_ratio_range(oracle_system* h, int p, int w_begin, int w_end, const do
     });
 }
 
+int oracle_ratio_range_phase(oracle_system* h, int p, int w_begin, int w_end, const double* new_pos,
+                             double* log_ratio, double* phase, double* new_grad, double* new_lap, double* new_pgrad,
+                             double* new_plap) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        if (p < 0 || static_cast<std::size_t>(p) >= s.np) throw std::invalid_argument("oracle: particle out of range");
+        const auto [w0, w1] = oracle_range(s, w_begin, w_end);
+        const std::size_t nw = s.nw;
+        for (std::size_t d = 0; d < 2; ++d)
+            for (std::size_t w = w0; w < w1; ++w) s.new_pos[d * nw + w] = new_pos[d * nw + w];
+        ratio_all(s, static_cast<std::size_t>(p));
+        for (std::size_t w = w0; w < w1; ++w) {
+            log_ratio[w] = s.log_ratios[w];
+            if (phase) phase[w] = s.phase_diffs[w];
+            for (std::size_t d = 0; d < 2; ++d) {
+                const std::size_t i = d * nw + w;
+                if (new_grad) new_grad[i] = s.new_grad[i];
+                if (new_lap) new_lap[i] = s.new_lap[i];
+                if (new_pgrad) new_pgrad[i] = s.new_pgrad.empty() ? 0.0 : s.new_pgrad[i];
+                if (new_plap) new_plap[i] = s.new_plap.empty() ? 0.0 : s.new_plap[i];
+            }
+        }
+    });
+}
+
+// log|psi| and the derivative planes of the walkers [w_begin, w_end) only.
+int oracle_get_derivatives_range(oracle_system* h, int w_begin, int w_end, double* lp, double* grad, double* lap,
+                                 double* pgrad, double* plap) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        const auto [w0, w1] = oracle_range(s, w_begin, w_end);
+        for (std::size_t w = w0; w < w1; ++w)
+            if (lp) lp[w] = log_psi(s, w);
+        if (!grad && !lap && !pgrad && !plap) return;
+        refresh_all(s);
+        std::vector<double> g, l, pg, pl;
+        totals(s, g, l, &pg, &pl);
+        const std::size_t nw = s.nw;
+        for (std::size_t r = 0; r < s.np * 2; ++r)
+            for (std::size_t w = w0; w < w1; ++w) {
+                const std::size_t i = r * nw + w;
+                if (grad) grad[i] = g[i];
+                if (lap) lap[i] = l[i];
+                if (pgrad) pgrad[i] = pg[i];
+                if (plap) plap[i] = pl[i];
+            }
+    });
+}
+
 // Hamiltonian::delta_potential_particle (hamiltonian.cpp:105-117) for each
 // walker of [w_begin, w_end): the potential of the batch with particle p at
 // new_pos[.][w] (wrapped into the cell) minus the potential now.  Ewald as

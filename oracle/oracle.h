@@ -96,6 +96,11 @@ int oracle_device_tape(uint64_t seed, uint64_t stream, int np, int nw, int n_swe
 int oracle_ratio_range(oracle_system* s, int p, int w_begin, int w_end, const double* new_pos, double* log_ratio,
                        double* phase, double* new_grad, double* new_lap);
 int oracle_accept_range(oracle_system* s, int p, int w_begin, int w_end, const unsigned char* mask);
+int oracle_ratio_range_phase(oracle_system* s, int p, int w_begin, int w_end, const double* new_pos,
+                             double* log_ratio, double* phase, double* new_grad, double* new_lap, double* new_pgrad,
+                             double* new_plap);
+int oracle_get_derivatives_range(oracle_system* s, int w_begin, int w_end, double* log_psi, double* grad, double* lap,
+                                 double* pgrad, double* plap);
 int oracle_delta_potential(oracle_system* s, int p, int w_begin, int w_end, const double* new_pos, double* delta);
 typedef struct oracle_param {
     int factor, index;
