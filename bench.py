@@ -321,6 +321,137 @@ def dmc_leg(c=DMC_CFG):
     return out
 
 
+COUPLED_CFG = dict(walkers=4096, dmc_steps=150, sweeps=200, window=20, seed=23)
+
+
+def reweighted_shift(forms, n, logw):
+    """E/N under the mixed process minus E/N under the FP64 one, from the
+    FP64 walkers at the end of a window and each walker's likelihood ratio
+    of its window's path (correlated sampling): sum(W E)/sum(W) - mean(E),
+    with a jackknife error over the walkers, and the spread of W."""
+    e = forms[0].local_energy()[0] / n
+    w = np.exp(logw - np.max(logw))
+    w /= w.mean()
+    m = len(e)
+    tot_we, tot_w, tot_e = float(np.sum(w * e)), float(np.sum(w)), float(np.sum(e))
+    est = tot_we / tot_w - tot_e / m
+    jack = ((tot_we - w * e) / (tot_w - w)) - (tot_e - e) / (m - 1)
+    err = math.sqrt((m - 1) / m * float(np.sum((jack - jack.mean()) ** 2)))
+    return est, err, float(np.std(w))
+
+
+def coupled_leg(c=COUPLED_CFG, d=DMC_CFG):
+    """The mixed form's effect on DMC sampling, measured without sampling
+    noise.  Walkers from a short FP64 DMC run of the N=74, rs=40 system are
+    loaded into an FP64 and a mixed batch; every move proposes the same
+    drift-diffusion step (FP64 drift, shared Gaussians) to both, each form
+    computes its own acceptance min(1, |psi'/psi|^2 G(R<-R')/G(R'<-R)) from its
+    own ratio and drifts at both ends (Langevin Green's function,
+    green_function.h:20-26; node crossings rejected), one shared uniform
+    decides both, and both batches commit the FP64 decision so they stay on
+    one trajectory.  Both inverses are rebuilt every second sweep, as the DMC
+    sweeps rebuild a block after 64 updates.  Reported: the fraction of
+    decisions the two forms take differently and which form accepted, the
+    log-ratio difference (mean, median, signed mean, max), the drift
+    difference, the local-energy difference on the final walkers, and the
+    shift of E/N that the mixed process makes, by reweighting the FP64 path
+    with the mixed process's likelihood of each step (its proposal density
+    and its probability of the decision taken) over
+    windows of `window` sweeps (longer than the DMC energy's correlation
+    time, about 8 steps)."""
+    from alea_amd.walkers import FP64, MIXED, gpu_walkers
+    spec = dmc_spec(d)
+    n = spec.n_particles
+    run = dict(d, target=c["walkers"], steps=c["dmc_steps"], equilibration=c["dmc_steps"])
+    _, _, _, (nw_f, pos_f) = dmc_run(spec, run, 0, d["pos_seed"], d["seed"])
+    nw = min(nw_f, c["walkers"])
+    pos = pos_f.reshape(n * 2, nw_f)[:, :nw].reshape(-1).copy()
+    tau = d["tau"]
+    rng = np.random.default_rng(c["seed"])
+    forms = [gpu_walkers(spec, nw, FP64), gpu_walkers(spec, nw, MIXED)]
+    try:
+        for w in forms:
+            w.set_positions(pos)
+            w.evaluate()
+        flips = moves = mixed_only = 0
+        signed_sum = 0.0
+        dlr_all = []
+        dlr_max = dlr_sum = 0.0
+        ddrift_max = ddrift_sum = 0.0
+        accepted = 0
+        logw = np.zeros(nw)
+        windows = []
+        for sweep in range(c["sweeps"]):
+            if sweep % c["window"] == 0 and sweep > 0:
+                windows.append(reweighted_shift(forms, n, logw))
+                logw[:] = 0.0
+            if sweep % 2 == 0:
+                # fresh inverses, as the DMC sweeps rebuild a block once 64
+                # updates accumulate (every second sweep at 37 moves a block)
+                for w in forms:
+                    w.refresh_inverse()
+            for p in range(n):
+                old = forms[0].positions().reshape(n, 2, nw)[p]
+                grads = [w.gradients()[0].reshape(n, 2, nw)[p] for w in forms]
+                step = np.sqrt(tau) * rng.standard_normal((2, nw))
+                new = old + tau * grads[0] + step
+                u = rng.random(nw)
+                acc, prob = [], []
+                outs = [w.ratio(p, new.reshape(-1)) for w in forms]
+                for (lr, ph, ng, _), v_old in zip(outs, grads):
+                    v_new = ng.reshape(2, nw)
+                    fwd = np.sum((new - old - tau * v_old) ** 2, axis=0)
+                    back = np.sum((old - new - tau * v_new) ** 2, axis=0)
+                    log_a = 2.0 * lr + (fwd - back) / (2.0 * tau)
+                    crossed = np.cos(ph) < 0.0
+                    acc.append((np.log(u) < log_a) & ~crossed)
+                    prob.append(np.where(crossed, 0.0, np.exp(np.minimum(log_a, 0.0))))
+                # the mixed process's likelihood of this step relative to the
+                # FP64 one: its proposal density (its own drift) and its
+                # probability of the FP64 decision
+                disp = new - old
+                log_t = -(np.sum((disp - tau * grads[1]) ** 2, axis=0) -
+                          np.sum((disp - tau * grads[0]) ** 2, axis=0)) / (2.0 * tau)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    step_w = np.where(acc[0], prob[1] / prob[0], (1.0 - prob[1]) / (1.0 - prob[0]))
+                    logw += log_t + np.log(np.maximum(step_w, 0.0))
+                flips += int(np.count_nonzero(acc[0] != acc[1]))
+                mixed_only += int(np.count_nonzero(acc[1] & ~acc[0]))
+                signed_sum += float(np.sum(outs[1][0] - outs[0][0]))
+                dlr_all.append(np.abs(outs[1][0] - outs[0][0]).astype(np.float32))
+                moves += nw
+                accepted += int(np.count_nonzero(acc[0]))
+                dlr = np.abs(outs[1][0] - outs[0][0])
+                dlr_max = max(dlr_max, float(dlr.max()))
+                dlr_sum += float(dlr.sum())
+                g64, g32 = outs[0][2].reshape(2, nw), outs[1][2].reshape(2, nw)
+                rel = np.linalg.norm(g32 - g64, axis=0) / np.maximum(np.linalg.norm(g64, axis=0), 1e-300)
+                ddrift_max = max(ddrift_max, float(rel.max()))
+                ddrift_sum += float(rel.sum())
+                mask = acc[0].astype(np.uint8)
+                for w in forms:
+                    w.accept(p, mask)
+        windows.append(reweighted_shift(forms, n, logw))
+        shift = np.array([x[0] for x in windows])
+        shift_err = math.sqrt(sum(x[1] ** 2 for x in windows)) / len(windows)
+        e = [w.local_energy()[0] / n for w in forms]
+        de = e[1] - e[0]
+        return dict(walkers=nw, sweeps=c["sweeps"], moves=moves, acceptance=accepted / moves,
+                    decision_flips=flips, flip_fraction=flips / moves,
+                    flips_mixed_accepts=mixed_only, flips_fp64_accepts=flips - mixed_only,
+                    log_ratio_diff_mean=dlr_sum / moves, log_ratio_diff_max=dlr_max,
+                    log_ratio_diff_median=float(np.median(np.concatenate(dlr_all))),
+                    log_ratio_diff_signed_mean=signed_sum / moves,
+                    drift_rel_diff_mean=ddrift_sum / moves, drift_rel_diff_max=ddrift_max,
+                    e_per_particle_diff_mean=float(np.mean(de)), e_per_particle_diff_max=float(np.max(np.abs(de))),
+                    sampling_shift=float(shift.mean()), sampling_shift_error=shift_err,
+                    sampling_shift_windows=[dict(shift=a, error=b, weight_spread=sw) for a, b, sw in windows],
+                    config=dict(c, tau=tau, system="N=74 (37+37) paramagnetic HEG, rs=40 (the DMC leg's)"))
+    finally:
+        for w in forms:
+            w.close()
+
+
 CRYSTAL_CFG = dict(rs=60.0, walkers=32768, steps=300, equilibration=60, tau=2.0, vmc_sweeps=20, seed=11,
                    pos_seed=5, spread=0.15, generations=10.0, alpha=2.0, refresh_updates=32)
 
@@ -537,6 +668,9 @@ def main():
     ap.add_argument("--dmc-only", action="store_true", help="run the DMC leg alone, print its dict")
     ap.add_argument("--crystal-only", action="store_true",
                     help="run the Wigner-crystal DMC leg (FP64 and mixed) alone, print its dict")
+    ap.add_argument("--coupled-only", action="store_true",
+                    help="count the DMC decisions the mixed form takes differently from FP64 on one "
+                         "shared trajectory, print its dict")
     ap.add_argument("--estimators-only", action="store_true",
                     help="time one g(r) + S(k) measurement of the workload's walkers, print its dict")
     ap.add_argument("--derivatives-only", action="store_true",
@@ -558,6 +692,9 @@ def main():
         return
     if args.crystal_only:
         print(json.dumps(crystal_leg()), flush=True)
+        return
+    if args.coupled_only:
+        print(json.dumps(coupled_leg()), flush=True)
         return
     if args.estimators_only:
         print(json.dumps(estimators_leg(args.walkers)), flush=True)
