@@ -321,7 +321,10 @@ def dmc_leg(c=DMC_CFG):
     return out
 
 
+CRYSTAL_CFG = dict(rs=60.0, walkers=32768, steps=300, equilibration=60, tau=2.0, vmc_sweeps=20, seed=11,
+                   pos_seed=5, spread=0.15, generations=10.0, alpha=2.0, refresh_updates=32)
 COUPLED_CFG = dict(walkers=4096, dmc_steps=150, sweeps=200, window=20, seed=23)
+COUPLED_CRYSTAL_CFG = dict(walkers=2048, dmc_steps=100, sweeps=100, window=20, seed=29)
 
 
 def reweighted_shift(forms, n, logw):
@@ -342,32 +345,75 @@ def reweighted_shift(forms, n, logw):
 
 def coupled_leg(c=COUPLED_CFG, d=DMC_CFG):
     """The mixed form's effect on DMC sampling, measured without sampling
-    noise.  Walkers from a short FP64 DMC run of the N=74, rs=40 system are
-    loaded into an FP64 and a mixed batch; every move proposes the same
-    drift-diffusion step (FP64 drift, shared Gaussians) to both, each form
-    computes its own acceptance min(1, |psi'/psi|^2 G(R<-R')/G(R'<-R)) from its
-    own ratio and drifts at both ends (Langevin Green's function,
-    green_function.h:20-26; node crossings rejected), one shared uniform
-    decides both, and both batches commit the FP64 decision so they stay on
-    one trajectory.  Both inverses are rebuilt every second sweep, as the DMC
-    sweeps rebuild a block after 64 updates.  Reported: the fraction of
-    decisions the two forms take differently and which form accepted, the
-    log-ratio difference (mean, median, signed mean, max), the drift
-    difference, the local-energy difference on the final walkers, and the
-    shift of E/N that the mixed process makes, by reweighting the FP64 path
-    with the mixed process's likelihood of each step (its proposal density
-    and its probability of the decision taken) over
-    windows of `window` sweeps (longer than the DMC energy's correlation
-    time, about 8 steps)."""
-    from alea_amd.walkers import FP64, MIXED, gpu_walkers
+    noise, on the N=74, rs=40 system of the DMC leg: walkers from a short
+    FP64 DMC run, inverses rebuilt every second sweep (as the DMC sweeps
+    rebuild a 37-orbital block after 64 updates).  See coupled_run."""
     spec = dmc_spec(d)
     n = spec.n_particles
     run = dict(d, target=c["walkers"], steps=c["dmc_steps"], equilibration=c["dmc_steps"])
     _, _, _, (nw_f, pos_f) = dmc_run(spec, run, 0, d["pos_seed"], d["seed"])
     nw = min(nw_f, c["walkers"])
     pos = pos_f.reshape(n * 2, nw_f)[:, :nw].reshape(-1).copy()
-    tau = d["tau"]
-    rng = np.random.default_rng(c["seed"])
+    out = coupled_run(spec, pos, nw, d["tau"], c["sweeps"], c["window"], c["seed"], refresh_moves=2 * n)
+    out["config"] = dict(c, tau=d["tau"], system="N=74 (37+37) paramagnetic HEG, rs=40 (the DMC leg's)")
+    return out
+
+
+def coupled_crystal_leg(c=COUPLED_CRYSTAL_CFG, d=CRYSTAL_CFG):
+    """coupled_run on the Wigner crystal of the crystal leg: walkers from a
+    short FP64 DMC run from the sites, the block's inverse rebuilt every 32
+    moves (the config's refresh)."""
+    from alea_amd import dmc
+    from alea_amd.system import site_positions, wigner_crystal_spec
+    from alea_amd.walkers import gpu_walkers, seed as seed_walkers
+    spec = wigner_crystal_spec(d["rs"])
+    n = spec.n_particles
+    a0 = spec.lattice[0] / 16
+    target = c["walkers"]
+    cap = 2 * target + 1
+    w = gpu_walkers(spec, cap, 0)
+    try:
+        w.set_refresh_updates(d["refresh_updates"])
+        w.set_positions(site_positions(spec, cap, np.random.default_rng(d["pos_seed"]), d["spread"] * a0))
+        w.evaluate()
+        w.permute(np.arange(target))
+        seed_walkers(w, d["seed"])
+        w.vmc_sweeps(d["vmc_sweeps"], (0.25 * a0) ** 2)
+        seed_walkers(w, d["seed"], dmc.K_DMC_THREAD_STREAM)
+        dm = dmc.DMC(w, d["tau"], target, n_substeps=1, generations=d["generations"], cutoff="universal",
+                     alpha=d["alpha"], seed=d["seed"])
+        for _ in range(c["dmc_steps"]):
+            dm.step()
+        nw_f, pos_f = w.nw, w.positions()
+    finally:
+        w.close()
+    nw = min(nw_f, c["walkers"])
+    pos = pos_f.reshape(n * 2, nw_f)[:, :nw].reshape(-1).copy()
+    out = coupled_run(spec, pos, nw, d["tau"], c["sweeps"], c["window"], c["seed"],
+                      refresh_moves=d["refresh_updates"])
+    out["config"] = dict(c, tau=d["tau"], system="polarized Wigner crystal N=128, rs=60 (the crystal leg's)")
+    return out
+
+
+def coupled_run(spec, pos, nw, tau, sweeps, window, seed, refresh_moves):
+    """An FP64 and a mixed batch loaded with the same walkers; every move
+    proposes the same drift-diffusion step (FP64 drift, shared Gaussians) to
+    both, each form computes its own acceptance min(1, |psi'/psi|^2
+    G(R<-R')/G(R'<-R)) from its own ratio and drifts at both ends (Langevin
+    Green's function, green_function.h:20-26; node crossings rejected), one
+    shared uniform decides both, and both batches commit the FP64 decision so
+    they stay on one trajectory.  Both inverses are rebuilt every
+    `refresh_moves` moves.  Reported: the fraction of decisions the two forms
+    take differently and which form accepted, the log-ratio difference
+    (mean, median, signed mean, max), the drift difference, the local-energy
+    difference on the final walkers, and the shift of E/N that the mixed
+    process makes, by reweighting the FP64 path with the mixed process's
+    likelihood of each step (its proposal density and its probability of the
+    decision taken) over windows of `window` sweeps (longer than the DMC
+    energy's correlation time)."""
+    from alea_amd.walkers import FP64, MIXED, gpu_walkers
+    n = spec.n_particles
+    rng = np.random.default_rng(seed)
     forms = [gpu_walkers(spec, nw, FP64), gpu_walkers(spec, nw, MIXED)]
     try:
         for w in forms:
@@ -381,16 +427,16 @@ def coupled_leg(c=COUPLED_CFG, d=DMC_CFG):
         accepted = 0
         logw = np.zeros(nw)
         windows = []
-        for sweep in range(c["sweeps"]):
-            if sweep % c["window"] == 0 and sweep > 0:
+        moved = 0
+        for sweep in range(sweeps):
+            if sweep % window == 0 and sweep > 0:
                 windows.append(reweighted_shift(forms, n, logw))
                 logw[:] = 0.0
-            if sweep % 2 == 0:
-                # fresh inverses, as the DMC sweeps rebuild a block once 64
-                # updates accumulate (every second sweep at 37 moves a block)
-                for w in forms:
-                    w.refresh_inverse()
             for p in range(n):
+                if moved % refresh_moves == 0:
+                    for w in forms:
+                        w.refresh_inverse()
+                moved += 1
                 old = forms[0].positions().reshape(n, 2, nw)[p]
                 grads = [w.gradients()[0].reshape(n, 2, nw)[p] for w in forms]
                 step = np.sqrt(tau) * rng.standard_normal((2, nw))
@@ -436,7 +482,7 @@ def coupled_leg(c=COUPLED_CFG, d=DMC_CFG):
         shift_err = math.sqrt(sum(x[1] ** 2 for x in windows)) / len(windows)
         e = [w.local_energy()[0] / n for w in forms]
         de = e[1] - e[0]
-        return dict(walkers=nw, sweeps=c["sweeps"], moves=moves, acceptance=accepted / moves,
+        return dict(walkers=nw, sweeps=sweeps, moves=moves, acceptance=accepted / moves,
                     decision_flips=flips, flip_fraction=flips / moves,
                     flips_mixed_accepts=mixed_only, flips_fp64_accepts=flips - mixed_only,
                     log_ratio_diff_mean=dlr_sum / moves, log_ratio_diff_max=dlr_max,
@@ -445,15 +491,10 @@ def coupled_leg(c=COUPLED_CFG, d=DMC_CFG):
                     drift_rel_diff_mean=ddrift_sum / moves, drift_rel_diff_max=ddrift_max,
                     e_per_particle_diff_mean=float(np.mean(de)), e_per_particle_diff_max=float(np.max(np.abs(de))),
                     sampling_shift=float(shift.mean()), sampling_shift_error=shift_err,
-                    sampling_shift_windows=[dict(shift=a, error=b, weight_spread=sw) for a, b, sw in windows],
-                    config=dict(c, tau=tau, system="N=74 (37+37) paramagnetic HEG, rs=40 (the DMC leg's)"))
+                    sampling_shift_windows=[dict(shift=a, error=b, weight_spread=sw) for a, b, sw in windows])
     finally:
         for w in forms:
             w.close()
-
-
-CRYSTAL_CFG = dict(rs=60.0, walkers=32768, steps=300, equilibration=60, tau=2.0, vmc_sweeps=20, seed=11,
-                   pos_seed=5, spread=0.15, generations=10.0, alpha=2.0, refresh_updates=32)
 
 
 def crystal_leg(c=CRYSTAL_CFG):
@@ -671,6 +712,8 @@ def main():
     ap.add_argument("--coupled-only", action="store_true",
                     help="count the DMC decisions the mixed form takes differently from FP64 on one "
                          "shared trajectory, print its dict")
+    ap.add_argument("--coupled-crystal-only", action="store_true",
+                    help="the same on the Wigner crystal, print its dict")
     ap.add_argument("--estimators-only", action="store_true",
                     help="time one g(r) + S(k) measurement of the workload's walkers, print its dict")
     ap.add_argument("--derivatives-only", action="store_true",
@@ -695,6 +738,9 @@ def main():
         return
     if args.coupled_only:
         print(json.dumps(coupled_leg()), flush=True)
+        return
+    if args.coupled_crystal_only:
+        print(json.dumps(coupled_crystal_leg()), flush=True)
         return
     if args.estimators_only:
         print(json.dumps(estimators_leg(args.walkers)), flush=True)
