@@ -20,13 +20,17 @@
 //   apply_move_batch commits the accepted walkers of the range and pushes every
 //                    particle's changed gradient and Laplacian into the totals
 //                    (the "eager" form: partners included);
+//   apply_move_batch_deferred commits and pushes log psi and the moved
+//                    particle's rows only; refresh_particle_derivatives and
+//                    refresh_derivatives bring the partners up to date;
 //   permute_caches   carries the device walkers and the host copies through a
 //                    source map (branching).
 // Every range call touches only its range's walkers, on the device and here,
 // so DMC crowds may share one instance (alea_ratio_range_phase,
 // alea_accept_range and alea_get_derivatives_range are range-local).  The
-// per-move path reads the range's gradient planes back after each accept; the
-// fused sweeps (alea_vmc_sweeps, alea_dmc_sweeps) are the fast path.
+// eager commit reads the range's gradient planes back after each accept, the
+// deferred one only at a refresh; the fused sweeps (alea_vmc_sweeps,
+// alea_dmc_sweeps) are the fast path.
 #ifndef ALEA_AMD_COMPONENT_HPP
 #define ALEA_AMD_COMPONENT_HPP
 
@@ -240,6 +244,44 @@ public:
         }
     }
 
+    // The deferred commit (wavefunction_component.h:236-252): the range's
+    // accepted walkers committed on the device and log psi pushed; this
+    // component's share of the moved particle's rows becomes its share of
+    // the proposal's gradient and Laplacian (the walker set overwrites the
+    // totals' row with the proposal's, WalkerSet::apply_move); partners stay
+    // stale until refresh_particle_derivatives / refresh_derivatives read
+    // the planes back.  A Metropolis sweep then reads no planes per move.
+    void apply_move_batch_deferred(alea::WalkerSet& batch, std::size_t p, std::span<const bool> accept,
+                                   std::size_t w_begin, std::size_t w_end) override {
+        const std::size_t nw = batch.num_walkers(), dim = batch.dim();
+        if (w_end > nw) w_end = nw;
+        if (w_begin >= w_end) return;
+        for (std::size_t w = w_begin; w < w_end; ++w) mask_[w] = accept[w] ? 1 : 0;
+        component_check(alea_accept_range(dev_.get(), static_cast<int>(p), static_cast<int>(w_begin),
+                                          static_cast<int>(w_end), mask_.data()));
+        component_check(alea_get_derivatives_range(dev_.get(), static_cast<int>(w_begin), static_cast<int>(w_end),
+                                                   lp_.data(), nullptr, nullptr, nullptr, nullptr));
+        push(batch.log_psi(), log_psi_c_, lp_, 1, nw, w_begin, w_end);
+        for (std::size_t d = 0; d < dim; ++d)
+            for (std::size_t w = w_begin; w < w_end; ++w) {
+                if (!accept[w]) continue;
+                const std::size_t i = (p * dim + d) * nw + w, j = d * nw + w;
+                grad_c_[i] = ng_[j];
+                lap_c_[i] = nl_[j];
+                if (complex_) {
+                    pgrad_c_[i] = npg_[j];
+                    plap_c_[i] = npl_[j];
+                }
+            }
+    }
+    void refresh_particle_derivatives(alea::WalkerSet& batch, std::size_t p, std::size_t w_begin,
+                                      std::size_t w_end) override {
+        refresh_rows(batch, p, p + 1, w_begin, w_end);
+    }
+    void refresh_derivatives(alea::WalkerSet& batch, std::size_t w_begin, std::size_t w_end) override {
+        refresh_rows(batch, 0, batch.n_particles(), w_begin, w_end);
+    }
+
     void permute_caches(std::span<const std::size_t> src) override {
         const std::size_t new_nw = src.size();
         if (new_nw > dev_.capacity())
@@ -308,14 +350,31 @@ public:
     }
 
 private:
+    // The rows of particles [p0, p1) over the walkers [w0, w1) read back and
+    // pushed into the totals.
+    void refresh_rows(alea::WalkerSet& batch, std::size_t p0, std::size_t p1, std::size_t w0, std::size_t w1) {
+        const std::size_t nw = batch.num_walkers(), dim = batch.dim();
+        if (w1 > nw) w1 = nw;
+        if (w0 >= w1) return;
+        component_check(alea_get_derivatives_range(dev_.get(), static_cast<int>(w0), static_cast<int>(w1), nullptr,
+                                                   g_.data(), l_.data(), complex_ ? pg_.data() : nullptr,
+                                                   complex_ ? pl_.data() : nullptr));
+        const std::size_t r0 = p0 * dim, r1 = p1 * dim;
+        push(batch.gradients(), grad_c_, g_, r1, nw, w0, w1, r0);
+        push(batch.laplacians(), lap_c_, l_, r1, nw, w0, w1, r0);
+        if (complex_) {
+            push(batch.phase_gradients(), pgrad_c_, pg_, r1, nw, w0, w1, r0);
+            push(batch.phase_laplacians(), plap_c_, pl_, r1, nw, w0, w1, r0);
+        }
+    }
     static void add(std::span<double> total, const std::vector<double>& part) {
         for (std::size_t k = 0; k < total.size() && k < part.size(); ++k) total[k] += part[k];
     }
     // cache <- fresh and total += fresh - cache over the range's columns of
     // `rows` planes
     static void push(std::span<double> total, std::vector<double>& cache, const std::vector<double>& fresh,
-                     std::size_t rows, std::size_t nw, std::size_t w0, std::size_t w1) {
-        for (std::size_t r = 0; r < rows; ++r)
+                     std::size_t rows, std::size_t nw, std::size_t w0, std::size_t w1, std::size_t r0 = 0) {
+        for (std::size_t r = r0; r < rows; ++r)
             for (std::size_t w = w0; w < w1; ++w) {
                 const std::size_t i = r * nw + w;
                 total[i] += fresh[i] - cache[i];

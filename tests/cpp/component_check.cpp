@@ -11,7 +11,8 @@
 // Each scenario drives the component contract the way VMC::sweep_walkers
 // and the DMC crowds do (vmc.cpp:82-155, dmc.cpp:300-340): evaluate_batch,
 // per-move ratio_batch / apply_move_batch over two crowd ranges, the deferred
-// commit with refresh_derivatives, branching (apply_permutation +
+// commit with refresh_derivatives (at the sweep's end, and per particle
+// before its move), branching (apply_permutation +
 // permute_caches, with copies and a shrink), clone, parameter derivatives and
 // rebuild after a parameter change, local energies and
 // delta_potential_particle.  Exit status 0 when every comparison holds to
@@ -171,8 +172,11 @@ void compare_totals(const std::string& tag, Side& a, Side& b) {
 
 // One sweep of single-particle moves, both sides from the same proposals and
 // side a's decisions; crowds: the walker ranges the calls are split into.
+// deferred: commit with apply_move_batch_deferred and refresh at the end;
+// refresh_each: also refresh each particle before its move (as the drift
+// proposal of VMC::sweep_walkers does, vmc.cpp:98-139) and compare its rows.
 void sweep(const std::string& tag, Pair& P, std::mt19937_64& rng, double sigma, bool deferred,
-           const std::vector<std::pair<std::size_t, std::size_t>>& crowds) {
+           const std::vector<std::pair<std::size_t, std::size_t>>& crowds, bool refresh_each = false) {
     const std::size_t nw = P.a.ws->num_walkers(), np = P.c->np;
     std::normal_distribution<double> gauss(0.0, 1.0);
     std::uniform_real_distribution<double> unif(0.0, 1.0);
@@ -183,6 +187,15 @@ void sweep(const std::string& tag, Pair& P, std::mt19937_64& rng, double sigma, 
             for (std::size_t w = 0; w < nw; ++w)
                 newp[d * nw + w] = pos[(p * 2 + d) * nw + w] + sigma * gauss(rng);
         for (auto& x : u) x = unif(rng);
+        if (deferred && refresh_each) {
+            for (Side* s : {&P.a, &P.b})
+                for (const auto& [w0, w1] : crowds) s->psi.refresh_particle_derivatives(*s->ws, p, w0, w1);
+            const std::size_t row = p * 2 * nw;
+            const std::string rp = tag + " p=" + std::to_string(p) + " refreshed row";
+            expect(rp + " gradient", P.b.ws->gradients().subspan(row, 2 * nw), P.a.ws->gradients().subspan(row, 2 * nw));
+            expect(rp + " laplacian", P.b.ws->laplacians().subspan(row, 2 * nw),
+                   P.a.ws->laplacians().subspan(row, 2 * nw));
+        }
         for (Side* s : {&P.a, &P.b}) {
             auto np_ = s->ws->new_pos();
             std::copy(newp.begin(), newp.end(), np_.begin());
@@ -366,6 +379,7 @@ void jastrow_scenario() {
     const double sigma = 0.6;
     sweep("jastrow: eager sweep", P, rng, sigma, false, {{0, nw}});
     sweep("jastrow: deferred sweep over two crowds", P, rng, sigma, true, {{0, 11}, {11, nw}});
+    sweep("jastrow: deferred sweep refreshing each particle", P, rng, sigma, true, {{0, 13}, {13, nw}}, true);
     reevaluate("jastrow: after two sweeps", P);
     expect("jastrow: d log psi / dp", param_derivs(P.b), param_derivs(P.a));
 
@@ -551,6 +565,7 @@ void slater_scenario(bool twisted) {
     const double sigma = 0.5;
     sweep(tag + ": eager sweep", P, rng, sigma, false, {{0, 8}, {8, nw}});
     sweep(tag + ": deferred sweep", P, rng, sigma, true, {{0, nw}});
+    sweep(tag + ": deferred sweep refreshing each particle", P, rng, sigma, true, {{0, 6}, {6, nw}}, true);
     branch(tag + ": permutation", P, rng, nw - 3);
     sweep(tag + ": sweep after the permutation", P, rng, sigma, false, {{0, 9}, {9, nw - 3}});
     reevaluate(tag + ": after the sweeps", P);

@@ -9,7 +9,8 @@ Drummond + Nosanow replaced by one DeviceComponent and the Ewald term by a
 DevicePotential, and drives both through the component contract as
 VMC::sweep_walkers and the DMC crowds do (vmc.cpp:82-155, dmc.cpp:300-340):
 evaluate_batch, per-move ratio_batch / apply_move_batch over crowd ranges,
-the deferred commit with refresh_derivatives, branching with copies, a shrink
+the deferred commit with refresh_derivatives (at the sweep's end, and per
+particle before its move), branching with copies, a shrink
 and a growth, clone, d log psi / dp, rebuild after a parameter change, local
 energies and delta_potential_particle.  Determinant blocks (the reference's
 SlaterDeterminant needs Eigen, absent here) sit in a DeviceComponent beside the
@@ -38,7 +39,7 @@ def _run(driver):
     last = out.strip().splitlines()[-1]
     assert last.endswith(" 0 failures"), out[-4000:]
     n_checks = int(last.split()[0])
-    assert n_checks > 900, last
+    assert n_checks > 1200, last
 
 
 def test_device_components_inside_the_reference_trial_oracle_backed():
