@@ -54,6 +54,7 @@ SIGNATURES = {
     "accept_range": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_U8]),
     "ratio_range_phase": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_D, _P_D, _P_D, _P_D, _P_D, _P_D, _P_D]),
     "get_derivatives_range": (C.c_int, [_V, C.c_int, C.c_int, _P_D, _P_D, _P_D, _P_D, _P_D]),
+    "get_particle_derivatives_range": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_D, _P_D, _P_D, _P_D]),
     "delta_potential": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _P_D, _P_D]),
     "parameter_derivatives": (C.c_int, [_V, C.POINTER(AleaParam), C.c_int, _P_D]),
     "local_energy_parameter_derivatives": (C.c_int, [_V, C.POINTER(AleaParam), C.c_int, _P_D, _P_D]),

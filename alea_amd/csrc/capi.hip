@@ -1258,18 +1258,45 @@ int alea_ratio_range_phase(alea_system* s, int p, int w_begin, int w_end, const 
     });
 }
 
-// log|psi| and the gradient planes of the walkers [w0, w1): k_measure on the
-// range (its grad_out planes offset to the range's first column, the batch's
-// walker stride kept), then the range's columns copied out.  Other walkers'
+// k_measure on the walkers [w0, w1) (its grad_out planes offset to the
+// range's first column, the batch's walker stride kept), then rows
+// [row0, row0 + n_rows) of each plane copied out of the range's columns into
+// outs[q] (rows from its start, the batch's walker stride).  Other walkers'
 // entries are neither read nor written.  The staging planes are shared by the
 // whole population: calls on one system are not made concurrently.
+static void derivative_rows(alea_system* s, int w0, int w1, size_t row0, size_t n_rows, double* const outs[4]) {
+    const size_t nw = s->nw, n = static_cast<size_t>(w1 - w0), rows = static_cast<size_t>(s->np) * 2;
+    if (n == 0 || (!outs[0] && !outs[1] && !outs[2] && !outs[3])) return;
+    const size_t plane = rows * nw;
+    // sized by the capacity: after the first call a population change
+    // (alea_permute) never reallocates it under a later range call
+    const size_t staging = 4 * rows * static_cast<size_t>(s->capacity);
+    if (s->grad_buf.n < staging) s->grad_buf.alloc(staging, s->stream);
+    DevState st = s->shifted(w0);
+    st.grad_out = s->grad_buf.p + w0;
+    st.lap_out = s->grad_buf.p + plane + w0;
+    st.pgrad_out = s->grad_buf.p + 2 * plane + w0;
+    st.plap_out = s->grad_buf.p + 3 * plane + w0;
+    s->set_model_nw(s->nw);
+    launch_measure(s->dmodel, st, static_cast<int>(n), s->mixed(), s->cplx, s->walker_lds, s->stream);
+    hip_check(hipGetLastError(), "launch measure");
+    for (int q = 0; q < 4; ++q) {
+        if (!outs[q]) continue;
+        hip_check(hipMemcpy2DAsync(outs[q] + w0, nw * sizeof(double), s->grad_buf.p + q * plane + row0 * nw + w0,
+                                   nw * sizeof(double), n * sizeof(double), n_rows, hipMemcpyDeviceToHost,
+                                   s->stream), "derivative planes");
+    }
+    s->sync();
+}
+
+// log|psi| and the gradient planes of the walkers [w0, w1).
 int alea_get_derivatives_range(alea_system* s, int w_begin, int w_end, double* log_psi, double* grad, double* lap,
                                double* pgrad, double* plap) {
     return guarded([&] {
         Sys(s);
         if (!s->evaluated) throw InvalidArg("evaluate first");
         const auto [w0, w1] = s->range(w_begin, w_end);
-        const size_t nw = s->nw, n = static_cast<size_t>(w1 - w0), rows = static_cast<size_t>(s->np) * 2;
+        const size_t n = static_cast<size_t>(w1 - w0);
         if (n == 0) return;
         if (log_psi) {
             std::vector<double> l(n * 4);
@@ -1281,28 +1308,21 @@ int alea_get_derivatives_range(alea_system* s, int w_begin, int w_end, double* l
                 log_psi[w0 + k] = t;
             }
         }
-        if (!grad && !lap && !pgrad && !plap) return;
-        const size_t plane = rows * nw;
-        // sized by the capacity: after the first call a population change
-        // (alea_permute) never reallocates it under a later range call
-        const size_t staging = 4 * rows * static_cast<size_t>(s->capacity);
-        if (s->grad_buf.n < staging) s->grad_buf.alloc(staging, s->stream);
-        DevState st = s->shifted(w0);
-        st.grad_out = s->grad_buf.p + w0;
-        st.lap_out = s->grad_buf.p + plane + w0;
-        st.pgrad_out = s->grad_buf.p + 2 * plane + w0;
-        st.plap_out = s->grad_buf.p + 3 * plane + w0;
-        s->set_model_nw(s->nw);
-        launch_measure(s->dmodel, st, static_cast<int>(n), s->mixed(), s->cplx, s->walker_lds, s->stream);
-        hip_check(hipGetLastError(), "launch measure");
-        double* outs[4] = {grad, lap, pgrad, plap};
-        for (int q = 0; q < 4; ++q) {
-            if (!outs[q]) continue;
-            hip_check(hipMemcpy2DAsync(outs[q] + w0, nw * sizeof(double), s->grad_buf.p + q * plane + w0,
-                                       nw * sizeof(double), n * sizeof(double), rows, hipMemcpyDeviceToHost,
-                                       s->stream), "derivative planes");
-        }
-        s->sync();
+        double* const outs[4] = {grad, lap, pgrad, plap};
+        derivative_rows(s, w0, w1, 0, static_cast<size_t>(s->np) * 2, outs);
+    });
+}
+
+// Particle p's rows of the same planes, [dim][nw].
+int alea_get_particle_derivatives_range(alea_system* s, int p, int w_begin, int w_end, double* grad, double* lap,
+                                        double* pgrad, double* plap) {
+    return guarded([&] {
+        Sys(s);
+        if (!s->evaluated) throw InvalidArg("evaluate first");
+        if (p < 0 || p >= s->np) throw InvalidArg("particle index out of range");
+        const auto [w0, w1] = s->range(w_begin, w_end);
+        double* const outs[4] = {grad, lap, pgrad, plap};
+        derivative_rows(s, w0, w1, static_cast<size_t>(p) * 2, 2, outs);
     });
 }
 

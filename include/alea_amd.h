@@ -159,6 +159,13 @@ int alea_ratio_range_phase(alea_system* s, int p, int w_begin, int w_end, const 
  * it, so calls on one system are serialized by the caller. */
 int alea_get_derivatives_range(alea_system* s, int w_begin, int w_end, double* log_psi, double* grad, double* lap,
                                double* pgrad, double* plap);
+/* Particle p's rows of those planes, each [dim][nw] at the batch stride, for
+ * the walkers [w_begin, w_end) (WavefunctionComponent::
+ * refresh_particle_derivatives over a crowd's range,
+ * wavefunction_component.h:245-248).  Any output may be NULL; calls share the
+ * staging of alea_get_derivatives_range. */
+int alea_get_particle_derivatives_range(alea_system* s, int p, int w_begin, int w_end, double* grad, double* lap,
+                                        double* pgrad, double* plap);
 
 /* Hamiltonian::delta_potential_particle (src/hamiltonian/hamiltonian.cpp:105-117),
  * for the walkers [w_begin, w_end): delta[w] = V(p at new_pos[.][w], wrapped

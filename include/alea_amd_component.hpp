@@ -274,9 +274,31 @@ public:
                 }
             }
     }
+    // particle p's rows only (alea_get_particle_derivatives_range)
     void refresh_particle_derivatives(alea::WalkerSet& batch, std::size_t p, std::size_t w_begin,
                                       std::size_t w_end) override {
-        refresh_rows(batch, p, p + 1, w_begin, w_end);
+        const std::size_t nw = batch.num_walkers(), dim = batch.dim();
+        if (w_end > nw) w_end = nw;
+        if (w_begin >= w_end) return;
+        component_check(alea_get_particle_derivatives_range(dev_.get(), static_cast<int>(p), static_cast<int>(w_begin),
+                                                            static_cast<int>(w_end), g_.data(), l_.data(),
+                                                            complex_ ? pg_.data() : nullptr,
+                                                            complex_ ? pl_.data() : nullptr));
+        auto row = [&](std::span<double> total, std::vector<double>& cache, const std::vector<double>& fresh) {
+            for (std::size_t d = 0; d < dim; ++d)
+                for (std::size_t w = w_begin; w < w_end; ++w) {
+                    const std::size_t i = (p * dim + d) * nw + w, j = d * nw + w;
+                    total[i] += fresh[j] - cache[i];
+                    cache[i] = fresh[j];
+                }
+        };
+        // the scratch planes, not the ratio's outputs, which a commit still reads
+        row(batch.gradients(), grad_c_, g_);
+        row(batch.laplacians(), lap_c_, l_);
+        if (complex_) {
+            row(batch.phase_gradients(), pgrad_c_, pg_);
+            row(batch.phase_laplacians(), plap_c_, pl_);
+        }
     }
     void refresh_derivatives(alea::WalkerSet& batch, std::size_t w_begin, std::size_t w_end) override {
         refresh_rows(batch, 0, batch.n_particles(), w_begin, w_end);

@@ -2291,6 +2291,30 @@ int oracle_get_derivatives_range(oracle_system* h, int w_begin, int w_end, doubl
     });
 }
 
+// Particle p's rows of the derivative planes, [dim][nw], for the walkers
+// [w_begin, w_end) only.
+int oracle_get_particle_derivatives_range(oracle_system* h, int p, int w_begin, int w_end, double* grad, double* lap,
+                                          double* pgrad, double* plap) {
+    return guarded([&] {
+        Sys& s = *S(h);
+        if (p < 0 || static_cast<std::size_t>(p) >= s.np) throw std::invalid_argument("oracle: particle out of range");
+        const auto [w0, w1] = oracle_range(s, w_begin, w_end);
+        if (!grad && !lap && !pgrad && !plap) return;
+        refresh_all(s);
+        std::vector<double> g, l, pg, pl;
+        totals(s, g, l, &pg, &pl);
+        const std::size_t nw = s.nw;
+        for (std::size_t d = 0; d < 2; ++d)
+            for (std::size_t w = w0; w < w1; ++w) {
+                const std::size_t i = (static_cast<std::size_t>(p) * 2 + d) * nw + w, o = d * nw + w;
+                if (grad) grad[o] = g[i];
+                if (lap) lap[o] = l[i];
+                if (pgrad) pgrad[o] = pg[i];
+                if (plap) plap[o] = pl[i];
+            }
+    });
+}
+
 // Hamiltonian::delta_potential_particle (hamiltonian.cpp:105-117) for each
 // walker of [w_begin, w_end): the potential of the batch with particle p at
 // new_pos[.][w] (wrapped into the cell) minus the potential now.  Ewald as

@@ -101,6 +101,8 @@ int oracle_ratio_range_phase(oracle_system* s, int p, int w_begin, int w_end, co
                              double* new_plap);
 int oracle_get_derivatives_range(oracle_system* s, int w_begin, int w_end, double* log_psi, double* grad, double* lap,
                                  double* pgrad, double* plap);
+int oracle_get_particle_derivatives_range(oracle_system* s, int p, int w_begin, int w_end, double* grad, double* lap,
+                                          double* pgrad, double* plap);
 int oracle_delta_potential(oracle_system* s, int p, int w_begin, int w_end, const double* new_pos, double* delta);
 typedef struct oracle_param {
     int factor, index;
